@@ -1,0 +1,14 @@
+"""MI355X (gfx950) HIP backend for BlitzRemesher's audit + scorer hot path.
+
+The compute lives in libblitz_hip.so (hand-written HIP kernels behind the C ABI
+of include/blitz_hip.h); this package is its Python view for tests and
+benchmarks. Importing it loads the library and fails if it is not built.
+"""
+from . import _lib
+from .audit import (PROFILE_ATTRIBUTES, PROFILE_COVERAGE, PROFILE_NORMALS, Audit, BlitzError, DeviceMesh,
+                    EvalSettings, Measurement, Mesh, bounds, cameras)
+
+_lib.load()
+
+__all__ = ["Audit", "DeviceMesh", "EvalSettings", "Measurement", "Mesh", "bounds", "cameras", "BlitzError",
+           "PROFILE_COVERAGE", "PROFILE_NORMALS", "PROFILE_ATTRIBUTES"]

@@ -1,0 +1,160 @@
+"""ctypes binding of include/blitz_hip.h (libblitz_hip.so, built in-tree).
+
+Loading fails loudly when the library is missing: there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+LIBRARY_PATH = Path(__file__).resolve().parent / "libblitz_hip.so"
+
+BLITZ_ABI_VERSION = 5
+BLITZ_HIP_API_VERSION = 1
+BLITZ_NEURAL_RASTER_HIP = 2
+
+STATUS = {
+    0: "BLITZ_OK",
+    1: "BLITZ_INVALID_ARGUMENT",
+    2: "BLITZ_OUT_OF_MEMORY",
+    3: "BLITZ_INTERNAL_ERROR",
+    4: "BLITZ_CANCELLED",
+    5: "BLITZ_BUDGET_LIMITED",
+    6: "BLITZ_UNAVAILABLE",
+}
+
+
+class BlitzStream(C.Structure):
+    _fields_ = [("data", C.c_void_p), ("count", C.c_size_t), ("stride", C.c_size_t)]
+
+
+class BlitzMesh(C.Structure):
+    """blitz_mesh (reference include/blitz/blitz.h:28-37)."""
+    _fields_ = [
+        ("struct_size", C.c_uint32), ("abi_version", C.c_uint32),
+        ("positions", BlitzStream), ("normals", BlitzStream), ("uv", BlitzStream),
+        ("colors", BlitzStream), ("tangents", BlitzStream),
+        ("indices", C.POINTER(C.c_uint32)), ("index_count", C.c_size_t),
+        ("materials", C.POINTER(C.c_uint16)), ("material_count", C.c_size_t),
+        ("double_sided", C.POINTER(C.c_uint8)), ("double_sided_count", C.c_size_t),
+    ]
+
+
+class Bounds(C.Structure):
+    _fields_ = [("center", C.c_float * 3), ("radius", C.c_double)]
+
+
+class Camera(C.Structure):
+    _fields_ = [
+        ("right", C.c_float * 3), ("up", C.c_float * 3), ("forward", C.c_float * 3),
+        ("distance", C.c_double), ("focal", C.c_double), ("scale", C.c_double),
+        ("perspective", C.c_uint8),
+    ]
+
+
+CANCEL_FN = C.CFUNCTYPE(C.c_int, C.c_void_p)
+
+
+class EvalSettingsC(C.Structure):
+    _fields_ = [
+        ("struct_size", C.c_uint32), ("version", C.c_uint32),
+        ("profile", C.c_uint8), ("supersample", C.c_uint8), ("max_supersample", C.c_uint8),
+        ("force_two_sided", C.c_uint8), ("conservative_screen", C.c_uint8), ("reserved0", C.c_uint8 * 3),
+        ("views_orthographic", C.c_uint16), ("views_perspective", C.c_uint16),
+        ("views_rotation_seed", C.c_uint32),
+        ("screen_size", C.c_double), ("limit", C.c_double), ("max_changed_area", C.c_double),
+        ("normal_weight", C.c_double), ("color_weight", C.c_double), ("material_weight", C.c_double),
+        ("cancelled", CANCEL_FN), ("user_data", C.c_void_p),
+    ]
+
+
+class MeasurementC(C.Structure):
+    _fields_ = [
+        ("error", C.c_double), ("coverage", C.c_double), ("coverage_upper", C.c_double),
+        ("changed_area", C.c_double), ("normal_degrees", C.c_double),
+        ("worst_view", C.c_uint32), ("changed_area_worst_view", C.c_uint32), ("views_evaluated", C.c_uint32),
+        ("supersample", C.c_uint8), ("complete", C.c_uint8), ("passed", C.c_uint8),
+        ("resource_limited", C.c_uint8), ("cancelled", C.c_uint8), ("reserved", C.c_uint8 * 3),
+    ]
+
+
+class AuditOptions(C.Structure):
+    _fields_ = [
+        ("struct_size", C.c_uint32), ("version", C.c_uint32), ("device", C.c_int32),
+        ("memory_mib", C.c_uint32), ("cache_rasters", C.c_uint8), ("reserved", C.c_uint8 * 7),
+    ]
+
+
+class AuditTiming(C.Structure):
+    _fields_ = [
+        ("struct_size", C.c_uint32), ("raster_tasks", C.c_uint32), ("distance_views", C.c_uint32),
+        ("passes", C.c_uint32),
+        ("raster_ms", C.c_double), ("compare_ms", C.c_double), ("distance_ms", C.c_double),
+        ("total_ms", C.c_double), ("raster_triangles", C.c_uint64), ("mask_bytes", C.c_uint64),
+    ]
+
+
+_P = C.c_void_p
+_ERR = [C.c_char_p, C.c_size_t]
+
+# name -> (restype, argtypes). Every symbol include/blitz_hip.h declares.
+PROTOTYPES = {
+    "blitz_hip_api_version": (C.c_uint32, []),
+    "blitz_hip_available": (C.c_int, [C.c_int32]),
+    "blitz_hip_audit_options_init": (C.c_int, [C.POINTER(AuditOptions), C.c_size_t]),
+    "blitz_hip_eval_settings_init": (C.c_int, [C.POINTER(EvalSettingsC), C.c_size_t]),
+    "blitz_hip_bounds_of": (C.c_int, [C.POINTER(BlitzMesh), C.POINTER(Bounds), *_ERR]),
+    "blitz_hip_cameras": (C.c_int, [C.POINTER(Bounds), C.c_double, C.c_uint16, C.c_uint16, C.c_uint32,
+                                    C.POINTER(Camera), C.c_size_t]),
+    "blitz_hip_audit_create": (C.c_int, [C.POINTER(AuditOptions), C.POINTER(_P), *_ERR]),
+    "blitz_hip_audit_destroy": (None, [_P]),
+    "blitz_hip_mesh_upload": (C.c_int, [_P, C.POINTER(BlitzMesh), C.POINTER(_P), *_ERR]),
+    "blitz_hip_mesh_release": (None, [_P]),
+    "blitz_hip_audit_evaluate": (C.c_int, [_P, C.POINTER(BlitzMesh), C.POINTER(BlitzMesh), C.POINTER(Bounds),
+                                           C.POINTER(EvalSettingsC), C.POINTER(MeasurementC), *_ERR]),
+    "blitz_hip_audit_evaluate_device": (C.c_int, [_P, _P, _P, C.POINTER(Bounds), C.POINTER(EvalSettingsC),
+                                                  C.POINTER(MeasurementC), *_ERR]),
+    "blitz_hip_audit_evaluate_batch": (C.c_int, [_P, _P, C.POINTER(_P), C.c_size_t, C.POINTER(Bounds),
+                                                 C.POINTER(EvalSettingsC), C.POINTER(MeasurementC), *_ERR]),
+    "blitz_hip_raster_coverage": (C.c_int, [_P, _P, C.POINTER(Bounds), C.POINTER(Camera), C.c_double, C.c_uint8,
+                                            C.c_uint8, C.POINTER(C.c_uint64), C.c_size_t, C.POINTER(C.c_uint32),
+                                            C.POINTER(C.c_uint8), *_ERR]),
+    "blitz_hip_coverage_distance": (C.c_int, [_P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.c_uint32,
+                                              C.c_uint8, C.POINTER(C.c_double), *_ERR]),
+    "blitz_hip_audit_last_timing": (C.c_int, [_P, C.POINTER(AuditTiming)]),
+    "blitz_hip_reduce_reuse": (C.c_int, [C.POINTER(BlitzMesh), C.c_size_t, C.POINTER(C.c_uint32), C.c_size_t,
+                                         C.POINTER(C.c_size_t), *_ERR]),
+}
+
+_lib: C.CDLL | None = None
+
+
+def load() -> C.CDLL:
+    """Loads libblitz_hip.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIBRARY_PATH.exists():
+        raise RuntimeError(f"{LIBRARY_PATH} is missing: run __graft_entry__.build() (hipcc, gfx950) first")
+    lib = C.CDLL(str(LIBRARY_PATH))
+    for name, (res, args) in PROTOTYPES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+class BlitzError(RuntimeError):
+    def __init__(self, status: int, message: str):
+        super().__init__(f"{STATUS.get(status, status)}: {message}")
+        self.status = status
+
+
+def check(status: int, err: C.Array | None = None) -> None:
+    if status != 0:
+        raise BlitzError(status, err.value.decode(errors="replace") if err is not None else "")
+
+
+def error_buffer() -> C.Array:
+    return C.create_string_buffer(512)

@@ -1,0 +1,307 @@
+"""Python view of the HIP coverage audit (include/blitz_hip.h).
+
+Mirrors the reference's C++ audit interface for tests and benchmarks:
+``EvalSettings`` and ``Measurement`` follow include/blitz/evaluate.hpp:24-39,
+``Audit.evaluate`` is ``evaluate_gpu`` / ``AuditCuda::evaluate``
+(include/blitz/neural.hpp:148-149), ``Audit.raster_coverage`` is
+``detail::rasterize_coverage`` (src/coverage.hpp:12) and
+``Audit.coverage_distance`` is ``detail::coverage_distance`` (src/coverage.hpp:13).
+All work runs in libblitz_hip.so on a gfx950 device.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from dataclasses import dataclass, field, fields
+
+import numpy as np
+
+from . import _lib
+from ._lib import BlitzError
+
+PROFILE_COVERAGE, PROFILE_NORMALS, PROFILE_ATTRIBUTES = 0, 1, 2
+
+
+@dataclass
+class Mesh:
+    """Positions (n, 3) float32 and triangle indices (m, 3) uint32, with the
+    optional per-face materials and per-material two-sided flags of
+    blitz::MeshView (include/blitz/mesh.hpp:91-105)."""
+    positions: np.ndarray
+    indices: np.ndarray
+    materials: np.ndarray | None = None
+    double_sided: np.ndarray | None = None
+
+    def __post_init__(self) -> None:
+        self.positions = np.ascontiguousarray(self.positions, dtype=np.float32).reshape(-1, 3)
+        self.indices = np.ascontiguousarray(self.indices, dtype=np.uint32).reshape(-1, 3)
+        if self.materials is not None:
+            self.materials = np.ascontiguousarray(self.materials, dtype=np.uint16).reshape(-1)
+        if self.double_sided is not None:
+            self.double_sided = np.ascontiguousarray(self.double_sided, dtype=np.uint8).reshape(-1)
+
+    @property
+    def triangles(self) -> int:
+        return int(self.indices.shape[0])
+
+    def descriptor(self) -> _lib.BlitzMesh:
+        """ABI-5 blitz_mesh borrowing this mesh's arrays."""
+        d = _lib.BlitzMesh()
+        d.struct_size = C.sizeof(_lib.BlitzMesh)
+        d.abi_version = _lib.BLITZ_ABI_VERSION
+        n = self.positions.shape[0]
+        d.positions = _lib.BlitzStream(self.positions.ctypes.data if n else None, n, 12)
+        d.indices = self.indices.ctypes.data_as(C.POINTER(C.c_uint32)) if self.indices.size else None
+        d.index_count = int(self.indices.size)
+        if self.materials is not None and self.materials.size:
+            d.materials = self.materials.ctypes.data_as(C.POINTER(C.c_uint16))
+            d.material_count = int(self.materials.size)
+        if self.double_sided is not None and self.double_sided.size:
+            d.double_sided = self.double_sided.ctypes.data_as(C.POINTER(C.c_uint8))
+            d.double_sided_count = int(self.double_sided.size)
+        return d
+
+
+@dataclass
+class EvalSettings:
+    """blitz::EvalSettings defaults (include/blitz/evaluate.hpp:24-33)."""
+    profile: int = PROFILE_NORMALS
+    supersample: int = 8
+    max_supersample: int = 32
+    screen_size: float = 512.0
+    limit: float = 2.0
+    max_changed_area: float = 1.0
+    orthographic: int = 642
+    perspective: int = 64
+    rotation_seed: int = 0xB1172026
+    force_two_sided: bool = False
+    conservative_screen: bool = False
+    normal_weight: float = 180.0 / (10 * math.pi)
+    color_weight: float = 4.0
+    material_weight: float = 4.0
+    cancelled: object | None = None  # callable() -> bool, polled once per view
+
+    def to_c(self) -> tuple[_lib.EvalSettingsC, object]:
+        s = _lib.EvalSettingsC()
+        s.struct_size = C.sizeof(_lib.EvalSettingsC)
+        s.version = _lib.BLITZ_HIP_API_VERSION
+        s.profile = self.profile
+        s.supersample = self.supersample
+        s.max_supersample = self.max_supersample
+        s.force_two_sided = int(self.force_two_sided)
+        s.conservative_screen = int(self.conservative_screen)
+        s.views_orthographic = self.orthographic
+        s.views_perspective = self.perspective
+        s.views_rotation_seed = self.rotation_seed & 0xFFFFFFFF
+        s.screen_size = self.screen_size
+        s.limit = self.limit
+        s.max_changed_area = self.max_changed_area
+        s.normal_weight = self.normal_weight
+        s.color_weight = self.color_weight
+        s.material_weight = self.material_weight
+        keep = None
+        if self.cancelled is not None:
+            fn = self.cancelled
+            keep = _lib.CANCEL_FN(lambda _user: 1 if fn() else 0)
+            s.cancelled = keep
+        return s, keep
+
+
+@dataclass
+class Measurement:
+    """blitz::Measurement (include/blitz/evaluate.hpp:34-39)."""
+    error: float = 0.0
+    coverage: float = 0.0
+    coverage_upper: float = 0.0
+    changed_area: float = 0.0
+    normal_degrees: float = 0.0
+    worst_view: int = 0
+    changed_area_worst_view: int = 0
+    views_evaluated: int = 0
+    supersample: int = 0
+    complete: bool = True
+    passed: bool = True
+    resource_limited: bool = False
+    cancelled: bool = False
+
+    @classmethod
+    def from_c(cls, m) -> "Measurement":
+        return cls(**{f.name: (bool(getattr(m, f.name)) if f.type == "bool" else getattr(m, f.name))
+                      for f in fields(cls)})
+
+    def as_tuple(self) -> tuple:
+        return tuple(getattr(self, f.name) for f in fields(self))
+
+
+def bounds(mesh: Mesh) -> tuple[np.ndarray, float]:
+    """blitz::bounds (src/mesh.cpp:77-86) computed by the library."""
+    lib = _lib.load()
+    b = _lib.Bounds()
+    err = _lib.error_buffer()
+    desc = mesh.descriptor()
+    _lib.check(lib.blitz_hip_bounds_of(C.byref(desc), C.byref(b), err, len(err)), err)
+    return np.array(b.center[:], dtype=np.float32), float(b.radius)
+
+
+def reduce_reuse(mesh: Mesh, target_triangles: int) -> Mesh:
+    """Classical quadric reduction sharing the input vertices (host C++)."""
+    lib = _lib.load()
+    out = np.zeros(mesh.indices.size, np.uint32)
+    count = C.c_size_t()
+    err = _lib.error_buffer()
+    desc = mesh.descriptor()
+    _lib.check(lib.blitz_hip_reduce_reuse(C.byref(desc), int(target_triangles),
+                                          out.ctypes.data_as(C.POINTER(C.c_uint32)), out.size, C.byref(count),
+                                          err, len(err)), err)
+    return Mesh(mesh.positions, out[: count.value].reshape(-1, 3).copy(), None, mesh.double_sided)
+
+
+def _bounds_c(center, radius) -> _lib.Bounds:
+    b = _lib.Bounds()
+    c = np.asarray(center, dtype=np.float32)
+    b.center[0], b.center[1], b.center[2] = float(c[0]), float(c[1]), float(c[2])
+    b.radius = float(radius)
+    return b
+
+
+def cameras(center, radius, screen: float, orthographic: int, perspective: int, seed: int) -> list[_lib.Camera]:
+    """blitz::cameras (src/evaluate.cpp:108-124)."""
+    lib = _lib.load()
+    n = orthographic + perspective
+    out = (_lib.Camera * max(1, n))()
+    b = _bounds_c(center, radius)
+    _lib.check(lib.blitz_hip_cameras(C.byref(b), screen, orthographic, perspective, seed & 0xFFFFFFFF, out, n))
+    return list(out)[:n]
+
+
+class DeviceMesh:
+    """A mesh resident in HBM (DeviceMeshView, src/neural/device_view.hpp:16)."""
+
+    def __init__(self, audit: "Audit", handle: C.c_void_p, mesh: Mesh):
+        self.audit = audit
+        self.handle = handle
+        self.triangles = mesh.triangles
+        self.vertices = mesh.positions.shape[0]
+
+    def release(self) -> None:
+        if self.handle:
+            _lib.load().blitz_hip_mesh_release(self.handle)
+            self.handle = None
+
+    def __del__(self) -> None:
+        try:
+            self.release()
+        except Exception:
+            pass
+
+
+class Audit:
+    """One HIP audit object (AuditCuda counterpart) on one device."""
+
+    def __init__(self, device: int = 0, memory_mib: int = 0):
+        lib = _lib.load()
+        self.lib = lib
+        opts = _lib.AuditOptions()
+        _lib.check(lib.blitz_hip_audit_options_init(C.byref(opts), C.sizeof(opts)))
+        opts.device = device
+        opts.memory_mib = memory_mib
+        handle = C.c_void_p()
+        err = _lib.error_buffer()
+        _lib.check(lib.blitz_hip_audit_create(C.byref(opts), C.byref(handle), err, len(err)), err)
+        self.handle = handle
+        self._meshes: list[DeviceMesh] = []
+
+    def close(self) -> None:
+        for m in self._meshes:
+            m.release()
+        self._meshes.clear()
+        if self.handle:
+            self.lib.blitz_hip_audit_destroy(self.handle)
+            self.handle = None
+
+    def __enter__(self) -> "Audit":
+        return self
+
+    def __exit__(self, *exc) -> None:
+        self.close()
+
+    def upload(self, mesh: Mesh) -> DeviceMesh:
+        handle = C.c_void_p()
+        err = _lib.error_buffer()
+        desc = mesh.descriptor()
+        _lib.check(self.lib.blitz_hip_mesh_upload(self.handle, C.byref(desc), C.byref(handle), err, len(err)), err)
+        d = DeviceMesh(self, handle, mesh)
+        self._meshes.append(d)
+        return d
+
+    def evaluate(self, reference, candidate, center, radius, settings: EvalSettings) -> Measurement:
+        """Coverage audit of candidate against reference (host or device meshes)."""
+        s, keep = settings.to_c()
+        b = _bounds_c(center, radius)
+        out = _lib.MeasurementC()
+        err = _lib.error_buffer()
+        if isinstance(reference, DeviceMesh) and isinstance(candidate, DeviceMesh):
+            st = self.lib.blitz_hip_audit_evaluate_device(self.handle, reference.handle, candidate.handle,
+                                                          C.byref(b), C.byref(s), C.byref(out), err, len(err))
+        else:
+            dr, dc = reference.descriptor(), candidate.descriptor()
+            st = self.lib.blitz_hip_audit_evaluate(self.handle, C.byref(dr), C.byref(dc), C.byref(b), C.byref(s),
+                                                   C.byref(out), err, len(err))
+        del keep
+        _lib.check(st, err)
+        return Measurement.from_c(out)
+
+    def evaluate_batch(self, reference: DeviceMesh, candidates: list[DeviceMesh], center, radius,
+                       settings: EvalSettings) -> list[Measurement]:
+        s, keep = settings.to_c()
+        b = _bounds_c(center, radius)
+        n = len(candidates)
+        handles = (C.c_void_p * max(1, n))(*[c.handle for c in candidates])
+        out = (_lib.MeasurementC * max(1, n))()
+        err = _lib.error_buffer()
+        st = self.lib.blitz_hip_audit_evaluate_batch(self.handle, reference.handle, handles, n, C.byref(b),
+                                                     C.byref(s), out, err, len(err))
+        del keep
+        _lib.check(st, err)
+        return [Measurement.from_c(out[i]) for i in range(n)]
+
+    def raster_coverage(self, mesh: DeviceMesh, center, radius, camera: _lib.Camera, screen: float, ss: int,
+                        force_two_sided: bool = False):
+        """Row-major packed mask words, raster side and clipped flag; None when
+        the raster exceeds the 64M-sample cap."""
+        size = math.ceil(screen + 8) * ss
+        capacity = (size * size + 63) // 64
+        words = np.zeros(max(1, capacity), dtype=np.uint64)
+        out_size = C.c_uint32()
+        clipped = C.c_uint8()
+        b = _bounds_c(center, radius)
+        err = _lib.error_buffer()
+        st = self.lib.blitz_hip_raster_coverage(self.handle, mesh.handle, C.byref(b), C.byref(camera), screen, ss,
+                                                int(force_two_sided),
+                                                words.ctypes.data_as(C.POINTER(C.c_uint64)), capacity,
+                                                C.byref(out_size), C.byref(clipped), err, len(err))
+        if st == 5:
+            return None
+        _lib.check(st, err)
+        return words[:capacity], int(out_size.value), bool(clipped.value)
+
+    def coverage_distance(self, a: np.ndarray, b: np.ndarray, size: int, ss: int) -> float:
+        a = np.ascontiguousarray(a, dtype=np.uint64)
+        b = np.ascontiguousarray(b, dtype=np.uint64)
+        out = C.c_double()
+        err = _lib.error_buffer()
+        _lib.check(self.lib.blitz_hip_coverage_distance(self.handle, a.ctypes.data_as(C.POINTER(C.c_uint64)),
+                                                         b.ctypes.data_as(C.POINTER(C.c_uint64)), size, ss,
+                                                         C.byref(out), err, len(err)), err)
+        return float(out.value)
+
+    def last_timing(self) -> dict:
+        t = _lib.AuditTiming()
+        t.struct_size = C.sizeof(t)
+        _lib.check(self.lib.blitz_hip_audit_last_timing(self.handle, C.byref(t)))
+        return {name: getattr(t, name) for name, _ in t._fields_ if name != "struct_size"}
+
+
+__all__ = ["Mesh", "EvalSettings", "Measurement", "Audit", "DeviceMesh", "bounds", "cameras", "reduce_reuse",
+           "BlitzError",
+           "PROFILE_COVERAGE", "PROFILE_NORMALS", "PROFILE_ATTRIBUTES"]

@@ -1,0 +1,580 @@
+// Host controller of the HIP coverage audit.
+//
+// The GPU measures every (candidate, view) pair of one refinement round in a
+// single set of launches; the host then replays the reference's sequential
+// per-view control flow (src/evaluate.cpp:312-352) over those measurements:
+// refinement while uncertain, stop at the first failing view, resource limit
+// at the 64M-sample cap, cancellation polled once per view. Because each
+// view's measurement depends only on that view, measuring views beyond the
+// first failure (which the reference never visits) changes no output field.
+#include "audit.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <map>
+#include <new>
+
+#include "audit_kernels.hpp"
+
+namespace blitz_hip {
+namespace {
+constexpr double kPi = 3.14159265358979323846;
+constexpr double kInf = std::numeric_limits<double>::infinity();
+constexpr uint64_t kMaxSamples = 64000000;  // src/evaluate.cpp:129, src/neural/internal.hpp:11
+constexpr uint32_t kNone = 0xFFFFFFFFu;
+
+double dot(F3 a, F3 b) { return double(a.x) * b.x + double(a.y) * b.y + double(a.z) * b.z; }
+F3 cross(F3 a, F3 b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+F3 normalized(F3 a) {
+    const double l = std::sqrt(dot(a, a));
+    if (!(l > 0)) return {0, 0, 0};
+    const double s = 1 / l;
+    return {float(a.x * s), float(a.y * s), float(a.z * s)};
+}
+// Seeded rotation of the view directions (src/evaluate.cpp:16-20).
+F3 rotate(F3 v, uint32_t seed) {
+    const double a = (seed % 65521) * .000731, b = ((seed >> 8) % 32749) * .000913;
+    const double x = v.x * std::cos(a) - v.y * std::sin(a), y = v.x * std::sin(a) + v.y * std::cos(a);
+    return {float(x), float(y * std::cos(b) - v.z * std::sin(b)), float(y * std::sin(b) + v.z * std::cos(b))};
+}
+
+struct Clock {
+    std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+    double ms() const {
+        return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
+};
+}  // namespace
+
+void hip_check(hipError_t e, const char* what) {
+    if (e == hipSuccess) return;
+    if (e == hipErrorOutOfMemory) throw std::bad_alloc();
+    throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+DeviceGuard::DeviceGuard(int device) {
+    hip_check(hipGetDevice(&previous), "hipGetDevice");
+    if (previous != device) hip_check(hipSetDevice(device), "hipSetDevice");
+}
+DeviceGuard::~DeviceGuard() {
+    if (previous >= 0) (void)hipSetDevice(previous);
+}
+
+// src/mesh.cpp:77-86
+Bounds bounds_of(const float* p, size_t n) {
+    if (!n) return {{0, 0, 0}, 0};
+    F3 lo{p[0], p[1], p[2]}, hi = lo;
+    for (size_t i = 1; i < n; ++i) {
+        const F3 q{p[3 * i], p[3 * i + 1], p[3 * i + 2]};
+        lo = {std::min(lo.x, q.x), std::min(lo.y, q.y), std::min(lo.z, q.z)};
+        hi = {std::max(hi.x, q.x), std::max(hi.y, q.y), std::max(hi.z, q.z)};
+    }
+    Bounds b{{float((double(lo.x) + hi.x) * .5), float((double(lo.y) + hi.y) * .5), float((double(lo.z) + hi.z) * .5)},
+             0};
+    for (size_t i = 0; i < n; ++i) {
+        const double x = double(p[3 * i]) - b.center.x, y = double(p[3 * i + 1]) - b.center.y,
+                     z = double(p[3 * i + 2]) - b.center.z;
+        b.radius = std::max(b.radius, std::sqrt(x * x + y * y + z * z));
+    }
+    return b;
+}
+
+// Fibonacci-sphere cameras, host libm (src/evaluate.cpp:108-124).
+std::vector<Camera> cameras(const Bounds& b, double screen, uint16_t ortho, uint16_t persp, uint32_t seed) {
+    std::vector<Camera> out;
+    out.reserve(size_t(ortho) + persp);
+    for (int projection = 0; projection < 2; ++projection) {
+        const unsigned count = projection ? persp : ortho;
+        for (unsigned i = 0; i < count; ++i) {
+            const double z = 1 - 2 * (i + .5) / count, r = std::sqrt(std::max(0.0, 1 - z * z));
+            const double angle = i * kPi * (3 - std::sqrt(5.0));
+            const F3 forward = normalized(
+                rotate({float(r * std::cos(angle)), float(z), float(r * std::sin(angle))}, seed + uint32_t(projection) * 104729u));
+            const F3 hint = std::abs(forward.y) > .99 ? F3{1, 0, 0} : F3{0, 1, 0};
+            const F3 right = normalized(cross(hint, forward)), up = normalized(cross(forward, right));
+            const double height = std::max(1024.0, 2 * screen), focal = height / (2 * std::tan(kPi / 6));
+            const double distance = b.radius * std::sqrt(1 + std::pow(2 * focal / screen, 2));
+            out.push_back({right, up, forward, distance, focal, screen / (b.radius * 2), uint8_t(projection)});
+        }
+    }
+    return out;
+}
+
+// src/evaluation_settings.hpp:10-27
+void validate_settings(const Bounds& b, const Settings& s) {
+    if (!(s.screen_size > 0) || !std::isfinite(s.screen_size) || s.screen_size > 16384 || !(b.radius > 0) ||
+        !std::isfinite(b.radius) || !std::isfinite(b.center.x) || !std::isfinite(b.center.y) ||
+        !std::isfinite(b.center.z) || unsigned(s.profile) > 2 || !(s.limit >= 0) || !std::isfinite(s.limit) ||
+        !s.supersample || s.max_supersample < s.supersample || s.max_supersample > 32 ||
+        !std::isfinite(s.normal_weight) || !std::isfinite(s.color_weight) || !std::isfinite(s.material_weight) ||
+        s.normal_weight < 0 || s.color_weight < 0 || s.material_weight < 0 || s.normal_weight > 1e12 ||
+        s.color_weight > 1e12 || s.material_weight > 1e12 || !std::isfinite(s.max_changed_area) ||
+        s.max_changed_area < 0 || s.max_changed_area > 1)
+        throw std::invalid_argument("invalid evaluation settings");
+    if (!s.orthographic && !s.perspective) throw std::invalid_argument("at least one camera required");
+}
+
+uint32_t raster_side(double screen, unsigned ss) { return uint32_t(std::ceil(screen + 8)) * ss; }
+
+DeviceMesh::~DeviceMesh() {
+    (void)hipFree(positions);
+    (void)hipFree(indices);
+    (void)hipFree(two_sided);
+}
+
+Audit::Audit(int device, uint64_t memory_bytes) : device_(device), memory_bytes_(memory_bytes) {
+    DeviceGuard guard(device_);
+    hip_check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
+    for (auto& e : events_) hip_check(hipEventCreate(&e), "hipEventCreate");
+}
+
+Audit::~Audit() {
+    DeviceGuard guard(device_);
+    (void)hipStreamSynchronize(stream_);
+    for (auto& s : slots_) (void)hipFree(s.first);
+    for (auto& e : events_) (void)hipEventDestroy(e);
+    (void)hipStreamDestroy(stream_);
+}
+
+void* Audit::scratch(size_t slot, size_t bytes) {
+    if (slots_.size() <= slot) slots_.resize(slot + 1, {nullptr, 0});
+    auto& s = slots_[slot];
+    if (s.second < bytes) {
+        hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+        hip_check(hipFree(s.first), "hipFree");
+        const size_t grown = std::max(bytes, s.second + s.second / 2);
+        s = {nullptr, 0};
+        hip_check(hipMalloc(&s.first, grown), "hipMalloc");
+        s.second = grown;
+    }
+    return s.first;
+}
+
+std::unique_ptr<DeviceMesh> Audit::upload(const HostMesh& m) {
+    DeviceGuard guard(device_);
+    auto d = std::make_unique<DeviceMesh>();
+    d->owner = this;
+    d->vertices = uint32_t(m.positions.size() / 3);
+    d->faces = uint32_t(m.indices.size() / 3);
+    d->digest[0] = m.digest[0];
+    d->digest[1] = m.digest[1];
+    d->serial = next_serial_++;
+    std::vector<float4> pos(d->vertices);
+    for (uint32_t i = 0; i < d->vertices; ++i)
+        pos[i] = make_float4(m.positions[3 * i], m.positions[3 * i + 1], m.positions[3 * i + 2], 0.f);
+    hip_check(hipMalloc(&d->positions, std::max<size_t>(1, pos.size()) * sizeof(float4)), "hipMalloc");
+    hip_check(hipMalloc(&d->indices, std::max<size_t>(1, m.indices.size()) * sizeof(uint32_t)), "hipMalloc");
+    if (!pos.empty())
+        hip_check(hipMemcpyAsync(d->positions, pos.data(), pos.size() * sizeof(float4), hipMemcpyHostToDevice, stream_),
+                  "upload positions");
+    if (!m.indices.empty())
+        hip_check(hipMemcpyAsync(d->indices, m.indices.data(), m.indices.size() * 4, hipMemcpyHostToDevice, stream_),
+                  "upload indices");
+    // Per-face sidedness (include/blitz/mesh.hpp:103-104).
+    std::vector<uint8_t> two(d->faces, 0);
+    bool any = false;
+    if (!m.double_sided.empty())
+        for (uint32_t f = 0; f < d->faces; ++f) {
+            const uint16_t mat = m.materials.empty() ? 0 : m.materials[f];
+            two[f] = mat < m.double_sided.size() && m.double_sided[mat];
+            any |= two[f] != 0;
+        }
+    if (any) {
+        hip_check(hipMalloc(&d->two_sided, two.size()), "hipMalloc");
+        hip_check(hipMemcpyAsync(d->two_sided, two.data(), two.size(), hipMemcpyHostToDevice, stream_),
+                  "upload sidedness");
+    }
+    hip_check(hipStreamSynchronize(stream_), "upload");
+    return d;
+}
+
+void Audit::release(DeviceMesh*) {}
+
+namespace {
+// Coverage-profile view finish (src/evaluate.cpp:271-298).
+Measurement finish_view(double coverage, uint64_t changed, uint64_t total, bool clipped, const Settings& s, uint8_t ss) {
+    Measurement cur;
+    cur.supersample = ss;
+    cur.coverage = coverage;
+    cur.coverage_upper = cur.coverage + 2 * std::sqrt(2.0) / ss + 1e-6;
+    if (clipped) cur.coverage_upper = kInf;
+    cur.error = cur.coverage_upper;
+    cur.changed_area = total ? double(changed) / double(total) : 0;
+    cur.passed = cur.error <= s.limit && cur.changed_area <= s.max_changed_area;
+    if (s.conservative_screen) {
+        cur.error = clipped ? 0 : std::max(0.0, cur.coverage - 2 * std::sqrt(2.0) / ss - 1e-6);
+        cur.passed = cur.error <= s.limit;
+    }
+    return cur;
+}
+// Refinement stop rule (src/evaluate.cpp:336-337).
+bool settled(const Measurement& cur, const Settings& s, unsigned ss) {
+    return s.conservative_screen || cur.passed || ss >= s.max_supersample ||
+           cur.coverage - 2 * std::sqrt(2.0) / ss > s.limit || !std::isfinite(cur.error) ||
+           cur.coverage_upper <= s.limit;
+}
+}  // namespace
+
+Measurement Audit::evaluate(const DeviceMesh& reference, const DeviceMesh& candidate, const Bounds& b,
+                            const Settings& s) {
+    return evaluate_batch(reference, {&candidate}, b, s)[0];
+}
+
+std::vector<Measurement> Audit::evaluate_batch(const DeviceMesh& reference,
+                                               const std::vector<const DeviceMesh*>& candidates, const Bounds& b,
+                                               const Settings& s) {
+    validate_settings(b, s);
+    if (s.profile != 0)
+        throw Unavailable("the HIP audit backend implements the coverage profile only");
+    Clock clock;
+    timing_ = {};
+    const auto cams = cameras(b, s.screen_size, s.orthographic, s.perspective, s.rotation_seed);
+    const uint32_t views = uint32_t(cams.size());
+    const size_t n = candidates.size();
+    struct State {
+        uint8_t ss = 0;
+        bool final = false, limited = false;
+        Measurement m;
+    };
+    std::vector<std::vector<State>> state(n, std::vector<State>(views));
+    std::vector<bool> identical(n, false);
+    for (size_t c = 0; c < n; ++c) {
+        const DeviceMesh& m = *candidates[c];
+        // src/evaluate.cpp:77-82,317: byte-identical streams skip every view.
+        identical[c] = &m == &reference ||
+                       (m.digest[0] == reference.digest[0] && m.digest[1] == reference.digest[1] &&
+                        m.vertices == reference.vertices && m.faces == reference.faces);
+        for (auto& st : state[c]) st.ss = s.supersample;
+    }
+    DeviceGuard guard(device_);
+    for (;;) {
+        std::vector<Item> items;
+        for (size_t c = 0; c < n; ++c) {
+            if (identical[c]) continue;
+            for (uint32_t v = 0; v < views; ++v) {
+                State& st = state[c][v];
+                if (st.final) {
+                    if (st.limited || !st.m.passed) break;
+                    continue;
+                }
+                const uint64_t side = raster_side(s.screen_size, st.ss);
+                if (side * side > kMaxSamples) {
+                    st.final = st.limited = true;
+                    break;
+                }
+                items.push_back({uint32_t(c), v, st.ss});
+            }
+        }
+        if (items.empty()) break;
+        // All pending views of one round share one supersampling: every view
+        // starts at s.supersample and refines by the same doubling rule.
+        std::vector<ViewResult> results;
+        run_items(reference, candidates, cams, b, s, items, results);
+        ++timing_.passes;
+        for (size_t i = 0; i < items.size(); ++i) {
+            const Item& it = items[i];
+            State& st = state[it.candidate][it.view];
+            st.m = finish_view(results[i].coverage, results[i].changed, results[i].total, results[i].clipped, s, it.ss);
+            if (settled(st.m, s, it.ss))
+                st.final = true;
+            else
+                st.ss = uint8_t(std::min<unsigned>(s.max_supersample, unsigned(it.ss) * 2));
+        }
+    }
+    // Sequential assembly, exactly as the reference's view loop.
+    std::vector<Measurement> out(n);
+    for (size_t c = 0; c < n; ++c) {
+        Measurement& r = out[c];
+        r.supersample = s.supersample;
+        if (identical[c]) continue;
+        for (uint32_t v = 0; v < views; ++v) {
+            if (s.cancelled && s.cancelled()) {
+                r.complete = r.passed = false;
+                r.cancelled = true;
+                break;
+            }
+            const State& st = state[c][v];
+            if (st.limited) {
+                r.complete = r.passed = false;
+                r.resource_limited = true;
+                r.error = kInf;
+                break;
+            }
+            const Measurement& cur = st.m;
+            ++r.views_evaluated;
+            if (cur.error > r.error) {
+                r.worst_view = v;
+                r.error = cur.error;
+                r.supersample = cur.supersample;
+            }
+            r.coverage = std::max(r.coverage, cur.coverage);
+            r.coverage_upper = std::max(r.coverage_upper, cur.coverage_upper);
+            if (cur.changed_area > r.changed_area) {
+                r.changed_area = cur.changed_area;
+                r.changed_area_worst_view = v;
+            }
+            if (!cur.passed) {
+                r.passed = r.complete = false;
+                break;
+            }
+        }
+    }
+    timing_.total_ms = clock.ms();
+    return out;
+}
+
+void Audit::run_items(const DeviceMesh& reference, const std::vector<const DeviceMesh*>& candidates,
+                      const std::vector<Camera>& cams, const Bounds& b, const Settings& s,
+                      const std::vector<Item>& all, std::vector<ViewResult>& out) {
+    out.assign(all.size(), {});
+    const uint8_t ss = all.front().ss;
+    const uint32_t size = raster_side(s.screen_size, ss), tiles = (size + kTile - 1) / kTile;
+    const size_t words = size_t(tiles) * tiles;
+    const size_t samples = size_t(size) * size;
+    // Per item: candidate mask, two row-distance planes, summary. Per view:
+    // one reference mask shared by every candidate of the chunk.
+    const size_t per_item = words * 8 + samples * 4 + sizeof(ViewSummary) + sizeof(RasterTask) + sizeof(ViewPair);
+    const uint64_t budget = memory_bytes_ ? memory_bytes_ : (uint64_t(8) << 30);
+    const size_t chunk = std::max<size_t>(1, std::min<size_t>(all.size(), budget / (per_item + words * 8)));
+    for (size_t begin = 0; begin < all.size(); begin += chunk) {
+        const size_t count = std::min(chunk, all.size() - begin);
+        // Reference raster tasks, one per distinct view in this chunk.
+        std::map<uint32_t, uint32_t> ref_slot;
+        for (size_t i = 0; i < count; ++i) ref_slot.emplace(all[begin + i].view, 0u);
+        uint32_t next = 0;
+        for (auto& kv : ref_slot) kv.second = next++;
+        const uint32_t nref = next, ncand = uint32_t(count), ntasks = nref + ncand;
+        uint64_t* ref_masks = scratch_as<uint64_t>(0, size_t(nref) * words);
+        uint64_t* cand_masks = scratch_as<uint64_t>(1, size_t(ncand) * words);
+        uint16_t* dist = scratch_as<uint16_t>(2, size_t(ncand) * samples * 2);
+        ViewSummary* summaries = scratch_as<ViewSummary>(3, ncand);
+        RasterTask* dtasks = scratch_as<RasterTask>(4, ntasks);
+        ViewPair* dpairs = scratch_as<ViewPair>(5, ncand);
+        uint32_t* status = scratch_as<uint32_t>(6, size_t(ntasks) * 2);
+
+        auto make_task = [&](const DeviceMesh& m, const Camera& cam, uint64_t* mask, uint32_t* st) {
+            RasterTask t{};
+            t.positions = m.positions;
+            t.indices = m.indices;
+            t.two_sided = m.two_sided;
+            t.mask = mask;
+            t.status = st;
+            t.camera = cam;
+            t.center = b.center;
+            t.faces = m.faces;
+            t.size = size;
+            t.tiles = tiles;
+            t.ss = ss;
+            t.force_two = s.force_two_sided;
+            return t;
+        };
+        std::vector<RasterTask> tasks(ntasks);
+        std::vector<ViewPair> pairs(ncand);
+        uint32_t max_faces = 0;
+        for (auto& kv : ref_slot) {
+            tasks[kv.second] = make_task(reference, cams[kv.first], ref_masks + size_t(kv.second) * words,
+                                         status + 2 * kv.second);
+            max_faces = std::max(max_faces, reference.faces);
+        }
+        for (uint32_t i = 0; i < ncand; ++i) {
+            const Item& it = all[begin + i];
+            const DeviceMesh& m = *candidates[it.candidate];
+            tasks[nref + i] = make_task(m, cams[it.view], cand_masks + size_t(i) * words, status + 2 * (nref + i));
+            max_faces = std::max(max_faces, m.faces);
+            ViewPair& p = pairs[i];
+            p.a = ref_masks + size_t(ref_slot[it.view]) * words;
+            p.c = cand_masks + size_t(i) * words;
+            p.dist_a = dist + size_t(i) * samples * 2;
+            p.dist_c = p.dist_a + samples;
+            p.size = size;
+            p.tiles = tiles;
+        }
+        std::vector<uint32_t> status_init(size_t(ntasks) * 2);
+        for (uint32_t i = 0; i < ntasks; ++i) {
+            status_init[2 * i] = kNone;
+            status_init[2 * i + 1] = 0;
+        }
+        hip_check(hipMemcpyAsync(dtasks, tasks.data(), tasks.size() * sizeof(RasterTask), hipMemcpyHostToDevice, stream_),
+                  "upload tasks");
+        hip_check(hipMemcpyAsync(dpairs, pairs.data(), pairs.size() * sizeof(ViewPair), hipMemcpyHostToDevice, stream_),
+                  "upload pairs");
+        hip_check(hipMemcpyAsync(status, status_init.data(), status_init.size() * 4, hipMemcpyHostToDevice, stream_),
+                  "upload status");
+        launch_fill(ref_masks, size_t(nref) * words, 0, stream_);
+        launch_fill(cand_masks, size_t(ncand) * words, 0, stream_);
+        launch_fill(reinterpret_cast<uint64_t*>(summaries), ncand * sizeof(ViewSummary) / 8, 0, stream_);
+        hip_check(hipEventRecord(events_[0], stream_), "event");
+        launch_raster(dtasks, ntasks, max_faces, stream_);
+        hip_check(hipEventRecord(events_[1], stream_), "event");
+        launch_compare(dpairs, summaries, ncand, tiles, stream_);
+        hip_check(hipEventRecord(events_[2], stream_), "event");
+        launch_distance(dpairs, summaries, ncand, size, tiles, stream_);
+        hip_check(hipEventRecord(events_[3], stream_), "event");
+        hip_check(hipGetLastError(), "audit launch");
+        std::vector<ViewSummary> sum(ncand);
+        std::vector<uint32_t> st(size_t(ntasks) * 2);
+        hip_check(hipMemcpyAsync(sum.data(), summaries, ncand * sizeof(ViewSummary), hipMemcpyDeviceToHost, stream_),
+                  "download summaries");
+        hip_check(hipMemcpyAsync(st.data(), status, st.size() * 4, hipMemcpyDeviceToHost, stream_), "download status");
+        hip_check(hipStreamSynchronize(stream_), "audit");
+        float t_r = 0, t_c = 0, t_d = 0;
+        hip_check(hipEventElapsedTime(&t_r, events_[0], events_[1]), "elapsed");
+        hip_check(hipEventElapsedTime(&t_c, events_[1], events_[2]), "elapsed");
+        hip_check(hipEventElapsedTime(&t_d, events_[2], events_[3]), "elapsed");
+        timing_.raster_ms += t_r;
+        timing_.compare_ms += t_c;
+        timing_.distance_ms += t_d;
+        timing_.raster_tasks += ntasks;
+        timing_.distance_views += ncand;
+        for (uint32_t i = 0; i < ntasks; ++i) timing_.raster_triangles += tasks[i].faces;
+        timing_.mask_bytes += uint64_t(ntasks) * words * 8;
+
+        // The reference stops rasterizing at the first out-of-frame face
+        // (src/evaluate.cpp:159-160). Redraw such masks from faces before it.
+        std::vector<uint32_t> redo;
+        for (uint32_t i = 0; i < ntasks; ++i)
+            if (st[2 * i] != kNone) {
+                tasks[i].faces = st[2 * i];
+                redo.push_back(i);
+            }
+        if (!redo.empty()) {
+            std::vector<RasterTask> again;
+            for (uint32_t i : redo) {
+                again.push_back(tasks[i]);
+                launch_fill(tasks[i].mask, words, 0, stream_);
+            }
+            RasterTask* dredo = scratch_as<RasterTask>(7, again.size());
+            uint32_t faces = 0;
+            for (auto& t : again) faces = std::max(faces, t.faces);
+            hip_check(hipMemcpyAsync(dredo, again.data(), again.size() * sizeof(RasterTask), hipMemcpyHostToDevice, stream_),
+                      "upload redo");
+            launch_raster(dredo, uint32_t(again.size()), faces, stream_);
+            launch_fill(reinterpret_cast<uint64_t*>(summaries), ncand * sizeof(ViewSummary) / 8, 0, stream_);
+            launch_compare(dpairs, summaries, ncand, tiles, stream_);
+            launch_distance(dpairs, summaries, ncand, size, tiles, stream_);
+            hip_check(hipGetLastError(), "audit redo launch");
+            hip_check(hipMemcpyAsync(sum.data(), summaries, ncand * sizeof(ViewSummary), hipMemcpyDeviceToHost, stream_),
+                      "download summaries");
+            hip_check(hipStreamSynchronize(stream_), "audit redo");
+        }
+        for (uint32_t i = 0; i < ncand; ++i) {
+            const Item& it = all[begin + i];
+            const uint32_t ra = ref_slot[it.view], rc = nref + i;
+            const bool clipped = st[2 * ra] != kNone || st[2 * ra + 1] || st[2 * rc] != kNone || st[2 * rc + 1];
+            const ViewSummary& v = sum[i];
+            ViewResult& r = out[begin + i];
+            r.changed = v.changed;
+            r.total = v.total;
+            r.clipped = clipped;
+            // src/evaluate.cpp:202,208: identical masks give 0, exactly one
+            // empty mask gives infinity, otherwise the directed maxima of the
+            // float distance field, rounded once here (rounding is monotone).
+            if (!v.changed)
+                r.coverage = 0;
+            else if ((v.count_a != 0) != (v.count_c != 0))
+                r.coverage = kInf;
+            else
+                r.coverage = std::sqrt(double(float(std::max(v.max_ac, v.max_ca)))) / ss;
+        }
+    }
+}
+
+bool Audit::raster_row_major(const DeviceMesh& m, const Bounds& b, const Camera& c, double screen, uint8_t ss,
+                             bool force_two, std::vector<uint64_t>& words_out, uint32_t& size_out, bool& clipped) {
+    if (!ss || !std::isfinite(screen) || screen <= 0 || screen > 16384)
+        throw std::invalid_argument("invalid raster extent");
+    const uint32_t size = raster_side(screen, ss);
+    size_out = size;
+    if (uint64_t(size) * size > kMaxSamples) return false;
+    DeviceGuard guard(device_);
+    const uint32_t tiles = (size + kTile - 1) / kTile;
+    const size_t words = size_t(tiles) * tiles;
+    uint64_t* mask = scratch_as<uint64_t>(0, words);
+    uint32_t* status = scratch_as<uint32_t>(6, 2);
+    RasterTask* dtask = scratch_as<RasterTask>(4, 1);
+    RasterTask t{};
+    t.positions = m.positions;
+    t.indices = m.indices;
+    t.two_sided = m.two_sided;
+    t.mask = mask;
+    t.status = status;
+    t.camera = c;
+    t.center = b.center;
+    t.faces = m.faces;
+    t.size = size;
+    t.tiles = tiles;
+    t.ss = ss;
+    t.force_two = force_two;
+    std::vector<uint32_t> st{kNone, 0};
+    clipped = false;
+    for (int pass = 0; pass < 2; ++pass) {
+        hip_check(hipMemcpyAsync(dtask, &t, sizeof t, hipMemcpyHostToDevice, stream_), "upload task");
+        hip_check(hipMemcpyAsync(status, st.data(), 8, hipMemcpyHostToDevice, stream_), "upload status");
+        launch_fill(mask, words, 0, stream_);
+        launch_raster(dtask, 1, t.faces, stream_);
+        hip_check(hipGetLastError(), "raster launch");
+        std::vector<uint32_t> got(2);
+        hip_check(hipMemcpyAsync(got.data(), status, 8, hipMemcpyDeviceToHost, stream_), "download status");
+        hip_check(hipStreamSynchronize(stream_), "raster");
+        // The redraw from faces before the first out-of-frame face keeps the
+        // clipped flag of the full pass.
+        clipped = clipped || got[0] != kNone || got[1];
+        if (got[0] == kNone) break;
+        t.faces = got[0];
+    }
+    std::vector<uint64_t> tiled(words);
+    hip_check(hipMemcpy(tiled.data(), mask, words * 8, hipMemcpyDeviceToHost), "download mask");
+    words_out.assign((size_t(size) * size + 63) / 64, 0);
+    for (uint32_t ty = 0; ty < tiles; ++ty)
+        for (uint32_t tx = 0; tx < tiles; ++tx) {
+            uint64_t w = tiled[size_t(ty) * tiles + tx];
+            while (w) {
+                const int bit = __builtin_ctzll(w);
+                w &= w - 1;
+                const size_t x = tx * 8 + (bit & 7), y = ty * 8 + (bit >> 3);
+                const size_t index = y * size + x;
+                words_out[index / 64] |= uint64_t(1) << (index % 64);
+            }
+        }
+    return true;
+}
+
+double Audit::coverage_distance(const std::vector<uint64_t>& a, const std::vector<uint64_t>& b, uint32_t size,
+                                uint8_t ss) {
+    if (!ss) throw std::invalid_argument("raster shapes differ");
+    const size_t need = (size_t(size) * size + 63) / 64;
+    if (a.size() < need || b.size() < need) throw std::invalid_argument("raster shapes differ");
+    DeviceGuard guard(device_);
+    const uint32_t tiles = (size + kTile - 1) / kTile;
+    const size_t words = size_t(tiles) * tiles, samples = size_t(size) * size;
+    std::vector<uint64_t> ta(words, 0), tb(words, 0);
+    for (size_t i = 0; i < samples; ++i) {
+        const size_t x = i % size, y = i / size, w = (y / 8) * tiles + x / 8;
+        const uint64_t bit = uint64_t(1) << ((y & 7) * 8 + (x & 7));
+        if ((a[i / 64] >> (i % 64)) & 1) ta[w] |= bit;
+        if ((b[i / 64] >> (i % 64)) & 1) tb[w] |= bit;
+    }
+    uint64_t* masks = scratch_as<uint64_t>(0, words * 2);
+    uint16_t* dist = scratch_as<uint16_t>(2, samples * 2);
+    ViewSummary* summary = scratch_as<ViewSummary>(3, 1);
+    ViewPair* pair = scratch_as<ViewPair>(5, 1);
+    ViewPair p{masks, masks + words, dist, dist + samples, size, tiles};
+    hip_check(hipMemcpyAsync(masks, ta.data(), words * 8, hipMemcpyHostToDevice, stream_), "upload a");
+    hip_check(hipMemcpyAsync(masks + words, tb.data(), words * 8, hipMemcpyHostToDevice, stream_), "upload b");
+    hip_check(hipMemcpyAsync(pair, &p, sizeof p, hipMemcpyHostToDevice, stream_), "upload pair");
+    launch_fill(reinterpret_cast<uint64_t*>(summary), sizeof(ViewSummary) / 8, 0, stream_);
+    launch_compare(pair, summary, 1, tiles, stream_);
+    launch_distance(pair, summary, 1, size, tiles, stream_);
+    hip_check(hipGetLastError(), "distance launch");
+    ViewSummary v{};
+    hip_check(hipMemcpyAsync(&v, summary, sizeof v, hipMemcpyDeviceToHost, stream_), "download summary");
+    hip_check(hipStreamSynchronize(stream_), "distance");
+    if (!v.changed) return 0;
+    if ((v.count_a != 0) != (v.count_c != 0)) return kInf;
+    return std::sqrt(double(float(std::max(v.max_ac, v.max_ca)))) / ss;
+}
+
+}  // namespace blitz_hip

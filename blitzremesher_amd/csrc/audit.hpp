@@ -1,0 +1,136 @@
+// Host controller of the HIP coverage audit: the counterpart of the
+// reference's AuditCuda (src/neural/internal.hpp:131-175) with the CPU
+// audit's semantics (src/evaluate.cpp:312-352).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <functional>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "audit_types.hpp"
+
+namespace blitz_hip {
+
+// Raised for settings the HIP backend does not implement (maps to
+// BLITZ_UNAVAILABLE, as blitz::NeuralUnavailable does).
+struct Unavailable : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+void hip_check(hipError_t e, const char* what);
+
+struct Settings {
+    uint8_t profile = 1;  // EvalSettings defaults: Normals
+    uint8_t supersample = 8, max_supersample = 32;
+    bool force_two_sided = false, conservative_screen = false;
+    uint16_t orthographic = 642, perspective = 64;
+    uint32_t rotation_seed = 0xB1172026u;
+    double screen_size = 512, limit = 2, max_changed_area = 1;
+    double normal_weight = 180.0 / (10 * 3.14159265358979323846), color_weight = 4, material_weight = 4;
+    std::function<bool()> cancelled;
+};
+
+struct Measurement {
+    double error = 0, coverage = 0, coverage_upper = 0, changed_area = 0, normal_degrees = 0;
+    uint32_t worst_view = 0, changed_area_worst_view = 0, views_evaluated = 0;
+    uint8_t supersample = 0;
+    bool complete = true, passed = true, resource_limited = false, cancelled = false;
+};
+
+// Host copy of a borrowed mesh, validated (reference src/mesh.cpp:92-106).
+struct HostMesh {
+    std::vector<float> positions;  // xyz
+    std::vector<uint32_t> indices;
+    std::vector<uint16_t> materials;
+    std::vector<uint8_t> double_sided;
+    uint64_t digest[2] = {0, 0};  // over every stream, including normals/uv/colors/tangents
+};
+
+class Audit;
+
+struct DeviceMesh {
+    Audit* owner = nullptr;
+    float4* positions = nullptr;
+    uint32_t* indices = nullptr;
+    uint8_t* two_sided = nullptr;  // per face; null when no face is two-sided
+    uint32_t vertices = 0, faces = 0;
+    uint64_t digest[2] = {0, 0};
+    uint64_t serial = 0;
+    ~DeviceMesh();
+};
+
+Bounds bounds_of(const float* xyz, size_t count);
+std::vector<Camera> cameras(const Bounds& b, double screen, uint16_t ortho, uint16_t persp, uint32_t seed);
+void validate_settings(const Bounds& b, const Settings& s);
+uint32_t raster_side(double screen, unsigned ss);  // ceil(screen + 8) * ss
+
+struct Timing {
+    uint32_t raster_tasks = 0, distance_views = 0, passes = 0;
+    double raster_ms = 0, compare_ms = 0, distance_ms = 0, total_ms = 0;
+    uint64_t raster_triangles = 0, mask_bytes = 0;
+};
+
+class Audit {
+  public:
+    Audit(int device, uint64_t memory_bytes);
+    ~Audit();
+    Audit(const Audit&) = delete;
+    Audit& operator=(const Audit&) = delete;
+
+    std::unique_ptr<DeviceMesh> upload(const HostMesh& m);
+    Measurement evaluate(const DeviceMesh& reference, const DeviceMesh& candidate, const Bounds& b,
+                         const Settings& s);
+    std::vector<Measurement> evaluate_batch(const DeviceMesh& reference,
+                                            const std::vector<const DeviceMesh*>& candidates, const Bounds& b,
+                                            const Settings& s);
+    // Row-major packed mask of one view; returns false when over the sample cap.
+    bool raster_row_major(const DeviceMesh& m, const Bounds& b, const Camera& c, double screen, uint8_t ss,
+                          bool force_two, std::vector<uint64_t>& words, uint32_t& size, bool& clipped);
+    double coverage_distance(const std::vector<uint64_t>& a, const std::vector<uint64_t>& b, uint32_t size,
+                             uint8_t ss);
+    const Timing& last_timing() const { return timing_; }
+    int device() const { return device_; }
+    hipStream_t stream() const { return stream_; }
+    void release(DeviceMesh* m);
+
+  private:
+    struct Item {
+        uint32_t candidate;  // index into the candidate list
+        uint32_t view;
+        uint8_t ss;
+    };
+    struct ViewResult {
+        double coverage;
+        uint64_t changed, total;
+        bool clipped;
+    };
+    void run_items(const DeviceMesh& reference, const std::vector<const DeviceMesh*>& candidates,
+                   const std::vector<Camera>& cams, const Bounds& b, const Settings& s, const std::vector<Item>& items,
+                   std::vector<ViewResult>& out);
+    void* scratch(size_t slot, size_t bytes);
+    template <class T> T* scratch_as(size_t slot, size_t count) {
+        return static_cast<T*>(scratch(slot, count * sizeof(T)));
+    }
+
+    int device_;
+    uint64_t memory_bytes_;
+    hipStream_t stream_{};
+    hipEvent_t events_[5]{};
+    std::vector<std::pair<void*, size_t>> slots_;
+    uint64_t next_serial_ = 1;
+    Timing timing_;
+};
+
+// Restores the caller's current device on scope exit (reference
+// src/neural/cuda.cuh:32-45).
+struct DeviceGuard {
+    int previous = -1;
+    explicit DeviceGuard(int device);
+    ~DeviceGuard();
+};
+
+}  // namespace blitz_hip

@@ -1,0 +1,370 @@
+// Coverage-audit kernels for gfx950 (wave64).
+//
+// Bit-exactness contract: every FP64 expression below is evaluated in the same
+// order as the reference CPU audit (src/evaluate.cpp:136-170) and the library
+// is compiled with -ffp-contract=off, so no multiply-add is fused. FP64
+// division on gfx950 (div_scale/div_fmas/div_fixup) is correctly rounded.
+#include <hip/hip_runtime.h>
+
+#include "audit_kernels.hpp"
+
+namespace blitz_hip {
+namespace {
+
+struct Projected {
+    double x, y, z;
+};
+
+// src/evaluate.cpp:138-141: x = dot(p - c, right) * scale * ss + size/2.
+__device__ __forceinline__ Projected project(const float4 p, const RasterTask& t) {
+    const double x = double(p.x) - double(t.center.x);
+    const double y = double(p.y) - double(t.center.y);
+    const double z0 = double(p.z) - double(t.center.z);
+    const Camera& c = t.camera;
+    const double along_f = x * double(c.forward.x) + y * double(c.forward.y) + z0 * double(c.forward.z);
+    const double z = c.distance - along_f;
+    const double scale = c.perspective ? c.focal / z : c.scale;
+    const double half = double(t.size) * .5;
+    const double ssd = double(t.ss);
+    const double along_r = x * double(c.right.x) + y * double(c.right.y) + z0 * double(c.right.z);
+    const double along_u = x * double(c.up.x) + y * double(c.up.y) + z0 * double(c.up.z);
+    return {along_r * scale * ssd + half, along_u * scale * ssd + half, z};
+}
+
+// src/evaluate.cpp:21
+__device__ __forceinline__ double edge(double ax, double ay, double bx, double by, double x, double y) {
+    return (bx - ax) * (y - ay) - (by - ay) * (x - ax);
+}
+
+// std::min / std::max over an initializer list: first smallest (largest)
+// element under operator<, which matters only for NaN inputs.
+__device__ __forceinline__ double min3(double a, double b, double c) {
+    double m = a;
+    if (b < m) m = b;
+    if (c < m) m = c;
+    return m;
+}
+__device__ __forceinline__ double max3(double a, double b, double c) {
+    double m = a;
+    if (m < b) m = b;
+    if (m < c) m = c;
+    return m;
+}
+
+struct Setup {
+    double ax, ay, bx, by, cx, cy, r0, r1, r2;
+    int x0, x1, y0, y1;
+};
+
+// Triangle setup of src/evaluate.cpp:144-165. Returns 0 = nothing to draw,
+// 1 = draw, 2 = out of frame (the reference stops at the first such face).
+__device__ __forceinline__ int setup_triangle(const RasterTask& t, uint32_t f, Setup& s, bool& behind) {
+    const uint32_t ia = t.indices[3 * f], ib = t.indices[3 * f + 1], ic = t.indices[3 * f + 2];
+    const Projected a = project(t.positions[ia], t);
+    Projected d = project(t.positions[ib], t);
+    Projected e = project(t.positions[ic], t);
+    if (t.camera.perspective && (a.z <= 0 || d.z <= 0 || e.z <= 0)) {
+        behind = true;
+        return 0;
+    }
+    double area = edge(a.x, a.y, d.x, d.y, e.x, e.y);
+    if (fabs(area) < 1e-16) return 0;
+    const bool back = area < 0;
+    const bool two = t.force_two || (t.two_sided && t.two_sided[f]);
+    if (back && !two) return 0;
+    if (back) {
+        const Projected tmp = d;
+        d = e;
+        e = tmp;
+    }
+    const double minx = min3(a.x, d.x, e.x), maxx = max3(a.x, d.x, e.x);
+    const double miny = min3(a.y, d.y, e.y), maxy = max3(a.y, d.y, e.y);
+    const double size = double(t.size);
+    if (!isfinite(minx) || !isfinite(maxx) || !isfinite(miny) || !isfinite(maxy) || minx < 0 || miny < 0 ||
+        maxx > size || maxy > size)
+        return 2;
+    s.x0 = max(0, int(floor(minx)));
+    s.x1 = min(int(t.size) - 1, int(floor(maxx)));
+    s.y0 = max(0, int(floor(miny)));
+    s.y1 = min(int(t.size) - 1, int(floor(maxy)));
+    s.r0 = .5 * (fabs(e.x - d.x) + fabs(e.y - d.y));
+    s.r1 = .5 * (fabs(a.x - e.x) + fabs(a.y - e.y));
+    s.r2 = .5 * (fabs(d.x - a.x) + fabs(d.y - a.y));
+    s.ax = a.x;
+    s.ay = a.y;
+    s.bx = d.x;
+    s.by = d.y;
+    s.cx = e.x;
+    s.cy = e.y;
+    return 1;
+}
+
+// Conservative coverage predicate at sample (x, y) (src/evaluate.cpp:166-168).
+__device__ __forceinline__ bool covers(const Setup& s, int x, int y) {
+    const double px = x + .5, py = y + .5;
+    const double u = edge(s.bx, s.by, s.cx, s.cy, px, py);
+    const double v = edge(s.cx, s.cy, s.ax, s.ay, px, py);
+    const double w = edge(s.ax, s.ay, s.bx, s.by, px, py);
+    return !(u + s.r0 < 0 || v + s.r1 < 0 || w + s.r2 < 0);
+}
+
+__device__ __forceinline__ Setup shuffle_setup(const Setup& s, int lane) {
+    Setup o;
+    o.ax = __shfl(s.ax, lane);
+    o.ay = __shfl(s.ay, lane);
+    o.bx = __shfl(s.bx, lane);
+    o.by = __shfl(s.by, lane);
+    o.cx = __shfl(s.cx, lane);
+    o.cy = __shfl(s.cy, lane);
+    o.r0 = __shfl(s.r0, lane);
+    o.r1 = __shfl(s.r1, lane);
+    o.r2 = __shfl(s.r2, lane);
+    o.x0 = __shfl(s.x0, lane);
+    o.x1 = __shfl(s.x1, lane);
+    o.y0 = __shfl(s.y0, lane);
+    o.y1 = __shfl(s.y1, lane);
+    return o;
+}
+
+// Triangles whose bounding box spans at most 2x2 tiles and few samples are
+// rasterized by their own lane; larger ones are walked tile by tile by the
+// whole wavefront, one sample per lane, with a ballot producing the word.
+constexpr int kLaneSamples = 24;
+
+__global__ __launch_bounds__(256) void raster_kernel(const RasterTask* __restrict__ tasks) {
+    const RasterTask& t = tasks[blockIdx.y];
+    const uint32_t f = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    Setup s{};
+    bool large = false;
+    if (f < t.faces) {
+        bool behind = false;
+        const int kind = setup_triangle(t, f, s, behind);
+        if (behind) t.status[1] = 1u;
+        if (kind == 2) {
+            atomicMin(&t.status[0], f);
+        } else if (kind == 1) {
+            const int w = s.x1 - s.x0 + 1, h = s.y1 - s.y0 + 1;
+            if (w <= 8 && h <= 8 && w * h <= kLaneSamples) {
+                // At most 2x2 tiles: accumulate four words in registers.
+                uint64_t word[4] = {0, 0, 0, 0};
+                const int tx0 = s.x0 >> 3, ty0 = s.y0 >> 3;
+                for (int y = s.y0; y <= s.y1; ++y)
+                    for (int x = s.x0; x <= s.x1; ++x)
+                        if (covers(s, x, y))
+                            word[(((y >> 3) - ty0) << 1) | ((x >> 3) - tx0)] |= uint64_t(1) << (((y & 7) << 3) | (x & 7));
+                for (int k = 0; k < 4; ++k)
+                    if (word[k]) {
+                        const uint32_t tx = uint32_t(tx0 + (k & 1)), ty = uint32_t(ty0 + (k >> 1));
+                        atomicOr(reinterpret_cast<unsigned long long*>(&t.mask[size_t(ty) * t.tiles + tx]),
+                                 static_cast<unsigned long long>(word[k]));
+                    }
+            } else {
+                large = true;
+            }
+        }
+    }
+    uint64_t pending = __ballot(large);
+    while (pending) {
+        const int src = __ffsll(static_cast<unsigned long long>(pending)) - 1;
+        pending &= pending - 1;
+        const Setup q = shuffle_setup(s, src);
+        const int lx = lane & 7, ly = lane >> 3;
+        for (int ty = q.y0 >> 3; ty <= (q.y1 >> 3); ++ty)
+            for (int tx = q.x0 >> 3; tx <= (q.x1 >> 3); ++tx) {
+                const int x = (tx << 3) | lx, y = (ty << 3) | ly;
+                const bool in = x >= q.x0 && x <= q.x1 && y >= q.y0 && y <= q.y1 && covers(q, x, y);
+                const uint64_t word = __ballot(in);
+                if (lane == 0 && word)
+                    atomicOr(reinterpret_cast<unsigned long long*>(&t.mask[size_t(ty) * t.tiles + tx]),
+                             static_cast<unsigned long long>(word));
+            }
+    }
+}
+
+__global__ void fill_kernel(uint64_t* p, size_t n, uint64_t value) {
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x)
+        p[i] = value;
+}
+
+__device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+// Mask summaries of src/evaluate.cpp:198-202,276-279: popcounts of A, C,
+// A^C and A|C. Padding bits are never set, so tile-major counts equal the
+// reference's row-major counts.
+__global__ __launch_bounds__(256) void compare_kernel(const ViewPair* __restrict__ pairs, ViewSummary* summaries,
+                                                      uint32_t words_per_block) {
+    const ViewPair& p = pairs[blockIdx.y];
+    const size_t n = size_t(p.tiles) * p.tiles;
+    const size_t begin = size_t(blockIdx.x) * words_per_block;
+    if (begin >= n) return;
+    const size_t end = min(n, begin + words_per_block);
+    unsigned long long ca = 0, cc = 0, changed = 0, total = 0;
+    for (size_t i = begin + threadIdx.x; i < end; i += blockDim.x) {
+        const uint64_t a = p.a[i], c = p.c[i];
+        ca += __popcll(a);
+        cc += __popcll(c);
+        changed += __popcll(a ^ c);
+        total += __popcll(a | c);
+    }
+    ca = wave_sum(ca);
+    cc = wave_sum(cc);
+    changed = wave_sum(changed);
+    total = wave_sum(total);
+    if ((threadIdx.x & 63) == 0 && (ca | cc | changed | total)) {
+        ViewSummary& s = summaries[blockIdx.y];
+        atomicAdd(&s.count_a, ca);
+        atomicAdd(&s.count_c, cc);
+        atomicAdd(&s.changed, changed);
+        atomicAdd(&s.total, total);
+    }
+}
+
+// A view needs the distance transform only when the masks differ and both are
+// nonempty (src/evaluate.cpp:202).
+__device__ __forceinline__ bool needs_distance(const ViewSummary& s) {
+    return s.changed && s.count_a && s.count_c;
+}
+
+// Row pass of the separable EDT: for every sample, the horizontal distance to
+// the nearest site in its row (kNoSite if the row has none). One block per
+// row; each thread owns one byte (8 samples) of the row and block-wide
+// max/min scans give the nearest sites left and right of its byte.
+__global__ __launch_bounds__(1024) void row_distance_kernel(const ViewPair* __restrict__ pairs,
+                                                            const ViewSummary* __restrict__ summaries) {
+    const uint32_t job = blockIdx.y;  // view*2 + direction
+    const ViewPair& p = pairs[job >> 1];
+    if (!needs_distance(summaries[job >> 1])) return;
+    const uint64_t* mask = (job & 1) ? p.a : p.c;
+    uint16_t* out = (job & 1) ? p.dist_a : p.dist_c;
+    const uint32_t y = blockIdx.x;
+    if (y >= p.size) return;
+    const uint32_t tx = threadIdx.x;
+    const bool live = tx < p.tiles;
+    const uint32_t byte = live ? uint32_t((mask[size_t(y >> 3) * p.tiles + tx] >> ((y & 7) << 3)) & 0xFF) : 0u;
+    __shared__ int left[1024], right[1024];
+    const int base = int(tx << 3);
+    left[tx] = byte ? base + 31 - __clz(byte) : -1;            // last site in this byte
+    right[tx] = byte ? base + __ffs(byte) - 1 : 0x7FFFFFFF;   // first site in this byte
+    __syncthreads();
+    for (uint32_t o = 1; o < blockDim.x; o <<= 1) {
+        const int l = tx >= o ? left[tx - o] : -1;
+        const int r = tx + o < blockDim.x ? right[tx + o] : 0x7FFFFFFF;
+        __syncthreads();
+        left[tx] = max(left[tx], l);
+        right[tx] = min(right[tx], r);
+        __syncthreads();
+    }
+    if (!live) return;
+    const int before = tx ? left[tx - 1] : -1;
+    const int after = tx + 1 < blockDim.x ? right[tx + 1] : 0x7FFFFFFF;
+    uint16_t d[8];
+    for (int k = 0; k < 8; ++k) {
+        const int x = base + k;
+        // Nearest site at or left of x, at or right of x.
+        const uint32_t lower = byte & ((2u << k) - 1);
+        const uint32_t upper = byte & ~((1u << k) - 1);
+        const int l = lower ? base + 31 - __clz(lower) : before;
+        const int r = upper ? base + __ffs(upper) - 1 : after;
+        int best = 0x7FFFFFFF;
+        if (l >= 0) best = x - l;
+        if (r != 0x7FFFFFFF) best = min(best, r - x);
+        d[k] = best == 0x7FFFFFFF ? kNoSite : uint16_t(best);
+    }
+    uint16_t* row = out + size_t(y) * p.size + base;
+    const uint32_t valid = min(8u, p.size - uint32_t(base));
+    for (uint32_t k = 0; k < valid; ++k) row[k] = d[k];
+}
+
+// float(d*d) of the reference's row pass (src/evaluate.cpp:50), as an integer:
+// exact below 2^24 and round-to-nearest-even above.
+__device__ __forceinline__ uint32_t rounded_square(uint32_t d) {
+    return uint32_t(float(d * d));
+}
+
+// Column pass restricted to the samples that can set the directed Hausdorff
+// maximum: every covered sample of `from` that is not covered in `to`. Each
+// query's exact squared distance is min over rows y' of (y-y')^2 + g(y'),
+// g = rounded row distance (the reference's lower envelope yields the same
+// minimum; its float output is a monotone rounding of it, applied once on the
+// host). The scan stops once (y-y')^2 reaches the best value or the best value
+// cannot exceed the maximum already found.
+__global__ __launch_bounds__(256) void hausdorff_kernel(const ViewPair* __restrict__ pairs,
+                                                        ViewSummary* summaries) {
+    const uint32_t job = blockIdx.y;  // view*2 + direction; direction 0: A -> C
+    ViewSummary& s = summaries[job >> 1];
+    if (!needs_distance(s)) return;
+    const ViewPair& p = pairs[job >> 1];
+    const uint64_t* from = (job & 1) ? p.c : p.a;
+    const uint64_t* to = (job & 1) ? p.a : p.c;
+    const uint16_t* dist = (job & 1) ? p.dist_a : p.dist_c;
+    uint32_t* target = (job & 1) ? &s.max_ca : &s.max_ac;
+    const size_t words = size_t(p.tiles) * p.tiles;
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= words) return;
+    uint64_t bits = from[i] & ~to[i];
+    if (!bits) return;
+    const uint32_t tx = uint32_t(i % p.tiles), ty = uint32_t(i / p.tiles);
+    const uint32_t size = p.size;
+    uint32_t known = __hip_atomic_load(target, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    uint32_t local = 0;
+    while (bits) {
+        const int b = __ffsll(static_cast<unsigned long long>(bits)) - 1;
+        bits &= bits - 1;
+        const uint32_t x = (tx << 3) | uint32_t(b & 7), y = (ty << 3) | uint32_t(b >> 3);
+        const uint16_t d0 = dist[size_t(y) * size + x];
+        uint32_t best = d0 == kNoSite ? 0xFFFFFFFFu : rounded_square(d0);
+        const uint32_t floor_value = max(known, local);
+        for (uint32_t k = 1; best > floor_value && uint64_t(k) * k < best; ++k) {
+            const uint32_t kk = k * k;
+            const bool up = y >= k, down = y + k < size;
+            if (!up && !down) break;
+            if (up) {
+                const uint16_t d = dist[size_t(y - k) * size + x];
+                if (d != kNoSite) best = min(best, kk + rounded_square(d));
+            }
+            if (down) {
+                const uint16_t d = dist[size_t(y + k) * size + x];
+                if (d != kNoSite) best = min(best, kk + rounded_square(d));
+            }
+        }
+        if (best > floor_value && best != 0xFFFFFFFFu) local = best;
+    }
+    if (local > known) atomicMax(target, local);
+}
+
+}  // namespace
+
+void launch_fill(uint64_t* p, size_t n, uint64_t value, hipStream_t stream) {
+    if (!n) return;
+    const size_t blocks = std::min<size_t>((n + 255) / 256, 65535);
+    fill_kernel<<<dim3(uint32_t(blocks)), dim3(256), 0, stream>>>(p, n, value);
+}
+
+void launch_raster(const RasterTask* tasks, uint32_t count, uint32_t max_faces, hipStream_t stream) {
+    if (!count || !max_faces) return;
+    raster_kernel<<<dim3((max_faces + 255) / 256, count), dim3(256), 0, stream>>>(tasks);
+}
+
+void launch_compare(const ViewPair* pairs, ViewSummary* summaries, uint32_t count, uint32_t tiles,
+                    hipStream_t stream) {
+    if (!count) return;
+    const size_t words = size_t(tiles) * tiles;
+    const uint32_t per_block = 256 * 16;
+    const uint32_t blocks = uint32_t((words + per_block - 1) / per_block);
+    compare_kernel<<<dim3(blocks, count), dim3(256), 0, stream>>>(pairs, summaries, per_block);
+}
+
+void launch_distance(const ViewPair* pairs, ViewSummary* summaries, uint32_t count, uint32_t size, uint32_t tiles,
+                     hipStream_t stream) {
+    if (!count) return;
+    const uint32_t threads = std::max<uint32_t>(64, (tiles + 63) / 64 * 64);
+    row_distance_kernel<<<dim3(size, count * 2), dim3(threads), 0, stream>>>(pairs, summaries);
+    const size_t words = size_t(tiles) * tiles;
+    hausdorff_kernel<<<dim3(uint32_t((words + 255) / 256), count * 2), dim3(256), 0, stream>>>(pairs, summaries);
+}
+
+}  // namespace blitz_hip

@@ -1,0 +1,63 @@
+// Plain data shared by the host controller and the audit kernels.
+#pragma once
+#include <cstdint>
+
+namespace blitz_hip {
+
+struct F3 {
+    float x, y, z;
+};
+
+// Same layout as blitz::Camera (reference include/blitz/evaluate.hpp:46).
+struct Camera {
+    F3 right, up, forward;
+    double distance, focal, scale;
+    uint8_t perspective;
+};
+
+struct Bounds {
+    F3 center;
+    double radius;
+};
+
+// Coverage masks are stored tile-major: one 64-bit word per 8x8 sample tile,
+// word = ty*tiles + tx, bit = (y&7)*8 + (x&7). A 64-lane wavefront owns one
+// tile with lane == bit, so a ballot over the tile is the tile's word.
+constexpr uint32_t kTile = 8;
+
+// One raster job: one mesh seen from one camera at one supersampling.
+struct RasterTask {
+    const float4* positions;   // xyz + pad, one per vertex
+    const uint32_t* indices;   // 3 per face
+    const uint8_t* two_sided;  // per-face flag or null (all one-sided)
+    uint64_t* mask;            // tile-major words, tiles*tiles
+    uint32_t* status;          // [0] first out-of-frame face, [1] behind-camera flag
+    Camera camera;
+    F3 center;
+    uint32_t faces;            // faces rasterized: [0, faces)
+    uint32_t size;             // raster side in samples
+    uint32_t tiles;            // ceil(size / 8)
+    uint8_t ss;
+    uint8_t force_two;
+};
+
+// Per-view comparison of a reference mask A and a candidate mask C.
+struct ViewPair {
+    const uint64_t* a;
+    const uint64_t* c;
+    uint16_t* dist_a;  // row distances to the nearest A sample (queries from C)
+    uint16_t* dist_c;  // row distances to the nearest C sample (queries from A)
+    uint32_t size, tiles;
+};
+
+// Written by the compare and distance kernels, read back by the host.
+struct ViewSummary {
+    unsigned long long count_a, count_c, changed, total;
+    uint32_t max_ac;  // max over A\C of the squared distance to C (pre-rounding)
+    uint32_t max_ca;  // max over C\A of the squared distance to A
+    uint32_t pad[2];
+};
+
+constexpr uint16_t kNoSite = 0xFFFF;
+
+}  // namespace blitz_hip

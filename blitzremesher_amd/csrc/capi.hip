@@ -1,0 +1,419 @@
+// extern "C" layer of include/blitz_hip.h over the C++ audit controller.
+// Mirrors the reference C shim: descriptor checks (src/c_api.cpp:131-139),
+// exception -> status mapping (src/c_api.cpp:187-202) and truncated error
+// messages (src/c_api.cpp:26-32). No exception leaves this file.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <memory>
+#include <new>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/blitz_hip.h"
+#include "audit.hpp"
+#include "simplify.hpp"
+
+using namespace blitz_hip;
+
+struct blitz_hip_audit {
+    std::unique_ptr<Audit> impl;
+};
+struct blitz_hip_device_mesh {
+    blitz_hip_audit* owner;
+    std::unique_ptr<DeviceMesh> mesh;
+};
+
+namespace {
+void message(char* out, size_t n, const char* text) {
+    if (out && n) {
+        const size_t k = std::min(n - 1, std::strlen(text));
+        std::memcpy(out, text, k);
+        out[k] = 0;
+    }
+}
+
+template <class F> blitz_status guarded(char* error, size_t capacity, F&& body) {
+    message(error, capacity, "");
+    try {
+        body();
+        return BLITZ_OK;
+    } catch (const Unavailable& e) {
+        message(error, capacity, e.what());
+        return BLITZ_UNAVAILABLE;
+    } catch (const std::invalid_argument& e) {
+        message(error, capacity, e.what());
+        return BLITZ_INVALID_ARGUMENT;
+    } catch (const std::bad_alloc&) {
+        message(error, capacity, "allocation failed");
+        return BLITZ_OUT_OF_MEMORY;
+    } catch (const std::exception& e) {
+        message(error, capacity, e.what());
+        return BLITZ_INTERNAL_ERROR;
+    } catch (...) {
+        message(error, capacity, "unknown exception");
+        return BLITZ_INTERNAL_ERROR;
+    }
+}
+
+// Two independent 64-bit digests (FNV-1a and a multiplicative mix) over the
+// streams blitz's identical() compares (src/evaluate.cpp:77-82).
+struct Digest {
+    uint64_t a = 1469598103934665603ull, b = 0x9E3779B97F4A7C15ull;
+    void bytes(const void* p, size_t n) {
+        const auto* c = static_cast<const unsigned char*>(p);
+        for (size_t i = 0; i < n; ++i) {
+            a = (a ^ c[i]) * 1099511628211ull;
+            b = (b ^ c[i]) * 0xff51afd7ed558ccdull;
+            b ^= b >> 29;
+        }
+    }
+    void value(uint64_t v) { bytes(&v, sizeof v); }
+};
+
+bool valid_stream(const blitz_stream& s, size_t n, size_t element, bool required) {
+    if (!s.count) return !required;
+    if (!s.data) return false;
+    return s.count == n && s.stride >= element && (!n || (n - 1) <= (SIZE_MAX - element) / s.stride);
+}
+
+void digest_stream(Digest& d, const blitz_stream& s, size_t element) {
+    d.value(s.count);
+    const auto* base = static_cast<const unsigned char*>(s.data);
+    for (size_t i = 0; i < s.count; ++i) d.bytes(base + i * s.stride, element);
+}
+
+HostMesh host_mesh(const blitz_mesh* m) {
+    if (!m || m->struct_size != sizeof(blitz_mesh) || m->abi_version != BLITZ_ABI_VERSION)
+        throw std::invalid_argument("null pointer, descriptor size or ABI version mismatch");
+    if ((!m->indices && m->index_count) || (!m->materials && m->material_count) ||
+        (!m->double_sided && m->double_sided_count))
+        throw std::invalid_argument("null index or material pointer");
+    if ((reinterpret_cast<uintptr_t>(m->indices) % alignof(uint32_t)) ||
+        (reinterpret_cast<uintptr_t>(m->materials) % alignof(uint16_t)))
+        throw std::invalid_argument("unaligned index or material stream");
+    const size_t n = m->positions.count;
+    if (n >= UINT32_MAX || m->index_count % 3 || m->index_count >= UINT32_MAX)
+        throw std::invalid_argument("mesh requires 32-bit representable counts and whole triangles");
+    if (!valid_stream(m->positions, n, 12, false) || !valid_stream(m->normals, n, 12, false) ||
+        !valid_stream(m->uv, n, 8, false) || !valid_stream(m->colors, n, 4, false) ||
+        !valid_stream(m->tangents, n, 16, false))
+        throw std::invalid_argument("invalid stream count, pointer or stride");
+    if (m->material_count && m->material_count != m->index_count / 3)
+        throw std::invalid_argument("material count differs from triangle count");
+    for (size_t i = 0; i < m->index_count; ++i)
+        if (m->indices[i] >= n) throw std::invalid_argument("index outside vertex stream");
+    HostMesh h;
+    h.positions.resize(n * 3);
+    const auto* base = static_cast<const unsigned char*>(m->positions.data);
+    for (size_t i = 0; i < n; ++i) std::memcpy(&h.positions[3 * i], base + i * m->positions.stride, 12);
+    h.indices.assign(m->indices, m->indices + m->index_count);
+    if (m->material_count) h.materials.assign(m->materials, m->materials + m->material_count);
+    if (m->double_sided_count) h.double_sided.assign(m->double_sided, m->double_sided + m->double_sided_count);
+    Digest d;
+    digest_stream(d, m->positions, 12);
+    digest_stream(d, m->normals, 12);
+    digest_stream(d, m->colors, 4);
+    d.value(m->index_count);
+    d.bytes(m->indices, m->index_count * 4);
+    d.value(m->material_count);
+    d.bytes(m->materials, m->material_count * 2);
+    d.value(m->double_sided_count);
+    d.bytes(m->double_sided, m->double_sided_count);
+    h.digest[0] = d.a;
+    h.digest[1] = d.b;
+    return h;
+}
+
+Settings settings_of(const blitz_hip_eval_settings* s) {
+    if (!s || s->struct_size != sizeof(blitz_hip_eval_settings) || s->version != BLITZ_HIP_API_VERSION)
+        throw std::invalid_argument("null pointer, settings size or version mismatch");
+    Settings o;
+    o.profile = s->profile;
+    o.supersample = s->supersample;
+    o.max_supersample = s->max_supersample;
+    o.force_two_sided = s->force_two_sided != 0;
+    o.conservative_screen = s->conservative_screen != 0;
+    o.orthographic = s->views_orthographic;
+    o.perspective = s->views_perspective;
+    o.rotation_seed = s->views_rotation_seed;
+    o.screen_size = s->screen_size;
+    o.limit = s->limit;
+    o.max_changed_area = s->max_changed_area;
+    o.normal_weight = s->normal_weight;
+    o.color_weight = s->color_weight;
+    o.material_weight = s->material_weight;
+    if (s->cancelled) {
+        auto fn = s->cancelled;
+        void* user = s->user_data;
+        o.cancelled = [fn, user] { return fn(user) != 0; };
+    }
+    return o;
+}
+
+Bounds bounds_from(const blitz_hip_bounds* b) {
+    if (!b) throw std::invalid_argument("null bounds");
+    return {{b->center[0], b->center[1], b->center[2]}, b->radius};
+}
+
+void to_c(const Measurement& m, blitz_hip_measurement* out) {
+    *out = {};
+    out->error = m.error;
+    out->coverage = m.coverage;
+    out->coverage_upper = m.coverage_upper;
+    out->changed_area = m.changed_area;
+    out->normal_degrees = m.normal_degrees;
+    out->worst_view = m.worst_view;
+    out->changed_area_worst_view = m.changed_area_worst_view;
+    out->views_evaluated = m.views_evaluated;
+    out->supersample = m.supersample;
+    out->complete = m.complete;
+    out->passed = m.passed;
+    out->resource_limited = m.resource_limited;
+    out->cancelled = m.cancelled;
+}
+
+Audit& audit_of(blitz_hip_audit* a) {
+    if (!a || !a->impl) throw std::invalid_argument("null audit");
+    return *a->impl;
+}
+
+const DeviceMesh& mesh_of(const blitz_hip_audit* a, const blitz_hip_device_mesh* m) {
+    if (!m || !m->mesh) throw std::invalid_argument("null device mesh");
+    if (m->owner != a) throw std::invalid_argument("device mesh belongs to another audit");
+    return *m->mesh;
+}
+}  // namespace
+
+extern "C" {
+uint32_t blitz_hip_api_version(void) { return BLITZ_HIP_API_VERSION; }
+
+int blitz_hip_available(int32_t device) {
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count) return 0;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) return 0;
+    return std::strncmp(prop.gcnArchName, "gfx950", 6) == 0;
+}
+
+blitz_status blitz_hip_audit_options_init(blitz_hip_audit_options* o, size_t n) {
+    if (!o || n != sizeof(*o)) return BLITZ_INVALID_ARGUMENT;
+    *o = {};
+    o->struct_size = sizeof(*o);
+    o->version = BLITZ_HIP_API_VERSION;
+    o->device = 0;
+    o->memory_mib = 0;
+    o->cache_rasters = 1;
+    return BLITZ_OK;
+}
+
+blitz_status blitz_hip_eval_settings_init(blitz_hip_eval_settings* s, size_t n) {
+    if (!s || n != sizeof(*s)) return BLITZ_INVALID_ARGUMENT;
+    const Settings d;
+    *s = {};
+    s->struct_size = sizeof(*s);
+    s->version = BLITZ_HIP_API_VERSION;
+    s->profile = d.profile;
+    s->supersample = d.supersample;
+    s->max_supersample = d.max_supersample;
+    s->views_orthographic = d.orthographic;
+    s->views_perspective = d.perspective;
+    s->views_rotation_seed = d.rotation_seed;
+    s->screen_size = d.screen_size;
+    s->limit = d.limit;
+    s->max_changed_area = d.max_changed_area;
+    s->normal_weight = d.normal_weight;
+    s->color_weight = d.color_weight;
+    s->material_weight = d.material_weight;
+    return BLITZ_OK;
+}
+
+blitz_status blitz_hip_bounds_of(const blitz_mesh* m, blitz_hip_bounds* out, char* error, size_t capacity) {
+    return guarded(error, capacity, [&] {
+        if (!out) throw std::invalid_argument("null bounds");
+        const HostMesh h = host_mesh(m);
+        const Bounds b = bounds_of(h.positions.data(), h.positions.size() / 3);
+        *out = {};
+        out->center[0] = b.center.x;
+        out->center[1] = b.center.y;
+        out->center[2] = b.center.z;
+        out->radius = b.radius;
+    });
+}
+
+blitz_status blitz_hip_cameras(const blitz_hip_bounds* b, double screen, uint16_t ortho, uint16_t persp, uint32_t seed,
+                               blitz_hip_camera* out, size_t capacity) {
+    return guarded(nullptr, 0, [&] {
+        const auto cams = cameras(bounds_from(b), screen, ortho, persp, seed);
+        if (!out || cams.size() > capacity) throw std::invalid_argument("camera capacity");
+        for (size_t i = 0; i < cams.size(); ++i) {
+            blitz_hip_camera& c = out[i];
+            c = {};
+            std::memcpy(c.right, &cams[i].right, 12);
+            std::memcpy(c.up, &cams[i].up, 12);
+            std::memcpy(c.forward, &cams[i].forward, 12);
+            c.distance = cams[i].distance;
+            c.focal = cams[i].focal;
+            c.scale = cams[i].scale;
+            c.perspective = cams[i].perspective;
+        }
+    });
+}
+
+blitz_status blitz_hip_audit_create(const blitz_hip_audit_options* o, blitz_hip_audit** out, char* error,
+                                    size_t capacity) {
+    if (out) *out = nullptr;
+    return guarded(error, capacity, [&] {
+        if (!o || !out || o->struct_size != sizeof(*o) || o->version != BLITZ_HIP_API_VERSION)
+            throw std::invalid_argument("null pointer, options size or version mismatch");
+        if (!blitz_hip_available(o->device)) throw Unavailable("no gfx950 HIP device at the requested index");
+        auto a = std::make_unique<blitz_hip_audit>();
+        a->impl = std::make_unique<Audit>(o->device, uint64_t(o->memory_mib) << 20);
+        *out = a.release();
+    });
+}
+
+void blitz_hip_audit_destroy(blitz_hip_audit* a) { delete a; }
+
+blitz_status blitz_hip_mesh_upload(blitz_hip_audit* a, const blitz_mesh* m, blitz_hip_device_mesh** out, char* error,
+                                   size_t capacity) {
+    if (out) *out = nullptr;
+    return guarded(error, capacity, [&] {
+        if (!out) throw std::invalid_argument("null mesh destination");
+        Audit& audit = audit_of(a);
+        const HostMesh h = host_mesh(m);
+        auto d = std::make_unique<blitz_hip_device_mesh>();
+        d->owner = a;
+        d->mesh = audit.upload(h);
+        *out = d.release();
+    });
+}
+
+void blitz_hip_mesh_release(blitz_hip_device_mesh* m) {
+    if (!m) return;
+    if (m->owner && m->owner->impl) {
+        DeviceGuard guard(m->owner->impl->device());
+        (void)hipStreamSynchronize(m->owner->impl->stream());
+        m->mesh.reset();
+    }
+    delete m;
+}
+
+blitz_status blitz_hip_audit_evaluate(blitz_hip_audit* a, const blitz_mesh* reference, const blitz_mesh* candidate,
+                                      const blitz_hip_bounds* b, const blitz_hip_eval_settings* s,
+                                      blitz_hip_measurement* out, char* error, size_t capacity) {
+    return guarded(error, capacity, [&] {
+        if (!out) throw std::invalid_argument("null measurement");
+        Audit& audit = audit_of(a);
+        const Settings settings = settings_of(s);
+        const Bounds bounds = bounds_from(b);
+        const HostMesh hr = host_mesh(reference), hc = host_mesh(candidate);
+        validate_settings(bounds, settings);
+        auto dr = audit.upload(hr);
+        auto dc = audit.upload(hc);
+        to_c(audit.evaluate(*dr, *dc, bounds, settings), out);
+    });
+}
+
+blitz_status blitz_hip_audit_evaluate_device(blitz_hip_audit* a, const blitz_hip_device_mesh* reference,
+                                             const blitz_hip_device_mesh* candidate, const blitz_hip_bounds* b,
+                                             const blitz_hip_eval_settings* s, blitz_hip_measurement* out, char* error,
+                                             size_t capacity) {
+    return guarded(error, capacity, [&] {
+        if (!out) throw std::invalid_argument("null measurement");
+        Audit& audit = audit_of(a);
+        to_c(audit.evaluate(mesh_of(a, reference), mesh_of(a, candidate), bounds_from(b), settings_of(s)), out);
+    });
+}
+
+blitz_status blitz_hip_audit_evaluate_batch(blitz_hip_audit* a, const blitz_hip_device_mesh* reference,
+                                            const blitz_hip_device_mesh* const* candidates, size_t count,
+                                            const blitz_hip_bounds* b, const blitz_hip_eval_settings* s,
+                                            blitz_hip_measurement* out, char* error, size_t capacity) {
+    return guarded(error, capacity, [&] {
+        if (!out || (!candidates && count)) throw std::invalid_argument("null candidate list or output");
+        Audit& audit = audit_of(a);
+        std::vector<const DeviceMesh*> list(count);
+        for (size_t i = 0; i < count; ++i) list[i] = &mesh_of(a, candidates[i]);
+        if (!count) return;
+        const auto results = audit.evaluate_batch(mesh_of(a, reference), list, bounds_from(b), settings_of(s));
+        for (size_t i = 0; i < count; ++i) to_c(results[i], out + i);
+    });
+}
+
+blitz_status blitz_hip_raster_coverage(blitz_hip_audit* a, const blitz_hip_device_mesh* m, const blitz_hip_bounds* b,
+                                       const blitz_hip_camera* camera, double screen, uint8_t ss, uint8_t force_two,
+                                       uint64_t* words, size_t word_capacity, uint32_t* size_out, uint8_t* clipped,
+                                       char* error, size_t capacity) {
+    bool limited = false;
+    const blitz_status status = guarded(error, capacity, [&] {
+        if (!camera || !words || !size_out || !clipped) throw std::invalid_argument("null raster output");
+        Audit& audit = audit_of(a);
+        Camera c{};
+        std::memcpy(&c.right, camera->right, 12);
+        std::memcpy(&c.up, camera->up, 12);
+        std::memcpy(&c.forward, camera->forward, 12);
+        c.distance = camera->distance;
+        c.focal = camera->focal;
+        c.scale = camera->scale;
+        c.perspective = camera->perspective;
+        std::vector<uint64_t> out;
+        uint32_t size = 0;
+        bool clip = false;
+        if (!audit.raster_row_major(mesh_of(a, m), bounds_from(b), c, screen, ss, force_two != 0, out, size, clip)) {
+            *size_out = size;
+            limited = true;
+            return;
+        }
+        if (out.size() > word_capacity) throw std::invalid_argument("word capacity too small");
+        std::memcpy(words, out.data(), out.size() * 8);
+        *size_out = size;
+        *clipped = clip;
+    });
+    if (status == BLITZ_OK && limited) {
+        message(error, capacity, "raster exceeds per-view memory budget");
+        return BLITZ_BUDGET_LIMITED;
+    }
+    return status;
+}
+
+blitz_status blitz_hip_coverage_distance(blitz_hip_audit* a, const uint64_t* x, const uint64_t* y, uint32_t size,
+                                         uint8_t ss, double* out, char* error, size_t capacity) {
+    return guarded(error, capacity, [&] {
+        if (!x || !y || !out) throw std::invalid_argument("null mask");
+        if (!size || uint64_t(size) * size > 64000000ull) throw std::invalid_argument("invalid raster size");
+        const size_t n = (size_t(size) * size + 63) / 64;
+        std::vector<uint64_t> va(x, x + n), vb(y, y + n);
+        *out = audit_of(a).coverage_distance(va, vb, size, ss);
+    });
+}
+
+blitz_status blitz_hip_audit_last_timing(const blitz_hip_audit* a, blitz_hip_audit_timing* out) {
+    if (!a || !a->impl || !out || out->struct_size != sizeof(*out)) return BLITZ_INVALID_ARGUMENT;
+    const Timing& t = a->impl->last_timing();
+    out->raster_tasks = t.raster_tasks;
+    out->distance_views = t.distance_views;
+    out->passes = t.passes;
+    out->raster_ms = t.raster_ms;
+    out->compare_ms = t.compare_ms;
+    out->distance_ms = t.distance_ms;
+    out->total_ms = t.total_ms;
+    out->raster_triangles = t.raster_triangles;
+    out->mask_bytes = t.mask_bytes;
+    return BLITZ_OK;
+}
+
+blitz_status blitz_hip_reduce_reuse(const blitz_mesh* m, size_t target, uint32_t* out, size_t out_capacity,
+                                    size_t* count, char* error, size_t capacity) {
+    return guarded(error, capacity, [&] {
+        if (!out || !count) throw std::invalid_argument("null reduction output");
+        const HostMesh h = host_mesh(m);
+        const auto idx = reduce_reuse(h.positions.data(), h.positions.size() / 3, h.indices.data(),
+                                      h.indices.size() / 3, target, nullptr);
+        if (idx.size() > out_capacity) throw std::invalid_argument("index capacity too small");
+        std::memcpy(out, idx.data(), idx.size() * 4);
+        *count = idx.size();
+    });
+}
+}

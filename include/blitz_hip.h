@@ -1,0 +1,203 @@
+/* MI355X (gfx950) HIP backend for BlitzRemesher's audit + scorer hot path.
+ *
+ * This is the thin C ABI between the reference's C++ host (chain search,
+ * neural generation hooks) and hand-written HIP kernels. It plugs in as one
+ * new raster backend value of the versioned neural options
+ * (reference include/blitz/blitz.h:135, raster_backend) and consumes the
+ * reference's ABI-5 mesh descriptor unchanged (include/blitz/blitz.h:18-37).
+ * No C++ exception crosses this boundary; errors map to blitz_status exactly
+ * as the reference's C shim does (src/c_api.cpp:187-202) and the message is
+ * copied, truncated, into the caller's buffer (src/c_api.cpp:26-32).
+ *
+ * Threading: one audit object per calling thread; each object owns its HIP
+ * stream and workspace. Every entry point restores the caller's current HIP
+ * device (reference src/neural/cuda.cuh:32-45).
+ */
+#ifndef BLITZ_HIP_H
+#define BLITZ_HIP_H
+#include <stddef.h>
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* The reference's ABI-5 descriptors. When blitz/blitz.h is included first its
+ * definitions are used; the layouts below are identical to it. */
+#ifndef BLITZ_REMESHER_H
+#define BLITZ_ABI_VERSION 5u
+typedef enum blitz_status {
+    BLITZ_OK = 0,
+    BLITZ_INVALID_ARGUMENT = 1,
+    BLITZ_OUT_OF_MEMORY = 2,
+    BLITZ_INTERNAL_ERROR = 3,
+    BLITZ_CANCELLED = 4,
+    BLITZ_BUDGET_LIMITED = 5,
+    BLITZ_UNAVAILABLE = 6
+} blitz_status;
+typedef struct blitz_stream {
+    const void* data;
+    size_t count, stride;
+} blitz_stream;
+typedef struct blitz_mesh {
+    uint32_t struct_size, abi_version;
+    blitz_stream positions, normals, uv, colors, tangents;
+    const uint32_t* indices;
+    size_t index_count;
+    const uint16_t* materials;
+    size_t material_count;
+    const uint8_t* double_sided;
+    size_t double_sided_count;
+} blitz_mesh;
+#endif
+
+/* New enumerator for blitz_neural_options_v2.raster_backend (CUDA = 0,
+ * Vulkan = 1 in the reference; include/blitz/neural.hpp:11). */
+#define BLITZ_NEURAL_RASTER_HIP 2u
+#define BLITZ_HIP_API_VERSION 1u
+
+/* ---- shared value types --------------------------------------------------- */
+
+/* blitz::Bounds (include/blitz/mesh.hpp:117). */
+typedef struct blitz_hip_bounds {
+    float center[3];
+    double radius;
+} blitz_hip_bounds;
+
+/* blitz::Camera (include/blitz/evaluate.hpp:46); cameras() output. */
+typedef struct blitz_hip_camera {
+    float right[3], up[3], forward[3];
+    double distance, focal, scale;
+    uint8_t perspective;
+} blitz_hip_camera;
+
+/* blitz::EvalSettings (include/blitz/evaluate.hpp:24-33) as a versioned POD.
+ * profile: 0 coverage (the HIP backend implements coverage; 1 normals and
+ * 2 attributes return BLITZ_UNAVAILABLE). cancelled is polled once per view
+ * in view order, as the reference does (src/evaluate.cpp:319). */
+typedef struct blitz_hip_eval_settings {
+    uint32_t struct_size, version;
+    uint8_t profile, supersample, max_supersample, force_two_sided;
+    uint8_t conservative_screen, reserved0[3];
+    uint16_t views_orthographic, views_perspective;
+    uint32_t views_rotation_seed;
+    double screen_size, limit, max_changed_area;
+    double normal_weight, color_weight, material_weight;
+    int (*cancelled)(void*);
+    void* user_data;
+} blitz_hip_eval_settings;
+
+/* blitz::Measurement (include/blitz/evaluate.hpp:34-39). */
+typedef struct blitz_hip_measurement {
+    double error, coverage, coverage_upper, changed_area, normal_degrees;
+    uint32_t worst_view, changed_area_worst_view, views_evaluated;
+    uint8_t supersample, complete, passed, resource_limited, cancelled;
+    uint8_t reserved[3];
+} blitz_hip_measurement;
+
+/* Options of one audit object; the HIP counterpart of the subset of
+ * blitz::NeuralOptions (include/blitz/neural.hpp:55-86) the audit reads. */
+typedef struct blitz_hip_audit_options {
+    uint32_t struct_size, version;
+    int32_t device;
+    uint32_t memory_mib;    /* workspace cap; 0 = no cap */
+    uint8_t cache_rasters;  /* keep bound-reference masks resident */
+    uint8_t reserved[7];
+} blitz_hip_audit_options;
+
+typedef struct blitz_hip_audit blitz_hip_audit;             /* AuditCuda (src/neural/internal.hpp:131) */
+typedef struct blitz_hip_device_mesh blitz_hip_device_mesh; /* DeviceMeshView (src/neural/device_view.hpp:16) */
+
+/* Per-call kernel timing of the last evaluate/certify call, for profiling. */
+typedef struct blitz_hip_audit_timing {
+    uint32_t struct_size;
+    uint32_t raster_tasks, distance_views, passes;
+    double raster_ms, compare_ms, distance_ms, total_ms;
+    uint64_t raster_triangles; /* triangle-view setups issued */
+    uint64_t mask_bytes;       /* coverage mask bytes written */
+} blitz_hip_audit_timing;
+
+/* ---- audit ---------------------------------------------------------------- */
+
+uint32_t blitz_hip_api_version(void);
+/* Nonzero when a HIP device with gfx950 kernels is usable. */
+int blitz_hip_available(int32_t device);
+
+blitz_status blitz_hip_audit_options_init(blitz_hip_audit_options*, size_t);
+/* Defaults of blitz::EvalSettings (include/blitz/evaluate.hpp:24-33). */
+blitz_status blitz_hip_eval_settings_init(blitz_hip_eval_settings*, size_t);
+
+/* blitz::bounds (src/mesh.cpp:77-86). */
+blitz_status blitz_hip_bounds_of(const blitz_mesh*, blitz_hip_bounds*, char* error, size_t capacity);
+/* blitz::cameras (src/evaluate.cpp:108-124), host libm. */
+blitz_status blitz_hip_cameras(const blitz_hip_bounds*, double screen, uint16_t orthographic,
+                               uint16_t perspective, uint32_t rotation_seed, blitz_hip_camera* out,
+                               size_t capacity);
+
+/* AuditCuda(const NeuralOptions&) (src/neural/internal.hpp:139). */
+blitz_status blitz_hip_audit_create(const blitz_hip_audit_options*, blitz_hip_audit**, char* error,
+                                    size_t capacity);
+void blitz_hip_audit_destroy(blitz_hip_audit*);
+
+/* Uploads a validated mesh (blitz::validate, src/mesh.cpp:92-106) into HBM.
+ * The handle stays valid until released or its audit is destroyed. */
+blitz_status blitz_hip_mesh_upload(blitz_hip_audit*, const blitz_mesh*, blitz_hip_device_mesh**,
+                                   char* error, size_t capacity);
+void blitz_hip_mesh_release(blitz_hip_device_mesh*);
+
+/* blitz::evaluate_gpu / AuditCuda::evaluate(MeshView, MeshView, ...)
+ * (include/blitz/neural.hpp:148-149, src/neural/evaluate.cu:2261-2341):
+ * coverage audit with the reference's refinement and first-failing-view
+ * semantics (src/evaluate.cpp:312-352). Host meshes are uploaded per call. */
+blitz_status blitz_hip_audit_evaluate(blitz_hip_audit*, const blitz_mesh* reference,
+                                      const blitz_mesh* candidate, const blitz_hip_bounds*,
+                                      const blitz_hip_eval_settings*, blitz_hip_measurement* out,
+                                      char* error, size_t capacity);
+/* Same, on meshes already resident in HBM (AuditCuda::evaluate(MeshView,
+ * DeviceMeshView, ...), src/neural/internal.hpp:163-166). */
+blitz_status blitz_hip_audit_evaluate_device(blitz_hip_audit*, const blitz_hip_device_mesh* reference,
+                                             const blitz_hip_device_mesh* candidate,
+                                             const blitz_hip_bounds*, const blitz_hip_eval_settings*,
+                                             blitz_hip_measurement* out, char* error,
+                                             size_t capacity);
+/* Batched audit of n candidates against one reference, all views of all
+ * candidates in shared launches (AuditCuda::certify_candidates,
+ * src/neural/evaluate.cu:2413-2578, without its 8-candidate limit). out[i]
+ * equals blitz_hip_audit_evaluate_device(reference, candidates[i]). */
+blitz_status blitz_hip_audit_evaluate_batch(blitz_hip_audit*, const blitz_hip_device_mesh* reference,
+                                            const blitz_hip_device_mesh* const* candidates,
+                                            size_t count, const blitz_hip_bounds*,
+                                            const blitz_hip_eval_settings*,
+                                            blitz_hip_measurement* out, char* error,
+                                            size_t capacity);
+
+/* blitz::detail::rasterize_coverage (src/coverage.hpp:12, src/evaluate.cpp:125-189)
+ * for one camera: row-major packed words (index = y*size + x), padding bits
+ * zero. *size_out receives the raster side; returns BLITZ_BUDGET_LIMITED when
+ * the raster exceeds the 64M-sample cap (src/evaluate.cpp:129). */
+blitz_status blitz_hip_raster_coverage(blitz_hip_audit*, const blitz_hip_device_mesh*,
+                                       const blitz_hip_bounds*, const blitz_hip_camera*,
+                                       double screen, uint8_t supersample, uint8_t force_two_sided,
+                                       uint64_t* words, size_t word_capacity, uint32_t* size_out,
+                                       uint8_t* clipped, char* error, size_t capacity);
+/* blitz::detail::coverage_distance (src/coverage.hpp:13, src/evaluate.cpp:196-216)
+ * on two row-major packed masks of side `size`. */
+blitz_status blitz_hip_coverage_distance(blitz_hip_audit*, const uint64_t* a, const uint64_t* b,
+                                         uint32_t size, uint8_t supersample, double* out,
+                                         char* error, size_t capacity);
+
+blitz_status blitz_hip_audit_last_timing(const blitz_hip_audit*, blitz_hip_audit_timing*);
+
+/* ---- classical proposer --------------------------------------------------- */
+
+/* Quadric edge collapse with reused vertices: the classical proposal
+ * (blitz::reduce with OutputMode::Reuse, src/simplify.cpp:337-367; greedy
+ * heap order). Writes at most `target_triangles` triangles (fewer collapses
+ * if no legal one remains) as indices into the input vertex array. Host CPU. */
+blitz_status blitz_hip_reduce_reuse(const blitz_mesh*, size_t target_triangles, uint32_t* indices_out,
+                                    size_t index_capacity, size_t* index_count, char* error,
+                                    size_t capacity);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

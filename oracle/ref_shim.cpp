@@ -1,0 +1,190 @@
+// C entry points over a build of the reference CPU sources.
+//
+// TEST INFRASTRUCTURE, NOT PRODUCT CODE. oracle/Makefile compiles this file
+// together with the reference's own CPU sources (read in place under the
+// reference checkout, never copied) into oracle/_ref/libblitz_ref.so. It is used
+// only to write the golden fixtures in tests/golden/, to cross-check the oracle
+// restatement, and as the "reference" CPU baseline in bench.py.
+//
+// Signatures mirror oracle/audit_oracle.cpp so tests can swap one for the other.
+#include "blitz/evaluate.hpp"
+#include "blitz/remesher.hpp"
+#include "coverage.hpp"
+#include <cstring>
+#include <stdexcept>
+
+using namespace blitz;
+
+namespace {
+struct Measure {
+    double error, coverage, coverage_upper, changed_area, normal_degrees;
+    uint32_t worst_view, changed_area_worst_view, views_evaluated;
+    uint8_t supersample, complete, passed, resource_limited, cancelled;
+};
+struct OracleSettings {
+    double screen_size, limit, max_changed_area;
+    uint32_t rotation_seed;
+    uint16_t orthographic, perspective;
+    uint8_t supersample, max_supersample, force_two_sided, conservative_screen;
+    int32_t profile;
+    double normal_weight, color_weight, material_weight;
+};
+MeshView view_of(const float* pos, uint64_t nv, const uint32_t* idx, uint64_t ni, const uint16_t* mat, uint64_t nm,
+                 const uint8_t* ds, uint64_t nds) {
+    MeshView v;
+    v.positions = Stream<Vec3>(reinterpret_cast<const Vec3*>(pos), nv);
+    v.indices = {idx, size_t(ni)};
+    v.materials = {mat, size_t(nm)};
+    v.double_sided = {ds, size_t(nds)};
+    return v;
+}
+Measure out_of(const Measurement& m) {
+    return {m.error, m.coverage, m.coverage_upper, m.changed_area, m.normal_degrees, m.worst_view,
+            m.changed_area_worst_view, m.views_evaluated, m.supersample, m.complete, m.passed,
+            m.resource_limited, m.cancelled};
+}
+}  // namespace
+
+extern "C" {
+int ref_bounds(const float* pos, uint64_t nv, float* center, double* radius) {
+    MeshView v;
+    v.positions = Stream<Vec3>(reinterpret_cast<const Vec3*>(pos), nv);
+    auto b = bounds(v);
+    center[0] = b.center.x;
+    center[1] = b.center.y;
+    center[2] = b.center.z;
+    *radius = b.radius;
+    return 0;
+}
+
+int ref_cameras(const float* center, double radius, double screen, uint16_t ortho, uint16_t persp, uint32_t seed,
+                Camera* out, uint64_t capacity) {
+    Bounds b{{center[0], center[1], center[2]}, radius};
+    auto cams = cameras(b, screen, {ortho, persp, seed});
+    if (cams.size() > capacity) return 1;
+    for (size_t i = 0; i < cams.size(); ++i) out[i] = cams[i];
+    return 0;
+}
+
+int ref_raster(const float* pos, uint64_t nv, const uint32_t* idx, uint64_t ni, const uint16_t* mat, uint64_t nm,
+               const uint8_t* ds, uint64_t nds, const float* center, double radius, const Camera* cam, double screen,
+               uint8_t ss, uint8_t force_two, uint64_t* words, uint64_t capacity, uint8_t* clipped,
+               uint32_t* size_out) {
+    try {
+        auto r = detail::rasterize_coverage(view_of(pos, nv, idx, ni, mat, nm, ds, nds),
+                                            {{center[0], center[1], center[2]}, radius}, *cam, screen, ss, force_two);
+        *size_out = r.width;
+        *clipped = r.clipped;
+        if (r.words.size() > capacity) return 1;
+        std::memcpy(words, r.words.data(), r.words.size() * 8);
+        return 0;
+    } catch (const std::length_error&) {
+        return 2;
+    } catch (...) {
+        return 1;
+    }
+}
+
+int ref_coverage_distance(const uint64_t* a, const uint64_t* b, uint32_t size, uint8_t ss, double* out) {
+    try {
+        detail::CoverageRaster x, y;
+        x.width = x.height = y.width = y.height = size;
+        size_t n = (size_t(size) * size + 63) / 64;
+        x.words.assign(a, a + n);
+        y.words.assign(b, b + n);
+        *out = detail::coverage_distance(x, y, ss, true);
+        return 0;
+    } catch (...) {
+        return 1;
+    }
+}
+
+int ref_evaluate(const float* rpos, uint64_t rnv, const uint32_t* ridx, uint64_t rni, const uint16_t* rmat,
+                 uint64_t rnm, const uint8_t* rds, uint64_t rnds, const float* cpos, uint64_t cnv,
+                 const uint32_t* cidx, uint64_t cni, const uint16_t* cmat, uint64_t cnm, const uint8_t* cds,
+                 uint64_t cnds, const float* center, double radius, const OracleSettings* s, Measure* out,
+                 int (*cancelled)(void*), void* user) {
+    try {
+        EvalSettings e;
+        e.profile = Profile(s->profile);
+        e.screen_size = s->screen_size;
+        e.limit = s->limit;
+        e.max_changed_area = s->max_changed_area;
+        e.views = {s->orthographic, s->perspective, s->rotation_seed};
+        e.supersample = s->supersample;
+        e.max_supersample = s->max_supersample;
+        e.force_two_sided = s->force_two_sided;
+        e.conservative_screen = s->conservative_screen;
+        e.weights = {s->normal_weight, s->color_weight, s->material_weight};
+        e.force_scalar = true;
+        if (cancelled) e.cancelled = [=] { return cancelled(user) != 0; };
+        auto m = evaluate(view_of(rpos, rnv, ridx, rni, rmat, rnm, rds, rnds),
+                          view_of(cpos, cnv, cidx, cni, cmat, cnm, cds, cnds),
+                          {{center[0], center[1], center[2]}, radius}, e);
+        *out = out_of(m);
+        return 0;
+    } catch (...) {
+        return 1;
+    }
+}
+
+// Classical quadric reduction (src/simplify.cpp reduce): Reuse output keeps the
+// source vertex IDs; returns the new index count or -1.
+int64_t ref_reduce_reuse(const float* pos, uint64_t nv, const uint32_t* idx, uint64_t ni, uint64_t target,
+                         uint32_t* out, uint64_t capacity) {
+    try {
+        ReduceSettings rs;
+        rs.output = OutputMode::Reuse;
+        rs.target_triangles = target;
+        auto lod = reduce(view_of(pos, nv, idx, ni, nullptr, 0, nullptr, 0), rs);
+        if (!lod.shared_vertices || lod.data.indices.size() > capacity) return -1;
+        std::memcpy(out, lod.data.indices.data(), lod.data.indices.size() * 4);
+        return int64_t(lod.data.indices.size());
+    } catch (...) {
+        return -1;
+    }
+}
+
+// Classical CPU generator (src/chain.cpp generate) on the coverage profile.
+// Writes per-level triangle counts and the adjacent/source audit maxima.
+struct ChainSettings {
+    uint8_t levels, beam_width, search_supersample, audit_supersample, max_supersample, pad[3];
+    uint16_t candidate_budget, search_ortho, search_persp, audit_ortho, audit_persp, pad2[3];
+    uint32_t search_seed, audit_seed;
+    double base_pixels, last_pixels, max_lod0_delta_px, transition0, transition1, max_changed_area;
+};
+int ref_generate(const float* pos, uint64_t nv, const uint32_t* idx, uint64_t ni, const ChainSettings* c,
+                 uint32_t* triangles, double* source_error, double* adjacent_error, uint64_t* audits,
+                 uint32_t* status) {
+    try {
+        Settings s;
+        s.levels = c->levels;
+        s.profile = Profile::Coverage;
+        s.beam_width = c->beam_width;
+        s.search_supersample = c->search_supersample;
+        s.audit_supersample = c->audit_supersample;
+        s.max_supersample = c->max_supersample;
+        s.candidate_budget = c->candidate_budget;
+        s.search_views = {c->search_ortho, c->search_persp, c->search_seed};
+        s.audit_views = {c->audit_ortho, c->audit_persp, c->audit_seed};
+        s.base_pixels = c->base_pixels;
+        s.last_pixels = c->last_pixels;
+        if (c->max_lod0_delta_px > 0) s.max_lod0_delta_px = c->max_lod0_delta_px;
+        s.transition.points = {{0, float(c->transition0)}, {1, float(c->transition1)}};
+        s.max_changed_area = c->max_changed_area;
+        auto r = generate(view_of(pos, nv, idx, ni, nullptr, 0, nullptr, 0), s);
+        for (size_t i = 0; i < r.lods.size(); ++i) {
+            triangles[i] = uint32_t(r.lods[i].data.indices.size() / 3);
+            source_error[i] = r.lods[i].source_error.error;
+            adjacent_error[i] = r.lods[i].adjacent.error;
+        }
+        uint64_t n = 0;
+        for (auto a : r.audit_evaluations) n += a;
+        *audits = n;
+        *status = uint32_t(r.status);
+        return int(r.lods.size());
+    } catch (...) {
+        return -1;
+    }
+}
+}

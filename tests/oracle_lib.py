@@ -1,0 +1,199 @@
+"""ctypes access to the CPU checkers built under oracle/ (test infrastructure).
+
+``Oracle("oracle")`` loads oracle/liboracle.so, the restatement of the
+reference CPU audit. ``Oracle("ref")`` loads oracle/_ref/libblitz_ref.so, a
+build of the reference's own CPU sources (present only where the reference
+checkout was available at build time).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from pathlib import Path
+
+import numpy as np
+
+from blitzremesher_amd import _lib
+from blitzremesher_amd.audit import EvalSettings, Measurement, Mesh
+
+ROOT = Path(__file__).resolve().parent.parent
+PATHS = {"oracle": ROOT / "oracle" / "liboracle.so", "ref": ROOT / "oracle" / "_ref" / "libblitz_ref.so"}
+
+
+class OracleSettings(C.Structure):
+    _fields_ = [
+        ("screen_size", C.c_double), ("limit", C.c_double), ("max_changed_area", C.c_double),
+        ("rotation_seed", C.c_uint32), ("orthographic", C.c_uint16), ("perspective", C.c_uint16),
+        ("supersample", C.c_uint8), ("max_supersample", C.c_uint8), ("force_two_sided", C.c_uint8),
+        ("conservative_screen", C.c_uint8), ("profile", C.c_int32),
+        ("normal_weight", C.c_double), ("color_weight", C.c_double), ("material_weight", C.c_double),
+    ]
+
+
+class OracleMeasurement(C.Structure):
+    _fields_ = [
+        ("error", C.c_double), ("coverage", C.c_double), ("coverage_upper", C.c_double),
+        ("changed_area", C.c_double), ("normal_degrees", C.c_double),
+        ("worst_view", C.c_uint32), ("changed_area_worst_view", C.c_uint32), ("views_evaluated", C.c_uint32),
+        ("supersample", C.c_uint8), ("complete", C.c_uint8), ("passed", C.c_uint8),
+        ("resource_limited", C.c_uint8), ("cancelled", C.c_uint8),
+    ]
+
+
+class ChainSettings(C.Structure):
+    _fields_ = [
+        ("levels", C.c_uint8), ("beam_width", C.c_uint8), ("search_supersample", C.c_uint8),
+        ("audit_supersample", C.c_uint8), ("max_supersample", C.c_uint8), ("pad", C.c_uint8 * 3),
+        ("candidate_budget", C.c_uint16), ("search_ortho", C.c_uint16), ("search_persp", C.c_uint16),
+        ("audit_ortho", C.c_uint16), ("audit_persp", C.c_uint16), ("pad2", C.c_uint16 * 3),
+        ("search_seed", C.c_uint32), ("audit_seed", C.c_uint32),
+        ("base_pixels", C.c_double), ("last_pixels", C.c_double), ("max_lod0_delta_px", C.c_double),
+        ("transition0", C.c_double), ("transition1", C.c_double), ("max_changed_area", C.c_double),
+    ]
+
+
+def available(kind: str) -> bool:
+    return PATHS[kind].exists()
+
+
+def _ptr(a: np.ndarray | None, ctype):
+    if a is None or a.size == 0:
+        return None
+    return a.ctypes.data_as(C.POINTER(ctype))
+
+
+def _mesh_args(m: Mesh):
+    mat = m.materials if m.materials is not None else None
+    ds = m.double_sided if m.double_sided is not None else None
+    return [_ptr(m.positions, C.c_float), m.positions.shape[0], _ptr(m.indices, C.c_uint32), int(m.indices.size),
+            _ptr(mat, C.c_uint16), 0 if mat is None else int(mat.size), _ptr(ds, C.c_uint8),
+            0 if ds is None else int(ds.size)]
+
+
+class Oracle:
+    def __init__(self, kind: str = "oracle", path: Path | None = None):
+        path = path or PATHS[kind]
+        if not path.exists():
+            raise RuntimeError(f"{path} not built")
+        self.kind = kind
+        self.lib = C.CDLL(str(path))
+        self.p = "oracle_" if kind == "oracle" else "ref_"
+
+    def fn(self, name):
+        return getattr(self.lib, self.p + name)
+
+    def bounds(self, mesh: Mesh):
+        c = (C.c_float * 3)()
+        r = C.c_double()
+        self.fn("bounds")(_ptr(mesh.positions, C.c_float), C.c_uint64(mesh.positions.shape[0]), c, C.byref(r))
+        return np.array(c[:], np.float32), float(r.value)
+
+    def cameras(self, center, radius, screen, ortho, persp, seed) -> list[_lib.Camera]:
+        n = ortho + persp
+        out = (_lib.Camera * max(1, n))()
+        cc = (C.c_float * 3)(*[float(x) for x in center])
+        f = self.fn("cameras")
+        f.argtypes = [C.POINTER(C.c_float), C.c_double, C.c_double, C.c_uint16, C.c_uint16, C.c_uint32,
+                      C.POINTER(_lib.Camera), C.c_uint64]
+        assert f(cc, radius, screen, ortho, persp, seed & 0xFFFFFFFF, out, n) == 0
+        return list(out)[:n]
+
+    def raster(self, mesh: Mesh, center, radius, camera: _lib.Camera, screen, ss, force_two=False):
+        """Row-major words, size, clipped; None over the sample cap."""
+        size = math.ceil(screen + 8) * ss
+        cap = (size * size + 63) // 64
+        words = np.zeros(max(1, cap), np.uint64)
+        clipped = C.c_uint8()
+        out_size = C.c_uint32()
+        cc = (C.c_float * 3)(*[float(x) for x in center])
+        f = self.fn("raster")
+        f.argtypes = [C.POINTER(C.c_float), C.c_uint64, C.POINTER(C.c_uint32), C.c_uint64, C.POINTER(C.c_uint16),
+                      C.c_uint64, C.POINTER(C.c_uint8), C.c_uint64, C.POINTER(C.c_float), C.c_double,
+                      C.POINTER(_lib.Camera), C.c_double, C.c_uint8, C.c_uint8, C.POINTER(C.c_uint64), C.c_uint64,
+                      C.POINTER(C.c_uint8), C.POINTER(C.c_uint32)]
+        st = f(*_mesh_args(mesh), cc, radius, C.byref(camera), screen, ss, int(force_two),
+               words.ctypes.data_as(C.POINTER(C.c_uint64)), cap, C.byref(clipped), C.byref(out_size))
+        if st == 2:
+            return None
+        assert st == 0, st
+        return words[:cap], int(out_size.value), bool(clipped.value)
+
+    def coverage_distance(self, a, b, size, ss) -> float:
+        out = C.c_double()
+        f = self.fn("coverage_distance")
+        f.argtypes = [C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.c_uint32, C.c_uint8, C.POINTER(C.c_double)]
+        a = np.ascontiguousarray(a, np.uint64)
+        b = np.ascontiguousarray(b, np.uint64)
+        assert f(a.ctypes.data_as(C.POINTER(C.c_uint64)), b.ctypes.data_as(C.POINTER(C.c_uint64)), size, ss,
+                 C.byref(out)) == 0
+        return float(out.value)
+
+    def distance_field(self, words, size) -> np.ndarray:
+        assert self.kind == "oracle"
+        out = np.zeros(size * size, np.float32)
+        w = np.ascontiguousarray(words, np.uint64)
+        self.lib.oracle_distance_field(w.ctypes.data_as(C.POINTER(C.c_uint64)), C.c_uint32(size),
+                                       out.ctypes.data_as(C.POINTER(C.c_float)))
+        return out.reshape(size, size)
+
+    def evaluate(self, ref: Mesh, cand: Mesh, center, radius, s: EvalSettings) -> Measurement:
+        o = OracleSettings(s.screen_size, s.limit, s.max_changed_area, s.rotation_seed & 0xFFFFFFFF, s.orthographic,
+                           s.perspective, s.supersample, s.max_supersample, int(s.force_two_sided),
+                           int(s.conservative_screen), s.profile, s.normal_weight, s.color_weight,
+                           s.material_weight)
+        out = OracleMeasurement()
+        cc = (C.c_float * 3)(*[float(x) for x in center])
+        cb = _lib.CANCEL_FN(lambda _u: 1 if s.cancelled() else 0) if s.cancelled else None
+        f = self.fn("evaluate")
+        f.argtypes = [C.POINTER(C.c_float), C.c_uint64, C.POINTER(C.c_uint32), C.c_uint64, C.POINTER(C.c_uint16),
+                      C.c_uint64, C.POINTER(C.c_uint8), C.c_uint64] * 2 + \
+                     [C.POINTER(C.c_float), C.c_double, C.POINTER(OracleSettings), C.POINTER(OracleMeasurement),
+                      _lib.CANCEL_FN, C.c_void_p]
+        st = f(*_mesh_args(ref), *_mesh_args(cand), cc, radius, C.byref(o), C.byref(out),
+               cb if cb else _lib.CANCEL_FN(), None)
+        if st != 0:
+            raise ValueError(f"{self.kind} evaluate rejected the settings")
+        return Measurement.from_c(out)
+
+    # reference-only helpers
+    def reduce_reuse(self, mesh: Mesh, target: int) -> Mesh:
+        assert self.kind == "ref"
+        out = np.zeros(mesh.indices.size, np.uint32)
+        f = self.lib.ref_reduce_reuse
+        f.restype = C.c_int64
+        f.argtypes = [C.POINTER(C.c_float), C.c_uint64, C.POINTER(C.c_uint32), C.c_uint64, C.c_uint64,
+                      C.POINTER(C.c_uint32), C.c_uint64]
+        n = f(_ptr(mesh.positions, C.c_float), mesh.positions.shape[0], _ptr(mesh.indices, C.c_uint32),
+              mesh.indices.size, target, out.ctypes.data_as(C.POINTER(C.c_uint32)), out.size)
+        assert n >= 0
+        return Mesh(mesh.positions, out[:n].reshape(-1, 3).copy(), None, mesh.double_sided)
+
+    def generate(self, mesh: Mesh, cs: ChainSettings):
+        assert self.kind == "ref"
+        tri = np.zeros(64, np.uint32)
+        se = np.zeros(64, np.float64)
+        ae = np.zeros(64, np.float64)
+        audits = C.c_uint64()
+        status = C.c_uint32()
+        f = self.lib.ref_generate
+        f.restype = C.c_int
+        n = f(_ptr(mesh.positions, C.c_float), C.c_uint64(mesh.positions.shape[0]), _ptr(mesh.indices, C.c_uint32),
+              C.c_uint64(mesh.indices.size), C.byref(cs), tri.ctypes.data_as(C.POINTER(C.c_uint32)),
+              se.ctypes.data_as(C.POINTER(C.c_double)), ae.ctypes.data_as(C.POINTER(C.c_double)), C.byref(audits),
+              C.byref(status))
+        assert n >= 0
+        return tri[:n].tolist(), se[:n].tolist(), ae[:n].tolist(), int(audits.value), int(status.value)
+
+
+def words_to_mask(words: np.ndarray, size: int) -> np.ndarray:
+    """Row-major packed words -> (size, size) bool array."""
+    bits = np.unpackbits(np.ascontiguousarray(words, np.uint64).view(np.uint8), bitorder="little")
+    return bits[: size * size].reshape(size, size).astype(bool)
+
+
+def mask_to_words(mask: np.ndarray) -> np.ndarray:
+    flat = np.ascontiguousarray(mask, dtype=bool).reshape(-1)
+    n = (flat.size + 63) // 64
+    padded = np.zeros(n * 64, np.uint8)
+    padded[: flat.size] = flat
+    return np.packbits(padded, bitorder="little").view(np.uint64)

@@ -1,0 +1,222 @@
+"""Pins the CPU oracle (oracle/audit_oracle.cpp) before it is trusted.
+
+1. Golden fixtures written by a build of the reference's own CPU sources:
+   cameras, mask digests, coverage distances and full Measurements.
+2. The reference build itself, where present, on fresh seeded inputs.
+3. What the mathematics fixes: brute-force Hausdorff distances, closed forms
+   for point pairs and discs, an order-independent OR over faces, and the
+   float rounding of squared distances above 2^24.
+A planted mistake in the oracle (dropped radius term, transposed edge,
+missing float rounding) fails at least one of these; see the last test.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import pytest
+
+from blitzremesher_amd import meshes
+from blitzremesher_amd.audit import EvalSettings, Mesh
+from golden_data import (audit_golden, camera_fields, camera_of, digest, golden_meshes, measurement_of,
+                         settings_of, unpack_mask)
+from oracle_lib import Oracle, available, mask_to_words, words_to_mask
+
+
+@pytest.fixture(scope="module")
+def oracle():
+    return Oracle("oracle")
+
+
+def test_cameras_match_golden(oracle):
+    for case in audit_golden()["cameras"]:
+        got = oracle.cameras(case["center"], case["radius"], case["screen"], case["ortho"], case["persp"], case["seed"])
+        assert [camera_fields(c) for c in got] == case["cameras"]
+
+
+def test_rasters_match_golden(oracle):
+    ms = golden_meshes()
+    for case in audit_golden()["rasters"]:
+        r = oracle.raster(ms[case["mesh"]], case["center"], case["radius"], camera_of(case["camera"]),
+                          case["screen"], case["ss"], case["two_sided"])
+        if case.get("limited"):
+            assert r is None
+            continue
+        words, size, clipped = r
+        assert (size, clipped, digest(words)) == (case["size"], case["clipped"], case["sha256"]), case["case"]
+
+
+def test_distances_match_golden(oracle):
+    for case in audit_golden()["distances"]:
+        a = unpack_mask(case["a"], case["size"])
+        b = unpack_mask(case["b"], case["size"])
+        assert oracle.coverage_distance(mask_to_words(a), mask_to_words(b), case["size"], case["ss"]) == case["value"]
+
+
+def test_evaluations_match_golden(oracle):
+    ms = golden_meshes()
+    cases = audit_golden()["evaluations"]
+    assert len(cases) >= 70
+    for case in cases:
+        got = oracle.evaluate(ms[case["reference"]], ms[case["candidate"]], case["center"], case["radius"],
+                              settings_of(case["settings"]))
+        assert got == measurement_of(case["measurement"]), case["case"]
+
+
+@pytest.mark.skipif(not available("ref"), reason="reference build not present")
+def test_oracle_matches_reference_build_on_fresh_inputs(oracle):
+    ref = Oracle("ref")
+    for seed in range(6):
+        m = meshes.icosphere(2 + seed % 2, 0.03, seed)
+        c, r = oracle.bounds(m)
+        cand = meshes.jitter(m, 0.003 * (1 + seed), 40 + seed, 0.5)
+        s = EvalSettings(profile=0, screen_size=20 + 7 * seed, supersample=1 + seed % 3, max_supersample=8,
+                         orthographic=5, perspective=3, rotation_seed=seed * 7919, limit=0.5 + 0.4 * seed)
+        assert oracle.evaluate(m, cand, c, r, s) == ref.evaluate(m, cand, c, r, s)
+    # Squared distances above 2^24 go through two float roundings.
+    size = 4200
+    a = np.zeros((size, size), bool)
+    b = np.zeros((size, size), bool)
+    a[0, 0] = a[size - 1, size - 1] = a[17, 4000] = True
+    b[0, 0] = b[size - 1, 17] = True
+    wa, wb = mask_to_words(a), mask_to_words(b)
+    assert oracle.coverage_distance(wa, wb, size, 1) == ref.coverage_distance(wa, wb, size, 1)
+
+
+def brute_hausdorff(a: np.ndarray, b: np.ndarray) -> float:
+    pa = np.argwhere(a).astype(np.float64)
+    pb = np.argwhere(b).astype(np.float64)
+    d = ((pa[:, None, :] - pb[None, :, :]) ** 2).sum(-1)
+    return max(d.min(1).max(), d.min(0).max())
+
+
+def test_distance_against_brute_force(oracle):
+    # tests/contracts.cpp:64-68,115-121: O(n^2) Hausdorff on random masks.
+    rng = np.random.default_rng(2921)
+    for trial in range(60):
+        size = int(rng.integers(2, 33))
+        a = rng.random((size, size)) < rng.uniform(0.02, 0.4)
+        b = rng.random((size, size)) < rng.uniform(0.02, 0.4)
+        a[0, 0] = True
+        b[size - 1, 0] = True
+        for ss in (1, 3, 8):
+            want = math.sqrt(brute_hausdorff(a, b)) / ss
+            assert oracle.coverage_distance(mask_to_words(a), mask_to_words(b), size, ss) == want
+
+
+def test_distance_field_against_brute_force(oracle):
+    rng = np.random.default_rng(5)
+    for trial in range(20):
+        size = int(rng.integers(1, 24))
+        m = rng.random((size, size)) < 0.15
+        if not m.any():
+            m[size // 2, size // 3] = True
+        field = oracle.distance_field(mask_to_words(m), size)
+        sites = np.argwhere(m)
+        yy, xx = np.mgrid[0:size, 0:size]
+        d = ((yy[..., None] - sites[:, 0]) ** 2 + (xx[..., None] - sites[:, 1]) ** 2).min(-1)
+        assert np.array_equal(field, d.astype(np.float32))
+
+
+def test_closed_forms(oracle):
+    # Empty/empty = 0; one side empty = infinity; single points at offset.
+    e = mask_to_words(np.zeros((5, 5), bool))
+    one = np.zeros((5, 5), bool)
+    one[2, 2] = True
+    assert oracle.coverage_distance(e, e, 5, 1) == 0
+    assert math.isinf(oracle.coverage_distance(e, mask_to_words(one), 5, 1))
+    for dx, dy, ss in [(3, 4, 1), (7, 0, 2), (11, 13, 8)]:
+        a = np.zeros((40, 40), bool)
+        b = np.zeros((40, 40), bool)
+        a[1, 2] = True
+        b[1 + dy, 2 + dx] = True
+        assert oracle.coverage_distance(mask_to_words(a), mask_to_words(b), 40, ss) == math.sqrt(dx * dx + dy * dy) / ss
+    # Concentric filled discs of radii R and r: Hausdorff R - r up to one sample.
+    yy, xx = np.mgrid[0:81, 0:81]
+    rr = (yy - 40) ** 2 + (xx - 40) ** 2
+    for R, r in [(30, 10), (25, 24), (12, 3)]:
+        d = oracle.coverage_distance(mask_to_words(rr <= R * R), mask_to_words(rr <= r * r), 81, 1)
+        assert abs(d - (R - r)) <= 1.0
+
+
+def test_mask_is_order_independent(oracle):
+    # Coverage is an OR over faces: permuting faces (without out-of-frame
+    # faces) cannot change the mask.
+    m = meshes.torus(1.0, 0.3, 30, 12, 0.01, 3)
+    c, r = oracle.bounds(m)
+    perm = np.random.default_rng(1).permutation(m.triangles)
+    shuffled = Mesh(m.positions, m.indices[perm])
+    for cam in oracle.cameras(c, r, 48, 3, 2, 11):
+        a = oracle.raster(m, c, r, cam, 48, 2)
+        b = oracle.raster(shuffled, c, r, cam, 48, 2)
+        assert np.array_equal(a[0], b[0]) and a[2] == b[2]
+
+
+def test_orthographic_translation_shifts_mask(oracle):
+    # An orthographic camera maps a translation along `right` by k samples /
+    # (scale * ss) to a k-sample shift of the mask.
+    m = meshes.grid(6)
+    c, r = oracle.bounds(m)
+    cam = oracle.cameras(c, r, 40, 1, 0, 3)[0]
+    ss = 2
+    step = 4 / (cam.scale * ss)  # four samples
+    right = np.array(cam.right[:], np.float64)
+    shifted = Mesh((m.positions.astype(np.float64) + right * step).astype(np.float32), m.indices,
+                   double_sided=m.double_sided)
+    a = words_to_mask(*oracle.raster(m, c, r, cam, 40, ss, True)[:2])
+    b = words_to_mask(*oracle.raster(shifted, c, r, cam, 40, ss, True)[:2])
+    # Compare interiors away from rounding at the shifted silhouette.
+    assert abs(int(a.sum()) - int(b.sum())) <= 2 * a.shape[0]
+    assert (np.roll(a, 4, axis=1)[:, 8:-8] == b[:, 8:-8]).mean() > 0.995
+
+
+def _golden_failures(o: Oracle) -> int:
+    """Number of golden or brute-force checks a (possibly planted) oracle fails."""
+    failures = 0
+    ms = golden_meshes()
+    for case in audit_golden()["rasters"][::7]:
+        r = o.raster(ms[case["mesh"]], case["center"], case["radius"], camera_of(case["camera"]), case["screen"],
+                     case["ss"], case["two_sided"])
+        failures += r is not None and digest(r[0]) != case["sha256"]
+    for case in audit_golden()["evaluations"]:
+        got = o.evaluate(ms[case["reference"]], ms[case["candidate"]], case["center"], case["radius"],
+                         settings_of(case["settings"]))
+        failures += got != measurement_of(case["measurement"])
+    failures += o.coverage_distance(*double_rounding_case(), 4200, 1) != DOUBLE_ROUNDED
+    return failures
+
+
+# Sites at offset (dy, dx) = (1, 4097): the reference rounds 4097^2 to float in
+# its row pass (16785409 -> 16785408), adds 1 and rounds again (16785408),
+# where one rounding of the exact 16785410 would give 16785410.
+DOUBLE_ROUNDED = math.sqrt(16785408.0)
+
+
+def double_rounding_case():
+    size = 4200
+    a = np.zeros((size, size), bool)
+    b = np.zeros((size, size), bool)
+    a[0, 0] = b[0, 0] = True
+    b[size - 1, 0] = True
+    a[size - 2, 4097] = True
+    return mask_to_words(a), mask_to_words(b)
+
+
+def test_double_float_rounding_closed_form(oracle):
+    assert oracle.coverage_distance(*double_rounding_case(), 4200, 1) == DOUBLE_ROUNDED
+
+
+@pytest.mark.parametrize("plant", [1, 2, 3, 4])
+def test_planted_mistakes_are_caught(tmp_path, plant):
+    import shutil
+    import subprocess
+    from pathlib import Path
+    gxx = shutil.which("g++")
+    if gxx is None:
+        pytest.skip("g++ not available")
+    root = Path(__file__).resolve().parent.parent
+    out = tmp_path / f"liboracle_plant{plant}.so"
+    subprocess.run([gxx, "-std=c++20", "-O2", "-fPIC", "-ffp-contract=off", "-shared", f"-DORACLE_PLANT={plant}",
+                    "-o", str(out), str(root / "oracle" / "audit_oracle.cpp")], check=True)
+    assert _golden_failures(Oracle("oracle", out)) > 0
+    assert _golden_failures(Oracle("oracle")) == 0
