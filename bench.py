@@ -33,8 +33,12 @@ def workload():
     ref = meshes.heightfield(512, 0.1, 2)
     cand = reduce_reuse(ref, int(round(ref.triangles * 0.75)))
     center, radius = bounds(ref)
+    # The heightfield is an open surface seen from all around the sphere, so
+    # both sides are drawn (force_two_sided); with one-sided faces the views
+    # from below are empty or near-empty and the audit stops there.
     settings = EvalSettings(profile=0, orthographic=64, perspective=0, rotation_seed=0xB1172026, screen_size=120,
-                            supersample=8, max_supersample=8, limit=1e3, max_changed_area=1.0)
+                            supersample=8, max_supersample=8, limit=1e3, max_changed_area=1.0,
+                            force_two_sided=True)
     return ref, cand, center, radius, settings
 
 
@@ -84,13 +88,15 @@ def main() -> int:
               f"has one GPU and the audit path has no multi-GPU leg", file=sys.stderr)
         return 2
 
-    from blitzremesher_amd import Audit
+    from blitzremesher_amd import Audit, _lib
     ref, cand, center, radius, settings = workload()
-    try:
-        import torch
-        sync = torch.cuda.synchronize if torch.cuda.is_available() else (lambda: None)
-    except Exception:  # torch is plumbing only
-        sync = lambda: None  # noqa: E731
+    # torch is not imported: its wheel bundles a second HIP runtime, and two
+    # HIP runtimes in one process cannot share the device. The library's own
+    # hipDeviceSynchronize brackets the timed region instead.
+    lib = _lib.load()
+
+    def sync() -> None:
+        _lib.check(lib.blitz_hip_device_synchronize(0))
 
     with Audit(0) as audit:
         dref, dcand = audit.upload(ref), audit.upload(cand)

@@ -101,6 +101,7 @@ _ERR = [C.c_char_p, C.c_size_t]
 PROTOTYPES = {
     "blitz_hip_api_version": (C.c_uint32, []),
     "blitz_hip_available": (C.c_int, [C.c_int32]),
+    "blitz_hip_device_synchronize": (C.c_int, [C.c_int32]),
     "blitz_hip_audit_options_init": (C.c_int, [C.POINTER(AuditOptions), C.c_size_t]),
     "blitz_hip_eval_settings_init": (C.c_int, [C.POINTER(EvalSettingsC), C.c_size_t]),
     "blitz_hip_bounds_of": (C.c_int, [C.POINTER(BlitzMesh), C.POINTER(Bounds), *_ERR]),

@@ -16,6 +16,7 @@
 #include <limits>
 #include <map>
 #include <new>
+#include <tuple>
 
 #include "audit_kernels.hpp"
 
@@ -161,6 +162,8 @@ std::unique_ptr<DeviceMesh> Audit::upload(const HostMesh& m) {
     d->faces = uint32_t(m.indices.size() / 3);
     d->digest[0] = m.digest[0];
     d->digest[1] = m.digest[1];
+    d->pos_digest[0] = m.pos_digest[0];
+    d->pos_digest[1] = m.pos_digest[1];
     d->serial = next_serial_++;
     std::vector<float4> pos(d->vertices);
     for (uint32_t i = 0; i < d->vertices; ++i)
@@ -326,6 +329,125 @@ std::vector<Measurement> Audit::evaluate_batch(const DeviceMesh& reference,
     return out;
 }
 
+void Audit::rasterize(const std::vector<RasterJob>& jobs, const std::vector<Camera>& cams, const Bounds& b,
+                      uint8_t ss, uint32_t size, bool force_two, std::vector<uint32_t>& status_out) {
+    const uint32_t tiles = (size + kTile - 1) / kTile;
+    const size_t words = size_t(tiles) * tiles;
+    // One projection per (vertex stream contents, view).
+    struct Key {
+        uint64_t d0, d1;
+        uint32_t vertices, view;
+        bool operator<(const Key& o) const {
+            return std::tie(d0, d1, vertices, view) < std::tie(o.d0, o.d1, o.vertices, o.view);
+        }
+    };
+    std::map<Key, uint32_t> slot;
+    std::vector<std::pair<const DeviceMesh*, uint32_t>> sets;
+    std::vector<uint32_t> job_slot(jobs.size());
+    for (size_t j = 0; j < jobs.size(); ++j) {
+        const Key k{jobs[j].mesh->pos_digest[0], jobs[j].mesh->pos_digest[1], jobs[j].mesh->vertices, jobs[j].view};
+        auto it = slot.find(k);
+        if (it == slot.end()) {
+            it = slot.emplace(k, uint32_t(sets.size())).first;
+            sets.push_back({jobs[j].mesh, jobs[j].view});
+        }
+        job_slot[j] = it->second;
+    }
+    std::vector<size_t> offset(sets.size() + 1, 0);
+    uint32_t max_vertices = 0;
+    bool perspective = false;
+    for (size_t i = 0; i < sets.size(); ++i) {
+        offset[i + 1] = offset[i] + sets[i].first->vertices;
+        max_vertices = std::max(max_vertices, sets[i].first->vertices);
+        perspective |= cams[sets[i].second].perspective != 0;
+    }
+    double2* xy = scratch_as<double2>(9, std::max<size_t>(1, offset.back()));
+    double* zs = perspective ? scratch_as<double>(10, std::max<size_t>(1, offset.back())) : nullptr;
+    std::vector<ProjectTask> projections(sets.size());
+    for (size_t i = 0; i < sets.size(); ++i) {
+        ProjectTask& p = projections[i];
+        p = {};
+        p.positions = sets[i].first->positions;
+        p.xy = xy + offset[i];
+        p.z = cams[sets[i].second].perspective ? zs + offset[i] : nullptr;
+        p.camera = cams[sets[i].second];
+        p.center = b.center;
+        p.vertices = sets[i].first->vertices;
+        p.size = size;
+        p.ss = ss;
+    }
+    std::vector<RasterTask> tasks(jobs.size());
+    uint32_t max_faces = 0;
+    for (size_t j = 0; j < jobs.size(); ++j) {
+        const DeviceMesh& m = *jobs[j].mesh;
+        RasterTask& t = tasks[j];
+        t = {};
+        t.xy = xy + offset[job_slot[j]];
+        t.z = projections[job_slot[j]].z;
+        t.indices = m.indices;
+        t.two_sided = m.two_sided;
+        t.mask = jobs[j].mask;
+        t.faces = m.faces;
+        t.size = size;
+        t.tiles = tiles;
+        t.force_two = force_two;
+        max_faces = std::max(max_faces, m.faces);
+    }
+    uint32_t* status = scratch_as<uint32_t>(6, std::max<size_t>(1, jobs.size() * 2));
+    for (size_t j = 0; j < jobs.size(); ++j) tasks[j].status = status + 2 * j;
+    std::vector<uint32_t> init(jobs.size() * 2);
+    for (size_t j = 0; j < jobs.size(); ++j) {
+        init[2 * j] = kNone;
+        init[2 * j + 1] = 0;
+    }
+    auto launch = [&](const std::vector<RasterTask>& list, size_t slot_id) {
+        RasterTask* d = scratch_as<RasterTask>(slot_id, std::max<size_t>(1, list.size()));
+        hip_check(hipMemcpyAsync(d, list.data(), list.size() * sizeof(RasterTask), hipMemcpyHostToDevice, stream_),
+                  "upload raster tasks");
+        uint32_t faces = 0;
+        for (const auto& t : list) faces = std::max(faces, t.faces);
+        RasterGrid g{};
+        g.tasks = uint32_t(list.size());
+        g.band_rows = std::max<uint32_t>(1, std::min<uint32_t>(tiles, kBandWords / tiles));
+        g.bands = (tiles + g.band_rows - 1) / g.band_rows;
+        // Up to 32 face chunks per task (one per CU of an XCD, so an XCD works
+        // on one view at a time and its L2 holds that view's projected
+        // vertices), each with at least 8 batches of 1024 faces.
+        g.chunks = std::max<uint32_t>(1, std::min<uint32_t>(32, faces / (8 * kRasterThreads)));
+        launch_raster(d, g, tiles, stream_);
+    };
+    hip_check(hipMemcpyAsync(status, init.data(), init.size() * 4, hipMemcpyHostToDevice, stream_), "upload status");
+    ProjectTask* dp = scratch_as<ProjectTask>(8, std::max<size_t>(1, projections.size()));
+    hip_check(hipMemcpyAsync(dp, projections.data(), projections.size() * sizeof(ProjectTask), hipMemcpyHostToDevice,
+                             stream_),
+              "upload projections");
+    launch_project(dp, uint32_t(projections.size()), max_vertices, stream_);
+    launch(tasks, 4);
+    hip_check(hipGetLastError(), "raster launch");
+    status_out.assign(jobs.size() * 2, 0);
+    hip_check(hipMemcpyAsync(status_out.data(), status, status_out.size() * 4, hipMemcpyDeviceToHost, stream_),
+              "download status");
+    hip_check(hipStreamSynchronize(stream_), "raster");
+    timing_.raster_tasks += uint32_t(jobs.size());
+    for (const auto& t : tasks) timing_.raster_triangles += t.faces;
+    timing_.mask_bytes += uint64_t(jobs.size()) * words * 8;
+    // The reference stops rasterizing at the first out-of-frame face
+    // (src/evaluate.cpp:159-160): redraw such masks from the faces before it.
+    std::vector<RasterTask> redo;
+    for (size_t j = 0; j < jobs.size(); ++j)
+        if (status_out[2 * j] != kNone) {
+            RasterTask t = tasks[j];
+            t.faces = status_out[2 * j];
+            redo.push_back(t);
+            launch_fill(t.mask, words, 0, stream_);
+        }
+    if (!redo.empty()) {
+        launch(redo, 7);
+        hip_check(hipGetLastError(), "raster redo launch");
+        hip_check(hipStreamSynchronize(stream_), "raster redo");
+    }
+}
+
 void Audit::run_items(const DeviceMesh& reference, const std::vector<const DeviceMesh*>& candidates,
                       const std::vector<Camera>& cams, const Bounds& b, const Settings& s,
                       const std::vector<Item>& all, std::vector<ViewResult>& out) {
@@ -334,56 +456,43 @@ void Audit::run_items(const DeviceMesh& reference, const std::vector<const Devic
     const uint32_t size = raster_side(s.screen_size, ss), tiles = (size + kTile - 1) / kTile;
     const size_t words = size_t(tiles) * tiles;
     const size_t samples = size_t(size) * size;
-    // Per item: candidate mask, two row-distance planes, summary. Per view:
-    // one reference mask shared by every candidate of the chunk.
-    const size_t per_item = words * 8 + samples * 4 + sizeof(ViewSummary) + sizeof(RasterTask) + sizeof(ViewPair);
+    // Per item: candidate mask, two row-distance planes, projection, summary.
+    // Per view: one reference mask shared by every candidate of the chunk.
+    const size_t per_item = words * 8 + samples * 4 + sizeof(ViewSummary) + sizeof(RasterTask) + sizeof(ViewPair) +
+                            24 * size_t(std::max(reference.vertices, 1u));
     const uint64_t budget = memory_bytes_ ? memory_bytes_ : (uint64_t(8) << 30);
     const size_t chunk = std::max<size_t>(1, std::min<size_t>(all.size(), budget / (per_item + words * 8)));
     for (size_t begin = 0; begin < all.size(); begin += chunk) {
         const size_t count = std::min(chunk, all.size() - begin);
-        // Reference raster tasks, one per distinct view in this chunk.
         std::map<uint32_t, uint32_t> ref_slot;
         for (size_t i = 0; i < count; ++i) ref_slot.emplace(all[begin + i].view, 0u);
         uint32_t next = 0;
         for (auto& kv : ref_slot) kv.second = next++;
-        const uint32_t nref = next, ncand = uint32_t(count), ntasks = nref + ncand;
+        const uint32_t nref = next, ncand = uint32_t(count);
         uint64_t* ref_masks = scratch_as<uint64_t>(0, size_t(nref) * words);
         uint64_t* cand_masks = scratch_as<uint64_t>(1, size_t(ncand) * words);
         uint16_t* dist = scratch_as<uint16_t>(2, size_t(ncand) * samples * 2);
         ViewSummary* summaries = scratch_as<ViewSummary>(3, ncand);
-        RasterTask* dtasks = scratch_as<RasterTask>(4, ntasks);
         ViewPair* dpairs = scratch_as<ViewPair>(5, ncand);
-        uint32_t* status = scratch_as<uint32_t>(6, size_t(ntasks) * 2);
-
-        auto make_task = [&](const DeviceMesh& m, const Camera& cam, uint64_t* mask, uint32_t* st) {
-            RasterTask t{};
-            t.positions = m.positions;
-            t.indices = m.indices;
-            t.two_sided = m.two_sided;
-            t.mask = mask;
-            t.status = st;
-            t.camera = cam;
-            t.center = b.center;
-            t.faces = m.faces;
-            t.size = size;
-            t.tiles = tiles;
-            t.ss = ss;
-            t.force_two = s.force_two_sided;
-            return t;
-        };
-        std::vector<RasterTask> tasks(ntasks);
-        std::vector<ViewPair> pairs(ncand);
-        uint32_t max_faces = 0;
-        for (auto& kv : ref_slot) {
-            tasks[kv.second] = make_task(reference, cams[kv.first], ref_masks + size_t(kv.second) * words,
-                                         status + 2 * kv.second);
-            max_faces = std::max(max_faces, reference.faces);
+        // Jobs in view order, each view's reference first: consecutive raster
+        // tasks then share one projection (and one XCD's L2).
+        std::vector<RasterJob> jobs;
+        std::vector<uint32_t> ref_job(nref), cand_job(ncand);
+        {
+            std::vector<std::vector<uint32_t>> by_view(nref);
+            for (uint32_t i = 0; i < ncand; ++i) by_view[ref_slot[all[begin + i].view]].push_back(i);
+            for (auto& kv : ref_slot) {
+                ref_job[kv.second] = uint32_t(jobs.size());
+                jobs.push_back({&reference, kv.first, ref_masks + size_t(kv.second) * words});
+                for (uint32_t i : by_view[kv.second]) {
+                    cand_job[i] = uint32_t(jobs.size());
+                    jobs.push_back({candidates[all[begin + i].candidate], kv.first, cand_masks + size_t(i) * words});
+                }
+            }
         }
+        std::vector<ViewPair> pairs(ncand);
         for (uint32_t i = 0; i < ncand; ++i) {
             const Item& it = all[begin + i];
-            const DeviceMesh& m = *candidates[it.candidate];
-            tasks[nref + i] = make_task(m, cams[it.view], cand_masks + size_t(i) * words, status + 2 * (nref + i));
-            max_faces = std::max(max_faces, m.faces);
             ViewPair& p = pairs[i];
             p.a = ref_masks + size_t(ref_slot[it.view]) * words;
             p.c = cand_masks + size_t(i) * words;
@@ -392,33 +501,23 @@ void Audit::run_items(const DeviceMesh& reference, const std::vector<const Devic
             p.size = size;
             p.tiles = tiles;
         }
-        std::vector<uint32_t> status_init(size_t(ntasks) * 2);
-        for (uint32_t i = 0; i < ntasks; ++i) {
-            status_init[2 * i] = kNone;
-            status_init[2 * i + 1] = 0;
-        }
-        hip_check(hipMemcpyAsync(dtasks, tasks.data(), tasks.size() * sizeof(RasterTask), hipMemcpyHostToDevice, stream_),
-                  "upload tasks");
-        hip_check(hipMemcpyAsync(dpairs, pairs.data(), pairs.size() * sizeof(ViewPair), hipMemcpyHostToDevice, stream_),
-                  "upload pairs");
-        hip_check(hipMemcpyAsync(status, status_init.data(), status_init.size() * 4, hipMemcpyHostToDevice, stream_),
-                  "upload status");
+        hip_check(hipEventRecord(events_[0], stream_), "event");
         launch_fill(ref_masks, size_t(nref) * words, 0, stream_);
         launch_fill(cand_masks, size_t(ncand) * words, 0, stream_);
-        launch_fill(reinterpret_cast<uint64_t*>(summaries), ncand * sizeof(ViewSummary) / 8, 0, stream_);
-        hip_check(hipEventRecord(events_[0], stream_), "event");
-        launch_raster(dtasks, ntasks, max_faces, stream_);
+        std::vector<uint32_t> st;
+        rasterize(jobs, cams, b, ss, size, s.force_two_sided, st);
         hip_check(hipEventRecord(events_[1], stream_), "event");
+        hip_check(hipMemcpyAsync(dpairs, pairs.data(), pairs.size() * sizeof(ViewPair), hipMemcpyHostToDevice, stream_),
+                  "upload pairs");
+        launch_fill(reinterpret_cast<uint64_t*>(summaries), ncand * sizeof(ViewSummary) / 8, 0, stream_);
         launch_compare(dpairs, summaries, ncand, tiles, stream_);
         hip_check(hipEventRecord(events_[2], stream_), "event");
         launch_distance(dpairs, summaries, ncand, size, tiles, stream_);
         hip_check(hipEventRecord(events_[3], stream_), "event");
         hip_check(hipGetLastError(), "audit launch");
         std::vector<ViewSummary> sum(ncand);
-        std::vector<uint32_t> st(size_t(ntasks) * 2);
         hip_check(hipMemcpyAsync(sum.data(), summaries, ncand * sizeof(ViewSummary), hipMemcpyDeviceToHost, stream_),
                   "download summaries");
-        hip_check(hipMemcpyAsync(st.data(), status, st.size() * 4, hipMemcpyDeviceToHost, stream_), "download status");
         hip_check(hipStreamSynchronize(stream_), "audit");
         float t_r = 0, t_c = 0, t_d = 0;
         hip_check(hipEventElapsedTime(&t_r, events_[0], events_[1]), "elapsed");
@@ -427,42 +526,10 @@ void Audit::run_items(const DeviceMesh& reference, const std::vector<const Devic
         timing_.raster_ms += t_r;
         timing_.compare_ms += t_c;
         timing_.distance_ms += t_d;
-        timing_.raster_tasks += ntasks;
         timing_.distance_views += ncand;
-        for (uint32_t i = 0; i < ntasks; ++i) timing_.raster_triangles += tasks[i].faces;
-        timing_.mask_bytes += uint64_t(ntasks) * words * 8;
-
-        // The reference stops rasterizing at the first out-of-frame face
-        // (src/evaluate.cpp:159-160). Redraw such masks from faces before it.
-        std::vector<uint32_t> redo;
-        for (uint32_t i = 0; i < ntasks; ++i)
-            if (st[2 * i] != kNone) {
-                tasks[i].faces = st[2 * i];
-                redo.push_back(i);
-            }
-        if (!redo.empty()) {
-            std::vector<RasterTask> again;
-            for (uint32_t i : redo) {
-                again.push_back(tasks[i]);
-                launch_fill(tasks[i].mask, words, 0, stream_);
-            }
-            RasterTask* dredo = scratch_as<RasterTask>(7, again.size());
-            uint32_t faces = 0;
-            for (auto& t : again) faces = std::max(faces, t.faces);
-            hip_check(hipMemcpyAsync(dredo, again.data(), again.size() * sizeof(RasterTask), hipMemcpyHostToDevice, stream_),
-                      "upload redo");
-            launch_raster(dredo, uint32_t(again.size()), faces, stream_);
-            launch_fill(reinterpret_cast<uint64_t*>(summaries), ncand * sizeof(ViewSummary) / 8, 0, stream_);
-            launch_compare(dpairs, summaries, ncand, tiles, stream_);
-            launch_distance(dpairs, summaries, ncand, size, tiles, stream_);
-            hip_check(hipGetLastError(), "audit redo launch");
-            hip_check(hipMemcpyAsync(sum.data(), summaries, ncand * sizeof(ViewSummary), hipMemcpyDeviceToHost, stream_),
-                      "download summaries");
-            hip_check(hipStreamSynchronize(stream_), "audit redo");
-        }
         for (uint32_t i = 0; i < ncand; ++i) {
             const Item& it = all[begin + i];
-            const uint32_t ra = ref_slot[it.view], rc = nref + i;
+            const uint32_t ra = ref_job[ref_slot[it.view]], rc = cand_job[i];
             const bool clipped = st[2 * ra] != kNone || st[2 * ra + 1] || st[2 * rc] != kNone || st[2 * rc + 1];
             const ViewSummary& v = sum[i];
             ViewResult& r = out[begin + i];
@@ -493,38 +560,10 @@ bool Audit::raster_row_major(const DeviceMesh& m, const Bounds& b, const Camera&
     const uint32_t tiles = (size + kTile - 1) / kTile;
     const size_t words = size_t(tiles) * tiles;
     uint64_t* mask = scratch_as<uint64_t>(0, words);
-    uint32_t* status = scratch_as<uint32_t>(6, 2);
-    RasterTask* dtask = scratch_as<RasterTask>(4, 1);
-    RasterTask t{};
-    t.positions = m.positions;
-    t.indices = m.indices;
-    t.two_sided = m.two_sided;
-    t.mask = mask;
-    t.status = status;
-    t.camera = c;
-    t.center = b.center;
-    t.faces = m.faces;
-    t.size = size;
-    t.tiles = tiles;
-    t.ss = ss;
-    t.force_two = force_two;
-    std::vector<uint32_t> st{kNone, 0};
-    clipped = false;
-    for (int pass = 0; pass < 2; ++pass) {
-        hip_check(hipMemcpyAsync(dtask, &t, sizeof t, hipMemcpyHostToDevice, stream_), "upload task");
-        hip_check(hipMemcpyAsync(status, st.data(), 8, hipMemcpyHostToDevice, stream_), "upload status");
-        launch_fill(mask, words, 0, stream_);
-        launch_raster(dtask, 1, t.faces, stream_);
-        hip_check(hipGetLastError(), "raster launch");
-        std::vector<uint32_t> got(2);
-        hip_check(hipMemcpyAsync(got.data(), status, 8, hipMemcpyDeviceToHost, stream_), "download status");
-        hip_check(hipStreamSynchronize(stream_), "raster");
-        // The redraw from faces before the first out-of-frame face keeps the
-        // clipped flag of the full pass.
-        clipped = clipped || got[0] != kNone || got[1];
-        if (got[0] == kNone) break;
-        t.faces = got[0];
-    }
+    launch_fill(mask, words, 0, stream_);
+    std::vector<uint32_t> st;
+    rasterize({{&m, 0, mask}}, {c}, b, ss, size, force_two, st);
+    clipped = st[0] != kNone || st[1];
     std::vector<uint64_t> tiled(words);
     hip_check(hipMemcpy(tiled.data(), mask, words * 8, hipMemcpyDeviceToHost), "download mask");
     words_out.assign((size_t(size) * size + 63) / 64, 0);

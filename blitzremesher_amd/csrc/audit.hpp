@@ -47,7 +47,9 @@ struct HostMesh {
     std::vector<uint32_t> indices;
     std::vector<uint16_t> materials;
     std::vector<uint8_t> double_sided;
-    uint64_t digest[2] = {0, 0};  // over every stream, including normals/uv/colors/tangents
+    uint64_t digest[2] = {0, 0};      // streams compared by identical(): positions, normals, colors,
+                                      // indices, materials, sidedness
+    uint64_t pos_digest[2] = {0, 0};  // positions only: equal digests share one projection per view
 };
 
 class Audit;
@@ -59,6 +61,7 @@ struct DeviceMesh {
     uint8_t* two_sided = nullptr;  // per face; null when no face is two-sided
     uint32_t vertices = 0, faces = 0;
     uint64_t digest[2] = {0, 0};
+    uint64_t pos_digest[2] = {0, 0};
     uint64_t serial = 0;
     ~DeviceMesh();
 };
@@ -108,6 +111,15 @@ class Audit {
         uint64_t changed, total;
         bool clipped;
     };
+    struct RasterJob {
+        const DeviceMesh* mesh;
+        uint32_t view;
+        uint64_t* mask;  // tile-major, zeroed by the caller
+    };
+    // Rasterizes every job at one supersampling; status receives per job the
+    // first out-of-frame face (0xFFFFFFFF if none) and the behind-camera flag.
+    void rasterize(const std::vector<RasterJob>& jobs, const std::vector<Camera>& cams, const Bounds& b, uint8_t ss,
+                   uint32_t size, bool force_two, std::vector<uint32_t>& status);
     void run_items(const DeviceMesh& reference, const std::vector<const DeviceMesh*>& candidates,
                    const std::vector<Camera>& cams, const Bounds& b, const Settings& s, const std::vector<Item>& items,
                    std::vector<ViewResult>& out);

@@ -4,6 +4,12 @@
 // order as the reference CPU audit (src/evaluate.cpp:136-170) and the library
 // is compiled with -ffp-contract=off, so no multiply-add is fused. FP64
 // division on gfx950 (div_scale/div_fmas/div_fixup) is correctly rounded.
+//
+// Raster pipeline: project_kernel projects every vertex of a vertex set once
+// per view (a candidate that shares its vertex stream with the reference
+// shares the projection); raster_band_kernel then lets one 1024-thread
+// workgroup own a band of tile rows of one view's mask in LDS, walk a chunk
+// of the mesh's faces, OR coverage into LDS and flush the band once.
 #include <hip/hip_runtime.h>
 
 #include "audit_kernels.hpp"
@@ -11,12 +17,13 @@
 namespace blitz_hip {
 namespace {
 
-struct Projected {
-    double x, y, z;
-};
-
-// src/evaluate.cpp:138-141: x = dot(p - c, right) * scale * ss + size/2.
-__device__ __forceinline__ Projected project(const float4 p, const RasterTask& t) {
+// src/evaluate.cpp:138-141: x = dot(p - c, right) * scale * ss + size/2;
+// z = distance - dot(p - c, forward); scale = focal / z in perspective.
+__global__ __launch_bounds__(256) void project_kernel(const ProjectTask* __restrict__ tasks) {
+    const ProjectTask& t = tasks[blockIdx.y];
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= t.vertices) return;
+    const float4 p = t.positions[i];
     const double x = double(p.x) - double(t.center.x);
     const double y = double(p.y) - double(t.center.y);
     const double z0 = double(p.z) - double(t.center.z);
@@ -28,7 +35,8 @@ __device__ __forceinline__ Projected project(const float4 p, const RasterTask& t
     const double ssd = double(t.ss);
     const double along_r = x * double(c.right.x) + y * double(c.right.y) + z0 * double(c.right.z);
     const double along_u = x * double(c.up.x) + y * double(c.up.y) + z0 * double(c.up.z);
-    return {along_r * scale * ssd + half, along_u * scale * ssd + half, z};
+    t.xy[i] = make_double2(along_r * scale * ssd + half, along_u * scale * ssd + half);
+    if (t.z) t.z[i] = z;
 }
 
 // src/evaluate.cpp:21
@@ -56,24 +64,24 @@ struct Setup {
     int x0, x1, y0, y1;
 };
 
-// Triangle setup of src/evaluate.cpp:144-165. Returns 0 = nothing to draw,
-// 1 = draw, 2 = out of frame (the reference stops at the first such face).
+// Triangle setup of src/evaluate.cpp:144-165 on projected vertices.
+// Returns 0 = nothing to draw, 1 = draw, 2 = out of frame (the reference stops
+// at the first such face).
 __device__ __forceinline__ int setup_triangle(const RasterTask& t, uint32_t f, Setup& s, bool& behind) {
     const uint32_t ia = t.indices[3 * f], ib = t.indices[3 * f + 1], ic = t.indices[3 * f + 2];
-    const Projected a = project(t.positions[ia], t);
-    Projected d = project(t.positions[ib], t);
-    Projected e = project(t.positions[ic], t);
-    if (t.camera.perspective && (a.z <= 0 || d.z <= 0 || e.z <= 0)) {
+    if (t.z && (t.z[ia] <= 0 || t.z[ib] <= 0 || t.z[ic] <= 0)) {
         behind = true;
         return 0;
     }
-    double area = edge(a.x, a.y, d.x, d.y, e.x, e.y);
+    const double2 a = t.xy[ia];
+    double2 d = t.xy[ib], e = t.xy[ic];
+    const double area = edge(a.x, a.y, d.x, d.y, e.x, e.y);
     if (fabs(area) < 1e-16) return 0;
     const bool back = area < 0;
     const bool two = t.force_two || (t.two_sided && t.two_sided[f]);
     if (back && !two) return 0;
     if (back) {
-        const Projected tmp = d;
+        const double2 tmp = d;
         d = e;
         e = tmp;
     }
@@ -131,54 +139,89 @@ __device__ __forceinline__ Setup shuffle_setup(const Setup& s, int lane) {
 // whole wavefront, one sample per lane, with a ballot producing the word.
 constexpr int kLaneSamples = 24;
 
-__global__ __launch_bounds__(256) void raster_kernel(const RasterTask* __restrict__ tasks) {
-    const RasterTask& t = tasks[blockIdx.y];
-    const uint32_t f = blockIdx.x * blockDim.x + threadIdx.x;
+// One block per (task, band, face chunk). Blocks are mapped so that the
+// blocks of one task land on one XCD (hardware dispatches block w to XCD
+// w % 8): that XCD's L2 then holds the task's projected vertices.
+__global__ __launch_bounds__(kRasterThreads) void raster_band_kernel(const RasterTask* __restrict__ tasks,
+                                                                     RasterGrid grid) {
+    extern __shared__ unsigned long long band[];
+    const uint32_t total = grid.tasks * grid.bands * grid.chunks;
+    const uint32_t per_xcd = (total + 7) / 8;
+    const uint32_t work = (blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
+    if (work >= total) return;
+    const uint32_t chunk = work % grid.chunks;
+    const uint32_t band_id = (work / grid.chunks) % grid.bands;
+    const RasterTask& t = tasks[work / (grid.chunks * grid.bands)];
+    const uint32_t ty0 = band_id * grid.band_rows;
+    if (ty0 >= t.tiles) return;
+    const uint32_t ty1 = min(t.tiles, ty0 + grid.band_rows);
+    const uint32_t words = (ty1 - ty0) * t.tiles;
+    for (uint32_t i = threadIdx.x; i < words; i += blockDim.x) band[i] = 0;
+    __syncthreads();
+    const int ymin = int(ty0 * kTile), ymax = min(int(ty1 * kTile), int(t.size)) - 1;
+    const uint32_t per = (t.faces + grid.chunks - 1) / grid.chunks;
+    const uint32_t f0 = chunk * per, f1 = min(t.faces, f0 + per);
     const int lane = threadIdx.x & 63;
-    Setup s{};
-    bool large = false;
-    if (f < t.faces) {
-        bool behind = false;
-        const int kind = setup_triangle(t, f, s, behind);
-        if (behind) t.status[1] = 1u;
-        if (kind == 2) {
-            atomicMin(&t.status[0], f);
-        } else if (kind == 1) {
-            const int w = s.x1 - s.x0 + 1, h = s.y1 - s.y0 + 1;
-            if (w <= 8 && h <= 8 && w * h <= kLaneSamples) {
-                // At most 2x2 tiles: accumulate four words in registers.
-                uint64_t word[4] = {0, 0, 0, 0};
-                const int tx0 = s.x0 >> 3, ty0 = s.y0 >> 3;
-                for (int y = s.y0; y <= s.y1; ++y)
-                    for (int x = s.x0; x <= s.x1; ++x)
-                        if (covers(s, x, y))
-                            word[(((y >> 3) - ty0) << 1) | ((x >> 3) - tx0)] |= uint64_t(1) << (((y & 7) << 3) | (x & 7));
-                for (int k = 0; k < 4; ++k)
-                    if (word[k]) {
-                        const uint32_t tx = uint32_t(tx0 + (k & 1)), ty = uint32_t(ty0 + (k >> 1));
-                        atomicOr(reinterpret_cast<unsigned long long*>(&t.mask[size_t(ty) * t.tiles + tx]),
-                                 static_cast<unsigned long long>(word[k]));
-                    }
-            } else {
-                large = true;
+    for (uint32_t base = f0; base < f1; base += blockDim.x) {
+        const uint32_t f = base + threadIdx.x;
+        Setup s{};
+        bool large = false;
+        if (f < f1) {
+            bool behind = false;
+            const int kind = setup_triangle(t, f, s, behind);
+            if (behind) t.status[1] = 1u;
+            if (kind == 2) {
+                atomicMin(&t.status[0], f);
+            } else if (kind == 1 && s.y1 >= ymin && s.y0 <= ymax) {
+                const int w = s.x1 - s.x0 + 1, h = s.y1 - s.y0 + 1;
+                if (w <= 8 && h <= 8 && w * h <= kLaneSamples) {
+                    // At most 2x2 tiles: accumulate four words in registers.
+                    uint64_t word[4] = {0, 0, 0, 0};
+                    const int tx0 = s.x0 >> 3, ty = s.y0 >> 3;
+                    const int ya = max(s.y0, ymin), yb = min(s.y1, ymax);
+                    for (int y = ya; y <= yb; ++y)
+                        for (int x = s.x0; x <= s.x1; ++x)
+                            if (covers(s, x, y))
+                                word[(((y >> 3) - ty) << 1) | ((x >> 3) - tx0)] |= uint64_t(1)
+                                                                                  << (((y & 7) << 3) | (x & 7));
+                    for (int k = 0; k < 4; ++k)
+                        if (word[k]) {
+                            const uint32_t row = uint32_t(ty + (k >> 1)) - ty0;
+                            atomicOr(&band[row * t.tiles + uint32_t(tx0 + (k & 1))],
+                                     static_cast<unsigned long long>(word[k]));
+                        }
+                } else {
+                    large = true;
+                }
             }
         }
+        uint64_t pending = __ballot(large);
+        while (pending) {
+            const int src = __ffsll(static_cast<unsigned long long>(pending)) - 1;
+            pending &= pending - 1;
+            const Setup q = shuffle_setup(s, src);
+            const int lx = lane & 7, ly = lane >> 3;
+            const int tya = max(q.y0, ymin) >> 3, tyb = min(q.y1, ymax) >> 3;
+            for (int ty = tya; ty <= tyb; ++ty)
+                for (int tx = q.x0 >> 3; tx <= (q.x1 >> 3); ++tx) {
+                    const int x = (tx << 3) | lx, y = (ty << 3) | ly;
+                    const bool in = x >= q.x0 && x <= q.x1 && y >= q.y0 && y <= q.y1 && covers(q, x, y);
+                    const uint64_t word = __ballot(in);
+                    if (lane == 0 && word)
+                        atomicOr(&band[uint32_t(ty - int(ty0)) * t.tiles + uint32_t(tx)],
+                                 static_cast<unsigned long long>(word));
+                }
+        }
     }
-    uint64_t pending = __ballot(large);
-    while (pending) {
-        const int src = __ffsll(static_cast<unsigned long long>(pending)) - 1;
-        pending &= pending - 1;
-        const Setup q = shuffle_setup(s, src);
-        const int lx = lane & 7, ly = lane >> 3;
-        for (int ty = q.y0 >> 3; ty <= (q.y1 >> 3); ++ty)
-            for (int tx = q.x0 >> 3; tx <= (q.x1 >> 3); ++tx) {
-                const int x = (tx << 3) | lx, y = (ty << 3) | ly;
-                const bool in = x >= q.x0 && x <= q.x1 && y >= q.y0 && y <= q.y1 && covers(q, x, y);
-                const uint64_t word = __ballot(in);
-                if (lane == 0 && word)
-                    atomicOr(reinterpret_cast<unsigned long long*>(&t.mask[size_t(ty) * t.tiles + tx]),
-                             static_cast<unsigned long long>(word));
-            }
+    __syncthreads();
+    uint64_t* out = t.mask + size_t(ty0) * t.tiles;
+    if (grid.chunks == 1) {
+        for (uint32_t i = threadIdx.x; i < words; i += blockDim.x) out[i] = band[i];
+    } else {
+        for (uint32_t i = threadIdx.x; i < words; i += blockDim.x) {
+            const unsigned long long v = band[i];
+            if (v) atomicOr(reinterpret_cast<unsigned long long*>(out + i), v);
+        }
     }
 }
 
@@ -229,6 +272,44 @@ __device__ __forceinline__ bool needs_distance(const ViewSummary& s) {
     return s.changed && s.count_a && s.count_c;
 }
 
+// Views where more than 1/32 of the samples changed get a dense row pass;
+// sparser views search each needed row directly in the bit mask.
+__device__ __forceinline__ bool dense_rows(const ViewSummary& s, uint32_t size) {
+    return needs_distance(s) && s.changed > (unsigned long long)size * size / 32;
+}
+
+constexpr uint32_t kFar = 0xFFFFFFFFu;
+
+// Horizontal distance from column x to the nearest site in row y of a
+// tile-major mask, looking at most `reach` columns beyond x's own byte on
+// each side; kFar if there is none that close.
+__device__ __forceinline__ uint32_t nearest_in_row(const uint64_t* mask, uint32_t tiles, uint32_t x, uint32_t y,
+                                                   uint32_t reach) {
+    const uint64_t* row = mask + size_t(y >> 3) * tiles;
+    const uint32_t shift = (y & 7) << 3, tx = x >> 3, bit = x & 7, base = tx << 3;
+    const uint32_t byte = uint32_t(row[tx] >> shift) & 0xFFu;
+    const uint32_t lower = byte & ((2u << bit) - 1), upper = byte & ~((1u << bit) - 1);
+    uint32_t left = lower ? x - (base + 31 - __clz(lower)) : kFar;
+    uint32_t right = upper ? (base + __ffs(upper) - 1) - x : kFar;
+    if (left == kFar)
+        for (int t = int(tx) - 1; t >= 0 && x - (uint32_t(t) * 8 + 7) <= reach; --t) {
+            const uint32_t b = uint32_t(row[t] >> shift) & 0xFFu;
+            if (b) {
+                left = x - (uint32_t(t) * 8 + 31 - __clz(b));
+                break;
+            }
+        }
+    if (right == kFar)
+        for (uint32_t t = tx + 1; t < tiles && t * 8 - x <= reach; ++t) {
+            const uint32_t b = uint32_t(row[t] >> shift) & 0xFFu;
+            if (b) {
+                right = t * 8 + __ffs(b) - 1 - x;
+                break;
+            }
+        }
+    return min(left, right);
+}
+
 // Row pass of the separable EDT: for every sample, the horizontal distance to
 // the nearest site in its row (kNoSite if the row has none). One block per
 // row; each thread owns one byte (8 samples) of the row and block-wide
@@ -237,7 +318,7 @@ __global__ __launch_bounds__(1024) void row_distance_kernel(const ViewPair* __re
                                                             const ViewSummary* __restrict__ summaries) {
     const uint32_t job = blockIdx.y;  // view*2 + direction
     const ViewPair& p = pairs[job >> 1];
-    if (!needs_distance(summaries[job >> 1])) return;
+    if (!dense_rows(summaries[job >> 1], p.size)) return;
     const uint64_t* mask = (job & 1) ? p.a : p.c;
     uint16_t* out = (job & 1) ? p.dist_a : p.dist_c;
     const uint32_t y = blockIdx.x;
@@ -309,29 +390,40 @@ __global__ __launch_bounds__(256) void hausdorff_kernel(const ViewPair* __restri
     if (!bits) return;
     const uint32_t tx = uint32_t(i % p.tiles), ty = uint32_t(i / p.tiles);
     const uint32_t size = p.size;
+    const bool dense = dense_rows(s, size);
+    uint32_t x_of_query = 0;
+    // Rounded squared row distance of row r, or kFar. Sparse views search the
+    // mask only as far as a site could still improve `best`; the rounding of
+    // d^2 above 2^24 is covered by the +2 column margin.
+    auto row_term = [&](uint32_t r, uint32_t best, uint32_t kk) -> uint32_t {
+        uint32_t d;
+        if (dense) {
+            const uint16_t v = dist[size_t(r) * size + x_of_query];
+            d = v == kNoSite ? kFar : v;
+        } else {
+            const uint32_t room = best == kFar ? kFar : best - kk;
+            const uint32_t reach = room == kFar ? size : uint32_t(sqrtf(float(room))) + 2;
+            d = nearest_in_row(to, p.tiles, x_of_query, r, reach);
+        }
+        return d == kFar ? kFar : kk + rounded_square(d);
+    };
     uint32_t known = __hip_atomic_load(target, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     uint32_t local = 0;
     while (bits) {
         const int b = __ffsll(static_cast<unsigned long long>(bits)) - 1;
         bits &= bits - 1;
-        const uint32_t x = (tx << 3) | uint32_t(b & 7), y = (ty << 3) | uint32_t(b >> 3);
-        const uint16_t d0 = dist[size_t(y) * size + x];
-        uint32_t best = d0 == kNoSite ? 0xFFFFFFFFu : rounded_square(d0);
+        x_of_query = (tx << 3) | uint32_t(b & 7);
+        const uint32_t y = (ty << 3) | uint32_t(b >> 3);
+        uint32_t best = row_term(y, kFar, 0);
         const uint32_t floor_value = max(known, local);
         for (uint32_t k = 1; best > floor_value && uint64_t(k) * k < best; ++k) {
             const uint32_t kk = k * k;
             const bool up = y >= k, down = y + k < size;
             if (!up && !down) break;
-            if (up) {
-                const uint16_t d = dist[size_t(y - k) * size + x];
-                if (d != kNoSite) best = min(best, kk + rounded_square(d));
-            }
-            if (down) {
-                const uint16_t d = dist[size_t(y + k) * size + x];
-                if (d != kNoSite) best = min(best, kk + rounded_square(d));
-            }
+            if (up) best = min(best, row_term(y - k, best, kk));
+            if (down) best = min(best, row_term(y + k, best, kk));
         }
-        if (best > floor_value && best != 0xFFFFFFFFu) local = best;
+        if (best > floor_value && best != kFar) local = best;
     }
     if (local > known) atomicMax(target, local);
 }
@@ -344,9 +436,23 @@ void launch_fill(uint64_t* p, size_t n, uint64_t value, hipStream_t stream) {
     fill_kernel<<<dim3(uint32_t(blocks)), dim3(256), 0, stream>>>(p, n, value);
 }
 
-void launch_raster(const RasterTask* tasks, uint32_t count, uint32_t max_faces, hipStream_t stream) {
-    if (!count || !max_faces) return;
-    raster_kernel<<<dim3((max_faces + 255) / 256, count), dim3(256), 0, stream>>>(tasks);
+void launch_project(const ProjectTask* tasks, uint32_t count, uint32_t max_vertices, hipStream_t stream) {
+    if (!count || !max_vertices) return;
+    project_kernel<<<dim3((max_vertices + 255) / 256, count), dim3(256), 0, stream>>>(tasks);
+}
+
+void launch_raster(const RasterTask* tasks, const RasterGrid& grid, uint32_t tiles, hipStream_t stream) {
+    const uint32_t total = grid.tasks * grid.bands * grid.chunks;
+    if (!total) return;
+    const size_t lds = size_t(grid.band_rows) * tiles * sizeof(uint64_t);
+    static bool configured = false;
+    if (!configured) {
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&raster_band_kernel),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, int(kBandWords * sizeof(uint64_t)));
+        configured = true;
+    }
+    const uint32_t blocks = (total + 7) / 8 * 8;
+    raster_band_kernel<<<dim3(blocks), dim3(kRasterThreads), lds, stream>>>(tasks, grid);
 }
 
 void launch_compare(const ViewPair* pairs, ViewSummary* summaries, uint32_t count, uint32_t tiles,

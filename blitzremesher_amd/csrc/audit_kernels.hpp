@@ -9,8 +9,10 @@
 
 namespace blitz_hip {
 void launch_fill(uint64_t* p, size_t n, uint64_t value, hipStream_t stream);
-// tasks: device array of `count` tasks; grid covers max_faces per task.
-void launch_raster(const RasterTask* tasks, uint32_t count, uint32_t max_faces, hipStream_t stream);
+// tasks: device array of `count` projections; grid covers max_vertices each.
+void launch_project(const ProjectTask* tasks, uint32_t count, uint32_t max_vertices, hipStream_t stream);
+// tasks: device array of grid.tasks raster tasks sharing one raster side.
+void launch_raster(const RasterTask* tasks, const RasterGrid& grid, uint32_t tiles, hipStream_t stream);
 // pairs/summaries: device arrays of `count` views sharing one raster side.
 void launch_compare(const ViewPair* pairs, ViewSummary* summaries, uint32_t count, uint32_t tiles,
                     hipStream_t stream);

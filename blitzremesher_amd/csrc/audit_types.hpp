@@ -25,21 +25,40 @@ struct Bounds {
 // tile with lane == bit, so a ballot over the tile is the tile's word.
 constexpr uint32_t kTile = 8;
 
-// One raster job: one mesh seen from one camera at one supersampling.
+// Projection of one vertex set from one camera at one supersampling.
+struct ProjectTask {
+    const float4* positions;  // xyz + pad, one per vertex
+    double2* xy;              // projected x, y per vertex (FP64, reference order)
+    double* z;                // view depth per vertex; null for orthographic views
+    Camera camera;
+    F3 center;
+    uint32_t vertices;
+    uint32_t size;            // raster side in samples
+    uint8_t ss;
+};
+
+// One raster job: one mesh's faces over one projected vertex set.
 struct RasterTask {
-    const float4* positions;   // xyz + pad, one per vertex
+    const double2* xy;
+    const double* z;           // null for orthographic views
     const uint32_t* indices;   // 3 per face
     const uint8_t* two_sided;  // per-face flag or null (all one-sided)
     uint64_t* mask;            // tile-major words, tiles*tiles
     uint32_t* status;          // [0] first out-of-frame face, [1] behind-camera flag
-    Camera camera;
-    F3 center;
     uint32_t faces;            // faces rasterized: [0, faces)
     uint32_t size;             // raster side in samples
     uint32_t tiles;            // ceil(size / 8)
-    uint8_t ss;
     uint8_t force_two;
 };
+
+// Launch shape of the band raster: every task is split into `bands` bands of
+// `band_rows` tile rows (one LDS-resident band per workgroup) and its faces
+// into `chunks` contiguous ranges.
+struct RasterGrid {
+    uint32_t tasks, bands, chunks, band_rows;
+};
+constexpr uint32_t kRasterThreads = 1024;
+constexpr uint32_t kBandWords = 16384;  // 128 KiB of LDS per workgroup
 
 // Per-view comparison of a reference mask A and a candidate mask C.
 struct ViewPair {

@@ -123,6 +123,10 @@ HostMesh host_mesh(const blitz_mesh* m) {
     d.bytes(m->double_sided, m->double_sided_count);
     h.digest[0] = d.a;
     h.digest[1] = d.b;
+    Digest pd;
+    digest_stream(pd, m->positions, 12);
+    h.pos_digest[0] = pd.a;
+    h.pos_digest[1] = pd.b;
     return h;
 }
 
@@ -195,6 +199,14 @@ int blitz_hip_available(int32_t device) {
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) != hipSuccess) return 0;
     return std::strncmp(prop.gcnArchName, "gfx950", 6) == 0;
+}
+
+blitz_status blitz_hip_device_synchronize(int32_t device) {
+    return guarded(nullptr, 0, [&] {
+        if (!blitz_hip_available(device)) throw Unavailable("no gfx950 HIP device at the requested index");
+        DeviceGuard guard(device);
+        hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+    });
 }
 
 blitz_status blitz_hip_audit_options_init(blitz_hip_audit_options* o, size_t n) {

@@ -1,0 +1,490 @@
+// Edge-graph proposal scorer kernels for gfx950.
+//
+// Network (the build's scorer; the reference's production policy is an
+// independent-action MLP, src/neural/action.hpp:6-10, and its legacy encoder a
+// 24-feature, width-64 graph network, src/neural/inference.cu:117-150):
+//   h0 = 32 vertex features;
+//   h(l+1) = ReLU(W_l [h_l ; mean_{w in N(v)} h_l(w)] + b_l), l = 0..2, width 128
+//            (mean aggregation as src/neural/inference.cu:18-30);
+//   for each directed edge a = (u -> v): g = ReLU(W_e [h3(u) ; h3(v)] + b_e),
+//   out(a) = W_o g + b_o: logit and a clamped placement offset.
+//
+// Two numeric modes:
+//  * FP32: deterministic. Aggregation sums neighbours in CSR order; every
+//    linear layer is the reference's compensated kernel
+//    (src/neural/inference.cu:75-108): eight strided Kahan partials per output
+//    combined in fixed order with the bias. A CPU restatement reproduces it
+//    bit for bit, so FP32 rankings are reproducible.
+//  * BF16: the fast path. Activations and weights in BF16, products on the
+//    gfx950 matrix cores (v_mfma_f32_16x16x32_bf16) with FP32 accumulation,
+//    aggregation in FP32; fused per 64-row tile (gather -> LDS -> MFMA ->
+//    bias/ReLU epilogue).
+//
+// Vertex features (32, FP64 internally, stored FP32); p = (P - centre) / diameter:
+//   0-2 p | 3-5 unit area-weighted normal | 6-15 area-normalised plane quadric
+//   (a00 a01 a02 a03 a11 a12 a13 a22 a23 a33, the Quadric layout of
+//   src/simplify.cpp:10-32) | 16 curvature: n . (mean neighbour p - p) / mean
+//   edge, clamped to [-1,1] | 17 valence / 16 | 18 on boundary | 19 UV stream
+//   present | 20 min(64 * mean edge, 4) | 21 incident faces / 8 | 22 boundary
+//   edges / 4 | 23-25 linear colour (white) | 26-27 uv / 16 | 28 normal stream
+//   present | 29 colour stream present | 30 min(sqrt(area) / mean edge, 4) |
+//   31 constant 1.
+#include <hip/hip_runtime.h>
+
+#include <hipcub/device/device_radix_sort.hpp>
+
+#include "scorer_kernels.hpp"
+
+namespace blitz_hip {
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+// ------------------------------------------------------------------ features
+
+__device__ __forceinline__ void normalised(const FeatureInput& in, uint32_t i, double* p) {
+    const float4 q = in.positions[i];
+    p[0] = (double(q.x) - in.cx) / in.diameter;
+    p[1] = (double(q.y) - in.cy) / in.diameter;
+    p[2] = (double(q.z) - in.cz) / in.diameter;
+}
+
+__global__ __launch_bounds__(256) void features_kernel(FeatureInput in) {
+    const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= in.vertices) return;
+    double p[3];
+    normalised(in, v, p);
+    double normal[3] = {0, 0, 0}, quad[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0}, total = 0;
+    const uint32_t fb = in.face_offsets[v], fe = in.face_offsets[v + 1];
+    for (uint32_t k = fb; k < fe; ++k) {
+        const uint32_t f = in.faces[k];
+        double a[3], b[3], c[3];
+        normalised(in, in.indices[3 * f], a);
+        normalised(in, in.indices[3 * f + 1], b);
+        normalised(in, in.indices[3 * f + 2], c);
+        const double u[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
+        const double w[3] = {c[0] - a[0], c[1] - a[1], c[2] - a[2]};
+        const double cr[3] = {u[1] * w[2] - u[2] * w[1], u[2] * w[0] - u[0] * w[2], u[0] * w[1] - u[1] * w[0]};
+        const double area = sqrt(cr[0] * cr[0] + cr[1] * cr[1] + cr[2] * cr[2]);
+        double plane[4] = {0, 0, 0, 0};
+        if (area > 0) {
+            plane[0] = cr[0] / area;
+            plane[1] = cr[1] / area;
+            plane[2] = cr[2] / area;
+        }
+        plane[3] = -(plane[0] * a[0] + plane[1] * a[1] + plane[2] * a[2]);
+        int k2 = 0;
+        for (int i = 0; i < 4; ++i)
+            for (int j = i; j < 4; ++j) quad[k2++] += plane[i] * plane[j] * area;
+        normal[0] += cr[0];
+        normal[1] += cr[1];
+        normal[2] += cr[2];
+        total += area;
+    }
+    const double nl = sqrt(normal[0] * normal[0] + normal[1] * normal[1] + normal[2] * normal[2]);
+    double unit[3] = {0, 0, 0};
+    if (nl > 0) {
+        unit[0] = normal[0] / nl;
+        unit[1] = normal[1] / nl;
+        unit[2] = normal[2] / nl;
+    }
+    const uint32_t nb = in.offsets[v], ne = in.offsets[v + 1];
+    double sum[3] = {0, 0, 0}, edge_sum = 0;
+    for (uint32_t k = nb; k < ne; ++k) {
+        double q[3];
+        normalised(in, in.neighbors[k], q);
+        sum[0] += q[0];
+        sum[1] += q[1];
+        sum[2] += q[2];
+        const double d[3] = {q[0] - p[0], q[1] - p[1], q[2] - p[2]};
+        edge_sum += sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+    }
+    const uint32_t deg = ne - nb;
+    const double degc = double(deg > 0 ? deg : 1u);
+    const double mean_edge = edge_sum / degc;
+    const double lap[3] = {sum[0] / degc - p[0], sum[1] / degc - p[1], sum[2] / degc - p[2]};
+    const double curvature =
+        fmin(fmax((unit[0] * lap[0] + unit[1] * lap[1] + unit[2] * lap[2]) / fmax(mean_edge, 1e-30), -1.0), 1.0);
+    const uint32_t bcount = in.boundary[v];
+    const double denom = fmax(total, 1e-30);
+    float* x = in.out + size_t(v) * kFeatures;
+    x[0] = float(p[0]);
+    x[1] = float(p[1]);
+    x[2] = float(p[2]);
+    x[3] = float(unit[0]);
+    x[4] = float(unit[1]);
+    x[5] = float(unit[2]);
+    for (int i = 0; i < 10; ++i) x[6 + i] = float(quad[i] / denom);
+    x[16] = float(curvature);
+    x[17] = float(double(deg) / 16.0);
+    x[18] = bcount ? 1.f : 0.f;
+    x[19] = 0.f;
+    x[20] = float(fmin(mean_edge * 64.0, 4.0));
+    x[21] = float(double(fe - fb) / 8.0);
+    x[22] = float(double(bcount) / 4.0);
+    x[23] = x[24] = x[25] = 1.f;
+    x[26] = x[27] = 0.f;
+    x[28] = x[29] = 0.f;
+    x[30] = float(fmin(sqrt(total) / fmax(mean_edge, 1e-30), 4.0));
+    x[31] = 1.f;
+}
+
+// ------------------------------------------------------------ FP32 (exact)
+
+// z[v] = [h[v], mean of h over N(v)], neighbours summed in CSR order.
+__global__ __launch_bounds__(256) void aggregate_fp32(const float* __restrict__ h, GraphView g, uint32_t width,
+                                                      float* __restrict__ z) {
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= size_t(g.vertices) * width) return;
+    const uint32_t v = uint32_t(i / width), c = uint32_t(i % width);
+    const uint32_t b = g.offsets[v], e = g.offsets[v + 1];
+    float sum = 0.f;
+    for (uint32_t k = b; k < e; ++k) sum = sum + h[size_t(g.neighbors[k]) * width + c];
+    z[size_t(v) * 2 * width + c] = h[i];
+    z[size_t(v) * 2 * width + width + c] = sum / float(e - b > 0 ? e - b : 1u);
+}
+
+// Compensated linear layer (src/neural/inference.cu:75-108). Eight lanes per
+// output; with `src` set, row r's input is [h[src[r]] ; h[dst[r]]].
+__global__ __launch_bounds__(256) void compensated_linear(const float* __restrict__ input, const uint32_t* src,
+                                                          const uint32_t* dst, const float* __restrict__ weights,
+                                                          const float* __restrict__ bias, float* __restrict__ output,
+                                                          uint32_t rows, uint32_t in, uint32_t out, bool relu) {
+    const size_t i = (size_t(blockIdx.x) * blockDim.x + threadIdx.x) / 8;
+    const uint32_t lane = threadIdx.x & 7;
+    const bool live = i < size_t(rows) * out;
+    const size_t row = live ? i / out : 0;
+    const uint32_t channel = live ? uint32_t(i % out) : 0;
+    const float* w = weights + size_t(channel) * in;
+    const uint32_t half = in / 2;
+    float sum = 0.f, correction = 0.f;
+    if (live)
+        for (uint32_t k = lane; k < in; k += 8) {
+            float x;
+            if (src)
+                x = k < half ? input[size_t(src[row]) * half + k] : input[size_t(dst[row]) * half + (k - half)];
+            else
+                x = input[row * in + k];
+            const float term = __fmul_rn(x, w[k]) - correction;
+            const float next = __fadd_rn(sum, term);
+            correction = __fsub_rn(__fsub_rn(next, sum), term);
+            sum = next;
+        }
+    float total = live ? bias[channel] : 0.f, error = 0.f;
+    for (int j = 0; j < 8; ++j) {
+        const float part = __shfl(sum, j, 8), residual = __shfl(correction, j, 8);
+        if (lane == 0) {
+            float term = __fsub_rn(part, error), next = __fadd_rn(total, term);
+            error = __fsub_rn(__fsub_rn(next, total), term);
+            total = next;
+            term = __fsub_rn(-residual, error);
+            next = __fadd_rn(total, term);
+            error = __fsub_rn(__fsub_rn(next, total), term);
+            total = next;
+        }
+    }
+    if (live && lane == 0) output[i] = relu && total < 0 ? 0.f : total;
+}
+
+// ------------------------------------------------------------- BF16 / MFMA
+
+// v_mfma_f32_16x16x32_bf16 operand layout (wave64): lane l supplies A row
+// (l % 16), k = 8 * (l / 16) .. +7, and B column (l % 16), same k; it receives
+// D rows 4 * (l / 16) .. +3 of column (l % 16).
+constexpr uint32_t kTileRows = 64;
+constexpr uint32_t kPad = 8;  // bf16 elements of row padding in LDS
+
+__device__ __forceinline__ floatx4 mfma(bf16x8 a, bf16x8 b, floatx4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+// 64 x 128 tile product of a [64][K] LDS tile and a [128][K] LDS weight tile:
+// wave w computes rows 16w..16w+15, eight 16-wide column blocks.
+template <uint32_t K>
+__device__ __forceinline__ void tile_product(const __bf16* z, const __bf16* w, floatx4 (&acc)[8]) {
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint32_t r = wave * 16 + (lane & 15), kq = (lane >> 4) * 8;
+    for (int nb = 0; nb < 8; ++nb) acc[nb] = floatx4{0, 0, 0, 0};
+#pragma unroll 2
+    for (uint32_t k0 = 0; k0 < K; k0 += 32) {
+        const bf16x8 a = *reinterpret_cast<const bf16x8*>(z + r * (K + kPad) + k0 + kq);
+#pragma unroll
+        for (int nb = 0; nb < 8; ++nb) {
+            const bf16x8 b = *reinterpret_cast<const bf16x8*>(w + (nb * 16 + (lane & 15)) * (K + kPad) + k0 + kq);
+            acc[nb] = mfma(a, b, acc[nb]);
+        }
+    }
+}
+
+template <uint32_t K>
+__device__ __forceinline__ void load_weights_bf16(const __bf16* __restrict__ src, __bf16* w) {
+    // [128][K] row-major into LDS rows of K + kPad.
+    for (uint32_t i = threadIdx.x; i < kHidden * K / 8; i += blockDim.x) {
+        const uint32_t row = i / (K / 8), col = (i % (K / 8)) * 8;
+        *reinterpret_cast<bf16x8*>(w + row * (K + kPad) + col) = *reinterpret_cast<const bf16x8*>(src + row * K + col);
+    }
+}
+
+// One message-passing layer: z = [h ; mean h(N)] (K = 2 * width), then
+// ReLU(z W^T + b) into h_out (BF16). Persistent blocks keep W in LDS.
+template <uint32_t K>
+__global__ __launch_bounds__(256) void mp_layer_bf16(const __bf16* __restrict__ h, GraphView g,
+                                                     const __bf16* __restrict__ wq, const float* __restrict__ bias,
+                                                     __bf16* __restrict__ out) {
+    constexpr uint32_t width = K / 2;
+    extern __shared__ __bf16 smem[];
+    __bf16* w = smem;
+    __bf16* z = smem + kHidden * (K + kPad);
+    load_weights_bf16<K>(wq, w);
+    const uint32_t tiles = (g.vertices + kTileRows - 1) / kTileRows;
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (uint32_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
+        __syncthreads();
+        // Gather: thread t builds 8 channels of one row at a time.
+        for (uint32_t i = threadIdx.x; i < kTileRows * (width / 8); i += blockDim.x) {
+            const uint32_t r = i / (width / 8), c = (i % (width / 8)) * 8;
+            const uint32_t v = tile * kTileRows + r;
+            bf16x8 self{}, mean{};
+            if (v < g.vertices) {
+                self = *reinterpret_cast<const bf16x8*>(h + size_t(v) * width + c);
+                float acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+                const uint32_t b = g.offsets[v], e = g.offsets[v + 1];
+                for (uint32_t k = b; k < e; ++k) {
+                    const bf16x8 q = *reinterpret_cast<const bf16x8*>(h + size_t(g.neighbors[k]) * width + c);
+                    for (int j = 0; j < 8; ++j) acc[j] += float(q[j]);
+                }
+                const float inv = 1.f / float(e - b > 0 ? e - b : 1u);
+                for (int j = 0; j < 8; ++j) mean[j] = __bf16(acc[j] * inv);
+            }
+            *reinterpret_cast<bf16x8*>(z + r * (K + kPad) + c) = self;
+            *reinterpret_cast<bf16x8*>(z + r * (K + kPad) + width + c) = mean;
+        }
+        __syncthreads();
+        floatx4 acc[8];
+        tile_product<K>(z, w, acc);
+        for (int nb = 0; nb < 8; ++nb) {
+            const uint32_t col = nb * 16 + (lane & 15);
+            const float b = bias[col];
+            for (int i = 0; i < 4; ++i) {
+                const uint32_t v = tile * kTileRows + wave * 16 + (lane >> 4) * 4 + i;
+                if (v < g.vertices) out[size_t(v) * kHidden + col] = __bf16(fmaxf(acc[nb][i] + b, 0.f));
+            }
+        }
+    }
+}
+
+// Edge MLP over actions [first, first + count): z = [h(src) ; h(dst)],
+// g = ReLU(z We^T + be) (MFMA), out = g Wo^T + bo (FP32, four outputs).
+__global__ __launch_bounds__(256) void edge_mlp_bf16(const __bf16* __restrict__ h, GraphView g,
+                                                     const __bf16* __restrict__ wq, const float* __restrict__ bias,
+                                                     const float* __restrict__ wo, const float* __restrict__ bo,
+                                                     float* __restrict__ out) {
+    constexpr uint32_t K = kEdgeIn;
+    extern __shared__ __bf16 smem[];
+    __bf16* w = smem;
+    __bf16* z = smem + kHidden * (K + kPad);
+    float* hidden = reinterpret_cast<float*>(z + kTileRows * (K + kPad));  // [64][kHidden + 1]
+    float* head = hidden + kTileRows * (kHidden + 1);                       // [4][kHidden] + 4
+    load_weights_bf16<K>(wq, w);
+    for (uint32_t i = threadIdx.x; i < kOutputs * kHidden; i += blockDim.x) head[i] = wo[i];
+    if (threadIdx.x < kOutputs) head[kOutputs * kHidden + threadIdx.x] = bo[threadIdx.x];
+    const uint32_t tiles = (g.actions + kTileRows - 1) / kTileRows;
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (uint32_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < kTileRows * (K / 8); i += blockDim.x) {
+            const uint32_t r = i / (K / 8), c = (i % (K / 8)) * 8;
+            const uint32_t a = tile * kTileRows + r;
+            bf16x8 q{};
+            if (a < g.actions) {
+                const uint32_t v = c < kHidden ? g.src[a] : g.neighbors[a];
+                q = *reinterpret_cast<const bf16x8*>(h + size_t(v) * kHidden + (c % kHidden));
+            }
+            *reinterpret_cast<bf16x8*>(z + r * (K + kPad) + c) = q;
+        }
+        __syncthreads();
+        floatx4 acc[8];
+        tile_product<K>(z, w, acc);
+        for (int nb = 0; nb < 8; ++nb) {
+            const uint32_t col = nb * 16 + (lane & 15);
+            const float b = bias[col];
+            for (int i = 0; i < 4; ++i) {
+                const uint32_t r = wave * 16 + (lane >> 4) * 4 + i;
+                hidden[r * (kHidden + 1) + col] = fmaxf(acc[nb][i] + b, 0.f);
+            }
+        }
+        __syncthreads();
+        {
+            const uint32_t r = threadIdx.x / kOutputs, o = threadIdx.x % kOutputs;
+            const uint32_t a = tile * kTileRows + r;
+            float s = 0.f;
+            for (uint32_t j = 0; j < kHidden; ++j) s += hidden[r * (kHidden + 1) + j] * head[o * kHidden + j];
+            if (a < g.actions) out[size_t(a) * kOutputs + o] = s + head[kOutputs * kHidden + o];
+        }
+    }
+}
+
+__global__ void to_bf16(const float* __restrict__ x, __bf16* __restrict__ y, size_t n) {
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x)
+        y[i] = __bf16(x[i]);
+}
+
+// ----------------------------------------------------------------- ranking
+
+// Scores of src/neural/action_gpu.cu:809-844: learned logit, shortest edge
+// (-|dst - src| / scale) or current plane (-sum over faces around src of
+// area-weighted squared plane distance of dst, FP64).
+__global__ void score_kernel(ScoreInput in) {
+    const uint32_t a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= in.actions) return;
+    float score = 0.f;
+    if (in.mode == 0) {
+        score = in.outputs[size_t(a) * kOutputs];
+    } else {
+        const uint32_t u = in.src[a], v = in.dst[a];
+        const float4 pu = in.positions[u], pv = in.positions[v];
+        if (in.mode == 1) {
+            const double dx = (double(pv.x) - pu.x) / in.scale, dy = (double(pv.y) - pu.y) / in.scale,
+                         dz = (double(pv.z) - pu.z) / in.scale;
+            score = -float(sqrt(dx * dx + dy * dy + dz * dz));
+        } else {
+            const double scale = fmax(1e-20, in.scale);
+            double cost = 0;
+            for (uint32_t k = in.face_offsets[u]; k < in.face_offsets[u + 1]; ++k) {
+                const uint32_t f = in.faces[k];
+                const float4 p = in.positions[in.indices[3 * f]], q = in.positions[in.indices[3 * f + 1]],
+                             r = in.positions[in.indices[3 * f + 2]];
+                const double e1[3] = {double(q.x) - p.x, double(q.y) - p.y, double(q.z) - p.z};
+                const double e2[3] = {double(r.x) - p.x, double(r.y) - p.y, double(r.z) - p.z};
+                const double n[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2],
+                                     e1[0] * e2[1] - e1[1] * e2[0]};
+                const double weight = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+                if (!(weight > 0)) continue;
+                const double d = (n[0] * (double(pv.x) - p.x) + n[1] * (double(pv.y) - p.y) +
+                                  n[2] * (double(pv.z) - p.z)) / weight / scale;
+                cost += d * d * weight / (scale * scale);
+            }
+            score = float(-cost);
+        }
+    }
+    // -0.0 sorts as +0.0 so equal scores keep ascending action order.
+    in.keys[a] = score + 0.f;
+    in.order[a] = a;
+}
+
+// Placement of src/neural/action_gpu.cu:1157-1188 without grid snapping:
+// midpoint + clamp(offset, -1, 1) * edge length, FP64.
+__global__ void decode_kernel(DecodeInput in) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= in.count) return;
+    const uint32_t a = in.selection ? in.selection[i] : i;
+    const float4 p = in.positions[in.src[a]], q = in.positions[in.dst[a]];
+    const double d[3] = {double(p.x) - q.x, double(p.y) - q.y, double(p.z) - q.z};
+    const double length = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+    const double mid[3] = {(double(p.x) + q.x) * .5, (double(p.y) + q.y) * .5, (double(p.z) + q.z) * .5};
+    for (int k = 0; k < 3; ++k) {
+        const double o = fmin(1.0, fmax(-1.0, double(in.outputs[size_t(a) * kOutputs + 1 + k])));
+        in.out[3 * size_t(i) + k] = mid[k] + o * length;
+    }
+}
+
+}  // namespace
+
+void launch_features(const FeatureInput& in, hipStream_t stream) {
+    if (!in.vertices) return;
+    features_kernel<<<dim3((in.vertices + 255) / 256), dim3(256), 0, stream>>>(in);
+}
+
+void launch_forward_fp32(const float* x, GraphView g, const float* weights, float* h_a, float* h_b, float* z,
+                         float* out, hipStream_t stream) {
+    const float* h = x;
+    uint32_t width = kFeatures;
+    float* targets[3] = {h_a, h_b, h_a};
+    for (int l = 0; l < 3; ++l) {
+        const size_t n = size_t(g.vertices) * width;
+        if (n) aggregate_fp32<<<dim3(uint32_t((n + 255) / 256)), dim3(256), 0, stream>>>(h, g, width, z);
+        const size_t threads = size_t(g.vertices) * kHidden * 8;
+        if (threads)
+            compensated_linear<<<dim3(uint32_t((threads + 255) / 256)), dim3(256), 0, stream>>>(
+                z, nullptr, nullptr, weights + layer_offset(l), weights + layer_offset(l) + kHidden * kLayerIn[l],
+                targets[l], g.vertices, kLayerIn[l], kHidden, true);
+        h = targets[l];
+        width = kHidden;
+    }
+    const size_t t1 = size_t(g.actions) * kHidden * 8;
+    if (t1)
+        compensated_linear<<<dim3(uint32_t((t1 + 255) / 256)), dim3(256), 0, stream>>>(
+            h, g.src, g.neighbors, weights + layer_offset(3), weights + layer_offset(3) + kHidden * kEdgeIn, z,
+            g.actions, kEdgeIn, kHidden, true);
+    const size_t t2 = size_t(g.actions) * kOutputs * 8;
+    if (t2)
+        compensated_linear<<<dim3(uint32_t((t2 + 255) / 256)), dim3(256), 0, stream>>>(
+            z, nullptr, nullptr, weights + layer_offset(4), weights + layer_offset(4) + kOutputs * kHidden, out,
+            g.actions, kHidden, kOutputs, false);
+}
+
+size_t forward_bf16_lds(uint32_t k) {
+    return size_t(kHidden + kTileRows) * (k + kPad) * 2 +
+           (k == kEdgeIn ? (kTileRows * (kHidden + 1) + kOutputs * kHidden + kOutputs) * 4 : 0);
+}
+
+void launch_convert_bf16(const float* x, __bf16* y, size_t n, hipStream_t stream) {
+    if (!n) return;
+    to_bf16<<<dim3(uint32_t(std::min<size_t>((n + 255) / 256, 4096))), dim3(256), 0, stream>>>(x, y, n);
+}
+
+void launch_forward_bf16(const __bf16* x, GraphView g, const __bf16* wq, const float* weights, __bf16* h_a,
+                         __bf16* h_b, float* out, uint32_t blocks, hipStream_t stream) {
+    static bool configured = false;
+    if (!configured) {
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&mp_layer_bf16<2 * kFeatures>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, int(forward_bf16_lds(2 * kFeatures)));
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&mp_layer_bf16<2 * kHidden>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, int(forward_bf16_lds(2 * kHidden)));
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&edge_mlp_bf16),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, int(forward_bf16_lds(kEdgeIn)));
+        configured = true;
+    }
+    if (!g.vertices) return;
+    const uint32_t vtiles = (g.vertices + kTileRows - 1) / kTileRows;
+    const uint32_t vb = std::min(blocks, vtiles);
+    mp_layer_bf16<2 * kFeatures><<<dim3(vb), dim3(256), forward_bf16_lds(2 * kFeatures), stream>>>(
+        x, g, wq + bf16_offset(0), weights + layer_offset(0) + kHidden * kLayerIn[0], h_a);
+    mp_layer_bf16<2 * kHidden><<<dim3(vb), dim3(256), forward_bf16_lds(2 * kHidden), stream>>>(
+        h_a, g, wq + bf16_offset(1), weights + layer_offset(1) + kHidden * kLayerIn[1], h_b);
+    mp_layer_bf16<2 * kHidden><<<dim3(vb), dim3(256), forward_bf16_lds(2 * kHidden), stream>>>(
+        h_b, g, wq + bf16_offset(2), weights + layer_offset(2) + kHidden * kLayerIn[2], h_a);
+    if (!g.actions) return;
+    const uint32_t atiles = (g.actions + kTileRows - 1) / kTileRows;
+    edge_mlp_bf16<<<dim3(std::min(blocks, atiles)), dim3(256), forward_bf16_lds(kEdgeIn), stream>>>(
+        h_a, g, wq + bf16_offset(3), weights + layer_offset(3) + kHidden * kEdgeIn, weights + layer_offset(4),
+        weights + layer_offset(4) + kOutputs * kHidden, out);
+}
+
+void launch_scores(const ScoreInput& in, hipStream_t stream) {
+    if (!in.actions) return;
+    score_kernel<<<dim3((in.actions + 255) / 256), dim3(256), 0, stream>>>(in);
+}
+
+size_t sort_workspace(uint32_t n) {
+    size_t bytes = 0;
+    (void)hipcub::DeviceRadixSort::SortPairsDescending(nullptr, bytes, static_cast<const float*>(nullptr),
+                                                       static_cast<float*>(nullptr),
+                                                       static_cast<const uint32_t*>(nullptr),
+                                                       static_cast<uint32_t*>(nullptr), int(n));
+    return bytes;
+}
+
+hipError_t launch_sort(void* temp, size_t temp_bytes, const float* keys_in, float* keys_out, const uint32_t* order_in,
+                       uint32_t* order_out, uint32_t n, hipStream_t stream) {
+    return hipcub::DeviceRadixSort::SortPairsDescending(temp, temp_bytes, keys_in, keys_out, order_in, order_out,
+                                                        int(n), 0, 32, stream);
+}
+
+void launch_decode(const DecodeInput& in, hipStream_t stream) {
+    if (!in.count) return;
+    decode_kernel<<<dim3((in.count + 255) / 256), dim3(256), 0, stream>>>(in);
+}
+
+}  // namespace blitz_hip

@@ -121,6 +121,9 @@ typedef struct blitz_hip_audit_timing {
 uint32_t blitz_hip_api_version(void);
 /* Nonzero when a HIP device with gfx950 kernels is usable. */
 int blitz_hip_available(int32_t device);
+/* Waits for all work on the device (hipDeviceSynchronize); restores the
+ * caller's current device. */
+blitz_status blitz_hip_device_synchronize(int32_t device);
 
 blitz_status blitz_hip_audit_options_init(blitz_hip_audit_options*, size_t);
 /* Defaults of blitz::EvalSettings (include/blitz/evaluate.hpp:24-33). */

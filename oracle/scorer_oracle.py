@@ -1,0 +1,247 @@
+"""CPU oracle of the edge-graph proposal scorer.
+
+TEST INFRASTRUCTURE, NOT PRODUCT CODE: only tests/, __graft_entry__.smoke()
+and bench.py's cpu_baseline leg import this module, as the checker.
+
+The network is the build's edge-graph scorer (the reference's production
+policy is a different, independent-action MLP; see the header comment of
+blitzremesher_amd/csrc/scorer_kernels.hip). Its oracle follows the structure
+of the reference's FP64 numeric oracle (src/neural/numeric.cpp:52-122): every
+layer is recomputed in float64 from float32 weights and inputs, with
+mean-aggregated neighbours (src/neural/inference.cu:18-30). `forward_fp32`
+restates, operation for operation, the library's deterministic FP32 mode,
+whose linear layers follow the reference's compensated kernel
+(src/neural/inference.cu:75-108): eight strided Kahan partial sums per output
+combined in fixed order with the bias.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+FEATURES = 32
+HIDDEN = 128
+LAYERS = 3
+EDGE_IN = 2 * HIDDEN
+OUTPUTS = 4  # logit, placement xyz
+
+# (out, in) of every linear layer, in weight-buffer order.
+SHAPES = [(HIDDEN, 2 * FEATURES), (HIDDEN, 2 * HIDDEN), (HIDDEN, 2 * HIDDEN), (HIDDEN, EDGE_IN),
+          (OUTPUTS, HIDDEN)]
+
+
+def weight_count() -> int:
+    return sum(o * (i + 1) for o, i in SHAPES)
+
+
+def split_weights(w: np.ndarray) -> list[tuple[np.ndarray, np.ndarray]]:
+    """Row-major [out, in] weights then bias per layer (src/neural/internal.hpp:42-43)."""
+    out, at = [], 0
+    for o, i in SHAPES:
+        W = w[at:at + o * i].reshape(o, i)
+        at += o * i
+        b = w[at:at + o]
+        at += o
+        out.append((W, b))
+    assert at == w.size
+    return out
+
+
+# ---------------------------------------------------------------- topology --
+
+def topology(indices: np.ndarray, vertices: int):
+    """Undirected neighbour CSR (sorted, unique), vertex-face CSR (faces in
+    index order) and the boundary-edge flags used by the features."""
+    f = indices.astype(np.int64)
+    e = np.concatenate([f[:, [0, 1]], f[:, [1, 2]], f[:, [2, 0]]])
+    e = e[e[:, 0] != e[:, 1]]
+    key = np.sort(e, axis=1)
+    uniq, counts = np.unique(key, axis=0, return_counts=True)
+    both = np.concatenate([uniq, uniq[:, ::-1]])
+    order = np.lexsort((both[:, 1], both[:, 0]))
+    both = both[order]
+    offsets = np.zeros(vertices + 1, np.int64)
+    np.add.at(offsets, both[:, 0] + 1, 1)
+    offsets = np.cumsum(offsets)
+    neighbors = both[:, 1]
+    boundary_edges = uniq[counts == 1]
+    vf_v = f.reshape(-1)
+    vf_f = np.repeat(np.arange(len(f)), 3)
+    order = np.lexsort((vf_f, vf_v))
+    face_offsets = np.zeros(vertices + 1, np.int64)
+    np.add.at(face_offsets, vf_v + 1, 1)
+    face_offsets = np.cumsum(face_offsets)
+    return offsets, neighbors, face_offsets, vf_f[order], boundary_edges
+
+
+# ---------------------------------------------------------------- features --
+
+def features(positions: np.ndarray, indices: np.ndarray, center, diameter: float) -> np.ndarray:
+    """32 per-vertex features in float64 (cast to float32 by the caller);
+    layout documented in blitzremesher_amd/csrc/scorer_kernels.hip."""
+    n = positions.shape[0]
+    offsets, neighbors, face_offsets, faces, boundary_edges = topology(indices, n)
+    p = (positions.astype(np.float64) - np.asarray(center, np.float64)) / diameter
+    f = indices.astype(np.int64)
+    a, b, c = p[f[:, 0]], p[f[:, 1]], p[f[:, 2]]
+    cr = np.cross(b - a, c - a)
+    area = np.sqrt((cr * cr).sum(1))
+    safe = np.where(area > 0, area, 1.0)
+    nf = np.where(area[:, None] > 0, cr / safe[:, None], 0.0)
+    d = -(nf * a).sum(1)
+    plane = np.concatenate([nf, d[:, None]], 1)
+    iu = [(0, 0), (0, 1), (0, 2), (0, 3), (1, 1), (1, 2), (1, 3), (2, 2), (2, 3), (3, 3)]
+    qf = np.stack([plane[:, i] * plane[:, j] * area for i, j in iu], 1)
+    deg_f = np.diff(face_offsets)
+    normal = np.zeros((n, 3))
+    quad = np.zeros((n, 10))
+    tot = np.zeros(n)
+    # Fixed order: incident faces in ascending face index.
+    for slot in range(int(deg_f.max(initial=0))):
+        has = deg_f > slot
+        v = np.nonzero(has)[0]
+        fi = faces[face_offsets[v] + slot]
+        normal[v] += cr[fi]
+        quad[v] += qf[fi]
+        tot[v] += area[fi]
+    nl = np.sqrt((normal * normal).sum(1))
+    unit = np.where(nl[:, None] > 0, normal / np.where(nl > 0, nl, 1)[:, None], 0.0)
+    deg = np.diff(offsets)
+    nbr_sum = np.zeros((n, 3))
+    edge_sum = np.zeros(n)
+    for slot in range(int(deg.max(initial=0))):
+        v = np.nonzero(deg > slot)[0]
+        w = neighbors[offsets[v] + slot]
+        nbr_sum[v] += p[w]
+        dd = p[w] - p[v]
+        edge_sum[v] += np.sqrt((dd * dd).sum(1))
+    degc = np.maximum(deg, 1)
+    mean_edge = edge_sum / degc
+    lap = nbr_sum / degc[:, None] - p
+    curvature = np.clip((unit * lap).sum(1) / np.maximum(mean_edge, 1e-30), -1.0, 1.0)
+    bcount = np.zeros(n)
+    np.add.at(bcount, boundary_edges[:, 0], 1)
+    np.add.at(bcount, boundary_edges[:, 1], 1)
+    x = np.zeros((n, FEATURES))
+    x[:, 0:3] = p
+    x[:, 3:6] = unit
+    x[:, 6:16] = quad / np.maximum(tot, 1e-30)[:, None]
+    x[:, 16] = curvature
+    x[:, 17] = deg / 16.0
+    x[:, 18] = (bcount > 0).astype(np.float64)
+    x[:, 19] = 0.0  # UV stream present
+    x[:, 20] = np.minimum(mean_edge * 64.0, 4.0)
+    x[:, 21] = deg_f / 8.0
+    x[:, 22] = bcount / 4.0
+    x[:, 23:26] = 1.0  # linear colour (white when absent)
+    x[:, 26:28] = 0.0  # uv / 16
+    x[:, 28] = 0.0  # normal stream present
+    x[:, 29] = 0.0  # colour stream present
+    x[:, 30] = np.minimum(np.sqrt(tot) / np.maximum(mean_edge, 1e-30), 4.0)
+    x[:, 31] = 1.0
+    return x
+
+
+# ----------------------------------------------------------------- network --
+
+def actions(offsets: np.ndarray, neighbors: np.ndarray) -> np.ndarray:
+    """Directed contractions, one per CSR entry: (source, destination)."""
+    src = np.repeat(np.arange(len(offsets) - 1), np.diff(offsets))
+    return np.stack([src, neighbors], 1)
+
+
+def forward_fp64(x: np.ndarray, offsets, neighbors, acts, weights: np.ndarray) -> np.ndarray:
+    layers = [(W.astype(np.float64), b.astype(np.float64)) for W, b in split_weights(weights)]
+    h = x.astype(np.float64)
+    deg = np.maximum(np.diff(offsets), 1)
+    src = np.repeat(np.arange(len(offsets) - 1), np.diff(offsets))
+    for W, b in layers[:LAYERS]:
+        agg = np.zeros_like(h)
+        np.add.at(agg, src, h[neighbors])
+        z = np.concatenate([h, agg / deg[:, None]], 1)
+        h = np.maximum(z @ W.T + b, 0.0)
+    W1, b1 = layers[3]
+    W2, b2 = layers[4]
+    z = np.concatenate([h[acts[:, 0]], h[acts[:, 1]]], 1)
+    g = np.maximum(z @ W1.T + b1, 0.0)
+    return g @ W2.T + b2
+
+
+def _compensated_linear(x: np.ndarray, W: np.ndarray, b: np.ndarray, relu: bool) -> np.ndarray:
+    f32 = np.float32
+    rows, k_in = x.shape
+    out = W.shape[0]
+    sums = np.zeros((8, rows, out), f32)
+    corrs = np.zeros((8, rows, out), f32)
+    for lane in range(8):
+        s = np.zeros((rows, out), f32)
+        c = np.zeros((rows, out), f32)
+        for k in range(lane, k_in, 8):
+            term = (x[:, k][:, None] * W[None, :, k]).astype(f32) - c
+            nxt = s + term
+            c = (nxt - s) - term
+            s = nxt
+        sums[lane], corrs[lane] = s, c
+    total = np.broadcast_to(b[None, :], (rows, out)).astype(f32).copy()
+    err = np.zeros((rows, out), f32)
+    for j in range(8):
+        term = sums[j] - err
+        nxt = total + term
+        err = (nxt - total) - term
+        total = nxt
+        term = -corrs[j] - err
+        nxt = total + term
+        err = (nxt - total) - term
+        total = nxt
+    if relu:
+        total = np.where(total < 0, f32(0), total)
+    return total
+
+
+def forward_fp32(x: np.ndarray, offsets, neighbors, acts, weights: np.ndarray) -> np.ndarray:
+    """Bit-exact restatement of the library's deterministic FP32 mode."""
+    f32 = np.float32
+    layers = split_weights(weights.astype(f32))
+    h = x.astype(f32)
+    deg = np.diff(offsets)
+    n = len(deg)
+    for W, b in layers[:LAYERS]:
+        agg = np.zeros_like(h)
+        for slot in range(int(deg.max(initial=0))):
+            v = np.nonzero(deg > slot)[0]
+            agg[v] = agg[v] + h[neighbors[offsets[v] + slot]]
+        agg = agg / np.maximum(deg, 1).astype(f32)[:, None]
+        z = np.concatenate([h, agg], 1)
+        h = _compensated_linear(z, W, b, True)
+        assert h.shape[0] == n
+    W1, b1 = layers[3]
+    W2, b2 = layers[4]
+    z = np.concatenate([h[acts[:, 0]], h[acts[:, 1]]], 1)
+    g = _compensated_linear(z, W1, b1, True)
+    return _compensated_linear(g, W2, b2, False)
+
+
+def rank(logits: np.ndarray) -> np.ndarray:
+    """Descending score order, ties by ascending action index."""
+    return np.lexsort((np.arange(len(logits)), -logits.astype(np.float64)))
+
+
+def decode_placements(positions: np.ndarray, acts: np.ndarray, outputs: np.ndarray) -> np.ndarray:
+    """Midpoint plus clamped offset times edge length (src/neural/action_gpu.cu:1157-1188)."""
+    p = positions.astype(np.float64)
+    a, b = p[acts[:, 0]], p[acts[:, 1]]
+    mid = (a + b) * 0.5
+    length = np.sqrt(((a - b) ** 2).sum(1))
+    return mid + np.clip(outputs[:, 1:4].astype(np.float64), -1, 1) * length[:, None]
+
+
+def he_weights(seed: int) -> np.ndarray:
+    """He-normal N(0, 2/fan_in) weights, zero biases, seeded."""
+    import sys
+    from pathlib import Path
+    sys.path.insert(0, str(Path(__file__).resolve().parent.parent))
+    from blitzremesher_amd.meshes import normal
+    parts = []
+    for k, (o, i) in enumerate(SHAPES):
+        parts.append((normal(seed, 1000 + k, o * i) * np.sqrt(2.0 / i)).astype(np.float32))
+        parts.append(np.zeros(o, np.float32))
+    return np.concatenate(parts)
