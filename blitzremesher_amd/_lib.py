@@ -125,6 +125,19 @@ PROTOTYPES = {
     "blitz_hip_audit_last_timing": (C.c_int, [_P, C.POINTER(AuditTiming)]),
     "blitz_hip_reduce_reuse": (C.c_int, [C.POINTER(BlitzMesh), C.c_size_t, C.POINTER(C.c_uint32), C.c_size_t,
                                          C.POINTER(C.c_size_t), *_ERR]),
+    "blitz_hip_scorer_weight_count": (C.c_size_t, []),
+    "blitz_hip_scorer_create": (C.c_int, [C.c_int32, C.POINTER(C.c_float), C.c_size_t, C.POINTER(_P), *_ERR]),
+    "blitz_hip_scorer_destroy": (None, [_P]),
+    "blitz_hip_scorer_set_weights": (C.c_int, [_P, C.POINTER(C.c_float), C.c_size_t, *_ERR]),
+    "blitz_hip_graph_create": (C.c_int, [_P, C.POINTER(BlitzMesh), C.POINTER(Bounds), C.POINTER(_P), *_ERR]),
+    "blitz_hip_graph_destroy": (None, [_P]),
+    "blitz_hip_graph_info": (C.c_int, [_P, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
+    "blitz_hip_graph_features": (C.c_int, [_P, _P, C.POINTER(C.c_float), C.c_size_t]),
+    "blitz_hip_graph_actions": (C.c_int, [_P, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.c_size_t]),
+    "blitz_hip_scorer_forward": (C.c_int, [_P, _P, C.c_uint8, C.POINTER(C.c_float), C.c_size_t, *_ERR]),
+    "blitz_hip_scorer_rank": (C.c_int, [_P, _P, C.c_uint8, C.POINTER(C.c_uint32), C.c_size_t, *_ERR]),
+    "blitz_hip_scorer_decode": (C.c_int, [_P, _P, C.POINTER(C.c_uint32), C.c_size_t, C.POINTER(C.c_double), *_ERR]),
+    "blitz_hip_scorer_last_forward_ms": (C.c_int, [_P, C.POINTER(C.c_double)]),
 }
 
 _lib: C.CDLL | None = None

@@ -13,6 +13,7 @@
 
 #include "../../include/blitz_hip.h"
 #include "audit.hpp"
+#include "scorer.hpp"
 #include "simplify.hpp"
 
 using namespace blitz_hip;
@@ -23,6 +24,14 @@ struct blitz_hip_audit {
 struct blitz_hip_device_mesh {
     blitz_hip_audit* owner;
     std::unique_ptr<DeviceMesh> mesh;
+};
+struct blitz_hip_scorer {
+    std::unique_ptr<Scorer> impl;
+};
+struct blitz_hip_graph {
+    blitz_hip_scorer* owner;
+    std::unique_ptr<Graph> graph;
+    std::vector<uint32_t> src, dst;  // host copy of the actions
 };
 
 namespace {
@@ -427,5 +436,125 @@ blitz_status blitz_hip_reduce_reuse(const blitz_mesh* m, size_t target, uint32_t
         std::memcpy(out, idx.data(), idx.size() * 4);
         *count = idx.size();
     });
+}
+}
+
+namespace {
+Scorer& scorer_of(blitz_hip_scorer* s) {
+    if (!s || !s->impl) throw std::invalid_argument("null scorer");
+    return *s->impl;
+}
+Graph& graph_of(const blitz_hip_scorer* s, blitz_hip_graph* g) {
+    if (!g || !g->graph) throw std::invalid_argument("null graph");
+    if (g->owner != s) throw std::invalid_argument("graph belongs to another scorer");
+    return *g->graph;
+}
+}  // namespace
+
+extern "C" {
+size_t blitz_hip_scorer_weight_count(void) { return kWeightCount; }
+
+blitz_status blitz_hip_scorer_create(int32_t device, const float* weights, size_t count, blitz_hip_scorer** out,
+                                     char* error, size_t capacity) {
+    if (out) *out = nullptr;
+    return guarded(error, capacity, [&] {
+        if (!out) throw std::invalid_argument("null scorer destination");
+        if (!blitz_hip_available(device)) throw Unavailable("no gfx950 HIP device at the requested index");
+        auto s = std::make_unique<blitz_hip_scorer>();
+        s->impl = std::make_unique<Scorer>(device, weights, count);
+        *out = s.release();
+    });
+}
+
+void blitz_hip_scorer_destroy(blitz_hip_scorer* s) { delete s; }
+
+blitz_status blitz_hip_scorer_set_weights(blitz_hip_scorer* s, const float* weights, size_t count, char* error,
+                                          size_t capacity) {
+    return guarded(error, capacity, [&] { scorer_of(s).set_weights(weights, count); });
+}
+
+blitz_status blitz_hip_graph_create(blitz_hip_scorer* s, const blitz_mesh* m, const blitz_hip_bounds* b,
+                                    blitz_hip_graph** out, char* error, size_t capacity) {
+    if (out) *out = nullptr;
+    return guarded(error, capacity, [&] {
+        if (!out) throw std::invalid_argument("null graph destination");
+        Scorer& scorer = scorer_of(s);
+        const HostMesh h = host_mesh(m);
+        auto g = std::make_unique<blitz_hip_graph>();
+        g->owner = s;
+        g->graph = scorer.graph(h, bounds_from(b));
+        const Topology t = build_topology(h.indices.data(), h.indices.size() / 3, h.positions.size() / 3);
+        g->src = t.src;
+        g->dst = t.neighbors;
+        *out = g.release();
+    });
+}
+
+void blitz_hip_graph_destroy(blitz_hip_graph* g) {
+    if (!g) return;
+    if (g->owner && g->owner->impl) {
+        DeviceGuard guard(g->owner->impl->device());
+        (void)hipStreamSynchronize(g->owner->impl->stream());
+        g->graph.reset();
+    }
+    delete g;
+}
+
+blitz_status blitz_hip_graph_info(const blitz_hip_graph* g, uint32_t* vertices, uint32_t* actions) {
+    if (!g || !g->graph || !vertices || !actions) return BLITZ_INVALID_ARGUMENT;
+    *vertices = g->graph->vertices;
+    *actions = g->graph->actions;
+    return BLITZ_OK;
+}
+
+blitz_status blitz_hip_graph_features(blitz_hip_scorer* s, const blitz_hip_graph* g, float* out, size_t capacity) {
+    return guarded(nullptr, 0, [&] {
+        Graph& graph = graph_of(s, const_cast<blitz_hip_graph*>(g));
+        if (!out || capacity < size_t(graph.vertices) * kFeatures) throw std::invalid_argument("feature capacity");
+        scorer_of(s).download_features(graph, out);
+    });
+}
+
+blitz_status blitz_hip_graph_actions(const blitz_hip_graph* g, uint32_t* src, uint32_t* dst, size_t capacity) {
+    if (!g || !g->graph || !src || !dst || capacity < g->src.size()) return BLITZ_INVALID_ARGUMENT;
+    std::memcpy(src, g->src.data(), g->src.size() * 4);
+    std::memcpy(dst, g->dst.data(), g->dst.size() * 4);
+    return BLITZ_OK;
+}
+
+blitz_status blitz_hip_scorer_forward(blitz_hip_scorer* s, blitz_hip_graph* g, uint8_t precision, float* outputs,
+                                      size_t capacity, char* error, size_t error_capacity) {
+    return guarded(error, error_capacity, [&] {
+        Scorer& scorer = scorer_of(s);
+        Graph& graph = graph_of(s, g);
+        if (precision > 1) throw std::invalid_argument("unknown scorer precision");
+        if (outputs && capacity < size_t(graph.actions) * kOutputs) throw std::invalid_argument("output capacity");
+        scorer.forward(graph, Precision(precision));
+        if (outputs) scorer.download_outputs(graph, outputs);
+    });
+}
+
+blitz_status blitz_hip_scorer_rank(blitz_hip_scorer* s, blitz_hip_graph* g, uint8_t ranking, uint32_t* order,
+                                   size_t capacity, char* error, size_t error_capacity) {
+    return guarded(error, error_capacity, [&] {
+        Graph& graph = graph_of(s, g);
+        if (ranking > 4) throw std::invalid_argument("unknown ranking");
+        if (!order || capacity < graph.actions) throw std::invalid_argument("order capacity");
+        scorer_of(s).rank(graph, Ranking(ranking), order);
+    });
+}
+
+blitz_status blitz_hip_scorer_decode(blitz_hip_scorer* s, blitz_hip_graph* g, const uint32_t* actions, size_t count,
+                                     double* placements, char* error, size_t capacity) {
+    return guarded(error, capacity, [&] {
+        if (!placements || (!actions && count)) throw std::invalid_argument("null decode buffers");
+        scorer_of(s).decode(graph_of(s, g), actions, count, placements);
+    });
+}
+
+blitz_status blitz_hip_scorer_last_forward_ms(const blitz_hip_scorer* s, double* ms) {
+    if (!s || !s->impl || !ms) return BLITZ_INVALID_ARGUMENT;
+    *ms = s->impl->last_forward_ms();
+    return BLITZ_OK;
 }
 }

@@ -1,0 +1,273 @@
+// Host controller of the edge-graph proposal scorer (see scorer.hpp).
+#include "scorer.hpp"
+
+#include <algorithm>
+#include <cstring>
+#include <stdexcept>
+
+#include "scorer_kernels.hpp"
+
+namespace blitz_hip {
+
+DeviceBuffer::~DeviceBuffer() { (void)hipFree(p); }
+
+void DeviceBuffer::resize(size_t n) {
+    if (n <= bytes && p) return;
+    hip_check(hipFree(p), "hipFree");
+    p = nullptr;
+    bytes = 0;
+    hip_check(hipMalloc(&p, std::max<size_t>(n, 16)), "hipMalloc");
+    bytes = std::max<size_t>(n, 16);
+}
+
+GraphView Graph::view() const {
+    return {offsets.as<uint32_t>(), neighbors.as<uint32_t>(), src.as<uint32_t>(), reverse.as<uint32_t>(), vertices,
+            actions};
+}
+
+// Undirected neighbour CSR with ascending rows (degenerate face edges
+// dropped), the directed action of every entry and its reverse, the
+// vertex -> face-corner CSR in ascending face order and per-vertex counts of
+// boundary edges (edges used by exactly one face corner pair).
+Topology build_topology(const uint32_t* idx, size_t nf, size_t nv) {
+    Topology t;
+    std::vector<uint64_t> edges;
+    edges.reserve(nf * 3);
+    for (size_t f = 0; f < nf; ++f)
+        for (int j = 0; j < 3; ++j) {
+            const uint32_t a = idx[3 * f + j], b = idx[3 * f + (j + 1) % 3];
+            if (a == b) continue;
+            edges.push_back(a < b ? (uint64_t(a) << 32) | b : (uint64_t(b) << 32) | a);
+        }
+    std::sort(edges.begin(), edges.end());
+    t.boundary.assign(nv, 0);
+    std::vector<uint64_t> unique;
+    unique.reserve(edges.size());
+    for (size_t i = 0; i < edges.size();) {
+        size_t j = i + 1;
+        while (j < edges.size() && edges[j] == edges[i]) ++j;
+        unique.push_back(edges[i]);
+        if (j - i == 1) {
+            ++t.boundary[edges[i] >> 32];
+            ++t.boundary[uint32_t(edges[i])];
+        }
+        i = j;
+    }
+    t.offsets.assign(nv + 1, 0);
+    for (uint64_t e : unique) {
+        ++t.offsets[(e >> 32) + 1];
+        ++t.offsets[uint32_t(e) + 1];
+    }
+    for (size_t v = 0; v < nv; ++v) t.offsets[v + 1] += t.offsets[v];
+    t.neighbors.resize(t.offsets[nv]);
+    std::vector<uint32_t> cursor(t.offsets.begin(), t.offsets.end() - 1);
+    for (uint64_t e : unique) {
+        const uint32_t a = uint32_t(e >> 32), b = uint32_t(e);
+        t.neighbors[cursor[a]++] = b;
+        t.neighbors[cursor[b]++] = a;
+    }
+    t.src.resize(t.neighbors.size());
+    t.reverse.resize(t.neighbors.size());
+    for (size_t v = 0; v < nv; ++v) {
+        std::sort(t.neighbors.begin() + t.offsets[v], t.neighbors.begin() + t.offsets[v + 1]);
+        for (uint32_t k = t.offsets[v]; k < t.offsets[v + 1]; ++k) t.src[k] = uint32_t(v);
+    }
+    for (size_t k = 0; k < t.neighbors.size(); ++k) {
+        const uint32_t w = t.neighbors[k], v = t.src[k];
+        const auto begin = t.neighbors.begin() + t.offsets[w], end = t.neighbors.begin() + t.offsets[w + 1];
+        t.reverse[k] = uint32_t(std::lower_bound(begin, end, v) - t.neighbors.begin());
+    }
+    t.face_offsets.assign(nv + 1, 0);
+    for (size_t i = 0; i < nf * 3; ++i) ++t.face_offsets[idx[i] + 1];
+    for (size_t v = 0; v < nv; ++v) t.face_offsets[v + 1] += t.face_offsets[v];
+    t.faces.resize(nf * 3);
+    std::vector<uint32_t> fc(t.face_offsets.begin(), t.face_offsets.end() - 1);
+    for (size_t f = 0; f < nf; ++f)
+        for (int j = 0; j < 3; ++j) t.faces[fc[idx[3 * f + j]]++] = uint32_t(f);
+    return t;
+}
+
+namespace {
+template <class T> void upload(DeviceBuffer& d, const std::vector<T>& v, hipStream_t s) {
+    d.resize(v.size() * sizeof(T));
+    if (!v.empty()) hip_check(hipMemcpyAsync(d.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s), "upload");
+}
+}  // namespace
+
+Scorer::Scorer(int device, const float* weights, size_t count) : device_(device) {
+    DeviceGuard guard(device_);
+    hip_check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
+    hip_check(hipEventCreate(&begin_), "hipEventCreate");
+    hip_check(hipEventCreate(&end_), "hipEventCreate");
+    hipDeviceProp_t prop;
+    hip_check(hipGetDeviceProperties(&prop, device_), "hipGetDeviceProperties");
+    blocks_ = uint32_t(std::max(1, prop.multiProcessorCount)) * 2;
+    set_weights(weights, count);
+}
+
+Scorer::~Scorer() {
+    DeviceGuard guard(device_);
+    (void)hipStreamSynchronize(stream_);
+    (void)hipEventDestroy(begin_);
+    (void)hipEventDestroy(end_);
+    (void)hipStreamDestroy(stream_);
+}
+
+void Scorer::set_weights(const float* w, size_t count) {
+    if (!w || count != kWeightCount) throw std::invalid_argument("scorer weight count mismatch");
+    for (size_t i = 0; i < count; ++i)
+        if (!std::isfinite(w[i])) throw std::invalid_argument("nonfinite scorer weight");
+    DeviceGuard guard(device_);
+    weights_.resize(count * 4);
+    hip_check(hipMemcpyAsync(weights_.p, w, count * 4, hipMemcpyHostToDevice, stream_), "upload weights");
+    // BF16 copy of the four MFMA layer matrices (biases stay FP32).
+    std::vector<float> mats(kBf16WeightCount);
+    for (int l = 0; l < 4; ++l)
+        std::memcpy(mats.data() + bf16_offset(l), w + layer_offset(l), size_t(kHidden) * kLayerIn[l] * 4);
+    DeviceBuffer staging;
+    staging.resize(mats.size() * 4);
+    hip_check(hipMemcpyAsync(staging.p, mats.data(), mats.size() * 4, hipMemcpyHostToDevice, stream_), "upload");
+    weights_bf16_.resize(mats.size() * 2);
+    launch_convert_bf16(staging.as<float>(), weights_bf16_.as<__bf16>(), mats.size(), stream_);
+    hip_check(hipStreamSynchronize(stream_), "weights");
+}
+
+std::unique_ptr<Graph> Scorer::graph(const HostMesh& m, const Bounds& b) {
+    if (!(b.radius > 0) || !std::isfinite(b.radius)) throw std::invalid_argument("invalid bounds");
+    DeviceGuard guard(device_);
+    auto g = std::make_unique<Graph>();
+    const size_t nv = m.positions.size() / 3, nf = m.indices.size() / 3;
+    const Topology t = build_topology(m.indices.data(), nf, nv);
+    g->vertices = uint32_t(nv);
+    g->faces = uint32_t(nf);
+    g->actions = uint32_t(t.neighbors.size());
+    g->diameter = b.radius * 2;
+    std::vector<float> pos(nv * 4, 0.f);
+    for (size_t i = 0; i < nv; ++i) std::memcpy(&pos[4 * i], &m.positions[3 * i], 12);
+    upload(g->positions, pos, stream_);
+    upload(g->indices, m.indices, stream_);
+    upload(g->offsets, t.offsets, stream_);
+    upload(g->neighbors, t.neighbors, stream_);
+    upload(g->src, t.src, stream_);
+    upload(g->reverse, t.reverse, stream_);
+    upload(g->face_offsets, t.face_offsets, stream_);
+    upload(g->faces_of, t.faces, stream_);
+    upload(g->boundary, t.boundary, stream_);
+    g->features.resize(nv * kFeatures * 4);
+    FeatureInput in{};
+    in.positions = g->positions.as<float4>();
+    in.indices = g->indices.as<uint32_t>();
+    in.face_offsets = g->face_offsets.as<uint32_t>();
+    in.faces = g->faces_of.as<uint32_t>();
+    in.offsets = g->offsets.as<uint32_t>();
+    in.neighbors = g->neighbors.as<uint32_t>();
+    in.boundary = g->boundary.as<uint32_t>();
+    in.cx = b.center.x;
+    in.cy = b.center.y;
+    in.cz = b.center.z;
+    in.diameter = g->diameter;
+    in.vertices = g->vertices;
+    in.out = g->features.as<float>();
+    launch_features(in, stream_);
+    g->features_bf16.resize(nv * kFeatures * 2);
+    launch_convert_bf16(g->features.as<float>(), g->features_bf16.as<__bf16>(), nv * kFeatures, stream_);
+    g->h_a.resize(nv * kHidden * 4);
+    g->h_b.resize(nv * kHidden * 4);
+    g->z.resize(std::max(nv * 2 * kHidden, size_t(g->actions) * kHidden) * 4);
+    g->hq_a.resize(nv * kHidden * 2);
+    g->hq_b.resize(nv * kHidden * 2);
+    g->outputs.resize(size_t(g->actions) * kOutputs * 4);
+    hip_check(hipGetLastError(), "features launch");
+    hip_check(hipStreamSynchronize(stream_), "graph");
+    return g;
+}
+
+void Scorer::forward(Graph& g, Precision p) {
+    DeviceGuard guard(device_);
+    hip_check(hipEventRecord(begin_, stream_), "event");
+    if (p == Precision::Fp32)
+        launch_forward_fp32(g.features.as<float>(), g.view(), weights_.as<float>(), g.h_a.as<float>(), g.h_b.as<float>(),
+                            g.z.as<float>(), g.outputs.as<float>(), stream_);
+    else
+        launch_forward_bf16(g.features_bf16.as<__bf16>(), g.view(), weights_bf16_.as<__bf16>(), weights_.as<float>(),
+                            g.hq_a.as<__bf16>(), g.hq_b.as<__bf16>(), g.outputs.as<float>(), blocks_, stream_);
+    hip_check(hipEventRecord(end_, stream_), "event");
+    hip_check(hipGetLastError(), "forward launch");
+    hip_check(hipStreamSynchronize(stream_), "forward");
+    float ms = 0;
+    hip_check(hipEventElapsedTime(&ms, begin_, end_), "elapsed");
+    forward_ms_ = ms;
+    g.has_outputs = true;
+}
+
+void Scorer::download_outputs(const Graph& g, float* out) {
+    if (!g.has_outputs) throw std::invalid_argument("forward() has not run on this graph");
+    DeviceGuard guard(device_);
+    hip_check(hipMemcpy(out, g.outputs.p, size_t(g.actions) * kOutputs * 4, hipMemcpyDeviceToHost), "download");
+}
+
+void Scorer::download_features(const Graph& g, float* out) {
+    DeviceGuard guard(device_);
+    hip_check(hipMemcpy(out, g.features.p, size_t(g.vertices) * kFeatures * 4, hipMemcpyDeviceToHost), "download");
+}
+
+void Scorer::rank(Graph& g, Ranking r, uint32_t* order) {
+    if (r == Ranking::Shuffled) throw Unavailable("shuffled ranking is not implemented by the HIP scorer");
+    if (r == Ranking::Learned && !g.has_outputs) throw std::invalid_argument("forward() has not run on this graph");
+    DeviceGuard guard(device_);
+    const uint32_t n = g.actions;
+    g.keys.resize(size_t(n) * 4);
+    g.order.resize(size_t(n) * 4);
+    g.keys_out.resize(size_t(n) * 4);
+    g.order_out.resize(size_t(n) * 4);
+    ScoreInput in{};
+    in.outputs = g.outputs.as<float>();
+    in.positions = g.positions.as<float4>();
+    in.indices = g.indices.as<uint32_t>();
+    in.face_offsets = g.face_offsets.as<uint32_t>();
+    in.faces = g.faces_of.as<uint32_t>();
+    in.src = g.src.as<uint32_t>();
+    in.dst = g.neighbors.as<uint32_t>();
+    in.scale = g.diameter;
+    in.actions = n;
+    in.mode = r == Ranking::Learned ? 0u : r == Ranking::ShortestEdge ? 1u : r == Ranking::CurrentPlane ? 2u : 3u;
+    in.keys = g.keys.as<float>();
+    in.order = g.order.as<uint32_t>();
+    launch_scores(in, stream_);
+    const size_t temp = sort_workspace(n);
+    g.sort_temp.resize(temp);
+    hip_check(launch_sort(g.sort_temp.p, temp, g.keys.as<float>(), g.keys_out.as<float>(), g.order.as<uint32_t>(),
+                          g.order_out.as<uint32_t>(), n, stream_),
+              "radix sort");
+    hip_check(hipGetLastError(), "rank launch");
+    hip_check(hipMemcpyAsync(order, g.order_out.p, size_t(n) * 4, hipMemcpyDeviceToHost, stream_), "download order");
+    hip_check(hipStreamSynchronize(stream_), "rank");
+}
+
+void Scorer::decode(Graph& g, const uint32_t* selection, size_t count, double* out) {
+    if (!g.has_outputs) throw std::invalid_argument("forward() has not run on this graph");
+    for (size_t i = 0; i < count; ++i)
+        if (selection && selection[i] >= g.actions) throw std::invalid_argument("action outside graph");
+    if (!selection && count > g.actions) throw std::invalid_argument("action count outside graph");
+    DeviceGuard guard(device_);
+    DeviceBuffer sel, res;
+    if (selection) {
+        sel.resize(count * 4);
+        hip_check(hipMemcpyAsync(sel.p, selection, count * 4, hipMemcpyHostToDevice, stream_), "upload selection");
+    }
+    res.resize(count * 3 * 8);
+    DecodeInput in{};
+    in.outputs = g.outputs.as<float>();
+    in.positions = g.positions.as<float4>();
+    in.src = g.src.as<uint32_t>();
+    in.dst = g.neighbors.as<uint32_t>();
+    in.selection = selection ? sel.as<uint32_t>() : nullptr;
+    in.count = uint32_t(count);
+    in.out = res.as<double>();
+    launch_decode(in, stream_);
+    hip_check(hipGetLastError(), "decode launch");
+    hip_check(hipMemcpyAsync(out, res.p, count * 3 * 8, hipMemcpyDeviceToHost, stream_), "download placements");
+    hip_check(hipStreamSynchronize(stream_), "decode");
+}
+
+}  // namespace blitz_hip

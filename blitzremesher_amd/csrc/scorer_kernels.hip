@@ -341,6 +341,8 @@ __global__ void score_kernel(ScoreInput in) {
     float score = 0.f;
     if (in.mode == 0) {
         score = in.outputs[size_t(a) * kOutputs];
+    } else if (in.mode == 3) {
+        score = 0.f;  // constant ranking: action order
     } else {
         const uint32_t u = in.src[a], v = in.dst[a];
         const float4 pu = in.positions[u], pv = in.positions[v];

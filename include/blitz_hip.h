@@ -200,6 +200,52 @@ blitz_status blitz_hip_reduce_reuse(const blitz_mesh*, size_t target_triangles, 
                                     size_t index_capacity, size_t* index_count, char* error,
                                     size_t capacity);
 
+/* ---- edge-graph proposal scorer ------------------------------------------ */
+
+/* The HIP scorer ranks every directed contraction (u -> v) of a mesh: 32
+ * vertex features, three mean-aggregation message-passing layers of width 128
+ * and an edge MLP [h_u ; h_v] 256 -> 128 -> 4 (logit, placement xyz). It takes
+ * the place of ActionCuda::predict_device and the ranking step of
+ * GpuActionState (src/neural/action.hpp:75, src/neural/action_gpu.cu:1751-1798).
+ * Weights: float32, row-major [out, in] then bias, layer after layer
+ * (src/neural/internal.hpp:42-43). */
+typedef struct blitz_hip_scorer blitz_hip_scorer;
+typedef struct blitz_hip_graph blitz_hip_graph;
+
+#define BLITZ_HIP_PRECISION_FP32 0u /* deterministic compensated FP32 */
+#define BLITZ_HIP_PRECISION_BF16 1u /* BF16 MFMA, FP32 accumulation */
+
+size_t blitz_hip_scorer_weight_count(void);
+blitz_status blitz_hip_scorer_create(int32_t device, const float* weights, size_t count, blitz_hip_scorer**,
+                                     char* error, size_t capacity);
+void blitz_hip_scorer_destroy(blitz_hip_scorer*);
+blitz_status blitz_hip_scorer_set_weights(blitz_hip_scorer*, const float* weights, size_t count, char* error,
+                                          size_t capacity);
+/* Builds the mesh graph (neighbour CSR, one action per directed edge in CSR
+ * order) and its vertex features on the device. */
+blitz_status blitz_hip_graph_create(blitz_hip_scorer*, const blitz_mesh*, const blitz_hip_bounds*,
+                                    blitz_hip_graph**, char* error, size_t capacity);
+void blitz_hip_graph_destroy(blitz_hip_graph*);
+blitz_status blitz_hip_graph_info(const blitz_hip_graph*, uint32_t* vertices, uint32_t* actions);
+blitz_status blitz_hip_graph_features(blitz_hip_scorer*, const blitz_hip_graph*, float* out, size_t capacity);
+blitz_status blitz_hip_graph_actions(const blitz_hip_graph*, uint32_t* src, uint32_t* dst, size_t capacity);
+/* Forward pass over every action; outputs (actions x 4) stay on the device
+ * and are copied to `outputs` when it is not null. */
+blitz_status blitz_hip_scorer_forward(blitz_hip_scorer*, blitz_hip_graph*, uint8_t precision, float* outputs,
+                                      size_t capacity, char* error, size_t capacity_error);
+/* Descending score order of every action (rank_actions + radix sort,
+ * src/neural/action_gpu.cu:809-844,1793-1795); `ranking` takes the values of
+ * blitz_neural_options_v2.ranking: 0 learned, 1 constant, 3 shortest edge,
+ * 4 current plane (the plain classical order); 2 (shuffled) is unavailable.
+ * Equal scores keep ascending action order. */
+blitz_status blitz_hip_scorer_rank(blitz_hip_scorer*, blitz_hip_graph*, uint8_t ranking, uint32_t* order,
+                                   size_t capacity, char* error, size_t error_capacity);
+/* Placements of the given actions (decode_placements,
+ * src/neural/action_gpu.cu:1157-1188, without grid snapping): FP64 xyz. */
+blitz_status blitz_hip_scorer_decode(blitz_hip_scorer*, blitz_hip_graph*, const uint32_t* actions, size_t count,
+                                     double* placements, char* error, size_t capacity);
+blitz_status blitz_hip_scorer_last_forward_ms(const blitz_hip_scorer*, double* ms);
+
 #ifdef __cplusplus
 }
 #endif

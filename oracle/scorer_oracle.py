@@ -234,14 +234,38 @@ def decode_placements(positions: np.ndarray, acts: np.ndarray, outputs: np.ndarr
     return mid + np.clip(outputs[:, 1:4].astype(np.float64), -1, 1) * length[:, None]
 
 
-def he_weights(seed: int) -> np.ndarray:
-    """He-normal N(0, 2/fan_in) weights, zero biases, seeded."""
-    import sys
-    from pathlib import Path
-    sys.path.insert(0, str(Path(__file__).resolve().parent.parent))
-    from blitzremesher_amd.meshes import normal
-    parts = []
-    for k, (o, i) in enumerate(SHAPES):
-        parts.append((normal(seed, 1000 + k, o * i) * np.sqrt(2.0 / i)).astype(np.float32))
-        parts.append(np.zeros(o, np.float32))
-    return np.concatenate(parts)
+def scores(positions: np.ndarray, indices: np.ndarray, acts: np.ndarray, mode: int, outputs=None,
+           diameter: float = 1.0) -> np.ndarray:
+    """Ranking scores of src/neural/action_gpu.cu:809-844 as float32: 0 learned
+    logit, 1 constant, 3 -|dst - src| / diameter, 4 current plane: minus the
+    sum over faces around src of (plane distance of dst / diameter)^2 times
+    the face's |cross| / diameter^2, FP64 in face order."""
+    n = len(acts)
+    if mode == 0:
+        return outputs[:, 0].astype(np.float32) + np.float32(0)
+    if mode == 1:
+        return np.zeros(n, np.float32)
+    p = positions.astype(np.float64)
+    if mode == 3:
+        d = (p[acts[:, 1]] - p[acts[:, 0]]) / diameter
+        return -np.sqrt((d * d).sum(1)).astype(np.float32)
+    f = indices.astype(np.int64)
+    _, _, face_offsets, faces, _ = topology(indices, len(positions))
+    scale = max(1e-20, diameter)
+    out = np.zeros(n, np.float32)
+    for a in range(n):
+        u, v = acts[a]
+        cost = 0.0
+        for fi in faces[face_offsets[u]:face_offsets[u + 1]]:
+            P, Q, R = p[f[fi, 0]], p[f[fi, 1]], p[f[fi, 2]]
+            e1, e2 = Q - P, R - P
+            nrm = np.array([e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2],
+                            e1[0] * e2[1] - e1[1] * e2[0]])
+            w = np.sqrt(nrm[0] * nrm[0] + nrm[1] * nrm[1] + nrm[2] * nrm[2])
+            if not w > 0:
+                continue
+            q = p[v] - P
+            dist = (nrm[0] * q[0] + nrm[1] * q[1] + nrm[2] * q[2]) / w / scale
+            cost += dist * dist * w / (scale * scale)
+        out[a] = np.float32(-cost) + np.float32(0)
+    return out

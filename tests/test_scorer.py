@@ -1,0 +1,233 @@
+"""Edge-graph scorer: oracle pins (CPU) and HIP parity (GPU).
+
+The FP64 oracle (oracle/scorer_oracle.py) is pinned by closed forms that
+follow the reference's own scorer tests (tests/neural.cpp:82-117: bias-only
+output, directed neighbour mean, isolated vertex) and by an independent
+per-vertex loop for the features. GPU checks: features and the FP32 mode are
+bit-exact against the oracle's restatements; the BF16 mode is within a stated
+tolerance of FP64; rankings and placements match exactly.
+"""
+from __future__ import annotations
+
+import sys
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+ROOT = Path(__file__).resolve().parent.parent
+sys.path.insert(0, str(ROOT / "oracle"))
+
+import scorer_oracle as so  # noqa: E402
+from blitzremesher_amd import meshes  # noqa: E402
+from blitzremesher_amd.audit import Mesh, bounds  # noqa: E402
+
+
+def tiny_graph():
+    # Directed-mean semantics of tests/neural.cpp:99-113 on an undirected
+    # graph: path 0-1-2 plus isolated vertex 3.
+    offsets = np.array([0, 1, 3, 4, 4])
+    neighbors = np.array([1, 0, 2, 1])
+    return offsets, neighbors
+
+
+def test_weight_layout():
+    from blitzremesher_amd.scorer import he_weights, weight_count
+    w = he_weights(4)
+    assert w.size == so.weight_count() == weight_count() == 107524
+    for (W, b), (o, i) in zip(so.split_weights(w), so.SHAPES):
+        assert W.shape == (o, i) and not b.any()
+        assert abs(W.std() - np.sqrt(2.0 / i)) < 0.1 * np.sqrt(2.0 / i)
+
+
+def test_oracle_bias_only_closed_form():
+    # All matrices zero: every output equals the last layer's bias.
+    w = np.zeros(so.weight_count(), np.float32)
+    layers = so.split_weights(w)
+    for k, (W, b) in enumerate(layers):
+        b[:] = k + 1
+    offsets, neighbors = tiny_graph()
+    x = np.random.default_rng(0).random((4, so.FEATURES))
+    acts = so.actions(offsets, neighbors)
+    out = so.forward_fp64(x, offsets, neighbors, acts, w)
+    assert np.all(out == 5.0)
+    assert np.array_equal(so.forward_fp32(x, offsets, neighbors, acts, w), out.astype(np.float32))
+
+
+def test_oracle_neighbour_mean_closed_form():
+    # Layer 0 copies the mean of feature 0 into channel 0, later layers pass
+    # channel 0 through, the edge layer copies h(dst) channel 0, head = it.
+    w = np.zeros(so.weight_count(), np.float32)
+    layers = so.split_weights(w)
+    layers[0][0][0, so.FEATURES] = 1.0
+    layers[1][0][0, 0] = 1.0
+    layers[2][0][0, 0] = 1.0
+    layers[3][0][0, so.HIDDEN] = 1.0
+    layers[4][0][0, 0] = 1.0
+    offsets, neighbors = tiny_graph()
+    x = np.zeros((4, so.FEATURES))
+    x[:, 0] = [2, 4, 8, 16]
+    acts = so.actions(offsets, neighbors)
+    out = so.forward_fp64(x, offsets, neighbors, acts, w)
+    # h1 channel 0 = neighbour means: v0 -> 4, v1 -> (2+8)/2 = 5, v2 -> 4, v3 -> 0 (isolated).
+    h = {0: 4.0, 1: 5.0, 2: 4.0, 3: 0.0}
+    assert [out[a, 0] for a in range(len(acts))] == [h[int(d)] for d in acts[:, 1]]
+    assert np.array_equal(so.forward_fp32(x, offsets, neighbors, acts, w)[:, 0], out[:, 0].astype(np.float32))
+
+
+def test_compensated_sum_recovers_cancellation():
+    # Compensated FP32 keeps terms a naive float32 sum loses.
+    # Lane 0 sees 1e8 followed by eight ones (float spacing at 1e8 is 8).
+    w = np.zeros((1, 72), np.float32)
+    x = np.zeros((1, 72), np.float32)
+    x[0, 0::8] = [1e8] + [1.0] * 8
+    w[0, 0::8] = 1.0
+    got = so._compensated_linear(x, w, np.zeros(1, np.float32), False)
+    assert got[0, 0] == np.float32(100000008.0)
+    naive = np.float32(1e8)
+    for _ in range(8):
+        naive = np.float32(naive + np.float32(1.0))
+    assert naive == np.float32(1e8)
+
+
+def test_oracle_fp32_close_to_fp64():
+    from blitzremesher_amd.scorer import he_weights
+    m = meshes.torus(1.0, 0.3, 12, 8, 0.01, 3)
+    c, r = bounds(m)
+    x = so.features(m.positions, m.indices, c, 2 * r).astype(np.float32)
+    offsets, neighbors, *_ = so.topology(m.indices, len(m.positions))
+    acts = so.actions(offsets, neighbors)
+    w = he_weights(4)
+    ref = so.forward_fp64(x, offsets, neighbors, acts, w)
+    f32 = so.forward_fp32(x, offsets, neighbors, acts, w)
+    assert np.abs(f32 - ref).max() <= 1e-5 * max(1.0, np.abs(ref).max())
+
+
+def test_features_closed_form_on_flat_grid():
+    m = meshes.grid(4, double_sided=False)
+    c, r = bounds(m)
+    x = so.features(m.positions, m.indices, c, 2 * r)
+    assert np.allclose(x[:, 3:6], [0, 0, 1])
+    assert np.allclose(x[:, 16], 0)
+    # Plane z = 0: only a22 (n_z^2) is nonzero after area normalisation.
+    q = x[:, 6:16]
+    assert np.allclose(q[:, 7], 1) and np.allclose(np.delete(q, 7, axis=1), 0)
+    # Corner (0,0): neighbours (1,0), (0,1), (1,1); two boundary edges.
+    assert x[0, 17] == 3 / 16 and x[0, 22] == 2 / 4 and x[0, 18] == 1
+    centre = 2 * 5 + 2
+    assert x[centre, 17] == 6 / 16 and x[centre, 18] == 0
+
+
+def test_topology_is_symmetric_with_reverse():
+    m = meshes.icosphere(2, 0.02, 1)
+    offsets, neighbors, face_offsets, faces, boundary = so.topology(m.indices, len(m.positions))
+    acts = so.actions(offsets, neighbors)
+    pairs = {(int(a), int(b)) for a, b in acts}
+    assert all((b, a) in pairs for a, b in pairs)
+    assert len(boundary) == 0 and np.all(np.diff(offsets) >= 5)
+    assert np.array_equal(np.diff(face_offsets), np.bincount(m.indices.reshape(-1), minlength=len(m.positions)))
+
+
+# ----------------------------------------------------------------------- GPU
+
+@pytest.fixture(scope="module")
+def scorer():
+    from blitzremesher_amd.scorer import Scorer, he_weights
+    s = Scorer(he_weights(4))
+    yield s
+    s.close()
+
+
+def graph_case(scorer, mesh):
+    c, r = bounds(mesh)
+    g = scorer.graph(mesh, c, r)
+    offsets, neighbors, *_ = so.topology(mesh.indices, len(mesh.positions))
+    acts = so.actions(offsets, neighbors)
+    return g, c, r, offsets, neighbors, acts
+
+
+CASES = {
+    "torus": lambda: meshes.torus(1.0, 0.3, 40, 16, 0.01, 3),
+    "icosphere": lambda: meshes.icosphere(3, 0.02, 1),
+    "heightfield": lambda: meshes.heightfield(24, 0.1, 2),
+}
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", sorted(CASES))
+def test_gpu_features_and_actions_match_oracle(scorer, case):
+    mesh = CASES[case]()
+    g, c, r, offsets, neighbors, acts = graph_case(scorer, mesh)
+    assert np.array_equal(g.actions_array(), acts)
+    want = so.features(mesh.positions, mesh.indices, c, 2 * r).astype(np.float32)
+    assert np.array_equal(g.features(), want)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", sorted(CASES))
+def test_gpu_fp32_forward_is_bit_exact(scorer, case):
+    from blitzremesher_amd.scorer import FP32
+    from blitzremesher_amd.scorer import he_weights
+    mesh = CASES[case]()
+    g, c, r, offsets, neighbors, acts = graph_case(scorer, mesh)
+    got = scorer.forward(g, FP32)
+    want = so.forward_fp32(g.features(), offsets, neighbors, acts, he_weights(4))
+    assert np.array_equal(got, want)
+
+
+# BF16 tolerance: activations and weights rounded to 8-bit mantissas
+# (relative 2^-9 each) through four layers of width <= 256 with FP32
+# accumulation; measured errors are about 1e-2 of the output scale. The
+# bound below is 2^-4 of the largest FP64 output magnitude.
+BF16_TOLERANCE = 2 ** -4
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", sorted(CASES))
+def test_gpu_bf16_forward_within_tolerance_of_fp64(scorer, case):
+    from blitzremesher_amd.scorer import BF16, FP32, he_weights
+    mesh = CASES[case]()
+    g, c, r, offsets, neighbors, acts = graph_case(scorer, mesh)
+    ref = so.forward_fp64(g.features(), offsets, neighbors, acts, he_weights(4))
+    bf = scorer.forward(g, BF16)
+    f32 = scorer.forward(g, FP32)
+    scale = max(1.0, np.abs(ref).max())
+    err_bf = np.abs(bf - ref).max() / scale
+    err_32 = np.abs(f32 - ref).max() / scale
+    print(f"{case}: bf16 max rel {err_bf:.3e} rms {np.sqrt(((bf - ref) ** 2).mean()) / scale:.3e}; "
+          f"fp32 max rel {err_32:.3e}")
+    assert err_32 <= 1e-5
+    assert err_bf <= BF16_TOLERANCE
+
+
+@pytest.mark.gpu
+def test_gpu_rankings_match_oracle(scorer):
+    from blitzremesher_amd.scorer import CONSTANT, CURRENT_PLANE, FP32, LEARNED, SHORTEST_EDGE
+    mesh = meshes.torus(1.0, 0.3, 30, 12, 0.01, 3)
+    g, c, r, offsets, neighbors, acts = graph_case(scorer, mesh)
+    out = scorer.forward(g, FP32)
+    for mode in (LEARNED, CONSTANT, SHORTEST_EDGE, CURRENT_PLANE):
+        want = so.rank(so.scores(mesh.positions, mesh.indices, acts, mode, out, 2 * r))
+        assert np.array_equal(scorer.rank(g, mode), want), mode
+
+
+@pytest.mark.gpu
+def test_gpu_decode_matches_oracle(scorer):
+    from blitzremesher_amd.scorer import BF16
+    mesh = meshes.icosphere(2, 0.02, 1)
+    g, c, r, offsets, neighbors, acts = graph_case(scorer, mesh)
+    out = scorer.forward(g, BF16)
+    pick = np.arange(0, len(acts), 7, dtype=np.uint32)
+    got = scorer.decode(g, pick)
+    want = so.decode_placements(mesh.positions, acts[pick], out[pick])
+    assert np.array_equal(got, want)
+
+
+@pytest.mark.gpu
+def test_gpu_shuffled_ranking_is_unavailable(scorer):
+    from blitzremesher_amd import BlitzError
+    from blitzremesher_amd.scorer import SHUFFLED
+    g = graph_case(scorer, meshes.icosphere(1, 0.0))[0]
+    with pytest.raises(BlitzError) as e:
+        scorer.rank(g, SHUFFLED)
+    assert e.value.status == 6
