@@ -85,6 +85,17 @@ class AuditOptions(C.Structure):
     ]
 
 
+class TrainSettingsC(C.Structure):
+    _fields_ = [
+        ("margin", C.c_float), ("penalty", C.c_float), ("lr", C.c_float), ("decay", C.c_float),
+        ("max_norm", C.c_float), ("placement_weight", C.c_float), ("pool", C.c_uint32), ("precision", C.c_uint8),
+    ]
+
+
+class TrainResultC(C.Structure):
+    _fields_ = [("loss", C.c_float), ("grad_norm", C.c_float), ("applied", C.c_uint8), ("step", C.c_uint32)]
+
+
 class AuditTiming(C.Structure):
     _fields_ = [
         ("struct_size", C.c_uint32), ("raster_tasks", C.c_uint32), ("distance_views", C.c_uint32),
@@ -138,6 +149,14 @@ PROTOTYPES = {
     "blitz_hip_scorer_rank": (C.c_int, [_P, _P, C.c_uint8, C.POINTER(C.c_uint32), C.c_size_t, *_ERR]),
     "blitz_hip_scorer_decode": (C.c_int, [_P, _P, C.POINTER(C.c_uint32), C.c_size_t, C.POINTER(C.c_double), *_ERR]),
     "blitz_hip_scorer_last_forward_ms": (C.c_int, [_P, C.POINTER(C.c_double)]),
+    "blitz_hip_train_settings_init": (None, [C.POINTER(TrainSettingsC)]),
+    "blitz_hip_scorer_train_step": (C.c_int, [_P, _P, C.POINTER(C.c_uint32), C.POINTER(C.c_uint8),
+                                              C.POINTER(C.c_float), C.c_size_t, C.POINTER(TrainSettingsC),
+                                              C.POINTER(TrainResultC), *_ERR]),
+    "blitz_hip_scorer_gradients": (C.c_int, [_P, C.POINTER(C.c_float), C.c_size_t]),
+    "blitz_hip_scorer_weights": (C.c_int, [_P, C.POINTER(C.c_float), C.c_size_t]),
+    "blitz_hip_scorer_reset_optimizer": (C.c_int, [_P]),
+    "blitz_hip_scorer_last_step_ms": (C.c_int, [_P, C.POINTER(C.c_double)]),
 }
 
 _lib: C.CDLL | None = None

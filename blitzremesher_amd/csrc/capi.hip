@@ -557,4 +557,65 @@ blitz_status blitz_hip_scorer_last_forward_ms(const blitz_hip_scorer* s, double*
     *ms = s->impl->last_forward_ms();
     return BLITZ_OK;
 }
+
+void blitz_hip_train_settings_init(blitz_hip_train_settings* s) {
+    if (!s) return;
+    const TrainSettings d;
+    s->margin = d.margin;
+    s->penalty = d.penalty;
+    s->lr = d.lr;
+    s->decay = d.decay;
+    s->max_norm = d.max_norm;
+    s->placement_weight = d.placement_weight;
+    s->pool = d.pool;
+    s->precision = d.precision;
+}
+
+blitz_status blitz_hip_scorer_train_step(blitz_hip_scorer* s, blitz_hip_graph* g, const uint32_t* actions,
+                                         const uint8_t* labels, const float* targets, size_t count,
+                                         const blitz_hip_train_settings* settings, blitz_hip_train_result* result,
+                                         char* error, size_t capacity) {
+    return guarded(error, capacity, [&] {
+        if (!settings || !result) throw std::invalid_argument("null training settings or result");
+        TrainSettings t;
+        t.margin = settings->margin;
+        t.penalty = settings->penalty;
+        t.lr = settings->lr;
+        t.decay = settings->decay;
+        t.max_norm = settings->max_norm;
+        t.placement_weight = settings->placement_weight;
+        t.pool = settings->pool;
+        t.precision = settings->precision;
+        Scorer& scorer = scorer_of(s);
+        const TrainResult r = scorer.train_step(graph_of(s, g), actions, labels, targets, count, t);
+        result->loss = r.loss;
+        result->grad_norm = r.grad_norm;
+        result->applied = r.applied;
+        result->step = scorer.optimizer_steps();
+    });
+}
+
+blitz_status blitz_hip_scorer_gradients(blitz_hip_scorer* s, float* out, size_t capacity) {
+    return guarded(nullptr, 0, [&] {
+        if (!out || capacity < kWeightCount) throw std::invalid_argument("gradient capacity");
+        scorer_of(s).download_gradients(out);
+    });
+}
+
+blitz_status blitz_hip_scorer_weights(blitz_hip_scorer* s, float* out, size_t capacity) {
+    return guarded(nullptr, 0, [&] {
+        if (!out || capacity < kWeightCount) throw std::invalid_argument("weight capacity");
+        scorer_of(s).download_weights(out);
+    });
+}
+
+blitz_status blitz_hip_scorer_reset_optimizer(blitz_hip_scorer* s) {
+    return guarded(nullptr, 0, [&] { scorer_of(s).reset_optimizer(); });
+}
+
+blitz_status blitz_hip_scorer_last_step_ms(const blitz_hip_scorer* s, double* ms) {
+    if (!s || !s->impl || !ms) return BLITZ_INVALID_ARGUMENT;
+    *ms = s->impl->last_step_ms();
+    return BLITZ_OK;
+}
 }

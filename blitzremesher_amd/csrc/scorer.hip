@@ -120,16 +120,18 @@ void Scorer::set_weights(const float* w, size_t count) {
     DeviceGuard guard(device_);
     weights_.resize(count * 4);
     hip_check(hipMemcpyAsync(weights_.p, w, count * 4, hipMemcpyHostToDevice, stream_), "upload weights");
-    // BF16 copy of the four MFMA layer matrices (biases stay FP32).
-    std::vector<float> mats(kBf16WeightCount);
-    for (int l = 0; l < 4; ++l)
-        std::memcpy(mats.data() + bf16_offset(l), w + layer_offset(l), size_t(kHidden) * kLayerIn[l] * 4);
-    DeviceBuffer staging;
-    staging.resize(mats.size() * 4);
-    hip_check(hipMemcpyAsync(staging.p, mats.data(), mats.size() * 4, hipMemcpyHostToDevice, stream_), "upload");
-    weights_bf16_.resize(mats.size() * 2);
-    launch_convert_bf16(staging.as<float>(), weights_bf16_.as<__bf16>(), mats.size(), stream_);
+    refresh_bf16();
+    reset_optimizer();
     hip_check(hipStreamSynchronize(stream_), "weights");
+}
+
+// BF16 copy of the four MFMA layer matrices from the FP32 master weights
+// (biases and the 4-wide head stay FP32).
+void Scorer::refresh_bf16() {
+    weights_bf16_.resize(size_t(kBf16WeightCount) * 2);
+    for (int l = 0; l < 4; ++l)
+        launch_convert_bf16(weights_.as<float>() + layer_offset(l), weights_bf16_.as<__bf16>() + bf16_offset(l),
+                            size_t(kHidden) * kLayerIn[l], stream_);
 }
 
 std::unique_ptr<Graph> Scorer::graph(const HostMesh& m, const Bounds& b) {

@@ -39,8 +39,25 @@ struct Graph {
     DeviceBuffer positions, indices, offsets, neighbors, src, reverse, face_offsets, faces_of, boundary;
     DeviceBuffer features, features_bf16, outputs, h_a, h_b, z, hq_a, hq_b;
     DeviceBuffer keys, order, keys_out, order_out, sort_temp;
+    // Training: activations of every layer (FP32 and BF16), layer inputs,
+    // batch buffers and gradients.
+    DeviceBuffer t_h[4], t_hq[4], t_z[3];
+    DeviceBuffer batch_actions, batch_src, batch_dst, labels, targets, hidden, batch_out, dy, t_e, z_e, dz_e, dh_a, dh_b, t_v,
+        accum;
     bool has_outputs = false;
     GraphView view() const;
+};
+
+// Loss and optimizer settings of one training step (defaults of the
+// reference's UpdateSettings, training/update_cuda.hpp:47-53).
+struct TrainSettings {
+    float margin = 1.f, penalty = 1e-4f, lr = 1e-3f, decay = 1e-4f, max_norm = 1.f, placement_weight = 1.f;
+    uint32_t pool = 32;
+    uint8_t precision = 1;  // 0 FP32 forward, 1 BF16 forward
+};
+struct TrainResult {
+    float loss = 0, grad_norm = 0;
+    bool applied = false;  // false when the gradient norm is not finite
 };
 
 enum class Precision : uint8_t { Fp32 = 0, Bf16 = 1 };
@@ -63,12 +80,26 @@ class Scorer {
     double last_forward_ms() const { return forward_ms_; }
     int device() const { return device_; }
     hipStream_t stream() const { return stream_; }
+    // One optimizer step on a batch of labelled actions of graph g (see
+    // train_kernels.hip for the loss). labels: bit 0 valid, bit 1 preferred,
+    // bit 2 placement target present; targets: 3 floats per entry or null.
+    TrainResult train_step(Graph& g, const uint32_t* actions, const uint8_t* labels, const float* targets,
+                           size_t count, const TrainSettings& s);
+    void download_gradients(float* out);
+    void download_weights(float* out);
+    void reset_optimizer();
+    double last_step_ms() const { return step_ms_; }
+    uint32_t optimizer_steps() const { return step_; }
 
   private:
+    void refresh_bf16();
     int device_;
     hipStream_t stream_{};
     hipEvent_t begin_{}, end_{};
     DeviceBuffer weights_, weights_bf16_;
+    DeviceBuffer grads_, mean_, variance_;
+    uint32_t step_ = 0;
+    double step_ms_ = 0;
     double forward_ms_ = 0;
     uint32_t blocks_ = 512;
 };

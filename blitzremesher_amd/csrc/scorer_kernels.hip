@@ -279,7 +279,7 @@ __global__ __launch_bounds__(256) void mp_layer_bf16(const __bf16* __restrict__ 
 __global__ __launch_bounds__(256) void edge_mlp_bf16(const __bf16* __restrict__ h, GraphView g,
                                                      const __bf16* __restrict__ wq, const float* __restrict__ bias,
                                                      const float* __restrict__ wo, const float* __restrict__ bo,
-                                                     float* __restrict__ out) {
+                                                     float* __restrict__ out, float* __restrict__ hidden_out) {
     constexpr uint32_t K = kEdgeIn;
     extern __shared__ __bf16 smem[];
     __bf16* w = smem;
@@ -315,6 +315,11 @@ __global__ __launch_bounds__(256) void edge_mlp_bf16(const __bf16* __restrict__ 
             }
         }
         __syncthreads();
+        if (hidden_out)
+            for (uint32_t i = threadIdx.x; i < kTileRows * kHidden; i += blockDim.x) {
+                const uint32_t r = i / kHidden, col = i % kHidden, a = tile * kTileRows + r;
+                if (a < g.actions) hidden_out[size_t(a) * kHidden + col] = hidden[r * (kHidden + 1) + col];
+            }
         {
             const uint32_t r = threadIdx.x / kOutputs, o = threadIdx.x % kOutputs;
             const uint32_t a = tile * kTileRows + r;
@@ -436,32 +441,62 @@ void launch_convert_bf16(const float* x, __bf16* y, size_t n, hipStream_t stream
     to_bf16<<<dim3(uint32_t(std::min<size_t>((n + 255) / 256, 4096))), dim3(256), 0, stream>>>(x, y, n);
 }
 
-void launch_forward_bf16(const __bf16* x, GraphView g, const __bf16* wq, const float* weights, __bf16* h_a,
-                         __bf16* h_b, float* out, uint32_t blocks, hipStream_t stream) {
+void launch_aggregate_fp32(const float* h, GraphView g, uint32_t width, float* z, hipStream_t stream) {
+    const size_t n = size_t(g.vertices) * width;
+    if (n) aggregate_fp32<<<dim3(uint32_t((n + 255) / 256)), dim3(256), 0, stream>>>(h, g, width, z);
+}
+
+void launch_compensated(const float* input, const uint32_t* src, const uint32_t* dst, const float* weights,
+                        const float* bias, float* output, uint32_t rows, uint32_t in, uint32_t out, bool relu,
+                        hipStream_t stream) {
+    const size_t threads = size_t(rows) * out * 8;
+    if (threads)
+        compensated_linear<<<dim3(uint32_t((threads + 255) / 256)), dim3(256), 0, stream>>>(
+            input, src, dst, weights, bias, output, rows, in, out, relu);
+}
+
+static void configure_bf16() {
     static bool configured = false;
-    if (!configured) {
-        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&mp_layer_bf16<2 * kFeatures>),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, int(forward_bf16_lds(2 * kFeatures)));
-        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&mp_layer_bf16<2 * kHidden>),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, int(forward_bf16_lds(2 * kHidden)));
-        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&edge_mlp_bf16),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, int(forward_bf16_lds(kEdgeIn)));
-        configured = true;
-    }
+    if (configured) return;
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&mp_layer_bf16<2 * kFeatures>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, int(forward_bf16_lds(2 * kFeatures)));
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&mp_layer_bf16<2 * kHidden>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, int(forward_bf16_lds(2 * kHidden)));
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&edge_mlp_bf16),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, int(forward_bf16_lds(kEdgeIn)));
+    configured = true;
+}
+
+void launch_mp_bf16(int layer, const __bf16* h, GraphView g, const __bf16* wq, const float* weights, __bf16* out,
+                    uint32_t blocks, hipStream_t stream) {
+    configure_bf16();
     if (!g.vertices) return;
-    const uint32_t vtiles = (g.vertices + kTileRows - 1) / kTileRows;
-    const uint32_t vb = std::min(blocks, vtiles);
-    mp_layer_bf16<2 * kFeatures><<<dim3(vb), dim3(256), forward_bf16_lds(2 * kFeatures), stream>>>(
-        x, g, wq + bf16_offset(0), weights + layer_offset(0) + kHidden * kLayerIn[0], h_a);
-    mp_layer_bf16<2 * kHidden><<<dim3(vb), dim3(256), forward_bf16_lds(2 * kHidden), stream>>>(
-        h_a, g, wq + bf16_offset(1), weights + layer_offset(1) + kHidden * kLayerIn[1], h_b);
-    mp_layer_bf16<2 * kHidden><<<dim3(vb), dim3(256), forward_bf16_lds(2 * kHidden), stream>>>(
-        h_b, g, wq + bf16_offset(2), weights + layer_offset(2) + kHidden * kLayerIn[2], h_a);
+    const uint32_t vb = std::min(blocks, (g.vertices + kTileRows - 1) / kTileRows);
+    const float* bias = weights + layer_offset(layer) + kHidden * kLayerIn[layer];
+    if (layer == 0)
+        mp_layer_bf16<2 * kFeatures><<<dim3(vb), dim3(256), forward_bf16_lds(2 * kFeatures), stream>>>(
+            h, g, wq + bf16_offset(0), bias, out);
+    else
+        mp_layer_bf16<2 * kHidden><<<dim3(vb), dim3(256), forward_bf16_lds(2 * kHidden), stream>>>(
+            h, g, wq + bf16_offset(layer), bias, out);
+}
+
+void launch_edge_bf16(const __bf16* h, GraphView g, const __bf16* wq, const float* weights, float* out,
+                      float* hidden_out, uint32_t blocks, hipStream_t stream) {
+    configure_bf16();
     if (!g.actions) return;
     const uint32_t atiles = (g.actions + kTileRows - 1) / kTileRows;
     edge_mlp_bf16<<<dim3(std::min(blocks, atiles)), dim3(256), forward_bf16_lds(kEdgeIn), stream>>>(
-        h_a, g, wq + bf16_offset(3), weights + layer_offset(3) + kHidden * kEdgeIn, weights + layer_offset(4),
-        weights + layer_offset(4) + kOutputs * kHidden, out);
+        h, g, wq + bf16_offset(3), weights + layer_offset(3) + kHidden * kEdgeIn, weights + layer_offset(4),
+        weights + layer_offset(4) + kOutputs * kHidden, out, hidden_out);
+}
+
+void launch_forward_bf16(const __bf16* x, GraphView g, const __bf16* wq, const float* weights, __bf16* h_a,
+                         __bf16* h_b, float* out, uint32_t blocks, hipStream_t stream) {
+    launch_mp_bf16(0, x, g, wq, weights, h_a, blocks, stream);
+    launch_mp_bf16(1, h_a, g, wq, weights, h_b, blocks, stream);
+    launch_mp_bf16(2, h_b, g, wq, weights, h_a, blocks, stream);
+    launch_edge_bf16(h_a, g, wq, weights, out, nullptr, blocks, stream);
 }
 
 void launch_scores(const ScoreInput& in, hipStream_t stream) {

@@ -1,0 +1,301 @@
+// Training kernels of the edge-graph scorer: loss, backward, optimizer.
+//
+// Loss over a batch of labelled actions grouped into pools of P consecutive
+// entries (the reference's per-state pools; the joint-scope terms of
+// training/update_cuda.cu:256-330 applied to this network's four outputs
+// [logit, placement x3]): pairwise hinge on the logit between preferred and
+// other valid actions of a pool (margin m, normalised by the number of pairs
+// over the batch), an L2 penalty of weight penalty/valid/3 on the logit of
+// valid actions, and a smooth-L1 (delta 1) loss, normalised by 3 x the number
+// of placed entries, on the three placement outputs of placed entries.
+// Gradient clipping by global norm and AdamW restate
+// training/update_cuda.cu:338-393 operation for operation.
+#include <hip/hip_runtime.h>
+
+#include "train_kernels.hpp"
+
+namespace blitz_hip {
+namespace {
+
+// Label bits of one batch entry.
+constexpr uint8_t kValid = 1, kPreferred = 2, kPlaced = 4;
+
+__global__ __launch_bounds__(256) void loss_kernel(const float* __restrict__ y, const uint8_t* __restrict__ labels,
+                                                   const float* __restrict__ targets, float* __restrict__ dy,
+                                                   float* __restrict__ loss, LossInput in) {
+    const uint32_t pool0 = blockIdx.x * in.pool;
+    const uint32_t j = pool0 + threadIdx.x;
+    float l = 0.f;
+    if (threadIdx.x < in.pool && j < in.count) {
+        const uint8_t label = labels[j];
+        float g[4] = {0, 0, 0, 0};
+        const float* p = y + size_t(j) * 4;
+        if (label & kValid) {
+            const bool preferred = label & kPreferred;
+            const float score = p[0];
+            const float inv = 1.f / float(in.pairs > 0 ? in.pairs : 1u);
+            const uint32_t end = min(pool0 + in.pool, in.count);
+            for (uint32_t k = pool0; k < end; ++k) {
+                const uint8_t other = labels[k];
+                if ((other & kValid) && bool(other & kPreferred) != preferred) {
+                    const float v = in.margin + (preferred ? y[size_t(k) * 4] - score : score - y[size_t(k) * 4]);
+                    if (v > 0) {
+                        g[0] += (preferred ? -1.f : 1.f) * inv;
+                        if (preferred) l += v * inv;
+                    }
+                }
+            }
+            const float w = in.penalty * (1.f / float(in.valid > 0 ? in.valid : 1u)) / 3;
+            l += w * score * score;
+            g[0] += 2 * w * score;
+        }
+        if (label & kPlaced) {
+            const float inv = in.placement_weight / (float(in.placed > 0 ? in.placed : 1u) * 3.f);
+            for (int c = 0; c < 3; ++c) {
+                const float delta = p[1 + c] - targets[size_t(j) * 3 + c];
+                const float a = fabsf(delta);
+                l += (a < 1 ? .5f * delta * delta : a - .5f) * inv;
+                g[1 + c] = fminf(1.f, fmaxf(-1.f, delta)) * inv;
+            }
+        }
+        for (int c = 0; c < 4; ++c) dy[size_t(j) * 4 + c] = g[c];
+    }
+    for (int o = 32; o > 0; o >>= 1) l += __shfl_xor(l, o);
+    if ((threadIdx.x & 63) == 0 && l != 0.f) atomicAdd(loss, l);
+}
+
+// C[M][N] += sum_r A(r, m) * B(r, n) over rows [r0, r1) of this block's
+// slice (split-R with atomic accumulation). A and B are row-major with row
+// strides lda/ldb; a_mask (optional) zeroes A(r, m) where mask(r, m) <= 0
+// (ReLU derivative), reading the mask with A's layout.
+__global__ __launch_bounds__(256) void gemm_at_b(const float* __restrict__ a, uint32_t lda,
+                                                 const float* __restrict__ a_mask, const float* __restrict__ b,
+                                                 uint32_t ldb, float* __restrict__ c, uint32_t rows, uint32_t m,
+                                                 uint32_t n, uint32_t rows_per_slice) {
+    __shared__ float as[16][64 + 1], bs[16][64 + 1];
+    const uint32_t tm = blockIdx.x * 64, tn = blockIdx.y * 64;
+    const uint32_t r0 = blockIdx.z * rows_per_slice, r1 = min(rows, r0 + rows_per_slice);
+    const uint32_t tx = threadIdx.x % 16, ty = threadIdx.x / 16;  // 4x4 outputs each
+    float acc[4][4] = {};
+    for (uint32_t r = r0; r < r1; r += 16) {
+        for (uint32_t i = threadIdx.x; i < 16 * 64; i += 256) {
+            const uint32_t rr = i / 64, cc = i % 64, row = r + rr;
+            float va = 0.f, vb = 0.f;
+            if (row < r1) {
+                if (tm + cc < m) {
+                    va = a[size_t(row) * lda + tm + cc];
+                    if (a_mask && !(a_mask[size_t(row) * lda + tm + cc] > 0.f)) va = 0.f;
+                }
+                if (tn + cc < n) vb = b[size_t(row) * ldb + tn + cc];
+            }
+            as[rr][cc] = va;
+            bs[rr][cc] = vb;
+        }
+        __syncthreads();
+        for (int k = 0; k < 16; ++k)
+            for (int i = 0; i < 4; ++i)
+                for (int jj = 0; jj < 4; ++jj) acc[i][jj] += as[k][ty * 4 + i] * bs[k][tx * 4 + jj];
+        __syncthreads();
+    }
+    for (int i = 0; i < 4; ++i)
+        for (int jj = 0; jj < 4; ++jj) {
+            const uint32_t mi = tm + ty * 4 + i, nj = tn + tx * 4 + jj;
+            if (mi < m && nj < n && acc[i][jj] != 0.f) atomicAdd(&c[size_t(mi) * n + nj], acc[i][jj]);
+        }
+}
+
+// T[R][N] = (A masked)[R][K] x W[K][N] (W row-major with K rows of N).
+__global__ __launch_bounds__(256) void gemm_a_w(const float* __restrict__ a, const float* __restrict__ a_mask,
+                                                const float* __restrict__ w, float* __restrict__ t, uint32_t rows,
+                                                uint32_t k_dim, uint32_t n) {
+    __shared__ float as[64][16 + 1], ws[16][64 + 1];
+    const uint32_t tr = blockIdx.x * 64, tn = blockIdx.y * 64;
+    const uint32_t tx = threadIdx.x % 16, ty = threadIdx.x / 16;
+    float acc[4][4] = {};
+    for (uint32_t k0 = 0; k0 < k_dim; k0 += 16) {
+        for (uint32_t i = threadIdx.x; i < 64 * 16; i += 256) {
+            const uint32_t rr = i / 16, kk = i % 16;
+            float v = 0.f;
+            if (tr + rr < rows && k0 + kk < k_dim) {
+                v = a[size_t(tr + rr) * k_dim + k0 + kk];
+                if (a_mask && !(a_mask[size_t(tr + rr) * k_dim + k0 + kk] > 0.f)) v = 0.f;
+            }
+            as[rr][kk] = v;
+        }
+        for (uint32_t i = threadIdx.x; i < 16 * 64; i += 256) {
+            const uint32_t kk = i / 64, cc = i % 64;
+            ws[kk][cc] = (k0 + kk < k_dim && tn + cc < n) ? w[size_t(k0 + kk) * n + tn + cc] : 0.f;
+        }
+        __syncthreads();
+        for (int k = 0; k < 16; ++k)
+            for (int i = 0; i < 4; ++i)
+                for (int jj = 0; jj < 4; ++jj) acc[i][jj] += as[ty * 4 + i][k] * ws[k][tx * 4 + jj];
+        __syncthreads();
+    }
+    for (int i = 0; i < 4; ++i)
+        for (int jj = 0; jj < 4; ++jj) {
+            const uint32_t ri = tr + ty * 4 + i, nj = tn + tx * 4 + jj;
+            if (ri < rows && nj < n) t[size_t(ri) * n + nj] = acc[i][jj];
+        }
+}
+
+// Column sums of (A masked): bias gradients.
+__global__ __launch_bounds__(256) void column_sum(const float* __restrict__ a, const float* __restrict__ a_mask,
+                                                  float* __restrict__ out, uint32_t rows, uint32_t cols) {
+    const uint32_t c = blockIdx.x;
+    float s = 0.f;
+    for (uint32_t r = threadIdx.x; r < rows; r += 256) {
+        float v = a[size_t(r) * cols + c];
+        if (a_mask && !(a_mask[size_t(r) * cols + c] > 0.f)) v = 0.f;
+        s += v;
+    }
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    __shared__ float part[4];
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) out[c] += part[0] + part[1] + part[2] + part[3];
+}
+
+// Endpoints of the batch's actions.
+__global__ void batch_endpoints(const uint32_t* __restrict__ actions, GraphView g, uint32_t* __restrict__ src,
+                                uint32_t* __restrict__ dst, uint32_t count) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    src[i] = g.src[actions[i]];
+    dst[i] = g.neighbors[actions[i]];
+}
+
+// z[r] = [h(src[r]) ; h(dst[r])], the edge layer's input.
+__global__ void gather_edge(const float* __restrict__ h, const uint32_t* __restrict__ src,
+                            const uint32_t* __restrict__ dst, float* __restrict__ z, uint32_t count) {
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= size_t(count) * kEdgeIn) return;
+    const uint32_t r = uint32_t(i / kEdgeIn), c = uint32_t(i % kEdgeIn);
+    const uint32_t v = c < kHidden ? src[r] : dst[r];
+    z[i] = h[size_t(v) * kHidden + (c % kHidden)];
+}
+
+// dz of the edge input [h(src) ; h(dst)] scattered into dh.
+__global__ void scatter_edge(const float* __restrict__ dz, const uint32_t* __restrict__ src,
+                             const uint32_t* __restrict__ dst, float* __restrict__ dh, uint32_t count) {
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= size_t(count) * kEdgeIn) return;
+    const uint32_t r = uint32_t(i / kEdgeIn), c = uint32_t(i % kEdgeIn);
+    const uint32_t v = c < kHidden ? src[r] : dst[r];
+    atomicAdd(&dh[size_t(v) * kHidden + (c % kHidden)], dz[i]);
+}
+
+// dh[v] = t[v][0:d] + sum over u in N(v) of t[u][d:2d] / deg(u): the
+// transpose of the mean aggregation on a symmetric neighbour graph.
+__global__ void aggregate_backward(const float* __restrict__ t, GraphView g, uint32_t width, float* __restrict__ dh) {
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= size_t(g.vertices) * width) return;
+    const uint32_t v = uint32_t(i / width), c = uint32_t(i % width);
+    float s = t[size_t(v) * 2 * width + c];
+    for (uint32_t k = g.offsets[v]; k < g.offsets[v + 1]; ++k) {
+        const uint32_t u = g.neighbors[k];
+        const uint32_t deg = g.offsets[u + 1] - g.offsets[u];
+        s += t[size_t(u) * 2 * width + width + c] / float(deg > 0 ? deg : 1u);
+    }
+    dh[i] = s;
+}
+
+__global__ void norm_kernel(const float* __restrict__ g, uint32_t n, float* __restrict__ sum) {
+    float s = 0.f;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const float v = g[i];
+        s += isfinite(v) ? v * v : INFINITY;
+    }
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    if ((threadIdx.x & 63) == 0) atomicAdd(sum, s);
+}
+
+// training/update_cuda.cu:378-393 with the clip and bias corrections
+// computed on the host from the norm (training/update_cuda.cu:358-376).
+__global__ void adamw_kernel(float* __restrict__ value, const float* __restrict__ grad, float* __restrict__ mean,
+                             float* __restrict__ variance, uint32_t n, AdamStep s) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float g = grad[i] * s.clip;
+    const float m = mean[i] * .9f + g * .1f, v = variance[i] * .999f + (g * g) * .001f;
+    mean[i] = m;
+    variance[i] = v;
+    value[i] = value[i] * (1 - s.lr * s.decay) - s.correction1 * m / (sqrtf(v) / s.correction2 + 1e-8f);
+}
+
+__global__ void bf16_to_fp32(const __bf16* __restrict__ x, float* __restrict__ y, size_t n) {
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x)
+        y[i] = float(x[i]);
+}
+
+}  // namespace
+
+void launch_loss(const float* y, const uint8_t* labels, const float* targets, float* dy, float* loss,
+                 const LossInput& in, hipStream_t stream) {
+    if (!in.count) return;
+    loss_kernel<<<dim3((in.count + in.pool - 1) / in.pool), dim3(256), 0, stream>>>(y, labels, targets, dy, loss, in);
+}
+
+void launch_gemm_at_b(const float* a, uint32_t lda, const float* a_mask, const float* b, uint32_t ldb, float* c,
+                      uint32_t rows, uint32_t m, uint32_t n, hipStream_t stream) {
+    if (!rows) return;
+    const uint32_t tiles = ((m + 63) / 64) * ((n + 63) / 64);
+    const uint32_t slices = std::max<uint32_t>(1, std::min<uint32_t>((rows + 255) / 256, 1024 / std::max(1u, tiles)));
+    const uint32_t per = (rows + slices - 1) / slices;
+    gemm_at_b<<<dim3((m + 63) / 64, (n + 63) / 64, slices), dim3(256), 0, stream>>>(a, lda, a_mask, b, ldb, c, rows, m,
+                                                                                      n, per);
+}
+
+void launch_gemm_a_w(const float* a, const float* a_mask, const float* w, float* t, uint32_t rows, uint32_t k,
+                     uint32_t n, hipStream_t stream) {
+    if (!rows) return;
+    gemm_a_w<<<dim3((rows + 63) / 64, (n + 63) / 64), dim3(256), 0, stream>>>(a, a_mask, w, t, rows, k, n);
+}
+
+void launch_column_sum(const float* a, const float* a_mask, float* out, uint32_t rows, uint32_t cols,
+                       hipStream_t stream) {
+    if (!rows || !cols) return;
+    column_sum<<<dim3(cols), dim3(256), 0, stream>>>(a, a_mask, out, rows, cols);
+}
+
+void launch_batch_endpoints(const uint32_t* actions, GraphView g, uint32_t* src, uint32_t* dst, uint32_t count,
+                            hipStream_t stream) {
+    if (!count) return;
+    batch_endpoints<<<dim3((count + 255) / 256), dim3(256), 0, stream>>>(actions, g, src, dst, count);
+}
+
+void launch_gather_edge(const float* h, const uint32_t* src, const uint32_t* dst, float* z, uint32_t count,
+                        hipStream_t stream) {
+    const size_t n = size_t(count) * kEdgeIn;
+    if (!n) return;
+    gather_edge<<<dim3(uint32_t((n + 255) / 256)), dim3(256), 0, stream>>>(h, src, dst, z, count);
+}
+
+void launch_scatter_edge(const float* dz, const uint32_t* src, const uint32_t* dst, float* dh, uint32_t count,
+                         hipStream_t stream) {
+    const size_t n = size_t(count) * kEdgeIn;
+    if (!n) return;
+    scatter_edge<<<dim3(uint32_t((n + 255) / 256)), dim3(256), 0, stream>>>(dz, src, dst, dh, count);
+}
+
+void launch_aggregate_backward(const float* t, GraphView g, uint32_t width, float* dh, hipStream_t stream) {
+    const size_t n = size_t(g.vertices) * width;
+    if (!n) return;
+    aggregate_backward<<<dim3(uint32_t((n + 255) / 256)), dim3(256), 0, stream>>>(t, g, width, dh);
+}
+
+void launch_norm(const float* g, uint32_t n, float* sum, hipStream_t stream) {
+    norm_kernel<<<dim3(std::min<uint32_t>((n + 255) / 256, 1024)), dim3(256), 0, stream>>>(g, n, sum);
+}
+
+void launch_adamw(float* value, const float* grad, float* mean, float* variance, uint32_t n, const AdamStep& s,
+                  hipStream_t stream) {
+    adamw_kernel<<<dim3((n + 255) / 256), dim3(256), 0, stream>>>(value, grad, mean, variance, n, s);
+}
+
+void launch_bf16_to_fp32(const __bf16* x, float* y, size_t n, hipStream_t stream) {
+    if (!n) return;
+    bf16_to_fp32<<<dim3(uint32_t(std::min<size_t>((n + 255) / 256, 4096))), dim3(256), 0, stream>>>(x, y, n);
+}
+
+}  // namespace blitz_hip

@@ -9,6 +9,7 @@ builds one mesh's neighbour graph and vertex features on the device;
 from __future__ import annotations
 
 import ctypes as C
+from dataclasses import dataclass
 
 import numpy as np
 
@@ -132,3 +133,71 @@ class Scorer:
         ms = C.c_double()
         _lib.check(self.lib.blitz_hip_scorer_last_forward_ms(self.handle, C.byref(ms)))
         return ms.value
+
+    # ---- training
+    def train_step(self, graph: Graph, actions: np.ndarray, labels: np.ndarray, targets: np.ndarray | None = None,
+                   settings: TrainSettings | None = None) -> TrainResult:
+        """One optimizer step on a labelled batch (blitz_hip_scorer_train_step)."""
+        s = settings or TrainSettings()
+        a = np.ascontiguousarray(actions, np.uint32)
+        lab = np.ascontiguousarray(labels, np.uint8)
+        if lab.shape != a.shape:
+            raise ValueError("labels must match actions")
+        t = None if targets is None else np.ascontiguousarray(targets, np.float32).reshape(a.size, 3)
+        c = s.to_c()
+        r = _lib.TrainResultC()
+        err = _lib.error_buffer()
+        _lib.check(self.lib.blitz_hip_scorer_train_step(
+            self.handle, graph.handle, a.ctypes.data_as(C.POINTER(C.c_uint32)),
+            lab.ctypes.data_as(C.POINTER(C.c_uint8)),
+            None if t is None else t.ctypes.data_as(C.POINTER(C.c_float)), a.size, C.byref(c), C.byref(r), err,
+            len(err)), err)
+        return TrainResult(float(r.loss), float(r.grad_norm), bool(r.applied), int(r.step))
+
+    def gradients(self) -> np.ndarray:
+        out = np.zeros(weight_count(), np.float32)
+        _lib.check(self.lib.blitz_hip_scorer_gradients(self.handle, out.ctypes.data_as(C.POINTER(C.c_float)),
+                                                       out.size))
+        return out
+
+    def weights(self) -> np.ndarray:
+        out = np.zeros(weight_count(), np.float32)
+        _lib.check(self.lib.blitz_hip_scorer_weights(self.handle, out.ctypes.data_as(C.POINTER(C.c_float)), out.size))
+        return out
+
+    def reset_optimizer(self) -> None:
+        _lib.check(self.lib.blitz_hip_scorer_reset_optimizer(self.handle))
+
+    def last_step_ms(self) -> float:
+        ms = C.c_double()
+        _lib.check(self.lib.blitz_hip_scorer_last_step_ms(self.handle, C.byref(ms)))
+        return ms.value
+
+
+# Label bits of a training batch entry.
+VALID, PREFERRED, PLACED = 1, 2, 4
+
+
+@dataclass
+class TrainSettings:
+    """Loss and AdamW settings (defaults of training/update_cuda.hpp:47-53)."""
+    margin: float = 1.0
+    penalty: float = 1e-4
+    lr: float = 1e-3
+    decay: float = 1e-4
+    max_norm: float = 1.0
+    placement_weight: float = 1.0
+    pool: int = 32
+    precision: int = BF16
+
+    def to_c(self) -> "_lib.TrainSettingsC":
+        return _lib.TrainSettingsC(self.margin, self.penalty, self.lr, self.decay, self.max_norm,
+                                   self.placement_weight, self.pool, self.precision)
+
+
+@dataclass
+class TrainResult:
+    loss: float
+    grad_norm: float
+    applied: bool
+    step: int

@@ -246,6 +246,52 @@ blitz_status blitz_hip_scorer_decode(blitz_hip_scorer*, blitz_hip_graph*, const 
                                      double* placements, char* error, size_t capacity);
 blitz_status blitz_hip_scorer_last_forward_ms(const blitz_hip_scorer*, double* ms);
 
+/* ---- scorer training ----------------------------------------------------- */
+
+/* One optimizer step of the scorer on a batch of labelled actions: forward
+ * with every activation kept (FP32 compensated or BF16 MFMA), loss, FP32
+ * backward, global-norm clip and AdamW on the FP32 master weights, then the
+ * BF16 copy refreshed. Takes the place of the reference's update step
+ * (training/update_cuda.hpp:47-53 settings, training/update_cuda.cu:256-393
+ * loss, clip and AdamW).
+ *
+ * Batch: `count` action ids of the graph in pools of `pool` consecutive
+ * entries. labels[i]: bit 0 valid, bit 1 preferred (ranking target within
+ * its pool), bit 2 placement target present; targets: 3 floats per entry
+ * (null when no entry is placed). Loss: pairwise hinge on the logit between
+ * preferred and other valid entries of a pool / pairs, logit L2 penalty x
+ * penalty / valid / 3, smooth-L1 placement x placement_weight / (3 placed). */
+typedef struct blitz_hip_train_settings {
+    float margin;           /* 1 */
+    float penalty;          /* 1e-4 */
+    float lr;               /* 1e-3, in (0, 1] */
+    float decay;            /* 1e-4 (decoupled weight decay) */
+    float max_norm;         /* 1 (global gradient norm clip) */
+    float placement_weight; /* 1 */
+    uint32_t pool;          /* 32, 1..256 */
+    uint8_t precision;      /* BLITZ_HIP_PRECISION_BF16 */
+} blitz_hip_train_settings;
+
+typedef struct blitz_hip_train_result {
+    float loss;
+    float grad_norm;  /* before clipping */
+    uint8_t applied;  /* 0 when loss or gradient norm is not finite (no update) */
+    uint32_t step;    /* optimizer steps applied so far */
+} blitz_hip_train_result;
+
+void blitz_hip_train_settings_init(blitz_hip_train_settings*);
+blitz_status blitz_hip_scorer_train_step(blitz_hip_scorer*, blitz_hip_graph*, const uint32_t* actions,
+                                         const uint8_t* labels, const float* targets, size_t count,
+                                         const blitz_hip_train_settings*, blitz_hip_train_result*, char* error,
+                                         size_t capacity);
+/* Gradient of the last step (before clipping) and current FP32 weights,
+ * blitz_hip_scorer_weight_count() floats each. */
+blitz_status blitz_hip_scorer_gradients(blitz_hip_scorer*, float* out, size_t capacity);
+blitz_status blitz_hip_scorer_weights(blitz_hip_scorer*, float* out, size_t capacity);
+/* Clears the Adam moments and the step counter. */
+blitz_status blitz_hip_scorer_reset_optimizer(blitz_hip_scorer*);
+blitz_status blitz_hip_scorer_last_step_ms(const blitz_hip_scorer*, double* ms);
+
 #ifdef __cplusplus
 }
 #endif
