@@ -7,8 +7,9 @@ views at 1024 x 1024 samples (screen 120 px, supersample 8), limit 1e3 so
 every view is measured. Meshes are resident in HBM before timing starts.
 
 Prints one JSON line with the throughput, the dominant kernel's roofline
-(HIP-event kernel times from the library's own stream, algorithmic bytes as
-documented in DESIGN.md) and a CPU baseline timed on a bounded sample of the
+(HIP-event kernel times from the library's own stream; algorithmic bytes:
+geometry once, one bit per sample per mask, 8 B per sample of distance
+transform) and a CPU baseline timed on a bounded sample of the
 same workload on this host.
 """
 from __future__ import annotations
@@ -24,6 +25,8 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
+BF16_MFMA_PEAK_TFLOPS = 2500.0  # dense
+FP32_PEAK_TFLOPS = 157.3
 METRIC = "audited collapses/s & ms per LOD chain at ≤1px error; raster audit % HBM roof"
 
 
@@ -43,8 +46,7 @@ def workload():
 
 
 def algorithmic_bytes(ref, cand, views: int, samples: int, distance_views: int) -> dict:
-    """Bytes an ideal audit must move ('Roofline accounting'):
-    geometry read once (12 B per vertex and per face of each mesh), one bit
+    """Bytes an ideal audit must move: geometry read once (12 B per vertex and per face of each mesh), one bit
     per sample written for each of the two masks per view, and for each view
     needing the distance transform two directions of 16-bit row distances
     written and read (8 B per sample)."""
@@ -74,6 +76,84 @@ def cpu_baseline(ref, cand, center, radius, settings, sample_views: int):
                       f"scaled to 64 views"}, sample, m
 
 
+def scorer_leg(steps: int) -> dict:
+    """Scorer forward on the 500 x 200 noisy torus (100,000 V / 600,000
+    directed actions), FP32 compensated and BF16 MFMA, and one training step
+    on an 8,192-action batch; kernel time from the library's HIP events."""
+    import numpy as np
+    from blitzremesher_amd import meshes
+    from blitzremesher_amd.audit import bounds
+    from blitzremesher_amd.scorer import BF16, FP32, Scorer, TrainSettings, he_weights
+    mesh = meshes.torus(1.0, 0.3, 500, 200, 0.01, 3)
+    c, r = bounds(mesh)
+    out = {"workload": "noisy torus R=1 r=0.3 500x200 (100,000 V / 200,000 F / 600,000 directed actions), "
+                       "32 features, 3 MP layers x 128, edge MLP 256-128-4, He-normal seed 4"}
+    with Scorer(he_weights(4)) as s:
+        g = s.graph(mesh, c, r)
+        nv, na = g.vertices, g.actions
+        flops = 2 * nv * (64 * 128 + 256 * 128 + 256 * 128) + 2 * na * (256 * 128 + 128 * 4)
+        out["flops_per_forward"] = flops
+        for name, p in (("fp32", FP32), ("bf16", BF16)):
+            for _ in range(2):
+                s.forward(g, p, download=False)
+            ms = []
+            for _ in range(steps):
+                s.forward(g, p, download=False)
+                ms.append(s.last_forward_ms())
+            t = float(np.median(ms))
+            peak = BF16_MFMA_PEAK_TFLOPS if p == BF16 else FP32_PEAK_TFLOPS
+            out[name] = {"ms": round(t, 4), "actions_per_s": round(na / (t * 1e-3), 1),
+                         "tflops": round(flops / (t * 1e-3) / 1e12, 2), "peak_tflops": peak,
+                         "frac": round(flops / (t * 1e-3) / 1e12 / peak, 4)}
+        rng = np.random.default_rng(5)
+        batch = 8192
+        ids = rng.choice(na, batch, replace=False).astype(np.uint32)
+        valid = rng.random(batch) < 0.85
+        pref = valid & (rng.random(batch) < 0.25)
+        labels = (valid * 1 | pref * 2 | (rng.random(batch) < 0.5) * 4).astype(np.uint8)
+        targets = rng.uniform(-1, 1, (batch, 3)).astype(np.float32)
+        settings = TrainSettings(pool=32, precision=BF16)
+        for _ in range(2):
+            s.train_step(g, ids, labels, targets, settings)
+        ms = []
+        for _ in range(steps):
+            s.train_step(g, ids, labels, targets, settings)
+            ms.append(s.last_step_ms())
+        t = float(np.median(ms))
+        out["train_bf16"] = {"batch": batch, "ms_per_step": round(t, 4), "edges_per_s": round(batch / (t * 1e-3), 1),
+                             "note": "full-graph forward (all 100,000 vertices), batch edge layer, FP32 backward, "
+                                     "AdamW"}
+    return out
+
+
+def copy_probe(mib: int) -> int:
+    """Runs the counter-calibration copy kernel (see profiles/) and prints its
+    bandwidth."""
+    import ctypes as C
+    from blitzremesher_amd.build import COPY_PROBE
+    lib = C.CDLL(str(COPY_PROBE))
+    ms = C.c_double()
+    rc = lib.blitz_copy_probe(0, C.c_size_t(mib << 20), 5, C.byref(ms))
+    if rc:
+        print(f"copy probe failed ({rc})", file=sys.stderr)
+        return 1
+    print(json.dumps({"copy_bytes": mib << 20, "ms": ms.value,
+                      "gbs": round(2 * (mib << 20) / (ms.value * 1e-3) / 1e9, 1)}), flush=True)
+    return 0
+
+
+def measured_traffic() -> int | None:
+    """HBM bytes per raster launch from the committed counter summary
+    (FETCH_SIZE + WRITE_SIZE scaled by the copy-kernel calibration), if any."""
+    path = ROOT / "profiles" / "raster_band_kernel_heightfield512_counters.md"
+    if not path.exists():
+        return None
+    for line in path.read_text().splitlines():
+        if line.startswith("traffic_bytes_per_launch:"):
+            return int(line.split(":", 1)[1].strip())
+    return None
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -81,7 +161,12 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cpu-sample-views", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-scorer", action="store_true")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="only run the counter-calibration copy of this many MiB")
     args = ap.parse_args()
+    if args.copy_probe_mib:
+        return copy_probe(args.copy_probe_mib)
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} (WORLD_SIZE {world}) is not supported: every box here "
@@ -134,6 +219,7 @@ def main() -> int:
             cpu, sample, cpu_m = cpu_baseline(ref, cand, center, radius, settings, args.cpu_sample_views)
             gpu_sample = audit.evaluate(dref, dcand, center, radius, sample)
             parity = gpu_sample.as_tuple() == cpu_m.as_tuple()
+    scorer = None if args.no_scorer else scorer_leg(max(3, min(args.steps, 10)))
 
     line = {
         "metric": METRIC,
@@ -148,7 +234,7 @@ def main() -> int:
                    "views": views, "samples_per_view": samples, "reference_triangles": ref.triangles,
                    "candidate_triangles": cand.triangles},
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": measured_traffic(),
                      "algorithmic_bytes_per_launch": per_launch_bytes,
                      "kernel_ms_per_launch": {k: round(v, 4) for k, v in kernels.items()}},
         "audit_roofline": {"algorithmic_bytes": b["audit"],
@@ -157,6 +243,7 @@ def main() -> int:
         "measurement": m.as_tuple() if m else None,
         "cpu_baseline": cpu,
         "parity_on_cpu_sample": parity,
+        "scorer": scorer,
     }
     print(json.dumps(line), flush=True)
     return 0

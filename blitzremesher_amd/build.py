@@ -14,6 +14,8 @@ from pathlib import Path
 PACKAGE = Path(__file__).resolve().parent
 ROOT = PACKAGE.parent
 LIBRARY = PACKAGE / "libblitz_hip.so"
+# Counter calibration probe (a plain copy kernel), loaded only by bench.py.
+COPY_PROBE = PACKAGE / "calib" / "libblitz_copy_probe.so"
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
@@ -46,6 +48,11 @@ def build_library(verbose: bool = False) -> Path:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
     os.replace(str(LIBRARY) + ".tmp", LIBRARY)
+    probe = [HIPCC, *HIP_FLAGS, "-o", str(COPY_PROBE) + ".tmp", str(PACKAGE / "calib" / "copy_probe.hip")]
+    if verbose:
+        print(" ".join(probe), flush=True)
+    subprocess.run(probe, check=True)
+    os.replace(str(COPY_PROBE) + ".tmp", COPY_PROBE)
     return LIBRARY
 
 

@@ -64,22 +64,53 @@ struct Setup {
     int x0, x1, y0, y1;
 };
 
+// Pointers read from task descriptors are generic; loads through them would
+// be flat loads, which wait on LDS and memory counters alike. Every buffer
+// here is device memory, so they are read through global pointers.
+template <class T> using global_ptr = const __attribute__((address_space(1))) T*;
+template <class T> __device__ __forceinline__ global_ptr<T> global(const T* p) {
+    return (global_ptr<T>)p;
+}
+
+// Everything one face's setup reads: its projected corners and flags. Loaded
+// one batch ahead of use (the indices two batches ahead) so the gathers of the
+// next batch are in flight while the current one is rasterized.
+struct Corners {
+    double2 a, b, c;
+    bool behind, two;  // a corner at z <= 0 (perspective); face drawn two-sided
+};
+
+__device__ __forceinline__ uint3 load_face(const RasterTask& t, uint32_t f) {
+    const global_ptr<uint32_t> tri = global(t.indices) + 3 * size_t(f);
+    return make_uint3(tri[0], tri[1], tri[2]);
+}
+
+__device__ __forceinline__ Corners load_corners(const RasterTask& t, uint3 tri, uint32_t f) {
+    Corners k;
+    k.behind = false;
+    if (t.z) {
+        const global_ptr<double> z = global(t.z);
+        k.behind = (z[tri.x] <= 0) | (z[tri.y] <= 0) | (z[tri.z] <= 0);
+    }
+    const global_ptr<double> xy = global(reinterpret_cast<const double*>(t.xy));
+    k.a = make_double2(xy[2 * size_t(tri.x)], xy[2 * size_t(tri.x) + 1]);
+    k.b = make_double2(xy[2 * size_t(tri.y)], xy[2 * size_t(tri.y) + 1]);
+    k.c = make_double2(xy[2 * size_t(tri.z)], xy[2 * size_t(tri.z) + 1]);
+    k.two = t.force_two || (t.two_sided && global(t.two_sided)[f]);
+    return k;
+}
+
 // Triangle setup of src/evaluate.cpp:144-165 on projected vertices.
 // Returns 0 = nothing to draw, 1 = draw, 2 = out of frame (the reference stops
 // at the first such face).
-__device__ __forceinline__ int setup_triangle(const RasterTask& t, uint32_t f, Setup& s, bool& behind) {
-    const uint32_t ia = t.indices[3 * f], ib = t.indices[3 * f + 1], ic = t.indices[3 * f + 2];
-    if (t.z && (t.z[ia] <= 0 || t.z[ib] <= 0 || t.z[ic] <= 0)) {
-        behind = true;
-        return 0;
-    }
-    const double2 a = t.xy[ia];
-    double2 d = t.xy[ib], e = t.xy[ic];
+__device__ __forceinline__ int setup_triangle(const RasterTask& t, const Corners& k, Setup& s) {
+    if (k.behind) return 0;
+    const double2 a = k.a;
+    double2 d = k.b, e = k.c;
     const double area = edge(a.x, a.y, d.x, d.y, e.x, e.y);
     if (fabs(area) < 1e-16) return 0;
     const bool back = area < 0;
-    const bool two = t.force_two || (t.two_sided && t.two_sided[f]);
-    if (back && !two) return 0;
+    if (back && !k.two) return 0;
     if (back) {
         const double2 tmp = d;
         d = e;
@@ -134,10 +165,70 @@ __device__ __forceinline__ Setup shuffle_setup(const Setup& s, int lane) {
     return o;
 }
 
-// Triangles whose bounding box spans at most 2x2 tiles and few samples are
-// rasterized by their own lane; larger ones are walked tile by tile by the
-// whole wavefront, one sample per lane, with a ballot producing the word.
-constexpr int kLaneSamples = 24;
+// Faces whose bounding box is at most 8x8 samples with few samples are
+// rasterized by their own lane; other faces within 8x8 samples take one
+// whole-wave pass (lane = sample, ballot = bitmap); larger ones are walked
+// tile by tile by the whole wavefront, with a ballot producing each word.
+#ifndef BLITZ_LANE_SAMPLES
+#define BLITZ_LANE_SAMPLES 24  // build-time tuning override
+#endif
+constexpr int kLaneSamples = BLITZ_LANE_SAMPLES;
+
+// Lane path: walks the bounding box rows [ya, yb] sample by sample in one
+// flattened loop (lanes of a wave finish together after max(w*h) steps) and
+// collects coverage in an 8x8 bitmap relative to (x0, y0), bit
+// 8*(y-y0) + (x-x0). Per sample, only the x-dependent part of each edge
+// function is evaluated; the y-dependent products are recomputed when the
+// row changes. Every value is the reference expression's own rounding:
+// u = (cx-bx)*(py-by) - (cy-by)*(px-bx), covered iff !(u+r0 < 0 || ...).
+__device__ __forceinline__ uint64_t lane_bitmap(const Setup& s, int ya, int yb) {
+    const double au = s.cx - s.bx, bu = s.cy - s.by;
+    const double av = s.ax - s.cx, bv = s.ay - s.cy;
+    const double aw = s.bx - s.ax, bw = s.by - s.ay;
+    const int w = s.x1 - s.x0 + 1, n = w * (yb - ya + 1);
+    double py = double(ya) + .5;
+    double ru = au * (py - s.by), rv = av * (py - s.cy), rw = aw * (py - s.ay);
+    int x = s.x0, idx = (ya - s.y0) * 8;
+    uint32_t lo = 0, hi = 0;
+    for (int k = 0; k < n; ++k) {
+        const double px = double(x) + .5;
+        const double u = ru - bu * (px - s.bx);
+        const double v = rv - bv * (px - s.cx);
+        const double q = rw - bw * (px - s.ax);
+        // Bitwise | so that all three tests issue without branches.
+        const bool in = !((u + s.r0 < 0) | (v + s.r1 < 0) | (q + s.r2 < 0));
+        const uint32_t bit = uint32_t(in) << (idx & 31);
+        if (idx < 32)
+            lo |= bit;
+        else
+            hi |= bit;
+        ++x;
+        ++idx;
+        if (x > s.x1) {
+            x = s.x0;
+            idx += 8 - w;
+            py += 1.0;
+            ru = au * (py - s.by);
+            rv = av * (py - s.cy);
+            rw = aw * (py - s.ay);
+        }
+    }
+    return (uint64_t(hi) << 32) | lo;
+}
+
+// Moves an 8x8 bitmap anchored at sample (x0, y0) into the up-to-four 8x8
+// tile words it overlaps: word[0] tile (x0>>3, y0>>3), word[1] the tile to
+// its right, word[2] below, word[3] below-right.
+__device__ __forceinline__ void scatter_bitmap(uint64_t local, int x0, int y0, uint64_t word[4]) {
+    constexpr uint64_t kColumn0 = 0x0101010101010101ull;
+    const int ox = x0 & 7, oy = y0 & 7;
+    const uint64_t left = (local & (kColumn0 * ((1u << (8 - ox)) - 1))) << ox;
+    const uint64_t right = ox ? (local >> (8 - ox)) & (kColumn0 * ((1u << ox) - 1)) : 0;
+    word[0] = left << (8 * oy);
+    word[1] = right << (8 * oy);
+    word[2] = oy ? left >> (64 - 8 * oy) : 0;
+    word[3] = oy ? right >> (64 - 8 * oy) : 0;
+}
 
 // One block per (task, band, face chunk). Blocks are mapped so that the
 // blocks of one task land on one XCD (hardware dispatches block w to XCD
@@ -162,40 +253,76 @@ __global__ __launch_bounds__(kRasterThreads) void raster_band_kernel(const Raste
     const uint32_t per = (t.faces + grid.chunks - 1) / grid.chunks;
     const uint32_t f0 = chunk * per, f1 = min(t.faces, f0 + per);
     const int lane = threadIdx.x & 63;
-    for (uint32_t base = f0; base < f1; base += blockDim.x) {
+    const uint32_t step = blockDim.x;
+    // Two-deep pipeline: corners of batch i+1 and indices of batch i+2 load
+    // while batch i is rasterized. Loads past the chunk re-read its last face
+    // (unconditional loads keep the pipeline free of divergent branches).
+    const uint32_t last = f1 > f0 ? f1 - 1 : f0;
+    const auto clamp_face = [&](uint32_t f) { return min(f, last); };
+    Corners next{};
+    uint3 tri_next = make_uint3(0, 0, 0);
+    if (f0 < f1) {
+        next = load_corners(t, load_face(t, clamp_face(f0 + threadIdx.x)), clamp_face(f0 + threadIdx.x));
+        tri_next = load_face(t, clamp_face(f0 + step + threadIdx.x));
+    }
+#pragma unroll 2
+    for (uint32_t base = f0; base < f1; base += step) {
         const uint32_t f = base + threadIdx.x;
-        Setup s{};
-        bool large = false;
+        const Corners corners = next;
+        next = load_corners(t, tri_next, clamp_face(f + step));
+        tri_next = load_face(t, clamp_face(f + 2 * step));
+        Setup s;
+        bool medium = false, large = false;
         if (f < f1) {
-            bool behind = false;
-            const int kind = setup_triangle(t, f, s, behind);
-            if (behind) t.status[1] = 1u;
+            const int kind = setup_triangle(t, corners, s);
+            if (corners.behind) t.status[1] = 1u;
             if (kind == 2) {
                 atomicMin(&t.status[0], f);
             } else if (kind == 1 && s.y1 >= ymin && s.y0 <= ymax) {
                 const int w = s.x1 - s.x0 + 1, h = s.y1 - s.y0 + 1;
                 if (w <= 8 && h <= 8 && w * h <= kLaneSamples) {
-                    // At most 2x2 tiles: accumulate four words in registers.
-                    uint64_t word[4] = {0, 0, 0, 0};
+                    // At most 2x2 tiles: an 8x8 local bitmap, then four words.
                     const int tx0 = s.x0 >> 3, ty = s.y0 >> 3;
                     const int ya = max(s.y0, ymin), yb = min(s.y1, ymax);
-                    for (int y = ya; y <= yb; ++y)
-                        for (int x = s.x0; x <= s.x1; ++x)
-                            if (covers(s, x, y))
-                                word[(((y >> 3) - ty) << 1) | ((x >> 3) - tx0)] |= uint64_t(1)
-                                                                                  << (((y & 7) << 3) | (x & 7));
+                    uint64_t word[4];
+                    scatter_bitmap(lane_bitmap(s, ya, yb), s.x0, s.y0, word);
+#pragma unroll
                     for (int k = 0; k < 4; ++k)
                         if (word[k]) {
                             const uint32_t row = uint32_t(ty + (k >> 1)) - ty0;
                             atomicOr(&band[row * t.tiles + uint32_t(tx0 + (k & 1))],
                                      static_cast<unsigned long long>(word[k]));
                         }
+                } else if (w <= 8 && h <= 8) {
+                    medium = true;
                 } else {
                     large = true;
                 }
             }
         }
-        uint64_t pending = __ballot(large);
+        // Medium faces (box within 8x8 samples): one wave pass each, lane l
+        // testing sample (x0 + l%8, y0 + l/8); the ballot is the face's local
+        // bitmap in the lane path's layout.
+        uint64_t pending = __ballot(medium);
+        while (pending) {
+            const int src = __ffsll(static_cast<unsigned long long>(pending)) - 1;
+            pending &= pending - 1;
+            const Setup q = shuffle_setup(s, src);
+            const int lx = lane & 7, ly = lane >> 3, x = q.x0 + lx, y = q.y0 + ly;
+            const bool in = x <= q.x1 && y <= q.y1 && y >= ymin && y <= ymax && covers(q, x, y);
+            const uint64_t local = __ballot(in);
+            if (lane < 4 && local) {
+                uint64_t word[4];
+                scatter_bitmap(local, q.x0, q.y0, word);
+                const uint64_t mine = lane == 0 ? word[0] : lane == 1 ? word[1] : lane == 2 ? word[2] : word[3];
+                if (mine) {
+                    const uint32_t row = uint32_t((q.y0 >> 3) + (lane >> 1)) - ty0;
+                    atomicOr(&band[row * t.tiles + uint32_t((q.x0 >> 3) + (lane & 1))],
+                             static_cast<unsigned long long>(mine));
+                }
+            }
+        }
+        pending = __ballot(large);
         while (pending) {
             const int src = __ffsll(static_cast<unsigned long long>(pending)) - 1;
             pending &= pending - 1;

@@ -12,6 +12,8 @@
 // training/update_cuda.cu:338-393 operation for operation.
 #include <hip/hip_runtime.h>
 
+#include <stdexcept>
+
 #include "train_kernels.hpp"
 
 namespace blitz_hip {
@@ -139,21 +141,28 @@ __global__ __launch_bounds__(256) void gemm_a_w(const float* __restrict__ a, con
         }
 }
 
-// Column sums of (A masked): bias gradients.
+// Column sums of (A masked): bias gradients. 256 threads cover 256 / cols
+// rows of `cols` consecutive floats per step (coalesced); each block folds a
+// slice of rows and adds its column partials atomically.
 __global__ __launch_bounds__(256) void column_sum(const float* __restrict__ a, const float* __restrict__ a_mask,
-                                                  float* __restrict__ out, uint32_t rows, uint32_t cols) {
-    const uint32_t c = blockIdx.x;
+                                                  float* __restrict__ out, uint32_t rows, uint32_t cols,
+                                                  uint32_t rows_per_block) {
+    const uint32_t c = threadIdx.x % cols, step = blockDim.x / cols;
+    const uint32_t r0 = blockIdx.x * rows_per_block, r1 = min(rows, r0 + rows_per_block);
     float s = 0.f;
-    for (uint32_t r = threadIdx.x; r < rows; r += 256) {
+    for (uint32_t r = r0 + threadIdx.x / cols; r < r1; r += step) {
         float v = a[size_t(r) * cols + c];
         if (a_mask && !(a_mask[size_t(r) * cols + c] > 0.f)) v = 0.f;
         s += v;
     }
-    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
-    __shared__ float part[4];
-    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = s;
+    __shared__ float part[256];
+    part[threadIdx.x] = s;
     __syncthreads();
-    if (threadIdx.x == 0) out[c] += part[0] + part[1] + part[2] + part[3];
+    if (threadIdx.x < cols) {
+        float t = 0.f;
+        for (uint32_t k = threadIdx.x; k < blockDim.x; k += cols) t += part[k];
+        atomicAdd(&out[threadIdx.x], t);
+    }
 }
 
 // Endpoints of the batch's actions.
@@ -255,7 +264,9 @@ void launch_gemm_a_w(const float* a, const float* a_mask, const float* w, float*
 void launch_column_sum(const float* a, const float* a_mask, float* out, uint32_t rows, uint32_t cols,
                        hipStream_t stream) {
     if (!rows || !cols) return;
-    column_sum<<<dim3(cols), dim3(256), 0, stream>>>(a, a_mask, out, rows, cols);
+    if (cols > 256 || 256 % cols) throw std::invalid_argument("column_sum: cols must divide 256");
+    const uint32_t per = 1024;
+    column_sum<<<dim3((rows + per - 1) / per), dim3(256), 0, stream>>>(a, a_mask, out, rows, cols, per);
 }
 
 void launch_batch_endpoints(const uint32_t* actions, GraphView g, uint32_t* src, uint32_t* dst, uint32_t count,
