@@ -52,7 +52,10 @@ def algorithmic_bytes(ref, cand, views: int, samples: int, distance_views: int) 
     written and read (8 B per sample)."""
     geometry = 12 * (ref.positions.shape[0] + ref.triangles) + 12 * (cand.positions.shape[0] + cand.triangles)
     masks = views * 2 * samples // 8
-    return {"geometry": geometry, "masks": masks, "raster": geometry + masks,
+    # One projection per distinct vertex stream and view: 16 B read, 16 B written per vertex.
+    shared = ref.positions.shape == cand.positions.shape and bool((ref.positions == cand.positions).all())
+    project = 32 * views * (ref.positions.shape[0] + (0 if shared else cand.positions.shape[0]))
+    return {"geometry": geometry, "masks": masks, "raster": geometry + masks, "project": project,
             "distance": distance_views * 8 * samples, "audit": geometry + 2 * masks + distance_views * 8 * samples}
 
 
@@ -189,14 +192,17 @@ def main() -> int:
         for _ in range(args.warmup):
             m = audit.evaluate(dref, dcand, center, radius, settings)
         sync()
-        raster_ms = compare_ms = distance_ms = 0.0
-        launches = 0
+        raster_ms = compare_ms = distance_ms = band_ms = project_ms = 0.0
+        launches = band_launches = 0
         distance_views = 0
         t0 = time.perf_counter()
         for _ in range(args.steps):
             m = audit.evaluate(dref, dcand, center, radius, settings)
             t = audit.last_timing()
             raster_ms += t["raster_ms"]
+            band_ms += t["band_ms"]
+            project_ms += t["project_ms"]
+            band_launches += t["band_launches"]
             compare_ms += t["compare_ms"]
             distance_ms += t["distance_ms"]
             launches += t["passes"]
@@ -207,10 +213,14 @@ def main() -> int:
         samples = (int(-(-(settings.screen_size + 8) // 1)) * settings.supersample) ** 2
         views = settings.orthographic + settings.perspective
         b = algorithmic_bytes(ref, cand, views, samples, views)
-        kernels = {"raster": raster_ms / launches, "compare": compare_ms / launches,
-                   "distance": distance_ms / launches}
+        # Device time per launch of each kernel of the audit (HIP events around
+        # the launches on the library's stream); the raster pass also holds
+        # the mask clears and the status read-back.
+        kernels = {"raster_band_kernel": band_ms / band_launches, "project_kernel": project_ms / band_launches,
+                   "compare_kernel": compare_ms / launches, "distance_kernels": distance_ms / launches}
         dominant = max(kernels, key=kernels.get)
-        per_launch_bytes = {"raster": b["raster"], "compare": 2 * b["masks"], "distance": b["distance"]}[dominant]
+        per_launch_bytes = {"raster_band_kernel": b["raster"], "project_kernel": b["project"],
+                            "compare_kernel": 2 * b["masks"], "distance_kernels": b["distance"]}[dominant]
         achieved = per_launch_bytes / (kernels[dominant] * 1e-3) / 1e9
         gpu_sample = None
         cpu = None
@@ -236,7 +246,8 @@ def main() -> int:
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": measured_traffic(),
                      "algorithmic_bytes_per_launch": per_launch_bytes,
-                     "kernel_ms_per_launch": {k: round(v, 4) for k, v in kernels.items()}},
+                     "kernel_ms_per_launch": {k: round(v, 4) for k, v in kernels.items()},
+                     "raster_pass_ms": round(raster_ms / launches, 4)},
         "audit_roofline": {"algorithmic_bytes": b["audit"],
                            "achieved_gbs": round(b["audit"] / (ms_per_step * 1e-3) / 1e9, 2),
                            "frac": round(b["audit"] / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 5)},

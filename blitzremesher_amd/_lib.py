@@ -102,6 +102,7 @@ class AuditTiming(C.Structure):
         ("passes", C.c_uint32),
         ("raster_ms", C.c_double), ("compare_ms", C.c_double), ("distance_ms", C.c_double),
         ("total_ms", C.c_double), ("raster_triangles", C.c_uint64), ("mask_bytes", C.c_uint64),
+        ("project_ms", C.c_double), ("band_ms", C.c_double), ("band_launches", C.c_uint32),
     ]
 
 

@@ -421,13 +421,24 @@ void Audit::rasterize(const std::vector<RasterJob>& jobs, const std::vector<Came
     hip_check(hipMemcpyAsync(dp, projections.data(), projections.size() * sizeof(ProjectTask), hipMemcpyHostToDevice,
                              stream_),
               "upload projections");
+    hip_check(hipEventRecord(events_[5], stream_), "event");
     launch_project(dp, uint32_t(projections.size()), max_vertices, stream_);
+    hip_check(hipEventRecord(events_[6], stream_), "event");
     launch(tasks, 4);
+    hip_check(hipEventRecord(events_[7], stream_), "event");
     hip_check(hipGetLastError(), "raster launch");
     status_out.assign(jobs.size() * 2, 0);
     hip_check(hipMemcpyAsync(status_out.data(), status, status_out.size() * 4, hipMemcpyDeviceToHost, stream_),
               "download status");
     hip_check(hipStreamSynchronize(stream_), "raster");
+    {
+        float t_p = 0, t_b = 0;
+        hip_check(hipEventElapsedTime(&t_p, events_[5], events_[6]), "elapsed");
+        hip_check(hipEventElapsedTime(&t_b, events_[6], events_[7]), "elapsed");
+        timing_.project_ms += t_p;
+        timing_.band_ms += t_b;
+        ++timing_.band_launches;
+    }
     timing_.raster_tasks += uint32_t(jobs.size());
     for (const auto& t : tasks) timing_.raster_triangles += t.faces;
     timing_.mask_bytes += uint64_t(jobs.size()) * words * 8;

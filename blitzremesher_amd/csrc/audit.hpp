@@ -75,6 +75,10 @@ struct Timing {
     uint32_t raster_tasks = 0, distance_views = 0, passes = 0;
     double raster_ms = 0, compare_ms = 0, distance_ms = 0, total_ms = 0;
     uint64_t raster_triangles = 0, mask_bytes = 0;
+    // Device time of the projection and band-raster launches alone (events
+    // immediately around each launch).
+    double project_ms = 0, band_ms = 0;
+    uint32_t band_launches = 0;
 };
 
 class Audit {
@@ -131,7 +135,7 @@ class Audit {
     int device_;
     uint64_t memory_bytes_;
     hipStream_t stream_{};
-    hipEvent_t events_[5]{};
+    hipEvent_t events_[8]{};
     std::vector<std::pair<void*, size_t>> slots_;
     uint64_t next_serial_ = 1;
     Timing timing_;

@@ -422,6 +422,9 @@ blitz_status blitz_hip_audit_last_timing(const blitz_hip_audit* a, blitz_hip_aud
     out->total_ms = t.total_ms;
     out->raster_triangles = t.raster_triangles;
     out->mask_bytes = t.mask_bytes;
+    out->project_ms = t.project_ms;
+    out->band_ms = t.band_ms;
+    out->band_launches = t.band_launches;
     return BLITZ_OK;
 }
 

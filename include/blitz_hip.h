@@ -114,6 +114,9 @@ typedef struct blitz_hip_audit_timing {
     double raster_ms, compare_ms, distance_ms, total_ms;
     uint64_t raster_triangles; /* triangle-view setups issued */
     uint64_t mask_bytes;       /* coverage mask bytes written */
+    double project_ms;         /* projection launches alone (device time) */
+    double band_ms;            /* band-raster launches alone (device time) */
+    uint32_t band_launches;
 } blitz_hip_audit_timing;
 
 /* ---- audit ---------------------------------------------------------------- */
