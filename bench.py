@@ -129,6 +129,43 @@ def scorer_leg(steps: int) -> dict:
     return out
 
 
+def patch_leg(ref, center, radius, settings, count: int, repeats: int) -> dict:
+    """The raster-audit workload's local patches: `count` seeded BFS regions
+    of 16-200 faces of the heightfield, each re-meshed by legal interior
+    collapses, audited against the bound heightfield over the same 64 views
+    in one batch (dirty tiles re-rasterized, exact EDT)."""
+    import numpy as np
+    from blitzremesher_amd import Audit
+    from blitzremesher_amd.audit import local_patches
+    t0 = time.perf_counter()
+    patches = local_patches(ref, count, 0xB1172026, 16, 200)
+    generate_s = time.perf_counter() - t0
+    removed = int(patches.removed_offsets[-1])
+    added = int(patches.added_offsets[-1])
+    with Audit(0) as audit:
+        t0 = time.perf_counter()
+        binding = audit.bind(ref, center, radius, settings)
+        bind_ms = 1000 * (time.perf_counter() - t0)
+        audit.evaluate_patches(binding, patches, settings)  # warm-up
+        times, kernel = [], []
+        for _ in range(repeats):
+            t0 = time.perf_counter()
+            ms, fallbacks = audit.evaluate_patches(binding, patches, settings)
+            times.append(time.perf_counter() - t0)
+            kernel.append(audit.last_timing()["patch_ms"])
+        t = float(np.median(times))
+    # An interior collapse removes the two faces of its edge.
+    collapses = (removed - added) // 2
+    return {"workload": f"{count} local patches (BFS regions of 16-200 faces, re-meshed by legal interior "
+                        "collapses) of the 512x512 heightfield, 64 views at 1024^2, one batch",
+            "patches": count, "removed_faces": removed, "added_faces": added, "collapses": collapses,
+            "bind_ms": round(bind_ms, 2), "batch_ms": round(1000 * t, 3),
+            "patch_kernel_ms": round(float(np.median(kernel)), 3),
+            "patch_audits_per_s": round(count / t, 1), "audited_collapses_per_s": round(collapses / t, 1),
+            "full_path_fallbacks": fallbacks, "passed": int(sum(m.passed for m in ms)),
+            "generate_s": round(generate_s, 2)}
+
+
 def copy_probe(mib: int) -> int:
     """Runs the counter-calibration copy kernel (see profiles/) and prints its
     bandwidth."""
@@ -165,6 +202,8 @@ def main() -> int:
     ap.add_argument("--cpu-sample-views", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-scorer", action="store_true")
+    ap.add_argument("--no-patches", action="store_true")
+    ap.add_argument("--patches", type=int, default=4096)
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run the counter-calibration copy of this many MiB")
     args = ap.parse_args()
@@ -177,7 +216,9 @@ def main() -> int:
         return 2
 
     from blitzremesher_amd import Audit, _lib
+    import numpy as np
     ref, cand, center, radius, settings = workload()
+    collapses_full = int(np.unique(ref.indices).size - np.unique(cand.indices).size)
     # torch is not imported: its wheel bundles a second HIP runtime, and two
     # HIP runtimes in one process cannot share the device. The library's own
     # hipDeviceSynchronize brackets the timed region instead.
@@ -230,6 +271,8 @@ def main() -> int:
             gpu_sample = audit.evaluate(dref, dcand, center, radius, sample)
             parity = gpu_sample.as_tuple() == cpu_m.as_tuple()
     scorer = None if args.no_scorer else scorer_leg(max(3, min(args.steps, 10)))
+    patches = None if args.no_patches else patch_leg(ref, center, radius, settings, args.patches,
+                                                       max(3, min(args.steps, 10)))
 
     line = {
         "metric": METRIC,
@@ -248,12 +291,16 @@ def main() -> int:
                      "algorithmic_bytes_per_launch": per_launch_bytes,
                      "kernel_ms_per_launch": {k: round(v, 4) for k, v in kernels.items()},
                      "raster_pass_ms": round(raster_ms / launches, 4)},
+        # Each collapse of the candidate removes one vertex from use.
+        "collapses_per_audit": collapses_full,
+        "audited_collapses_per_s": round(collapses_full * 1000.0 / ms_per_step, 1),
         "audit_roofline": {"algorithmic_bytes": b["audit"],
                            "achieved_gbs": round(b["audit"] / (ms_per_step * 1e-3) / 1e9, 2),
                            "frac": round(b["audit"] / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 5)},
         "measurement": m.as_tuple() if m else None,
         "cpu_baseline": cpu,
         "parity_on_cpu_sample": parity,
+        "patches": patches,
         "scorer": scorer,
     }
     print(json.dumps(line), flush=True)

@@ -103,6 +103,7 @@ class AuditTiming(C.Structure):
         ("raster_ms", C.c_double), ("compare_ms", C.c_double), ("distance_ms", C.c_double),
         ("total_ms", C.c_double), ("raster_triangles", C.c_uint64), ("mask_bytes", C.c_uint64),
         ("project_ms", C.c_double), ("band_ms", C.c_double), ("band_launches", C.c_uint32),
+        ("patch_fallbacks", C.c_uint32), ("patch_ms", C.c_double),
     ]
 
 
@@ -129,6 +130,17 @@ PROTOTYPES = {
                                                   C.POINTER(MeasurementC), *_ERR]),
     "blitz_hip_audit_evaluate_batch": (C.c_int, [_P, _P, C.POINTER(_P), C.c_size_t, C.POINTER(Bounds),
                                                  C.POINTER(EvalSettingsC), C.POINTER(MeasurementC), *_ERR]),
+    "blitz_hip_audit_bind": (C.c_int, [_P, C.POINTER(BlitzMesh), C.POINTER(Bounds), C.POINTER(EvalSettingsC),
+                                       C.POINTER(_P), *_ERR]),
+    "blitz_hip_binding_release": (None, [_P]),
+    "blitz_hip_audit_evaluate_patches": (C.c_int, [_P, _P, C.c_size_t, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
+                                                   C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
+                                                   C.POINTER(EvalSettingsC), C.POINTER(MeasurementC),
+                                                   C.POINTER(C.c_uint32), *_ERR]),
+    "blitz_hip_local_patches": (C.c_int, [C.POINTER(BlitzMesh), C.c_uint32, C.c_uint64, C.c_uint32, C.c_uint32,
+                                          C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.c_size_t,
+                                          C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.c_size_t,
+                                          C.POINTER(C.c_size_t), C.POINTER(C.c_size_t), *_ERR]),
     "blitz_hip_raster_coverage": (C.c_int, [_P, _P, C.POINTER(Bounds), C.POINTER(Camera), C.c_double, C.c_uint8,
                                             C.c_uint8, C.POINTER(C.c_uint64), C.c_size_t, C.POINTER(C.c_uint32),
                                             C.POINTER(C.c_uint8), *_ERR]),

@@ -195,6 +195,75 @@ class DeviceMesh:
             pass
 
 
+@dataclass
+class Patches:
+    """Local patches of a mesh as CSR arrays (blitz_hip_local_patches):
+    patch i removes faces removed[removed_offsets[i]:removed_offsets[i+1]]
+    (ascending) and appends added[added_offsets[i]:added_offsets[i+1]]
+    (n x 3 vertex ids)."""
+    removed_offsets: np.ndarray
+    removed: np.ndarray
+    added_offsets: np.ndarray
+    added: np.ndarray
+
+    def __len__(self) -> int:
+        return int(self.removed_offsets.size - 1)
+
+    def subset(self, ids) -> "Patches":
+        ro, rv, ao, av = [0], [], [0], []
+        for i in ids:
+            r = self.removed[self.removed_offsets[i]:self.removed_offsets[i + 1]]
+            a = self.added[self.added_offsets[i]:self.added_offsets[i + 1]]
+            rv.append(r)
+            av.append(a)
+            ro.append(ro[-1] + r.size)
+            ao.append(ao[-1] + a.shape[0])
+        return Patches(np.asarray(ro, np.uint32), np.concatenate(rv).astype(np.uint32) if rv else
+                       np.zeros(0, np.uint32), np.asarray(ao, np.uint32),
+                       np.concatenate(av).astype(np.uint32).reshape(-1, 3) if av else np.zeros((0, 3), np.uint32))
+
+    def candidate(self, mesh: Mesh, i: int) -> Mesh:
+        """The candidate mesh of patch i: mesh faces not in the region, in
+        order, then the patch's new faces."""
+        keep = np.ones(mesh.triangles, bool)
+        keep[self.removed[self.removed_offsets[i]:self.removed_offsets[i + 1]]] = False
+        idx = np.concatenate([mesh.indices[keep], self.added[self.added_offsets[i]:self.added_offsets[i + 1]]])
+        return Mesh(mesh.positions, idx, None, mesh.double_sided)
+
+
+def local_patches(mesh: Mesh, count: int, seed: int, min_faces: int = 16, max_faces: int = 200) -> Patches:
+    """Seeded breadth-first face regions re-meshed by legal interior collapses."""
+    lib = _lib.load()
+    desc = mesh.descriptor()
+    nr, na = C.c_size_t(), C.c_size_t()
+    err = _lib.error_buffer()
+    _lib.check(lib.blitz_hip_local_patches(C.byref(desc), count, seed, min_faces, max_faces, None, None, 0, None,
+                                           None, 0, C.byref(nr), C.byref(na), err, len(err)), err)
+    ro = np.zeros(count + 1, np.uint32)
+    ao = np.zeros(count + 1, np.uint32)
+    rv = np.zeros(max(1, nr.value), np.uint32)
+    av = np.zeros((max(1, na.value), 3), np.uint32)
+    u32 = C.POINTER(C.c_uint32)
+    _lib.check(lib.blitz_hip_local_patches(C.byref(desc), count, seed, min_faces, max_faces, ro.ctypes.data_as(u32),
+                                           rv.ctypes.data_as(u32), rv.size, ao.ctypes.data_as(u32),
+                                           av.ctypes.data_as(u32), av.shape[0], C.byref(nr), C.byref(na), err,
+                                           len(err)), err)
+    return Patches(ro, rv[:nr.value], ao, av[:na.value])
+
+
+class Binding:
+    """A reference bound for patch audits (blitz_hip_audit_bind)."""
+
+    def __init__(self, audit: "Audit", handle, mesh: Mesh, center, radius, settings: "EvalSettings"):
+        self.audit, self.handle, self.mesh = audit, handle, mesh
+        self.center, self.radius, self.settings = center, radius, settings
+
+    def release(self) -> None:
+        if self.handle:
+            _lib.load().blitz_hip_binding_release(self.handle)
+            self.handle = None
+
+
 class Audit:
     """One HIP audit object (AuditCuda counterpart) on one device."""
 
@@ -210,8 +279,12 @@ class Audit:
         _lib.check(lib.blitz_hip_audit_create(C.byref(opts), C.byref(handle), err, len(err)), err)
         self.handle = handle
         self._meshes: list[DeviceMesh] = []
+        self._bindings: list[Binding] = []
 
     def close(self) -> None:
+        for b in self._bindings:
+            b.release()
+        self._bindings.clear()
         for m in self._meshes:
             m.release()
         self._meshes.clear()
@@ -265,6 +338,42 @@ class Audit:
         _lib.check(st, err)
         return [Measurement.from_c(out[i]) for i in range(n)]
 
+    def bind(self, mesh: Mesh, center, radius, settings: EvalSettings) -> Binding:
+        """Binds a reference for patch audits (per-view masks and tile bins in HBM)."""
+        s, keep = settings.to_c()
+        b = _bounds_c(center, radius)
+        desc = mesh.descriptor()
+        handle = C.c_void_p()
+        err = _lib.error_buffer()
+        st = self.lib.blitz_hip_audit_bind(self.handle, C.byref(desc), C.byref(b), C.byref(s), C.byref(handle), err,
+                                           len(err))
+        del keep
+        _lib.check(st, err)
+        binding = Binding(self, handle, mesh, center, radius, settings)
+        self._bindings.append(binding)
+        return binding
+
+    def evaluate_patches(self, binding: Binding, patches: Patches, settings: EvalSettings | None = None):
+        """Measurements of every patch against the bound reference, and the
+        number of patches that took the full audit path."""
+        s, keep = (settings or binding.settings).to_c()
+        n = len(patches)
+        u32 = C.POINTER(C.c_uint32)
+        ro = np.ascontiguousarray(patches.removed_offsets, np.uint32)
+        rv = np.ascontiguousarray(patches.removed, np.uint32)
+        ao = np.ascontiguousarray(patches.added_offsets, np.uint32)
+        av = np.ascontiguousarray(patches.added, np.uint32).reshape(-1)
+        out = (_lib.MeasurementC * max(1, n))()
+        fallbacks = C.c_uint32()
+        err = _lib.error_buffer()
+        st = self.lib.blitz_hip_audit_evaluate_patches(
+            self.handle, binding.handle, n, ro.ctypes.data_as(u32), rv.ctypes.data_as(u32) if rv.size else None,
+            ao.ctypes.data_as(u32), av.ctypes.data_as(u32) if av.size else None, C.byref(s), out,
+            C.byref(fallbacks), err, len(err))
+        del keep
+        _lib.check(st, err)
+        return [Measurement.from_c(out[i]) for i in range(n)], int(fallbacks.value)
+
     def raster_coverage(self, mesh: DeviceMesh, center, radius, camera: _lib.Camera, screen: float, ss: int,
                         force_two_sided: bool = False):
         """Row-major packed mask words, raster side and clipped flag; None when
@@ -303,5 +412,5 @@ class Audit:
 
 
 __all__ = ["Mesh", "EvalSettings", "Measurement", "Audit", "DeviceMesh", "bounds", "cameras", "reduce_reuse",
-           "BlitzError",
+           "BlitzError", "Binding", "Patches", "local_patches",
            "PROFILE_COVERAGE", "PROFILE_NORMALS", "PROFILE_ATTRIBUTES"]

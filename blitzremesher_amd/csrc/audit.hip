@@ -196,7 +196,6 @@ std::unique_ptr<DeviceMesh> Audit::upload(const HostMesh& m) {
 
 void Audit::release(DeviceMesh*) {}
 
-namespace {
 // Coverage-profile view finish (src/evaluate.cpp:271-298).
 Measurement finish_view(double coverage, uint64_t changed, uint64_t total, bool clipped, const Settings& s, uint8_t ss) {
     Measurement cur;
@@ -219,7 +218,6 @@ bool settled(const Measurement& cur, const Settings& s, unsigned ss) {
            cur.coverage - 2 * std::sqrt(2.0) / ss > s.limit || !std::isfinite(cur.error) ||
            cur.coverage_upper <= s.limit;
 }
-}  // namespace
 
 Measurement Audit::evaluate(const DeviceMesh& reference, const DeviceMesh& candidate, const Bounds& b,
                             const Settings& s) {
@@ -406,15 +404,7 @@ void Audit::rasterize(const std::vector<RasterJob>& jobs, const std::vector<Came
                   "upload raster tasks");
         uint32_t faces = 0;
         for (const auto& t : list) faces = std::max(faces, t.faces);
-        RasterGrid g{};
-        g.tasks = uint32_t(list.size());
-        g.band_rows = std::max<uint32_t>(1, std::min<uint32_t>(tiles, kBandWords / tiles));
-        g.bands = (tiles + g.band_rows - 1) / g.band_rows;
-        // Up to 32 face chunks per task (one per CU of an XCD, so an XCD works
-        // on one view at a time and its L2 holds that view's projected
-        // vertices), each with at least 8 batches of 1024 faces.
-        g.chunks = std::max<uint32_t>(1, std::min<uint32_t>(32, faces / (8 * kRasterThreads)));
-        launch_raster(d, g, tiles, stream_);
+        launch_raster(d, make_raster_grid(uint32_t(list.size()), tiles, faces), tiles, stream_);
     };
     hip_check(hipMemcpyAsync(status, init.data(), init.size() * 4, hipMemcpyHostToDevice, stream_), "upload status");
     ProjectTask* dp = scratch_as<ProjectTask>(8, std::max<size_t>(1, projections.size()));

@@ -70,6 +70,11 @@ Bounds bounds_of(const float* xyz, size_t count);
 std::vector<Camera> cameras(const Bounds& b, double screen, uint16_t ortho, uint16_t persp, uint32_t seed);
 void validate_settings(const Bounds& b, const Settings& s);
 uint32_t raster_side(double screen, unsigned ss);  // ceil(screen + 8) * ss
+// One view's coverage-profile result (src/evaluate.cpp:271-298) and the
+// refinement stop rule (src/evaluate.cpp:336-337).
+Measurement finish_view(double coverage, uint64_t changed, uint64_t total, bool clipped, const Settings& s,
+                        uint8_t ss);
+bool settled(const Measurement& cur, const Settings& s, unsigned ss);
 
 struct Timing {
     uint32_t raster_tasks = 0, distance_views = 0, passes = 0;
@@ -79,6 +84,20 @@ struct Timing {
     // immediately around each launch).
     double project_ms = 0, band_ms = 0;
     uint32_t band_launches = 0;
+    double patch_ms = 0;      // patch kernel (device time)
+    uint32_t patch_fallbacks = 0;
+};
+
+// A reference bound for patch audits (patch_audit.hpp).
+struct Binding;
+// Patches as CSR arrays: removed source face ids (ascending) and added faces
+// (3 vertex ids each) of every patch.
+struct PatchSet {
+    const uint32_t* removed_offsets = nullptr;  // count + 1
+    const uint32_t* removed = nullptr;
+    const uint32_t* added_offsets = nullptr;    // count + 1, in faces
+    const uint32_t* added = nullptr;
+    size_t count = 0;
 };
 
 class Audit {
@@ -99,6 +118,14 @@ class Audit {
                           bool force_two, std::vector<uint64_t>& words, uint32_t& size, bool& clipped);
     double coverage_distance(const std::vector<uint64_t>& a, const std::vector<uint64_t>& b, uint32_t size,
                              uint8_t ss);
+    // Binds a reference: per view its projection, mask, covered count and
+    // per-tile face bins at s.supersample (src/neural/internal.hpp:141-144).
+    std::unique_ptr<Binding> bind(const HostMesh& reference, const Bounds& b, const Settings& s);
+    // out[i] equals evaluate(reference, candidate_i), candidate_i being the
+    // reference with patch i applied; `fallbacks` receives how many patches
+    // took the full audit path.
+    std::vector<Measurement> evaluate_patches(Binding& binding, const PatchSet& patches, const Settings& s,
+                                              uint32_t* fallbacks);
     const Timing& last_timing() const { return timing_; }
     int device() const { return device_; }
     hipStream_t stream() const { return stream_; }

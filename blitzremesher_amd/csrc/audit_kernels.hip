@@ -13,9 +13,12 @@
 #include <hip/hip_runtime.h>
 
 #include "audit_kernels.hpp"
+#include "raster_device.hpp"
 
 namespace blitz_hip {
 namespace {
+
+using namespace device;
 
 // src/evaluate.cpp:138-141: x = dot(p - c, right) * scale * ss + size/2;
 // z = distance - dot(p - c, forward); scale = focal / z in perspective.
@@ -39,132 +42,6 @@ __global__ __launch_bounds__(256) void project_kernel(const ProjectTask* __restr
     if (t.z) t.z[i] = z;
 }
 
-// src/evaluate.cpp:21
-__device__ __forceinline__ double edge(double ax, double ay, double bx, double by, double x, double y) {
-    return (bx - ax) * (y - ay) - (by - ay) * (x - ax);
-}
-
-// std::min / std::max over an initializer list: first smallest (largest)
-// element under operator<, which matters only for NaN inputs.
-__device__ __forceinline__ double min3(double a, double b, double c) {
-    double m = a;
-    if (b < m) m = b;
-    if (c < m) m = c;
-    return m;
-}
-__device__ __forceinline__ double max3(double a, double b, double c) {
-    double m = a;
-    if (m < b) m = b;
-    if (m < c) m = c;
-    return m;
-}
-
-struct Setup {
-    double ax, ay, bx, by, cx, cy, r0, r1, r2;
-    int x0, x1, y0, y1;
-};
-
-// Pointers read from task descriptors are generic; loads through them would
-// be flat loads, which wait on LDS and memory counters alike. Every buffer
-// here is device memory, so they are read through global pointers.
-template <class T> using global_ptr = const __attribute__((address_space(1))) T*;
-template <class T> __device__ __forceinline__ global_ptr<T> global(const T* p) {
-    return (global_ptr<T>)p;
-}
-
-// Everything one face's setup reads: its projected corners and flags. Loaded
-// one batch ahead of use (the indices two batches ahead) so the gathers of the
-// next batch are in flight while the current one is rasterized.
-struct Corners {
-    double2 a, b, c;
-    bool behind, two;  // a corner at z <= 0 (perspective); face drawn two-sided
-};
-
-__device__ __forceinline__ uint3 load_face(const RasterTask& t, uint32_t f) {
-    const global_ptr<uint32_t> tri = global(t.indices) + 3 * size_t(f);
-    return make_uint3(tri[0], tri[1], tri[2]);
-}
-
-__device__ __forceinline__ Corners load_corners(const RasterTask& t, uint3 tri, uint32_t f) {
-    Corners k;
-    k.behind = false;
-    if (t.z) {
-        const global_ptr<double> z = global(t.z);
-        k.behind = (z[tri.x] <= 0) | (z[tri.y] <= 0) | (z[tri.z] <= 0);
-    }
-    const global_ptr<double> xy = global(reinterpret_cast<const double*>(t.xy));
-    k.a = make_double2(xy[2 * size_t(tri.x)], xy[2 * size_t(tri.x) + 1]);
-    k.b = make_double2(xy[2 * size_t(tri.y)], xy[2 * size_t(tri.y) + 1]);
-    k.c = make_double2(xy[2 * size_t(tri.z)], xy[2 * size_t(tri.z) + 1]);
-    k.two = t.force_two || (t.two_sided && global(t.two_sided)[f]);
-    return k;
-}
-
-// Triangle setup of src/evaluate.cpp:144-165 on projected vertices.
-// Returns 0 = nothing to draw, 1 = draw, 2 = out of frame (the reference stops
-// at the first such face).
-__device__ __forceinline__ int setup_triangle(const RasterTask& t, const Corners& k, Setup& s) {
-    if (k.behind) return 0;
-    const double2 a = k.a;
-    double2 d = k.b, e = k.c;
-    const double area = edge(a.x, a.y, d.x, d.y, e.x, e.y);
-    if (fabs(area) < 1e-16) return 0;
-    const bool back = area < 0;
-    if (back && !k.two) return 0;
-    if (back) {
-        const double2 tmp = d;
-        d = e;
-        e = tmp;
-    }
-    const double minx = min3(a.x, d.x, e.x), maxx = max3(a.x, d.x, e.x);
-    const double miny = min3(a.y, d.y, e.y), maxy = max3(a.y, d.y, e.y);
-    const double size = double(t.size);
-    if (!isfinite(minx) || !isfinite(maxx) || !isfinite(miny) || !isfinite(maxy) || minx < 0 || miny < 0 ||
-        maxx > size || maxy > size)
-        return 2;
-    s.x0 = max(0, int(floor(minx)));
-    s.x1 = min(int(t.size) - 1, int(floor(maxx)));
-    s.y0 = max(0, int(floor(miny)));
-    s.y1 = min(int(t.size) - 1, int(floor(maxy)));
-    s.r0 = .5 * (fabs(e.x - d.x) + fabs(e.y - d.y));
-    s.r1 = .5 * (fabs(a.x - e.x) + fabs(a.y - e.y));
-    s.r2 = .5 * (fabs(d.x - a.x) + fabs(d.y - a.y));
-    s.ax = a.x;
-    s.ay = a.y;
-    s.bx = d.x;
-    s.by = d.y;
-    s.cx = e.x;
-    s.cy = e.y;
-    return 1;
-}
-
-// Conservative coverage predicate at sample (x, y) (src/evaluate.cpp:166-168).
-__device__ __forceinline__ bool covers(const Setup& s, int x, int y) {
-    const double px = x + .5, py = y + .5;
-    const double u = edge(s.bx, s.by, s.cx, s.cy, px, py);
-    const double v = edge(s.cx, s.cy, s.ax, s.ay, px, py);
-    const double w = edge(s.ax, s.ay, s.bx, s.by, px, py);
-    return !(u + s.r0 < 0 || v + s.r1 < 0 || w + s.r2 < 0);
-}
-
-__device__ __forceinline__ Setup shuffle_setup(const Setup& s, int lane) {
-    Setup o;
-    o.ax = __shfl(s.ax, lane);
-    o.ay = __shfl(s.ay, lane);
-    o.bx = __shfl(s.bx, lane);
-    o.by = __shfl(s.by, lane);
-    o.cx = __shfl(s.cx, lane);
-    o.cy = __shfl(s.cy, lane);
-    o.r0 = __shfl(s.r0, lane);
-    o.r1 = __shfl(s.r1, lane);
-    o.r2 = __shfl(s.r2, lane);
-    o.x0 = __shfl(s.x0, lane);
-    o.x1 = __shfl(s.x1, lane);
-    o.y0 = __shfl(s.y0, lane);
-    o.y1 = __shfl(s.y1, lane);
-    return o;
-}
-
 // Faces whose bounding box is at most 8x8 samples with few samples are
 // rasterized by their own lane; other faces within 8x8 samples take one
 // whole-wave pass (lane = sample, ballot = bitmap); larger ones are walked
@@ -173,62 +50,6 @@ __device__ __forceinline__ Setup shuffle_setup(const Setup& s, int lane) {
 #define BLITZ_LANE_SAMPLES 24  // build-time tuning override
 #endif
 constexpr int kLaneSamples = BLITZ_LANE_SAMPLES;
-
-// Lane path: walks the bounding box rows [ya, yb] sample by sample in one
-// flattened loop (lanes of a wave finish together after max(w*h) steps) and
-// collects coverage in an 8x8 bitmap relative to (x0, y0), bit
-// 8*(y-y0) + (x-x0). Per sample, only the x-dependent part of each edge
-// function is evaluated; the y-dependent products are recomputed when the
-// row changes. Every value is the reference expression's own rounding:
-// u = (cx-bx)*(py-by) - (cy-by)*(px-bx), covered iff !(u+r0 < 0 || ...).
-__device__ __forceinline__ uint64_t lane_bitmap(const Setup& s, int ya, int yb) {
-    const double au = s.cx - s.bx, bu = s.cy - s.by;
-    const double av = s.ax - s.cx, bv = s.ay - s.cy;
-    const double aw = s.bx - s.ax, bw = s.by - s.ay;
-    const int w = s.x1 - s.x0 + 1, n = w * (yb - ya + 1);
-    double py = double(ya) + .5;
-    double ru = au * (py - s.by), rv = av * (py - s.cy), rw = aw * (py - s.ay);
-    int x = s.x0, idx = (ya - s.y0) * 8;
-    uint32_t lo = 0, hi = 0;
-    for (int k = 0; k < n; ++k) {
-        const double px = double(x) + .5;
-        const double u = ru - bu * (px - s.bx);
-        const double v = rv - bv * (px - s.cx);
-        const double q = rw - bw * (px - s.ax);
-        // Bitwise | so that all three tests issue without branches.
-        const bool in = !((u + s.r0 < 0) | (v + s.r1 < 0) | (q + s.r2 < 0));
-        const uint32_t bit = uint32_t(in) << (idx & 31);
-        if (idx < 32)
-            lo |= bit;
-        else
-            hi |= bit;
-        ++x;
-        ++idx;
-        if (x > s.x1) {
-            x = s.x0;
-            idx += 8 - w;
-            py += 1.0;
-            ru = au * (py - s.by);
-            rv = av * (py - s.cy);
-            rw = aw * (py - s.ay);
-        }
-    }
-    return (uint64_t(hi) << 32) | lo;
-}
-
-// Moves an 8x8 bitmap anchored at sample (x0, y0) into the up-to-four 8x8
-// tile words it overlaps: word[0] tile (x0>>3, y0>>3), word[1] the tile to
-// its right, word[2] below, word[3] below-right.
-__device__ __forceinline__ void scatter_bitmap(uint64_t local, int x0, int y0, uint64_t word[4]) {
-    constexpr uint64_t kColumn0 = 0x0101010101010101ull;
-    const int ox = x0 & 7, oy = y0 & 7;
-    const uint64_t left = (local & (kColumn0 * ((1u << (8 - ox)) - 1))) << ox;
-    const uint64_t right = ox ? (local >> (8 - ox)) & (kColumn0 * ((1u << ox) - 1)) : 0;
-    word[0] = left << (8 * oy);
-    word[1] = right << (8 * oy);
-    word[2] = oy ? left >> (64 - 8 * oy) : 0;
-    word[3] = oy ? right >> (64 - 8 * oy) : 0;
-}
 
 // One block per (task, band, face chunk). Blocks are mapped so that the
 // blocks of one task land on one XCD (hardware dispatches block w to XCD
@@ -405,38 +226,6 @@ __device__ __forceinline__ bool dense_rows(const ViewSummary& s, uint32_t size) 
     return needs_distance(s) && s.changed > (unsigned long long)size * size / 32;
 }
 
-constexpr uint32_t kFar = 0xFFFFFFFFu;
-
-// Horizontal distance from column x to the nearest site in row y of a
-// tile-major mask, looking at most `reach` columns beyond x's own byte on
-// each side; kFar if there is none that close.
-__device__ __forceinline__ uint32_t nearest_in_row(const uint64_t* mask, uint32_t tiles, uint32_t x, uint32_t y,
-                                                   uint32_t reach) {
-    const uint64_t* row = mask + size_t(y >> 3) * tiles;
-    const uint32_t shift = (y & 7) << 3, tx = x >> 3, bit = x & 7, base = tx << 3;
-    const uint32_t byte = uint32_t(row[tx] >> shift) & 0xFFu;
-    const uint32_t lower = byte & ((2u << bit) - 1), upper = byte & ~((1u << bit) - 1);
-    uint32_t left = lower ? x - (base + 31 - __clz(lower)) : kFar;
-    uint32_t right = upper ? (base + __ffs(upper) - 1) - x : kFar;
-    if (left == kFar)
-        for (int t = int(tx) - 1; t >= 0 && x - (uint32_t(t) * 8 + 7) <= reach; --t) {
-            const uint32_t b = uint32_t(row[t] >> shift) & 0xFFu;
-            if (b) {
-                left = x - (uint32_t(t) * 8 + 31 - __clz(b));
-                break;
-            }
-        }
-    if (right == kFar)
-        for (uint32_t t = tx + 1; t < tiles && t * 8 - x <= reach; ++t) {
-            const uint32_t b = uint32_t(row[t] >> shift) & 0xFFu;
-            if (b) {
-                right = t * 8 + __ffs(b) - 1 - x;
-                break;
-            }
-        }
-    return min(left, right);
-}
-
 // Row pass of the separable EDT: for every sample, the horizontal distance to
 // the nearest site in its row (kNoSite if the row has none). One block per
 // row; each thread owns one byte (8 samples) of the row and block-wide
@@ -485,12 +274,6 @@ __global__ __launch_bounds__(1024) void row_distance_kernel(const ViewPair* __re
     uint16_t* row = out + size_t(y) * p.size + base;
     const uint32_t valid = min(8u, p.size - uint32_t(base));
     for (uint32_t k = 0; k < valid; ++k) row[k] = d[k];
-}
-
-// float(d*d) of the reference's row pass (src/evaluate.cpp:50), as an integer:
-// exact below 2^24 and round-to-nearest-even above.
-__device__ __forceinline__ uint32_t rounded_square(uint32_t d) {
-    return uint32_t(float(d * d));
 }
 
 // Column pass restricted to the samples that can set the directed Hausdorff
@@ -566,6 +349,18 @@ void launch_fill(uint64_t* p, size_t n, uint64_t value, hipStream_t stream) {
 void launch_project(const ProjectTask* tasks, uint32_t count, uint32_t max_vertices, hipStream_t stream) {
     if (!count || !max_vertices) return;
     project_kernel<<<dim3((max_vertices + 255) / 256, count), dim3(256), 0, stream>>>(tasks);
+}
+
+RasterGrid make_raster_grid(uint32_t tasks, uint32_t tiles, uint32_t max_faces) {
+    RasterGrid g{};
+    g.tasks = tasks;
+    g.band_rows = std::max<uint32_t>(1, std::min<uint32_t>(tiles, kBandWords / std::max(1u, tiles)));
+    g.bands = (tiles + g.band_rows - 1) / g.band_rows;
+    // Up to 32 face chunks per task (one per CU of an XCD, so an XCD works on
+    // one view at a time and its L2 holds that view's projected vertices),
+    // each with at least 8 batches of 1024 faces.
+    g.chunks = std::max<uint32_t>(1, std::min<uint32_t>(32, max_faces / (8 * kRasterThreads)));
+    return g;
 }
 
 void launch_raster(const RasterTask* tasks, const RasterGrid& grid, uint32_t tiles, hipStream_t stream) {

@@ -13,6 +13,9 @@ void launch_fill(uint64_t* p, size_t n, uint64_t value, hipStream_t stream);
 void launch_project(const ProjectTask* tasks, uint32_t count, uint32_t max_vertices, hipStream_t stream);
 // tasks: device array of grid.tasks raster tasks sharing one raster side.
 void launch_raster(const RasterTask* tasks, const RasterGrid& grid, uint32_t tiles, hipStream_t stream);
+// Band and chunk split of `tasks` raster tasks of `tiles` x `tiles` words
+// with up to `max_faces` faces each.
+RasterGrid make_raster_grid(uint32_t tasks, uint32_t tiles, uint32_t max_faces);
 // pairs/summaries: device arrays of `count` views sharing one raster side.
 void launch_compare(const ViewPair* pairs, ViewSummary* summaries, uint32_t count, uint32_t tiles,
                     hipStream_t stream);

@@ -13,6 +13,8 @@
 
 #include "../../include/blitz_hip.h"
 #include "audit.hpp"
+#include "patch_audit.hpp"
+#include "patches.hpp"
 #include "scorer.hpp"
 #include "simplify.hpp"
 
@@ -27,6 +29,10 @@ struct blitz_hip_device_mesh {
 };
 struct blitz_hip_scorer {
     std::unique_ptr<Scorer> impl;
+};
+struct blitz_hip_binding {
+    blitz_hip_audit* owner;
+    std::unique_ptr<Binding> impl;
 };
 struct blitz_hip_graph {
     blitz_hip_scorer* owner;
@@ -363,6 +369,74 @@ blitz_status blitz_hip_audit_evaluate_batch(blitz_hip_audit* a, const blitz_hip_
     });
 }
 
+blitz_status blitz_hip_audit_bind(blitz_hip_audit* a, const blitz_mesh* reference, const blitz_hip_bounds* b,
+                                  const blitz_hip_eval_settings* s, blitz_hip_binding** out, char* error,
+                                  size_t capacity) {
+    return guarded(error, capacity, [&] {
+        if (!out) throw std::invalid_argument("null binding output");
+        *out = nullptr;
+        Audit& audit = audit_of(a);
+        const HostMesh h = host_mesh(reference);
+        auto binding = std::make_unique<blitz_hip_binding>();
+        binding->owner = a;
+        binding->impl = audit.bind(h, bounds_from(b), settings_of(s));
+        *out = binding.release();
+    });
+}
+
+void blitz_hip_binding_release(blitz_hip_binding* b) {
+    if (!b) return;
+    if (b->owner && b->owner->impl) {
+        DeviceGuard guard(b->owner->impl->device());
+        b->impl.reset();
+    }
+    delete b;
+}
+
+blitz_status blitz_hip_audit_evaluate_patches(blitz_hip_audit* a, blitz_hip_binding* binding, size_t count,
+                                              const uint32_t* removed_offsets, const uint32_t* removed,
+                                              const uint32_t* added_offsets, const uint32_t* added,
+                                              const blitz_hip_eval_settings* s, blitz_hip_measurement* out,
+                                              uint32_t* fallbacks, char* error, size_t capacity) {
+    return guarded(error, capacity, [&] {
+        if (!binding || !binding->impl || binding->owner != a) throw std::invalid_argument("binding of another audit");
+        if (count && !out) throw std::invalid_argument("null measurement output");
+        Audit& audit = audit_of(a);
+        PatchSet set;
+        set.removed_offsets = removed_offsets;
+        set.removed = removed;
+        set.added_offsets = added_offsets;
+        set.added = added;
+        set.count = count;
+        const auto results = audit.evaluate_patches(*binding->impl, set, settings_of(s), fallbacks);
+        for (size_t i = 0; i < count; ++i) to_c(results[i], out + i);
+    });
+}
+
+blitz_status blitz_hip_local_patches(const blitz_mesh* m, uint32_t count, uint64_t seed, uint32_t min_faces,
+                                     uint32_t max_faces, uint32_t* removed_offsets, uint32_t* removed,
+                                     size_t removed_capacity, uint32_t* added_offsets, uint32_t* added,
+                                     size_t added_capacity, size_t* removed_count, size_t* added_count, char* error,
+                                     size_t capacity) {
+    return guarded(error, capacity, [&] {
+        if (!removed_count || !added_count) throw std::invalid_argument("null size outputs");
+        const HostMesh h = host_mesh(m);
+        const Patches p = local_patches(h.positions.data(), h.positions.size() / 3, h.indices.data(),
+                                        h.indices.size() / 3, count, seed, min_faces, max_faces);
+        *removed_count = p.removed.size();
+        *added_count = p.added.size() / 3;
+        if (!removed_offsets && !removed && !added_offsets && !added) return;
+        if (!removed_offsets || !added_offsets || (!removed && !p.removed.empty()) || (!added && !p.added.empty()))
+            throw std::invalid_argument("null patch output arrays");
+        if (removed_capacity < p.removed.size() || added_capacity < p.added.size() / 3)
+            throw std::invalid_argument("patch output capacity too small");
+        std::memcpy(removed_offsets, p.removed_offsets.data(), p.removed_offsets.size() * 4);
+        std::memcpy(added_offsets, p.added_offsets.data(), p.added_offsets.size() * 4);
+        if (!p.removed.empty()) std::memcpy(removed, p.removed.data(), p.removed.size() * 4);
+        if (!p.added.empty()) std::memcpy(added, p.added.data(), p.added.size() * 4);
+    });
+}
+
 blitz_status blitz_hip_raster_coverage(blitz_hip_audit* a, const blitz_hip_device_mesh* m, const blitz_hip_bounds* b,
                                        const blitz_hip_camera* camera, double screen, uint8_t ss, uint8_t force_two,
                                        uint64_t* words, size_t word_capacity, uint32_t* size_out, uint8_t* clipped,
@@ -425,6 +499,8 @@ blitz_status blitz_hip_audit_last_timing(const blitz_hip_audit* a, blitz_hip_aud
     out->project_ms = t.project_ms;
     out->band_ms = t.band_ms;
     out->band_launches = t.band_launches;
+    out->patch_fallbacks = t.patch_fallbacks;
+    out->patch_ms = t.patch_ms;
     return BLITZ_OK;
 }
 

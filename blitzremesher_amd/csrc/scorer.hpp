@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "audit.hpp"
+#include "device_buffer.hpp"
 #include "scorer_types.hpp"
 
 namespace blitz_hip {
@@ -22,16 +23,6 @@ struct Topology {
 };
 Topology build_topology(const uint32_t* indices, size_t faces, size_t vertices);
 
-struct DeviceBuffer {
-    void* p = nullptr;
-    size_t bytes = 0;
-    DeviceBuffer() = default;
-    DeviceBuffer(const DeviceBuffer&) = delete;
-    DeviceBuffer& operator=(const DeviceBuffer&) = delete;
-    ~DeviceBuffer();
-    void resize(size_t n);
-    template <class T> T* as() const { return static_cast<T*>(p); }
-};
 
 struct Graph {
     uint32_t vertices = 0, actions = 0, faces = 0;

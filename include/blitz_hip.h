@@ -117,6 +117,8 @@ typedef struct blitz_hip_audit_timing {
     double project_ms;         /* projection launches alone (device time) */
     double band_ms;            /* band-raster launches alone (device time) */
     uint32_t band_launches;
+    uint32_t patch_fallbacks;  /* patches audited by the full path */
+    double patch_ms;           /* patch kernel launches (device time) */
 } blitz_hip_audit_timing;
 
 /* ---- audit ---------------------------------------------------------------- */
@@ -175,6 +177,48 @@ blitz_status blitz_hip_audit_evaluate_batch(blitz_hip_audit*, const blitz_hip_de
                                             const blitz_hip_eval_settings*,
                                             blitz_hip_measurement* out, char* error,
                                             size_t capacity);
+
+/* ---- bound reference and local patches ------------------------------------ */
+
+/* A reference bound for patch audits: per view its projection, coverage mask,
+ * covered sample count and the reference faces binned per 8x8 tile, kept in
+ * HBM (AuditCuda::bind_reference and the reference raster memo,
+ * src/neural/internal.hpp:141-144, src/neural/evaluate.cu:386-511). The
+ * reference mesh must not carry per-face materials. */
+typedef struct blitz_hip_binding blitz_hip_binding;
+blitz_status blitz_hip_audit_bind(blitz_hip_audit*, const blitz_mesh* reference, const blitz_hip_bounds*,
+                                  const blitz_hip_eval_settings*, blitz_hip_binding**, char* error,
+                                  size_t capacity);
+void blitz_hip_binding_release(blitz_hip_binding*);
+
+/* Audits `count` local patches of the bound reference in shared launches
+ * (certify_candidates with dirty-tile re-rasterization,
+ * src/neural/evaluate.cu:216-273,2413-2578). Patch i removes the reference
+ * faces removed[removed_offsets[i] .. removed_offsets[i+1]) (ascending ids)
+ * and appends the faces added[3*added_offsets[i] .. 3*added_offsets[i+1])
+ * (vertex ids of the reference); out[i] equals blitz_hip_audit_evaluate of
+ * the reference against that candidate (reference faces minus the region, in
+ * order, then the added faces). settings must have the bind's screen size,
+ * supersample, view set and sidedness; limit, max_changed_area and
+ * max_supersample may differ. *fallbacks (optional) receives the number of
+ * patches audited by the full path (clipped views, refinement, regions over
+ * 512 faces or 1024 dirty tiles). */
+blitz_status blitz_hip_audit_evaluate_patches(blitz_hip_audit*, blitz_hip_binding*, size_t count,
+                                              const uint32_t* removed_offsets, const uint32_t* removed,
+                                              const uint32_t* added_offsets, const uint32_t* added,
+                                              const blitz_hip_eval_settings*, blitz_hip_measurement* out,
+                                              uint32_t* fallbacks, char* error, size_t capacity);
+
+/* Seeded local patches of a mesh (host): `count` breadth-first face regions
+ * of uniform size in [min_faces, max_faces], each re-meshed by legal interior
+ * edge collapses onto existing vertices. Outputs CSR arrays as taken by
+ * blitz_hip_audit_evaluate_patches; *removed_count and *added_count (faces)
+ * receive the sizes needed, and null output arrays only query them. */
+blitz_status blitz_hip_local_patches(const blitz_mesh*, uint32_t count, uint64_t seed, uint32_t min_faces,
+                                     uint32_t max_faces, uint32_t* removed_offsets, uint32_t* removed,
+                                     size_t removed_capacity, uint32_t* added_offsets, uint32_t* added,
+                                     size_t added_capacity, size_t* removed_count, size_t* added_count,
+                                     char* error, size_t capacity);
 
 /* blitz::detail::rasterize_coverage (src/coverage.hpp:12, src/evaluate.cpp:125-189)
  * for one camera: row-major packed words (index = y*size + x), padding bits
