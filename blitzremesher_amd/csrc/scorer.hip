@@ -190,9 +190,13 @@ void Scorer::forward(Graph& g, Precision p) {
     if (p == Precision::Fp32)
         launch_forward_fp32(g.features.as<float>(), g.view(), weights_.as<float>(), g.h_a.as<float>(), g.h_b.as<float>(),
                             g.z.as<float>(), g.outputs.as<float>(), stream_);
-    else
+    else {
+        g.zq.resize(size_t(g.vertices) * 2 * kHidden * 2);
+        g.pq.resize(size_t(g.vertices) * 2 * kHidden * 2);
         launch_forward_bf16(g.features_bf16.as<__bf16>(), g.view(), weights_bf16_.as<__bf16>(), weights_.as<float>(),
-                            g.hq_a.as<__bf16>(), g.hq_b.as<__bf16>(), g.outputs.as<float>(), blocks_, stream_);
+                            g.hq_a.as<__bf16>(), g.hq_b.as<__bf16>(), g.zq.as<__bf16>(), g.pq.as<__bf16>(),
+                            g.outputs.as<float>(), stream_);
+    }
     hip_check(hipEventRecord(end_, stream_), "event");
     hip_check(hipGetLastError(), "forward launch");
     hip_check(hipStreamSynchronize(stream_), "forward");

@@ -28,7 +28,7 @@ struct Graph {
     uint32_t vertices = 0, actions = 0, faces = 0;
     double diameter = 0;
     DeviceBuffer positions, indices, offsets, neighbors, src, reverse, face_offsets, faces_of, boundary;
-    DeviceBuffer features, features_bf16, outputs, h_a, h_b, z, hq_a, hq_b;
+    DeviceBuffer features, features_bf16, outputs, h_a, h_b, z, hq_a, hq_b, zq, pq;
     DeviceBuffer keys, order, keys_out, order_out, sort_temp;
     // Training: activations of every layer (FP32 and BF16), layer inputs,
     // batch buffers and gradients.

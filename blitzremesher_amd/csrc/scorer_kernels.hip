@@ -16,9 +16,13 @@
 //    combined in fixed order with the bias. A CPU restatement reproduces it
 //    bit for bit, so FP32 rankings are reproducible.
 //  * BF16: the fast path. Activations and weights in BF16, products on the
-//    gfx950 matrix cores (v_mfma_f32_16x16x32_bf16) with FP32 accumulation,
-//    aggregation in FP32; fused per 64-row tile (gather -> LDS -> MFMA ->
-//    bias/ReLU epilogue).
+//    gfx950 matrix cores (v_mfma_f32_16x16x32_bf16) with FP32 accumulation.
+//    Each message-passing layer is a high-occupancy aggregation kernel
+//    (neighbour sums in FP32) into a BF16 [h ; mean] buffer followed by a
+//    persistent MFMA GEMM with a bias/ReLU epilogue. The edge layer is
+//    factorised: W_e [h_u ; h_v] = W_e(:, :128) h_u + W_e(:, 128:) h_v, so one
+//    GEMM over the vertices gives both halves and each action only gathers
+//    two 128-vectors, adds, applies ReLU and the 128 -> 4 head in FP32.
 //
 // Vertex features (32, FP64 internally, stored FP32); p = (P - centre) / diameter:
 //   0-2 p | 3-5 unit area-weighted normal | 6-15 area-normalised plane quadric
@@ -32,6 +36,7 @@
 #include <hip/hip_runtime.h>
 
 #include <hipcub/device/device_radix_sort.hpp>
+#include <stdexcept>
 
 #include "scorer_kernels.hpp"
 
@@ -199,135 +204,116 @@ __device__ __forceinline__ floatx4 mfma(bf16x8 a, bf16x8 b, floatx4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
 }
 
-// 64 x 128 tile product of a [64][K] LDS tile and a [128][K] LDS weight tile:
-// wave w computes rows 16w..16w+15, eight 16-wide column blocks.
+// z[v] = [h[v] ; mean of h over N(v)] in BF16 (K = 2 * width): one thread per
+// 8 channels of one vertex, neighbour sums in FP32 in CSR order.
+template <uint32_t W>
+__global__ __launch_bounds__(256) void aggregate_bf16(const __bf16* __restrict__ h, GraphView g,
+                                                      __bf16* __restrict__ z) {
+    constexpr uint32_t groups = W / 8;
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= size_t(g.vertices) * groups) return;
+    const uint32_t v = uint32_t(i / groups), c = uint32_t(i % groups) * 8;
+    const bf16x8 self = *reinterpret_cast<const bf16x8*>(h + size_t(v) * W + c);
+    float acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    const uint32_t b = g.offsets[v], e = g.offsets[v + 1];
+    for (uint32_t k = b; k < e; ++k) {
+        const bf16x8 q = *reinterpret_cast<const bf16x8*>(h + size_t(g.neighbors[k]) * W + c);
+        for (int j = 0; j < 8; ++j) acc[j] += float(q[j]);
+    }
+    const float inv = 1.f / float(e - b > 0 ? e - b : 1u);
+    bf16x8 mean;
+    for (int j = 0; j < 8; ++j) mean[j] = __bf16(acc[j] * inv);
+    *reinterpret_cast<bf16x8*>(z + size_t(v) * 2 * W + c) = self;
+    *reinterpret_cast<bf16x8*>(z + size_t(v) * 2 * W + W + c) = mean;
+}
+
+// C[:, 128 y .. 128 y + 127] = act(A B_y^T + bias) for a row-major BF16
+// A [M][K] and B_y = rows 0..127 of W + y * block_offset (row stride ldw):
+// persistent blocks keep B_y in LDS; each wave owns 16 rows of a 64-row tile
+// and eight 16-column MFMA blocks; the A fragments of the next tile are
+// loaded from HBM while the current tile is multiplied. FP32 accumulation.
 template <uint32_t K>
-__device__ __forceinline__ void tile_product(const __bf16* z, const __bf16* w, floatx4 (&acc)[8]) {
+__global__ __launch_bounds__(256) void gemm_bf16(const __bf16* __restrict__ a, uint32_t m,
+                                                 const __bf16* __restrict__ wq, uint32_t ldw, uint32_t block_offset,
+                                                 const float* __restrict__ bias, bool relu, __bf16* __restrict__ c,
+                                                 uint32_t ldc) {
+    constexpr uint32_t steps = K / 32;
+    extern __shared__ __bf16 smem[];
+    const __bf16* src = wq + size_t(blockIdx.y) * block_offset;
+    for (uint32_t i = threadIdx.x; i < kHidden * (K / 8); i += blockDim.x) {
+        const uint32_t row = i / (K / 8), col = (i % (K / 8)) * 8;
+        *reinterpret_cast<bf16x8*>(smem + row * (K + kPad) + col) =
+            *reinterpret_cast<const bf16x8*>(src + size_t(row) * ldw + col);
+    }
+    __syncthreads();
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const uint32_t r = wave * 16 + (lane & 15), kq = (lane >> 4) * 8;
-    for (int nb = 0; nb < 8; ++nb) acc[nb] = floatx4{0, 0, 0, 0};
-#pragma unroll 2
-    for (uint32_t k0 = 0; k0 < K; k0 += 32) {
-        const bf16x8 a = *reinterpret_cast<const bf16x8*>(z + r * (K + kPad) + k0 + kq);
+    const uint32_t rl = wave * 16 + (lane & 15), kq = (lane >> 4) * 8;
+    const uint32_t tiles = (m + kTileRows - 1) / kTileRows;
+    bf16x8 next[steps];
+    auto load_a = [&](uint32_t tile) {
+        const uint32_t row = tile * kTileRows + rl;
+#pragma unroll
+        for (uint32_t s = 0; s < steps; ++s)
+            next[s] = row < m ? *reinterpret_cast<const bf16x8*>(a + size_t(row) * K + s * 32 + kq) : bf16x8{};
+    };
+    if (blockIdx.x < tiles) load_a(blockIdx.x);
+    for (uint32_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
+        bf16x8 cur[steps];
+#pragma unroll
+        for (uint32_t s = 0; s < steps; ++s) cur[s] = next[s];
+        if (tile + gridDim.x < tiles) load_a(tile + gridDim.x);
+        floatx4 acc[8];
+#pragma unroll
+        for (int nb = 0; nb < 8; ++nb) acc[nb] = floatx4{0, 0, 0, 0};
+#pragma unroll
+        for (uint32_t s = 0; s < steps; ++s)
+#pragma unroll
+            for (int nb = 0; nb < 8; ++nb) {
+                const bf16x8 b =
+                    *reinterpret_cast<const bf16x8*>(smem + (nb * 16 + (lane & 15)) * (K + kPad) + s * 32 + kq);
+                acc[nb] = mfma(cur[s], b, acc[nb]);
+            }
 #pragma unroll
         for (int nb = 0; nb < 8; ++nb) {
-            const bf16x8 b = *reinterpret_cast<const bf16x8*>(w + (nb * 16 + (lane & 15)) * (K + kPad) + k0 + kq);
-            acc[nb] = mfma(a, b, acc[nb]);
-        }
-    }
-}
-
-template <uint32_t K>
-__device__ __forceinline__ void load_weights_bf16(const __bf16* __restrict__ src, __bf16* w) {
-    // [128][K] row-major into LDS rows of K + kPad.
-    for (uint32_t i = threadIdx.x; i < kHidden * K / 8; i += blockDim.x) {
-        const uint32_t row = i / (K / 8), col = (i % (K / 8)) * 8;
-        *reinterpret_cast<bf16x8*>(w + row * (K + kPad) + col) = *reinterpret_cast<const bf16x8*>(src + row * K + col);
-    }
-}
-
-// One message-passing layer: z = [h ; mean h(N)] (K = 2 * width), then
-// ReLU(z W^T + b) into h_out (BF16). Persistent blocks keep W in LDS.
-template <uint32_t K>
-__global__ __launch_bounds__(256) void mp_layer_bf16(const __bf16* __restrict__ h, GraphView g,
-                                                     const __bf16* __restrict__ wq, const float* __restrict__ bias,
-                                                     __bf16* __restrict__ out) {
-    constexpr uint32_t width = K / 2;
-    extern __shared__ __bf16 smem[];
-    __bf16* w = smem;
-    __bf16* z = smem + kHidden * (K + kPad);
-    load_weights_bf16<K>(wq, w);
-    const uint32_t tiles = (g.vertices + kTileRows - 1) / kTileRows;
-    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    for (uint32_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
-        __syncthreads();
-        // Gather: thread t builds 8 channels of one row at a time.
-        for (uint32_t i = threadIdx.x; i < kTileRows * (width / 8); i += blockDim.x) {
-            const uint32_t r = i / (width / 8), c = (i % (width / 8)) * 8;
-            const uint32_t v = tile * kTileRows + r;
-            bf16x8 self{}, mean{};
-            if (v < g.vertices) {
-                self = *reinterpret_cast<const bf16x8*>(h + size_t(v) * width + c);
-                float acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-                const uint32_t b = g.offsets[v], e = g.offsets[v + 1];
-                for (uint32_t k = b; k < e; ++k) {
-                    const bf16x8 q = *reinterpret_cast<const bf16x8*>(h + size_t(g.neighbors[k]) * width + c);
-                    for (int j = 0; j < 8; ++j) acc[j] += float(q[j]);
-                }
-                const float inv = 1.f / float(e - b > 0 ? e - b : 1u);
-                for (int j = 0; j < 8; ++j) mean[j] = __bf16(acc[j] * inv);
-            }
-            *reinterpret_cast<bf16x8*>(z + r * (K + kPad) + c) = self;
-            *reinterpret_cast<bf16x8*>(z + r * (K + kPad) + width + c) = mean;
-        }
-        __syncthreads();
-        floatx4 acc[8];
-        tile_product<K>(z, w, acc);
-        for (int nb = 0; nb < 8; ++nb) {
             const uint32_t col = nb * 16 + (lane & 15);
-            const float b = bias[col];
+            const float bv = bias ? bias[col] : 0.f;
+#pragma unroll
             for (int i = 0; i < 4; ++i) {
-                const uint32_t v = tile * kTileRows + wave * 16 + (lane >> 4) * 4 + i;
-                if (v < g.vertices) out[size_t(v) * kHidden + col] = __bf16(fmaxf(acc[nb][i] + b, 0.f));
+                const uint32_t row = tile * kTileRows + wave * 16 + (lane >> 4) * 4 + i;
+                float v = acc[nb][i] + bv;
+                if (relu) v = fmaxf(v, 0.f);
+                if (row < m) c[size_t(row) * ldc + blockIdx.y * kHidden + col] = __bf16(v);
             }
         }
     }
 }
 
-// Edge MLP over actions [first, first + count): z = [h(src) ; h(dst)],
-// g = ReLU(z We^T + be) (MFMA), out = g Wo^T + bo (FP32, four outputs).
-__global__ __launch_bounds__(256) void edge_mlp_bf16(const __bf16* __restrict__ h, GraphView g,
-                                                     const __bf16* __restrict__ wq, const float* __restrict__ bias,
-                                                     const float* __restrict__ wo, const float* __restrict__ bo,
-                                                     float* __restrict__ out, float* __restrict__ hidden_out) {
-    constexpr uint32_t K = kEdgeIn;
-    extern __shared__ __bf16 smem[];
-    __bf16* w = smem;
-    __bf16* z = smem + kHidden * (K + kPad);
-    float* hidden = reinterpret_cast<float*>(z + kTileRows * (K + kPad));  // [64][kHidden + 1]
-    float* head = hidden + kTileRows * (kHidden + 1);                       // [4][kHidden] + 4
-    load_weights_bf16<K>(wq, w);
-    for (uint32_t i = threadIdx.x; i < kOutputs * kHidden; i += blockDim.x) head[i] = wo[i];
-    if (threadIdx.x < kOutputs) head[kOutputs * kHidden + threadIdx.x] = bo[threadIdx.x];
-    const uint32_t tiles = (g.actions + kTileRows - 1) / kTileRows;
-    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    for (uint32_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
-        __syncthreads();
-        for (uint32_t i = threadIdx.x; i < kTileRows * (K / 8); i += blockDim.x) {
-            const uint32_t r = i / (K / 8), c = (i % (K / 8)) * 8;
-            const uint32_t a = tile * kTileRows + r;
-            bf16x8 q{};
-            if (a < g.actions) {
-                const uint32_t v = c < kHidden ? g.src[a] : g.neighbors[a];
-                q = *reinterpret_cast<const bf16x8*>(h + size_t(v) * kHidden + (c % kHidden));
-            }
-            *reinterpret_cast<bf16x8*>(z + r * (K + kPad) + c) = q;
-        }
-        __syncthreads();
-        floatx4 acc[8];
-        tile_product<K>(z, w, acc);
-        for (int nb = 0; nb < 8; ++nb) {
-            const uint32_t col = nb * 16 + (lane & 15);
-            const float b = bias[col];
-            for (int i = 0; i < 4; ++i) {
-                const uint32_t r = wave * 16 + (lane >> 4) * 4 + i;
-                hidden[r * (kHidden + 1) + col] = fmaxf(acc[nb][i] + b, 0.f);
-            }
-        }
-        __syncthreads();
+// Edge MLP in factorised form: with p = h3 [W_e(:, :128) | W_e(:, 128:)]^T
+// precomputed per vertex, g(a) = ReLU(p_src(a) + p_dst(a) + b_e) and
+// out(a) = W_o g + b_o (FP32). Sixteen lanes per action, eight channels each.
+__global__ __launch_bounds__(256) void edge_bf16(const __bf16* __restrict__ p, GraphView g,
+                                                 const float* __restrict__ be, const float* __restrict__ wo,
+                                                 const float* __restrict__ bo, float* __restrict__ out,
+                                                 float* __restrict__ hidden_out) {
+    const uint32_t lane = threadIdx.x & 15, c = lane * 8;
+    const uint32_t a = (blockIdx.x * blockDim.x + threadIdx.x) >> 4;
+    const bool live = a < g.actions;
+    float part[kOutputs] = {0, 0, 0, 0};
+    if (live) {
+        const uint32_t u = g.src[a], v = g.neighbors[a];
+        const bf16x8 pa = *reinterpret_cast<const bf16x8*>(p + size_t(u) * 2 * kHidden + c);
+        const bf16x8 pb = *reinterpret_cast<const bf16x8*>(p + size_t(v) * 2 * kHidden + kHidden + c);
+        float hid[8];
+        for (int j = 0; j < 8; ++j) hid[j] = fmaxf(float(pa[j]) + float(pb[j]) + be[c + j], 0.f);
         if (hidden_out)
-            for (uint32_t i = threadIdx.x; i < kTileRows * kHidden; i += blockDim.x) {
-                const uint32_t r = i / kHidden, col = i % kHidden, a = tile * kTileRows + r;
-                if (a < g.actions) hidden_out[size_t(a) * kHidden + col] = hidden[r * (kHidden + 1) + col];
-            }
-        {
-            const uint32_t r = threadIdx.x / kOutputs, o = threadIdx.x % kOutputs;
-            const uint32_t a = tile * kTileRows + r;
-            float s = 0.f;
-            for (uint32_t j = 0; j < kHidden; ++j) s += hidden[r * (kHidden + 1) + j] * head[o * kHidden + j];
-            if (a < g.actions) out[size_t(a) * kOutputs + o] = s + head[kOutputs * kHidden + o];
-        }
+            for (int j = 0; j < 8; ++j) hidden_out[size_t(a) * kHidden + c + j] = hid[j];
+        for (uint32_t o = 0; o < kOutputs; ++o)
+            for (int j = 0; j < 8; ++j) part[o] += hid[j] * wo[o * kHidden + c + j];
     }
+    for (uint32_t o = 0; o < kOutputs; ++o)
+        for (int off = 8; off > 0; off >>= 1) part[o] += __shfl_xor(part[o], off);
+    if (live && lane == 0)
+        for (uint32_t o = 0; o < kOutputs; ++o) out[size_t(a) * kOutputs + o] = part[o] + bo[o];
 }
 
 __global__ void to_bf16(const float* __restrict__ x, __bf16* __restrict__ y, size_t n) {
@@ -431,10 +417,8 @@ void launch_forward_fp32(const float* x, GraphView g, const float* weights, floa
             g.actions, kHidden, kOutputs, false);
 }
 
-size_t forward_bf16_lds(uint32_t k) {
-    return size_t(kHidden + kTileRows) * (k + kPad) * 2 +
-           (k == kEdgeIn ? (kTileRows * (kHidden + 1) + kOutputs * kHidden + kOutputs) * 4 : 0);
-}
+// LDS of gemm_bf16<K>: the 128 x K weight block with row padding.
+size_t gemm_bf16_lds(uint32_t k) { return size_t(kHidden) * (k + kPad) * 2; }
 
 void launch_convert_bf16(const float* x, __bf16* y, size_t n, hipStream_t stream) {
     if (!n) return;
@@ -458,45 +442,60 @@ void launch_compensated(const float* input, const uint32_t* src, const uint32_t*
 static void configure_bf16() {
     static bool configured = false;
     if (configured) return;
-    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&mp_layer_bf16<2 * kFeatures>),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, int(forward_bf16_lds(2 * kFeatures)));
-    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&mp_layer_bf16<2 * kHidden>),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, int(forward_bf16_lds(2 * kHidden)));
-    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&edge_mlp_bf16),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, int(forward_bf16_lds(kEdgeIn)));
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&gemm_bf16<2 * kFeatures>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, int(gemm_bf16_lds(2 * kFeatures)));
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&gemm_bf16<2 * kHidden>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, int(gemm_bf16_lds(2 * kHidden)));
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&gemm_bf16<kHidden>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, int(gemm_bf16_lds(kHidden)));
     configured = true;
 }
 
+// Two persistent 256-thread blocks per CU.
+constexpr uint32_t kGemmBlocks = 512;
+
 void launch_mp_bf16(int layer, const __bf16* h, GraphView g, const __bf16* wq, const float* weights, __bf16* out,
-                    uint32_t blocks, hipStream_t stream) {
+                    __bf16* z, hipStream_t stream) {
     configure_bf16();
     if (!g.vertices) return;
-    const uint32_t vb = std::min(blocks, (g.vertices + kTileRows - 1) / kTileRows);
+    if (!h || !wq || !out || !z) throw std::invalid_argument("null buffer for a BF16 layer");
+    const uint32_t tiles = (g.vertices + kTileRows - 1) / kTileRows;
+    const dim3 grid(std::min(kGemmBlocks, tiles), 1);
     const float* bias = weights + layer_offset(layer) + kHidden * kLayerIn[layer];
-    if (layer == 0)
-        mp_layer_bf16<2 * kFeatures><<<dim3(vb), dim3(256), forward_bf16_lds(2 * kFeatures), stream>>>(
-            h, g, wq + bf16_offset(0), bias, out);
-    else
-        mp_layer_bf16<2 * kHidden><<<dim3(vb), dim3(256), forward_bf16_lds(2 * kHidden), stream>>>(
-            h, g, wq + bf16_offset(layer), bias, out);
+    if (layer == 0) {
+        const size_t n = size_t(g.vertices) * (kFeatures / 8);
+        aggregate_bf16<kFeatures><<<dim3(uint32_t((n + 255) / 256)), dim3(256), 0, stream>>>(h, g, z);
+        gemm_bf16<2 * kFeatures><<<grid, dim3(256), gemm_bf16_lds(2 * kFeatures), stream>>>(
+            z, g.vertices, wq + bf16_offset(0), 2 * kFeatures, 0, bias, true, out, kHidden);
+    } else {
+        const size_t n = size_t(g.vertices) * (kHidden / 8);
+        aggregate_bf16<kHidden><<<dim3(uint32_t((n + 255) / 256)), dim3(256), 0, stream>>>(h, g, z);
+        gemm_bf16<2 * kHidden><<<grid, dim3(256), gemm_bf16_lds(2 * kHidden), stream>>>(
+            z, g.vertices, wq + bf16_offset(layer), 2 * kHidden, 0, bias, true, out, kHidden);
+    }
 }
 
 void launch_edge_bf16(const __bf16* h, GraphView g, const __bf16* wq, const float* weights, float* out,
-                      float* hidden_out, uint32_t blocks, hipStream_t stream) {
+                      float* hidden_out, __bf16* p, hipStream_t stream) {
     configure_bf16();
-    if (!g.actions) return;
-    const uint32_t atiles = (g.actions + kTileRows - 1) / kTileRows;
-    edge_mlp_bf16<<<dim3(std::min(blocks, atiles)), dim3(256), forward_bf16_lds(kEdgeIn), stream>>>(
-        h, g, wq + bf16_offset(3), weights + layer_offset(3) + kHidden * kEdgeIn, weights + layer_offset(4),
+    if (!g.actions || !g.vertices) return;
+    if (!h || !wq || !out || !p) throw std::invalid_argument("null buffer for the BF16 edge layer");
+    // p = h [W_e(:, :128) | W_e(:, 128:)]^T: two 128-column blocks of W_e.
+    const uint32_t tiles = (g.vertices + kTileRows - 1) / kTileRows;
+    gemm_bf16<kHidden><<<dim3(std::min(kGemmBlocks, tiles), 2), dim3(256), gemm_bf16_lds(kHidden), stream>>>(
+        h, g.vertices, wq + bf16_offset(3), kEdgeIn, kHidden, nullptr, false, p, 2 * kHidden);
+    const size_t threads = size_t(g.actions) * 16;
+    edge_bf16<<<dim3(uint32_t((threads + 255) / 256)), dim3(256), 0, stream>>>(
+        p, g, weights + layer_offset(3) + kHidden * kEdgeIn, weights + layer_offset(4),
         weights + layer_offset(4) + kOutputs * kHidden, out, hidden_out);
 }
 
 void launch_forward_bf16(const __bf16* x, GraphView g, const __bf16* wq, const float* weights, __bf16* h_a,
-                         __bf16* h_b, float* out, uint32_t blocks, hipStream_t stream) {
-    launch_mp_bf16(0, x, g, wq, weights, h_a, blocks, stream);
-    launch_mp_bf16(1, h_a, g, wq, weights, h_b, blocks, stream);
-    launch_mp_bf16(2, h_b, g, wq, weights, h_a, blocks, stream);
-    launch_edge_bf16(h_a, g, wq, weights, out, nullptr, blocks, stream);
+                         __bf16* h_b, __bf16* z, __bf16* p, float* out, hipStream_t stream) {
+    launch_mp_bf16(0, x, g, wq, weights, h_a, z, stream);
+    launch_mp_bf16(1, h_a, g, wq, weights, h_b, z, stream);
+    launch_mp_bf16(2, h_b, g, wq, weights, h_a, z, stream);
+    launch_edge_bf16(h_a, g, wq, weights, out, nullptr, p, stream);
 }
 
 void launch_scores(const ScoreInput& in, hipStream_t stream) {

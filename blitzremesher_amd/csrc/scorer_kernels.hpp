@@ -61,20 +61,22 @@ void launch_features(const FeatureInput& in, hipStream_t stream);
 void launch_forward_fp32(const float* x, GraphView g, const float* weights, float* h_a, float* h_b, float* z,
                          float* out, hipStream_t stream);
 void launch_convert_bf16(const float* x, __bf16* y, size_t n, hipStream_t stream);
-// x: vertices x kFeatures (BF16); h_a, h_b: vertices x kHidden (BF16).
+// x: vertices x kFeatures (BF16); h_a, h_b: vertices x kHidden (BF16);
+// z: vertices x 2 kHidden (BF16 scratch); p: vertices x 2 kHidden (BF16 scratch).
 void launch_forward_bf16(const __bf16* x, GraphView g, const __bf16* wq, const float* weights, __bf16* h_a,
-                         __bf16* h_b, float* out, uint32_t blocks, hipStream_t stream);
+                         __bf16* h_b, __bf16* z, __bf16* p, float* out, hipStream_t stream);
 // Single layers, for training (activations of every layer kept).
 void launch_aggregate_fp32(const float* h, GraphView g, uint32_t width, float* z, hipStream_t stream);
 void launch_compensated(const float* input, const uint32_t* src, const uint32_t* dst, const float* weights,
                         const float* bias, float* output, uint32_t rows, uint32_t in, uint32_t out, bool relu,
                         hipStream_t stream);
 void launch_mp_bf16(int layer, const __bf16* h, GraphView g, const __bf16* wq, const float* weights, __bf16* out,
-                    uint32_t blocks, hipStream_t stream);
-// Edge MLP over g.actions entries (src = g.src, dst = g.neighbors); the
-// hidden layer is saved to hidden_out when it is not null.
+                    __bf16* z, hipStream_t stream);
+// Edge MLP over g.actions entries (src = g.src, dst = g.neighbors) in
+// factorised form; p (vertices x 2 kHidden BF16) is scratch; the hidden layer
+// is saved to hidden_out when it is not null.
 void launch_edge_bf16(const __bf16* h, GraphView g, const __bf16* wq, const float* weights, float* out,
-                      float* hidden_out, uint32_t blocks, hipStream_t stream);
+                      float* hidden_out, __bf16* p, hipStream_t stream);
 void launch_scores(const ScoreInput& in, hipStream_t stream);
 size_t sort_workspace(uint32_t n);
 hipError_t launch_sort(void* temp, size_t temp_bytes, const float* keys_in, float* keys_out, const uint32_t* order_in,

@@ -116,9 +116,10 @@ TrainResult Scorer::train_step(Graph& g, const uint32_t* actions, const uint8_t*
         __bf16* hq[4];
         hq[0] = g.features_bf16.as<__bf16>();
         for (int l = 1; l < 4; ++l) hq[l] = grow<__bf16>(g.t_hq[l], size_t(nv) * kHidden);
-        for (int l = 0; l < 3; ++l)
-            launch_mp_bf16(l, hq[l], gv, weights_bf16_.as<__bf16>(), w, hq[l + 1], blocks_, st);
-        launch_edge_bf16(hq[3], batch, weights_bf16_.as<__bf16>(), w, out, hidden, blocks_, st);
+        __bf16* zq = grow<__bf16>(g.zq, size_t(nv) * 2 * kHidden);
+        __bf16* pq = grow<__bf16>(g.pq, size_t(nv) * 2 * kHidden);
+        for (int l = 0; l < 3; ++l) launch_mp_bf16(l, hq[l], gv, weights_bf16_.as<__bf16>(), w, hq[l + 1], zq, st);
+        launch_edge_bf16(hq[3], batch, weights_bf16_.as<__bf16>(), w, out, hidden, pq, st);
         // FP32 views of the BF16 activations for the backward pass.
         for (int l = 1; l < 4; ++l) launch_bf16_to_fp32(hq[l], h[l], size_t(nv) * kHidden, st);
         uint32_t width = kFeatures;
