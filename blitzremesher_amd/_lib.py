@@ -160,7 +160,9 @@ PROTOTYPES = {
     "blitz_hip_graph_actions": (C.c_int, [_P, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.c_size_t]),
     "blitz_hip_scorer_forward": (C.c_int, [_P, _P, C.c_uint8, C.POINTER(C.c_float), C.c_size_t, *_ERR]),
     "blitz_hip_scorer_rank": (C.c_int, [_P, _P, C.c_uint8, C.POINTER(C.c_uint32), C.c_size_t, *_ERR]),
-    "blitz_hip_scorer_decode": (C.c_int, [_P, _P, C.POINTER(C.c_uint32), C.c_size_t, C.POINTER(C.c_double), *_ERR]),
+    "blitz_hip_scorer_decode": (C.c_int, [_P, _P, C.POINTER(C.c_uint32), C.c_size_t, C.POINTER(C.c_float), *_ERR]),
+    "blitz_hip_scorer_teacher": (C.c_int, [_P, _P, C.POINTER(C.c_uint32), C.c_size_t, C.POINTER(C.c_float),
+                                           C.POINTER(C.c_float), *_ERR]),
     "blitz_hip_scorer_last_forward_ms": (C.c_int, [_P, C.POINTER(C.c_double)]),
     "blitz_hip_train_settings_init": (None, [C.POINTER(TrainSettingsC)]),
     "blitz_hip_scorer_train_step": (C.c_int, [_P, _P, C.POINTER(C.c_uint32), C.POINTER(C.c_uint8),

@@ -624,10 +624,18 @@ blitz_status blitz_hip_scorer_rank(blitz_hip_scorer* s, blitz_hip_graph* g, uint
 }
 
 blitz_status blitz_hip_scorer_decode(blitz_hip_scorer* s, blitz_hip_graph* g, const uint32_t* actions, size_t count,
-                                     double* placements, char* error, size_t capacity) {
+                                     float* placements, char* error, size_t capacity) {
     return guarded(error, capacity, [&] {
-        if (!placements || (!actions && count)) throw std::invalid_argument("null decode buffers");
+        if ((!placements && count) || (!actions && count)) throw std::invalid_argument("null decode buffers");
         scorer_of(s).decode(graph_of(s, g), actions, count, placements);
+    });
+}
+
+blitz_status blitz_hip_scorer_teacher(blitz_hip_scorer* s, blitz_hip_graph* g, const uint32_t* actions, size_t count,
+                                      float* positions, float* targets, char* error, size_t capacity) {
+    return guarded(error, capacity, [&] {
+        if (count && (!actions || !positions || !targets)) throw std::invalid_argument("null teacher buffers");
+        scorer_of(s).teacher(graph_of(s, g), actions, count, positions, targets);
     });
 }
 

@@ -250,18 +250,21 @@ void Scorer::rank(Graph& g, Ranking r, uint32_t* order) {
     hip_check(hipStreamSynchronize(stream_), "rank");
 }
 
-void Scorer::decode(Graph& g, const uint32_t* selection, size_t count, double* out) {
+void Scorer::decode(Graph& g, const uint32_t* selection, size_t count, float* out) {
     if (!g.has_outputs) throw std::invalid_argument("forward() has not run on this graph");
     for (size_t i = 0; i < count; ++i)
         if (selection && selection[i] >= g.actions) throw std::invalid_argument("action outside graph");
     if (!selection && count > g.actions) throw std::invalid_argument("action count outside graph");
+    if (!count) return;
     DeviceGuard guard(device_);
-    DeviceBuffer sel, res;
+    DeviceBuffer sel, res, err;
     if (selection) {
         sel.resize(count * 4);
         hip_check(hipMemcpyAsync(sel.p, selection, count * 4, hipMemcpyHostToDevice, stream_), "upload selection");
     }
-    res.resize(count * 3 * 8);
+    res.resize(count * 3 * 4);
+    err.resize(4);
+    hip_check(hipMemsetAsync(err.p, 0, 4, stream_), "clear error");
     DecodeInput in{};
     in.outputs = g.outputs.as<float>();
     in.positions = g.positions.as<float4>();
@@ -269,11 +272,45 @@ void Scorer::decode(Graph& g, const uint32_t* selection, size_t count, double* o
     in.dst = g.neighbors.as<uint32_t>();
     in.selection = selection ? sel.as<uint32_t>() : nullptr;
     in.count = uint32_t(count);
-    in.out = res.as<double>();
+    in.out = res.as<float>();
+    in.error = err.as<uint32_t>();
     launch_decode(in, stream_);
     hip_check(hipGetLastError(), "decode launch");
-    hip_check(hipMemcpyAsync(out, res.p, count * 3 * 8, hipMemcpyDeviceToHost, stream_), "download placements");
+    uint32_t error = 0;
+    hip_check(hipMemcpyAsync(&error, err.p, 4, hipMemcpyDeviceToHost, stream_), "download error");
+    hip_check(hipMemcpyAsync(out, res.p, count * 3 * 4, hipMemcpyDeviceToHost, stream_), "download placements");
     hip_check(hipStreamSynchronize(stream_), "decode");
+    if (error) throw std::runtime_error("non-finite placement prediction");
+}
+
+void Scorer::teacher(Graph& g, const uint32_t* selection, size_t count, float* positions, float* targets) {
+    if (!selection && count) throw std::invalid_argument("null action list");
+    for (size_t i = 0; i < count; ++i)
+        if (selection[i] >= g.actions) throw std::invalid_argument("action outside graph");
+    if (!count) return;
+    DeviceGuard guard(device_);
+    DeviceBuffer sel, pos, tgt;
+    sel.resize(count * 4);
+    hip_check(hipMemcpyAsync(sel.p, selection, count * 4, hipMemcpyHostToDevice, stream_), "upload selection");
+    const size_t values = count * kTeacherCandidates * 3;
+    pos.resize(values * 4);
+    tgt.resize(values * 4);
+    TeacherInput in{};
+    in.positions = g.positions.as<float4>();
+    in.indices = g.indices.as<uint32_t>();
+    in.face_offsets = g.face_offsets.as<uint32_t>();
+    in.faces = g.faces_of.as<uint32_t>();
+    in.src = g.src.as<uint32_t>();
+    in.dst = g.neighbors.as<uint32_t>();
+    in.selection = sel.as<uint32_t>();
+    in.count = uint32_t(count);
+    in.positions_out = pos.as<float>();
+    in.targets_out = tgt.as<float>();
+    launch_teacher(in, stream_);
+    hip_check(hipGetLastError(), "teacher launch");
+    hip_check(hipMemcpyAsync(positions, pos.p, values * 4, hipMemcpyDeviceToHost, stream_), "download positions");
+    hip_check(hipMemcpyAsync(targets, tgt.p, values * 4, hipMemcpyDeviceToHost, stream_), "download targets");
+    hip_check(hipStreamSynchronize(stream_), "teacher");
 }
 
 }  // namespace blitz_hip

@@ -67,7 +67,9 @@ class Scorer {
     void download_outputs(const Graph& g, float* out);
     void download_features(const Graph& g, float* out);
     void rank(Graph& g, Ranking r, uint32_t* order);
-    void decode(Graph& g, const uint32_t* selection, size_t count, double* out);
+    void decode(Graph& g, const uint32_t* selection, size_t count, float* out);
+    // Ten teacher positions per action and their clamped offset targets.
+    void teacher(Graph& g, const uint32_t* selection, size_t count, float* positions, float* targets);
     double last_forward_ms() const { return forward_ms_; }
     int device() const { return device_; }
     hipStream_t stream() const { return stream_; }

@@ -368,17 +368,125 @@ __global__ void score_kernel(ScoreInput in) {
 
 // Placement of src/neural/action_gpu.cu:1157-1188 without grid snapping:
 // midpoint + clamp(offset, -1, 1) * edge length, FP64.
+// Vec3 arithmetic of the reference's action kernels
+// (src/neural/action_gpu.cu:16-38): float components, mul3 scaling in double
+// with a float result, dot3/len3 in double.
+__device__ __forceinline__ float3 add3(float3 a, float3 b) { return make_float3(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ __forceinline__ float3 sub3(float3 a, float3 b) { return make_float3(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ __forceinline__ float3 mul3(float3 a, double s) {
+    return make_float3(float(a.x * s), float(a.y * s), float(a.z * s));
+}
+__device__ __forceinline__ double dot3(float3 a, float3 b) {
+    return double(a.x) * b.x + double(a.y) * b.y + double(a.z) * b.z;
+}
+__device__ __forceinline__ float3 cross3(float3 a, float3 b) {
+    return make_float3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+__device__ __forceinline__ double len3(float3 a) { return sqrt(dot3(a, a)); }
+__device__ __forceinline__ float3 norm3(float3 a) {
+    const double n = len3(a);
+    return n > 0 ? mul3(a, 1 / n) : make_float3(0.f, 0.f, 0.f);
+}
+__device__ __forceinline__ float3 xyz(float4 p) { return make_float3(p.x, p.y, p.z); }
+__device__ __forceinline__ float clamp1(float v) { return fminf(1.f, fmaxf(-1.f, v)); }
+
+// decode_placements (src/neural/action_gpu.cu:1157-1188) without grid
+// snapping: middle + clamp(y, -1, 1) * edge length, float positions; a
+// non-finite prediction of a selected action sets the error flag.
 __global__ void decode_kernel(DecodeInput in) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= in.count) return;
     const uint32_t a = in.selection ? in.selection[i] : i;
-    const float4 p = in.positions[in.src[a]], q = in.positions[in.dst[a]];
-    const double d[3] = {double(p.x) - q.x, double(p.y) - q.y, double(p.z) - q.z};
-    const double length = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
-    const double mid[3] = {(double(p.x) + q.x) * .5, (double(p.y) + q.y) * .5, (double(p.z) + q.z) * .5};
-    for (int k = 0; k < 3; ++k) {
-        const double o = fmin(1.0, fmax(-1.0, double(in.outputs[size_t(a) * kOutputs + 1 + k])));
-        in.out[3 * size_t(i) + k] = mid[k] + o * length;
+    const float* y = in.outputs + size_t(a) * kOutputs;
+    for (uint32_t j = 0; j < kOutputs; ++j)
+        if (!isfinite(y[j])) {
+            atomicExch(in.error, 2u);
+            return;
+        }
+    const float3 p = xyz(in.positions[in.src[a]]), q = xyz(in.positions[in.dst[a]]);
+    const float3 middle = mul3(add3(p, q), .5);
+    const double length = len3(sub3(p, q));
+    const float3 pos = add3(middle, mul3(make_float3(clamp1(y[1]), clamp1(y[2]), clamp1(y[3])), length));
+    in.out[3 * size_t(i)] = pos.x;
+    in.out[3 * size_t(i) + 1] = pos.y;
+    in.out[3 * size_t(i) + 2] = pos.z;
+}
+
+// teacher_placements (src/neural/action_gpu.cu:1304-1373) for position
+// candidates only (no grid snapping, no normals): 0 from, 1 to, 2 middle,
+// 3 the optimum of the area-weighted plane quadrics of the faces around both
+// endpoints (FP64, Gauss-Jordan with partial pivoting; middle when singular),
+// 4..9 middle +/- a quarter edge along x, y, z; targets are the clamped
+// offsets from the middle in edge lengths.
+__global__ void teacher_kernel(TeacherInput in) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= in.count) return;
+    const uint32_t a = in.selection[i];
+    const uint32_t from = in.src[a], to = in.dst[a];
+    const float3 pf = xyz(in.positions[from]), pt = xyz(in.positions[to]);
+    const float3 middle = mul3(add3(pf, pt), .5);
+    const double scale = len3(sub3(pf, pt));
+    float3 cand[kTeacherCandidates];
+    cand[0] = pf;
+    cand[1] = pt;
+    cand[2] = middle;
+    cand[3] = middle;
+    double m[3][4] = {};
+    for (int side = 0; side < 2; ++side) {
+        const uint32_t v = side ? to : from;
+        for (uint32_t k = in.face_offsets[v]; k < in.face_offsets[v + 1]; ++k) {
+            const uint32_t f = in.faces[k];
+            const float3 fa = xyz(in.positions[in.indices[3 * f]]), fb = xyz(in.positions[in.indices[3 * f + 1]]),
+                         fc = xyz(in.positions[in.indices[3 * f + 2]]);
+            const float3 cr = cross3(sub3(fb, fa), sub3(fc, fa));
+            const float3 n = norm3(cr);
+            const double length = len3(cr), w[3] = {n.x, n.y, n.z}, d = dot3(n, sub3(fa, middle));
+            for (int r = 0; r < 3; ++r) {
+                for (int c = 0; c < 3; ++c) m[r][c] += w[r] * w[c] * length;
+                m[r][3] += w[r] * d * length;
+            }
+        }
+    }
+    bool solved = true;
+    for (int col = 0; col < 3; ++col) {
+        int pivot = col;
+        for (int r = col + 1; r < 3; ++r)
+            if (fabs(m[r][col]) > fabs(m[pivot][col])) pivot = r;
+        if (fabs(m[pivot][col]) < 1e-15) {
+            solved = false;
+            break;
+        }
+        for (int j = col; j < 4; ++j) {
+            const double t = m[col][j];
+            m[col][j] = m[pivot][j];
+            m[pivot][j] = t;
+        }
+        const double d = m[col][col];
+        for (int j = col; j < 4; ++j) m[col][j] /= d;
+        for (int r = 0; r < 3; ++r)
+            if (r != col) {
+                const double factor = m[r][col];
+                for (int j = col; j < 4; ++j) m[r][j] -= factor * m[col][j];
+            }
+    }
+    if (solved)
+        cand[3] = add3(middle, make_float3(float(fmin(scale, fmax(-scale, m[0][3]))),
+                                           float(fmin(scale, fmax(-scale, m[1][3]))),
+                                           float(fmin(scale, fmax(-scale, m[2][3])))));
+    for (int k = 4; k < 10; ++k) {
+        float delta[3] = {0.f, 0.f, 0.f};
+        delta[(k - 4) / 2] = float(((k - 4) % 2 ? -.25 : .25) * scale);
+        cand[k] = add3(middle, make_float3(delta[0], delta[1], delta[2]));
+    }
+    for (int k = 0; k < kTeacherCandidates; ++k) {
+        const size_t o = (size_t(i) * kTeacherCandidates + k) * 3;
+        in.positions_out[o] = cand[k].x;
+        in.positions_out[o + 1] = cand[k].y;
+        in.positions_out[o + 2] = cand[k].z;
+        const float3 off = mul3(sub3(cand[k], middle), 1 / scale);
+        in.targets_out[o] = clamp1(off.x);
+        in.targets_out[o + 1] = clamp1(off.y);
+        in.targets_out[o + 2] = clamp1(off.z);
     }
 }
 
@@ -516,6 +624,11 @@ hipError_t launch_sort(void* temp, size_t temp_bytes, const float* keys_in, floa
                        uint32_t* order_out, uint32_t n, hipStream_t stream) {
     return hipcub::DeviceRadixSort::SortPairsDescending(temp, temp_bytes, keys_in, keys_out, order_in, order_out,
                                                         int(n), 0, 32, stream);
+}
+
+void launch_teacher(const TeacherInput& in, hipStream_t stream) {
+    if (!in.count) return;
+    teacher_kernel<<<dim3((in.count + 127) / 128), dim3(128), 0, stream>>>(in);
 }
 
 void launch_decode(const DecodeInput& in, hipStream_t stream) {

@@ -44,7 +44,23 @@ struct DecodeInput {
     const uint32_t* dst;
     const uint32_t* selection;  // optional action ids
     uint32_t count;
-    double* out;  // count x 3
+    float* out;       // count x 3
+    uint32_t* error;  // set to 2 on a non-finite prediction
+};
+
+constexpr int kTeacherCandidates = 10;
+
+struct TeacherInput {
+    const float4* positions;
+    const uint32_t* indices;
+    const uint32_t* face_offsets;  // vertex -> incident faces, ascending
+    const uint32_t* faces;
+    const uint32_t* src;
+    const uint32_t* dst;
+    const uint32_t* selection;  // action ids
+    uint32_t count;
+    float* positions_out;  // count x kTeacherCandidates x 3
+    float* targets_out;    // count x kTeacherCandidates x 3
 };
 
 // Offset of layer l's matrix in the BF16 copy holding only the matrices of
@@ -82,5 +98,6 @@ size_t sort_workspace(uint32_t n);
 hipError_t launch_sort(void* temp, size_t temp_bytes, const float* keys_in, float* keys_out, const uint32_t* order_in,
                        uint32_t* order_out, uint32_t n, hipStream_t stream);
 void launch_decode(const DecodeInput& in, hipStream_t stream);
+void launch_teacher(const TeacherInput& in, hipStream_t stream);
 
 }  // namespace blitz_hip

@@ -122,12 +122,24 @@ class Scorer:
 
     def decode(self, graph: Graph, actions: np.ndarray) -> np.ndarray:
         a = np.ascontiguousarray(actions, np.uint32)
-        out = np.zeros((a.size, 3), np.float64)
+        out = np.zeros((a.size, 3), np.float32)
         err = _lib.error_buffer()
         _lib.check(self.lib.blitz_hip_scorer_decode(self.handle, graph.handle, a.ctypes.data_as(C.POINTER(C.c_uint32)),
-                                                    a.size, out.ctypes.data_as(C.POINTER(C.c_double)), err, len(err)),
+                                                    a.size, out.ctypes.data_as(C.POINTER(C.c_float)), err, len(err)),
                    err)
         return out
+
+    def teacher(self, graph: Graph, actions: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
+        """Ten teacher placement candidates per action and their targets."""
+        a = np.ascontiguousarray(actions, np.uint32)
+        pos = np.zeros((a.size, 10, 3), np.float32)
+        tgt = np.zeros((a.size, 10, 3), np.float32)
+        err = _lib.error_buffer()
+        f32 = C.POINTER(C.c_float)
+        _lib.check(self.lib.blitz_hip_scorer_teacher(self.handle, graph.handle, a.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                                     a.size, pos.ctypes.data_as(f32), tgt.ctypes.data_as(f32), err,
+                                                     len(err)), err)
+        return pos, tgt
 
     def last_forward_ms(self) -> float:
         ms = C.c_double()

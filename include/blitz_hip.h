@@ -288,9 +288,19 @@ blitz_status blitz_hip_scorer_forward(blitz_hip_scorer*, blitz_hip_graph*, uint8
 blitz_status blitz_hip_scorer_rank(blitz_hip_scorer*, blitz_hip_graph*, uint8_t ranking, uint32_t* order,
                                    size_t capacity, char* error, size_t error_capacity);
 /* Placements of the given actions (decode_placements,
- * src/neural/action_gpu.cu:1157-1188, without grid snapping): FP64 xyz. */
+ * src/neural/action_gpu.cu:1157-1188, without grid snapping): float xyz,
+ * middle + clamp(y, -1, 1) * edge length in the reference's Vec3 arithmetic.
+ * A non-finite prediction returns BLITZ_INTERNAL_ERROR. */
 blitz_status blitz_hip_scorer_decode(blitz_hip_scorer*, blitz_hip_graph*, const uint32_t* actions, size_t count,
-                                     double* placements, char* error, size_t capacity);
+                                     float* placements, char* error, size_t capacity);
+/* Teacher placement candidates of the given actions (teacher_placements,
+ * src/neural/action_gpu.cu:1304-1373, positions only, without grid
+ * snapping): 10 float xyz per action (from, to, middle, plane-quadric optimum
+ * of the faces around both endpoints solved in FP64, middle +/- a quarter
+ * edge along x, y, z) and the clamped offset targets of each. */
+#define BLITZ_HIP_TEACHER_CANDIDATES 10u
+blitz_status blitz_hip_scorer_teacher(blitz_hip_scorer*, blitz_hip_graph*, const uint32_t* actions, size_t count,
+                                      float* positions, float* targets, char* error, size_t capacity);
 blitz_status blitz_hip_scorer_last_forward_ms(const blitz_hip_scorer*, double* ms);
 
 /* ---- scorer training ----------------------------------------------------- */

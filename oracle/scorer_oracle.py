@@ -225,13 +225,111 @@ def rank(logits: np.ndarray) -> np.ndarray:
     return np.lexsort((np.arange(len(logits)), -logits.astype(np.float64)))
 
 
+# Vec3 helpers of src/neural/action_gpu.cu:16-38: float components; mul3
+# scales in double and rounds to float; dot3 and len3 are double.
+_f32 = np.float32
+
+
+def _add3(a, b):
+    return (a.astype(_f32) + b.astype(_f32)).astype(_f32)
+
+
+def _sub3(a, b):
+    return (a.astype(_f32) - b.astype(_f32)).astype(_f32)
+
+
+def _mul3(a, s):
+    return (a.astype(np.float64) * s).astype(_f32)
+
+
+def _dot3(a, b):
+    a = a.astype(np.float64)
+    b = b.astype(np.float64)
+    return a[..., 0] * b[..., 0] + a[..., 1] * b[..., 1] + a[..., 2] * b[..., 2]
+
+
+def _len3(a):
+    return np.sqrt(_dot3(a, a))
+
+
+def _cross3(a, b):
+    a = a.astype(_f32)
+    b = b.astype(_f32)
+    return np.stack([a[..., 1] * b[..., 2] - a[..., 2] * b[..., 1], a[..., 2] * b[..., 0] - a[..., 0] * b[..., 2],
+                     a[..., 0] * b[..., 1] - a[..., 1] * b[..., 0]], -1).astype(_f32)
+
+
+def _norm3(a):
+    n = _len3(a)
+    with np.errstate(divide="ignore"):
+        inv = np.where(n > 0, 1.0 / np.where(n > 0, n, 1.0), 0.0)
+    return np.where((n > 0)[..., None], _mul3(a, inv[..., None]), _f32(0))
+
+
 def decode_placements(positions: np.ndarray, acts: np.ndarray, outputs: np.ndarray) -> np.ndarray:
-    """Midpoint plus clamped offset times edge length (src/neural/action_gpu.cu:1157-1188)."""
-    p = positions.astype(np.float64)
+    """src/neural/action_gpu.cu:1157-1188 without grid snapping: middle of the
+    edge plus the clamped offset times the edge length, float positions."""
+    p = positions.astype(_f32)
     a, b = p[acts[:, 0]], p[acts[:, 1]]
-    mid = (a + b) * 0.5
-    length = np.sqrt(((a - b) ** 2).sum(1))
-    return mid + np.clip(outputs[:, 1:4].astype(np.float64), -1, 1) * length[:, None]
+    middle = _mul3(_add3(a, b), 0.5)
+    length = _len3(_sub3(a, b))
+    y = np.minimum(_f32(1), np.maximum(_f32(-1), outputs[:, 1:4].astype(_f32)))
+    return _add3(middle, _mul3(y, length[:, None]))
+
+
+def teacher_placements(positions: np.ndarray, indices: np.ndarray, face_offsets, faces, acts: np.ndarray):
+    """src/neural/action_gpu.cu:1304-1373 (positions, no grid snapping, no
+    normals) for each action: 10 candidate positions (from, to, middle, the
+    plane-quadric optimum of the faces around both endpoints, middle +/- a
+    quarter edge along x, y, z) and their clamped offset targets."""
+    p = positions.astype(_f32)
+    idx = indices.reshape(-1, 3)
+    out = np.zeros((len(acts), 10, 3), _f32)
+    tgt = np.zeros((len(acts), 10, 3), _f32)
+    for i, (u, v) in enumerate(acts):
+        pf, pt = p[u], p[v]
+        middle = _mul3(_add3(pf, pt), 0.5)
+        scale = float(_len3(_sub3(pf, pt)))
+        cand = [pf, pt, middle, middle]
+        m = np.zeros((3, 4))
+        for vert in (u, v):
+            for k in range(face_offsets[vert], face_offsets[vert + 1]):
+                a, b, c = p[idx[faces[k]]]
+                cr = _cross3(_sub3(b, a), _sub3(c, a))
+                n = _norm3(cr)
+                length = float(_len3(cr))
+                vv = n.astype(np.float64)
+                d = float(_dot3(n, _sub3(a, middle)))
+                for r in range(3):
+                    for col in range(3):
+                        m[r, col] += vv[r] * vv[col] * length
+                    m[r, 3] += vv[r] * d * length
+        solved = True
+        for col in range(3):
+            piv = col
+            for r in range(col + 1, 3):
+                if abs(m[r, col]) > abs(m[piv, col]):
+                    piv = r
+            if abs(m[piv, col]) < 1e-15:
+                solved = False
+                break
+            m[[col, piv]] = m[[piv, col]]
+            m[col, col:] = m[col, col:] / m[col, col]
+            for r in range(3):
+                if r != col:
+                    m[r, col:] = m[r, col:] - m[r, col] * m[col, col:]
+        if solved:
+            cand[3] = _add3(middle, np.array([min(scale, max(-scale, m[r, 3])) for r in range(3)]).astype(_f32))
+        for k in range(4, 10):
+            delta = np.zeros(3, _f32)
+            delta[(k - 4) // 2] = _f32((-0.25 if (k - 4) % 2 else 0.25) * scale)
+            cand.append(_add3(middle, delta))
+        with np.errstate(divide="ignore", invalid="ignore"):
+            for k in range(10):
+                out[i, k] = cand[k]
+                off = _mul3(_sub3(cand[k], middle), 1.0 / scale if scale else np.inf)
+                tgt[i, k] = np.fmin(_f32(1), np.fmax(_f32(-1), off))
+    return out, tgt
 
 
 def scores(positions: np.ndarray, indices: np.ndarray, acts: np.ndarray, mode: int, outputs=None,

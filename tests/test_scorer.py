@@ -223,6 +223,58 @@ def test_gpu_decode_matches_oracle(scorer):
     assert np.array_equal(got, want)
 
 
+def corner_mesh():
+    # Three unit quads on the planes x = 0, y = 0, z = 0 meeting at the
+    # origin (vertex 0), two triangles each.
+    p = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1], [1, 1, 0], [0, 1, 1], [1, 0, 1]], np.float32)
+    idx = np.array([[0, 2, 4], [0, 4, 1],      # z = 0
+                    [0, 1, 6], [0, 6, 3],      # y = 0
+                    [0, 3, 5], [0, 5, 2]],     # x = 0
+                   np.uint32)
+    return p, idx
+
+
+def test_teacher_oracle_closed_forms():
+    # Corner: the plane quadrics of the faces around the origin and around
+    # (1, 0, 0) vanish only at the origin; flat grid: the quadric is rank one
+    # (singular) and the candidate falls back to the middle of the edge.
+    p, idx = corner_mesh()
+    offsets, neighbors, face_offsets, faces, _ = so.topology(idx, len(p))
+    acts = np.array([[0, 1], [1, 0]], np.uint32)
+    pos, tgt = so.teacher_placements(p, idx, face_offsets, faces, acts)
+    assert np.allclose(pos[:, 3], 0, atol=1e-6)
+    assert np.array_equal(pos[0, 0], p[0]) and np.array_equal(pos[0, 1], p[1])
+    assert np.array_equal(pos[0, 2], [0.5, 0, 0])
+    assert np.allclose(pos[0, 4:], [[0.75, 0, 0], [0.25, 0, 0], [0.5, 0.25, 0], [0.5, -0.25, 0], [0.5, 0, 0.25],
+                                    [0.5, 0, -0.25]])
+    assert np.allclose(tgt[0, 0], [-0.5, 0, 0]) and np.allclose(tgt[0, 4], [0.25, 0, 0])
+    grid = meshes.grid(4)
+    offsets, neighbors, face_offsets, faces, _ = so.topology(grid.indices, len(grid.positions))
+    acts = so.actions(offsets, neighbors)
+    pos, _ = so.teacher_placements(grid.positions, grid.indices, face_offsets, faces, acts[:40])
+    assert np.array_equal(pos[:, 3], pos[:, 2])
+
+
+def test_decode_oracle_closed_form():
+    p = np.array([[0, 0, 0], [2, 0, 0]], np.float32)
+    out = np.array([[9, 0.5, -3, 0.25]], np.float32)
+    got = so.decode_placements(p, np.array([[0, 1]]), out)
+    assert np.array_equal(got, np.array([[2.0, -2.0, 0.5]], np.float32))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", sorted(CASES))
+def test_gpu_teacher_matches_oracle(scorer, case):
+    mesh = CASES[case]()
+    g, c, r, offsets, neighbors, acts = graph_case(scorer, mesh)
+    _, _, face_offsets, faces, _ = so.topology(mesh.indices, len(mesh.positions))
+    pick = np.arange(0, len(acts), max(1, len(acts) // 300), dtype=np.uint32)
+    got_p, got_t = scorer.teacher(g, pick)
+    want_p, want_t = so.teacher_placements(mesh.positions, mesh.indices, face_offsets, faces, acts[pick])
+    assert np.array_equal(got_p, want_p)
+    assert np.array_equal(got_t, want_t)
+
+
 @pytest.mark.gpu
 def test_gpu_shuffled_ranking_is_unavailable(scorer):
     from blitzremesher_amd import BlitzError
