@@ -25,12 +25,15 @@ PROFILE_COVERAGE, PROFILE_NORMALS, PROFILE_ATTRIBUTES = 0, 1, 2
 @dataclass
 class Mesh:
     """Positions (n, 3) float32 and triangle indices (m, 3) uint32, with the
-    optional per-face materials and per-material two-sided flags of
-    blitz::MeshView (include/blitz/mesh.hpp:91-105)."""
+    optional per-face materials, per-material two-sided flags, per-vertex
+    normals (n, 3) float32 and RGBA8 colours (n, 4) uint8 of blitz::MeshView
+    (include/blitz/mesh.hpp:91-105)."""
     positions: np.ndarray
     indices: np.ndarray
     materials: np.ndarray | None = None
     double_sided: np.ndarray | None = None
+    normals: np.ndarray | None = None
+    colors: np.ndarray | None = None
 
     def __post_init__(self) -> None:
         self.positions = np.ascontiguousarray(self.positions, dtype=np.float32).reshape(-1, 3)
@@ -39,6 +42,10 @@ class Mesh:
             self.materials = np.ascontiguousarray(self.materials, dtype=np.uint16).reshape(-1)
         if self.double_sided is not None:
             self.double_sided = np.ascontiguousarray(self.double_sided, dtype=np.uint8).reshape(-1)
+        if self.normals is not None:
+            self.normals = np.ascontiguousarray(self.normals, dtype=np.float32).reshape(-1, 3)
+        if self.colors is not None:
+            self.colors = np.ascontiguousarray(self.colors, dtype=np.uint8).reshape(-1, 4)
 
     @property
     def triangles(self) -> int:
@@ -59,6 +66,10 @@ class Mesh:
         if self.double_sided is not None and self.double_sided.size:
             d.double_sided = self.double_sided.ctypes.data_as(C.POINTER(C.c_uint8))
             d.double_sided_count = int(self.double_sided.size)
+        if self.normals is not None and n:
+            d.normals = _lib.BlitzStream(self.normals.ctypes.data, n, 12)
+        if self.colors is not None and n:
+            d.colors = _lib.BlitzStream(self.colors.ctypes.data, n, 4)
         return d
 
 

@@ -216,3 +216,24 @@ def drop_faces(mesh: Mesh, fraction: float, seed: int) -> Mesh:
     keep = uniform(seed, 200, mesh.triangles) >= fraction
     mats = mesh.materials[keep] if mesh.materials is not None else None
     return Mesh(mesh.positions, mesh.indices[keep], mats, mesh.double_sided)
+
+
+def with_attributes(mesh: Mesh, seed: int, normal_noise: float = 0.05, colors: bool = True) -> Mesh:
+    """Copy with per-vertex normals (area-weighted face normals plus seeded
+    noise, unnormalised) and seeded RGBA8 colours that vary smoothly with the
+    position."""
+    p = mesh.positions.astype(np.float64)
+    f = mesh.indices.astype(np.int64)
+    fn = np.cross(p[f[:, 1]] - p[f[:, 0]], p[f[:, 2]] - p[f[:, 0]])
+    n = np.zeros_like(p)
+    for k in range(3):
+        np.add.at(n, f[:, k], fn)
+    count = len(p)
+    n += np.stack([normal(seed, 11, count), normal(seed, 12, count), normal(seed, 13, count)], -1) * \
+        normal_noise * np.linalg.norm(n, axis=1, keepdims=True).clip(1e-12)
+    col = None
+    if colors:
+        phase = uniform(seed, 14, 3) * 6.283185307179586
+        rgb = 0.5 + 0.5 * np.sin(p * 3.0 + phase)
+        col = np.concatenate([np.round(rgb * 255), np.full((count, 1), 255)], 1).astype(np.uint8)
+    return Mesh(mesh.positions, mesh.indices, mesh.materials, mesh.double_sided, n.astype(np.float32), col)

@@ -128,6 +128,76 @@ int ref_evaluate(const float* rpos, uint64_t rnv, const uint32_t* ridx, uint64_t
     }
 }
 
+// blitz::evaluate with normal and colour streams (null when absent).
+int ref_evaluate_streams(const float* rpos, uint64_t rnv, const uint32_t* ridx, uint64_t rni, const uint16_t* rmat,
+                         uint64_t rnm, const uint8_t* rds, uint64_t rnds, const float* rnormals,
+                         const uint8_t* rcolors, const float* cpos, uint64_t cnv, const uint32_t* cidx, uint64_t cni,
+                         const uint16_t* cmat, uint64_t cnm, const uint8_t* cds, uint64_t cnds,
+                         const float* cnormals, const uint8_t* ccolors, const float* center, double radius,
+                         const OracleSettings* s, Measure* out, int (*cancelled)(void*), void* user) {
+    try {
+        EvalSettings e;
+        e.profile = Profile(s->profile);
+        e.screen_size = s->screen_size;
+        e.limit = s->limit;
+        e.max_changed_area = s->max_changed_area;
+        e.views = {s->orthographic, s->perspective, s->rotation_seed};
+        e.supersample = s->supersample;
+        e.max_supersample = s->max_supersample;
+        e.force_two_sided = s->force_two_sided;
+        e.conservative_screen = s->conservative_screen;
+        e.weights = {s->normal_weight, s->color_weight, s->material_weight};
+        e.force_scalar = true;
+        if (cancelled) e.cancelled = [=] { return cancelled(user) != 0; };
+        MeshView r = view_of(rpos, rnv, ridx, rni, rmat, rnm, rds, rnds);
+        MeshView c = view_of(cpos, cnv, cidx, cni, cmat, cnm, cds, cnds);
+        if (rnormals) r.normals = Stream<Vec3>(reinterpret_cast<const Vec3*>(rnormals), rnv);
+        if (cnormals) c.normals = Stream<Vec3>(reinterpret_cast<const Vec3*>(cnormals), cnv);
+        if (rcolors) r.colors = Stream<ColorRGBA8>(reinterpret_cast<const ColorRGBA8*>(rcolors), rnv);
+        if (ccolors) c.colors = Stream<ColorRGBA8>(reinterpret_cast<const ColorRGBA8*>(ccolors), cnv);
+        *out = out_of(evaluate(r, c, {{center[0], center[1], center[2]}, radius}, e));
+        return 0;
+    } catch (...) {
+        return 1;
+    }
+}
+
+// blitz::rasterize (attribute raster) of one view as row-major arrays.
+int ref_raster_pixels(const float* pos, uint64_t nv, const uint32_t* idx, uint64_t ni, const uint16_t* mat,
+                      uint64_t nm, const uint8_t* ds, uint64_t nds, const float* normals, const uint8_t* colors,
+                      const float* center, double radius, const Camera* cam, double screen, uint8_t ss,
+                      uint8_t force_two, uint8_t* covered, uint8_t* visible, float* depth, float* normal, float* color,
+                      uint16_t* material, uint64_t capacity, uint8_t* clipped, uint32_t* size_out) {
+    try {
+        MeshView m = view_of(pos, nv, idx, ni, mat, nm, ds, nds);
+        if (normals) m.normals = Stream<Vec3>(reinterpret_cast<const Vec3*>(normals), nv);
+        if (colors) m.colors = Stream<ColorRGBA8>(reinterpret_cast<const ColorRGBA8*>(colors), nv);
+        const Raster r = rasterize(m, {{center[0], center[1], center[2]}, radius}, *cam, screen, ss, force_two);
+        *size_out = r.width;
+        *clipped = r.clipped;
+        if (r.pixels.size() > capacity) return 1;
+        for (size_t i = 0; i < r.pixels.size(); ++i) {
+            const Pixel& p = r.pixels[i];
+            covered[i] = p.covered;
+            visible[i] = p.visible;
+            depth[i] = p.depth;
+            normal[3 * i] = p.normal.x;
+            normal[3 * i + 1] = p.normal.y;
+            normal[3 * i + 2] = p.normal.z;
+            color[4 * i] = p.color.x;
+            color[4 * i + 1] = p.color.y;
+            color[4 * i + 2] = p.color.z;
+            color[4 * i + 3] = p.color.w;
+            material[i] = p.material;
+        }
+        return 0;
+    } catch (const std::length_error&) {
+        return 2;
+    } catch (...) {
+        return 1;
+    }
+}
+
 // Classical quadric reduction (src/simplify.cpp reduce): Reuse output keeps the
 // source vertex IDs; returns the new index count or -1.
 int64_t ref_reduce_reuse(const float* pos, uint64_t nv, const uint32_t* idx, uint64_t ni, uint64_t target,

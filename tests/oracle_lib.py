@@ -136,7 +136,44 @@ class Oracle:
                                        out.ctypes.data_as(C.POINTER(C.c_float)))
         return out.reshape(size, size)
 
+    def raster_pixels(self, mesh: Mesh, center, radius, camera: _lib.Camera, screen, ss, force_two=False):
+        """Attribute raster (row-major): dict of covered, visible, depth,
+        normal (n, 3), color (n, 4), material; size; clipped. None over the
+        sample cap."""
+        size = math.ceil(screen + 8) * ss
+        n = size * size
+        out = {"covered": np.zeros(n, np.uint8), "visible": np.zeros(n, np.uint8), "depth": np.zeros(n, np.float32),
+               "normal": np.zeros((n, 3), np.float32), "color": np.zeros((n, 4), np.float32),
+               "material": np.zeros(n, np.uint16)}
+        clipped = C.c_uint8()
+        out_size = C.c_uint32()
+        cc = (C.c_float * 3)(*[float(x) for x in center])
+        f = self.fn("raster_pixels")
+        f.argtypes = [C.POINTER(C.c_float), C.c_uint64, C.POINTER(C.c_uint32), C.c_uint64, C.POINTER(C.c_uint16),
+                      C.c_uint64, C.POINTER(C.c_uint8), C.c_uint64, C.POINTER(C.c_float), C.POINTER(C.c_uint8),
+                      C.POINTER(C.c_float), C.c_double, C.POINTER(_lib.Camera), C.c_double, C.c_uint8, C.c_uint8,
+                      C.POINTER(C.c_uint8), C.POINTER(C.c_uint8), C.POINTER(C.c_float), C.POINTER(C.c_float),
+                      C.POINTER(C.c_float), C.POINTER(C.c_uint16), C.c_uint64, C.POINTER(C.c_uint8),
+                      C.POINTER(C.c_uint32)]
+        st = f(*_mesh_args(mesh), *_stream_args(mesh), cc, radius, C.byref(camera), screen, ss, int(force_two),
+               _ptr(out["covered"], C.c_uint8), _ptr(out["visible"], C.c_uint8), _ptr(out["depth"], C.c_float),
+               _ptr(out["normal"], C.c_float), _ptr(out["color"], C.c_float), _ptr(out["material"], C.c_uint16), n,
+               C.byref(clipped), C.byref(out_size))
+        if st == 2:
+            return None
+        assert st == 0, st
+        return out, int(out_size.value), bool(clipped.value)
+
+    def metric_acos(self, x: float) -> float:
+        assert self.kind == "oracle"
+        f = self.lib.oracle_metric_acos
+        f.restype = C.c_double
+        f.argtypes = [C.c_double]
+        return float(f(x))
+
     def evaluate(self, ref: Mesh, cand: Mesh, center, radius, s: EvalSettings) -> Measurement:
+        if ref.normals is not None or ref.colors is not None or cand.normals is not None or cand.colors is not None:
+            return self._evaluate_streams(ref, cand, center, radius, s)
         o = OracleSettings(s.screen_size, s.limit, s.max_changed_area, s.rotation_seed & 0xFFFFFFFF, s.orthographic,
                            s.perspective, s.supersample, s.max_supersample, int(s.force_two_sided),
                            int(s.conservative_screen), s.profile, s.normal_weight, s.color_weight,
@@ -151,6 +188,25 @@ class Oracle:
                       _lib.CANCEL_FN, C.c_void_p]
         st = f(*_mesh_args(ref), *_mesh_args(cand), cc, radius, C.byref(o), C.byref(out),
                cb if cb else _lib.CANCEL_FN(), None)
+        if st != 0:
+            raise ValueError(f"{self.kind} evaluate rejected the settings")
+        return Measurement.from_c(out)
+
+    def _evaluate_streams(self, ref: Mesh, cand: Mesh, center, radius, s: EvalSettings) -> Measurement:
+        o = OracleSettings(s.screen_size, s.limit, s.max_changed_area, s.rotation_seed & 0xFFFFFFFF, s.orthographic,
+                           s.perspective, s.supersample, s.max_supersample, int(s.force_two_sided),
+                           int(s.conservative_screen), s.profile, s.normal_weight, s.color_weight,
+                           s.material_weight)
+        out = OracleMeasurement()
+        cc = (C.c_float * 3)(*[float(x) for x in center])
+        cb = _lib.CANCEL_FN(lambda _u: 1 if s.cancelled() else 0) if s.cancelled else None
+        f = self.fn("evaluate_streams")
+        mesh = [C.POINTER(C.c_float), C.c_uint64, C.POINTER(C.c_uint32), C.c_uint64, C.POINTER(C.c_uint16),
+                C.c_uint64, C.POINTER(C.c_uint8), C.c_uint64, C.POINTER(C.c_float), C.POINTER(C.c_uint8)]
+        f.argtypes = mesh * 2 + [C.POINTER(C.c_float), C.c_double, C.POINTER(OracleSettings),
+                                 C.POINTER(OracleMeasurement), _lib.CANCEL_FN, C.c_void_p]
+        st = f(*_mesh_args(ref), *_stream_args(ref), *_mesh_args(cand), *_stream_args(cand), cc, radius, C.byref(o),
+               C.byref(out), cb if cb else _lib.CANCEL_FN(), None)
         if st != 0:
             raise ValueError(f"{self.kind} evaluate rejected the settings")
         return Measurement.from_c(out)
@@ -183,6 +239,12 @@ class Oracle:
               C.byref(status))
         assert n >= 0
         return tri[:n].tolist(), se[:n].tolist(), ae[:n].tolist(), int(audits.value), int(status.value)
+
+
+def _stream_args(mesh: Mesh):
+    n = None if mesh.normals is None else _ptr(mesh.normals, C.c_float)
+    c = None if mesh.colors is None else _ptr(mesh.colors, C.c_uint8)
+    return n, c
 
 
 def words_to_mask(words: np.ndarray, size: int) -> np.ndarray:
