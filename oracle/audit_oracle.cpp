@@ -393,7 +393,7 @@ Pixels rasterize_pixels(const Mesh& m, const Bounds& b, const Camera& c, double 
                 px.normal = normalized(add3(add3(scale3(a.n, u), scale3(d.n, v)), scale3(e.n, w)));
                 if (length3(px.normal) < .5)
                     px.normal = geometric;
-                else if (back)
+                else if (back && ORACLE_PLANT != 5)
                     px.normal = scale3(px.normal, -1);
                 px.color = m.colors ? F4{float(u * a.col.x + v * d.col.x + w * e.col.x),
                                          float(u * a.col.y + v * d.col.y + w * e.col.y),
@@ -460,7 +460,7 @@ double sample_cost(const Pixel& a, const Pixel& b, double spatial, const Setting
         const double x = double(a.color.x) - b.color.x, y = double(a.color.y) - b.color.y,
                      z = double(a.color.z) - b.color.z;
         cost += s.color_weight * s.color_weight * (x * x + y * y + z * z);
-        if (a.material != b.material) cost += s.material_weight * s.material_weight;
+        if (a.material != b.material && ORACLE_PLANT != 6) cost += s.material_weight * s.material_weight;
     }
     return cost;
 }

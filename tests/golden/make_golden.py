@@ -29,6 +29,10 @@ def digest(words: np.ndarray) -> str:
     return hashlib.sha256(np.ascontiguousarray(words, np.uint64).tobytes()).hexdigest()
 
 
+def pixel_digest(a: np.ndarray) -> str:
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
 def camera_dict(c) -> dict:
     return {"right": list(c.right), "up": list(c.up), "forward": list(c.forward), "distance": c.distance,
             "focal": c.focal, "scale": c.scale, "perspective": int(c.perspective)}
@@ -68,6 +72,16 @@ def fixture_meshes(ref: Oracle) -> dict[str, Mesh]:
     hf = meshes.heightfield(48, 0.1, 2)
     out["heightfield48"] = hf
     out["heightfield48_qem"] = ref.reduce_reuse(hf, int(hf.triangles * 0.75))
+    # Appearance streams (blitz::MeshView normals/colours, per-face materials).
+    out["ico2_attr"] = meshes.with_attributes(meshes.icosphere(2, 0.02, 1), 7)
+    out["ico2_attr_jitter"] = meshes.with_attributes(meshes.jitter(meshes.icosphere(2, 0.02, 1), 0.01, 3, 0.4), 7)
+    hfa = meshes.with_attributes(meshes.heightfield(20, 0.1, 2), 9, colors=True)
+    out["hf20_attr"] = hfa
+    out["hf20_attr_materials"] = Mesh(hfa.positions, hfa.indices, (np.arange(hfa.triangles) % 3).astype(np.uint16),
+                                      np.array([0, 1, 0], np.uint8), hfa.normals, hfa.colors)
+    out["hf20_attr_relabelled"] = Mesh(hfa.positions, hfa.indices, (np.arange(hfa.triangles) % 3).astype(np.uint16),
+                                       np.zeros(3, np.uint8), hfa.normals, hfa.colors)
+    out["ico2_plain"] = meshes.icosphere(2, 0.02, 1)
     return out
 
 
@@ -85,8 +99,9 @@ def main() -> None:
     for name, m in ms.items():
         arrays[f"{name}/positions"] = m.positions
         arrays[f"{name}/indices"] = m.indices
-        if m.double_sided is not None:
-            arrays[f"{name}/double_sided"] = m.double_sided
+        for field in ("double_sided", "materials", "normals", "colors"):
+            if getattr(m, field) is not None:
+                arrays[f"{name}/{field}"] = getattr(m, field)
     np.savez_compressed(HERE / "meshes.npz", **arrays)
 
     # Cameras (src/evaluate.cpp:108-124).
@@ -188,11 +203,43 @@ def main() -> None:
                  EvalSettings(**dict(base, orthographic=2, perspective=6), conservative_screen=conservative,
                               limit=1.5))
 
+    # Attribute rasters (src/evaluate.cpp:218-280): per-stream digests.
+    pixels = []
+    for mesh_name in ("ico2_attr", "hf20_attr_materials"):
+        c, r = ref.bounds(ms[mesh_name])
+        for cam in ref.cameras(c, r, 24, 4, 3, 77):
+            for ss in (1, 3):
+                for two in (False, True):
+                    got = ref.raster_pixels(ms[mesh_name], c, r, cam, 24, ss, two)
+                    entry = {"mesh": mesh_name, "center": [float(x) for x in c], "radius": r,
+                             "camera": camera_dict(cam), "screen": 24, "ss": ss, "two_sided": two}
+                    if got is None:
+                        entry["limited"] = True
+                    else:
+                        streams, size, clipped = got
+                        entry.update({"size": size, "clipped": clipped,
+                                      "sha256": {k: pixel_digest(v) for k, v in streams.items()},
+                                      "visible": int(streams["visible"].sum())})
+                    pixels.append(entry)
+
+    # Normals and attributes profiles (src/evaluate.cpp:282-352).
+    attributed = []
+    for profile in (1, 2):
+        for ref_name, cand_name in [("ico2_attr", "ico2_attr_jitter"), ("hf20_attr", "hf20_attr_materials"),
+                                    ("hf20_attr", "hf20_attr_relabelled"), ("ico2_plain", "ico2_attr"), ("ico2_attr", "ico2_attr")]:
+            for limit in (0.5, 2.0, 40.0):
+                for ss, mx in ((1, 4), (2, 2)):
+                    s = EvalSettings(profile=profile, screen_size=20, supersample=ss, max_supersample=mx,
+                                     orthographic=6, perspective=3, limit=limit, max_changed_area=1.0)
+                    add_eval(f"profile{profile}", ref_name, cand_name, s)
+                    attributed.append(evals.pop())
+
     (HERE / "audit_golden.json").write_text(json.dumps(
         {"source": "reference CPU sources (src/evaluate.cpp, src/mesh.cpp, src/simplify.cpp) built by oracle/Makefile",
-         "cameras": cams, "rasters": rasters, "distances": distances, "evaluations": evals}, indent=1))
+         "cameras": cams, "rasters": rasters, "distances": distances, "evaluations": evals, "pixels": pixels,
+         "attributed": attributed}, indent=1))
     print(f"wrote {len(cams)} camera sets, {len(rasters)} rasters, {len(distances)} distances, "
-          f"{len(evals)} evaluations")
+          f"{len(evals)} evaluations, {len(pixels)} attribute rasters, {len(attributed)} attributed evaluations")
 
 
 if __name__ == "__main__":

@@ -25,8 +25,9 @@ def golden_meshes() -> dict[str, Mesh]:
     names = sorted({k.split("/")[0] for k in data.files})
     out = {}
     for n in names:
-        ds = data[f"{n}/double_sided"] if f"{n}/double_sided" in data.files else None
-        out[n] = Mesh(data[f"{n}/positions"], data[f"{n}/indices"], double_sided=ds)
+        opt = {k: data[f"{n}/{k}"] if f"{n}/{k}" in data.files else None
+               for k in ("double_sided", "materials", "normals", "colors")}
+        out[n] = Mesh(data[f"{n}/positions"], data[f"{n}/indices"], **opt)
     return out
 
 
@@ -53,6 +54,10 @@ def measurement_of(d: dict) -> Measurement:
 
 def digest(words) -> str:
     return hashlib.sha256(np.ascontiguousarray(words, np.uint64).tobytes()).hexdigest()
+
+
+def pixel_digest(a: np.ndarray) -> str:
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
 
 
 def unpack_mask(bits: list[int], size: int) -> np.ndarray:

@@ -1,9 +1,10 @@
 """Appearance audit (normals and attributes profiles).
 
-CPU: the oracle's attribute raster, metric_acos and attributed evaluation
-are pinned to the reference's own CPU sources built here (oracle/_ref, when
-present) on fresh seeded inputs, and metric_acos to its closed forms and to
-libm within one ulp.
+CPU: the oracle's attribute raster and attributed evaluation are pinned to
+golden fixtures written by the reference's own CPU sources (per-stream
+digests of attribute rasters, full Measurements of profiles 1 and 2), to
+that build itself on fresh seeded inputs (oracle/_ref, when present), and
+metric_acos to its closed forms and to libm within two ulps.
 """
 from __future__ import annotations
 
@@ -13,6 +14,7 @@ import numpy as np
 import pytest
 
 from blitzremesher_amd import EvalSettings, Mesh, meshes
+from golden_data import audit_golden, camera_of, golden_meshes, measurement_of, pixel_digest, settings_of
 from oracle_lib import Oracle, available
 
 needs_ref = pytest.mark.skipif(not available("ref"), reason="reference CPU build (oracle/_ref) not present")
@@ -27,6 +29,33 @@ def attributed_cases():
     plain = meshes.icosphere(2, 0.02, 1)
     return {"icosphere": ico, "icosphere_moved": ico_moved, "heightfield": hf, "materials": hf_mat,
             "no_streams": plain}
+
+
+def test_pixels_match_golden():
+    o = Oracle("oracle")
+    ms = golden_meshes()
+    cases = audit_golden()["pixels"]
+    assert len(cases) >= 50
+    for case in cases:
+        got = o.raster_pixels(ms[case["mesh"]], case["center"], case["radius"], camera_of(case["camera"]),
+                              case["screen"], case["ss"], case["two_sided"])
+        if case.get("limited"):
+            assert got is None
+            continue
+        streams, size, clipped = got
+        assert (size, clipped) == (case["size"], case["clipped"])
+        assert {k: pixel_digest(v) for k, v in streams.items()} == case["sha256"], case
+
+
+def test_attributed_evaluations_match_golden():
+    o = Oracle("oracle")
+    ms = golden_meshes()
+    cases = audit_golden()["attributed"]
+    assert len(cases) >= 40 and any(c["measurement"]["normal_degrees"] > 0 for c in cases)
+    for case in cases:
+        got = o.evaluate(ms[case["reference"]], ms[case["candidate"]], case["center"], case["radius"],
+                         settings_of(case["settings"]))
+        assert got == measurement_of(case["measurement"]), case["case"]
 
 
 def test_metric_acos_closed_forms():

@@ -7,7 +7,9 @@
    for point pairs and discs, an order-independent OR over faces, and the
    float rounding of squared distances above 2^24.
 A planted mistake in the oracle (dropped radius term, transposed edge,
-missing float rounding) fails at least one of these; see the last test.
+missing float rounding, strict/loose error comparison, unflipped back-face
+normal, dropped material penalty) fails at least one of these; see the last
+test.
 """
 from __future__ import annotations
 
@@ -19,7 +21,7 @@ import pytest
 from blitzremesher_amd import meshes
 from blitzremesher_amd.audit import EvalSettings, Mesh
 from golden_data import (audit_golden, camera_fields, camera_of, digest, golden_meshes, measurement_of,
-                         settings_of, unpack_mask)
+                         pixel_digest, settings_of, unpack_mask)
 from oracle_lib import Oracle, available, mask_to_words, words_to_mask
 
 
@@ -182,6 +184,14 @@ def _golden_failures(o: Oracle) -> int:
         got = o.evaluate(ms[case["reference"]], ms[case["candidate"]], case["center"], case["radius"],
                          settings_of(case["settings"]))
         failures += got != measurement_of(case["measurement"])
+    for case in audit_golden()["pixels"][::3]:
+        r = o.raster_pixels(ms[case["mesh"]], case["center"], case["radius"], camera_of(case["camera"]),
+                            case["screen"], case["ss"], case["two_sided"])
+        failures += r is not None and {k: pixel_digest(v) for k, v in r[0].items()} != case["sha256"]
+    for case in audit_golden()["attributed"]:
+        got = o.evaluate(ms[case["reference"]], ms[case["candidate"]], case["center"], case["radius"],
+                         settings_of(case["settings"]))
+        failures += got != measurement_of(case["measurement"])
     failures += o.coverage_distance(*double_rounding_case(), 4200, 1) != DOUBLE_ROUNDED
     return failures
 
@@ -206,7 +216,7 @@ def test_double_float_rounding_closed_form(oracle):
     assert oracle.coverage_distance(*double_rounding_case(), 4200, 1) == DOUBLE_ROUNDED
 
 
-@pytest.mark.parametrize("plant", [1, 2, 3, 4])
+@pytest.mark.parametrize("plant", [1, 2, 3, 4, 5, 6])
 def test_planted_mistakes_are_caught(tmp_path, plant):
     import shutil
     import subprocess
