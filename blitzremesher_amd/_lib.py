@@ -103,7 +103,8 @@ class AuditTiming(C.Structure):
         ("raster_ms", C.c_double), ("compare_ms", C.c_double), ("distance_ms", C.c_double),
         ("total_ms", C.c_double), ("raster_triangles", C.c_uint64), ("mask_bytes", C.c_uint64),
         ("project_ms", C.c_double), ("band_ms", C.c_double), ("band_launches", C.c_uint32),
-        ("patch_fallbacks", C.c_uint32), ("patch_ms", C.c_double),
+        ("patch_fallbacks", C.c_uint32), ("patch_ms", C.c_double), ("appearance_ms", C.c_double),
+        ("attributed_views", C.c_uint32), ("reserved", C.c_uint32),
     ]
 
 
@@ -144,6 +145,11 @@ PROTOTYPES = {
     "blitz_hip_raster_coverage": (C.c_int, [_P, _P, C.POINTER(Bounds), C.POINTER(Camera), C.c_double, C.c_uint8,
                                             C.c_uint8, C.POINTER(C.c_uint64), C.c_size_t, C.POINTER(C.c_uint32),
                                             C.POINTER(C.c_uint8), *_ERR]),
+    "blitz_hip_raster_pixels": (C.c_int, [_P, _P, C.POINTER(Bounds), C.POINTER(Camera), C.c_double, C.c_uint8,
+                                          C.c_uint8, C.POINTER(C.c_uint8), C.POINTER(C.c_uint8),
+                                          C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float),
+                                          C.POINTER(C.c_uint16), C.c_size_t, C.POINTER(C.c_uint32),
+                                          C.POINTER(C.c_uint8), *_ERR]),
     "blitz_hip_coverage_distance": (C.c_int, [_P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.c_uint32,
                                               C.c_uint8, C.POINTER(C.c_double), *_ERR]),
     "blitz_hip_audit_last_timing": (C.c_int, [_P, C.POINTER(AuditTiming)]),

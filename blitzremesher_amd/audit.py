@@ -319,7 +319,8 @@ class Audit:
         return d
 
     def evaluate(self, reference, candidate, center, radius, settings: EvalSettings) -> Measurement:
-        """Coverage audit of candidate against reference (host or device meshes)."""
+        """Audit of candidate against reference (host or device meshes) under
+        the settings' profile: coverage, normals or attributes."""
         s, keep = settings.to_c()
         b = _bounds_c(center, radius)
         out = _lib.MeasurementC()
@@ -404,6 +405,34 @@ class Audit:
             return None
         _lib.check(st, err)
         return words[:capacity], int(out_size.value), bool(clipped.value)
+
+    def raster_pixels(self, mesh: DeviceMesh, center, radius, camera: _lib.Camera, screen: float, ss: int,
+                      force_two_sided: bool = False):
+        """Attribute raster of one view (the reference's Pixel raster),
+        row-major: dict of covered, visible, depth, normal (n, 3), color
+        (n, 4), material; raster side; clipped flag. None over the 64M-sample
+        cap."""
+        size = math.ceil(screen + 8) * ss
+        n = size * size
+        out = {"covered": np.zeros(n, np.uint8), "visible": np.zeros(n, np.uint8), "depth": np.zeros(n, np.float32),
+               "normal": np.zeros((n, 3), np.float32), "color": np.zeros((n, 4), np.float32),
+               "material": np.zeros(n, np.uint16)}
+        out_size = C.c_uint32()
+        clipped = C.c_uint8()
+        b = _bounds_c(center, radius)
+        err = _lib.error_buffer()
+
+        def ptr(a, t):
+            return a.ctypes.data_as(C.POINTER(t))
+        st = self.lib.blitz_hip_raster_pixels(
+            self.handle, mesh.handle, C.byref(b), C.byref(camera), screen, ss, int(force_two_sided),
+            ptr(out["covered"], C.c_uint8), ptr(out["visible"], C.c_uint8), ptr(out["depth"], C.c_float),
+            ptr(out["normal"], C.c_float), ptr(out["color"], C.c_float), ptr(out["material"], C.c_uint16), n,
+            C.byref(out_size), C.byref(clipped), err, len(err))
+        if st == 5:
+            return None
+        _lib.check(st, err)
+        return out, int(out_size.value), bool(clipped.value)
 
     def coverage_distance(self, a: np.ndarray, b: np.ndarray, size: int, ss: int) -> float:
         a = np.ascontiguousarray(a, dtype=np.uint64)

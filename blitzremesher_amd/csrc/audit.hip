@@ -18,6 +18,7 @@
 #include <new>
 #include <tuple>
 
+#include "appearance_kernels.hpp"
 #include "audit_kernels.hpp"
 
 namespace blitz_hip {
@@ -124,6 +125,9 @@ DeviceMesh::~DeviceMesh() {
     (void)hipFree(positions);
     (void)hipFree(indices);
     (void)hipFree(two_sided);
+    (void)hipFree(normals);
+    (void)hipFree(colors);
+    (void)hipFree(materials);
 }
 
 Audit::Audit(int device, uint64_t memory_bytes) : device_(device), memory_bytes_(memory_bytes) {
@@ -190,6 +194,34 @@ std::unique_ptr<DeviceMesh> Audit::upload(const HostMesh& m) {
         hip_check(hipMemcpyAsync(d->two_sided, two.data(), two.size(), hipMemcpyHostToDevice, stream_),
                   "upload sidedness");
     }
+    // Appearance streams as the attribute raster reads them
+    // (src/evaluate.cpp:141-143): unit normals and colours / 255 per vertex.
+    std::vector<float4> nrm, col;
+    if (!m.normals.empty() && d->vertices) {
+        nrm.resize(d->vertices);
+        for (uint32_t i = 0; i < d->vertices; ++i) {
+            const F3 n = normalized({m.normals[3 * i], m.normals[3 * i + 1], m.normals[3 * i + 2]});
+            nrm[i] = make_float4(n.x, n.y, n.z, 0.f);
+        }
+        hip_check(hipMalloc(&d->normals, nrm.size() * sizeof(float4)), "hipMalloc");
+        hip_check(hipMemcpyAsync(d->normals, nrm.data(), nrm.size() * sizeof(float4), hipMemcpyHostToDevice, stream_),
+                  "upload normals");
+    }
+    if (!m.colors.empty() && d->vertices) {
+        col.resize(d->vertices);
+        for (uint32_t i = 0; i < d->vertices; ++i)
+            col[i] = make_float4(m.colors[4 * i] / 255.f, m.colors[4 * i + 1] / 255.f, m.colors[4 * i + 2] / 255.f,
+                                 m.colors[4 * i + 3] / 255.f);
+        hip_check(hipMalloc(&d->colors, col.size() * sizeof(float4)), "hipMalloc");
+        hip_check(hipMemcpyAsync(d->colors, col.data(), col.size() * sizeof(float4), hipMemcpyHostToDevice, stream_),
+                  "upload colours");
+    }
+    if (!m.materials.empty() && d->faces) {
+        hip_check(hipMalloc(&d->materials, size_t(d->faces) * 2), "hipMalloc");
+        hip_check(hipMemcpyAsync(d->materials, m.materials.data(), size_t(d->faces) * 2, hipMemcpyHostToDevice,
+                                 stream_),
+                  "upload materials");
+    }
     hip_check(hipStreamSynchronize(stream_), "upload");
     return d;
 }
@@ -212,6 +244,21 @@ Measurement finish_view(double coverage, uint64_t changed, uint64_t total, bool 
     }
     return cur;
 }
+// Normals and attributes profiles (src/evaluate.cpp:282-298).
+Measurement finish_view_attributed(double coverage, uint64_t changed, uint64_t total, bool clipped,
+                                   double normal_acos, double attributed, double previous_degrees,
+                                   const Settings& s, uint8_t ss) {
+    Measurement cur = finish_view(coverage, changed, total, clipped, s, ss);
+    cur.normal_degrees = std::max(previous_degrees, normal_acos * 180 / kPi);
+    cur.error = cur.coverage_upper > s.limit ? cur.coverage_upper : std::max(cur.coverage_upper, attributed);
+    cur.passed = cur.error <= s.limit && cur.changed_area <= s.max_changed_area;
+    return cur;
+}
+
+bool attributed_weights(const Settings& s) {
+    return s.profile != 0 && (s.normal_weight != 0 || (s.profile == 2 && (s.color_weight != 0 || s.material_weight != 0)));
+}
+
 // Refinement stop rule (src/evaluate.cpp:336-337).
 bool settled(const Measurement& cur, const Settings& s, unsigned ss) {
     return s.conservative_screen || cur.passed || ss >= s.max_supersample ||
@@ -228,8 +275,9 @@ std::vector<Measurement> Audit::evaluate_batch(const DeviceMesh& reference,
                                                const std::vector<const DeviceMesh*>& candidates, const Bounds& b,
                                                const Settings& s) {
     validate_settings(b, s);
-    if (s.profile != 0)
-        throw Unavailable("the HIP audit backend implements the coverage profile only");
+    // Conservative screens use the coverage rule under every profile
+    // (src/evaluate.cpp:324-326).
+    const bool attributed = s.profile != 0 && !s.conservative_screen;
     Clock clock;
     timing_ = {};
     const auto cams = cameras(b, s.screen_size, s.orthographic, s.perspective, s.rotation_seed);
@@ -278,7 +326,10 @@ std::vector<Measurement> Audit::evaluate_batch(const DeviceMesh& reference,
         for (size_t i = 0; i < items.size(); ++i) {
             const Item& it = items[i];
             State& st = state[it.candidate][it.view];
-            st.m = finish_view(results[i].coverage, results[i].changed, results[i].total, results[i].clipped, s, it.ss);
+            const ViewResult& res = results[i];
+            st.m = attributed ? finish_view_attributed(res.coverage, res.changed, res.total, res.clipped,
+                                                       res.normal_acos, res.attributed, st.m.normal_degrees, s, it.ss)
+                              : finish_view(res.coverage, res.changed, res.total, res.clipped, s, it.ss);
             if (settled(st.m, s, it.ss))
                 st.final = true;
             else
@@ -317,6 +368,7 @@ std::vector<Measurement> Audit::evaluate_batch(const DeviceMesh& reference,
                 r.changed_area = cur.changed_area;
                 r.changed_area_worst_view = v;
             }
+            r.normal_degrees = std::max(r.normal_degrees, cur.normal_degrees);
             if (!cur.passed) {
                 r.passed = r.complete = false;
                 break;
@@ -328,7 +380,8 @@ std::vector<Measurement> Audit::evaluate_batch(const DeviceMesh& reference,
 }
 
 void Audit::rasterize(const std::vector<RasterJob>& jobs, const std::vector<Camera>& cams, const Bounds& b,
-                      uint8_t ss, uint32_t size, bool force_two, std::vector<uint32_t>& status_out) {
+                      uint8_t ss, uint32_t size, bool force_two, std::vector<uint32_t>& status_out,
+                      std::vector<RasterTask>* drawn, std::vector<const double*>* depth) {
     const uint32_t tiles = (size + kTile - 1) / kTile;
     const size_t words = size_t(tiles) * tiles;
     // One projection per (vertex stream contents, view).
@@ -360,14 +413,16 @@ void Audit::rasterize(const std::vector<RasterJob>& jobs, const std::vector<Came
         perspective |= cams[sets[i].second].perspective != 0;
     }
     double2* xy = scratch_as<double2>(9, std::max<size_t>(1, offset.back()));
-    double* zs = perspective ? scratch_as<double>(10, std::max<size_t>(1, offset.back())) : nullptr;
+    // View depths: perspective views divide by them; attribute rasters
+    // interpolate them in every view.
+    double* zs = perspective || drawn ? scratch_as<double>(10, std::max<size_t>(1, offset.back())) : nullptr;
     std::vector<ProjectTask> projections(sets.size());
     for (size_t i = 0; i < sets.size(); ++i) {
         ProjectTask& p = projections[i];
         p = {};
         p.positions = sets[i].first->positions;
         p.xy = xy + offset[i];
-        p.z = cams[sets[i].second].perspective ? zs + offset[i] : nullptr;
+        p.z = cams[sets[i].second].perspective || drawn ? zs + offset[i] : nullptr;
         p.camera = cams[sets[i].second];
         p.center = b.center;
         p.vertices = sets[i].first->vertices;
@@ -381,7 +436,7 @@ void Audit::rasterize(const std::vector<RasterJob>& jobs, const std::vector<Came
         RasterTask& t = tasks[j];
         t = {};
         t.xy = xy + offset[job_slot[j]];
-        t.z = projections[job_slot[j]].z;
+        t.z = cams[jobs[j].view].perspective ? zs + offset[job_slot[j]] : nullptr;
         t.indices = m.indices;
         t.two_sided = m.two_sided;
         t.mask = jobs[j].mask;
@@ -447,6 +502,15 @@ void Audit::rasterize(const std::vector<RasterJob>& jobs, const std::vector<Came
         hip_check(hipGetLastError(), "raster redo launch");
         hip_check(hipStreamSynchronize(stream_), "raster redo");
     }
+    if (drawn) {
+        *drawn = tasks;
+        for (size_t j = 0; j < jobs.size(); ++j)
+            if (status_out[2 * j] != kNone) (*drawn)[j].faces = status_out[2 * j];
+    }
+    if (depth) {
+        depth->resize(jobs.size());
+        for (size_t j = 0; j < jobs.size(); ++j) (*depth)[j] = zs + offset[job_slot[j]];
+    }
 }
 
 void Audit::run_items(const DeviceMesh& reference, const std::vector<const DeviceMesh*>& candidates,
@@ -459,8 +523,12 @@ void Audit::run_items(const DeviceMesh& reference, const std::vector<const Devic
     const size_t samples = size_t(size) * size;
     // Per item: candidate mask, two row-distance planes, projection, summary.
     // Per view: one reference mask shared by every candidate of the chunk.
+    const bool attributed = s.profile != 0 && !s.conservative_screen;
+    // Appearance profiles add two attribute rasters (normal, colour: 32 bytes
+    // per sample) per item and per reference view, and the items' face bins.
     const size_t per_item = words * 8 + samples * 4 + sizeof(ViewSummary) + sizeof(RasterTask) + sizeof(ViewPair) +
-                            24 * size_t(std::max(reference.vertices, 1u));
+                            24 * size_t(std::max(reference.vertices, 1u)) +
+                            (attributed ? words * 64 * 64 + words * 16 : 0);
     const uint64_t budget = memory_bytes_ ? memory_bytes_ : (uint64_t(8) << 30);
     const size_t chunk = std::max<size_t>(1, std::min<size_t>(all.size(), budget / (per_item + words * 8)));
     for (size_t begin = 0; begin < all.size(); begin += chunk) {
@@ -506,7 +574,10 @@ void Audit::run_items(const DeviceMesh& reference, const std::vector<const Devic
         launch_fill(ref_masks, size_t(nref) * words, 0, stream_);
         launch_fill(cand_masks, size_t(ncand) * words, 0, stream_);
         std::vector<uint32_t> st;
-        rasterize(jobs, cams, b, ss, size, s.force_two_sided, st);
+        std::vector<RasterTask> drawn;
+        std::vector<const double*> zdepth;
+        rasterize(jobs, cams, b, ss, size, s.force_two_sided, st, attributed ? &drawn : nullptr,
+                  attributed ? &zdepth : nullptr);
         hip_check(hipEventRecord(events_[1], stream_), "event");
         hip_check(hipMemcpyAsync(dpairs, pairs.data(), pairs.size() * sizeof(ViewPair), hipMemcpyHostToDevice, stream_),
                   "upload pairs");
@@ -516,6 +587,35 @@ void Audit::run_items(const DeviceMesh& reference, const std::vector<const Devic
         launch_distance(dpairs, summaries, ncand, size, tiles, stream_);
         hip_check(hipEventRecord(events_[3], stream_), "event");
         hip_check(hipGetLastError(), "audit launch");
+        // Appearance profiles: attribute rasters of every job, then per item
+        // the normal angle (src/evaluate.cpp:282-290).
+        const size_t per = words * 64;
+        std::vector<AttrPair> apairs;
+        unsigned long long* aout = nullptr;
+        std::vector<unsigned long long> angles;
+        if (attributed) {
+            hip_check(hipEventRecord(events_[8], stream_), "event");
+            float4* anorm = scratch_as<float4>(30, jobs.size() * per);
+            float4* acol = scratch_as<float4>(31, jobs.size() * per);
+            attribute_rasters(jobs, drawn, zdepth, cams, b, anorm, acol, nullptr);
+            aout = scratch_as<unsigned long long>(32, 2 * size_t(ncand));
+            launch_fill(reinterpret_cast<uint64_t*>(aout), 2 * size_t(ncand), 0, stream_);
+            apairs.resize(ncand);
+            for (uint32_t i = 0; i < ncand; ++i) {
+                const size_t ra = ref_job[ref_slot[all[begin + i].view]], rc = cand_job[i];
+                apairs[i] = {anorm + ra * per, acol + ra * per, anorm + rc * per, acol + rc * per, aout + 2 * size_t(i),
+                             size, tiles};
+            }
+            AttrPair* dap = scratch_as<AttrPair>(33, ncand);
+            hip_check(hipMemcpyAsync(dap, apairs.data(), ncand * sizeof(AttrPair), hipMemcpyHostToDevice, stream_),
+                      "upload attribute pairs");
+            for (uint32_t i0 = 0; i0 < ncand; i0 += 65535)
+                launch_normal_angle(dap + i0, std::min<uint32_t>(65535, ncand - i0), tiles, stream_);
+            hip_check(hipGetLastError(), "normal angle launch");
+            angles.resize(2 * size_t(ncand));
+            hip_check(hipMemcpyAsync(angles.data(), aout, angles.size() * 8, hipMemcpyDeviceToHost, stream_),
+                      "download angles");
+        }
         std::vector<ViewSummary> sum(ncand);
         hip_check(hipMemcpyAsync(sum.data(), summaries, ncand * sizeof(ViewSummary), hipMemcpyDeviceToHost, stream_),
                   "download summaries");
@@ -546,6 +646,54 @@ void Audit::run_items(const DeviceMesh& reference, const std::vector<const Devic
                 r.coverage = kInf;
             else
                 r.coverage = std::sqrt(double(float(std::max(v.max_ac, v.max_ca)))) / ss;
+            r.normal_acos = 0;
+            r.attributed = 0;
+            if (attributed) {
+                double a;
+                std::memcpy(&a, &angles[2 * size_t(i)], 8);
+                r.normal_acos = a;
+            }
+        }
+        if (!attributed) continue;
+        // The attributed distance, only where the coverage bound is within
+        // the limit (src/evaluate.cpp:292-293) and the weights can make it
+        // non-zero (src/evaluate.cpp:218-219).
+        std::vector<uint32_t> need;
+        std::vector<AttrPair> list;
+        if (attributed_weights(s))
+            for (uint32_t i = 0; i < ncand; ++i) {
+                const ViewResult& r = out[begin + i];
+                if (finish_view(r.coverage, r.changed, r.total, r.clipped, s, ss).coverage_upper <= s.limit)
+                    need.push_back(i);
+            }
+        if (!need.empty()) {
+            list.resize(need.size());
+            for (size_t k = 0; k < need.size(); ++k) list[k] = apairs[need[k]];
+            AttrPair* dl = scratch_as<AttrPair>(34, list.size());
+            hip_check(hipMemcpyAsync(dl, list.data(), list.size() * sizeof(AttrPair), hipMemcpyHostToDevice, stream_),
+                      "upload attributed pairs");
+            CostSettings cs{s.normal_weight, s.color_weight, s.material_weight, s.limit, s.profile, ss};
+            for (size_t k0 = 0; k0 < list.size(); k0 += 65535)
+                launch_attributed(dl + k0, uint32_t(std::min<size_t>(65535, list.size() - k0)), tiles, cs, stream_);
+            hip_check(hipGetLastError(), "attributed launch");
+            hip_check(hipMemcpyAsync(angles.data(), aout, angles.size() * 8, hipMemcpyDeviceToHost, stream_),
+                      "download attributed");
+        }
+        hip_check(hipEventRecord(events_[9], stream_), "event");
+        hip_check(hipStreamSynchronize(stream_), "attributed");
+        {
+            float t_a = 0;
+            hip_check(hipEventElapsedTime(&t_a, events_[8], events_[9]), "elapsed");
+            timing_.appearance_ms += t_a;
+            timing_.attributed_views += uint32_t(need.size());
+        }
+        const double limit2 = s.limit * s.limit;
+        for (uint32_t i : need) {
+            double m;
+            std::memcpy(&m, &angles[2 * size_t(i) + 1], 8);
+            // src/evaluate.cpp:262-266: any sample beyond the limit makes the
+            // distance infinite; otherwise the root of the largest cost.
+            out[begin + i].attributed = m > limit2 ? kInf : std::sqrt(m);
         }
     }
 }

@@ -47,6 +47,8 @@ struct HostMesh {
     std::vector<uint32_t> indices;
     std::vector<uint16_t> materials;
     std::vector<uint8_t> double_sided;
+    std::vector<float> normals;    // xyz per vertex as given, or empty
+    std::vector<uint8_t> colors;   // RGBA8 per vertex, or empty
     uint64_t digest[2] = {0, 0};      // streams compared by identical(): positions, normals, colors,
                                       // indices, materials, sidedness
     uint64_t pos_digest[2] = {0, 0};  // positions only: equal digests share one projection per view
@@ -59,6 +61,11 @@ struct DeviceMesh {
     float4* positions = nullptr;
     uint32_t* indices = nullptr;
     uint8_t* two_sided = nullptr;  // per face; null when no face is two-sided
+    // Appearance streams (include/blitz/mesh.hpp:20): unit vertex normals,
+    // vertex colours / 255 and per-face materials; null when absent.
+    float4* normals = nullptr;
+    float4* colors = nullptr;
+    uint16_t* materials = nullptr;
     uint32_t vertices = 0, faces = 0;
     uint64_t digest[2] = {0, 0};
     uint64_t pos_digest[2] = {0, 0};
@@ -74,6 +81,16 @@ uint32_t raster_side(double screen, unsigned ss);  // ceil(screen + 8) * ss
 // refinement stop rule (src/evaluate.cpp:336-337).
 Measurement finish_view(double coverage, uint64_t changed, uint64_t total, bool clipped, const Settings& s,
                         uint8_t ss);
+// The same view under the normals and attributes profiles: normal_acos is
+// the largest metric_acos of the normal angle over samples visible in both
+// rasters, attributed the attributed distance (used only when the coverage
+// bound is within the limit) and previous_degrees the view's normal angle
+// from its earlier refinement passes.
+Measurement finish_view_attributed(double coverage, uint64_t changed, uint64_t total, bool clipped,
+                                   double normal_acos, double attributed, double previous_degrees,
+                                   const Settings& s, uint8_t ss);
+// Whether the attributed distance can be non-zero (src/evaluate.cpp:218-219).
+bool attributed_weights(const Settings& s);
 bool settled(const Measurement& cur, const Settings& s, unsigned ss);
 
 struct Timing {
@@ -86,6 +103,8 @@ struct Timing {
     uint32_t band_launches = 0;
     double patch_ms = 0;      // patch kernel (device time)
     uint32_t patch_fallbacks = 0;
+    double appearance_ms = 0;  // attribute rasters, normal angle and attributed distance (device time)
+    uint32_t attributed_views = 0;
 };
 
 // A reference bound for patch audits (patch_audit.hpp).
@@ -118,6 +137,18 @@ class Audit {
                           bool force_two, std::vector<uint64_t>& words, uint32_t& size, bool& clipped);
     double coverage_distance(const std::vector<uint64_t>& a, const std::vector<uint64_t>& b, uint32_t size,
                              uint8_t ss);
+    // Attribute raster of one view (src/evaluate.cpp:125-189), row-major:
+    // covered, visible, stored depth, normal xyz, colour rgba, material.
+    // Returns false when over the sample cap.
+    struct PixelRaster {
+        uint32_t size = 0;
+        bool clipped = false;
+        std::vector<uint8_t> covered, visible;
+        std::vector<float> depth, normal, color;
+        std::vector<uint16_t> material;
+    };
+    bool raster_pixels(const DeviceMesh& m, const Bounds& b, const Camera& c, double screen, uint8_t ss,
+                       bool force_two, PixelRaster& out);
     // Binds a reference: per view its projection, mask, covered count and
     // per-tile face bins at s.supersample (src/neural/internal.hpp:141-144).
     std::unique_ptr<Binding> bind(const HostMesh& reference, const Bounds& b, const Settings& s);
@@ -141,6 +172,8 @@ class Audit {
         double coverage;
         uint64_t changed, total;
         bool clipped;
+        double normal_acos;  // appearance profiles: largest normal angle (metric_acos)
+        double attributed;   // appearance profiles: attributed distance, 0 when not needed
     };
     struct RasterJob {
         const DeviceMesh* mesh;
@@ -149,8 +182,17 @@ class Audit {
     };
     // Rasterizes every job at one supersampling; status receives per job the
     // first out-of-frame face (0xFFFFFFFF if none) and the behind-camera flag.
+    // With `drawn` set, also returns every job's raster task as drawn (faces
+    // cut at the first out-of-frame face) and `depth` its vertex set's view
+    // depths, computed for every view.
     void rasterize(const std::vector<RasterJob>& jobs, const std::vector<Camera>& cams, const Bounds& b, uint8_t ss,
-                   uint32_t size, bool force_two, std::vector<uint32_t>& status);
+                   uint32_t size, bool force_two, std::vector<uint32_t>& status,
+                   std::vector<RasterTask>* drawn = nullptr, std::vector<const double*>* depth = nullptr);
+    // Attribute rasters of drawn jobs: job j writes words*64 samples at
+    // normal_out/color_out (and depth_out, when given) + j*words*64.
+    void attribute_rasters(const std::vector<RasterJob>& jobs, const std::vector<RasterTask>& drawn,
+                           const std::vector<const double*>& depth, const std::vector<Camera>& cams, const Bounds& b,
+                           float4* normal_out, float4* color_out, float* depth_out);
     void run_items(const DeviceMesh& reference, const std::vector<const DeviceMesh*>& candidates,
                    const std::vector<Camera>& cams, const Bounds& b, const Settings& s, const std::vector<Item>& items,
                    std::vector<ViewResult>& out);
@@ -162,7 +204,7 @@ class Audit {
     int device_;
     uint64_t memory_bytes_;
     hipStream_t stream_{};
-    hipEvent_t events_[8]{};
+    hipEvent_t events_[10]{};
     std::vector<std::pair<void*, size_t>> slots_;
     uint64_t next_serial_ = 1;
     Timing timing_;

@@ -126,6 +126,16 @@ HostMesh host_mesh(const blitz_mesh* m) {
     h.indices.assign(m->indices, m->indices + m->index_count);
     if (m->material_count) h.materials.assign(m->materials, m->materials + m->material_count);
     if (m->double_sided_count) h.double_sided.assign(m->double_sided, m->double_sided + m->double_sided_count);
+    if (m->normals.count) {
+        h.normals.resize(n * 3);
+        const auto* nb = static_cast<const unsigned char*>(m->normals.data);
+        for (size_t i = 0; i < n; ++i) std::memcpy(&h.normals[3 * i], nb + i * m->normals.stride, 12);
+    }
+    if (m->colors.count) {
+        h.colors.resize(n * 4);
+        const auto* cb = static_cast<const unsigned char*>(m->colors.data);
+        for (size_t i = 0; i < n; ++i) std::memcpy(&h.colors[4 * i], cb + i * m->colors.stride, 4);
+    }
     Digest d;
     digest_stream(d, m->positions, 12);
     digest_stream(d, m->normals, 12);
@@ -473,6 +483,48 @@ blitz_status blitz_hip_raster_coverage(blitz_hip_audit* a, const blitz_hip_devic
     return status;
 }
 
+blitz_status blitz_hip_raster_pixels(blitz_hip_audit* a, const blitz_hip_device_mesh* m, const blitz_hip_bounds* b,
+                                     const blitz_hip_camera* camera, double screen, uint8_t ss, uint8_t force_two,
+                                     uint8_t* covered, uint8_t* visible, float* depth, float* normal, float* color,
+                                     uint16_t* material, size_t sample_capacity, uint32_t* size_out, uint8_t* clipped,
+                                     char* error, size_t capacity) {
+    bool limited = false;
+    const blitz_status status = guarded(error, capacity, [&] {
+        if (!camera || !covered || !visible || !depth || !normal || !color || !material || !size_out || !clipped)
+            throw std::invalid_argument("null raster output");
+        Audit& audit = audit_of(a);
+        Camera c{};
+        std::memcpy(&c.right, camera->right, 12);
+        std::memcpy(&c.up, camera->up, 12);
+        std::memcpy(&c.forward, camera->forward, 12);
+        c.distance = camera->distance;
+        c.focal = camera->focal;
+        c.scale = camera->scale;
+        c.perspective = camera->perspective;
+        Audit::PixelRaster r;
+        if (!audit.raster_pixels(mesh_of(a, m), bounds_from(b), c, screen, ss, force_two != 0, r)) {
+            *size_out = r.size;
+            limited = true;
+            return;
+        }
+        const size_t n = size_t(r.size) * r.size;
+        if (n > sample_capacity) throw std::invalid_argument("sample capacity too small");
+        std::memcpy(covered, r.covered.data(), n);
+        std::memcpy(visible, r.visible.data(), n);
+        std::memcpy(depth, r.depth.data(), n * 4);
+        std::memcpy(normal, r.normal.data(), n * 12);
+        std::memcpy(color, r.color.data(), n * 16);
+        std::memcpy(material, r.material.data(), n * 2);
+        *size_out = r.size;
+        *clipped = r.clipped;
+    });
+    if (status == BLITZ_OK && limited) {
+        message(error, capacity, "raster exceeds per-view memory budget");
+        return BLITZ_BUDGET_LIMITED;
+    }
+    return status;
+}
+
 blitz_status blitz_hip_coverage_distance(blitz_hip_audit* a, const uint64_t* x, const uint64_t* y, uint32_t size,
                                          uint8_t ss, double* out, char* error, size_t capacity) {
     return guarded(error, capacity, [&] {
@@ -501,6 +553,8 @@ blitz_status blitz_hip_audit_last_timing(const blitz_hip_audit* a, blitz_hip_aud
     out->band_launches = t.band_launches;
     out->patch_fallbacks = t.patch_fallbacks;
     out->patch_ms = t.patch_ms;
+    out->appearance_ms = t.appearance_ms;
+    out->attributed_views = t.attributed_views;
     return BLITZ_OK;
 }
 

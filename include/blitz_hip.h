@@ -119,6 +119,9 @@ typedef struct blitz_hip_audit_timing {
     uint32_t band_launches;
     uint32_t patch_fallbacks;  /* patches audited by the full path */
     double patch_ms;           /* patch kernel launches (device time) */
+    double appearance_ms;      /* attribute rasters, normal angle, attributed distance (device time) */
+    uint32_t attributed_views; /* views whose attributed distance was measured */
+    uint32_t reserved;
 } blitz_hip_audit_timing;
 
 /* ---- audit ---------------------------------------------------------------- */
@@ -229,6 +232,19 @@ blitz_status blitz_hip_raster_coverage(blitz_hip_audit*, const blitz_hip_device_
                                        double screen, uint8_t supersample, uint8_t force_two_sided,
                                        uint64_t* words, size_t word_capacity, uint32_t* size_out,
                                        uint8_t* clipped, char* error, size_t capacity);
+/* The attributed instantiation of blitz::detail::rasterize (src/evaluate.cpp:125-189,
+ * Pixel of include/blitz/evaluate.hpp:40-44) for one camera, row-major
+ * (index = y*size + x): covered and visible flags, the stored float depth
+ * (+inf where nothing is visible), normal xyz, colour rgba and material per
+ * sample; each array holds size*size entries (normal 3, colour 4 per sample)
+ * and `sample_capacity` is the entry count of the byte arrays. Returns
+ * BLITZ_BUDGET_LIMITED over the 64M-sample cap, with *size_out set. */
+blitz_status blitz_hip_raster_pixels(blitz_hip_audit*, const blitz_hip_device_mesh*,
+                                     const blitz_hip_bounds*, const blitz_hip_camera*,
+                                     double screen, uint8_t supersample, uint8_t force_two_sided,
+                                     uint8_t* covered, uint8_t* visible, float* depth, float* normal,
+                                     float* color, uint16_t* material, size_t sample_capacity,
+                                     uint32_t* size_out, uint8_t* clipped, char* error, size_t capacity);
 /* blitz::detail::coverage_distance (src/coverage.hpp:13, src/evaluate.cpp:196-216)
  * on two row-major packed masks of side `size`. */
 blitz_status blitz_hip_coverage_distance(blitz_hip_audit*, const uint64_t* a, const uint64_t* b,

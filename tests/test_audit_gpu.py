@@ -277,12 +277,13 @@ def test_evaluate_batch_matches_single(audit, oracle):
         assert_same_measurement(b, oracle.evaluate(ico, c, center, radius, s), f"candidate {k}")
 
 
-def test_unsupported_profile_is_loud(audit, oracle):
+def test_unsupported_patch_profile_is_loud(audit, oracle):
+    # Bound references (patch audits) implement the coverage profile only.
     m = meshes.grid(4)
     center, radius = oracle.bounds(m)
     with pytest.raises(Exception) as e:
-        audit.evaluate(m, m, center, radius, EvalSettings(profile=1, screen_size=16, supersample=1,
-                                                          max_supersample=1, orthographic=1, perspective=0))
+        audit.bind(m, center, radius, EvalSettings(profile=1, screen_size=16, supersample=1, max_supersample=1,
+                                                   orthographic=1, perspective=0))
     assert "UNAVAILABLE" in str(e.value)
 
 
