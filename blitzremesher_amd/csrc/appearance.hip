@@ -74,7 +74,7 @@ void Audit::attribute_rasters(const std::vector<RasterJob>& jobs, const std::vec
         t.bin_offsets = offsets + j * words;
         t.bin_faces = faces;
         t.normal_out = normal_out + j * per;
-        t.color_out = color_out + j * per;
+        t.color_out = color_out ? color_out + j * per : nullptr;
         t.depth_out = depth_out ? depth_out + j * per : nullptr;
         t.diameter = b.radius * 2;  // src/evaluate.cpp:135
         t.perspective = cams[jobs[j].view].perspective;

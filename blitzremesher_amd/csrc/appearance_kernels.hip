@@ -22,6 +22,7 @@
 // samples whose cost cannot be lower, so the minimum is the reference's.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cfloat>
 #include <cmath>
 
@@ -206,7 +207,7 @@ __global__ __launch_bounds__(256) void attribute_kernel(const AttrTask* __restri
     const uint32_t win = last ? last - 1 : k;
     if (!real || win == kNoFace) {
         T.normal_out[out] = make_float4(0, 0, 0, 0);
-        T.color_out[out] = make_float4(0, 0, 0, __uint_as_float(0));
+        if (T.color_out) T.color_out[out] = make_float4(0, 0, 0, __uint_as_float(0));
         if (T.depth_out) T.depth_out[out] = INFINITY;
         return;
     }
@@ -260,7 +261,7 @@ __global__ __launch_bounds__(256) void attribute_kernel(const AttrTask* __restri
     const uint32_t material = T.materials ? T.materials[win] : 0u;
     col.w = __uint_as_float(material);
     T.normal_out[out] = make_float4(n.x, n.y, n.z, 1.f);
-    T.color_out[out] = col;
+    if (T.color_out) T.color_out[out] = col;
     if (T.depth_out) T.depth_out[out] = S;
 }
 
@@ -269,21 +270,35 @@ __device__ __forceinline__ double wave_max(double v) {
     return v;
 }
 
-// Blocks of 256 samples; grid (blocks per raster, pairs).
+// Largest value of a 256-thread block, valid in thread 0.
+__device__ __forceinline__ double block_max(double v) {
+    __shared__ double part[4];
+    v = wave_max(v);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = v;
+    __syncthreads();
+    return fmax(fmax(part[0], part[1]), fmax(part[2], part[3]));
+}
+
+__device__ __forceinline__ double load_max(const unsigned long long* p) {
+    return __longlong_as_double((long long)__hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+}
+
+constexpr uint32_t kReduceBlocks = 256;  // blocks per raster pair of the reductions
+
+// Grid (blocks, pairs); each block strides over the pair's samples.
 __global__ __launch_bounds__(256) void normal_angle_kernel(const AttrPair* __restrict__ pairs) {
     const AttrPair& p = pairs[blockIdx.y];
-    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
     const uint32_t n = p.tiles * p.tiles * 64;
     double a = 0;
-    if (i < n) {
+    for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
         const float4 x = p.normal_a[i], y = p.normal_c[i];
         if (x.w != 0 && y.w != 0) {
             const double v = metric_acos(clamp1(dot3(xyz(x), xyz(y))));
             if (v > a) a = v;  // a NaN never raises the maximum
         }
     }
-    a = wave_max(a);
-    if ((threadIdx.x & 63) == 0 && a > 0) atomicMax(&p.out[0], (unsigned long long)__double_as_longlong(a));
+    a = block_max(a);
+    if (threadIdx.x == 0 && a > 0) atomicMax(&p.out[0], (unsigned long long)__double_as_longlong(a));
 }
 
 // src/evaluate.cpp:83-95
@@ -304,9 +319,15 @@ __device__ __forceinline__ double sample_cost(float4 na, float4 ca, float4 nb, f
     return cost;
 }
 
-// Grid (blocks per raster, pairs, 2 directions): per visible sample of the
-// source raster, the reference's windowed minimum cost (src/evaluate.cpp:
-// 229-262); out[1] receives the largest one.
+// Grid (blocks, pairs, 2 directions); each thread strides over samples of
+// the source raster and, for each visible one, finds the reference's
+// windowed minimum cost (src/evaluate.cpp:229-262). out[1] holds the largest
+// minimum found so far, shared by both directions as the reference's
+// `maximum` is. A sample whose running minimum is already <= that value
+// cannot raise it and stops (the early break of Taha and Hanbury, "An
+// efficient algorithm for calculating the exact Hausdorff distance", IEEE
+// TPAMI 2015): out[1] ends at the reference's maximum whenever that is within
+// limit^2, and above limit^2 (an infinite distance) otherwise.
 __global__ __launch_bounds__(256) void attributed_kernel(const AttrPair* __restrict__ pairs, CostSettings s) {
     const AttrPair& p = pairs[blockIdx.y];
     const bool reverse = blockIdx.z != 0;
@@ -314,37 +335,58 @@ __global__ __launch_bounds__(256) void attributed_kernel(const AttrPair* __restr
     const float4* fc = reverse ? p.color_c : p.color_a;
     const float4* tn = reverse ? p.normal_a : p.normal_c;
     const float4* tc = reverse ? p.color_a : p.color_c;
-    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
     const uint32_t n = p.tiles * p.tiles * 64;
-    double worst = 0;
-    if (i < n && fn[i].w != 0) {
-        const float4 an = fn[i], ac = fc[i];
+    const int size = int(p.size);
+    const double limit2 = s.limit * s.limit, invs2 = 1.0 / double(s.ss * s.ss);
+    const int radius = int(fmin(double(size), ceil(s.limit * s.ss)));
+    unsigned long long* slot = &p.out[1];
+    // Colours and materials enter the cost under the attributes profile only.
+    const bool colored = s.profile == 2;
+    const float4 none = make_float4(0, 0, 0, 0);
+    for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+        if (fn[i].w == 0) continue;
+        const float4 an = fn[i], ac = colored ? fc[i] : none;
+        double best = INFINITY;
+        if (tn[i].w != 0) best = sample_cost(an, ac, tn[i], colored ? tc[i] : none, 0, s);
+        if (best <= 1e-18) continue;
+        double known = load_max(slot);
+        if (known > limit2) return;  // the distance is already infinite
+        if (best <= known) continue;
         const uint32_t tile = i >> 6, lane = i & 63;
         const int x = int((tile % p.tiles) * 8 + (lane & 7)), y = int((tile / p.tiles) * 8 + (lane >> 3));
-        const int size = int(p.size);
-        const double limit2 = s.limit * s.limit, invs2 = 1.0 / double(s.ss * s.ss);
-        double best = INFINITY;
-        if (tn[i].w != 0) best = sample_cost(an, ac, tn[i], tc[i], 0, s);
-        if (best > 1e-18) {
-            const int radius = int(fmin(double(size), ceil(s.limit * s.ss)));
-            const int r = int(fmin(double(radius), ceil(sqrt(fmin(best, limit2)) * s.ss)));
-            const int y0 = max(0, y - r), y1 = min(size - 1, y + r);
-            const int x0 = max(0, x - r), x1 = min(size - 1, x + r);
-            for (int yy = y0; yy <= y1; ++yy)
-                for (int xx = x0; xx <= x1; ++xx) {
-                    const uint32_t j = tile_sample(p.tiles, uint32_t(xx), uint32_t(yy));
-                    const float4 bn = tn[j];
-                    if (bn.w == 0) continue;
-                    const double dx = x - xx, dy = y - yy, spatial = (dx * dx + dy * dy) * invs2;
-                    if (spatial >= best || spatial > limit2) continue;
-                    best = fmin(best, sample_cost(an, ac, bn, tc[j], spatial, s));
-                }
-            worst = best;
+        const int r = int(fmin(double(radius), ceil(sqrt(fmin(best, limit2)) * s.ss)));
+        // The reference scans the (2r+1)^2 window row by row; its minimum
+        // does not depend on the order, so the window is walked in square
+        // rings around the sample. Every sample of ring k has spatial >=
+        // k*k*invs2 (the same rounded expression at dx = k, dy = 0), and the
+        // reference skips samples with spatial >= best or > limit^2, so the
+        // walk ends at the first ring past either.
+        auto visit = [&](int xx, int yy) {
+            if (xx < 0 || yy < 0 || xx >= size || yy >= size) return;
+            const uint32_t j = tile_sample(p.tiles, uint32_t(xx), uint32_t(yy));
+            const float4 bn = tn[j];
+            if (bn.w == 0) return;
+            const double dx = x - xx, dy = y - yy, spatial = (dx * dx + dy * dy) * invs2;
+            if (spatial >= best || spatial > limit2) return;
+            const double cost = sample_cost(an, ac, bn, colored ? tc[j] : none, spatial, s);
+            if (cost < best) best = cost;
+        };
+        for (int k = 1; k <= r; ++k) {
+            const double ring = double(k * k) * invs2;
+            if (ring >= best || ring > limit2) break;
+            for (int d = -k; d <= k; ++d) {
+                visit(x + d, y - k);
+                visit(x + d, y + k);
+            }
+            for (int d = -k + 1; d <= k - 1; ++d) {
+                visit(x - k, y + d);
+                visit(x + k, y + d);
+            }
+            known = load_max(slot);
+            if (best <= known) break;
         }
+        if (best > known) atomicMax(slot, (unsigned long long)__double_as_longlong(best));
     }
-    worst = wave_max(worst);
-    if ((threadIdx.x & 63) == 0 && worst > 0)
-        atomicMax(&p.out[1], (unsigned long long)__double_as_longlong(worst));
 }
 
 }  // namespace
@@ -355,15 +397,17 @@ void launch_attribute_raster(const AttrTask* tasks, uint32_t count, uint32_t til
     attribute_kernel<<<dim3(blocks, count), dim3(64 * kWaves), 0, stream>>>(tasks);
 }
 
+uint32_t reduce_blocks(uint32_t tiles) { return std::min(kReduceBlocks, (tiles * tiles * 64 + 255) / 256); }
+
 void launch_normal_angle(const AttrPair* pairs, uint32_t count, uint32_t tiles, hipStream_t stream) {
     if (!count) return;
-    normal_angle_kernel<<<dim3((tiles * tiles * 64 + 255) / 256, count), dim3(256), 0, stream>>>(pairs);
+    normal_angle_kernel<<<dim3(reduce_blocks(tiles), count), dim3(256), 0, stream>>>(pairs);
 }
 
 void launch_attributed(const AttrPair* pairs, uint32_t count, uint32_t tiles, const CostSettings& s,
                        hipStream_t stream) {
     if (!count) return;
-    attributed_kernel<<<dim3((tiles * tiles * 64 + 255) / 256, count, 2), dim3(256), 0, stream>>>(pairs, s);
+    attributed_kernel<<<dim3(reduce_blocks(tiles), count, 2), dim3(256), 0, stream>>>(pairs, s);
 }
 
 }  // namespace blitz_hip

@@ -23,7 +23,7 @@ struct AttrTask {
     const unsigned long long* bin_offsets;  // this task's tiles*tiles cells (+1 past the end)
     const uint32_t* bin_faces;
     float4* normal_out;  // xyz; w = 1 when visible, 0 otherwise
-    float4* color_out;   // rgb; w = material id bits (uint32)
+    float4* color_out;   // rgb; w = material id bits (uint32); null when not needed
     float* depth_out;    // optional: the stored float depth, +inf where nothing is visible
     double diameter;
     uint8_t perspective;
@@ -37,7 +37,7 @@ struct AttrPair {
     const float4* normal_a;
     const float4* color_a;
     const float4* normal_c;
-    const float4* color_c;
+    const float4* color_c;    // colours may be null unless the profile is attributes
     unsigned long long* out;  // 2 per pair
     uint32_t size, tiles;
 };

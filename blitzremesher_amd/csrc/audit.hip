@@ -596,15 +596,16 @@ void Audit::run_items(const DeviceMesh& reference, const std::vector<const Devic
         if (attributed) {
             hip_check(hipEventRecord(events_[8], stream_), "event");
             float4* anorm = scratch_as<float4>(30, jobs.size() * per);
-            float4* acol = scratch_as<float4>(31, jobs.size() * per);
+            // Colours and materials matter to the attributes profile only.
+            float4* acol = s.profile == 2 ? scratch_as<float4>(31, jobs.size() * per) : nullptr;
             attribute_rasters(jobs, drawn, zdepth, cams, b, anorm, acol, nullptr);
             aout = scratch_as<unsigned long long>(32, 2 * size_t(ncand));
             launch_fill(reinterpret_cast<uint64_t*>(aout), 2 * size_t(ncand), 0, stream_);
             apairs.resize(ncand);
             for (uint32_t i = 0; i < ncand; ++i) {
                 const size_t ra = ref_job[ref_slot[all[begin + i].view]], rc = cand_job[i];
-                apairs[i] = {anorm + ra * per, acol + ra * per, anorm + rc * per, acol + rc * per, aout + 2 * size_t(i),
-                             size, tiles};
+                apairs[i] = {anorm + ra * per, acol ? acol + ra * per : nullptr, anorm + rc * per,
+                             acol ? acol + rc * per : nullptr, aout + 2 * size_t(i), size, tiles};
             }
             AttrPair* dap = scratch_as<AttrPair>(33, ncand);
             hip_check(hipMemcpyAsync(dap, apairs.data(), ncand * sizeof(AttrPair), hipMemcpyHostToDevice, stream_),

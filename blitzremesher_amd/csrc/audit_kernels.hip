@@ -278,11 +278,13 @@ __global__ __launch_bounds__(1024) void row_distance_kernel(const ViewPair* __re
 
 // Column pass restricted to the samples that can set the directed Hausdorff
 // maximum: every covered sample of `from` that is not covered in `to`. Each
-// query's exact squared distance is min over rows y' of (y-y')^2 + g(y'),
-// g = rounded row distance (the reference's lower envelope yields the same
-// minimum; its float output is a monotone rounding of it, applied once on the
-// host). The scan stops once (y-y')^2 reaches the best value or the best value
-// cannot exceed the maximum already found.
+// wavefront reads 64 tile words (one per lane) and then takes the words with
+// such samples one after another, one lane per sample, so the queries of a
+// tile run side by side. Each query's exact squared distance is min over rows
+// y' of (y-y')^2 + g(y'), g = rounded row distance (the reference's lower
+// envelope yields the same minimum; its float output is a monotone rounding
+// of it, applied once on the host). The scan stops once (y-y')^2 reaches the
+// best value or the best value cannot exceed the maximum already found.
 __global__ __launch_bounds__(256) void hausdorff_kernel(const ViewPair* __restrict__ pairs,
                                                         ViewSummary* summaries) {
     const uint32_t job = blockIdx.y;  // view*2 + direction; direction 0: A -> C
@@ -294,38 +296,42 @@ __global__ __launch_bounds__(256) void hausdorff_kernel(const ViewPair* __restri
     const uint16_t* dist = (job & 1) ? p.dist_a : p.dist_c;
     uint32_t* target = (job & 1) ? &s.max_ca : &s.max_ac;
     const size_t words = size_t(p.tiles) * p.tiles;
-    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= words) return;
-    uint64_t bits = from[i] & ~to[i];
-    if (!bits) return;
-    const uint32_t tx = uint32_t(i % p.tiles), ty = uint32_t(i / p.tiles);
+    const uint32_t lane = threadIdx.x & 63;
+    const size_t first = (size_t(blockIdx.x) * 4 + (threadIdx.x >> 6)) * 64;
+    if (first >= words) return;  // whole waves
+    const size_t mine = first + lane;
+    const uint64_t my_bits = mine < words ? from[mine] & ~to[mine] : 0;
+    uint64_t live = __ballot(my_bits != 0);
+    if (!live) return;  // whole waves
     const uint32_t size = p.size;
     const bool dense = dense_rows(s, size);
-    uint32_t x_of_query = 0;
-    // Rounded squared row distance of row r, or kFar. Sparse views search the
-    // mask only as far as a site could still improve `best`; the rounding of
-    // d^2 above 2^24 is covered by the +2 column margin.
-    auto row_term = [&](uint32_t r, uint32_t best, uint32_t kk) -> uint32_t {
-        uint32_t d;
-        if (dense) {
-            const uint16_t v = dist[size_t(r) * size + x_of_query];
-            d = v == kNoSite ? kFar : v;
-        } else {
-            const uint32_t room = best == kFar ? kFar : best - kk;
-            const uint32_t reach = room == kFar ? size : uint32_t(sqrtf(float(room))) + 2;
-            d = nearest_in_row(to, p.tiles, x_of_query, r, reach);
-        }
-        return d == kFar ? kFar : kk + rounded_square(d);
-    };
     uint32_t known = __hip_atomic_load(target, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     uint32_t local = 0;
-    while (bits) {
-        const int b = __ffsll(static_cast<unsigned long long>(bits)) - 1;
-        bits &= bits - 1;
-        x_of_query = (tx << 3) | uint32_t(b & 7);
-        const uint32_t y = (ty << 3) | uint32_t(b >> 3);
-        uint32_t best = row_term(y, kFar, 0);
+    while (live) {
+        const int w = __ffsll(static_cast<unsigned long long>(live)) - 1;
+        live &= live - 1;
+        const uint64_t bits = __shfl(my_bits, w);
+        if (!((bits >> lane) & 1)) continue;
+        const size_t i = first + uint32_t(w);
+        const uint32_t tx = uint32_t(i % p.tiles), ty = uint32_t(i / p.tiles);
+        const uint32_t x = (tx << 3) | (lane & 7), y = (ty << 3) | (lane >> 3);
+        // Rounded squared row distance of row r, or kFar. Sparse views search
+        // the mask only as far as a site could still improve `best`; the
+        // rounding of d^2 above 2^24 is covered by the +2 column margin.
+        auto row_term = [&](uint32_t r, uint32_t best, uint32_t kk) -> uint32_t {
+            uint32_t d;
+            if (dense) {
+                const uint16_t v = dist[size_t(r) * size + x];
+                d = v == kNoSite ? kFar : v;
+            } else {
+                const uint32_t room = best == kFar ? kFar : best - kk;
+                const uint32_t reach = room == kFar ? size : uint32_t(sqrtf(float(room))) + 2;
+                d = nearest_in_row(to, p.tiles, x, r, reach);
+            }
+            return d == kFar ? kFar : kk + rounded_square(d);
+        };
         const uint32_t floor_value = max(known, local);
+        uint32_t best = row_term(y, kFar, 0);
         for (uint32_t k = 1; best > floor_value && uint64_t(k) * k < best; ++k) {
             const uint32_t kk = k * k;
             const bool up = y >= k, down = y + k < size;
@@ -335,7 +341,8 @@ __global__ __launch_bounds__(256) void hausdorff_kernel(const ViewPair* __restri
         }
         if (best > floor_value && best != kFar) local = best;
     }
-    if (local > known) atomicMax(target, local);
+    for (int o = 32; o > 0; o >>= 1) local = max(local, uint32_t(__shfl_xor(int(local), o)));
+    if (lane == 0 && local > known) atomicMax(target, local);
 }
 
 }  // namespace
