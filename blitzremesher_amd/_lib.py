@@ -150,6 +150,8 @@ PROTOTYPES = {
                                           C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float),
                                           C.POINTER(C.c_uint16), C.c_size_t, C.POINTER(C.c_uint32),
                                           C.POINTER(C.c_uint8), *_ERR]),
+    "blitz_hip_overlap": (C.c_int, [_P, _P, C.POINTER(Bounds), C.c_double, C.c_uint16, C.c_uint16, C.c_uint32,
+                                    C.POINTER(C.c_double), *_ERR]),
     "blitz_hip_coverage_distance": (C.c_int, [_P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.c_uint32,
                                               C.c_uint8, C.POINTER(C.c_double), *_ERR]),
     "blitz_hip_audit_last_timing": (C.c_int, [_P, C.POINTER(AuditTiming)]),

@@ -434,6 +434,23 @@ class Audit:
         _lib.check(st, err)
         return out, int(out_size.value), bool(clipped.value)
 
+    def overlap(self, mesh, center, radius, screen: float, orthographic: int, perspective: int = 0,
+                seed: int = 0xB1172026) -> float:
+        """overlap_cuda (include/blitz/neural.hpp:150-151): mean overdraw samples
+        per covered pixel under the top-left fill rule; inf when any view is
+        clipped. `mesh` is a Mesh or a DeviceMesh."""
+        dev = mesh if isinstance(mesh, DeviceMesh) else self.upload(mesh)
+        try:
+            out = C.c_double()
+            b = _bounds_c(center, radius)
+            err = _lib.error_buffer()
+            _lib.check(self.lib.blitz_hip_overlap(self.handle, dev.handle, C.byref(b), screen, orthographic,
+                                                  perspective, seed & 0xFFFFFFFF, C.byref(out), err, len(err)), err)
+            return float(out.value)
+        finally:
+            if dev is not mesh:
+                dev.release()
+
     def coverage_distance(self, a: np.ndarray, b: np.ndarray, size: int, ss: int) -> float:
         a = np.ascontiguousarray(a, dtype=np.uint64)
         b = np.ascontiguousarray(b, dtype=np.uint64)

@@ -1,9 +1,11 @@
 // Host side of the normals and attributes audit profiles: per-tile face bins
-// of every drawn raster job, the attribute raster launch, and the row-major
-// attribute raster of one view (src/evaluate.cpp:125-189).
+// of every drawn raster job, the attribute raster launch, the row-major
+// attribute raster of one view (src/evaluate.cpp:125-189) and the
+// top-left overlap measure (src/neural/evaluate.cu:2665-2685).
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <limits>
 
 #include "appearance_kernels.hpp"
 #include "audit.hpp"
@@ -29,13 +31,11 @@ constexpr uint64_t kMaxSamples = 64000000;  // src/evaluate.cpp:129
 constexpr uint32_t kNone = 0xFFFFFFFFu;
 }  // namespace
 
-void Audit::attribute_rasters(const std::vector<RasterJob>& jobs, const std::vector<RasterTask>& drawn,
-                              const std::vector<const double*>& depth, const std::vector<Camera>& cams,
-                              const Bounds& b, float4* normal_out, float4* color_out, float* depth_out) {
-    const size_t n = jobs.size();
-    if (!n) return;
+void Audit::build_bins(const std::vector<RasterTask>& drawn, const unsigned long long*& offsets_out,
+                       const uint32_t*& faces_out) {
+    const size_t n = drawn.size();
     const uint32_t tiles = drawn[0].tiles;
-    const size_t words = size_t(tiles) * tiles, per = words * 64;
+    const size_t words = size_t(tiles) * tiles;
     uint32_t max_faces = 0;
     for (const auto& t : drawn) max_faces = std::max(max_faces, t.faces);
     // Per (job, tile) face bins: count, exclusive scan, fill.
@@ -60,6 +60,20 @@ void Audit::attribute_rasters(const std::vector<RasterJob>& jobs, const std::vec
     for (size_t j0 = 0; j0 < n; j0 += kMaxGridY)
         launch_bin_fill(dtasks + j0, uint32_t(std::min<size_t>(kMaxGridY, n - j0)), max_faces, counts + j0 * words,
                         faces, stream_);
+    offsets_out = offsets;
+    faces_out = faces;
+}
+
+void Audit::attribute_rasters(const std::vector<RasterJob>& jobs, const std::vector<RasterTask>& drawn,
+                              const std::vector<const double*>& depth, const std::vector<Camera>& cams,
+                              const Bounds& b, float4* normal_out, float4* color_out, float* depth_out) {
+    const size_t n = jobs.size();
+    if (!n) return;
+    const uint32_t tiles = drawn[0].tiles;
+    const size_t words = size_t(tiles) * tiles, per = words * 64;
+    const unsigned long long* offsets = nullptr;
+    const uint32_t* faces = nullptr;
+    build_bins(drawn, offsets, faces);
     std::vector<AttrTask> tasks(n);
     for (size_t j = 0; j < n; ++j) {
         const DeviceMesh& m = *jobs[j].mesh;
@@ -85,6 +99,59 @@ void Audit::attribute_rasters(const std::vector<RasterJob>& jobs, const std::vec
     for (size_t j0 = 0; j0 < n; j0 += kMaxGridY)
         launch_attribute_raster(dattr + j0, uint32_t(std::min<size_t>(kMaxGridY, n - j0)), tiles, stream_);
     hip_check(hipGetLastError(), "attribute raster launch");
+}
+
+// overlap_cuda (src/neural/evaluate.cu:2665-2685): every view is drawn at one
+// sample per pixel on an even raster side (src/neural/evaluate.cu:599); a view
+// with a face behind the camera or out of frame makes the result infinite.
+double Audit::overlap(const DeviceMesh& m, const Bounds& b, double screen, uint16_t ortho, uint16_t persp,
+                      uint32_t seed) {
+    if (!std::isfinite(screen) || screen <= 0 || screen > 16384) throw std::invalid_argument("invalid raster extent");
+    const auto cams = cameras(b, screen, ortho, persp, seed);
+    if (cams.empty()) throw std::invalid_argument("overlap requires views");
+    const uint32_t size = (uint32_t(std::ceil(screen + 8)) + 1) & ~1u;
+    if (uint64_t(size) * size > kMaxSamples) throw std::invalid_argument("raster exceeds per-view sample cap");
+    DeviceGuard guard(device_);
+    const uint32_t tiles = (size + kTile - 1) / kTile;
+    const size_t words = size_t(tiles) * tiles;
+    const size_t n = cams.size();
+    uint64_t* masks = scratch_as<uint64_t>(kSlotPixelMask, n * words);
+    launch_fill(masks, n * words, 0, stream_);
+    std::vector<RasterJob> jobs(n);
+    for (size_t v = 0; v < n; ++v) jobs[v] = {&m, uint32_t(v), masks + v * words};
+    std::vector<uint32_t> st;
+    std::vector<RasterTask> drawn;
+    std::vector<const double*> zs;
+    rasterize(jobs, cams, b, 1, size, false, st, &drawn, &zs);
+    for (size_t v = 0; v < n; ++v)
+        if (st[2 * v] != kNone || st[2 * v + 1]) return std::numeric_limits<double>::infinity();
+    // Per-tile face bins of every view, then the per-sample counts.
+    std::vector<AttrTask> tasks;
+    const unsigned long long* offsets = nullptr;
+    const uint32_t* faces = nullptr;
+    build_bins(drawn, offsets, faces);
+    tasks.resize(n);
+    for (size_t v = 0; v < n; ++v) {
+        AttrTask& t = tasks[v];
+        t = {};
+        t.raster = drawn[v];
+        t.bin_offsets = offsets + v * words;
+        t.bin_faces = faces;
+    }
+    AttrTask* dattr = scratch_as<AttrTask>(kSlotAttrTasks, n);
+    hip_check(hipMemcpyAsync(dattr, tasks.data(), n * sizeof(AttrTask), hipMemcpyHostToDevice, stream_),
+              "upload overlap tasks");
+    auto* sums = scratch_as<unsigned long long>(kSlotPixelDepth, 2 * n);
+    launch_fill(reinterpret_cast<uint64_t*>(sums), 2 * n, 0, stream_);
+    for (size_t j0 = 0; j0 < n; j0 += kMaxGridY)
+        launch_overlap(dattr + j0, uint32_t(std::min<size_t>(kMaxGridY, n - j0)), tiles, sums + 2 * j0, stream_);
+    hip_check(hipGetLastError(), "overlap launch");
+    std::vector<unsigned long long> host(2 * n);
+    hip_check(hipMemcpyAsync(host.data(), sums, host.size() * 8, hipMemcpyDeviceToHost, stream_), "download overlap");
+    hip_check(hipStreamSynchronize(stream_), "overlap");
+    double total = 0;
+    for (size_t v = 0; v < n; ++v) total += host[2 * v + 1] ? double(host[2 * v]) / host[2 * v + 1] : 0;
+    return total / n;
 }
 
 bool Audit::raster_pixels(const DeviceMesh& m, const Bounds& b, const Camera& c, double screen, uint8_t ss,

@@ -99,13 +99,14 @@ __device__ __forceinline__ uint32_t tile_sample(uint32_t tiles, uint32_t x, uint
 }
 
 // One binned face after setup: corners after the back-face swap, their view
-// depths, the positive area and the sample box.
+// depths, the positive area, the sample box and the top-left flags of the
+// edges bc, ca, ab (bits 0, 1, 2).
 struct FaceRec {
     double ax, ay, bx, by, cx, cy;
     double za, zb, zc, area;
     int x0, x1, y0, y1;
     uint32_t face;
-    uint32_t pad;
+    uint32_t top_left;
 };
 
 constexpr int kWaves = 4;
@@ -131,6 +132,47 @@ __device__ __forceinline__ bool inside_depth(const FaceRec& r, double px, double
     return true;
 }
 
+// Setup of binned face f (src/evaluate.cpp:144-165) as a FaceRec, with the
+// top-left edge test of src/neural/evaluate.cu:198-200; depths are loaded when
+// zs is given.
+__device__ __forceinline__ bool top_left(double ax, double ay, double bx, double by) {
+    return by < ay || (by == ay && bx < ax);
+}
+__device__ FaceRec face_record(const AttrTask& T, uint32_t f, global_ptr<double> zs) {
+    const RasterTask& t = T.raster;
+    const uint3 tri = load_face(t, f);
+    const Corners c = load_corners(t, tri, f);
+    Setup s;
+    FaceRec r;
+    r.face = f;
+    if (setup_triangle(t, c, s) == 1) {
+        const double area = edge(c.a.x, c.a.y, c.b.x, c.b.y, c.c.x, c.c.y);
+        const bool back = area < 0;
+        r.ax = s.ax;
+        r.ay = s.ay;
+        r.bx = s.bx;
+        r.by = s.by;
+        r.cx = s.cx;
+        r.cy = s.cy;
+        r.area = back ? -area : area;
+        if (zs) {
+            r.za = zs[tri.x];
+            r.zb = back ? zs[tri.z] : zs[tri.y];
+            r.zc = back ? zs[tri.y] : zs[tri.z];
+        }
+        r.x0 = s.x0;
+        r.x1 = s.x1;
+        r.y0 = s.y0;
+        r.y1 = s.y1;
+        r.top_left = uint32_t(top_left(s.bx, s.by, s.cx, s.cy)) | uint32_t(top_left(s.cx, s.cy, s.ax, s.ay)) << 1 |
+                uint32_t(top_left(s.ax, s.ay, s.bx, s.by)) << 2;
+    } else {  // binned faces are drawable; keep an empty box regardless
+        r.x0 = r.y0 = 1;
+        r.x1 = r.y1 = 0;
+    }
+    return r;
+}
+
 __global__ __launch_bounds__(256) void attribute_kernel(const AttrTask* __restrict__ tasks) {
     __shared__ FaceRec recs[kWaves][64];
     const AttrTask& T = tasks[blockIdx.y];
@@ -151,36 +193,7 @@ __global__ __launch_bounds__(256) void attribute_kernel(const AttrTask* __restri
     FaceRec* mine = recs[wave];
     for (unsigned long long base = b0; base < b1; base += 64) {
         const uint32_t n = uint32_t(min(64ull, b1 - base));
-        if (lane < n) {
-            const uint32_t f = T.bin_faces[base + lane];
-            const uint3 tri = load_face(t, f);
-            const Corners c = load_corners(t, tri, f);
-            Setup s;
-            FaceRec r;
-            r.face = f;
-            if (setup_triangle(t, c, s) == 1) {
-                const double area = edge(c.a.x, c.a.y, c.b.x, c.b.y, c.c.x, c.c.y);
-                const bool back = area < 0;
-                r.ax = s.ax;
-                r.ay = s.ay;
-                r.bx = s.bx;
-                r.by = s.by;
-                r.cx = s.cx;
-                r.cy = s.cy;
-                r.area = back ? -area : area;
-                r.za = zs[tri.x];
-                r.zb = back ? zs[tri.z] : zs[tri.y];
-                r.zc = back ? zs[tri.y] : zs[tri.z];
-                r.x0 = s.x0;
-                r.x1 = s.x1;
-                r.y0 = s.y0;
-                r.y1 = s.y1;
-            } else {  // binned faces are drawable; keep an empty box regardless
-                r.x0 = r.y0 = 1;
-                r.x1 = r.y1 = 0;
-            }
-            mine[lane] = r;
-        }
+        if (lane < n) mine[lane] = face_record(T, T.bin_faces[base + lane], zs);
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         if (real)
@@ -263,6 +276,54 @@ __global__ __launch_bounds__(256) void attribute_kernel(const AttrTask* __restri
     T.normal_out[out] = make_float4(n.x, n.y, n.z, 1.f);
     if (T.color_out) T.color_out[out] = col;
     if (T.depth_out) T.depth_out[out] = S;
+}
+
+// Overlap (src/neural/evaluate.cu:301-309,352-355): per sample, the faces
+// whose interior holds its centre under the top-left fill rule; per view, the
+// sum of those counts and the number of samples with a non-zero count.
+__global__ __launch_bounds__(256) void overlap_kernel(const AttrTask* __restrict__ tasks,
+                                                      unsigned long long* __restrict__ sums) {
+    __shared__ FaceRec recs[kWaves][64];
+    const AttrTask& T = tasks[blockIdx.y];
+    const RasterTask& t = T.raster;
+    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const uint32_t words = t.tiles * t.tiles;
+    const uint32_t tile = blockIdx.x * kWaves + wave;
+    if (tile >= words) return;  // whole waves exit together
+    const uint32_t tx = tile % t.tiles, ty = tile / t.tiles;
+    const int x = int(tx * 8 + (lane & 7)), y = int(ty * 8 + (lane >> 3));
+    const bool real = x < int(t.size) && y < int(t.size);
+    const double px = x + .5, py = y + .5;
+    const unsigned long long b0 = T.bin_offsets[tile], b1 = T.bin_offsets[tile + 1];
+    uint32_t count = 0;
+    FaceRec* mine = recs[wave];
+    for (unsigned long long base = b0; base < b1; base += 64) {
+        const uint32_t n = uint32_t(min(64ull, b1 - base));
+        if (lane < n) mine[lane] = face_record(T, T.bin_faces[base + lane], global_ptr<double>(nullptr));
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        if (real)
+            for (uint32_t j = 0; j < n; ++j) {
+                const FaceRec& r = mine[j];
+                if (x < r.x0 || x > r.x1 || y < r.y0 || y > r.y1) continue;
+                const double u = edge(r.bx, r.by, r.cx, r.cy, px, py);
+                const double v = edge(r.cx, r.cy, r.ax, r.ay, px, py);
+                const double w = edge(r.ax, r.ay, r.bx, r.by, px, py);
+                if (u < 0 || v < 0 || w < 0) continue;
+                if ((u > 0 || (r.top_left & 1)) && (v > 0 || (r.top_left & 2)) && (w > 0 || (r.top_left & 4))) ++count;
+            }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+    }
+    unsigned long long samples = count, pixels = count != 0;
+    for (int o = 32; o > 0; o >>= 1) {
+        samples += __shfl_xor(samples, o);
+        pixels += __shfl_xor(pixels, o);
+    }
+    if (lane == 0 && pixels) {
+        atomicAdd(&sums[2 * blockIdx.y], samples);
+        atomicAdd(&sums[2 * blockIdx.y + 1], pixels);
+    }
 }
 
 __device__ __forceinline__ double wave_max(double v) {
@@ -398,6 +459,13 @@ void launch_attribute_raster(const AttrTask* tasks, uint32_t count, uint32_t til
 }
 
 uint32_t reduce_blocks(uint32_t tiles) { return std::min(kReduceBlocks, (tiles * tiles * 64 + 255) / 256); }
+
+void launch_overlap(const AttrTask* tasks, uint32_t count, uint32_t tiles, unsigned long long* sums,
+                    hipStream_t stream) {
+    if (!count) return;
+    const uint32_t blocks = (tiles * tiles + kWaves - 1) / kWaves;
+    overlap_kernel<<<dim3(blocks, count), dim3(64 * kWaves), 0, stream>>>(tasks, sums);
+}
 
 void launch_normal_angle(const AttrPair* pairs, uint32_t count, uint32_t tiles, hipStream_t stream) {
     if (!count) return;

@@ -10,8 +10,8 @@
 
 namespace blitz_hip {
 
-// One attribute raster: one mesh's faces over one projected vertex set,
-// drawn from per-tile face bins. Outputs are tile-major like the masks
+// One attribute raster (or overlap count): one mesh's faces over one
+// projected vertex set, drawn from per-tile face bins. Outputs are tile-major like the masks
 // (sample = tile * 64 + (y & 7) * 8 + (x & 7)), every sample written.
 struct AttrTask {
     RasterTask raster;                      // faces [0, faces), projection, sidedness, side, tiles
@@ -51,6 +51,10 @@ struct CostSettings {
 };
 
 void launch_attribute_raster(const AttrTask* tasks, uint32_t count, uint32_t tiles, hipStream_t stream);
+// Per task: sums[2j] = overlap samples, sums[2j+1] = samples with any (zeroed
+// by the caller).
+void launch_overlap(const AttrTask* tasks, uint32_t count, uint32_t tiles, unsigned long long* sums,
+                    hipStream_t stream);
 void launch_normal_angle(const AttrPair* pairs, uint32_t count, uint32_t tiles, hipStream_t stream);
 void launch_attributed(const AttrPair* pairs, uint32_t count, uint32_t tiles, const CostSettings& s,
                        hipStream_t stream);

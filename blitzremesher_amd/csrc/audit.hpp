@@ -149,6 +149,11 @@ class Audit {
     };
     bool raster_pixels(const DeviceMesh& m, const Bounds& b, const Camera& c, double screen, uint8_t ss,
                        bool force_two, PixelRaster& out);
+    // Mean over views of the overdraw samples per covered pixel under the
+    // top-left fill rule (overlap_cuda, src/neural/evaluate.cu:2665-2685);
+    // infinite when any view is clipped.
+    double overlap(const DeviceMesh& m, const Bounds& b, double screen, uint16_t ortho, uint16_t persp,
+                   uint32_t seed);
     // Binds a reference: per view its projection, mask, covered count and
     // per-tile face bins at s.supersample (src/neural/internal.hpp:141-144).
     std::unique_ptr<Binding> bind(const HostMesh& reference, const Bounds& b, const Settings& s);
@@ -188,6 +193,10 @@ class Audit {
     void rasterize(const std::vector<RasterJob>& jobs, const std::vector<Camera>& cams, const Bounds& b, uint8_t ss,
                    uint32_t size, bool force_two, std::vector<uint32_t>& status,
                    std::vector<RasterTask>* drawn = nullptr, std::vector<const double*>* depth = nullptr);
+    // Per (job, tile) bins of the faces each drawn job draws, face ids into
+    // `faces`; job j's cells start at offsets + j*tiles*tiles.
+    void build_bins(const std::vector<RasterTask>& drawn, const unsigned long long*& offsets,
+                    const uint32_t*& faces);
     // Attribute rasters of drawn jobs: job j writes words*64 samples at
     // normal_out/color_out (and depth_out, when given) + j*words*64.
     void attribute_rasters(const std::vector<RasterJob>& jobs, const std::vector<RasterTask>& drawn,

@@ -525,6 +525,15 @@ blitz_status blitz_hip_raster_pixels(blitz_hip_audit* a, const blitz_hip_device_
     return status;
 }
 
+blitz_status blitz_hip_overlap(blitz_hip_audit* a, const blitz_hip_device_mesh* m, const blitz_hip_bounds* b,
+                               double screen, uint16_t ortho, uint16_t persp, uint32_t seed, double* out, char* error,
+                               size_t capacity) {
+    return guarded(error, capacity, [&] {
+        if (!out) throw std::invalid_argument("null overlap output");
+        *out = audit_of(a).overlap(mesh_of(a, m), bounds_from(b), screen, ortho, persp, seed);
+    });
+}
+
 blitz_status blitz_hip_coverage_distance(blitz_hip_audit* a, const uint64_t* x, const uint64_t* y, uint32_t size,
                                          uint8_t ss, double* out, char* error, size_t capacity) {
     return guarded(error, capacity, [&] {

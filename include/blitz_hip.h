@@ -245,6 +245,14 @@ blitz_status blitz_hip_raster_pixels(blitz_hip_audit*, const blitz_hip_device_me
                                      uint8_t* covered, uint8_t* visible, float* depth, float* normal,
                                      float* color, uint16_t* material, size_t sample_capacity,
                                      uint32_t* size_out, uint8_t* clipped, char* error, size_t capacity);
+/* blitz::overlap_cuda (include/blitz/neural.hpp:150-151, src/neural/evaluate.cu:2665-2685):
+ * the mean over the views {orthographic, perspective, seed} of the overdraw
+ * samples per covered pixel, counting every drawable face whose interior holds
+ * a pixel centre under the top-left fill rule, at one sample per pixel;
+ * +inf when a face of any view lies behind the camera or out of frame. */
+blitz_status blitz_hip_overlap(blitz_hip_audit*, const blitz_hip_device_mesh*, const blitz_hip_bounds*,
+                               double screen, uint16_t orthographic, uint16_t perspective, uint32_t seed,
+                               double* out, char* error, size_t capacity);
 /* blitz::detail::coverage_distance (src/coverage.hpp:13, src/evaluate.cpp:196-216)
  * on two row-major packed masks of side `size`. */
 blitz_status blitz_hip_coverage_distance(blitz_hip_audit*, const uint64_t* a, const uint64_t* b,

@@ -24,7 +24,9 @@
 // tests/test_oracle.py to show that the pins catch plausible mistakes:
 // 1 drops the conservative radius of edge 0, 2 transposes an edge function's
 // operands, 3 keeps the row pass in double (one float rounding instead of the
-// reference's two), 4 breaks worst-view ties towards the later view.
+// reference's two), 4 breaks worst-view ties towards the later view,
+// 5 leaves back-face normals unflipped, 6 drops the material penalty, 7
+// counts pixel centres on every edge in the overlap count (no top-left rule).
 #ifndef ORACLE_PLANT
 #define ORACLE_PLANT 0
 #endif
@@ -615,6 +617,72 @@ Measurement evaluate(const Mesh& ref, const Mesh& cand, const Bounds& b, const S
     return result;
 }
 
+// overlap_cuda (src/neural/evaluate.cu:2665-2685): per view, drawn at one
+// sample per pixel on the even raster side of src/neural/evaluate.cu:599, the
+// drawable faces (setup of src/neural/evaluate.cu:119-176: no stop at an
+// out-of-frame face, which only marks the view clipped) are counted per pixel
+// when its centre is strictly inside or on a top-left edge
+// (src/neural/evaluate.cu:198-200,301-309). A clipped view makes the result
+// infinite; otherwise the mean over views of samples / pixels (0 for a view
+// with no counted pixel).
+double overlap(const Mesh& m, const Bounds& b, double screen, uint16_t ortho, uint16_t persp, uint32_t seed) {
+    if (!std::isfinite(screen) || screen <= 0 || screen > 16384) throw std::invalid_argument("invalid raster extent");
+    const auto cams = make_cameras(b, screen, ortho, persp, seed);
+    if (cams.empty()) throw std::invalid_argument("overlap requires views");
+    const uint32_t size = (uint32_t(std::ceil(screen + 8)) + 1) & ~1u;
+    if (uint64_t(size) * size > kMaxSamples) throw LengthError{};
+    struct P { double x, y, z; };
+    auto top_left = [](P a, P c) { return c.y < a.y || (c.y == a.y && c.x < a.x); };
+    double total = 0;
+    for (const Camera& c : cams) {
+        std::vector<P> proj(m.vertices);
+        for (uint64_t i = 0; i < m.vertices; ++i) {
+            const F3 a = m.vertex(i);
+            const double x = double(a.x) - b.center.x, y = double(a.y) - b.center.y, z0 = double(a.z) - b.center.z;
+            auto along = [&](F3 axis) { return x * axis.x + y * axis.y + z0 * axis.z; };
+            const double z = c.distance - along(c.forward);
+            const double scale = c.perspective ? c.focal / z : c.scale;
+            proj[i] = {along(c.right) * scale + size * .5, along(c.up) * scale + size * .5, z};
+        }
+        std::vector<uint32_t> count(size_t(size) * size, 0);
+        bool clipped = false;
+        for (uint64_t f = 0; f < m.faces(); ++f) {
+            P a = proj[m.indices[3 * f]], d = proj[m.indices[3 * f + 1]], e = proj[m.indices[3 * f + 2]];
+            if (c.perspective && (a.z <= 0 || d.z <= 0 || e.z <= 0)) { clipped = true; continue; }
+            const double area = edge(a.x, a.y, d.x, d.y, e.x, e.y);
+            if (std::abs(area) < 1e-16) continue;
+            const bool back = area < 0;
+            if (back && !m.two_sided(f)) continue;
+            if (back) std::swap(d, e);
+            const double minx = std::min({a.x, d.x, e.x}), maxx = std::max({a.x, d.x, e.x});
+            const double miny = std::min({a.y, d.y, e.y}), maxy = std::max({a.y, d.y, e.y});
+            if (!std::isfinite(minx) || !std::isfinite(maxx) || !std::isfinite(miny) || !std::isfinite(maxy) ||
+                minx < 0 || miny < 0 || maxx > size || maxy > size) { clipped = true; continue; }
+            const int x0 = std::max(0, int(std::floor(minx))), x1 = std::min(int(size) - 1, int(std::floor(maxx)));
+            const int y0 = std::max(0, int(std::floor(miny))), y1 = std::min(int(size) - 1, int(std::floor(maxy)));
+            const bool t0 = top_left(d, e), t1 = top_left(e, a), t2 = top_left(a, d);
+            for (int y = y0; y <= y1; ++y)
+                for (int x = x0; x <= x1; ++x) {
+                    const double u = edge(d.x, d.y, e.x, e.y, x + .5, y + .5);
+                    const double v = edge(e.x, e.y, a.x, a.y, x + .5, y + .5);
+                    const double w = edge(a.x, a.y, d.x, d.y, x + .5, y + .5);
+                    if (u < 0 || v < 0 || w < 0) continue;
+                    if (ORACLE_PLANT == 7 || ((u > 0 || t0) && (v > 0 || t1) && (w > 0 || t2)))
+                        ++count[size_t(y) * size + x];
+                }
+        }
+        if (clipped) return kInf;
+        uint64_t samples = 0, pixels = 0;
+        for (uint32_t k : count)
+            if (k) {
+                samples += k;
+                ++pixels;
+            }
+        total += pixels ? double(samples) / pixels : 0;
+    }
+    return total / cams.size();
+}
+
 // src/mesh.cpp:77-86
 Bounds bounds_of(const Mesh& m) {
     if (!m.vertices) return {{0, 0, 0}, 0};
@@ -671,6 +739,20 @@ int oracle_raster(const float* pos, uint64_t nv, const uint32_t* idx, uint64_t n
         *clipped = r.clipped;
         if (r.words.size() > capacity) return 1;
         std::copy(r.words.begin(), r.words.end(), words);
+        return 0;
+    } catch (const LengthError&) {
+        return 2;
+    } catch (...) {
+        return 1;
+    }
+}
+
+int oracle_overlap(const float* pos, uint64_t nv, const uint32_t* idx, uint64_t ni, const uint16_t* mat, uint64_t nm,
+                   const uint8_t* ds, uint64_t nds, const float* center, double radius, double screen, uint16_t ortho,
+                   uint16_t persp, uint32_t seed, double* out) {
+    try {
+        *out = overlap(mesh_of(pos, nv, idx, ni, mat, nm, ds, nds), {{center[0], center[1], center[2]}, radius}, screen,
+                       ortho, persp, seed);
         return 0;
     } catch (const LengthError&) {
         return 2;

@@ -164,6 +164,20 @@ class Oracle:
         assert st == 0, st
         return out, int(out_size.value), bool(clipped.value)
 
+    def overlap(self, mesh: Mesh, center, radius, screen, ortho, persp, seed) -> float:
+        """Oracle restatement of overlap_cuda (src/neural/evaluate.cu:2665-2685)."""
+        assert self.kind == "oracle"
+        out = C.c_double()
+        cc = (C.c_float * 3)(*[float(x) for x in center])
+        f = self.fn("overlap")
+        f.argtypes = [C.POINTER(C.c_float), C.c_uint64, C.POINTER(C.c_uint32), C.c_uint64, C.POINTER(C.c_uint16),
+                      C.c_uint64, C.POINTER(C.c_uint8), C.c_uint64, C.POINTER(C.c_float), C.c_double, C.c_double,
+                      C.c_uint16, C.c_uint16, C.c_uint32, C.POINTER(C.c_double)]
+        st = f(*_mesh_args(mesh), cc, radius, screen, ortho, persp, seed & 0xFFFFFFFF, C.byref(out))
+        if st != 0:
+            raise ValueError(f"overlap status {st}")
+        return float(out.value)
+
     def metric_acos(self, x: float) -> float:
         assert self.kind == "oracle"
         f = self.lib.oracle_metric_acos
