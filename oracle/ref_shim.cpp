@@ -10,8 +10,11 @@
 #include "blitz/evaluate.hpp"
 #include "blitz/remesher.hpp"
 #include "coverage.hpp"
+#include "neural/action.hpp"
 #include <cstring>
+#include <memory>
 #include <stdexcept>
+#include <vector>
 
 using namespace blitz;
 
@@ -253,6 +256,84 @@ int ref_generate(const float* pos, uint64_t nv, const uint32_t* idx, uint64_t ni
         *audits = n;
         *status = uint32_t(r.status);
         return int(r.lods.size());
+    } catch (...) {
+        return -1;
+    }
+}
+
+// ---- ActionState (src/neural/action.cpp): collapse legality, independent
+// batches and trial edits. The state owns copies of the streams it borrows.
+struct RefActions {
+    std::vector<Vec3> positions;
+    std::vector<Vec2> uv;
+    std::vector<uint32_t> indices;
+    std::vector<uint16_t> materials;
+    std::vector<uint8_t> double_sided;
+    std::unique_ptr<neural::ActionState> state;
+    std::vector<neural::ActionRecord> rows;
+};
+void* ref_actions_create(const float* pos, uint64_t nv, const uint32_t* idx, uint64_t ni, const uint16_t* mat,
+                         uint64_t nm, const uint8_t* ds, uint64_t nds, const float* uv) {
+    try {
+        auto r = std::make_unique<RefActions>();
+        r->positions.assign(reinterpret_cast<const Vec3*>(pos), reinterpret_cast<const Vec3*>(pos) + nv);
+        if (uv) r->uv.assign(reinterpret_cast<const Vec2*>(uv), reinterpret_cast<const Vec2*>(uv) + nv);
+        r->indices.assign(idx, idx + ni);
+        if (nm) r->materials.assign(mat, mat + nm);
+        if (nds) r->double_sided.assign(ds, ds + nds);
+        MeshView v;
+        v.positions = Stream<Vec3>(r->positions.data(), nv);
+        if (uv) v.uv = Stream<Vec2>(r->uv.data(), nv);
+        v.indices = {r->indices.data(), r->indices.size()};
+        v.materials = {r->materials.data(), r->materials.size()};
+        v.double_sided = {r->double_sided.data(), r->double_sided.size()};
+        r->state = std::make_unique<neural::ActionState>(v);
+        r->rows = r->state->actions({});
+        return r.release();
+    } catch (...) {
+        return nullptr;
+    }
+}
+void ref_actions_destroy(void* h) { delete static_cast<RefActions*>(h); }
+// Legal directed actions in the state's enumeration order (edges ascending,
+// a->b then b->a): from/to pairs.
+int64_t ref_actions_legal(void* h, uint32_t* pairs, uint64_t capacity) {
+    auto* r = static_cast<RefActions*>(h);
+    if (r->rows.size() > capacity) return -1;
+    for (size_t i = 0; i < r->rows.size(); ++i) {
+        pairs[2 * i] = r->rows[i].action.from;
+        pairs[2 * i + 1] = r->rows[i].action.to;
+    }
+    return int64_t(r->rows.size());
+}
+// ActionState::independent over the legal rows in `order` (indices into the
+// legal list); writes the selected rows' indices.
+int64_t ref_actions_independent(void* h, const uint32_t* order, uint64_t n, uint32_t maximum,
+                                uint64_t maximum_removed, uint32_t* selected) {
+    try {
+        auto* r = static_cast<RefActions*>(h);
+        auto picked = r->state->independent(r->rows, {order, size_t(n)}, maximum, size_t(maximum_removed));
+        for (size_t i = 0; i < picked.size(); ++i) {
+            uint32_t k = 0;
+            while (!(r->rows[k].action == picked[i])) ++k;
+            selected[i] = k;
+        }
+        return int64_t(picked.size());
+    } catch (...) {
+        return -1;
+    }
+}
+// ActionState::trial of the legal rows `rows` (indices into the legal list):
+// the candidate's index buffer.
+int64_t ref_actions_trial(void* h, const uint32_t* rows, uint64_t n, uint32_t* indices, uint64_t capacity) {
+    try {
+        auto* r = static_cast<RefActions*>(h);
+        std::vector<neural::Action> batch;
+        for (uint64_t i = 0; i < n; ++i) batch.push_back(r->rows[rows[i]].action);
+        const Lod lod = r->state->trial(batch);
+        if (lod.data.indices.size() > capacity) return -1;
+        std::memcpy(indices, lod.data.indices.data(), lod.data.indices.size() * 4);
+        return int64_t(lod.data.indices.size());
     } catch (...) {
         return -1;
     }

@@ -1,0 +1,109 @@
+"""Seeded meshes for the collapse-action tests: closed and open manifolds,
+welded wedges (seams), material junctions, UV charts that can flip,
+non-manifold edges, bow-tie vertices, degenerate faces and signed zeros."""
+from __future__ import annotations
+
+import numpy as np
+
+from blitzremesher_amd import Mesh, meshes
+
+
+def seam_grid(n: int = 8, seed: int = 3):
+    """Grid whose middle column of vertices is duplicated (a UV seam): faces
+    left of it use the originals, faces right of it the copies; per-vertex UVs
+    jump across the seam."""
+    g = meshes.jitter(meshes.grid(n), 0.002, seed, 1.0)
+    pos = g.positions.copy()
+    cols = n
+    mid = cols // 2
+    dup_of = {}
+    extra = []
+    for r in range(n):
+        v = r * cols + mid
+        dup_of[v] = len(pos) + len(extra)
+        extra.append(pos[v])
+    pos = np.concatenate([pos, np.array(extra, np.float32)])
+    idx = g.indices.copy()
+    centroid_x = g.positions[idx].mean(1)[:, 0]
+    right = centroid_x > g.positions[mid, 0]
+    for f in np.nonzero(right)[0]:
+        for j in range(3):
+            if int(idx[f, j]) in dup_of:
+                idx[f, j] = dup_of[int(idx[f, j])]
+    uv = pos[:, :2].copy()
+    uv[len(g.positions):, 0] += 0.25
+    return Mesh(pos, idx), uv
+
+
+def junction_grid(n: int = 8):
+    g = meshes.grid(n)
+    c = g.positions[g.indices].mean(1)
+    mats = (c[:, 0] > c[:, 0].mean()).astype(np.uint16) + 2 * (c[:, 1] > c[:, 1].mean()).astype(np.uint16)
+    return Mesh(g.positions, g.indices, mats)
+
+
+def folded_heightfield(n: int = 24, seed: int = 5):
+    # Strong relief: many collapses would fold a neighbouring face.
+    hf = meshes.heightfield(n, 0.6, seed)
+    return meshes.jitter(hf, 0.01, seed, 0.5)
+
+
+def nonmanifold_grid(n: int = 6):
+    g = meshes.grid(n)
+    pos = np.concatenate([g.positions, np.array([[0.5, 0.5, 0.7]], np.float32)])
+    a, b = int(g.indices[7, 0]), int(g.indices[7, 1])
+    idx = np.concatenate([g.indices, np.array([[a, b, len(pos) - 1]], np.uint32)])  # third face on edge ab
+    return Mesh(pos, idx)
+
+
+def bowtie(n: int = 5):
+    g = meshes.grid(n)
+    shift = g.positions.copy()
+    shift[:, 0] += 1.0
+    shift[:, 1] += 1.0
+    pos = np.concatenate([g.positions, shift])
+    idx = np.concatenate([g.indices, g.indices + len(g.positions)])
+    # Weld the second grid's first corner onto the first grid's last corner:
+    # that vertex joins two fans touching at one point.
+    pos[len(g.positions)] = g.positions[len(g.positions) - 1]
+    return Mesh(pos, idx)
+
+
+def degenerate_grid(n: int = 6):
+    g = meshes.grid(n)
+    pos = np.concatenate([g.positions, g.positions[[10]]])  # a copy of vertex 10
+    idx = np.concatenate([g.indices, np.array([[10, len(pos) - 1, 11]], np.uint32)])
+    return Mesh(pos, idx)
+
+
+def signed_zero_grid(n: int = 6):
+    g = meshes.grid(n)
+    pos = g.positions.copy()
+    pos[:, 2] = 0.0
+    extra = pos[[3]].copy()
+    extra[0, 2] = -0.0
+    pos = np.concatenate([pos, extra])
+    idx = g.indices.copy()
+    idx[idx == 3] = len(pos) - 1  # every face now uses the -0 copy of vertex 3 ...
+    idx[0, np.nonzero(idx[0] == len(pos) - 1)[0]] = 3  # ... except one, which keeps +0
+    return Mesh(pos, idx)
+
+
+def action_cases() -> dict:
+    """name -> (mesh, uv or None)"""
+    seam, seam_uv = seam_grid()
+    return {
+        "grid": (meshes.grid(7), None),
+        "icosphere": (meshes.icosphere(3, 0.02, 1), None),
+        "torus": (meshes.torus(1.0, 0.3, 24, 10, 0.01, 3), None),
+        "heightfield": (meshes.heightfield(20, 0.1, 2), None),
+        "folded": (folded_heightfield(), None),
+        "tiles": (meshes.four_tiles(), None),
+        "seam": (seam, None),
+        "seam_uv": (seam, seam_uv),
+        "junction": (junction_grid(), None),
+        "nonmanifold": (nonmanifold_grid(), None),
+        "bowtie": (bowtie(), None),
+        "degenerate": (degenerate_grid(), None),
+        "signed_zero": (signed_zero_grid(), None),
+    }

@@ -273,3 +273,65 @@ def mask_to_words(mask: np.ndarray) -> np.ndarray:
     padded = np.zeros(n * 64, np.uint8)
     padded[: flat.size] = flat
     return np.packbits(padded, bitorder="little").view(np.uint64)
+
+
+class Actions:
+    """Collapse-action state of the oracle (oracle/action_oracle.cpp) or of the
+    reference build (src/neural/action.cpp ActionState): legal directed
+    actions in enumeration order, greedy independent batches and trial edits.
+    """
+
+    def __init__(self, kind: str, mesh: Mesh, uv: np.ndarray | None = None):
+        self.oracle = Oracle(kind)
+        self.mesh = mesh
+        self.uv = None if uv is None else np.ascontiguousarray(uv, np.float32).reshape(-1, 2)
+        f = self.oracle.fn("actions_create")
+        f.restype = C.c_void_p
+        f.argtypes = [C.POINTER(C.c_float), C.c_uint64, C.POINTER(C.c_uint32), C.c_uint64, C.POINTER(C.c_uint16),
+                      C.c_uint64, C.POINTER(C.c_uint8), C.c_uint64, C.POINTER(C.c_float)]
+        self.handle = f(*_mesh_args(mesh), _ptr(self.uv, C.c_float))
+        if not self.handle:
+            raise ValueError("action state rejected the mesh")
+        n = 6 * mesh.triangles + 2
+        pairs = np.zeros(2 * n, np.uint32)
+        g = self.oracle.fn("actions_legal")
+        g.restype = C.c_int64
+        g.argtypes = [C.c_void_p, C.POINTER(C.c_uint32), C.c_uint64]
+        count = g(self.handle, pairs.ctypes.data_as(C.POINTER(C.c_uint32)), n)
+        assert count >= 0
+        self.legal = pairs[: 2 * count].reshape(-1, 2).copy()
+
+    def close(self):
+        if self.handle:
+            d = self.oracle.fn("actions_destroy")
+            d.argtypes = [C.c_void_p]
+            d(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        self.close()
+
+    def independent(self, order: np.ndarray, maximum: int, maximum_removed: int) -> np.ndarray:
+        order = np.ascontiguousarray(order, np.uint32)
+        out = np.zeros(64, np.uint32)
+        f = self.oracle.fn("actions_independent")
+        f.restype = C.c_int64
+        f.argtypes = [C.c_void_p, C.POINTER(C.c_uint32), C.c_uint64, C.c_uint32, C.c_uint64,
+                      C.POINTER(C.c_uint32)]
+        n = f(self.handle, order.ctypes.data_as(C.POINTER(C.c_uint32)), order.size, maximum, maximum_removed,
+              out.ctypes.data_as(C.POINTER(C.c_uint32)))
+        if n < 0:
+            raise ValueError("independent rejected its input")
+        return out[:n].copy()
+
+    def trial(self, rows: np.ndarray) -> np.ndarray:
+        rows = np.ascontiguousarray(rows, np.uint32)
+        out = np.zeros(self.mesh.indices.size, np.uint32)
+        f = self.oracle.fn("actions_trial")
+        f.restype = C.c_int64
+        f.argtypes = [C.c_void_p, C.POINTER(C.c_uint32), C.c_uint64, C.POINTER(C.c_uint32), C.c_uint64]
+        n = f(self.handle, rows.ctypes.data_as(C.POINTER(C.c_uint32)), rows.size,
+              out.ctypes.data_as(C.POINTER(C.c_uint32)), out.size)
+        if n < 0:
+            raise ValueError("trial rejected its batch")
+        return out[:n].reshape(-1, 3).copy()
