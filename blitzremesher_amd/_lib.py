@@ -152,6 +152,18 @@ PROTOTYPES = {
                                           C.POINTER(C.c_uint8), *_ERR]),
     "blitz_hip_overlap": (C.c_int, [_P, _P, C.POINTER(Bounds), C.c_double, C.c_uint16, C.c_uint16, C.c_uint32,
                                     C.POINTER(C.c_double), *_ERR]),
+    "blitz_hip_actions_create": (C.c_int, [C.c_int32, C.POINTER(BlitzMesh), C.POINTER(C.c_void_p), *_ERR]),
+    "blitz_hip_actions_destroy": (None, [_P]),
+    "blitz_hip_actions_info": (C.c_int, [_P, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_double)]),
+    "blitz_hip_actions_legal": (C.c_int, [_P, C.POINTER(C.c_uint32), C.c_size_t, *_ERR]),
+    "blitz_hip_actions_independent": (C.c_int, [_P, C.POINTER(C.c_uint32), C.c_size_t, C.c_uint32, C.c_uint64,
+                                                C.POINTER(C.c_uint32), C.POINTER(C.c_size_t), *_ERR]),
+    "blitz_hip_actions_trial": (C.c_int, [_P, C.POINTER(C.c_uint32), C.c_size_t, C.POINTER(C.c_uint32), C.c_size_t,
+                                          C.POINTER(C.c_size_t), *_ERR]),
+    "blitz_hip_actions_patches": (C.c_int, [_P, C.POINTER(C.c_uint32), C.c_size_t, C.POINTER(C.c_uint32),
+                                            C.POINTER(C.c_uint32), C.c_size_t, C.POINTER(C.c_uint32),
+                                            C.POINTER(C.c_uint32), C.c_size_t, C.POINTER(C.c_size_t),
+                                            C.POINTER(C.c_size_t), *_ERR]),
     "blitz_hip_coverage_distance": (C.c_int, [_P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.c_uint32,
                                               C.c_uint8, C.POINTER(C.c_double), *_ERR]),
     "blitz_hip_audit_last_timing": (C.c_int, [_P, C.POINTER(AuditTiming)]),

@@ -1,0 +1,97 @@
+"""Python view of the device collapse-action state (include/blitz_hip.h,
+blitz_hip_actions_*): the GPU counterpart of blitz::neural::ActionState
+(src/neural/action.cpp) — legal directed half-edge collapses in enumeration
+order, greedy independent batches (``independent``), trial index buffers
+(``trial``) and per-action patches for the bound-reference audit
+(``patches``). All work runs in libblitz_hip.so on a gfx950 device.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from .audit import Mesh, Patches
+
+
+def _u32p(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_uint32))
+
+
+class ActionSet:
+    def __init__(self, mesh: Mesh, uv: np.ndarray | None = None, device: int = 0):
+        self.lib = _lib.load()
+        self.mesh = mesh
+        self.uv = None if uv is None else np.ascontiguousarray(uv, np.float32).reshape(-1, 2)
+        desc = mesh.descriptor()
+        if self.uv is not None:
+            desc.uv = _lib.BlitzStream(self.uv.ctypes.data, self.uv.shape[0], 8)
+        handle = C.c_void_p()
+        err = _lib.error_buffer()
+        _lib.check(self.lib.blitz_hip_actions_create(device, C.byref(desc), C.byref(handle), err, len(err)), err)
+        self.handle = handle
+        edges, legal, ms = C.c_uint32(), C.c_uint32(), C.c_double()
+        _lib.check(self.lib.blitz_hip_actions_info(self.handle, C.byref(edges), C.byref(legal), C.byref(ms)))
+        self.edges, self.build_ms = int(edges.value), float(ms.value)
+        pairs = np.zeros(2 * max(1, legal.value), np.uint32)
+        _lib.check(self.lib.blitz_hip_actions_legal(self.handle, _u32p(pairs), int(legal.value), err, len(err)), err)
+        self.legal = pairs[: 2 * legal.value].reshape(-1, 2).copy()
+
+    def close(self) -> None:
+        if self.handle:
+            self.lib.blitz_hip_actions_destroy(self.handle)
+            self.handle = None
+
+    def __enter__(self) -> "ActionSet":
+        return self
+
+    def __exit__(self, *exc) -> None:
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def independent(self, order: np.ndarray, maximum: int, maximum_removed: int) -> np.ndarray:
+        """ActionState::independent over legal-action indices in rank order."""
+        order = np.ascontiguousarray(order, np.uint32)
+        out = np.zeros(64, np.uint32)
+        count = C.c_size_t()
+        err = _lib.error_buffer()
+        _lib.check(self.lib.blitz_hip_actions_independent(self.handle, _u32p(order), order.size, maximum,
+                                                          maximum_removed, _u32p(out), C.byref(count), err, len(err)),
+                   err)
+        return out[: count.value].copy()
+
+    def trial(self, rows: np.ndarray) -> np.ndarray:
+        """Index buffer (m, 3) with the legal actions `rows` applied."""
+        rows = np.ascontiguousarray(rows, np.uint32)
+        out = np.zeros(self.mesh.indices.size, np.uint32)
+        count = C.c_size_t()
+        err = _lib.error_buffer()
+        _lib.check(self.lib.blitz_hip_actions_trial(self.handle, _u32p(rows), rows.size, _u32p(out), out.size,
+                                                    C.byref(count), err, len(err)), err)
+        return out[: count.value].reshape(-1, 3).copy()
+
+    def patches(self, rows: np.ndarray) -> Patches:
+        """Each legal action as a patch of the source mesh."""
+        rows = np.ascontiguousarray(rows, np.uint32)
+        nr, na = C.c_size_t(), C.c_size_t()
+        err = _lib.error_buffer()
+        null = C.POINTER(C.c_uint32)()
+        _lib.check(self.lib.blitz_hip_actions_patches(self.handle, _u32p(rows), rows.size, null, null, 0, null, null,
+                                                      0, C.byref(nr), C.byref(na), err, len(err)), err)
+        ro = np.zeros(rows.size + 1, np.uint32)
+        ao = np.zeros(rows.size + 1, np.uint32)
+        rv = np.zeros(max(1, nr.value), np.uint32)
+        av = np.zeros(max(1, 3 * na.value), np.uint32)
+        _lib.check(self.lib.blitz_hip_actions_patches(self.handle, _u32p(rows), rows.size, _u32p(ro), _u32p(rv),
+                                                      rv.size, _u32p(ao), _u32p(av), av.size // 3, C.byref(nr),
+                                                      C.byref(na), err, len(err)), err)
+        return Patches(ro, rv[: nr.value].copy(), ao, av[: 3 * na.value].reshape(-1, 3).copy())
+
+
+__all__ = ["ActionSet"]

@@ -1,0 +1,853 @@
+// Collapse actions on gfx950: device topology, legality, greedy independent
+// batches, trial edits and per-action patches, with the semantics of the
+// reference's CPU ActionState (src/neural/action.cpp).
+//
+// Topology (src/neural/action.cpp:27-145). Vertices with bit-equal positions
+// (-0 read as +0) weld to the smallest index: three stable radix passes sort
+// (x, y, z, index) and an inclusive max-scan carries each run's head. Face
+// corners give the per-vertex face lists (a stable sort by geometric vertex
+// keeps faces ascending) and the undirected edge keys; a run-length pass
+// counts each edge's corners (1: boundary, >2: non-manifold). Neighbour lists
+// come from sorting both directions of every edge. A vertex whose link is not
+// one connected fan is marked invalid by a per-vertex relaxation over its
+// neighbour slots.
+//
+// Legality (src/neural/action.cpp:146-256), one thread per directed edge:
+// the wedge mapping, boundary rule, link condition (common neighbours are
+// exactly the opposite vertices), material junctions, duplicate and
+// degenerate results, the 0.05 fold bound and UV orientation, with the same
+// float/double arithmetic. Legal actions are compacted in enumeration order.
+//
+// Selection (src/neural/action.cpp:261-304) is inherently sequential (each
+// pick blocks its footprint for later ones); one wavefront walks the rank
+// order, testing and claiming a footprint (both endpoints and their
+// neighbours) 64 neighbour slots at a time.
+#include "actions.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <hipcub/device/device_radix_sort.hpp>
+#include <hipcub/device/device_run_length_encode.hpp>
+#include <hipcub/device/device_scan.hpp>
+#include <stdexcept>
+
+namespace blitz_hip {
+namespace {
+
+constexpr uint32_t kNone = 0xFFFFFFFFu;
+constexpr unsigned long long kNoEdge = ~0ull;
+
+struct Topo {
+    const float4* positions;
+    const float2* uv;  // or null
+    const uint32_t* indices;
+    const uint16_t* materials;  // or null
+    const uint32_t* geometry;
+    const uint8_t* invalid;
+    const uint8_t* boundary;
+    const uint32_t* offsets;
+    const uint32_t* neighbors;
+    const uint32_t* face_offsets;
+    const uint32_t* faces;
+    uint32_t vertices, triangles;
+};
+
+__global__ void weld_keys_kernel(const float4* __restrict__ p, uint32_t n, uint32_t* kx, uint32_t* ky, uint32_t* kz,
+                                 uint32_t* ids) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 v = p[i];
+    kx[i] = __float_as_uint(v.x == 0 ? 0.f : v.x);
+    ky[i] = __float_as_uint(v.y == 0 ? 0.f : v.y);
+    kz[i] = __float_as_uint(v.z == 0 ? 0.f : v.z);
+    ids[i] = i;
+}
+
+__global__ void gather_kernel(const uint32_t* __restrict__ key, const uint32_t* __restrict__ order, uint32_t n,
+                              uint32_t* out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = key[order[i]];
+}
+
+__global__ void weld_heads_kernel(const uint32_t* __restrict__ order, const uint32_t* kx, const uint32_t* ky,
+                                  const uint32_t* kz, uint32_t n, uint32_t* head) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    bool same = false;
+    if (k) {
+        const uint32_t a = order[k], b = order[k - 1];
+        same = kx[a] == kx[b] && ky[a] == ky[b] && kz[a] == kz[b];
+    }
+    head[k] = same ? 0 : k;
+}
+
+__global__ void weld_assign_kernel(const uint32_t* __restrict__ order, const uint32_t* __restrict__ head, uint32_t n,
+                                   uint32_t* geometry) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) geometry[order[k]] = order[head[k]];
+}
+
+// src/neural/action.cpp:58-76: per corner, its geometric vertex and face,
+// the face-count histogram, degenerate corners and undirected edge keys.
+__global__ void corners_kernel(const uint32_t* __restrict__ indices, const uint32_t* __restrict__ geometry,
+                               uint32_t corners, uint32_t* vertex, uint32_t* face, uint32_t* face_count,
+                               uint8_t* invalid, unsigned long long* keys) {
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= corners) return;
+    const uint32_t f = c / 3, j = c % 3;
+    const uint32_t a = geometry[indices[c]], b = geometry[indices[3 * f + (j + 1) % 3]];
+    vertex[c] = a;
+    face[c] = f;
+    atomicAdd(&face_count[a], 1u);
+    if (a == b) {
+        for (uint32_t t = 0; t < 3; ++t) invalid[geometry[indices[3 * f + t]]] = 1;
+        keys[c] = kNoEdge;
+        return;
+    }
+    keys[c] = (static_cast<unsigned long long>(min(a, b)) << 32) | max(a, b);
+}
+
+// src/neural/action.cpp:77-92
+__global__ void edge_flags_kernel(const unsigned long long* __restrict__ keys, const uint32_t* __restrict__ counts,
+                                  uint32_t edges, uint8_t* boundary, uint8_t* invalid, uint32_t* degree,
+                                  unsigned long long* directed) {
+    const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= edges) return;
+    const uint32_t a = uint32_t(keys[e] >> 32), b = uint32_t(keys[e]);
+    if (counts[e] == 1) boundary[a] = boundary[b] = 1;
+    if (counts[e] > 2) invalid[a] = invalid[b] = 1;
+    atomicAdd(&degree[a], 1u);
+    atomicAdd(&degree[b], 1u);
+    directed[2 * e] = (static_cast<unsigned long long>(a) << 32) | b;
+    directed[2 * e + 1] = (static_cast<unsigned long long>(b) << 32) | a;
+}
+
+__global__ void low_words_kernel(const unsigned long long* __restrict__ keys, uint32_t n, uint32_t* out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = uint32_t(keys[i]);
+}
+
+__device__ uint32_t find_neighbor(const Topo& t, uint32_t u, uint32_t v) {
+    uint32_t lo = t.offsets[u], hi = t.offsets[u + 1];
+    const uint32_t end = hi;
+    while (lo < hi) {
+        const uint32_t mid = lo + (hi - lo) / 2;
+        if (t.neighbors[mid] < v)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return lo < end && t.neighbors[lo] == v ? lo : kNone;
+}
+
+// The two vertices of face f other than u (the first two met, as
+// src/neural/action.cpp:120-126); false when there are not two.
+__device__ bool link_of(const Topo& t, uint32_t f, uint32_t u, uint32_t other[2]) {
+    uint32_t count = 0;
+    for (uint32_t j = 0; j < 3; ++j) {
+        const uint32_t v = t.geometry[t.indices[3 * f + j]];
+        if (v != u && count < 2) other[count++] = v;
+    }
+    return count == 2;
+}
+
+// src/neural/action.cpp:107-144: the link of u must be one connected fan
+// over all of u's neighbours.
+__global__ void fans_kernel(Topo t, uint8_t* invalid, uint8_t* visited) {
+    const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= t.vertices) return;
+    const uint32_t f0 = t.face_offsets[u], f1 = t.face_offsets[u + 1];
+    if (f0 == f1 || invalid[u]) return;
+    uint32_t other[2];
+    for (uint32_t k = f0; k < f1; ++k)
+        if (!link_of(t, t.faces[k], u, other)) {
+            invalid[u] = 1;
+            return;
+        }
+    const uint32_t n0 = t.offsets[u], n1 = t.offsets[u + 1];
+    for (uint32_t k = n0; k < n1; ++k) visited[k] = 0;
+    link_of(t, t.faces[f0], u, other);
+    const uint32_t seed = find_neighbor(t, u, other[0]);
+    if (seed == kNone) {
+        invalid[u] = 1;
+        return;
+    }
+    visited[seed] = 1;
+    for (bool changed = true; changed;) {
+        changed = false;
+        for (uint32_t k = f0; k < f1; ++k) {
+            link_of(t, t.faces[k], u, other);
+            const uint32_t a = find_neighbor(t, u, other[0]), b = find_neighbor(t, u, other[1]);
+            if (a == kNone || b == kNone) continue;
+            if (visited[a] != visited[b]) {
+                visited[a] = visited[b] = 1;
+                changed = true;
+            }
+        }
+    }
+    for (uint32_t k = n0; k < n1; ++k)
+        if (!visited[k]) {
+            invalid[u] = 1;
+            return;
+        }
+}
+
+struct F3 {
+    float x, y, z;
+};
+// include/blitz/mesh.hpp:22-26
+__device__ __forceinline__ F3 sub3(float4 a, float4 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ F3 cross3(F3 a, F3 b) {
+    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+__device__ __forceinline__ double dot3(F3 a, F3 b) {
+    return double(a.x) * b.x + double(a.y) * b.y + double(a.z) * b.z;
+}
+__device__ __forceinline__ double len3(F3 a) { return sqrt(dot3(a, a)); }
+
+__device__ __forceinline__ uint16_t material_of(const Topo& t, uint32_t f) {
+    return t.materials ? t.materials[f] : uint16_t(0);
+}
+
+__device__ __forceinline__ void sort3(uint32_t v[3]) {
+    uint32_t a = v[0], b = v[1], c = v[2];
+    if (a > b) { const uint32_t s = a; a = b; b = s; }
+    if (b > c) { const uint32_t s = b; b = c; c = s; }
+    if (a > b) { const uint32_t s = a; a = b; b = s; }
+    v[0] = a;
+    v[1] = b;
+    v[2] = c;
+}
+
+// src/neural/action.cpp:13-15
+__device__ __forceinline__ double uv_area(const float2* uv, const uint32_t ids[3]) {
+    const float2 a = uv[ids[0]], b = uv[ids[1]], c = uv[ids[2]];
+    return (double(b.x) - a.x) * (double(c.y) - a.y) - (double(b.y) - a.y) * (double(c.x) - a.x);
+}
+
+__device__ __forceinline__ uint32_t map_id(const LegalAction& a, uint32_t id) {
+    for (uint32_t w = 0; w < a.wedges; ++w)
+        if (a.wedge_from[w] == id) return a.wedge_to[w];
+    return id;
+}
+
+// Sorted geometric triple of face f after the mapping of `a` (identity when
+// `mapped` is false).
+__device__ __forceinline__ void mapped_triple(const Topo& t, const LegalAction& a, uint32_t f, bool mapped,
+                                              uint32_t out[3]) {
+    for (uint32_t j = 0; j < 3; ++j) {
+        const uint32_t id = t.indices[3 * f + j];
+        out[j] = t.geometry[mapped ? map_id(a, id) : id];
+    }
+    sort3(out);
+}
+
+// src/neural/action.cpp:146-256
+__device__ bool legal_action(const Topo& t, uint32_t from, uint32_t to, LegalAction& a) {
+    a.from = from;
+    a.to = to;
+    a.wedges = 0;
+    a.removed = 0;
+    if (from == to || t.invalid[from] || t.invalid[to]) return false;
+    if (find_neighbor(t, from, to) == kNone) return false;
+    const uint32_t f0 = t.face_offsets[from], f1 = t.face_offsets[from + 1];
+    const uint32_t g0 = t.face_offsets[to], g1 = t.face_offsets[to + 1];
+    if (f0 == f1) return false;
+    uint32_t opposite[2], pair_u[2], pair_v[2], wedge[2];
+    uint32_t edge_faces = 0, wedges = 0;
+    bool junction = false;
+    const uint16_t first_material = material_of(t, t.faces[f0]);
+    for (uint32_t k = f0; k < f1; ++k) {
+        const uint32_t f = t.faces[k];
+        junction |= material_of(t, f) != first_material;
+        uint32_t u = kNone, v = kNone, w = kNone;
+        for (uint32_t j = 0; j < 3; ++j) {
+            const uint32_t id = t.indices[3 * f + j], g = t.geometry[id];
+            if (g == from)
+                u = id;
+            else if (g == to)
+                v = id;
+            else
+                w = g;
+        }
+        if (u == kNone) return false;
+        // Distinct wedges of `from`; more than two can never match the at most
+        // two mapped pairs.
+        bool seen = false;
+        for (uint32_t i = 0; i < wedges; ++i) seen |= wedge[i] == u;
+        if (!seen) {
+            if (wedges == 2) return false;
+            wedge[wedges++] = u;
+        }
+        if (v != kNone) {
+            if (w == kNone || edge_faces == 2) return false;
+            opposite[edge_faces] = w;
+            pair_u[edge_faces] = u;
+            pair_v[edge_faces] = v;
+            ++edge_faces;
+        }
+    }
+    if (edge_faces < 1 || edge_faces >= t.triangles) return false;
+    // A boundary vertex may only move along its boundary.
+    if (t.boundary[from] && (!t.boundary[to] || edge_faces != 1)) return false;
+    // Common neighbours must be exactly the (distinct) opposite vertices.
+    uint32_t nopp = edge_faces;
+    if (nopp == 2) {
+        if (opposite[0] > opposite[1]) {
+            const uint32_t s = opposite[0];
+            opposite[0] = opposite[1];
+            opposite[1] = s;
+        }
+        if (opposite[0] == opposite[1]) nopp = 1;
+    }
+    if (nopp != edge_faces) return false;
+    {
+        uint32_t i = t.offsets[from], i1 = t.offsets[from + 1], j = t.offsets[to], j1 = t.offsets[to + 1];
+        uint32_t matched = 0;
+        while (i < i1 && j < j1) {
+            const uint32_t x = t.neighbors[i], y = t.neighbors[j];
+            if (x < y)
+                ++i;
+            else if (y < x)
+                ++j;
+            else {
+                if (matched >= nopp || opposite[matched] != x) return false;
+                ++matched;
+                ++i;
+                ++j;
+            }
+        }
+        if (matched != nopp) return false;
+    }
+    // Each wedge of `from` maps to exactly one destination wedge.
+    uint32_t npairs = edge_faces;
+    if (npairs == 2) {
+        const bool swap = pair_u[0] > pair_u[1] || (pair_u[0] == pair_u[1] && pair_v[0] > pair_v[1]);
+        if (swap) {
+            uint32_t s = pair_u[0];
+            pair_u[0] = pair_u[1];
+            pair_u[1] = s;
+            s = pair_v[0];
+            pair_v[0] = pair_v[1];
+            pair_v[1] = s;
+        }
+        if (pair_u[0] == pair_u[1] && pair_v[0] == pair_v[1]) npairs = 1;
+    }
+    if (wedges == 2 && wedge[0] > wedge[1]) {
+        const uint32_t s = wedge[0];
+        wedge[0] = wedge[1];
+        wedge[1] = s;
+    }
+    if (npairs != wedges) return false;
+    for (uint32_t i = 0; i < npairs; ++i)
+        if (pair_u[i] != wedge[i]) return false;
+    a.wedges = npairs;
+    for (uint32_t i = 0; i < npairs; ++i) {
+        a.wedge_from[i] = pair_u[i];
+        a.wedge_to[i] = pair_v[i];
+    }
+    a.removed = edge_faces;
+    // Material sets around both endpoints must agree at a junction.
+    if (junction) {
+        for (uint32_t k = f0; k < f1; ++k) {
+            const uint16_t m = material_of(t, t.faces[k]);
+            bool found = false;
+            for (uint32_t l = g0; l < g1 && !found; ++l) found = material_of(t, t.faces[l]) == m;
+            if (!found) return false;
+        }
+        for (uint32_t l = g0; l < g1; ++l) {
+            const uint16_t m = material_of(t, t.faces[l]);
+            bool found = false;
+            for (uint32_t k = f0; k < f1 && !found; ++k) found = material_of(t, t.faces[k]) == m;
+            if (!found) return false;
+        }
+    }
+    // The moved faces: not degenerate, not duplicating a face around `to` or
+    // each other, not folded, not flipped in UV.
+    for (uint32_t k = f0; k < f1; ++k) {
+        const uint32_t f = t.faces[k];
+        uint32_t old[3], next[3];
+        bool removed = false;
+        for (uint32_t j = 0; j < 3; ++j) {
+            old[j] = t.indices[3 * f + j];
+            next[j] = map_id(a, old[j]);
+            removed |= t.geometry[old[j]] == to;
+        }
+        if (removed) continue;
+        uint32_t ids[3] = {t.geometry[next[0]], t.geometry[next[1]], t.geometry[next[2]]};
+        sort3(ids);
+        if (ids[0] == ids[1] || ids[1] == ids[2]) return false;
+        for (uint32_t l = g0; l < g1; ++l) {
+            uint32_t other[3];
+            mapped_triple(t, a, t.faces[l], false, other);
+            if (other[0] == ids[0] && other[1] == ids[1] && other[2] == ids[2]) return false;
+        }
+        for (uint32_t l = f0; l < k; ++l) {
+            const uint32_t g = t.faces[l];
+            bool skip = false;
+            for (uint32_t j = 0; j < 3; ++j) skip |= t.geometry[t.indices[3 * g + j]] == to;
+            if (skip) continue;
+            uint32_t other[3];
+            mapped_triple(t, a, g, true, other);
+            if (other[0] == ids[0] && other[1] == ids[1] && other[2] == ids[2]) return false;
+        }
+        const float4 p0 = t.positions[old[0]], p1 = t.positions[old[1]], p2 = t.positions[old[2]];
+        const float4 q0 = t.positions[next[0]], q1 = t.positions[next[1]], q2 = t.positions[next[2]];
+        const F3 before = cross3(sub3(p1, p0), sub3(p2, p0));
+        const F3 after = cross3(sub3(q1, q0), sub3(q2, q0));
+        if (len3(after) <= 1e-15 || dot3(before, after) <= .05 * len3(before) * len3(after)) return false;
+        if (t.uv) {
+            const double x = uv_area(t.uv, old), y = uv_area(t.uv, next);
+            if (fabs(x) > 1e-20 && x * y <= 0) return false;
+        }
+    }
+    return true;
+}
+
+__global__ void legal_kernel(Topo t, const unsigned long long* __restrict__ edges, uint32_t count, LegalAction* out,
+                             uint32_t* flags) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= 2 * count) return;
+    const unsigned long long key = edges[i / 2];
+    const uint32_t a = uint32_t(key >> 32), b = uint32_t(key);
+    LegalAction act;
+    const bool ok = (i & 1) ? legal_action(t, b, a, act) : legal_action(t, a, b, act);
+    out[i] = act;
+    flags[i] = ok;
+}
+
+__global__ void compact_kernel(const LegalAction* __restrict__ in, const uint32_t* __restrict__ flags,
+                               const uint32_t* __restrict__ slots, uint32_t n, LegalAction* out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && flags[i]) out[slots[i]] = in[i];
+}
+
+__device__ __forceinline__ uint32_t load_used(const uint32_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void store_used(uint32_t* p) {
+    __hip_atomic_store(p, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// src/neural/action.cpp:261-304 on one wavefront: out[0] = picked count,
+// out[1] = 1 when a strict batch (a trial) is not independent, out[2..] the
+// picked rows.
+__global__ __launch_bounds__(64) void claim_kernel(Topo t, const LegalAction* __restrict__ legal,
+                                                   const uint32_t* __restrict__ rows, uint32_t n, uint32_t maximum,
+                                                   unsigned long long removable, uint32_t strict, uint32_t* used,
+                                                   uint32_t* out) {
+    const uint32_t lane = threadIdx.x;
+    uint32_t picked = 0, error = 0;
+    for (uint32_t k = 0; k < n; ++k) {
+        const LegalAction a = legal[rows[k]];
+        if (a.removed > removable) {
+            error |= strict;
+            continue;
+        }
+        bool hit = false;
+        for (uint32_t e = 0; e < 2; ++e) {
+            const uint32_t u = e ? a.to : a.from;
+            if (lane == 0) hit |= load_used(&used[u]) != 0;
+            for (uint32_t j = t.offsets[u] + lane; j < t.offsets[u + 1]; j += 64)
+                hit |= load_used(&used[t.neighbors[j]]) != 0;
+        }
+        if (__ballot(hit)) {
+            error |= strict;
+            continue;
+        }
+        for (uint32_t e = 0; e < 2; ++e) {
+            const uint32_t u = e ? a.to : a.from;
+            if (lane == 0) store_used(&used[u]);
+            for (uint32_t j = t.offsets[u] + lane; j < t.offsets[u + 1]; j += 64) store_used(&used[t.neighbors[j]]);
+        }
+        __threadfence();
+        if (lane == 0) out[2 + picked] = rows[k];
+        ++picked;
+        removable -= a.removed;
+        if (picked == maximum || !removable) break;
+    }
+    if (lane == 0) {
+        out[0] = picked;
+        out[1] = error;
+    }
+}
+
+__global__ void remap_kernel(const LegalAction* __restrict__ legal, const uint32_t* __restrict__ rows, uint32_t n,
+                             uint32_t* remap) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const LegalAction a = legal[rows[i]];
+    for (uint32_t w = 0; w < a.wedges; ++w) remap[a.wedge_from[w]] = a.wedge_to[w];
+}
+
+// src/neural/action.cpp:322-340: remapped faces that became degenerate drop out.
+__global__ void trial_keep_kernel(Topo t, const uint32_t* __restrict__ remap, uint32_t* next, uint32_t* keep) {
+    const uint32_t f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= t.triangles) return;
+    uint32_t ids[3];
+    bool changed = false;
+    for (uint32_t j = 0; j < 3; ++j) {
+        ids[j] = t.indices[3 * f + j];
+        if (remap[ids[j]] != kNone) {
+            ids[j] = remap[ids[j]];
+            changed = true;
+        }
+        next[3 * f + j] = ids[j];
+    }
+    const uint32_t a = t.geometry[ids[0]], b = t.geometry[ids[1]], c = t.geometry[ids[2]];
+    keep[f] = !(changed && (a == b || b == c || a == c));
+}
+
+__global__ void trial_copy_kernel(const uint32_t* __restrict__ next, const uint32_t* __restrict__ keep,
+                                  const uint32_t* __restrict__ slots, uint32_t faces, uint32_t* out) {
+    const uint32_t f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= faces || !keep[f]) return;
+    for (uint32_t j = 0; j < 3; ++j) out[3 * slots[f] + j] = next[3 * f + j];
+}
+
+// Per patch: faces around `from` removed, the remapped ones not holding `to`
+// added (a legal action never makes them degenerate).
+__global__ void patch_count_kernel(Topo t, const LegalAction* __restrict__ legal, const uint32_t* __restrict__ rows,
+                                   uint32_t n, uint32_t* removed_count, uint32_t* added_count) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const LegalAction a = legal[rows[i]];
+    const uint32_t len = t.face_offsets[a.from + 1] - t.face_offsets[a.from];
+    removed_count[i] = len;
+    added_count[i] = len - a.removed;
+}
+
+__global__ void patch_fill_kernel(Topo t, const LegalAction* __restrict__ legal, const uint32_t* __restrict__ rows,
+                                  uint32_t n, const uint32_t* __restrict__ removed_at,
+                                  const uint32_t* __restrict__ added_at, uint32_t* removed, uint32_t* added) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const LegalAction a = legal[rows[i]];
+    uint32_t r = removed_at[i], w = added_at[i];
+    for (uint32_t k = t.face_offsets[a.from]; k < t.face_offsets[a.from + 1]; ++k) {
+        const uint32_t f = t.faces[k];
+        removed[r++] = f;
+        bool holds_to = false;
+        for (uint32_t j = 0; j < 3; ++j) holds_to |= t.geometry[t.indices[3 * f + j]] == a.to;
+        if (holds_to) continue;
+        for (uint32_t j = 0; j < 3; ++j) added[3 * w + j] = map_id(a, t.indices[3 * f + j]);
+        ++w;
+    }
+}
+
+__global__ void fill_u32_kernel(uint32_t* p, size_t n, uint32_t v) {
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x)
+        p[i] = v;
+}
+
+uint32_t blocks_for(size_t n) { return uint32_t(std::max<size_t>(1, (n + 255) / 256)); }
+
+template <class T> T* grow(DeviceBuffer& b, size_t count) {
+    b.resize(std::max<size_t>(count, 1) * sizeof(T));
+    return b.as<T>();
+}
+
+struct Clock {
+    std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+    double ms() const {
+        return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
+};
+
+}  // namespace
+
+ActionSet::ActionSet(int device, const HostMesh& m, const float* uv) : device_(device) {
+    DeviceGuard guard(device_);
+    hip_check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
+    Clock clock;
+    const hipStream_t st = stream_;
+    vertices_ = uint32_t(m.positions.size() / 3);
+    faces_ = uint32_t(m.indices.size() / 3);
+    if (!vertices_ || !faces_) throw std::invalid_argument("action state needs a non-empty mesh");
+    if (uint64_t(faces_) * 6 >= uint64_t(INT32_MAX)) throw std::invalid_argument("mesh too large for 32-bit corners");
+    const uint32_t n = vertices_, corners = 3 * faces_;
+    // Every buffer of the build is sized here, before any launch, so nothing
+    // is reallocated under queued work.
+    DeviceBuffer weld, weld2, corner_buf, key_buf, temp, all_buf, flag_buf;
+    uint32_t* kx = grow<uint32_t>(weld, 6 * size_t(n));
+    uint32_t* ky = kx + n;
+    uint32_t* kz = ky + n;
+    uint32_t* ids = kz + n;
+    uint32_t* key_tmp = ids + n;
+    uint32_t* ord = key_tmp + n;
+    uint32_t* ord2 = grow<uint32_t>(weld2, 2 * size_t(n));
+    uint32_t* key_sorted = ord2 + n;
+    uint32_t* cv = grow<uint32_t>(corner_buf, 3 * size_t(corners) + 2 * (size_t(n) + 1));
+    uint32_t* cf = cv + corners;
+    uint32_t* cv_sorted = cf + corners;
+    uint32_t* face_count = cv_sorted + corners;  // n + 1
+    uint32_t* degree = face_count + (n + 1);     // n + 1
+    // Corner keys and their sorted copy, then both directions of every edge
+    // (at most 2 * corners) and their sorted copy.
+    auto* keys = grow<unsigned long long>(key_buf, 4 * size_t(corners));
+    auto* keys_sorted = keys + corners;
+    auto* directed = keys;
+    auto* directed_sorted = keys + 2 * size_t(corners);
+    auto* unique = grow<unsigned long long>(edge_keys_, corners);
+    uint32_t* counts = grow<uint32_t>(edge_counts_, corners + 1);
+    uint32_t* runs = counts + corners;
+    uint32_t* geometry = grow<uint32_t>(geometry_, n);
+    uint8_t* invalid = grow<uint8_t>(invalid_, n);
+    uint8_t* boundary = grow<uint8_t>(boundary_, n);
+    uint32_t* faces = grow<uint32_t>(face_lists_, corners);
+    uint32_t* face_offsets = grow<uint32_t>(face_offsets_, n + 1);
+    uint32_t* offsets = grow<uint32_t>(offsets_, n + 1);
+    size_t bytes = 16;
+    {
+        size_t need = 0;
+        auto take = [&](hipError_t e) {
+            hip_check(e, "workspace query");
+            bytes = std::max(bytes, need);
+        };
+        take(hipcub::DeviceRadixSort::SortPairs(nullptr, need, key_tmp, key_sorted, ids, ord, int(n), 0, 32, st));
+        take(hipcub::DeviceScan::InclusiveScan(nullptr, need, ord2, key_sorted, hipcub::Max(), int(n), st));
+        take(hipcub::DeviceRadixSort::SortPairs(nullptr, need, cv, cv_sorted, cf, faces, int(corners), 0, 32, st));
+        take(hipcub::DeviceScan::ExclusiveSum(nullptr, need, face_count, face_offsets, int(n + 1), st));
+        take(hipcub::DeviceRadixSort::SortKeys(nullptr, need, keys, keys_sorted, int(corners), 0, 64, st));
+        take(hipcub::DeviceRunLengthEncode::Encode(nullptr, need, keys_sorted, unique, counts, runs, int(corners), st));
+        take(hipcub::DeviceRadixSort::SortKeys(nullptr, need, keys, keys_sorted, int(2 * corners), 0, 64, st));
+        take(hipcub::DeviceScan::ExclusiveSum(nullptr, need, face_count, face_offsets, int(2 * corners), st));
+    }
+    void* work = grow<char>(temp, bytes);
+    auto run = [&](const char* what, auto call) {
+        size_t b = bytes;
+        hip_check(call(b), what);
+    };
+    {
+        std::vector<float4> pos(n);
+        for (uint32_t i = 0; i < n; ++i)
+            pos[i] = make_float4(m.positions[3 * i], m.positions[3 * i + 1], m.positions[3 * i + 2], 0.f);
+        hip_check(hipMemcpyAsync(grow<float4>(positions_, n), pos.data(), n * sizeof(float4), hipMemcpyHostToDevice, st),
+                  "upload positions");
+        hip_check(hipStreamSynchronize(st), "upload");
+    }
+    hip_check(hipMemcpyAsync(grow<uint32_t>(indices_, corners), m.indices.data(), size_t(corners) * 4,
+                             hipMemcpyHostToDevice, st),
+              "upload indices");
+    if (!m.materials.empty())
+        hip_check(hipMemcpyAsync(grow<uint16_t>(materials_, faces_), m.materials.data(), size_t(faces_) * 2,
+                                 hipMemcpyHostToDevice, st),
+                  "upload materials");
+    if (uv)
+        hip_check(hipMemcpyAsync(grow<float2>(uv_, n), uv, size_t(n) * 8, hipMemcpyHostToDevice, st), "upload uv");
+    // ---- weld: stable radix passes on the z, y, x keys carry the index order.
+    weld_keys_kernel<<<blocks_for(n), 256, 0, st>>>(positions_.as<float4>(), n, kx, ky, kz, ids);
+    auto sort_pass = [&](const uint32_t* key, const uint32_t* order_in, uint32_t* order_out) {
+        gather_kernel<<<blocks_for(n), 256, 0, st>>>(key, order_in, n, key_tmp);
+        run("weld sort", [&](size_t& b) {
+            return hipcub::DeviceRadixSort::SortPairs(work, b, key_tmp, key_sorted, order_in, order_out, int(n), 0, 32,
+                                                      st);
+        });
+    };
+    sort_pass(kz, ids, ord);
+    sort_pass(ky, ord, ord2);
+    sort_pass(kx, ord2, ord);
+    weld_heads_kernel<<<blocks_for(n), 256, 0, st>>>(ord, kx, ky, kz, n, ord2);
+    run("weld scan", [&](size_t& b) {
+        return hipcub::DeviceScan::InclusiveScan(work, b, ord2, key_sorted, hipcub::Max(), int(n), st);
+    });
+    weld_assign_kernel<<<blocks_for(n), 256, 0, st>>>(ord, key_sorted, n, geometry);
+    // ---- corners: face lists and edge keys.
+    hip_check(hipMemsetAsync(invalid, 0, n, st), "clear");
+    hip_check(hipMemsetAsync(boundary, 0, n, st), "clear");
+    hip_check(hipMemsetAsync(face_count, 0, size_t(n + 1) * 4 * 2, st), "clear");
+    corners_kernel<<<blocks_for(corners), 256, 0, st>>>(indices_.as<uint32_t>(), geometry, corners, cv, cf, face_count,
+                                                        invalid, keys);
+    run("face lists", [&](size_t& b) {
+        return hipcub::DeviceRadixSort::SortPairs(work, b, cv, cv_sorted, cf, faces, int(corners), 0, 32, st);
+    });
+    run("face offsets", [&](size_t& b) {
+        return hipcub::DeviceScan::ExclusiveSum(work, b, face_count, face_offsets, int(n + 1), st);
+    });
+    run("edge sort", [&](size_t& b) {
+        return hipcub::DeviceRadixSort::SortKeys(work, b, keys, keys_sorted, int(corners), 0, 64, st);
+    });
+    run("edge runs", [&](size_t& b) {
+        return hipcub::DeviceRunLengthEncode::Encode(work, b, keys_sorted, unique, counts, runs, int(corners), st);
+    });
+    uint32_t nruns = 0;
+    hip_check(hipMemcpyAsync(&nruns, runs, 4, hipMemcpyDeviceToHost, st), "download runs");
+    hip_check(hipStreamSynchronize(st), "edge runs");
+    if (nruns) {
+        unsigned long long last = 0;
+        hip_check(hipMemcpy(&last, unique + nruns - 1, 8, hipMemcpyDeviceToHost), "download last edge");
+        if (last == kNoEdge) --nruns;
+    }
+    edges_ = nruns;
+    // ---- edges: flags, degrees, sorted neighbour lists (both directions of
+    // every edge).
+    const uint32_t slots = 2 * edges_;
+    uint32_t* neighbors = grow<uint32_t>(neighbors_, slots);
+    uint8_t* visited = grow<uint8_t>(used_, std::max<size_t>(slots, size_t(n) * 4));
+    auto* all = grow<LegalAction>(all_buf, slots);
+    uint32_t* flags = grow<uint32_t>(flag_buf, 2 * size_t(slots) + 1);
+    uint32_t* where = flags + slots;
+    edge_flags_kernel<<<blocks_for(edges_), 256, 0, st>>>(unique, counts, edges_, boundary, invalid, degree, directed);
+    run("neighbour offsets", [&](size_t& b) {
+        return hipcub::DeviceScan::ExclusiveSum(work, b, degree, offsets, int(n + 1), st);
+    });
+    if (slots) {
+        run("neighbour sort", [&](size_t& b) {
+            return hipcub::DeviceRadixSort::SortKeys(work, b, directed, directed_sorted, int(slots), 0, 64, st);
+        });
+        low_words_kernel<<<blocks_for(slots), 256, 0, st>>>(directed_sorted, slots, neighbors);
+    }
+    Topo t{positions_.as<float4>(), uv ? uv_.as<float2>() : nullptr, indices_.as<uint32_t>(),
+           m.materials.empty() ? nullptr : materials_.as<uint16_t>(), geometry, invalid, boundary, offsets,
+           neighbors, face_offsets, faces, n, faces_};
+    fans_kernel<<<blocks_for(n), 256, 0, st>>>(t, invalid, visited);
+    // ---- legality of every directed edge, compacted in enumeration order.
+    if (slots) {
+        legal_kernel<<<blocks_for(slots), 256, 0, st>>>(t, unique, edges_, all, flags);
+        run("legal slots", [&](size_t& b) {
+            return hipcub::DeviceScan::ExclusiveSum(work, b, flags, where, int(slots), st);
+        });
+        uint32_t tail[2] = {0, 0};
+        hip_check(hipMemcpyAsync(&tail[0], where + slots - 1, 4, hipMemcpyDeviceToHost, st), "download count");
+        hip_check(hipMemcpyAsync(&tail[1], flags + slots - 1, 4, hipMemcpyDeviceToHost, st), "download count");
+        hip_check(hipStreamSynchronize(st), "legal count");
+        legal_count_ = tail[0] + tail[1];
+        auto* legal = grow<LegalAction>(legal_, legal_count_);
+        compact_kernel<<<blocks_for(slots), 256, 0, st>>>(all, flags, where, slots, legal);
+    }
+    hip_check(hipGetLastError(), "action topology launch");
+    hip_check(hipStreamSynchronize(st), "action topology");
+    build_ms_ = clock.ms();
+}
+
+ActionSet::~ActionSet() {
+    DeviceGuard guard(device_);
+    (void)hipStreamSynchronize(stream_);
+    (void)hipStreamDestroy(stream_);
+}
+
+void ActionSet::download_legal(uint32_t* pairs) const {
+    if (!legal_count_) return;
+    DeviceGuard guard(device_);
+    std::vector<LegalAction> host(legal_count_);
+    hip_check(hipMemcpy(host.data(), legal_.p, host.size() * sizeof(LegalAction), hipMemcpyDeviceToHost),
+              "download legal actions");
+    for (size_t i = 0; i < host.size(); ++i) {
+        pairs[2 * i] = host[i].from;
+        pairs[2 * i + 1] = host[i].to;
+    }
+}
+
+void ActionSet::claim(const uint32_t* rows, size_t n, uint32_t maximum, uint64_t removable, bool strict,
+                      std::vector<uint32_t>& picked) {
+    DeviceGuard guard(device_);
+    const hipStream_t st = stream_;
+    for (size_t i = 0; i < n; ++i)
+        if (rows[i] >= legal_count_) throw std::invalid_argument("action order index outside the legal actions");
+    uint32_t* used = grow<uint32_t>(used_, vertices_);
+    uint32_t* drows = grow<uint32_t>(selection_, n + 2 + 64);
+    uint32_t* out = drows + n;
+    if (n) hip_check(hipMemcpyAsync(drows, rows, n * 4, hipMemcpyHostToDevice, st), "upload order");
+    fill_u32_kernel<<<std::min<uint32_t>(1024, blocks_for(vertices_)), 256, 0, st>>>(used, vertices_, 0);
+    Topo t{positions_.as<float4>(), nullptr, indices_.as<uint32_t>(), nullptr, geometry_.as<uint32_t>(),
+           invalid_.as<uint8_t>(), boundary_.as<uint8_t>(), offsets_.as<uint32_t>(), neighbors_.as<uint32_t>(),
+           face_offsets_.as<uint32_t>(), face_lists_.as<uint32_t>(), vertices_, faces_};
+    claim_kernel<<<1, 64, 0, st>>>(t, legal_.as<LegalAction>(), drows, uint32_t(n), maximum, removable, strict ? 1u : 0u,
+                                   used, out);
+    hip_check(hipGetLastError(), "claim launch");
+    uint32_t host[2 + 64];
+    hip_check(hipMemcpyAsync(host, out, sizeof host, hipMemcpyDeviceToHost, st), "download selection");
+    hip_check(hipStreamSynchronize(st), "claim");
+    if (strict && host[1]) throw std::invalid_argument("illegal, stale or overlapping endpoint actions");
+    picked.assign(host + 2, host + 2 + host[0]);
+}
+
+std::vector<uint32_t> ActionSet::independent(const uint32_t* order, size_t n, uint32_t maximum, uint64_t removable) {
+    if (!maximum || maximum > 64) throw std::invalid_argument("action batch outside 1..64");
+    std::vector<uint32_t> picked;
+    if (!n || !removable) return picked;
+    claim(order, n, maximum, removable, false, picked);
+    return picked;
+}
+
+std::vector<uint32_t> ActionSet::trial(const uint32_t* rows, size_t n) {
+    if (!n || n > 64) throw std::invalid_argument("invalid action batch");
+    std::vector<uint32_t> picked;
+    claim(rows, n, uint32_t(n), ~uint64_t(0), true, picked);
+    DeviceGuard guard(device_);
+    const hipStream_t st = stream_;
+    uint32_t* remap = grow<uint32_t>(scratch_a_, vertices_ + 2 * size_t(faces_) + 1 + 3 * size_t(faces_));
+    uint32_t* keep = remap + vertices_;
+    uint32_t* slot = keep + faces_ + 1;
+    uint32_t* next = slot + faces_;
+    fill_u32_kernel<<<std::min<uint32_t>(1024, blocks_for(vertices_)), 256, 0, st>>>(remap, vertices_, kNone);
+    const uint32_t* drows = selection_.as<uint32_t>();
+    remap_kernel<<<1, 64, 0, st>>>(legal_.as<LegalAction>(), drows, uint32_t(n), remap);
+    Topo t{positions_.as<float4>(), nullptr, indices_.as<uint32_t>(), nullptr, geometry_.as<uint32_t>(),
+           invalid_.as<uint8_t>(), boundary_.as<uint8_t>(), offsets_.as<uint32_t>(), neighbors_.as<uint32_t>(),
+           face_offsets_.as<uint32_t>(), face_lists_.as<uint32_t>(), vertices_, faces_};
+    trial_keep_kernel<<<blocks_for(faces_), 256, 0, st>>>(t, remap, next, keep);
+    size_t need = 0;
+    hip_check(hipcub::DeviceScan::ExclusiveSum(nullptr, need, keep, slot, int(faces_), st), "scan workspace");
+    void* temp = grow<char>(temp_, std::max(need, temp_.bytes));
+    size_t b = temp_.bytes;
+    hip_check(hipcub::DeviceScan::ExclusiveSum(temp, b, keep, slot, int(faces_), st), "trial slots");
+    uint32_t tail[2] = {0, 0};
+    hip_check(hipMemcpyAsync(&tail[0], slot + faces_ - 1, 4, hipMemcpyDeviceToHost, st), "download count");
+    hip_check(hipMemcpyAsync(&tail[1], keep + faces_ - 1, 4, hipMemcpyDeviceToHost, st), "download count");
+    hip_check(hipStreamSynchronize(st), "trial count");
+    const uint32_t kept = tail[0] + tail[1];
+    if (!kept) throw std::invalid_argument("batch would remove the entire mesh");
+    uint32_t* out = grow<uint32_t>(scratch_b_, 3 * size_t(kept));
+    trial_copy_kernel<<<blocks_for(faces_), 256, 0, st>>>(next, keep, slot, faces_, out);
+    std::vector<uint32_t> host(3 * size_t(kept));
+    hip_check(hipMemcpyAsync(host.data(), out, host.size() * 4, hipMemcpyDeviceToHost, st), "download trial");
+    hip_check(hipStreamSynchronize(st), "trial");
+    return host;
+}
+
+void ActionSet::patches(const uint32_t* rows, size_t n, std::vector<uint32_t>& removed_offsets,
+                        std::vector<uint32_t>& removed, std::vector<uint32_t>& added_offsets,
+                        std::vector<uint32_t>& added) {
+    removed_offsets.assign(n + 1, 0);
+    added_offsets.assign(n + 1, 0);
+    removed.clear();
+    added.clear();
+    if (!n) return;
+    for (size_t i = 0; i < n; ++i)
+        if (rows[i] >= legal_count_) throw std::invalid_argument("patch row outside the legal actions");
+    DeviceGuard guard(device_);
+    const hipStream_t st = stream_;
+    uint32_t* drows = grow<uint32_t>(selection_, 5 * n + 2);
+    uint32_t* rc = drows + n;           // n + 1
+    uint32_t* ac = rc + n + 1;          // n + 1
+    uint32_t* ro = ac + n + 1;          // n (exclusive offsets)
+    uint32_t* ao = ro + n;              // n
+    hip_check(hipMemcpyAsync(drows, rows, n * 4, hipMemcpyHostToDevice, st), "upload rows");
+    Topo t{positions_.as<float4>(), nullptr, indices_.as<uint32_t>(), nullptr, geometry_.as<uint32_t>(),
+           invalid_.as<uint8_t>(), boundary_.as<uint8_t>(), offsets_.as<uint32_t>(), neighbors_.as<uint32_t>(),
+           face_offsets_.as<uint32_t>(), face_lists_.as<uint32_t>(), vertices_, faces_};
+    patch_count_kernel<<<blocks_for(n), 256, 0, st>>>(t, legal_.as<LegalAction>(), drows, uint32_t(n), rc, ac);
+    std::vector<uint32_t> hrc(n), hac(n);
+    hip_check(hipMemcpyAsync(hrc.data(), rc, n * 4, hipMemcpyDeviceToHost, st), "download counts");
+    hip_check(hipMemcpyAsync(hac.data(), ac, n * 4, hipMemcpyDeviceToHost, st), "download counts");
+    hip_check(hipStreamSynchronize(st), "patch counts");
+    for (size_t i = 0; i < n; ++i) {
+        removed_offsets[i + 1] = removed_offsets[i] + hrc[i];
+        added_offsets[i + 1] = added_offsets[i] + hac[i];
+    }
+    hip_check(hipMemcpyAsync(ro, removed_offsets.data(), n * 4, hipMemcpyHostToDevice, st), "upload offsets");
+    hip_check(hipMemcpyAsync(ao, added_offsets.data(), n * 4, hipMemcpyHostToDevice, st), "upload offsets");
+    const size_t nr = removed_offsets[n], na = added_offsets[n];
+    uint32_t* dr = grow<uint32_t>(scratch_b_, nr + 3 * na);
+    uint32_t* da = dr + nr;
+    patch_fill_kernel<<<blocks_for(n), 256, 0, st>>>(t, legal_.as<LegalAction>(), drows, uint32_t(n), ro, ao, dr, da);
+    removed.resize(nr);
+    added.resize(3 * na);
+    if (nr) hip_check(hipMemcpyAsync(removed.data(), dr, nr * 4, hipMemcpyDeviceToHost, st), "download removed");
+    if (na) hip_check(hipMemcpyAsync(added.data(), da, 3 * na * 4, hipMemcpyDeviceToHost, st), "download added");
+    hip_check(hipGetLastError(), "patch launch");
+    hip_check(hipStreamSynchronize(st), "patches");
+}
+
+}  // namespace blitz_hip

@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "../../include/blitz_hip.h"
+#include "actions.hpp"
 #include "audit.hpp"
 #include "patch_audit.hpp"
 #include "patches.hpp"
@@ -33,6 +34,9 @@ struct blitz_hip_scorer {
 struct blitz_hip_binding {
     blitz_hip_audit* owner;
     std::unique_ptr<Binding> impl;
+};
+struct blitz_hip_actions {
+    std::unique_ptr<ActionSet> impl;
 };
 struct blitz_hip_graph {
     blitz_hip_scorer* owner;
@@ -444,6 +448,97 @@ blitz_status blitz_hip_local_patches(const blitz_mesh* m, uint32_t count, uint64
         std::memcpy(added_offsets, p.added_offsets.data(), p.added_offsets.size() * 4);
         if (!p.removed.empty()) std::memcpy(removed, p.removed.data(), p.removed.size() * 4);
         if (!p.added.empty()) std::memcpy(added, p.added.data(), p.added.size() * 4);
+    });
+}
+
+// ---- collapse actions
+namespace {
+ActionSet& actions_of(blitz_hip_actions* a) {
+    if (!a || !a->impl) throw std::invalid_argument("null action state");
+    return *a->impl;
+}
+}  // namespace
+
+blitz_status blitz_hip_actions_create(int32_t device, const blitz_mesh* m, blitz_hip_actions** out, char* error,
+                                      size_t capacity) {
+    if (out) *out = nullptr;
+    return guarded(error, capacity, [&] {
+        if (!out) throw std::invalid_argument("null action state output");
+        if (!blitz_hip_available(device)) throw Unavailable("no gfx950 HIP device at the requested index");
+        const HostMesh h = host_mesh(m);
+        std::vector<float> uv;
+        if (m->uv.count) {
+            const size_t n = h.positions.size() / 3;
+            uv.resize(2 * n);
+            const auto* base = static_cast<const unsigned char*>(m->uv.data);
+            for (size_t i = 0; i < n; ++i) std::memcpy(&uv[2 * i], base + i * m->uv.stride, 8);
+        }
+        auto a = std::make_unique<blitz_hip_actions>();
+        a->impl = std::make_unique<ActionSet>(device, h, uv.empty() ? nullptr : uv.data());
+        *out = a.release();
+    });
+}
+
+void blitz_hip_actions_destroy(blitz_hip_actions* a) { delete a; }
+
+blitz_status blitz_hip_actions_info(const blitz_hip_actions* a, uint32_t* edges, uint32_t* legal, double* build_ms) {
+    if (!a || !a->impl) return BLITZ_INVALID_ARGUMENT;
+    if (edges) *edges = a->impl->edge_count();
+    if (legal) *legal = a->impl->legal_count();
+    if (build_ms) *build_ms = a->impl->build_ms();
+    return BLITZ_OK;
+}
+
+blitz_status blitz_hip_actions_legal(blitz_hip_actions* a, uint32_t* pairs, size_t capacity, char* error,
+                                     size_t error_capacity) {
+    return guarded(error, error_capacity, [&] {
+        ActionSet& s = actions_of(a);
+        if (!pairs || capacity < s.legal_count()) throw std::invalid_argument("legal action capacity too small");
+        s.download_legal(pairs);
+    });
+}
+
+blitz_status blitz_hip_actions_independent(blitz_hip_actions* a, const uint32_t* order, size_t n, uint32_t maximum,
+                                           uint64_t maximum_removed, uint32_t* selected, size_t* selected_count,
+                                           char* error, size_t capacity) {
+    return guarded(error, capacity, [&] {
+        if ((!order && n) || !selected || !selected_count) throw std::invalid_argument("null selection arrays");
+        const auto picked = actions_of(a).independent(order, n, maximum, maximum_removed);
+        std::copy(picked.begin(), picked.end(), selected);
+        *selected_count = picked.size();
+    });
+}
+
+blitz_status blitz_hip_actions_trial(blitz_hip_actions* a, const uint32_t* rows, size_t n, uint32_t* indices,
+                                     size_t index_capacity, size_t* index_count, char* error, size_t capacity) {
+    return guarded(error, capacity, [&] {
+        if (!rows || !indices || !index_count) throw std::invalid_argument("null trial arrays");
+        const auto out = actions_of(a).trial(rows, n);
+        if (out.size() > index_capacity) throw std::invalid_argument("trial index capacity too small");
+        std::memcpy(indices, out.data(), out.size() * 4);
+        *index_count = out.size();
+    });
+}
+
+blitz_status blitz_hip_actions_patches(blitz_hip_actions* a, const uint32_t* rows, size_t n,
+                                       uint32_t* removed_offsets, uint32_t* removed, size_t removed_capacity,
+                                       uint32_t* added_offsets, uint32_t* added, size_t added_capacity,
+                                       size_t* removed_count, size_t* added_count, char* error, size_t capacity) {
+    return guarded(error, capacity, [&] {
+        if ((!rows && n) || !removed_count || !added_count) throw std::invalid_argument("null patch arguments");
+        std::vector<uint32_t> ro, rv, ao, av;
+        actions_of(a).patches(rows, n, ro, rv, ao, av);
+        *removed_count = rv.size();
+        *added_count = av.size() / 3;
+        if (!removed_offsets && !removed && !added_offsets && !added) return;
+        if (!removed_offsets || !added_offsets || (!removed && !rv.empty()) || (!added && !av.empty()))
+            throw std::invalid_argument("null patch output arrays");
+        if (removed_capacity < rv.size() || added_capacity < av.size() / 3)
+            throw std::invalid_argument("patch output capacity too small");
+        std::memcpy(removed_offsets, ro.data(), ro.size() * 4);
+        std::memcpy(added_offsets, ao.data(), ao.size() * 4);
+        if (!rv.empty()) std::memcpy(removed, rv.data(), rv.size() * 4);
+        if (!av.empty()) std::memcpy(added, av.data(), av.size() * 4);
     });
 }
 

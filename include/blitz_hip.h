@@ -223,6 +223,44 @@ blitz_status blitz_hip_local_patches(const blitz_mesh*, uint32_t count, uint64_t
                                      size_t added_capacity, size_t* removed_count, size_t* added_count,
                                      char* error, size_t capacity);
 
+/* ---- collapse actions ------------------------------------------------------ */
+
+/* Device state of one mesh's collapse actions: the GPU action topology of
+ * src/neural/action_gpu.cu:178-343 with the semantics of the CPU
+ * blitz::neural::ActionState (src/neural/action.cpp:27-145). Vertices with
+ * bit-equal positions weld to the smallest index; an optional uv stream in the
+ * mesh descriptor enables the UV orientation test. */
+typedef struct blitz_hip_actions blitz_hip_actions;
+blitz_status blitz_hip_actions_create(int32_t device, const blitz_mesh*, blitz_hip_actions**, char* error,
+                                      size_t capacity);
+void blitz_hip_actions_destroy(blitz_hip_actions*);
+/* Undirected edges, legal directed actions and the device build time. */
+blitz_status blitz_hip_actions_info(const blitz_hip_actions*, uint32_t* edges, uint32_t* legal, double* build_ms);
+/* Legal directed collapses (from, to pairs) in ActionState::actions order
+ * (src/neural/action.cpp:371-383: edges ascending, a->b then b->a), the
+ * enumerate/legal kernels of src/neural/action_gpu.cu:487-643. */
+blitz_status blitz_hip_actions_legal(blitz_hip_actions*, uint32_t* pairs, size_t capacity, char* error,
+                                     size_t error_capacity);
+/* ActionState::independent (src/neural/action.cpp:280-304) / select_batch
+ * (src/neural/action_gpu.cu:865-899): greedy non-overlapping footprints over
+ * `order` (indices of legal actions), at most `maximum` (1..64) actions and
+ * `maximum_removed` faces; writes the selected indices. */
+blitz_status blitz_hip_actions_independent(blitz_hip_actions*, const uint32_t* order, size_t n, uint32_t maximum,
+                                           uint64_t maximum_removed, uint32_t* selected, size_t* selected_count,
+                                           char* error, size_t capacity);
+/* ActionState::trial (src/neural/action.cpp:308-344) / keep_faces + copy_trial
+ * (src/neural/action_gpu.cu:757-792): the index buffer with the legal actions
+ * `rows` (1..64, non-overlapping) applied. */
+blitz_status blitz_hip_actions_trial(blitz_hip_actions*, const uint32_t* rows, size_t n, uint32_t* indices,
+                                     size_t index_capacity, size_t* index_count, char* error, size_t capacity);
+/* Each legal action rows[i] as a patch for blitz_hip_audit_evaluate_patches:
+ * the faces around its `from` vertex (ascending) removed and their remapped
+ * survivors added. Null output arrays only query the sizes (faces). */
+blitz_status blitz_hip_actions_patches(blitz_hip_actions*, const uint32_t* rows, size_t n,
+                                       uint32_t* removed_offsets, uint32_t* removed, size_t removed_capacity,
+                                       uint32_t* added_offsets, uint32_t* added, size_t added_capacity,
+                                       size_t* removed_count, size_t* added_count, char* error, size_t capacity);
+
 /* blitz::detail::rasterize_coverage (src/coverage.hpp:12, src/evaluate.cpp:125-189)
  * for one camera: row-major packed words (index = y*size + x), padding bits
  * zero. *size_out receives the raster side; returns BLITZ_BUDGET_LIMITED when

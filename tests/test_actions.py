@@ -134,3 +134,89 @@ def test_tetrahedron_has_no_legal_collapse():
     pos = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1]], np.float32)
     idx = np.array([[0, 2, 1], [0, 1, 3], [1, 2, 3], [0, 3, 2]], np.uint32)
     assert len(Actions("oracle", Mesh(pos, idx)).legal) == 0
+
+
+# ----------------------------------------------------------------------- GPU
+
+def face_multiset(idx: np.ndarray):
+    return sorted(map(tuple, np.asarray(idx).reshape(-1, 3).tolist()))
+
+
+@pytest.mark.gpu
+def test_gpu_actions_match_golden():
+    from blitzremesher_amd.actions import ActionSet
+    check_against_golden(lambda m, uv: ActionSet(m, uv))
+
+
+@pytest.mark.gpu
+def test_gpu_actions_match_oracle():
+    from blitzremesher_amd.actions import ActionSet
+    cases = {"icosphere4": (meshes.icosphere(4, 0.03, 7), None),
+             "heightfield": (meshes.jitter(meshes.heightfield(64, 0.5, 8), 0.01, 8, 0.5), None),
+             "torus": (meshes.torus(1.0, 0.3, 60, 24, 0.02, 9), None)}
+    cases.update(action_cases())
+    for name, (mesh, uv) in cases.items():
+        gpu, cpu = ActionSet(mesh, uv), Actions("oracle", mesh, uv)
+        assert np.array_equal(gpu.legal, cpu.legal), name
+        rng = np.random.default_rng(31)
+        for maximum, budget in ((64, 10 ** 9), (7, 10 ** 9), (64, 9), (64, 1)):
+            order = rng.permutation(len(cpu.legal)).astype(np.uint32)
+            s1, s2 = gpu.independent(order, maximum, budget), cpu.independent(order, maximum, budget)
+            assert np.array_equal(s1, s2), (name, maximum, budget)
+            if len(s1):
+                assert np.array_equal(gpu.trial(s1), cpu.trial(s2)), (name, maximum, budget)
+        gpu.close()
+
+
+@pytest.mark.gpu
+def test_gpu_action_patches_equal_trials():
+    from blitzremesher_amd.actions import ActionSet
+    for name in ("heightfield", "seam", "junction", "torus"):
+        mesh, uv = action_cases()[name]
+        with ActionSet(mesh, uv) as gpu:
+            rows = np.arange(0, len(gpu.legal), max(1, len(gpu.legal) // 60), dtype=np.uint32)
+            p = gpu.patches(rows)
+            assert len(p) == len(rows)
+            for i, r in enumerate(rows):
+                assert np.all(np.diff(p.removed[p.removed_offsets[i]:p.removed_offsets[i + 1]].astype(np.int64)) > 0)
+                assert face_multiset(p.candidate(mesh, i).indices) == face_multiset(gpu.trial(np.array([r]))), (name, r)
+
+
+@pytest.mark.gpu
+def test_gpu_trial_rejects_overlapping_actions():
+    from blitzremesher_amd._lib import BlitzError
+    from blitzremesher_amd.actions import ActionSet
+    mesh = meshes.icosphere(2, 0.0, 1)
+    with ActionSet(mesh) as gpu:
+        u = gpu.legal[0, 0]
+        same = np.nonzero(gpu.legal[:, 0] == u)[0][:2].astype(np.uint32)
+        with pytest.raises(BlitzError):
+            gpu.trial(same)
+        with pytest.raises(BlitzError):
+            gpu.independent(np.array([len(gpu.legal)], np.uint32), 4, 10)
+
+
+@pytest.mark.gpu
+def test_gpu_action_patches_audit_like_full_candidates():
+    # Every selected collapse of a seeded rank order, audited as a patch of the
+    # bound source, equals the full audit of its trial mesh.
+    from blitzremesher_amd import Audit, EvalSettings
+    from blitzremesher_amd.actions import ActionSet
+    from blitzremesher_amd.audit import bounds
+    mesh = meshes.jitter(meshes.heightfield(48, 0.2, 4), 0.004, 4, 0.5)
+    # limit 0.75 at ss 4 passes a view only with zero coverage distance
+    # (allowance 2*sqrt(2)/4 + 1e-6): both verdicts occur.
+    s = EvalSettings(profile=0, orthographic=12, perspective=0, screen_size=40, supersample=4, max_supersample=4,
+                     limit=0.75, force_two_sided=True)
+    center, radius = bounds(mesh)
+    with ActionSet(mesh) as gpu, Audit(0) as audit:
+        rows = np.random.default_rng(2).choice(len(gpu.legal), 96, replace=False).astype(np.uint32)
+        p = gpu.patches(rows)
+        binding = audit.bind(mesh, center, radius, s)
+        got, _ = audit.evaluate_patches(binding, p, s)
+        ref = audit.upload(mesh)
+        for i, r in enumerate(rows):
+            cand = audit.upload(Mesh(mesh.positions, gpu.trial(np.array([r]))))
+            assert got[i] == audit.evaluate(ref, cand, center, radius, s), (i, r)
+            cand.release()
+        assert any(not m.passed for m in got) and any(m.passed for m in got)
