@@ -108,6 +108,11 @@ class AuditTiming(C.Structure):
     ]
 
 
+class ActionStats(C.Structure):
+    _fields_ = [("struct_size", C.c_uint32), ("states", C.c_uint32), ("ranked", C.c_uint64), ("trials", C.c_uint64),
+                ("accepted", C.c_uint64), ("rejected", C.c_uint64), ("ms", C.c_double)]
+
+
 _P = C.c_void_p
 _ERR = [C.c_char_p, C.c_size_t]
 
@@ -160,6 +165,10 @@ PROTOTYPES = {
                                                 C.POINTER(C.c_uint32), C.POINTER(C.c_size_t), *_ERR]),
     "blitz_hip_actions_trial": (C.c_int, [_P, C.POINTER(C.c_uint32), C.c_size_t, C.POINTER(C.c_uint32), C.c_size_t,
                                           C.POINTER(C.c_size_t), *_ERR]),
+    "blitz_hip_actions_rank": (C.c_int, [_P, C.c_uint8, _P, _P, C.POINTER(C.c_uint32), C.c_size_t, *_ERR]),
+    "blitz_hip_execute_actions": (C.c_int, [_P, C.POINTER(BlitzMesh), C.POINTER(Bounds), C.POINTER(EvalSettingsC),
+                                            C.c_uint64, C.c_uint32, C.c_uint8, C.c_uint8, C.POINTER(C.c_uint32),
+                                            C.c_size_t, C.POINTER(C.c_size_t), C.POINTER(ActionStats), *_ERR]),
     "blitz_hip_actions_patches": (C.c_int, [_P, C.POINTER(C.c_uint32), C.c_size_t, C.POINTER(C.c_uint32),
                                             C.POINTER(C.c_uint32), C.c_size_t, C.POINTER(C.c_uint32),
                                             C.POINTER(C.c_uint32), C.c_size_t, C.POINTER(C.c_size_t),

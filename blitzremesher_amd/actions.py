@@ -55,6 +55,16 @@ class ActionSet:
         except Exception:
             pass
 
+    def rank(self, ranking: int, scorer=None, graph=None) -> np.ndarray:
+        """Rank order of the legal actions (0 learned from `graph` after a
+        scorer forward, 1 constant, 3 shortest edge, 4 current plane)."""
+        order = np.zeros(max(1, len(self.legal)), np.uint32)
+        err = _lib.error_buffer()
+        _lib.check(self.lib.blitz_hip_actions_rank(self.handle, ranking, scorer.handle if scorer else None,
+                                                   graph.handle if graph else None, _u32p(order), order.size, err,
+                                                   len(err)), err)
+        return order[: len(self.legal)].copy()
+
     def independent(self, order: np.ndarray, maximum: int, maximum_removed: int) -> np.ndarray:
         """ActionState::independent over legal-action indices in rank order."""
         order = np.ascontiguousarray(order, np.uint32)
@@ -94,4 +104,30 @@ class ActionSet:
         return Patches(ro, rv[: nr.value].copy(), ao, av[: 3 * na.value].reshape(-1, 3).copy())
 
 
-__all__ = ["ActionSet"]
+def execute_actions(audit, mesh: Mesh, center, radius, gate, target: int, budget: int, ranking: int,
+                    batch: int, uv: np.ndarray | None = None):
+    """neural::execute_actions (src/neural/action.cpp:453-525) on the device:
+    every trial gated by an exact audit against `mesh` under the EvalSettings
+    `gate`. Returns (index buffer (m, 3), stats dict)."""
+    from .audit import _bounds_c
+    lib = _lib.load()
+    desc = mesh.descriptor()
+    if uv is not None:
+        uv = np.ascontiguousarray(uv, np.float32).reshape(-1, 2)
+        desc.uv = _lib.BlitzStream(uv.ctypes.data, uv.shape[0], 8)
+    s, keep = gate.to_c()
+    b = _bounds_c(center, radius)
+    out = np.zeros(mesh.indices.size, np.uint32)
+    count = C.c_size_t()
+    st = _lib.ActionStats()
+    st.struct_size = C.sizeof(st)
+    err = _lib.error_buffer()
+    _lib.check(lib.blitz_hip_execute_actions(audit.handle, C.byref(desc), C.byref(b), C.byref(s), int(target), budget,
+                                             ranking, batch, _u32p(out), out.size, C.byref(count), C.byref(st), err,
+                                             len(err)), err)
+    del keep
+    stats = {k: getattr(st, k) for k, _ in st._fields_ if k != "struct_size"}
+    return out[: count.value].reshape(-1, 3).copy(), stats
+
+
+__all__ = ["ActionSet", "execute_actions"]

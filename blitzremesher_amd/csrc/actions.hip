@@ -37,20 +37,7 @@ namespace {
 constexpr uint32_t kNone = 0xFFFFFFFFu;
 constexpr unsigned long long kNoEdge = ~0ull;
 
-struct Topo {
-    const float4* positions;
-    const float2* uv;  // or null
-    const uint32_t* indices;
-    const uint16_t* materials;  // or null
-    const uint32_t* geometry;
-    const uint8_t* invalid;
-    const uint8_t* boundary;
-    const uint32_t* offsets;
-    const uint32_t* neighbors;
-    const uint32_t* face_offsets;
-    const uint32_t* faces;
-    uint32_t vertices, triangles;
-};
+using Topo = ActionTopo;
 
 __global__ void weld_keys_kernel(const float4* __restrict__ p, uint32_t n, uint32_t* kx, uint32_t* ky, uint32_t* kz,
                                  uint32_t* ids) {
@@ -430,14 +417,14 @@ __device__ __forceinline__ void store_used(uint32_t* p) {
 }
 
 // src/neural/action.cpp:261-304 on one wavefront: out[0] = picked count,
-// out[1] = 1 when a strict batch (a trial) is not independent, out[2..] the
-// picked rows.
+// out[1] = 1 when a strict batch (a trial) is not independent, out[2] = the
+// position in `rows` of the first pick, out[3..] the picked rows.
 __global__ __launch_bounds__(64) void claim_kernel(Topo t, const LegalAction* __restrict__ legal,
                                                    const uint32_t* __restrict__ rows, uint32_t n, uint32_t maximum,
                                                    unsigned long long removable, uint32_t strict, uint32_t* used,
                                                    uint32_t* out) {
     const uint32_t lane = threadIdx.x;
-    uint32_t picked = 0, error = 0;
+    uint32_t picked = 0, error = 0, first = kNone;
     for (uint32_t k = 0; k < n; ++k) {
         const LegalAction a = legal[rows[k]];
         if (a.removed > removable) {
@@ -461,7 +448,8 @@ __global__ __launch_bounds__(64) void claim_kernel(Topo t, const LegalAction* __
             for (uint32_t j = t.offsets[u] + lane; j < t.offsets[u + 1]; j += 64) store_used(&used[t.neighbors[j]]);
         }
         __threadfence();
-        if (lane == 0) out[2 + picked] = rows[k];
+        if (lane == 0) out[3 + picked] = rows[k];
+        if (!picked) first = k;
         ++picked;
         removable -= a.removed;
         if (picked == maximum || !removable) break;
@@ -469,7 +457,74 @@ __global__ __launch_bounds__(64) void claim_kernel(Topo t, const LegalAction* __
     if (lane == 0) {
         out[0] = picked;
         out[1] = error;
+        out[2] = first;
     }
+}
+
+// Ranking keys of the legal actions (src/neural/action_gpu.cu:809-844 with
+// the scores of src/neural/action.cpp:355-370, tests/neural_action_gpu.cpp:
+// 155-160): 1 constant, 3 shortest edge, 4 current plane, 0 learned (the
+// logit of the scorer action from -> to, found in the scorer graph's sorted
+// neighbour lists). A non-finite score is an error, as in
+// src/neural/action.cpp:477-479. -0 sorts as +0 so
+// that equal scores keep ascending action order, as a stable sort does.
+struct RankInput {
+    const LegalAction* legal;
+    uint32_t count;
+    uint32_t mode;
+    double diameter;
+    const uint32_t* graph_offsets;
+    const uint32_t* graph_neighbors;
+    const float* outputs;
+    uint32_t outputs_per_action;
+    float* keys;
+    uint32_t* order;
+    uint32_t* bad;  // set when a score is not finite
+};
+
+__device__ __forceinline__ F3 f3(float4 p) { return {p.x, p.y, p.z}; }
+
+__global__ void rank_kernel(Topo t, RankInput in) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= in.count) return;
+    const LegalAction a = in.legal[i];
+    const double scale = fmax(1e-20, in.diameter);
+    float score = 0.f;
+    if (in.mode == 3) {
+        const F3 d = sub3(t.positions[a.to], t.positions[a.from]);
+        const double s = 1 / scale;
+        const F3 e{float(d.x * s), float(d.y * s), float(d.z * s)};
+        score = -float(len3(e));
+    } else if (in.mode == 4) {
+        double cost = 0;
+        for (uint32_t k = t.face_offsets[a.from]; k < t.face_offsets[a.from + 1]; ++k) {
+            const uint32_t f = t.faces[k];
+            const float4 p = t.positions[t.indices[3 * f]], q = t.positions[t.indices[3 * f + 1]],
+                         r = t.positions[t.indices[3 * f + 2]];
+            const F3 normal = cross3(sub3(q, p), sub3(r, p));
+            const double weight = len3(normal);
+            const double l = len3(normal);
+            const F3 unit = l > 0 ? F3{float(normal.x * (1 / l)), float(normal.y * (1 / l)), float(normal.z * (1 / l))}
+                                  : F3{0, 0, 0};
+            const double distance = dot3(unit, sub3(t.positions[a.to], p)) / scale;
+            cost += distance * distance * weight / (scale * scale);
+        }
+        score = float(-cost);
+    } else if (in.mode == 0) {
+        uint32_t lo = in.graph_offsets[a.from], hi = in.graph_offsets[a.from + 1];
+        const uint32_t end = hi;
+        while (lo < hi) {
+            const uint32_t mid = lo + (hi - lo) / 2;
+            if (in.graph_neighbors[mid] < a.to)
+                lo = mid + 1;
+            else
+                hi = mid;
+        }
+        score = lo < end && in.graph_neighbors[lo] == a.to ? in.outputs[size_t(lo) * in.outputs_per_action] : NAN;
+    }
+    if (!isfinite(score)) atomicExch(in.bad, 1u);
+    in.keys[i] = score + 0.f;
+    in.order[i] = i;
 }
 
 __global__ void remap_kernel(const LegalAction* __restrict__ legal, const uint32_t* __restrict__ rows, uint32_t n,
@@ -499,10 +554,12 @@ __global__ void trial_keep_kernel(Topo t, const uint32_t* __restrict__ remap, ui
 }
 
 __global__ void trial_copy_kernel(const uint32_t* __restrict__ next, const uint32_t* __restrict__ keep,
-                                  const uint32_t* __restrict__ slots, uint32_t faces, uint32_t* out) {
+                                  const uint32_t* __restrict__ slots, const uint16_t* __restrict__ materials,
+                                  uint32_t faces, uint32_t* out, uint16_t* out_materials) {
     const uint32_t f = blockIdx.x * blockDim.x + threadIdx.x;
     if (f >= faces || !keep[f]) return;
     for (uint32_t j = 0; j < 3; ++j) out[3 * slots[f] + j] = next[3 * f + j];
+    if (materials) out_materials[slots[f]] = materials[f];
 }
 
 // Per patch: faces around `from` removed, the remapped ones not holding `to`
@@ -566,6 +623,7 @@ ActionSet::ActionSet(int device, const HostMesh& m, const float* uv) : device_(d
     if (!vertices_ || !faces_) throw std::invalid_argument("action state needs a non-empty mesh");
     if (uint64_t(faces_) * 6 >= uint64_t(INT32_MAX)) throw std::invalid_argument("mesh too large for 32-bit corners");
     const uint32_t n = vertices_, corners = 3 * faces_;
+    diameter_ = std::max(1e-20, bounds_of(m.positions.data(), n).radius * 2);
     // Every buffer of the build is sized here, before any launch, so nothing
     // is reallocated under queued work.
     DeviceBuffer weld, weld2, corner_buf, key_buf, temp, all_buf, flag_buf;
@@ -629,7 +687,8 @@ ActionSet::ActionSet(int device, const HostMesh& m, const float* uv) : device_(d
     hip_check(hipMemcpyAsync(grow<uint32_t>(indices_, corners), m.indices.data(), size_t(corners) * 4,
                              hipMemcpyHostToDevice, st),
               "upload indices");
-    if (!m.materials.empty())
+    has_materials_ = !m.materials.empty();
+    if (has_materials_)
         hip_check(hipMemcpyAsync(grow<uint16_t>(materials_, faces_), m.materials.data(), size_t(faces_) * 2,
                                  hipMemcpyHostToDevice, st),
                   "upload materials");
@@ -738,28 +797,98 @@ void ActionSet::download_legal(uint32_t* pairs) const {
     }
 }
 
-void ActionSet::claim(const uint32_t* rows, size_t n, uint32_t maximum, uint64_t removable, bool strict,
-                      std::vector<uint32_t>& picked) {
-    DeviceGuard guard(device_);
+ActionTopo ActionSet::topo() const {
+    return {positions_.as<float4>(), uv_.p ? uv_.as<float2>() : nullptr, indices_.as<uint32_t>(),
+            has_materials_ ? materials_.as<uint16_t>() : nullptr, geometry_.as<uint32_t>(), invalid_.as<uint8_t>(),
+            boundary_.as<uint8_t>(), offsets_.as<uint32_t>(), neighbors_.as<uint32_t>(),
+            face_offsets_.as<uint32_t>(), face_lists_.as<uint32_t>(), vertices_, faces_};
+}
+
+uint32_t ActionSet::claim_device(const uint32_t* drows, size_t n, uint32_t maximum, uint64_t removable, bool strict,
+                                 std::vector<uint32_t>& picked) {
     const hipStream_t st = stream_;
-    for (size_t i = 0; i < n; ++i)
-        if (rows[i] >= legal_count_) throw std::invalid_argument("action order index outside the legal actions");
     uint32_t* used = grow<uint32_t>(used_, vertices_);
-    uint32_t* drows = grow<uint32_t>(selection_, n + 2 + 64);
-    uint32_t* out = drows + n;
-    if (n) hip_check(hipMemcpyAsync(drows, rows, n * 4, hipMemcpyHostToDevice, st), "upload order");
+    uint32_t* out = grow<uint32_t>(claim_out_, 3 + 64);
     fill_u32_kernel<<<std::min<uint32_t>(1024, blocks_for(vertices_)), 256, 0, st>>>(used, vertices_, 0);
-    Topo t{positions_.as<float4>(), nullptr, indices_.as<uint32_t>(), nullptr, geometry_.as<uint32_t>(),
-           invalid_.as<uint8_t>(), boundary_.as<uint8_t>(), offsets_.as<uint32_t>(), neighbors_.as<uint32_t>(),
-           face_offsets_.as<uint32_t>(), face_lists_.as<uint32_t>(), vertices_, faces_};
-    claim_kernel<<<1, 64, 0, st>>>(t, legal_.as<LegalAction>(), drows, uint32_t(n), maximum, removable, strict ? 1u : 0u,
-                                   used, out);
+    claim_kernel<<<1, 64, 0, st>>>(topo(), legal_.as<LegalAction>(), drows, uint32_t(n), maximum, removable,
+                                   strict ? 1u : 0u, used, out);
     hip_check(hipGetLastError(), "claim launch");
-    uint32_t host[2 + 64];
+    uint32_t host[3 + 64];
     hip_check(hipMemcpyAsync(host, out, sizeof host, hipMemcpyDeviceToHost, st), "download selection");
     hip_check(hipStreamSynchronize(st), "claim");
     if (strict && host[1]) throw std::invalid_argument("illegal, stale or overlapping endpoint actions");
-    picked.assign(host + 2, host + 2 + host[0]);
+    picked.assign(host + 3, host + 3 + host[0]);
+    return host[2];
+}
+
+void ActionSet::claim(const uint32_t* rows, size_t n, uint32_t maximum, uint64_t removable, bool strict,
+                      std::vector<uint32_t>& picked) {
+    DeviceGuard guard(device_);
+    for (size_t i = 0; i < n; ++i)
+        if (rows[i] >= legal_count_) throw std::invalid_argument("action order index outside the legal actions");
+    uint32_t* drows = grow<uint32_t>(selection_, n);
+    if (n) hip_check(hipMemcpyAsync(drows, rows, n * 4, hipMemcpyHostToDevice, stream_), "upload order");
+    claim_device(drows, n, maximum, removable, strict, picked);
+}
+
+void ActionSet::rank(uint32_t mode, double diameter, const Graph* graph, uint32_t* order_out) {
+    if (mode == 2) throw Unavailable("shuffled ranking is not implemented by the HIP action state");
+    if (mode > 4) throw std::invalid_argument("unknown ranking");
+    if (mode == 0 && (!graph || !graph->has_outputs)) throw std::invalid_argument("learned ranking needs scorer outputs");
+    DeviceGuard guard(device_);
+    const hipStream_t st = stream_;
+    const uint32_t n = legal_count_;
+    ranked_ = 0;
+    if (!n) return;
+    float* keys = grow<float>(rank_keys_, 2 * size_t(n));
+    uint32_t* idx = grow<uint32_t>(rank_idx_, n);
+    uint32_t* order = grow<uint32_t>(order_, n);
+    uint32_t* bad = grow<uint32_t>(claim_out_, 3 + 64);
+    hip_check(hipMemsetAsync(bad, 0, 4, st), "clear");
+    RankInput in{};
+    in.legal = legal_.as<LegalAction>();
+    in.count = n;
+    in.mode = mode;
+    in.diameter = diameter;
+    if (mode == 0) {
+        in.graph_offsets = graph->offsets.as<uint32_t>();
+        in.graph_neighbors = graph->neighbors.as<uint32_t>();
+        in.outputs = graph->outputs.as<float>();
+        in.outputs_per_action = kOutputs;
+        if (graph->vertices != vertices_) throw std::invalid_argument("scorer graph of another mesh");
+    }
+    in.keys = keys;
+    in.order = idx;
+    in.bad = bad;
+    rank_kernel<<<blocks_for(n), 256, 0, st>>>(topo(), in);
+    size_t need = 0;
+    hip_check(hipcub::DeviceRadixSort::SortPairsDescending(nullptr, need, keys, keys + n, idx, order, int(n), 0, 32, st),
+              "sort workspace");
+    void* temp = grow<char>(temp_, std::max(need, temp_.bytes));
+    size_t b = temp_.bytes;
+    hip_check(hipcub::DeviceRadixSort::SortPairsDescending(temp, b, keys, keys + n, idx, order, int(n), 0, 32, st),
+              "rank sort");
+    uint32_t flag = 0;
+    hip_check(hipMemcpyAsync(&flag, bad, 4, hipMemcpyDeviceToHost, st), "download rank flag");
+    if (order_out) hip_check(hipMemcpyAsync(order_out, order, size_t(n) * 4, hipMemcpyDeviceToHost, st), "download order");
+    hip_check(hipGetLastError(), "rank launch");
+    hip_check(hipStreamSynchronize(st), "rank");
+    if (flag) throw std::invalid_argument("nonfinite action rank");
+    ranked_ = n;
+}
+
+std::vector<uint32_t> ActionSet::independent_ranked(size_t position, uint32_t maximum, uint64_t removable,
+                                                    size_t* first) {
+    if (!maximum || maximum > 64) throw std::invalid_argument("action batch outside 1..64");
+    if (ranked_ != legal_count_) throw std::invalid_argument("rank() has not run");
+    std::vector<uint32_t> picked;
+    *first = size_t(-1);
+    if (position >= legal_count_ || !removable) return picked;
+    DeviceGuard guard(device_);
+    const uint32_t k = claim_device(order_.as<uint32_t>() + position, legal_count_ - position, maximum, removable,
+                                    false, picked);
+    if (!picked.empty()) *first = position + k;
+    return picked;
 }
 
 std::vector<uint32_t> ActionSet::independent(const uint32_t* order, size_t n, uint32_t maximum, uint64_t removable) {
@@ -770,7 +899,7 @@ std::vector<uint32_t> ActionSet::independent(const uint32_t* order, size_t n, ui
     return picked;
 }
 
-std::vector<uint32_t> ActionSet::trial(const uint32_t* rows, size_t n) {
+std::vector<uint32_t> ActionSet::trial(const uint32_t* rows, size_t n, std::vector<uint16_t>* materials) {
     if (!n || n > 64) throw std::invalid_argument("invalid action batch");
     std::vector<uint32_t> picked;
     claim(rows, n, uint32_t(n), ~uint64_t(0), true, picked);
@@ -783,9 +912,7 @@ std::vector<uint32_t> ActionSet::trial(const uint32_t* rows, size_t n) {
     fill_u32_kernel<<<std::min<uint32_t>(1024, blocks_for(vertices_)), 256, 0, st>>>(remap, vertices_, kNone);
     const uint32_t* drows = selection_.as<uint32_t>();
     remap_kernel<<<1, 64, 0, st>>>(legal_.as<LegalAction>(), drows, uint32_t(n), remap);
-    Topo t{positions_.as<float4>(), nullptr, indices_.as<uint32_t>(), nullptr, geometry_.as<uint32_t>(),
-           invalid_.as<uint8_t>(), boundary_.as<uint8_t>(), offsets_.as<uint32_t>(), neighbors_.as<uint32_t>(),
-           face_offsets_.as<uint32_t>(), face_lists_.as<uint32_t>(), vertices_, faces_};
+    const Topo t = topo();
     trial_keep_kernel<<<blocks_for(faces_), 256, 0, st>>>(t, remap, next, keep);
     size_t need = 0;
     hip_check(hipcub::DeviceScan::ExclusiveSum(nullptr, need, keep, slot, int(faces_), st), "scan workspace");
@@ -798,12 +925,73 @@ std::vector<uint32_t> ActionSet::trial(const uint32_t* rows, size_t n) {
     hip_check(hipStreamSynchronize(st), "trial count");
     const uint32_t kept = tail[0] + tail[1];
     if (!kept) throw std::invalid_argument("batch would remove the entire mesh");
-    uint32_t* out = grow<uint32_t>(scratch_b_, 3 * size_t(kept));
-    trial_copy_kernel<<<blocks_for(faces_), 256, 0, st>>>(next, keep, slot, faces_, out);
+    uint32_t* out = grow<uint32_t>(scratch_b_, 4 * size_t(kept));
+    uint16_t* out_materials = reinterpret_cast<uint16_t*>(out + 3 * size_t(kept));
+    trial_copy_kernel<<<blocks_for(faces_), 256, 0, st>>>(next, keep, slot,
+                                                          has_materials_ ? materials_.as<uint16_t>() : nullptr,
+                                                          faces_, out, out_materials);
     std::vector<uint32_t> host(3 * size_t(kept));
     hip_check(hipMemcpyAsync(host.data(), out, host.size() * 4, hipMemcpyDeviceToHost, st), "download trial");
+    if (materials) {
+        materials->assign(has_materials_ ? kept : 0, 0);
+        if (has_materials_)
+            hip_check(hipMemcpyAsync(materials->data(), out_materials, size_t(kept) * 2, hipMemcpyDeviceToHost, st),
+                      "download trial materials");
+    }
     hip_check(hipStreamSynchronize(st), "trial");
     return host;
+}
+
+// src/neural/action.cpp:453-525 with the gate supplied by the caller.
+std::vector<uint32_t> execute_actions(int device, const HostMesh& input, const float* uv, uint64_t target,
+                                      uint32_t budget, uint32_t ranking, uint32_t batch, const ActionGate& gate,
+                                      ExecuteStats& stats, std::vector<uint16_t>* materials_out) {
+    if (!batch || batch > 64) throw std::invalid_argument("action batch outside 1..64");
+    if (ranking == 0 || ranking == 2)
+        throw Unavailable("the HIP executor ranks by the constant, shortest-edge and current-plane orders");
+    if (ranking > 4) throw std::invalid_argument("unknown ranking");
+    const auto clock0 = std::chrono::steady_clock::now();
+    stats = {};
+    const double diameter = std::max(1e-20, bounds_of(input.positions.data(), input.positions.size() / 3).radius * 2);
+    HostMesh state = input;
+    target = std::max<uint64_t>(1, target);
+    while (state.indices.size() / 3 > target && stats.trials < budget) {
+        ActionSet set(device, state, uv);
+        ++stats.states;
+        const size_t n = set.legal_count();
+        if (!n) break;
+        set.rank(ranking, diameter, nullptr, nullptr);
+        stats.ranked += n;
+        const uint64_t faces = state.indices.size() / 3;
+        bool accepted = false;
+        for (size_t position = 0; position < n && stats.trials < budget && !accepted;) {
+            size_t first = 0;
+            std::vector<uint32_t> picked = set.independent_ranked(position, batch, faces - target, &first);
+            if (picked.empty()) break;
+            for (;;) {
+                if (stats.trials == budget) break;
+                std::vector<uint16_t> materials;
+                std::vector<uint32_t> candidate = set.trial(picked.data(), picked.size(), &materials);
+                ++stats.trials;
+                if (gate(candidate, materials)) {
+                    state.indices = std::move(candidate);
+                    state.materials = std::move(materials);
+                    stats.accepted += picked.size();
+                    accepted = true;
+                    break;
+                }
+                stats.rejected += picked.size();
+                if (picked.size() == 1) break;
+                picked.resize(picked.size() / 2);
+            }
+            // A rejected singleton is not retried: continue after the first pick.
+            position = first + 1;
+        }
+        if (!accepted) break;
+    }
+    stats.ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - clock0).count();
+    if (materials_out) *materials_out = state.materials;
+    return state.indices;
 }
 
 void ActionSet::patches(const uint32_t* rows, size_t n, std::vector<uint32_t>& removed_offsets,
@@ -824,9 +1012,7 @@ void ActionSet::patches(const uint32_t* rows, size_t n, std::vector<uint32_t>& r
     uint32_t* ro = ac + n + 1;          // n (exclusive offsets)
     uint32_t* ao = ro + n;              // n
     hip_check(hipMemcpyAsync(drows, rows, n * 4, hipMemcpyHostToDevice, st), "upload rows");
-    Topo t{positions_.as<float4>(), nullptr, indices_.as<uint32_t>(), nullptr, geometry_.as<uint32_t>(),
-           invalid_.as<uint8_t>(), boundary_.as<uint8_t>(), offsets_.as<uint32_t>(), neighbors_.as<uint32_t>(),
-           face_offsets_.as<uint32_t>(), face_lists_.as<uint32_t>(), vertices_, faces_};
+    const Topo t = topo();
     patch_count_kernel<<<blocks_for(n), 256, 0, st>>>(t, legal_.as<LegalAction>(), drows, uint32_t(n), rc, ac);
     std::vector<uint32_t> hrc(n), hac(n);
     hip_check(hipMemcpyAsync(hrc.data(), rc, n * 4, hipMemcpyDeviceToHost, st), "download counts");

@@ -6,10 +6,12 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <functional>
 #include <vector>
 
 #include "audit.hpp"
 #include "device_buffer.hpp"
+#include "scorer.hpp"
 
 namespace blitz_hip {
 
@@ -21,6 +23,22 @@ struct LegalAction {
     uint32_t wedge_from[2], wedge_to[2];
     uint32_t wedges;   // 1 or 2
     uint32_t removed;  // faces the collapse removes (1 boundary, 2 interior)
+};
+
+// Device views of one mesh's action topology (all device pointers).
+struct ActionTopo {
+    const float4* positions;
+    const float2* uv;  // or null
+    const uint32_t* indices;
+    const uint16_t* materials;  // or null
+    const uint32_t* geometry;   // geometric vertex of each index
+    const uint8_t* invalid;
+    const uint8_t* boundary;
+    const uint32_t* offsets;       // neighbour lists (sorted)
+    const uint32_t* neighbors;
+    const uint32_t* face_offsets;  // faces around each geometric vertex (ascending)
+    const uint32_t* faces;
+    uint32_t vertices, triangles;
 };
 
 class ActionSet {
@@ -39,31 +57,65 @@ class ActionSet {
     // ActionState::independent (src/neural/action.cpp:280-304): greedy batch
     // of non-overlapping footprints in `order` (indices of legal actions).
     std::vector<uint32_t> independent(const uint32_t* order, size_t n, uint32_t maximum, uint64_t removable);
+    // Rank order of the legal actions (src/neural/action_gpu.cu:809-844,
+    // 1793-1795; ranking values of blitz_neural_options_v2): 0 learned (the
+    // scorer logits of `graph`), 1 constant, 3 shortest edge, 4 current plane;
+    // kept on the device for independent_ranked, copied to order_out if given.
+    void rank(uint32_t mode, double diameter, const Graph* graph, uint32_t* order_out);
+    // independent() over the device rank order from `position`; *first
+    // receives the rank position of the first pick.
+    std::vector<uint32_t> independent_ranked(size_t position, uint32_t maximum, uint64_t removable, size_t* first);
     // ActionState::trial (src/neural/action.cpp:308-344): the index buffer of
-    // the mesh with the legal actions `rows` applied.
-    std::vector<uint32_t> trial(const uint32_t* rows, size_t n);
+    // the mesh with the legal actions `rows` applied (and the kept faces'
+    // materials, when the mesh has materials).
+    std::vector<uint32_t> trial(const uint32_t* rows, size_t n, std::vector<uint16_t>* materials = nullptr);
     // Each legal action `rows[i]` as a patch of the source: the faces around
     // `from` (ascending) removed, their remapped survivors added.
     void patches(const uint32_t* rows, size_t n, std::vector<uint32_t>& removed_offsets,
                  std::vector<uint32_t>& removed, std::vector<uint32_t>& added_offsets,
                  std::vector<uint32_t>& added);
     double build_ms() const { return build_ms_; }
+    // max(1e-20, diameter of the input's bounds), the scale of the
+    // shortest-edge and current-plane scores (src/neural/action.cpp:29).
+    double diameter() const { return diameter_; }
     int device() const { return device_; }
 
   private:
     void claim(const uint32_t* rows, size_t n, uint32_t maximum, uint64_t removable, bool strict,
                std::vector<uint32_t>& picked);
+    uint32_t claim_device(const uint32_t* drows, size_t n, uint32_t maximum, uint64_t removable, bool strict,
+                          std::vector<uint32_t>& picked);
+    ActionTopo topo() const;
 
     int device_;
     hipStream_t stream_{};
-    uint32_t vertices_ = 0, faces_ = 0, edges_ = 0, legal_count_ = 0;
-    double build_ms_ = 0;
+    uint32_t vertices_ = 0, faces_ = 0, edges_ = 0, legal_count_ = 0, ranked_ = 0;
+    bool has_materials_ = false;
+    double build_ms_ = 0, diameter_ = 1;
     DeviceBuffer positions_, uv_, indices_, materials_;
     DeviceBuffer geometry_, invalid_, boundary_;
     DeviceBuffer offsets_, neighbors_, face_offsets_, face_lists_;
     DeviceBuffer edge_keys_, edge_counts_;
     DeviceBuffer legal_;  // LegalAction per legal action
     DeviceBuffer used_, selection_, scratch_a_, scratch_b_, scratch_c_, temp_;
+    DeviceBuffer order_, rank_keys_, rank_idx_, claim_out_;
 };
+
+// The gate of the executor: true when the candidate (index buffer, kept
+// materials) is accepted.
+using ActionGate = std::function<bool(const std::vector<uint32_t>&, const std::vector<uint16_t>&)>;
+struct ExecuteStats {
+    uint64_t ranked = 0, trials = 0, accepted = 0, rejected = 0;
+    uint32_t states = 0;  // action states built (one per accepted batch, plus the last)
+    double ms = 0;
+};
+// neural::execute_actions (src/neural/action.cpp:453-525) on the device
+// action state: rank (1 constant, 3 shortest edge, 4 current plane), take
+// the greedy batch of up to `batch` actions from the rank position, trial it,
+// ask the gate, commit or halve; a rejected singleton moves past the batch's
+// first action. Returns the final index buffer.
+std::vector<uint32_t> execute_actions(int device, const HostMesh& input, const float* uv, uint64_t target,
+                                      uint32_t budget, uint32_t ranking, uint32_t batch, const ActionGate& gate,
+                                      ExecuteStats& stats, std::vector<uint16_t>* materials_out = nullptr);
 
 }  // namespace blitz_hip

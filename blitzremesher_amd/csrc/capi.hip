@@ -520,6 +520,75 @@ blitz_status blitz_hip_actions_trial(blitz_hip_actions* a, const uint32_t* rows,
     });
 }
 
+blitz_status blitz_hip_actions_rank(blitz_hip_actions* a, uint8_t ranking, blitz_hip_scorer* scorer,
+                                    blitz_hip_graph* graph, uint32_t* order, size_t capacity, char* error,
+                                    size_t error_capacity) {
+    return guarded(error, error_capacity, [&] {
+        ActionSet& s = actions_of(a);
+        if (!order || capacity < s.legal_count()) throw std::invalid_argument("order capacity too small");
+        const Graph* g = nullptr;
+        if (ranking == 0) {
+            if (!scorer || !scorer->impl || !graph || !graph->graph || graph->owner != scorer)
+                throw std::invalid_argument("learned ranking needs a scorer and its graph");
+            if (scorer->impl->device() != s.device()) throw std::invalid_argument("scorer on another device");
+            DeviceGuard guard(scorer->impl->device());
+            hip_check(hipStreamSynchronize(scorer->impl->stream()), "scorer");
+            g = graph->graph.get();
+        }
+        s.rank(ranking, s.diameter(), g, order);
+    });
+}
+
+blitz_status blitz_hip_execute_actions(blitz_hip_audit* a, const blitz_mesh* input, const blitz_hip_bounds* b,
+                                       const blitz_hip_eval_settings* gate, uint64_t target, uint32_t budget,
+                                       uint8_t ranking, uint8_t batch, uint32_t* indices, size_t index_capacity,
+                                       size_t* index_count, blitz_hip_action_stats* stats, char* error,
+                                       size_t capacity) {
+    return guarded(error, capacity, [&] {
+        if (!indices || !index_count) throw std::invalid_argument("null output");
+        if (stats && stats->struct_size != sizeof(*stats)) throw std::invalid_argument("stats size mismatch");
+        Audit& audit = audit_of(a);
+        const Settings settings = settings_of(gate);
+        const Bounds bounds = bounds_from(b);
+        validate_settings(bounds, settings);
+        const HostMesh source = host_mesh(input);
+        std::vector<float> uv;
+        if (input->uv.count) {
+            const size_t n = source.positions.size() / 3;
+            uv.resize(2 * n);
+            const auto* base = static_cast<const unsigned char*>(input->uv.data);
+            for (size_t i = 0; i < n; ++i) std::memcpy(&uv[2 * i], base + i * input->uv.stride, 8);
+        }
+        auto reference = audit.upload(source);
+        uint64_t audits = 0;
+        // The gate: an exact audit of the candidate against the input.
+        const ActionGate accept = [&](const std::vector<uint32_t>& idx, const std::vector<uint16_t>& mats) {
+            blitz_mesh d = *input;
+            d.indices = idx.data();
+            d.index_count = idx.size();
+            d.materials = mats.empty() ? nullptr : mats.data();
+            d.material_count = mats.size();
+            auto candidate = audit.upload(host_mesh(&d));
+            ++audits;
+            return audit.evaluate(*reference, *candidate, bounds, settings).passed;
+        };
+        ExecuteStats st;
+        const auto out = execute_actions(audit.device(), source, uv.empty() ? nullptr : uv.data(), target, budget,
+                                         ranking, batch, accept, st);
+        if (out.size() > index_capacity) throw std::invalid_argument("index capacity too small");
+        std::memcpy(indices, out.data(), out.size() * 4);
+        *index_count = out.size();
+        if (stats) {
+            stats->ranked = st.ranked;
+            stats->trials = st.trials;
+            stats->accepted = st.accepted;
+            stats->rejected = st.rejected;
+            stats->states = st.states;
+            stats->ms = st.ms;
+        }
+    });
+}
+
 blitz_status blitz_hip_actions_patches(blitz_hip_actions* a, const uint32_t* rows, size_t n,
                                        uint32_t* removed_offsets, uint32_t* removed, size_t removed_capacity,
                                        uint32_t* added_offsets, uint32_t* added, size_t added_capacity,

@@ -261,6 +261,27 @@ blitz_status blitz_hip_actions_patches(blitz_hip_actions*, const uint32_t* rows,
                                        uint32_t* added_offsets, uint32_t* added, size_t added_capacity,
                                        size_t* removed_count, size_t* added_count, char* error, size_t capacity);
 
+/* Work counters of blitz::neural::ActionStats (src/neural/action.hpp:62-67). */
+typedef struct blitz_hip_action_stats {
+    uint32_t struct_size;
+    uint32_t states; /* action states built on the device */
+    uint64_t ranked, trials, accepted, rejected;
+    double ms;       /* wall time of the whole call */
+} blitz_hip_action_stats;
+/* blitz::neural::execute_actions (src/neural/action.cpp:453-525; the GPU
+ * executor of src/neural/action_gpu.cu, checked against it by
+ * tests/neural_action_gpu.cpp:140-190): collapse `input` towards `target`
+ * triangles within `budget` trials, ranking by `ranking` (1 constant,
+ * 3 shortest edge, 4 current plane), batches of up to `batch` (1..64)
+ * independent actions, each trial gated by an exact audit against `input`
+ * under `gate` (accepted when the Measurement passes). Writes the final index
+ * buffer. Needs the blitz_hip_audit of the device the actions run on. */
+blitz_status blitz_hip_execute_actions(blitz_hip_audit*, const blitz_mesh* input, const blitz_hip_bounds*,
+                                       const blitz_hip_eval_settings* gate, uint64_t target, uint32_t budget,
+                                       uint8_t ranking, uint8_t batch, uint32_t* indices, size_t index_capacity,
+                                       size_t* index_count, blitz_hip_action_stats* stats, char* error,
+                                       size_t capacity);
+
 /* blitz::detail::rasterize_coverage (src/coverage.hpp:12, src/evaluate.cpp:125-189)
  * for one camera: row-major packed words (index = y*size + x), padding bits
  * zero. *size_out receives the raster side; returns BLITZ_BUDGET_LIMITED when
@@ -320,6 +341,16 @@ blitz_status blitz_hip_reduce_reuse(const blitz_mesh*, size_t target_triangles, 
  * (src/neural/internal.hpp:42-43). */
 typedef struct blitz_hip_scorer blitz_hip_scorer;
 typedef struct blitz_hip_graph blitz_hip_graph;
+
+/* Rank order of the legal actions (rank_actions + SortPairsDescending,
+ * src/neural/action_gpu.cu:809-844,1793-1795): `ranking` takes
+ * blitz_neural_options_v2.ranking values, 0 learned (the logits of a scorer
+ * graph of the same mesh after blitz_hip_scorer_forward), 1 constant,
+ * 3 shortest edge, 4 current plane; 2 (shuffled) returns BLITZ_UNAVAILABLE.
+ * Ties keep ascending action order. */
+blitz_status blitz_hip_actions_rank(blitz_hip_actions*, uint8_t ranking, blitz_hip_scorer* scorer,
+                                    blitz_hip_graph* graph, uint32_t* order, size_t capacity, char* error,
+                                    size_t error_capacity);
 
 #define BLITZ_HIP_PRECISION_FP32 0u /* deterministic compensated FP32 */
 #define BLITZ_HIP_PRECISION_BF16 1u /* BF16 MFMA, FP32 accumulation */

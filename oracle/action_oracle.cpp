@@ -113,6 +113,40 @@ class Actions {
         return out;
     }
 
+    // Ranking scores of the legal actions (src/neural/action.cpp:355-370 and
+    // tests/neural_action_gpu.cpp:155-160): 1 constant (0), 3 shortest edge
+    // (-float(length((to - from) * (1 / diameter)))), 4 current plane
+    // (-float(teacher_cost)).
+    std::vector<float> scores(const std::vector<std::pair<uint32_t, uint32_t>>& rows, int mode, double diameter) const {
+        std::vector<float> out(rows.size(), 0.f);
+        const double scale = std::max(1e-20, diameter);
+        for (size_t i = 0; i < rows.size(); ++i) {
+            const auto [u, v] = rows[i];
+            if (mode == 3) {
+                const V3 d = sub(positions_[v], positions_[u]);
+                const double s = 1 / scale;
+                const V3 e{float(d.x * s), float(d.y * s), float(d.z * s)};
+                out[i] = -float(length(e));
+            } else if (mode == 4) {
+                double cost = 0;
+                for (uint32_t k = face_offsets_[u]; k < face_offsets_[u + 1]; ++k) {
+                    const uint32_t f = faces_[k];
+                    const V3 p = positions_[indices_[3 * f]], q = positions_[indices_[3 * f + 1]],
+                             r = positions_[indices_[3 * f + 2]];
+                    const V3 normal = cross(sub(q, p), sub(r, p));
+                    const double weight = length(normal);
+                    const double l = length(normal);
+                    const V3 unit = l > 0 ? V3{float(normal.x * (1 / l)), float(normal.y * (1 / l)), float(normal.z * (1 / l))}
+                                          : V3{0, 0, 0};
+                    const double distance = dot(unit, sub(positions_[v], p)) / scale;
+                    cost += distance * distance * weight / (scale * scale);
+                }
+                out[i] = float(-cost);
+            }
+        }
+        return out;
+    }
+
   private:
     struct Edge {
         uint32_t a, b, count;  // count of incident face corners, capped at 3
@@ -401,6 +435,12 @@ int64_t oracle_actions_independent(void* p, const uint32_t* order, uint64_t n, u
     } catch (...) {
         return -1;
     }
+}
+int oracle_actions_scores(void* p, int mode, double diameter, float* out) {
+    auto* h = static_cast<Handle*>(p);
+    const auto s = h->state->scores(h->rows, mode, diameter);
+    std::copy(s.begin(), s.end(), out);
+    return 0;
 }
 int64_t oracle_actions_trial(void* p, const uint32_t* rows, uint64_t n, uint32_t* indices, uint64_t capacity) {
     try {

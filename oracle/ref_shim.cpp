@@ -338,4 +338,52 @@ int64_t ref_actions_trial(void* h, const uint32_t* rows, uint64_t n, uint32_t* i
         return -1;
     }
 }
+
+// neural::execute_actions (src/neural/action.cpp:453-525) with the rankings of
+// tests/neural_action_gpu.cpp:155-160 (1 constant, 3 shortest edge, 4 current
+// plane) and, as the gate, blitz::evaluate of the candidate against the input
+// under `s` passing. Writes the result's index buffer and the work counters
+// ranked, trials, accepted, rejected.
+int64_t ref_execute_actions(const float* pos, uint64_t nv, const uint32_t* idx, uint64_t ni, const uint8_t* ds,
+                            uint64_t nds, uint64_t target, uint32_t budget, int ranking, uint32_t batch,
+                            const float* center, double radius, const OracleSettings* s, uint32_t* out,
+                            uint64_t capacity, uint64_t* stats) {
+    try {
+        MeshView input = view_of(pos, nv, idx, ni, nullptr, 0, ds, nds);
+        EvalSettings e;
+        e.profile = Profile(s->profile);
+        e.screen_size = s->screen_size;
+        e.limit = s->limit;
+        e.max_changed_area = s->max_changed_area;
+        e.views = {s->orthographic, s->perspective, s->rotation_seed};
+        e.supersample = s->supersample;
+        e.max_supersample = s->max_supersample;
+        e.force_two_sided = s->force_two_sided;
+        e.conservative_screen = s->conservative_screen;
+        e.weights = {s->normal_weight, s->color_weight, s->material_weight};
+        e.force_scalar = true;
+        const Bounds b{{center[0], center[1], center[2]}, radius};
+        auto rank = [&](const neural::ActionState& state, std::span<const neural::ActionRecord> rows) {
+            std::vector<float> scores(rows.size());
+            for (size_t i = 0; i < rows.size(); ++i)
+                if (ranking == 3)
+                    scores[i] = -rows[i].x[59];
+                else if (ranking == 4)
+                    scores[i] = float(-state.teacher_cost(rows[i].action));
+            return scores;
+        };
+        auto gate = [&](MeshView candidate) { return evaluate(input, candidate, b, e).passed; };
+        neural::ActionStats st;
+        const Lod lod = neural::execute_actions(input, {}, size_t(target), budget, rank, gate, &st, {}, batch);
+        if (lod.data.indices.size() > capacity) return -1;
+        std::memcpy(out, lod.data.indices.data(), lod.data.indices.size() * 4);
+        stats[0] = st.ranked;
+        stats[1] = st.trials;
+        stats[2] = st.accepted;
+        stats[3] = st.rejected;
+        return int64_t(lod.data.indices.size());
+    } catch (...) {
+        return -1;
+    }
+}
 }

@@ -107,3 +107,21 @@ def action_cases() -> dict:
         "degenerate": (degenerate_grid(), None),
         "signed_zero": (signed_zero_grid(), None),
     }
+
+
+def executor_cases():
+    """name, mesh, gate EvalSettings, target, budget for execute_actions."""
+    from blitzremesher_amd import EvalSettings
+    gate = EvalSettings(profile=0, screen_size=16, limit=2.5, supersample=2, max_supersample=4, orthographic=2,
+                        perspective=1, rotation_seed=819)
+    tight = EvalSettings(profile=0, screen_size=24, limit=1.0, supersample=4, max_supersample=8, orthographic=4,
+                         perspective=2, rotation_seed=0xB1172026)
+    ico = meshes.icosphere(2, 0.02, 1)
+    hf = meshes.heightfield(12, 0.2, 2)
+    return [
+        ("grid", meshes.grid(5), gate, 16, 40),
+        ("icosphere", ico, gate, ico.triangles // 3, 40),
+        ("heightfield", hf, gate, hf.triangles // 3, 40),
+        ("torus_tight", meshes.torus(1.0, 0.3, 12, 6, 0.01, 3), tight, 20, 60),
+        ("tiles", meshes.four_tiles(), gate, 1, 40),
+    ]

@@ -21,8 +21,8 @@ HERE = Path(__file__).resolve().parent
 sys.path.insert(0, str(HERE.parent.parent))
 sys.path.insert(0, str(HERE.parent))
 
-from action_meshes import action_cases  # noqa: E402
-from oracle_lib import Actions  # noqa: E402
+from action_meshes import action_cases, executor_cases  # noqa: E402
+from oracle_lib import Actions, Oracle, ref_execute_actions  # noqa: E402
 
 BATCHES = ((64, 10 ** 9, 11), (5, 10 ** 9, 12), (64, 7, 13), (1, 10 ** 9, 14))
 
@@ -52,11 +52,27 @@ def main() -> None:
             entry["batches"].append(b)
         cases.append(entry)
         ref.close()
+    # neural::execute_actions with the reference CPU audit as the gate.
+    executions = []
+    ref = Oracle("ref")
+    for name, mesh, gate, target, budget in executor_cases():
+        arrays[f"exec_{name}/positions"] = mesh.positions
+        arrays[f"exec_{name}/indices"] = mesh.indices
+        if mesh.double_sided is not None:
+            arrays[f"exec_{name}/double_sided"] = mesh.double_sided
+        center, radius = ref.bounds(mesh)
+        for ranking in (1, 3, 4):
+            for batch in (1, 4, 16):
+                out, stats = ref_execute_actions(mesh, center, radius, gate, target, budget, ranking, batch)
+                executions.append({"mesh": f"exec_{name}", "center": [float(x) for x in center], "radius": radius,
+                                   "gate": {k: v for k, v in gate.__dict__.items() if k != "cancelled"},
+                                   "target": target, "budget": budget, "ranking": ranking, "batch": batch,
+                                   "faces": int(len(out)), "indices_sha256": sha(out), "stats": stats})
     np.savez_compressed(HERE / "action_meshes.npz", **arrays)
     (HERE / "action_golden.json").write_text(json.dumps(
-        {"source": "reference ActionState (src/neural/action.cpp) built by oracle/Makefile", "cases": cases},
-        indent=1))
-    print(f"wrote {len(cases)} action cases")
+        {"source": "reference ActionState and execute_actions (src/neural/action.cpp) built by oracle/Makefile",
+         "cases": cases, "executions": executions}, indent=1))
+    print(f"wrote {len(cases)} action cases, {len(executions)} executions")
 
 
 if __name__ == "__main__":

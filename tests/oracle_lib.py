@@ -324,6 +324,18 @@ class Actions:
             raise ValueError("independent rejected its input")
         return out[:n].copy()
 
+    def scores(self, ranking: int, diameter: float) -> np.ndarray:
+        assert self.oracle.kind == "oracle"
+        out = np.zeros(max(1, len(self.legal)), np.float32)
+        f = self.oracle.fn("actions_scores")
+        f.argtypes = [C.c_void_p, C.c_int, C.c_double, C.POINTER(C.c_float)]
+        f(self.handle, ranking, diameter, out.ctypes.data_as(C.POINTER(C.c_float)))
+        return out[: len(self.legal)]
+
+    def rank(self, ranking: int, diameter: float) -> np.ndarray:
+        """Stable descending order of the scores (-0 equal to +0)."""
+        return np.argsort(-self.scores(ranking, diameter), kind="stable").astype(np.uint32)
+
     def trial(self, rows: np.ndarray) -> np.ndarray:
         rows = np.ascontiguousarray(rows, np.uint32)
         out = np.zeros(self.mesh.indices.size, np.uint32)
@@ -335,3 +347,81 @@ class Actions:
         if n < 0:
             raise ValueError("trial rejected its batch")
         return out[:n].reshape(-1, 3).copy()
+
+
+def execute_actions(mesh: Mesh, target: int, budget: int, ranking: int, batch: int, gate, uv=None):
+    """Restatement of neural::execute_actions (src/neural/action.cpp:453-525)
+    on the oracle's action state: rank, take the greedy batch from the rank
+    position, trial it, gate, commit or halve; a rejected singleton moves past
+    the batch's first action. `gate(indices) -> bool`. Returns the index buffer
+    and the ranked / trials / accepted / rejected counters. Meshes without
+    materials."""
+    assert mesh.materials is None
+    stats = {"ranked": 0, "trials": 0, "accepted": 0, "rejected": 0}
+    o = Oracle("oracle")
+    _, radius = o.bounds(mesh)
+    diameter = max(1e-20, 2 * radius)
+    current = mesh.indices.copy()
+    target = max(1, target)
+    while len(current) > target and stats["trials"] < budget:
+        state = Actions("oracle", Mesh(mesh.positions, current), uv)
+        if not len(state.legal):
+            break
+        order = state.rank(ranking, diameter)
+        stats["ranked"] += len(state.legal)
+        accepted, position = False, 0
+        while position < len(order) and stats["trials"] < budget and not accepted:
+            picked = state.independent(order[position:], batch, len(current) - target)
+            if not len(picked):
+                break
+            first = picked[0]
+            b = list(picked)
+            while True:
+                if stats["trials"] == budget:
+                    break
+                candidate = state.trial(np.array(b, np.uint32))
+                stats["trials"] += 1
+                if gate(candidate):
+                    current = candidate
+                    stats["accepted"] += len(b)
+                    accepted = True
+                    break
+                stats["rejected"] += len(b)
+                if len(b) == 1:
+                    break
+                b = b[: len(b) // 2]
+            while position < len(order) and order[position] != first:
+                position += 1
+            position += 1
+        state.close()
+        if not accepted:
+            break
+    return current, stats
+
+
+def ref_execute_actions(mesh: Mesh, center, radius, s: EvalSettings, target: int, budget: int, ranking: int,
+                        batch: int):
+    """The reference build's own neural::execute_actions with its CPU audit as
+    the gate (oracle/ref_shim.cpp). Meshes without materials."""
+    assert mesh.materials is None
+    r = Oracle("ref")
+    f = r.lib.ref_execute_actions
+    f.restype = C.c_int64
+    f.argtypes = [C.POINTER(C.c_float), C.c_uint64, C.POINTER(C.c_uint32), C.c_uint64, C.POINTER(C.c_uint8),
+                  C.c_uint64, C.c_uint64, C.c_uint32, C.c_int, C.c_uint32, C.POINTER(C.c_float), C.c_double,
+                  C.POINTER(OracleSettings), C.POINTER(C.c_uint32), C.c_uint64, C.POINTER(C.c_uint64)]
+    o = OracleSettings(s.screen_size, s.limit, s.max_changed_area, s.rotation_seed & 0xFFFFFFFF, s.orthographic,
+                       s.perspective, s.supersample, s.max_supersample, int(s.force_two_sided),
+                       int(s.conservative_screen), s.profile, s.normal_weight, s.color_weight, s.material_weight)
+    out = np.zeros(mesh.indices.size, np.uint32)
+    stats = np.zeros(4, np.uint64)
+    cc = (C.c_float * 3)(*[float(x) for x in center])
+    ds = mesh.double_sided
+    n = f(_ptr(mesh.positions, C.c_float), mesh.positions.shape[0], _ptr(mesh.indices, C.c_uint32),
+          int(mesh.indices.size), _ptr(ds, C.c_uint8), 0 if ds is None else int(ds.size), target, budget, ranking,
+          batch, cc, radius, C.byref(o),
+          out.ctypes.data_as(C.POINTER(C.c_uint32)), out.size, stats.ctypes.data_as(C.POINTER(C.c_uint64)))
+    if n < 0:
+        raise ValueError("reference execute_actions failed")
+    return out[:n].reshape(-1, 3).copy(), dict(zip(("ranked", "trials", "accepted", "rejected"),
+                                                   (int(x) for x in stats)))

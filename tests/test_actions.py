@@ -7,8 +7,12 @@ reference's own ActionState and, where that build is present, the build
 itself; and it keeps what a legal collapse must keep: a closed manifold stays
 closed and manifold with the same Euler characteristic after any batch, each
 interior collapse removes two faces and each boundary collapse one, and a
-tetrahedron has no legal collapse. GPU: device topology, legality, selection
-and trials equal the golden fixtures and the oracle exactly.
+tetrahedron has no legal collapse. The oracle's restatement of
+execute_actions (rank, greedy batch, trial, gate, commit or halve) equals the
+reference's own loop with its CPU audit as the gate, output and work counters.
+GPU: device topology, legality, ranking, selection, trials and the executor
+(with the HIP audit as the gate) equal the golden fixtures and the oracle
+exactly.
 """
 from __future__ import annotations
 
@@ -20,9 +24,9 @@ from pathlib import Path
 import numpy as np
 import pytest
 
-from blitzremesher_amd import Mesh, meshes
-from action_meshes import action_cases
-from oracle_lib import Actions, available
+from blitzremesher_amd import EvalSettings, Mesh, meshes
+from action_meshes import action_cases, executor_cases
+from oracle_lib import Actions, Oracle, available, execute_actions, ref_execute_actions
 
 GOLDEN = Path(__file__).resolve().parent / "golden"
 needs_ref = pytest.mark.skipif(not available("ref"), reason="reference CPU build (oracle/_ref) not present")
@@ -42,9 +46,9 @@ def golden_meshes():
     data = np.load(GOLDEN / "action_meshes.npz")
     out = {}
     for n in sorted({k.split("/")[0] for k in data.files}):
-        mats = data[f"{n}/materials"] if f"{n}/materials" in data.files else None
-        uv = data[f"{n}/uv"] if f"{n}/uv" in data.files else None
-        out[n] = (Mesh(data[f"{n}/positions"], data[f"{n}/indices"], mats), uv)
+        opt = {k: data[f"{n}/{k}"] if f"{n}/{k}" in data.files else None for k in ("materials", "double_sided", "uv")}
+        out[n] = (Mesh(data[f"{n}/positions"], data[f"{n}/indices"], opt["materials"], opt["double_sided"]),
+                  opt["uv"])
     return out
 
 
@@ -84,6 +88,38 @@ def test_oracle_matches_reference_build(seed):
             assert np.array_equal(s1, s2), (name, maximum, budget)
             if len(s1):
                 assert np.array_equal(a.trial(s1), r.trial(s2)), (name, maximum, budget)
+
+
+def oracle_gate(mesh, center, radius, settings):
+    o = Oracle("oracle")
+    return lambda idx: o.evaluate(mesh, Mesh(mesh.positions, idx, double_sided=mesh.double_sided), center, radius,
+                                  settings).passed
+
+
+def test_oracle_executor_matches_golden():
+    ms = golden_meshes()
+    runs = golden()["executions"]
+    assert len(runs) >= 40 and any(r["stats"]["rejected"] for r in runs)
+    for r in runs:
+        mesh, _ = ms[r["mesh"]]
+        gate = EvalSettings(**r["gate"])
+        out, stats = execute_actions(mesh, r["target"], r["budget"], r["ranking"], r["batch"],
+                                     oracle_gate(mesh, r["center"], r["radius"], gate))
+        assert (len(out), sha(out), stats) == (r["faces"], r["indices_sha256"], r["stats"]), r
+
+
+@needs_ref
+def test_oracle_executor_matches_reference_build():
+    gate = EvalSettings(profile=0, screen_size=20, limit=1.5, supersample=2, max_supersample=8, orthographic=3,
+                        perspective=2, rotation_seed=77)
+    o = Oracle("oracle")
+    for mesh in (meshes.icosphere(2, 0.05, 9), meshes.jitter(meshes.heightfield(10, 0.4, 9), 0.01, 9, 0.5),
+                 meshes.torus(1.0, 0.3, 10, 6, 0.03, 9)):
+        center, radius = o.bounds(mesh)
+        for ranking, batch in ((1, 2), (3, 8), (4, 64)):
+            a = execute_actions(mesh, mesh.triangles // 4, 30, ranking, batch, oracle_gate(mesh, center, radius, gate))
+            b = ref_execute_actions(mesh, center, radius, gate, mesh.triangles // 4, 30, ranking, batch)
+            assert np.array_equal(a[0], b[0]) and a[1] == b[1], (ranking, batch, a[1], b[1])
 
 
 def edge_multiplicity(indices: np.ndarray, geometry: np.ndarray) -> np.ndarray:
@@ -220,3 +256,65 @@ def test_gpu_action_patches_audit_like_full_candidates():
             assert got[i] == audit.evaluate(ref, cand, center, radius, s), (i, r)
             cand.release()
         assert any(not m.passed for m in got) and any(m.passed for m in got)
+
+
+@pytest.mark.gpu
+def test_gpu_rank_matches_oracle():
+    from blitzremesher_amd.actions import ActionSet
+    o = Oracle("oracle")
+    for name in ("icosphere", "heightfield", "folded", "seam", "torus"):
+        mesh, uv = action_cases()[name]
+        _, radius = o.bounds(mesh)
+        cpu = Actions("oracle", mesh, uv)
+        with ActionSet(mesh, uv) as gpu:
+            for ranking in (1, 3, 4):
+                assert np.array_equal(gpu.rank(ranking), cpu.rank(ranking, max(1e-20, 2 * radius))), (name, ranking)
+
+
+@pytest.mark.gpu
+def test_gpu_learned_rank_orders_scorer_logits():
+    from blitzremesher_amd.actions import ActionSet
+    from blitzremesher_amd.scorer import BF16, Scorer, he_weights
+    mesh = meshes.torus(1.0, 0.3, 40, 16, 0.01, 3)
+    o = Oracle("oracle")
+    center, radius = o.bounds(mesh)
+    with Scorer(he_weights(4)) as scorer, ActionSet(mesh) as gpu:
+        g = scorer.graph(mesh, center, radius)
+        logits = scorer.forward(g, BF16)[:, 0]
+        pairs = g.actions_array()
+        key = {(int(a), int(b)): i for i, (a, b) in enumerate(pairs)}
+        scores = np.array([logits[key[(int(u), int(v))]] for u, v in gpu.legal], np.float32)
+        want = np.argsort(-(scores + np.float32(0)), kind="stable").astype(np.uint32)
+        assert np.array_equal(gpu.rank(0, scorer, g), want)
+
+
+@pytest.mark.gpu
+def test_gpu_executor_matches_golden():
+    from blitzremesher_amd import Audit
+    from blitzremesher_amd.actions import execute_actions as gpu_execute
+    ms = golden_meshes()
+    with Audit(0) as audit:
+        for r in golden()["executions"]:
+            mesh, _ = ms[r["mesh"]]
+            out, stats = gpu_execute(audit, mesh, r["center"], r["radius"], EvalSettings(**r["gate"]), r["target"],
+                                     r["budget"], r["ranking"], r["batch"])
+            got = {k: stats[k] for k in ("ranked", "trials", "accepted", "rejected")}
+            assert (len(out), sha(out), got) == (r["faces"], r["indices_sha256"], r["stats"]), (r, got)
+
+
+@pytest.mark.gpu
+def test_gpu_executor_matches_oracle():
+    from blitzremesher_amd import Audit
+    from blitzremesher_amd.actions import execute_actions as gpu_execute
+    gate = EvalSettings(profile=0, screen_size=32, limit=1.25, supersample=2, max_supersample=8, orthographic=4,
+                        perspective=2, rotation_seed=5)
+    o = Oracle("oracle")
+    with Audit(0) as audit:
+        for mesh in (meshes.icosphere(3, 0.03, 11), meshes.jitter(meshes.heightfield(16, 0.3, 11), 0.01, 11, 0.5)):
+            center, radius = o.bounds(mesh)
+            for ranking, batch in ((3, 16), (4, 64), (1, 4)):
+                want = execute_actions(mesh, mesh.triangles // 3, 50, ranking, batch,
+                                       oracle_gate(mesh, center, radius, gate))
+                out, stats = gpu_execute(audit, mesh, center, radius, gate, mesh.triangles // 3, 50, ranking, batch)
+                got = {k: stats[k] for k in ("ranked", "trials", "accepted", "rejected")}
+                assert np.array_equal(out, want[0]) and got == want[1], (ranking, batch, got, want[1])
