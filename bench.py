@@ -174,6 +174,53 @@ def patch_leg(ref, center, radius, settings, count: int, repeats: int) -> dict:
             "generate_s": round(generate_s, 2)}
 
 
+def executor_leg(budget: int, cpu_budget: int) -> dict:
+    """The plain classical method's audited collapses on the CPU-runnable
+    icosphere (10,242 V / 20,480 F, radial noise 0.02): execute_actions with
+    the current-plane order, batches of 64 independent collapses, each trial
+    gated by an exact coverage audit against the source from 16 Fibonacci
+    views at 256 px (ss 8, refinement to 16, limit 1 px), towards half the
+    triangles. The reference's own CPU loop (oracle/_ref build, one thread)
+    runs the first `cpu_budget` trials of the same run; the GPU repeats those
+    trials for a parity check of output and work counters."""
+    from blitzremesher_amd import Audit, EvalSettings, meshes
+    from blitzremesher_amd.actions import execute_actions
+    from blitzremesher_amd.audit import bounds
+    mesh = meshes.icosphere(5, 0.02, 1)
+    center, radius = bounds(mesh)
+    gate = EvalSettings(profile=0, screen_size=256, supersample=8, max_supersample=16, orthographic=16,
+                        perspective=0, rotation_seed=0xB1172026, limit=1.0)
+    target = mesh.triangles // 2
+    with Audit(0) as audit:
+        execute_actions(audit, mesh, center, radius, gate, target, 2, 4, 64)  # warm-up
+        out, st = execute_actions(audit, mesh, center, radius, gate, target, budget, 4, 64)
+        sample_out, sample_st = execute_actions(audit, mesh, center, radius, gate, target, cpu_budget, 4, 64)
+    seconds = st["ms"] / 1000.0
+    leg = {"workload": "execute_actions (current-plane order, batch 64) on the 10,242-vertex icosphere towards "
+                       f"{target} triangles, each trial gated by a coverage audit (16 views, S=256, ss 8->16, "
+                       "limit 1 px) against the source",
+           "budget": budget, "trials": st["trials"], "accepted": st["accepted"], "rejected": st["rejected"],
+           "ranked": st["ranked"], "states": st["states"], "triangles": int(len(out)), "ms": round(st["ms"], 2),
+           "ms_per_trial": round(st["ms"] / max(1, st["trials"]), 3),
+           "audited_collapses_per_s": round(st["accepted"] / seconds, 1) if seconds else None}
+    try:
+        sys.path.insert(0, str(ROOT / "tests"))
+        from oracle_lib import available, ref_execute_actions
+        if cpu_budget and available("ref"):
+            t0 = time.perf_counter()
+            cpu_out, cpu_st = ref_execute_actions(mesh, center, radius, gate, target, cpu_budget, 4, 64)
+            dt = time.perf_counter() - t0
+            same = (len(cpu_out) == len(sample_out) and (cpu_out == sample_out).all() and
+                    all(cpu_st[k] == sample_st[k] for k in ("ranked", "trials", "accepted", "rejected")))
+            leg["cpu_reference"] = {"trials": cpu_st["trials"], "accepted": cpu_st["accepted"], "s": round(dt, 2),
+                                    "audited_collapses_per_s": round(cpu_st["accepted"] / dt, 1), "cores": 1,
+                                    "gpu_same_trials_ms": round(sample_st["ms"], 2),
+                                    "identical_output_and_counters": bool(same)}
+    except Exception as e:  # the CPU comparison is optional
+        leg["cpu_reference"] = {"error": str(e)}
+    return leg
+
+
 def copy_probe(mib: int) -> int:
     """Runs the counter-calibration copy kernel (see profiles/) and prints its
     bandwidth."""
@@ -212,6 +259,9 @@ def main() -> int:
     ap.add_argument("--no-scorer", action="store_true")
     ap.add_argument("--no-patches", action="store_true")
     ap.add_argument("--patches", type=int, default=4096)
+    ap.add_argument("--no-executor", action="store_true")
+    ap.add_argument("--executor-budget", type=int, default=400)
+    ap.add_argument("--executor-cpu-budget", type=int, default=3)
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run the counter-calibration copy of this many MiB")
     args = ap.parse_args()
@@ -281,6 +331,8 @@ def main() -> int:
     scorer = None if args.no_scorer else scorer_leg(max(3, min(args.steps, 10)))
     patches = None if args.no_patches else patch_leg(ref, center, radius, settings, args.patches,
                                                        max(3, min(args.steps, 10)))
+    executor = None if args.no_executor else executor_leg(args.executor_budget,
+                                                          0 if args.no_cpu else args.executor_cpu_budget)
 
     line = {
         "metric": METRIC,
@@ -309,6 +361,7 @@ def main() -> int:
         "cpu_baseline": cpu,
         "parity_on_cpu_sample": parity,
         "patches": patches,
+        "executor": executor,
         "scorer": scorer,
     }
     print(json.dumps(line), flush=True)
