@@ -202,6 +202,7 @@ def executor_leg(budget: int, cpu_budget: int) -> dict:
            "budget": budget, "trials": st["trials"], "accepted": st["accepted"], "rejected": st["rejected"],
            "ranked": st["ranked"], "states": st["states"], "triangles": int(len(out)), "ms": round(st["ms"], 2),
            "ms_per_trial": round(st["ms"] / max(1, st["trials"]), 3),
+           "stage_ms": {k: round(st[f"{k}_ms"], 2) for k in ("build", "rank", "select", "trial", "gate")},
            "audited_collapses_per_s": round(st["accepted"] / seconds, 1) if seconds else None}
     try:
         sys.path.insert(0, str(ROOT / "tests"))

@@ -110,7 +110,8 @@ class AuditTiming(C.Structure):
 
 class ActionStats(C.Structure):
     _fields_ = [("struct_size", C.c_uint32), ("states", C.c_uint32), ("ranked", C.c_uint64), ("trials", C.c_uint64),
-                ("accepted", C.c_uint64), ("rejected", C.c_uint64), ("ms", C.c_double)]
+                ("accepted", C.c_uint64), ("rejected", C.c_uint64), ("ms", C.c_double), ("build_ms", C.c_double),
+                ("rank_ms", C.c_double), ("select_ms", C.c_double), ("trial_ms", C.c_double), ("gate_ms", C.c_double)]
 
 
 _P = C.c_void_p

@@ -955,25 +955,39 @@ std::vector<uint32_t> execute_actions(int device, const HostMesh& input, const f
     const double diameter = std::max(1e-20, bounds_of(input.positions.data(), input.positions.size() / 3).radius * 2);
     HostMesh state = input;
     target = std::max<uint64_t>(1, target);
+    auto since = [](std::chrono::steady_clock::time_point t) {
+        return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t).count();
+    };
     while (state.indices.size() / 3 > target && stats.trials < budget) {
+        auto t = std::chrono::steady_clock::now();
         ActionSet set(device, state, uv);
+        stats.build_ms += since(t);
         ++stats.states;
         const size_t n = set.legal_count();
         if (!n) break;
+        t = std::chrono::steady_clock::now();
         set.rank(ranking, diameter, nullptr, nullptr);
+        stats.rank_ms += since(t);
         stats.ranked += n;
         const uint64_t faces = state.indices.size() / 3;
         bool accepted = false;
         for (size_t position = 0; position < n && stats.trials < budget && !accepted;) {
             size_t first = 0;
+            t = std::chrono::steady_clock::now();
             std::vector<uint32_t> picked = set.independent_ranked(position, batch, faces - target, &first);
+            stats.select_ms += since(t);
             if (picked.empty()) break;
             for (;;) {
                 if (stats.trials == budget) break;
                 std::vector<uint16_t> materials;
+                t = std::chrono::steady_clock::now();
                 std::vector<uint32_t> candidate = set.trial(picked.data(), picked.size(), &materials);
+                stats.trial_ms += since(t);
                 ++stats.trials;
-                if (gate(candidate, materials)) {
+                t = std::chrono::steady_clock::now();
+                const bool pass = gate(candidate, materials);
+                stats.gate_ms += since(t);
+                if (pass) {
                     state.indices = std::move(candidate);
                     state.materials = std::move(materials);
                     stats.accepted += picked.size();

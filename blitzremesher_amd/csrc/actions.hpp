@@ -108,6 +108,9 @@ struct ExecuteStats {
     uint64_t ranked = 0, trials = 0, accepted = 0, rejected = 0;
     uint32_t states = 0;  // action states built (one per accepted batch, plus the last)
     double ms = 0;
+    // Wall time of each stage: state builds, ranking, batch selection,
+    // trials, gate calls.
+    double build_ms = 0, rank_ms = 0, select_ms = 0, trial_ms = 0, gate_ms = 0;
 };
 // neural::execute_actions (src/neural/action.cpp:453-525) on the device
 // action state: rank (1 constant, 3 shortest edge, 4 current plane), take

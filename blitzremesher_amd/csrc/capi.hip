@@ -585,6 +585,11 @@ blitz_status blitz_hip_execute_actions(blitz_hip_audit* a, const blitz_mesh* inp
             stats->rejected = st.rejected;
             stats->states = st.states;
             stats->ms = st.ms;
+            stats->build_ms = st.build_ms;
+            stats->rank_ms = st.rank_ms;
+            stats->select_ms = st.select_ms;
+            stats->trial_ms = st.trial_ms;
+            stats->gate_ms = st.gate_ms;
         }
     });
 }

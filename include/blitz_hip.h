@@ -267,6 +267,9 @@ typedef struct blitz_hip_action_stats {
     uint32_t states; /* action states built on the device */
     uint64_t ranked, trials, accepted, rejected;
     double ms;       /* wall time of the whole call */
+    /* wall time of the stages: state builds, ranking, batch selection,
+     * trials, gate audits */
+    double build_ms, rank_ms, select_ms, trial_ms, gate_ms;
 } blitz_hip_action_stats;
 /* blitz::neural::execute_actions (src/neural/action.cpp:453-525; the GPU
  * executor of src/neural/action_gpu.cu, checked against it by
