@@ -191,7 +191,9 @@ def executor_leg(budget: int, cpu_budget: int) -> dict:
     gate = EvalSettings(profile=0, screen_size=256, supersample=8, max_supersample=16, orthographic=16,
                         perspective=0, rotation_seed=0xB1172026, limit=1.0)
     target = mesh.triangles // 2
-    with Audit(0) as audit:
+    # Every gate audits against the same uploaded source: its masks are kept
+    # across trials (cache_rasters), as the reference's RasterMemo keeps them.
+    with Audit(0, cache_rasters=True) as audit:
         execute_actions(audit, mesh, center, radius, gate, target, 2, 4, 64)  # warm-up
         out, st = execute_actions(audit, mesh, center, radius, gate, target, budget, 4, 64)
         sample_out, sample_st = execute_actions(audit, mesh, center, radius, gate, target, cpu_budget, 4, 64)

@@ -278,13 +278,14 @@ class Binding:
 class Audit:
     """One HIP audit object (AuditCuda counterpart) on one device."""
 
-    def __init__(self, device: int = 0, memory_mib: int = 0):
+    def __init__(self, device: int = 0, memory_mib: int = 0, cache_rasters: bool = False):
         lib = _lib.load()
         self.lib = lib
         opts = _lib.AuditOptions()
         _lib.check(lib.blitz_hip_audit_options_init(C.byref(opts), C.sizeof(opts)))
         opts.device = device
         opts.memory_mib = memory_mib
+        opts.cache_rasters = int(cache_rasters)
         handle = C.c_void_p()
         err = _lib.error_buffer()
         _lib.check(lib.blitz_hip_audit_create(C.byref(opts), C.byref(handle), err, len(err)), err)

@@ -130,14 +130,40 @@ DeviceMesh::~DeviceMesh() {
     (void)hipFree(materials);
 }
 
-Audit::Audit(int device, uint64_t memory_bytes) : device_(device), memory_bytes_(memory_bytes) {
+Audit::Audit(int device, uint64_t memory_bytes, bool cache_rasters)
+    : device_(device), memory_bytes_(memory_bytes), cache_rasters_(cache_rasters) {
     DeviceGuard guard(device_);
     hip_check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
     for (auto& e : events_) hip_check(hipEventCreate(&e), "hipEventCreate");
 }
 
+void Audit::cache_clear() {
+    if (!cache_.masks.empty()) (void)hipStreamSynchronize(stream_);
+    for (auto& kv : cache_.masks) (void)hipFree(kv.second.mask);
+    cache_.masks.clear();
+    cache_.bytes = 0;
+}
+
+void Audit::cache_context(const DeviceMesh& reference, const Bounds& b, const Settings& s) {
+    RasterCache& c = cache_;
+    const bool same = c.serial == reference.serial && c.bounds.center.x == b.center.x &&
+                      c.bounds.center.y == b.center.y && c.bounds.center.z == b.center.z &&
+                      c.bounds.radius == b.radius && c.screen == s.screen_size && c.ortho == s.orthographic &&
+                      c.persp == s.perspective && c.seed == s.rotation_seed && c.two == s.force_two_sided;
+    if (same) return;
+    cache_clear();
+    c.serial = reference.serial;
+    c.bounds = b;
+    c.screen = s.screen_size;
+    c.ortho = s.orthographic;
+    c.persp = s.perspective;
+    c.seed = s.rotation_seed;
+    c.two = s.force_two_sided;
+}
+
 Audit::~Audit() {
     DeviceGuard guard(device_);
+    cache_clear();
     (void)hipStreamSynchronize(stream_);
     for (auto& s : slots_) (void)hipFree(s.first);
     for (auto& e : events_) (void)hipEventDestroy(e);
@@ -543,16 +569,28 @@ void Audit::run_items(const DeviceMesh& reference, const std::vector<const Devic
         uint16_t* dist = scratch_as<uint16_t>(2, size_t(ncand) * samples * 2);
         ViewSummary* summaries = scratch_as<ViewSummary>(3, ncand);
         ViewPair* dpairs = scratch_as<ViewPair>(5, ncand);
+        // Reference masks kept from earlier audits of the same reference
+        // (coverage rule only) are not drawn again.
+        const bool cached = cache_rasters_ && !attributed;
+        if (cached) cache_context(reference, b, s);
+        std::vector<const CachedMask*> hit(nref, nullptr);
+        if (cached)
+            for (auto& kv : ref_slot) {
+                auto it = cache_.masks.find({ss, kv.first});
+                if (it != cache_.masks.end()) hit[kv.second] = &it->second;
+            }
         // Jobs in view order, each view's reference first: consecutive raster
         // tasks then share one projection (and one XCD's L2).
         std::vector<RasterJob> jobs;
-        std::vector<uint32_t> ref_job(nref), cand_job(ncand);
+        std::vector<uint32_t> ref_job(nref, kNone), cand_job(ncand);
         {
             std::vector<std::vector<uint32_t>> by_view(nref);
             for (uint32_t i = 0; i < ncand; ++i) by_view[ref_slot[all[begin + i].view]].push_back(i);
             for (auto& kv : ref_slot) {
-                ref_job[kv.second] = uint32_t(jobs.size());
-                jobs.push_back({&reference, kv.first, ref_masks + size_t(kv.second) * words});
+                if (!hit[kv.second]) {
+                    ref_job[kv.second] = uint32_t(jobs.size());
+                    jobs.push_back({&reference, kv.first, ref_masks + size_t(kv.second) * words});
+                }
                 for (uint32_t i : by_view[kv.second]) {
                     cand_job[i] = uint32_t(jobs.size());
                     jobs.push_back({candidates[all[begin + i].candidate], kv.first, cand_masks + size_t(i) * words});
@@ -563,7 +601,8 @@ void Audit::run_items(const DeviceMesh& reference, const std::vector<const Devic
         for (uint32_t i = 0; i < ncand; ++i) {
             const Item& it = all[begin + i];
             ViewPair& p = pairs[i];
-            p.a = ref_masks + size_t(ref_slot[it.view]) * words;
+            const uint32_t slot = ref_slot[it.view];
+            p.a = hit[slot] ? hit[slot]->mask : ref_masks + size_t(slot) * words;
             p.c = cand_masks + size_t(i) * words;
             p.dist_a = dist + size_t(i) * samples * 2;
             p.dist_c = p.dist_a + samples;
@@ -578,6 +617,24 @@ void Audit::run_items(const DeviceMesh& reference, const std::vector<const Devic
         std::vector<const double*> zdepth;
         rasterize(jobs, cams, b, ss, size, s.force_two_sided, st, attributed ? &drawn : nullptr,
                   attributed ? &zdepth : nullptr);
+        // Reference clip flags, and new reference masks into the cache while
+        // it stays within a quarter of the memory budget.
+        std::vector<uint8_t> ref_clipped(nref, 0);
+        for (uint32_t r = 0; r < nref; ++r)
+            ref_clipped[r] = hit[r] ? hit[r]->clipped
+                                    : (st[2 * ref_job[r]] != kNone || st[2 * ref_job[r] + 1] != 0);
+        if (cached)
+            for (auto& kv : ref_slot) {
+                const uint32_t r = kv.second;
+                if (hit[r] || cache_.bytes + words * 8 > budget / 4) continue;
+                uint64_t* keep = nullptr;
+                hip_check(hipMalloc(&keep, words * 8), "hipMalloc");
+                hip_check(hipMemcpyAsync(keep, ref_masks + size_t(r) * words, words * 8, hipMemcpyDeviceToDevice,
+                                         stream_),
+                          "cache reference mask");
+                cache_.masks[{ss, kv.first}] = {keep, ref_clipped[r] != 0};
+                cache_.bytes += words * 8;
+            }
         hip_check(hipEventRecord(events_[1], stream_), "event");
         hip_check(hipMemcpyAsync(dpairs, pairs.data(), pairs.size() * sizeof(ViewPair), hipMemcpyHostToDevice, stream_),
                   "upload pairs");
@@ -631,8 +688,8 @@ void Audit::run_items(const DeviceMesh& reference, const std::vector<const Devic
         timing_.distance_views += ncand;
         for (uint32_t i = 0; i < ncand; ++i) {
             const Item& it = all[begin + i];
-            const uint32_t ra = ref_job[ref_slot[it.view]], rc = cand_job[i];
-            const bool clipped = st[2 * ra] != kNone || st[2 * ra + 1] || st[2 * rc] != kNone || st[2 * rc + 1];
+            const uint32_t rc = cand_job[i];
+            const bool clipped = ref_clipped[ref_slot[it.view]] || st[2 * rc] != kNone || st[2 * rc + 1];
             const ViewSummary& v = sum[i];
             ViewResult& r = out[begin + i];
             r.changed = v.changed;

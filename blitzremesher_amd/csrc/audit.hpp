@@ -6,6 +6,7 @@
 
 #include <cstdint>
 #include <functional>
+#include <map>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -121,7 +122,10 @@ struct PatchSet {
 
 class Audit {
   public:
-    Audit(int device, uint64_t memory_bytes);
+    // cache_rasters: reuse the reference's coverage masks across audits
+    // against the same uploaded reference (the reference's RasterMemo,
+    // src/neural/evaluate.cu:386-511); results are unchanged.
+    Audit(int device, uint64_t memory_bytes, bool cache_rasters = false);
     ~Audit();
     Audit(const Audit&) = delete;
     Audit& operator=(const Audit&) = delete;
@@ -210,8 +214,29 @@ class Audit {
         return static_cast<T*>(scratch(slot, count * sizeof(T)));
     }
 
+    // Reference masks kept across audits, per (supersampling, view), valid
+    // for one (reference upload, bounds, screen, view set, sidedness).
+    struct CachedMask {
+        uint64_t* mask;
+        bool clipped;
+    };
+    struct RasterCache {
+        uint64_t serial = 0;
+        Bounds bounds{};
+        double screen = 0;
+        uint16_t ortho = 0, persp = 0;
+        uint32_t seed = 0;
+        bool two = false;
+        size_t bytes = 0;
+        std::map<std::pair<uint8_t, uint32_t>, CachedMask> masks;
+    };
+    void cache_context(const DeviceMesh& reference, const Bounds& b, const Settings& s);
+    void cache_clear();
+
     int device_;
     uint64_t memory_bytes_;
+    bool cache_rasters_ = false;
+    RasterCache cache_;
     hipStream_t stream_{};
     hipEvent_t events_[10]{};
     std::vector<std::pair<void*, size_t>> slots_;

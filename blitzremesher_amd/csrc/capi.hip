@@ -245,7 +245,7 @@ blitz_status blitz_hip_audit_options_init(blitz_hip_audit_options* o, size_t n) 
     o->version = BLITZ_HIP_API_VERSION;
     o->device = 0;
     o->memory_mib = 0;
-    o->cache_rasters = 1;
+    o->cache_rasters = 0;
     return BLITZ_OK;
 }
 
@@ -310,7 +310,7 @@ blitz_status blitz_hip_audit_create(const blitz_hip_audit_options* o, blitz_hip_
             throw std::invalid_argument("null pointer, options size or version mismatch");
         if (!blitz_hip_available(o->device)) throw Unavailable("no gfx950 HIP device at the requested index");
         auto a = std::make_unique<blitz_hip_audit>();
-        a->impl = std::make_unique<Audit>(o->device, uint64_t(o->memory_mib) << 20);
+        a->impl = std::make_unique<Audit>(o->device, uint64_t(o->memory_mib) << 20, o->cache_rasters != 0);
         *out = a.release();
     });
 }

@@ -100,7 +100,8 @@ typedef struct blitz_hip_audit_options {
     uint32_t struct_size, version;
     int32_t device;
     uint32_t memory_mib;    /* workspace cap; 0 = no cap */
-    uint8_t cache_rasters;  /* keep bound-reference masks resident */
+    uint8_t cache_rasters;  /* reuse the reference's masks across audits of one uploaded reference
+                             * (RasterMemo, src/neural/evaluate.cu:386-511); 0 by default */
     uint8_t reserved[7];
 } blitz_hip_audit_options;
 

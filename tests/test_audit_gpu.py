@@ -320,3 +320,27 @@ def test_gpu_evaluations_match_golden(audit):
         got = audit.evaluate(dev[case["reference"]], dev[case["candidate"]], case["center"], case["radius"],
                              settings_of(case["settings"]))
         assert got == measurement_of(case["measurement"]), case["case"]
+
+
+def test_reference_mask_cache_keeps_results(oracle):
+    # Audits against one uploaded reference with its masks kept across calls
+    # equal fresh audits: refinement, first failures, perspective views, a
+    # change of bounds or screen (new cache context), batches.
+    ico = meshes.icosphere(3, 0.02, 1)
+    center, radius = oracle.bounds(ico)
+    cands = [meshes.jitter(ico, 0.002 * (k + 1), 40 + k, 0.4) for k in range(4)]
+    settings = [eval_settings(limit=1.0), eval_settings(limit=0.6), eval_settings(limit=3.0)]
+    with Audit(0, cache_rasters=True) as cached, Audit(0) as fresh:
+        rc, rf = cached.upload(ico), fresh.upload(ico)
+        for s in settings + settings:
+            for c in cands:
+                got = cached.evaluate(rc, cached.upload(c), center, radius, s)
+                want = fresh.evaluate(rf, fresh.upload(c), center, radius, s)
+                assert got.as_tuple() == want.as_tuple()
+        s = eval_settings(limit=1.0)
+        moved = (center + np.float32(0.01), radius * 1.1)
+        assert cached.evaluate(rc, cached.upload(cands[0]), *moved, s).as_tuple() == \
+            fresh.evaluate(rf, fresh.upload(cands[0]), *moved, s).as_tuple()
+        batch = cached.evaluate_batch(rc, [cached.upload(c) for c in cands], center, radius, s)
+        for c, m in zip(cands, batch):
+            assert m.as_tuple() == fresh.evaluate(rf, fresh.upload(c), center, radius, s).as_tuple()
