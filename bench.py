@@ -174,6 +174,45 @@ def patch_leg(ref, center, radius, settings, count: int, repeats: int) -> dict:
             "generate_s": round(generate_s, 2)}
 
 
+def appearance_leg(steps: int) -> dict:
+    """Normals and attributes profiles on the CPU-runnable icosphere with
+    normal and colour streams (10,242 V / 20,480 F) against its 15,000-face
+    quadric copy carrying the same streams: 16 views at 256 px, ss 8 (refined
+    to 16 where uncertain), limits 1 px and 3 px; and the top-left overlap
+    measure of the icosphere over the same views."""
+    import numpy as np
+    from blitzremesher_amd import Audit, EvalSettings, Mesh, meshes
+    from blitzremesher_amd.audit import bounds, reduce_reuse
+    ico = meshes.with_attributes(meshes.icosphere(5, 0.02, 1), 7)
+    red = reduce_reuse(ico, 15000)
+    cand = Mesh(red.positions, red.indices, normals=ico.normals, colors=ico.colors)
+    center, radius = bounds(ico)
+    out = {"workload": "icosphere 10,242 V / 20,480 F with normals and colours vs its 15,000-face quadric copy, "
+                       "16 views at 256 px, ss 8 -> 16"}
+    with Audit(0) as audit:
+        dr, dc = audit.upload(ico), audit.upload(cand)
+        for profile in (1, 2):
+            for limit in (1.0, 3.0):
+                s = EvalSettings(profile=profile, screen_size=256, supersample=8, max_supersample=16,
+                                 orthographic=16, perspective=0, limit=limit)
+                audit.evaluate(dr, dc, center, radius, s)
+                times, app = [], []
+                for _ in range(steps):
+                    t0 = time.perf_counter()
+                    m = audit.evaluate(dr, dc, center, radius, s)
+                    times.append(time.perf_counter() - t0)
+                    app.append(audit.last_timing()["appearance_ms"])
+                out[f"profile{profile}_limit{limit:g}"] = {
+                    "ms": round(1000 * float(np.median(times)), 3), "appearance_ms": round(float(np.median(app)), 3),
+                    "error": m.error, "normal_degrees": round(m.normal_degrees, 4), "views": m.views_evaluated,
+                    "passed": m.passed}
+        audit.overlap(dr, center, radius, 256, 16, 0)
+        t0 = time.perf_counter()
+        value = audit.overlap(dr, center, radius, 256, 16, 0)
+        out["overlap"] = {"value": value, "ms": round(1000 * (time.perf_counter() - t0), 3)}
+    return out
+
+
 def executor_leg(budget: int, cpu_budget: int) -> dict:
     """The plain classical method's audited collapses on the CPU-runnable
     icosphere (10,242 V / 20,480 F, radial noise 0.02): execute_actions with
@@ -263,6 +302,7 @@ def main() -> int:
     ap.add_argument("--no-patches", action="store_true")
     ap.add_argument("--patches", type=int, default=4096)
     ap.add_argument("--no-executor", action="store_true")
+    ap.add_argument("--no-appearance", action="store_true")
     ap.add_argument("--executor-budget", type=int, default=400)
     ap.add_argument("--executor-cpu-budget", type=int, default=3)
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -334,6 +374,7 @@ def main() -> int:
     scorer = None if args.no_scorer else scorer_leg(max(3, min(args.steps, 10)))
     patches = None if args.no_patches else patch_leg(ref, center, radius, settings, args.patches,
                                                        max(3, min(args.steps, 10)))
+    appearance = None if args.no_appearance else appearance_leg(max(3, min(args.steps, 10)))
     executor = None if args.no_executor else executor_leg(args.executor_budget,
                                                           0 if args.no_cpu else args.executor_cpu_budget)
 
@@ -365,6 +406,7 @@ def main() -> int:
         "parity_on_cpu_sample": parity,
         "patches": patches,
         "executor": executor,
+        "appearance": appearance,
         "scorer": scorer,
     }
     print(json.dumps(line), flush=True)
