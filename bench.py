@@ -263,6 +263,72 @@ def executor_leg(budget: int, cpu_budget: int) -> dict:
     return leg
 
 
+def neural_leg(train_steps: int, budget: int) -> dict:
+    """The learned proposer end to end on the CPU-runnable icosphere: every
+    legal collapse of the source is audited as a patch of the bound source
+    (16 views at 256 px, ss 8, limit 1 px: accept/reject labels); the
+    edge-graph scorer (He-normal init, seed 4) takes `train_steps` BF16 AdamW
+    steps of the pairwise ranking loss on pools of 32 labelled collapses
+    (batch 8,192); then execute_actions runs with the learned order, the
+    untrained scorer's order and the plain current-plane order, `budget` trials
+    each, every trial gated by the executor leg's exact audit."""
+    import numpy as np
+    from blitzremesher_amd import Audit, EvalSettings, meshes
+    from blitzremesher_amd.actions import ActionSet, execute_actions
+    from blitzremesher_amd.audit import bounds
+    from blitzremesher_amd.scorer import BF16, PREFERRED, VALID, Scorer, TrainSettings, he_weights
+    mesh = meshes.icosphere(5, 0.02, 1)
+    center, radius = bounds(mesh)
+    gate = EvalSettings(profile=0, screen_size=256, supersample=8, max_supersample=16, orthographic=16,
+                        perspective=0, rotation_seed=0xB1172026, limit=1.0)
+    label_gate = EvalSettings(**{**gate.__dict__, "max_supersample": 8})
+    target = mesh.triangles // 2
+    out = {"workload": "labels: every legal collapse of the 10,242-vertex icosphere audited as a patch (16 views, "
+                       "S=256, ss 8, limit 1 px); training: BF16 ranking loss, batch 8,192 in pools of 32; executor: "
+                       f"{budget} trials towards {target} triangles, batch 64, gate as the executor leg"}
+    with Audit(0, cache_rasters=True) as audit, Scorer(he_weights(4)) as scorer, \
+            Scorer(he_weights(4)) as untrained, ActionSet(mesh) as acts:
+        n = len(acts.legal)
+        t0 = time.perf_counter()
+        patches = acts.patches(np.arange(n, dtype=np.uint32))
+        binding = audit.bind(mesh, center, radius, label_gate)
+        ms, fallbacks = audit.evaluate_patches(binding, patches, label_gate)
+        label_s = time.perf_counter() - t0
+        passed = np.array([m.passed for m in ms], bool)
+        g = scorer.graph(mesh, center, radius)
+        pairs = g.actions_array()
+        keys = pairs[:, 0].astype(np.uint64) << np.uint64(32) | pairs[:, 1].astype(np.uint64)
+        legal_keys = acts.legal[:, 0].astype(np.uint64) << np.uint64(32) | acts.legal[:, 1].astype(np.uint64)
+        idx = np.searchsorted(keys, legal_keys).astype(np.uint32)
+        assert np.array_equal(keys[idx], legal_keys)
+        rng = np.random.default_rng(7)
+        ts = TrainSettings(precision=BF16)
+        losses = []
+        t0 = time.perf_counter()
+        for _ in range(train_steps):
+            sel = rng.choice(n, 8192, replace=False)
+            labels = (VALID | np.where(passed[sel], PREFERRED, 0)).astype(np.uint8)
+            losses.append(scorer.train_step(g, idx[sel], labels, settings=ts).loss)
+        train_s = time.perf_counter() - t0
+        logits = scorer.forward(g, BF16)[:, 0][idx]
+        top = np.argsort(-logits, kind="stable")[: n // 10]
+        out["labels"] = {"legal": n, "passed": int(passed.sum()), "patch_audit_s": round(label_s, 3),
+                         "full_path_fallbacks": int(fallbacks),
+                         "labels_per_s": round(n / label_s, 1)}
+        out["training"] = {"steps": train_steps, "s": round(train_s, 3), "loss_first": round(losses[0], 4),
+                           "loss_last": round(float(np.mean(losses[-10:])), 4),
+                           "pass_rate_top_decile": round(float(passed[top].mean()), 4),
+                           "pass_rate_all": round(float(passed.mean()), 4)}
+        g.release()
+        for name, ranking, sc in (("current_plane", 4, None), ("learned", 0, scorer), ("untrained", 0, untrained)):
+            o, st = execute_actions(audit, mesh, center, radius, gate, target, budget, ranking, 64, scorer=sc)
+            out[name] = {"trials": st["trials"], "accepted": st["accepted"], "rejected": st["rejected"],
+                         "triangles": int(len(o)), "ms": round(st["ms"], 1),
+                         "accepted_per_trial": round(st["accepted"] / max(1, st["trials"]), 3),
+                         "audited_collapses_per_s": round(st["accepted"] / (st["ms"] / 1000.0), 1)}
+    return out
+
+
 def copy_probe(mib: int) -> int:
     """Runs the counter-calibration copy kernel (see profiles/) and prints its
     bandwidth."""
@@ -303,6 +369,8 @@ def main() -> int:
     ap.add_argument("--patches", type=int, default=4096)
     ap.add_argument("--no-executor", action="store_true")
     ap.add_argument("--no-appearance", action="store_true")
+    ap.add_argument("--no-neural", action="store_true")
+    ap.add_argument("--neural-train-steps", type=int, default=300)
     ap.add_argument("--executor-budget", type=int, default=400)
     ap.add_argument("--executor-cpu-budget", type=int, default=3)
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -375,6 +443,7 @@ def main() -> int:
     patches = None if args.no_patches else patch_leg(ref, center, radius, settings, args.patches,
                                                        max(3, min(args.steps, 10)))
     appearance = None if args.no_appearance else appearance_leg(max(3, min(args.steps, 10)))
+    neural = None if args.no_neural else neural_leg(args.neural_train_steps, args.executor_budget)
     executor = None if args.no_executor else executor_leg(args.executor_budget,
                                                           0 if args.no_cpu else args.executor_cpu_budget)
 
@@ -407,6 +476,7 @@ def main() -> int:
         "patches": patches,
         "executor": executor,
         "appearance": appearance,
+        "neural": neural,
         "scorer": scorer,
     }
     print(json.dumps(line), flush=True)

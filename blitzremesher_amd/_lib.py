@@ -168,7 +168,7 @@ PROTOTYPES = {
                                           C.POINTER(C.c_size_t), *_ERR]),
     "blitz_hip_actions_rank": (C.c_int, [_P, C.c_uint8, _P, _P, C.POINTER(C.c_uint32), C.c_size_t, *_ERR]),
     "blitz_hip_execute_actions": (C.c_int, [_P, C.POINTER(BlitzMesh), C.POINTER(Bounds), C.POINTER(EvalSettingsC),
-                                            C.c_uint64, C.c_uint32, C.c_uint8, C.c_uint8, C.POINTER(C.c_uint32),
+                                            C.c_uint64, C.c_uint32, C.c_uint8, _P, C.c_uint8, C.POINTER(C.c_uint32),
                                             C.c_size_t, C.POINTER(C.c_size_t), C.POINTER(ActionStats), *_ERR]),
     "blitz_hip_actions_patches": (C.c_int, [_P, C.POINTER(C.c_uint32), C.c_size_t, C.POINTER(C.c_uint32),
                                             C.POINTER(C.c_uint32), C.c_size_t, C.POINTER(C.c_uint32),

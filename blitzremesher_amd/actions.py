@@ -105,10 +105,11 @@ class ActionSet:
 
 
 def execute_actions(audit, mesh: Mesh, center, radius, gate, target: int, budget: int, ranking: int,
-                    batch: int, uv: np.ndarray | None = None):
+                    batch: int, uv: np.ndarray | None = None, scorer=None):
     """neural::execute_actions (src/neural/action.cpp:453-525) on the device:
     every trial gated by an exact audit against `mesh` under the EvalSettings
-    `gate`. Returns (index buffer (m, 3), stats dict)."""
+    `gate`; ranking 0 orders by the BF16 logits of `scorer`. Returns (index
+    buffer (m, 3), stats dict)."""
     from .audit import _bounds_c
     lib = _lib.load()
     desc = mesh.descriptor()
@@ -123,8 +124,8 @@ def execute_actions(audit, mesh: Mesh, center, radius, gate, target: int, budget
     st.struct_size = C.sizeof(st)
     err = _lib.error_buffer()
     _lib.check(lib.blitz_hip_execute_actions(audit.handle, C.byref(desc), C.byref(b), C.byref(s), int(target), budget,
-                                             ranking, batch, _u32p(out), out.size, C.byref(count), C.byref(st), err,
-                                             len(err)), err)
+                                             ranking, scorer.handle if scorer is not None else None, batch,
+                                             _u32p(out), out.size, C.byref(count), C.byref(st), err, len(err)), err)
     del keep
     stats = {k: getattr(st, k) for k, _ in st._fields_ if k != "struct_size"}
     return out[: count.value].reshape(-1, 3).copy(), stats

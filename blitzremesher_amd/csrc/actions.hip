@@ -945,11 +945,13 @@ std::vector<uint32_t> ActionSet::trial(const uint32_t* rows, size_t n, std::vect
 // src/neural/action.cpp:453-525 with the gate supplied by the caller.
 std::vector<uint32_t> execute_actions(int device, const HostMesh& input, const float* uv, uint64_t target,
                                       uint32_t budget, uint32_t ranking, uint32_t batch, const ActionGate& gate,
-                                      ExecuteStats& stats, std::vector<uint16_t>* materials_out) {
+                                      ExecuteStats& stats, std::vector<uint16_t>* materials_out, Scorer* scorer,
+                                      const Bounds* bounds) {
     if (!batch || batch > 64) throw std::invalid_argument("action batch outside 1..64");
-    if (ranking == 0 || ranking == 2)
-        throw Unavailable("the HIP executor ranks by the constant, shortest-edge and current-plane orders");
+    if (ranking == 2) throw Unavailable("shuffled ranking is not implemented by the HIP executor");
     if (ranking > 4) throw std::invalid_argument("unknown ranking");
+    if (ranking == 0 && (!scorer || !bounds)) throw std::invalid_argument("learned ranking needs a scorer");
+    if (ranking == 0 && scorer->device() != device) throw std::invalid_argument("scorer on another device");
     const auto clock0 = std::chrono::steady_clock::now();
     stats = {};
     const double diameter = std::max(1e-20, bounds_of(input.positions.data(), input.positions.size() / 3).radius * 2);
@@ -966,7 +968,12 @@ std::vector<uint32_t> execute_actions(int device, const HostMesh& input, const f
         const size_t n = set.legal_count();
         if (!n) break;
         t = std::chrono::steady_clock::now();
-        set.rank(ranking, diameter, nullptr, nullptr);
+        std::unique_ptr<Graph> graph;
+        if (ranking == 0) {
+            graph = scorer->graph(state, *bounds);
+            scorer->forward(*graph, Precision::Bf16);
+        }
+        set.rank(ranking, diameter, graph.get(), nullptr);
         stats.rank_ms += since(t);
         stats.ranked += n;
         const uint64_t faces = state.indices.size() / 3;

@@ -113,12 +113,14 @@ struct ExecuteStats {
     double build_ms = 0, rank_ms = 0, select_ms = 0, trial_ms = 0, gate_ms = 0;
 };
 // neural::execute_actions (src/neural/action.cpp:453-525) on the device
-// action state: rank (1 constant, 3 shortest edge, 4 current plane), take
-// the greedy batch of up to `batch` actions from the rank position, trial it,
-// ask the gate, commit or halve; a rejected singleton moves past the batch's
-// first action. Returns the final index buffer.
+// action state: rank (0 learned: BF16 logits of `scorer` on each state's
+// graph, built with `bounds`; 1 constant, 3 shortest edge, 4 current plane),
+// take the greedy batch of up to `batch` actions from the rank position,
+// trial it, ask the gate, commit or halve; a rejected singleton moves past
+// the batch's first action. Returns the final index buffer.
 std::vector<uint32_t> execute_actions(int device, const HostMesh& input, const float* uv, uint64_t target,
                                       uint32_t budget, uint32_t ranking, uint32_t batch, const ActionGate& gate,
-                                      ExecuteStats& stats, std::vector<uint16_t>* materials_out = nullptr);
+                                      ExecuteStats& stats, std::vector<uint16_t>* materials_out = nullptr,
+                                      Scorer* scorer = nullptr, const Bounds* bounds = nullptr);
 
 }  // namespace blitz_hip

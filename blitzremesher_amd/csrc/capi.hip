@@ -541,9 +541,9 @@ blitz_status blitz_hip_actions_rank(blitz_hip_actions* a, uint8_t ranking, blitz
 
 blitz_status blitz_hip_execute_actions(blitz_hip_audit* a, const blitz_mesh* input, const blitz_hip_bounds* b,
                                        const blitz_hip_eval_settings* gate, uint64_t target, uint32_t budget,
-                                       uint8_t ranking, uint8_t batch, uint32_t* indices, size_t index_capacity,
-                                       size_t* index_count, blitz_hip_action_stats* stats, char* error,
-                                       size_t capacity) {
+                                       uint8_t ranking, blitz_hip_scorer* scorer, uint8_t batch, uint32_t* indices,
+                                       size_t index_capacity, size_t* index_count, blitz_hip_action_stats* stats,
+                                       char* error, size_t capacity) {
     return guarded(error, capacity, [&] {
         if (!indices || !index_count) throw std::invalid_argument("null output");
         if (stats && stats->struct_size != sizeof(*stats)) throw std::invalid_argument("stats size mismatch");
@@ -573,8 +573,10 @@ blitz_status blitz_hip_execute_actions(blitz_hip_audit* a, const blitz_mesh* inp
             return audit.evaluate(*reference, *candidate, bounds, settings).passed;
         };
         ExecuteStats st;
+        if (ranking == 0 && (!scorer || !scorer->impl)) throw std::invalid_argument("learned ranking needs a scorer");
         const auto out = execute_actions(audit.device(), source, uv.empty() ? nullptr : uv.data(), target, budget,
-                                         ranking, batch, accept, st);
+                                         ranking, batch, accept, st, nullptr, scorer ? scorer->impl.get() : nullptr,
+                                         &bounds);
         if (out.size() > index_capacity) throw std::invalid_argument("index capacity too small");
         std::memcpy(indices, out.data(), out.size() * 4);
         *index_count = out.size();

@@ -275,16 +275,19 @@ typedef struct blitz_hip_action_stats {
 /* blitz::neural::execute_actions (src/neural/action.cpp:453-525; the GPU
  * executor of src/neural/action_gpu.cu, checked against it by
  * tests/neural_action_gpu.cpp:140-190): collapse `input` towards `target`
- * triangles within `budget` trials, ranking by `ranking` (1 constant,
- * 3 shortest edge, 4 current plane), batches of up to `batch` (1..64)
+ * triangles within `budget` trials, ranking by `ranking` (0 learned: the BF16
+ * logits of `scorer` on each mesh state's graph; 1 constant, 3 shortest edge,
+ * 4 current plane; `scorer` may be null except for 0), batches of up to
+ * `batch` (1..64)
  * independent actions, each trial gated by an exact audit against `input`
  * under `gate` (accepted when the Measurement passes). Writes the final index
  * buffer. Needs the blitz_hip_audit of the device the actions run on. */
+typedef struct blitz_hip_scorer blitz_hip_scorer;
 blitz_status blitz_hip_execute_actions(blitz_hip_audit*, const blitz_mesh* input, const blitz_hip_bounds*,
                                        const blitz_hip_eval_settings* gate, uint64_t target, uint32_t budget,
-                                       uint8_t ranking, uint8_t batch, uint32_t* indices, size_t index_capacity,
-                                       size_t* index_count, blitz_hip_action_stats* stats, char* error,
-                                       size_t capacity);
+                                       uint8_t ranking, blitz_hip_scorer* scorer, uint8_t batch, uint32_t* indices,
+                                       size_t index_capacity, size_t* index_count, blitz_hip_action_stats* stats,
+                                       char* error, size_t capacity);
 
 /* blitz::detail::rasterize_coverage (src/coverage.hpp:12, src/evaluate.cpp:125-189)
  * for one camera: row-major packed words (index = y*size + x), padding bits

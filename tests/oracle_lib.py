@@ -349,11 +349,12 @@ class Actions:
         return out[:n].reshape(-1, 3).copy()
 
 
-def execute_actions(mesh: Mesh, target: int, budget: int, ranking: int, batch: int, gate, uv=None):
+def execute_actions(mesh: Mesh, target: int, budget: int, ranking: int, batch: int, gate, uv=None, scores=None):
     """Restatement of neural::execute_actions (src/neural/action.cpp:453-525)
     on the oracle's action state: rank, take the greedy batch from the rank
     position, trial it, gate, commit or halve; a rejected singleton moves past
-    the batch's first action. `gate(indices) -> bool`. Returns the index buffer
+    the batch's first action. `gate(indices) -> bool`; for ranking 0,
+    `scores(indices, legal pairs) -> float32 scores`. Returns the index buffer
     and the ranked / trials / accepted / rejected counters. Meshes without
     materials."""
     assert mesh.materials is None
@@ -367,7 +368,11 @@ def execute_actions(mesh: Mesh, target: int, budget: int, ranking: int, batch: i
         state = Actions("oracle", Mesh(mesh.positions, current), uv)
         if not len(state.legal):
             break
-        order = state.rank(ranking, diameter)
+        if ranking == 0:
+            sc = np.asarray(scores(current, state.legal), np.float32)
+            order = np.argsort(-(sc + np.float32(0)), kind="stable").astype(np.uint32)
+        else:
+            order = state.rank(ranking, diameter)
         stats["ranked"] += len(state.legal)
         accepted, position = False, 0
         while position < len(order) and stats["trials"] < budget and not accepted:

@@ -318,3 +318,32 @@ def test_gpu_executor_matches_oracle():
                 out, stats = gpu_execute(audit, mesh, center, radius, gate, mesh.triangles // 3, 50, ranking, batch)
                 got = {k: stats[k] for k in ("ranked", "trials", "accepted", "rejected")}
                 assert np.array_equal(out, want[0]) and got == want[1], (ranking, batch, got, want[1])
+
+
+@pytest.mark.gpu
+def test_gpu_learned_executor_matches_oracle_loop():
+    # The executor's learned order: per mesh state the scorer's BF16 logits of
+    # that state's graph. The oracle loop is driven by the same logits (read
+    # back from the scorer per state), so output and counters must agree.
+    from blitzremesher_amd import Audit
+    from blitzremesher_amd.actions import execute_actions as gpu_execute
+    from blitzremesher_amd.scorer import BF16, Scorer, he_weights
+    mesh = meshes.icosphere(3, 0.03, 5)
+    o = Oracle("oracle")
+    center, radius = o.bounds(mesh)
+    gate = EvalSettings(profile=0, screen_size=32, limit=1.25, supersample=2, max_supersample=8, orthographic=4,
+                        perspective=2, rotation_seed=5)
+    with Audit(0) as audit, Scorer(he_weights(4)) as scorer:
+        def scores(indices, legal):
+            g = scorer.graph(Mesh(mesh.positions, indices), center, radius)
+            logits = scorer.forward(g, BF16)[:, 0]
+            key = {(int(a), int(b)): i for i, (a, b) in enumerate(g.actions_array())}
+            g.release()
+            return np.array([logits[key[(int(u), int(v))]] for u, v in legal], np.float32)
+        for batch in (4, 64):
+            want = execute_actions(mesh, mesh.triangles // 3, 40, 0, batch, oracle_gate(mesh, center, radius, gate),
+                                   scores=scores)
+            out, stats = gpu_execute(audit, mesh, center, radius, gate, mesh.triangles // 3, 40, 0, batch,
+                                     scorer=scorer)
+            got = {k: stats[k] for k in ("ranked", "trials", "accepted", "rejected")}
+            assert np.array_equal(out, want[0]) and got == want[1], (batch, got, want[1])
