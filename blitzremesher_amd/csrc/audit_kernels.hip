@@ -44,12 +44,17 @@ __global__ __launch_bounds__(256) void project_kernel(const ProjectTask* __restr
 
 // Faces whose bounding box is at most 8x8 samples with few samples are
 // rasterized by their own lane; other faces within 8x8 samples take one
-// whole-wave pass (lane = sample, ballot = bitmap); larger ones are walked
-// tile by tile by the whole wavefront, with a ballot producing each word.
+// whole-wave pass (lane = sample, ballot = bitmap); for larger ones the
+// wavefront first sorts the face's tiles into empty, fully covered and
+// undecided (classify_rect), then produces each undecided tile's word with a
+// ballot.
 #ifndef BLITZ_LANE_SAMPLES
 #define BLITZ_LANE_SAMPLES 24  // build-time tuning override
 #endif
 constexpr int kLaneSamples = BLITZ_LANE_SAMPLES;
+
+// Workgroups a raster launch should reach (four per CU of the 256-CU part).
+constexpr uint32_t kFillBlocks = 1024;
 
 // One block per (task, band, face chunk). Blocks are mapped so that the
 // blocks of one task land on one XCD (hardware dispatches block w to XCD
@@ -143,6 +148,9 @@ __global__ __launch_bounds__(kRasterThreads) void raster_band_kernel(const Raste
                 }
             }
         }
+        // Large faces: the wave classifies up to 64 of the face's tiles at
+        // once (lane = tile) as empty, fully covered (within the box) or
+        // undecided; only undecided tiles get the per-sample pass.
         pending = __ballot(large);
         while (pending) {
             const int src = __ffsll(static_cast<unsigned long long>(pending)) - 1;
@@ -150,15 +158,34 @@ __global__ __launch_bounds__(kRasterThreads) void raster_band_kernel(const Raste
             const Setup q = shuffle_setup(s, src);
             const int lx = lane & 7, ly = lane >> 3;
             const int tya = max(q.y0, ymin) >> 3, tyb = min(q.y1, ymax) >> 3;
-            for (int ty = tya; ty <= tyb; ++ty)
-                for (int tx = q.x0 >> 3; tx <= (q.x1 >> 3); ++tx) {
-                    const int x = (tx << 3) | lx, y = (ty << 3) | ly;
+            const int txa = q.x0 >> 3, across = (q.x1 >> 3) - txa + 1;
+            const int count = across * (tyb - tya + 1);
+            for (int first = 0; first < count; first += 64) {
+                const int k = first + lane;
+                int kind = 0, tx = 0, ty = 0;
+                if (k < count) {
+                    tx = txa + k % across;
+                    ty = tya + k / across;
+                    const int xa = max(tx << 3, q.x0), xb = min((tx << 3) | 7, q.x1);
+                    const int ya = max(ty << 3, q.y0), yb = min((ty << 3) | 7, q.y1);
+                    kind = classify_rect(q, xa, xb, ya, yb);
+                    if (kind == 1)
+                        atomicOr(&band[uint32_t(ty - int(ty0)) * t.tiles + uint32_t(tx)],
+                                 static_cast<unsigned long long>(rect_word(xa & 7, xb & 7, ya & 7, yb & 7)));
+                }
+                uint64_t open = __ballot(kind == 2);
+                while (open) {
+                    const int at = __ffsll(static_cast<unsigned long long>(open)) - 1;
+                    open &= open - 1;
+                    const int ux = __shfl(tx, at), uy = __shfl(ty, at);
+                    const int x = (ux << 3) | lx, y = (uy << 3) | ly;
                     const bool in = x >= q.x0 && x <= q.x1 && y >= q.y0 && y <= q.y1 && covers(q, x, y);
                     const uint64_t word = __ballot(in);
                     if (lane == 0 && word)
-                        atomicOr(&band[uint32_t(ty - int(ty0)) * t.tiles + uint32_t(tx)],
+                        atomicOr(&band[uint32_t(uy - int(ty0)) * t.tiles + uint32_t(ux)],
                                  static_cast<unsigned long long>(word));
                 }
+            }
         }
     }
     __syncthreads();
@@ -367,6 +394,13 @@ RasterGrid make_raster_grid(uint32_t tasks, uint32_t tiles, uint32_t max_faces) 
     // one view at a time and its L2 holds that view's projected vertices),
     // each with at least 8 batches of 1024 faces.
     g.chunks = std::max<uint32_t>(1, std::min<uint32_t>(32, max_faces / (8 * kRasterThreads)));
+    // Few views of a large raster (few tasks x bands): split the faces
+    // further, down to one batch per chunk, so the launch still has about
+    // four workgroups per CU.
+    const uint32_t base = std::max(1u, tasks * g.bands);
+    const uint32_t wanted = (kFillBlocks + base - 1) / base;
+    if (g.chunks < wanted)
+        g.chunks = std::max(g.chunks, std::min(wanted, (max_faces + kRasterThreads - 1) / kRasterThreads));
     return g;
 }
 

@@ -120,6 +120,45 @@ __device__ __forceinline__ bool covers(const Setup& s, int x, int y) {
     return !(u + s.r0 < 0 || v + s.r1 < 0 || w + s.r2 < 0);
 }
 
+// Whole-rectangle form of covers() over the samples [xa, xb] x [ya, yb]:
+// 0 = no sample covered, 1 = every sample covered, 2 = undecided (test each
+// sample). Exact, with no tolerance: an edge value is
+// fl(fl(A*fl(py-Py)) - fl(B*fl(px-Px))) and every rounded operation is
+// monotone in each operand, so over a rectangle the rounded value (and
+// fl(value + r)) is smallest at the row end chosen by sign(A) combined with
+// the column end chosen by sign(B), and largest at the opposite pair, both
+// of which are samples of the rectangle. Non-finite corners are left
+// undecided.
+__device__ __forceinline__ int classify_rect(const Setup& s, int xa, int xb, int ya, int yb) {
+    const double pxa = xa + .5, pxb = xb + .5, pya = ya + .5, pyb = yb + .5;
+    bool all = true, none = false;
+    const auto edge_range = [&](double ax, double ay, double bx, double by, double r) {
+        const double a = bx - ax, b = by - ay;  // value = a*(py-ay) - b*(px-ax)
+        const double t1a = a * (pya - ay), t1b = a * (pyb - ay);
+        const double t2a = b * (pxa - ax), t2b = b * (pxb - ax);
+        const double t1min = a >= 0 ? t1a : t1b, t1max = a >= 0 ? t1b : t1a;
+        const double t2min = b >= 0 ? t2a : t2b, t2max = b >= 0 ? t2b : t2a;
+        const double lo = (t1min - t2max) + r, hi = (t1max - t2min) + r;
+        all = all && !(lo < 0);
+        none = none || hi < 0;
+    };
+    edge_range(s.bx, s.by, s.cx, s.cy, s.r0);
+    edge_range(s.cx, s.cy, s.ax, s.ay, s.r1);
+    edge_range(s.ax, s.ay, s.bx, s.by, s.r2);
+    const bool finite = isfinite(s.ax) && isfinite(s.ay) && isfinite(s.bx) && isfinite(s.by) && isfinite(s.cx) &&
+                        isfinite(s.cy);
+    if (!finite) return 2;
+    return none ? 0 : all ? 1 : 2;
+}
+
+// Bits of the samples [xa, xb] x [ya, yb] of one 8x8 tile word (coordinates
+// within the tile, bit 8*y + x).
+__device__ __forceinline__ uint64_t rect_word(int xa, int xb, int ya, int yb) {
+    const uint64_t row = uint64_t((0xFFu >> (7 - xb)) & (0xFFu << xa));
+    const uint64_t rows = (~0ull >> (8 * (7 - yb))) & (~0ull << (8 * ya));
+    return (row * 0x0101010101010101ull) & rows;
+}
+
 __device__ __forceinline__ Setup shuffle_setup(const Setup& s, int lane) {
     Setup o;
     o.ax = __shfl(s.ax, lane);

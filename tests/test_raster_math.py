@@ -1,0 +1,131 @@
+"""CPU checks of the raster's device arithmetic (blitzremesher_amd/csrc/
+raster_device.hpp): the whole-rectangle coverage classifier used for large
+faces must agree with the per-sample predicate (src/evaluate.cpp:166-168) on
+every sample. The functions are taken verbatim from the header, compiled for
+the host with g++ -ffp-contract=off and checked by brute force over seeded
+triangles, including corners on sample centres and strict (r = 0) edges."""
+from __future__ import annotations
+
+import shutil
+import subprocess
+from pathlib import Path
+
+import pytest
+
+ROOT = Path(__file__).resolve().parent.parent
+HEADER = ROOT / "blitzremesher_amd" / "csrc" / "raster_device.hpp"
+
+PRELUDE = """#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <random>
+using std::fabs; using std::isfinite; using std::max; using std::min;
+#define __device__
+#define __forceinline__ inline
+struct Setup {
+    double ax, ay, bx, by, cx, cy, r0, r1, r2;
+    int x0, x1, y0, y1;
+};
+"""
+
+CHECK = r"""
+int main(int argc, char** argv) {
+    const int iterations = atoi(argv[1]);
+    std::mt19937_64 rng(11);
+    long rects = 0, full = 0, empty = 0, bad = 0;
+    for (int it = 0; it < iterations; ++it) {
+        const int mode = it % 4;
+        const double span = it % 3 == 0 ? 24 : 80;
+        auto coord = [&]() {
+            double v = std::uniform_real_distribution<double>(0, span)(rng);
+            if (mode == 1) v = std::round(v * 2) / 2;
+            if (mode == 2) v = std::round(v * 8) / 8;
+            if (mode == 3) v = std::round(v) + .5 + std::uniform_int_distribution<int>(-2, 2)(rng) * 1e-13;
+            return v;
+        };
+        Setup s;
+        s.ax = coord(); s.ay = coord(); s.bx = coord(); s.by = coord(); s.cx = coord(); s.cy = coord();
+        if (it % 5 == 0) s.bx = s.ax;
+        if (it % 7 == 0) s.cy = s.by;
+        s.r0 = .5 * (fabs(s.cx - s.bx) + fabs(s.cy - s.by));
+        s.r1 = .5 * (fabs(s.ax - s.cx) + fabs(s.ay - s.cy));
+        s.r2 = .5 * (fabs(s.bx - s.ax) + fabs(s.by - s.ay));
+        if (it % 11 == 0) s.r0 = s.r1 = s.r2 = 0;
+        s.x0 = int(std::floor(std::min({s.ax, s.bx, s.cx})));
+        s.x1 = int(std::floor(std::max({s.ax, s.bx, s.cx})));
+        s.y0 = int(std::floor(std::min({s.ay, s.by, s.cy})));
+        s.y1 = int(std::floor(std::max({s.ay, s.by, s.cy})));
+        for (int ty = s.y0 >> 3; ty <= (s.y1 >> 3); ++ty)
+            for (int tx = s.x0 >> 3; tx <= (s.x1 >> 3); ++tx) {
+                const int xa = max(tx << 3, s.x0), xb = min((tx << 3) | 7, s.x1);
+                const int ya = max(ty << 3, s.y0), yb = min((ty << 3) | 7, s.y1);
+                const int kind = classify_rect(s, xa, xb, ya, yb);
+                int n = 0, in = 0;
+                uint64_t box = 0;
+                for (int y = ya; y <= yb; ++y)
+                    for (int x = xa; x <= xb; ++x) {
+                        ++n;
+                        in += covers(s, x, y);
+                        box |= 1ull << (8 * (y & 7) + (x & 7));
+                    }
+                ++rects;
+                bad += rect_word(xa & 7, xb & 7, ya & 7, yb & 7) != box;
+                if (kind == 0) { ++empty; bad += in != 0; }
+                if (kind == 1) { ++full; bad += in != n; }
+            }
+    }
+    printf("%ld %ld %ld %ld\n", rects, full, empty, bad);
+    return 0;
+}
+"""
+
+
+def _function(text: str, signature: str) -> str:
+    i = text.index(signature)
+    i = text.rfind("\n", 0, i) + 1
+    depth, j = 0, text.index("{", i)
+    for k in range(j, len(text)):
+        depth += {"{": 1, "}": -1}.get(text[k], 0)
+        if depth == 0:
+            return text[i:k + 1]
+    raise AssertionError(signature)
+
+
+def _run(tmp_path: Path, source: str, iterations: int) -> tuple[int, int, int, int]:
+    src = tmp_path / "check.cpp"
+    exe = tmp_path / "check"
+    src.write_text(source)
+    subprocess.run(["g++", "-O2", "-std=c++17", "-ffp-contract=off", "-o", str(exe), str(src)], check=True)
+    out = subprocess.run([str(exe), str(iterations)], check=True, capture_output=True, text=True).stdout.split()
+    return tuple(int(v) for v in out)
+
+
+def _source(classify: str | None = None) -> str:
+    text = HEADER.read_text()
+    parts = [_function(text, "double edge("), _function(text, "bool covers("),
+             classify or _function(text, "int classify_rect("), _function(text, "uint64_t rect_word(")]
+    return PRELUDE + "\n\n".join(parts) + CHECK
+
+
+needs_gxx = pytest.mark.skipif(shutil.which("g++") is None, reason="g++ not available")
+
+
+@needs_gxx
+def test_classify_rect_agrees_with_every_sample(tmp_path):
+    rects, full, empty, bad = _run(tmp_path, _source(), 40000)
+    assert bad == 0
+    # Every class occurs, so each branch of the classifier is exercised.
+    assert full > 1000 and empty > 1000 and rects - full - empty > 1000
+
+
+@needs_gxx
+def test_classify_rect_check_catches_wrong_corner(tmp_path):
+    # Choosing the row end by the wrong sign of A claims coverage the
+    # per-sample predicate denies; the brute-force check must see it.
+    text = HEADER.read_text()
+    broken = _function(text, "int classify_rect(").replace("a >= 0 ? t1a : t1b, t1max = a >= 0 ? t1b : t1a",
+                                                           "a >= 0 ? t1b : t1a, t1max = a >= 0 ? t1a : t1b")
+    assert broken != _function(text, "int classify_rect(")
+    _, _, _, bad = _run(tmp_path, _source(broken), 4000)
+    assert bad > 0
