@@ -56,19 +56,33 @@ constexpr int kLaneSamples = BLITZ_LANE_SAMPLES;
 // Workgroups a raster launch should reach (four per CU of the 256-CU part).
 constexpr uint32_t kFillBlocks = 1024;
 
-// One block per (task, band, face chunk). Blocks are mapped so that the
-// blocks of one task land on one XCD (hardware dispatches block w to XCD
-// w % 8): that XCD's L2 then holds the task's projected vertices.
+// One block per (task, band, face chunk); the hardware dispatches block w to
+// XCD w % 8. With a multiple of 8 chunks, chunk c of every task runs on XCD
+// c % 8, tasks in order: each XCD's L2 keeps the index ranges of its chunks
+// across all views, so the index buffers come from HBM about once per launch
+// rather than once per view. Otherwise the blocks of one task land on one
+// XCD, whose L2 then holds the task's projected vertices.
 __global__ __launch_bounds__(kRasterThreads) void raster_band_kernel(const RasterTask* __restrict__ tasks,
                                                                      RasterGrid grid) {
     extern __shared__ unsigned long long band[];
-    const uint32_t total = grid.tasks * grid.bands * grid.chunks;
-    const uint32_t per_xcd = (total + 7) / 8;
-    const uint32_t work = (blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
-    if (work >= total) return;
-    const uint32_t chunk = work % grid.chunks;
-    const uint32_t band_id = (work / grid.chunks) % grid.bands;
-    const RasterTask& t = tasks[work / (grid.chunks * grid.bands)];
+    uint32_t task_id, band_id, chunk;
+    if (grid.chunks % 8 == 0) {
+        const uint32_t xcd = blockIdx.x % 8, slot = blockIdx.x / 8, per_band = grid.chunks / 8;
+        task_id = slot / (grid.bands * per_band);
+        if (task_id >= grid.tasks) return;
+        const uint32_t r = slot % (grid.bands * per_band);
+        band_id = r / per_band;
+        chunk = (r % per_band) * 8 + xcd;
+    } else {
+        const uint32_t total = grid.tasks * grid.bands * grid.chunks;
+        const uint32_t per_xcd = (total + 7) / 8;
+        const uint32_t work = (blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
+        if (work >= total) return;
+        chunk = work % grid.chunks;
+        band_id = (work / grid.chunks) % grid.bands;
+        task_id = work / (grid.chunks * grid.bands);
+    }
+    const RasterTask& t = tasks[task_id];
     const uint32_t ty0 = band_id * grid.band_rows;
     if (ty0 >= t.tiles) return;
     const uint32_t ty1 = min(t.tiles, ty0 + grid.band_rows);
@@ -305,13 +319,38 @@ __global__ __launch_bounds__(1024) void row_distance_kernel(const ViewPair* __re
 
 // Column pass restricted to the samples that can set the directed Hausdorff
 // maximum: every covered sample of `from` that is not covered in `to`. Each
-// wavefront reads 64 tile words (one per lane) and then takes the words with
-// such samples one after another, one lane per sample, so the queries of a
-// tile run side by side. Each query's exact squared distance is min over rows
-// y' of (y-y')^2 + g(y'), g = rounded row distance (the reference's lower
-// envelope yields the same minimum; its float output is a monotone rounding
-// of it, applied once on the host). The scan stops once (y-y')^2 reaches the
-// best value or the best value cannot exceed the maximum already found.
+// wavefront reads 64 tile words (one per lane), numbers the wave's query
+// samples with a prefix sum of the words' popcounts and hands them out 64 at a
+// time, one per lane, so a wave with many sparse words (a silhouette running
+// along a tile row) does not walk its words one after another. Each query's
+// exact squared distance is min over rows y' of (y-y')^2 + g(y'), g = rounded
+// row distance (the reference's lower envelope yields the same minimum; its
+// float output is a monotone rounding of it, applied once on the host). The
+// scan stops once (y-y')^2 reaches the best value or the best value cannot
+// exceed the maximum already found.
+__device__ __forceinline__ uint32_t wave_max(uint32_t v) {
+    for (int o = 32; o > 0; o >>= 1) v = max(v, uint32_t(__shfl_xor(int(v), o)));
+    return v;
+}
+
+// Position of the j-th (0-based) set bit of `bits` (j < popcount(bits)).
+__device__ __forceinline__ uint32_t nth_bit(uint64_t bits, uint32_t j) {
+    uint32_t pos = 0;
+#pragma unroll
+    for (uint32_t width = 32; width > 0; width >>= 1) {
+        const uint64_t low = bits & ((1ull << width) - 1);
+        const uint32_t c = uint32_t(__popcll(low));
+        if (j >= c) {
+            j -= c;
+            bits >>= width;
+            pos += width;
+        } else {
+            bits = low;
+        }
+    }
+    return pos;
+}
+
 __global__ __launch_bounds__(256) void hausdorff_kernel(const ViewPair* __restrict__ pairs,
                                                         ViewSummary* summaries) {
     const uint32_t job = blockIdx.y;  // view*2 + direction; direction 0: A -> C
@@ -328,47 +367,67 @@ __global__ __launch_bounds__(256) void hausdorff_kernel(const ViewPair* __restri
     if (first >= words) return;  // whole waves
     const size_t mine = first + lane;
     const uint64_t my_bits = mine < words ? from[mine] & ~to[mine] : 0;
-    uint64_t live = __ballot(my_bits != 0);
-    if (!live) return;  // whole waves
+    if (!__ballot(my_bits != 0)) return;  // whole waves
+    // Inclusive prefix sum of the words' query counts over the wave.
+    const uint32_t count = uint32_t(__popcll(my_bits));
+    uint32_t inclusive = count;
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t v = uint32_t(__shfl_up(int(inclusive), o));
+        if (lane >= uint32_t(o)) inclusive += v;
+    }
+    const uint32_t exclusive = inclusive - count;
+    const uint32_t total = uint32_t(__shfl(int(inclusive), 63));
     const uint32_t size = p.size;
     const bool dense = dense_rows(s, size);
-    uint32_t known = __hip_atomic_load(target, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const uint32_t known = __hip_atomic_load(target, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     uint32_t local = 0;
-    while (live) {
-        const int w = __ffsll(static_cast<unsigned long long>(live)) - 1;
-        live &= live - 1;
-        const uint64_t bits = __shfl(my_bits, w);
-        if (!((bits >> lane) & 1)) continue;
-        const size_t i = first + uint32_t(w);
-        const uint32_t tx = uint32_t(i % p.tiles), ty = uint32_t(i / p.tiles);
-        const uint32_t x = (tx << 3) | (lane & 7), y = (ty << 3) | (lane >> 3);
-        // Rounded squared row distance of row r, or kFar. Sparse views search
-        // the mask only as far as a site could still improve `best`; the
-        // rounding of d^2 above 2^24 is covered by the +2 column margin.
-        auto row_term = [&](uint32_t r, uint32_t best, uint32_t kk) -> uint32_t {
-            uint32_t d;
-            if (dense) {
-                const uint16_t v = dist[size_t(r) * size + x];
-                d = v == kNoSite ? kFar : v;
-            } else {
-                const uint32_t room = best == kFar ? kFar : best - kk;
-                const uint32_t reach = room == kFar ? size : uint32_t(sqrtf(float(room))) + 2;
-                d = nearest_in_row(to, p.tiles, x, r, reach);
-            }
-            return d == kFar ? kFar : kk + rounded_square(d);
-        };
-        const uint32_t floor_value = max(known, local);
-        uint32_t best = row_term(y, kFar, 0);
-        for (uint32_t k = 1; best > floor_value && uint64_t(k) * k < best; ++k) {
-            const uint32_t kk = k * k;
-            const bool up = y >= k, down = y + k < size;
-            if (!up && !down) break;
-            if (up) best = min(best, row_term(y - k, best, kk));
-            if (down) best = min(best, row_term(y + k, best, kk));
+    for (uint32_t base = 0; base < total; base += 64) {
+        const uint32_t g = base + lane;
+        // Owner word: the first lane whose inclusive sum exceeds g.
+        uint32_t w = 0;
+#pragma unroll
+        for (uint32_t step = 32; step > 0; step >>= 1) {
+            const uint32_t probe = uint32_t(__shfl(int(inclusive), int(w + step - 1)));
+            if (probe <= g) w += step;
         }
-        if (best > floor_value && best != kFar) local = best;
+        w = min(w, 63u);
+        const uint64_t bits = __shfl(my_bits, int(w));
+        const uint32_t rank = g - uint32_t(__shfl(int(exclusive), int(w)));
+        const uint32_t floor_value = max(known, local);
+        if (g < total) {
+            const uint32_t bit = nth_bit(bits, rank);
+            const size_t i = first + w;
+            const uint32_t tx = uint32_t(i % p.tiles), ty = uint32_t(i / p.tiles);
+            const uint32_t x = (tx << 3) | (bit & 7), y = (ty << 3) | (bit >> 3);
+            // Rounded squared row distance of row r, or kFar. Sparse views
+            // search the mask only as far as a site could still improve
+            // `best`; the rounding of d^2 above 2^24 is covered by the +2
+            // column margin.
+            auto row_term = [&](uint32_t r, uint32_t best, uint32_t kk) -> uint32_t {
+                uint32_t d;
+                if (dense) {
+                    const uint16_t v = dist[size_t(r) * size + x];
+                    d = v == kNoSite ? kFar : v;
+                } else {
+                    const uint32_t room = best == kFar ? kFar : best - kk;
+                    const uint32_t reach = room == kFar ? size : uint32_t(sqrtf(float(room))) + 2;
+                    d = nearest_in_row(to, p.tiles, x, r, reach);
+                }
+                return d == kFar ? kFar : kk + rounded_square(d);
+            };
+            uint32_t best = row_term(y, kFar, 0);
+            for (uint32_t k = 1; best > floor_value && uint64_t(k) * k < best; ++k) {
+                const uint32_t kk = k * k;
+                const bool up = y >= k, down = y + k < size;
+                if (!up && !down) break;
+                if (up) best = min(best, row_term(y - k, best, kk));
+                if (down) best = min(best, row_term(y + k, best, kk));
+            }
+            if (best > floor_value && best != kFar) local = best;
+        }
+        // Share the round's maximum so later rounds stop earlier.
+        local = wave_max(local);
     }
-    for (int o = 32; o > 0; o >>= 1) local = max(local, uint32_t(__shfl_xor(int(local), o)));
     if (lane == 0 && local > known) atomicMax(target, local);
 }
 
@@ -401,6 +460,8 @@ RasterGrid make_raster_grid(uint32_t tasks, uint32_t tiles, uint32_t max_faces) 
     const uint32_t wanted = (kFillBlocks + base - 1) / base;
     if (g.chunks < wanted)
         g.chunks = std::max(g.chunks, std::min(wanted, (max_faces + kRasterThreads - 1) / kRasterThreads));
+    // Beyond 8 chunks, a multiple of 8 (one chunk set per XCD).
+    if (g.chunks > 8) g.chunks = (g.chunks + 7) / 8 * 8;
     return g;
 }
 

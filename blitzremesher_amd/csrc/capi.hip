@@ -217,6 +217,11 @@ const DeviceMesh& mesh_of(const blitz_hip_audit* a, const blitz_hip_device_mesh*
     if (m->owner != a) throw std::invalid_argument("device mesh belongs to another audit");
     return *m->mesh;
 }
+
+ActionSet& actions_of(blitz_hip_actions* a) {
+    if (!a || !a->impl) throw std::invalid_argument("null action state");
+    return *a->impl;
+}
 }  // namespace
 
 extern "C" {
@@ -452,13 +457,6 @@ blitz_status blitz_hip_local_patches(const blitz_mesh* m, uint32_t count, uint64
 }
 
 // ---- collapse actions
-namespace {
-ActionSet& actions_of(blitz_hip_actions* a) {
-    if (!a || !a->impl) throw std::invalid_argument("null action state");
-    return *a->impl;
-}
-}  // namespace
-
 blitz_status blitz_hip_actions_create(int32_t device, const blitz_mesh* m, blitz_hip_actions** out, char* error,
                                       size_t capacity) {
     if (out) *out = nullptr;

@@ -177,46 +177,37 @@ __device__ __forceinline__ Setup shuffle_setup(const Setup& s, int lane) {
     return o;
 }
 
-// Lane path: walks the bounding box rows [ya, yb] sample by sample in one
-// flattened loop (lanes of a wave finish together after max(w*h) steps) and
-// collects coverage in an 8x8 bitmap relative to (x0, y0), bit
-// 8*(y-y0) + (x-x0). Per sample, only the x-dependent part of each edge
-// function is evaluated; the y-dependent products are recomputed when the
-// row changes. Every value is the reference expression's own rounding:
+// Lane path: walks the bounding box rows [ya, yb] two samples of a row per
+// step, in one flattened loop (lanes of a wave finish together after
+// max(rows * ceil(w/2)) steps), and collects coverage in an 8x8 bitmap
+// relative to (x0, y0), bit 8*(y-y0) + (x-x0). The y-dependent products are
+// recomputed every step, which costs less than a divergent row change. Every
+// value is the reference expression's own rounding:
 // u = (cx-bx)*(py-by) - (cy-by)*(px-bx), covered iff !(u+r0 < 0 || ...).
 __device__ __forceinline__ uint64_t lane_bitmap(const Setup& s, int ya, int yb) {
     const double au = s.cx - s.bx, bu = s.cy - s.by;
     const double av = s.ax - s.cx, bv = s.ay - s.cy;
     const double aw = s.bx - s.ax, bw = s.by - s.ay;
-    const int w = s.x1 - s.x0 + 1, n = w * (yb - ya + 1);
-    double py = double(ya) + .5;
-    double ru = au * (py - s.by), rv = av * (py - s.cy), rw = aw * (py - s.ay);
-    int x = s.x0, idx = (ya - s.y0) * 8;
-    uint32_t lo = 0, hi = 0;
+    const int n = ((s.x1 - s.x0 + 2) >> 1) * (yb - ya + 1);
+    int x = s.x0, y = ya;
+    uint64_t bits = 0;
     for (int k = 0; k < n; ++k) {
-        const double px = double(x) + .5;
-        const double u = ru - bu * (px - s.bx);
-        const double v = rv - bv * (px - s.cx);
-        const double q = rw - bw * (px - s.ax);
+        const double py = double(y) + .5;
+        const double ru = au * (py - s.by), rv = av * (py - s.cy), rw = aw * (py - s.ay);
+        const double px = double(x) + .5, qx = px + 1.0;
         // Bitwise | so that all three tests issue without branches.
-        const bool in = !((u + s.r0 < 0) | (v + s.r1 < 0) | (q + s.r2 < 0));
-        const uint32_t bit = uint32_t(in) << (idx & 31);
-        if (idx < 32)
-            lo |= bit;
-        else
-            hi |= bit;
-        ++x;
-        ++idx;
+        const bool in0 = !(((ru - bu * (px - s.bx)) + s.r0 < 0) | ((rv - bv * (px - s.cx)) + s.r1 < 0) |
+                           ((rw - bw * (px - s.ax)) + s.r2 < 0));
+        const bool in1 = (x < s.x1) & !(((ru - bu * (qx - s.bx)) + s.r0 < 0) | ((rv - bv * (qx - s.cx)) + s.r1 < 0) |
+                                         ((rw - bw * (qx - s.ax)) + s.r2 < 0));
+        bits |= uint64_t(uint32_t(in0) | (uint32_t(in1) << 1)) << (8 * (y - s.y0) + (x - s.x0));
+        x += 2;
         if (x > s.x1) {
             x = s.x0;
-            idx += 8 - w;
-            py += 1.0;
-            ru = au * (py - s.by);
-            rv = av * (py - s.cy);
-            rw = aw * (py - s.ay);
+            ++y;
         }
     }
-    return (uint64_t(hi) << 32) | lo;
+    return bits;
 }
 
 // Moves an 8x8 bitmap anchored at sample (x0, y0) into the up-to-four 8x8

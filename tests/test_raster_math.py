@@ -1,9 +1,10 @@
 """CPU checks of the raster's device arithmetic (blitzremesher_amd/csrc/
 raster_device.hpp): the whole-rectangle coverage classifier used for large
-faces must agree with the per-sample predicate (src/evaluate.cpp:166-168) on
-every sample. The functions are taken verbatim from the header, compiled for
-the host with g++ -ffp-contract=off and checked by brute force over seeded
-triangles, including corners on sample centres and strict (r = 0) edges."""
+faces and the lane path's two-samples-per-step bitmap must agree with the
+per-sample predicate (src/evaluate.cpp:166-168) on every sample. The
+functions are taken verbatim from the header, compiled for the host with
+g++ -ffp-contract=off and checked by brute force over seeded triangles,
+including corners on sample centres and strict (r = 0) edges."""
 from __future__ import annotations
 
 import shutil
@@ -75,6 +76,34 @@ int main(int argc, char** argv) {
                 if (kind == 1) { ++full; bad += in != n; }
             }
     }
+    // Lane path: the bitmap of a box of at most 8x8 samples, rows [ya, yb].
+    for (int it = 0; it < iterations; ++it) {
+        Setup s;
+        const double ox = std::uniform_int_distribution<int>(0, 40)(rng), oy = std::uniform_int_distribution<int>(0, 40)(rng);
+        auto c = [&](double o) {
+            double v = o + std::uniform_real_distribution<double>(0, 7.99)(rng);
+            if (it % 3 == 1) v = std::round(v * 2) / 2;
+            return v;
+        };
+        s.ax = c(ox); s.ay = c(oy); s.bx = c(ox); s.by = c(oy); s.cx = c(ox); s.cy = c(oy);
+        s.r0 = .5 * (fabs(s.cx - s.bx) + fabs(s.cy - s.by));
+        s.r1 = .5 * (fabs(s.ax - s.cx) + fabs(s.ay - s.cy));
+        s.r2 = .5 * (fabs(s.bx - s.ax) + fabs(s.by - s.ay));
+        if (it % 7 == 0) s.r0 = s.r1 = s.r2 = 0;
+        s.x0 = int(std::floor(std::min({s.ax, s.bx, s.cx})));
+        s.x1 = int(std::floor(std::max({s.ax, s.bx, s.cx})));
+        s.y0 = int(std::floor(std::min({s.ay, s.by, s.cy})));
+        s.y1 = int(std::floor(std::max({s.ay, s.by, s.cy})));
+        if (s.x1 - s.x0 > 7 || s.y1 - s.y0 > 7) continue;
+        const int ya = s.y0 + (it % 5 == 0), yb = s.y1 - (it % 4 == 0);
+        if (ya > yb) continue;
+        uint64_t want = 0;
+        for (int y = ya; y <= yb; ++y)
+            for (int x = s.x0; x <= s.x1; ++x)
+                if (covers(s, x, y)) want |= 1ull << (8 * (y - s.y0) + (x - s.x0));
+        ++rects;
+        bad += lane_bitmap(s, ya, yb) != want;
+    }
     printf("%ld %ld %ld %ld\n", rects, full, empty, bad);
     return 0;
 }
@@ -101,10 +130,15 @@ def _run(tmp_path: Path, source: str, iterations: int) -> tuple[int, int, int, i
     return tuple(int(v) for v in out)
 
 
-def _source(classify: str | None = None) -> str:
+def _source(plant: tuple[str, str] | None = None) -> str:
+    """The check program; `plant` = (text, replacement) plants a mistake in
+    the extracted copy of the header."""
     text = HEADER.read_text()
-    parts = [_function(text, "double edge("), _function(text, "bool covers("),
-             classify or _function(text, "int classify_rect("), _function(text, "uint64_t rect_word(")]
+    if plant:
+        assert plant[0] in text
+        text = text.replace(plant[0], plant[1])
+    parts = [_function(text, "double edge("), _function(text, "bool covers("), _function(text, "int classify_rect("),
+             _function(text, "uint64_t rect_word("), _function(text, "uint64_t lane_bitmap(")]
     return PRELUDE + "\n\n".join(parts) + CHECK
 
 
@@ -112,7 +146,7 @@ needs_gxx = pytest.mark.skipif(shutil.which("g++") is None, reason="g++ not avai
 
 
 @needs_gxx
-def test_classify_rect_agrees_with_every_sample(tmp_path):
+def test_device_raster_math_agrees_with_every_sample(tmp_path):
     rects, full, empty, bad = _run(tmp_path, _source(), 40000)
     assert bad == 0
     # Every class occurs, so each branch of the classifier is exercised.
@@ -120,12 +154,15 @@ def test_classify_rect_agrees_with_every_sample(tmp_path):
 
 
 @needs_gxx
-def test_classify_rect_check_catches_wrong_corner(tmp_path):
+@pytest.mark.parametrize("plant", [
     # Choosing the row end by the wrong sign of A claims coverage the
-    # per-sample predicate denies; the brute-force check must see it.
-    text = HEADER.read_text()
-    broken = _function(text, "int classify_rect(").replace("a >= 0 ? t1a : t1b, t1max = a >= 0 ? t1b : t1a",
-                                                           "a >= 0 ? t1b : t1a, t1max = a >= 0 ? t1a : t1b")
-    assert broken != _function(text, "int classify_rect(")
-    _, _, _, bad = _run(tmp_path, _source(broken), 4000)
+    # per-sample predicate denies.
+    ("a >= 0 ? t1a : t1b, t1max = a >= 0 ? t1b : t1a", "a >= 0 ? t1b : t1a, t1max = a >= 0 ? t1a : t1b"),
+    # The second sample of a step past the end of the row.
+    ("const bool in1 = (x < s.x1) &", "const bool in1 = (x <= s.x1) &"),
+    # The second sample tested at the first one's position.
+    ("qx = px + 1.0;", "qx = px;"),
+])
+def test_check_catches_planted_mistakes(tmp_path, plant):
+    _, _, _, bad = _run(tmp_path, _source(plant), 4000)
     assert bad > 0
