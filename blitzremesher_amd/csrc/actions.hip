@@ -26,6 +26,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cstring>
 #include <hipcub/device/device_radix_sort.hpp>
 #include <hipcub/device/device_run_length_encode.hpp>
 #include <hipcub/device/device_scan.hpp>
@@ -783,6 +784,7 @@ ActionSet::~ActionSet() {
     DeviceGuard guard(device_);
     (void)hipStreamSynchronize(stream_);
     (void)hipStreamDestroy(stream_);
+    (void)hipHostFree(staging_);
 }
 
 void ActionSet::download_legal(uint32_t* pairs) const {
@@ -930,15 +932,19 @@ std::vector<uint32_t> ActionSet::trial(const uint32_t* rows, size_t n, std::vect
     trial_copy_kernel<<<blocks_for(faces_), 256, 0, st>>>(next, keep, slot,
                                                           has_materials_ ? materials_.as<uint16_t>() : nullptr,
                                                           faces_, out, out_materials);
-    std::vector<uint32_t> host(3 * size_t(kept));
-    hip_check(hipMemcpyAsync(host.data(), out, host.size() * 4, hipMemcpyDeviceToHost, st), "download trial");
+    // One copy of indices and materials through pinned staging memory (sized
+    // for the state's whole mesh once).
+    const size_t bytes = size_t(kept) * (12 + (has_materials_ ? 2 : 0));
+    if (!staging_) hip_check(hipHostMalloc(&staging_, size_t(faces_) * 14, hipHostMallocDefault), "hipHostMalloc");
+    hip_check(hipMemcpyAsync(staging_, out, bytes, hipMemcpyDeviceToHost, st), "download trial");
+    hip_check(hipStreamSynchronize(st), "trial");
+    const auto* staged = static_cast<const uint32_t*>(staging_);
+    std::vector<uint32_t> host(staged, staged + 3 * size_t(kept));
     if (materials) {
         materials->assign(has_materials_ ? kept : 0, 0);
         if (has_materials_)
-            hip_check(hipMemcpyAsync(materials->data(), out_materials, size_t(kept) * 2, hipMemcpyDeviceToHost, st),
-                      "download trial materials");
+            std::memcpy(materials->data(), staged + 3 * size_t(kept), size_t(kept) * 2);
     }
-    hip_check(hipStreamSynchronize(st), "trial");
     return host;
 }
 

@@ -99,6 +99,7 @@ class ActionSet {
     DeviceBuffer legal_;  // LegalAction per legal action
     DeviceBuffer used_, selection_, scratch_a_, scratch_b_, scratch_c_, temp_;
     DeviceBuffer order_, rank_keys_, rank_idx_, claim_out_;
+    void* staging_ = nullptr;  // pinned host memory for trial downloads
 };
 
 // The gate of the executor: true when the candidate (index buffer, kept

@@ -122,11 +122,13 @@ void validate_settings(const Bounds& b, const Settings& s) {
 uint32_t raster_side(double screen, unsigned ss) { return uint32_t(std::ceil(screen + 8)) * ss; }
 
 DeviceMesh::~DeviceMesh() {
-    (void)hipFree(positions);
+    if (!borrowed_vertices) {
+        (void)hipFree(positions);
+        (void)hipFree(normals);
+        (void)hipFree(colors);
+    }
     (void)hipFree(indices);
     (void)hipFree(two_sided);
-    (void)hipFree(normals);
-    (void)hipFree(colors);
     (void)hipFree(materials);
 }
 
@@ -194,6 +196,8 @@ std::unique_ptr<DeviceMesh> Audit::upload(const HostMesh& m) {
     d->digest[1] = m.digest[1];
     d->pos_digest[0] = m.pos_digest[0];
     d->pos_digest[1] = m.pos_digest[1];
+    d->vertex_digest[0] = m.vertex_digest[0];
+    d->vertex_digest[1] = m.vertex_digest[1];
     d->serial = next_serial_++;
     std::vector<float4> pos(d->vertices);
     for (uint32_t i = 0; i < d->vertices; ++i)
@@ -249,6 +253,59 @@ std::unique_ptr<DeviceMesh> Audit::upload(const HostMesh& m) {
                   "upload materials");
     }
     hip_check(hipStreamSynchronize(stream_), "upload");
+    return d;
+}
+
+std::unique_ptr<DeviceMesh> Audit::derive(const DeviceMesh& base, const std::vector<uint32_t>& indices,
+                                          const std::vector<uint16_t>& materials,
+                                          const std::vector<uint8_t>& double_sided, const uint64_t digest[2]) {
+    if (base.owner != this) throw std::invalid_argument("base mesh belongs to another audit");
+    if (indices.size() % 3 || indices.size() / 3 >= UINT32_MAX)
+        throw std::invalid_argument("mesh requires 32-bit representable counts and whole triangles");
+    if (!materials.empty() && materials.size() != indices.size() / 3)
+        throw std::invalid_argument("material count differs from triangle count");
+    for (uint32_t v : indices)
+        if (v >= base.vertices) throw std::invalid_argument("index outside vertex stream");
+    DeviceGuard guard(device_);
+    auto d = std::make_unique<DeviceMesh>();
+    d->owner = this;
+    d->borrowed_vertices = true;
+    d->positions = base.positions;
+    d->normals = base.normals;
+    d->colors = base.colors;
+    d->vertices = base.vertices;
+    d->faces = uint32_t(indices.size() / 3);
+    d->digest[0] = digest[0];
+    d->digest[1] = digest[1];
+    d->pos_digest[0] = base.pos_digest[0];
+    d->pos_digest[1] = base.pos_digest[1];
+    d->vertex_digest[0] = base.vertex_digest[0];
+    d->vertex_digest[1] = base.vertex_digest[1];
+    d->serial = next_serial_++;
+    hip_check(hipMalloc(&d->indices, std::max<size_t>(1, indices.size()) * sizeof(uint32_t)), "hipMalloc");
+    if (!indices.empty())
+        hip_check(hipMemcpyAsync(d->indices, indices.data(), indices.size() * 4, hipMemcpyHostToDevice, stream_),
+                  "upload indices");
+    std::vector<uint8_t> two(d->faces, 0);
+    bool any = false;
+    if (!double_sided.empty())
+        for (uint32_t f = 0; f < d->faces; ++f) {
+            const uint16_t mat = materials.empty() ? 0 : materials[f];
+            two[f] = mat < double_sided.size() && double_sided[mat];
+            any |= two[f] != 0;
+        }
+    if (any) {
+        hip_check(hipMalloc(&d->two_sided, two.size()), "hipMalloc");
+        hip_check(hipMemcpyAsync(d->two_sided, two.data(), two.size(), hipMemcpyHostToDevice, stream_),
+                  "upload sidedness");
+    }
+    if (!materials.empty() && d->faces) {
+        hip_check(hipMalloc(&d->materials, size_t(d->faces) * 2), "hipMalloc");
+        hip_check(hipMemcpyAsync(d->materials, materials.data(), size_t(d->faces) * 2, hipMemcpyHostToDevice,
+                                 stream_),
+                  "upload materials");
+    }
+    hip_check(hipStreamSynchronize(stream_), "derive");
     return d;
 }
 

@@ -53,6 +53,7 @@ struct HostMesh {
     uint64_t digest[2] = {0, 0};      // streams compared by identical(): positions, normals, colors,
                                       // indices, materials, sidedness
     uint64_t pos_digest[2] = {0, 0};  // positions only: equal digests share one projection per view
+    uint64_t vertex_digest[2] = {0, 0};  // digest state after the vertex streams (positions, normals, colors)
 };
 
 class Audit;
@@ -70,7 +71,11 @@ struct DeviceMesh {
     uint32_t vertices = 0, faces = 0;
     uint64_t digest[2] = {0, 0};
     uint64_t pos_digest[2] = {0, 0};
+    uint64_t vertex_digest[2] = {0, 0};
     uint64_t serial = 0;
+    // Vertex streams (positions, normals, colours) owned by another device
+    // mesh of the same audit (derive()): not freed by this one.
+    bool borrowed_vertices = false;
     ~DeviceMesh();
 };
 
@@ -131,6 +136,14 @@ class Audit {
     Audit& operator=(const Audit&) = delete;
 
     std::unique_ptr<DeviceMesh> upload(const HostMesh& m);
+    // A mesh with `base`'s vertex streams (borrowed: `base` must outlive it)
+    // and its own triangles: `indices` (3 per face, each < base.vertices),
+    // per-face `materials` (empty or one per face) and the per-material
+    // sidedness table `double_sided`. `digest` is the identity digest of the
+    // whole mesh, as host_mesh computes it.
+    std::unique_ptr<DeviceMesh> derive(const DeviceMesh& base, const std::vector<uint32_t>& indices,
+                                       const std::vector<uint16_t>& materials,
+                                       const std::vector<uint8_t>& double_sided, const uint64_t digest[2]);
     Measurement evaluate(const DeviceMesh& reference, const DeviceMesh& candidate, const Bounds& b,
                          const Settings& s);
     std::vector<Measurement> evaluate_batch(const DeviceMesh& reference,

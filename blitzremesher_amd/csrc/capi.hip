@@ -76,19 +76,27 @@ template <class F> blitz_status guarded(char* error, size_t capacity, F&& body) 
     }
 }
 
-// Two independent 64-bit digests (FNV-1a and a multiplicative mix) over the
-// streams blitz's identical() compares (src/evaluate.cpp:77-82).
+// Two independent 64-bit digests (FNV-1a style and a multiplicative mix, over
+// 8-byte words, then the remaining bytes) of the streams blitz's identical()
+// compares (src/evaluate.cpp:77-82).
 struct Digest {
     uint64_t a = 1469598103934665603ull, b = 0x9E3779B97F4A7C15ull;
+    void word(uint64_t w) {
+        a = (a ^ w) * 1099511628211ull;
+        b = (b ^ w) * 0xff51afd7ed558ccdull;
+        b ^= b >> 29;
+    }
     void bytes(const void* p, size_t n) {
         const auto* c = static_cast<const unsigned char*>(p);
-        for (size_t i = 0; i < n; ++i) {
-            a = (a ^ c[i]) * 1099511628211ull;
-            b = (b ^ c[i]) * 0xff51afd7ed558ccdull;
-            b ^= b >> 29;
+        size_t i = 0;
+        for (; i + 8 <= n; i += 8) {
+            uint64_t w;
+            std::memcpy(&w, c + i, 8);
+            word(w);
         }
+        for (; i < n; ++i) word(c[i]);
     }
-    void value(uint64_t v) { bytes(&v, sizeof v); }
+    void value(uint64_t v) { word(v); }
 };
 
 bool valid_stream(const blitz_stream& s, size_t n, size_t element, bool required) {
@@ -100,7 +108,23 @@ bool valid_stream(const blitz_stream& s, size_t n, size_t element, bool required
 void digest_stream(Digest& d, const blitz_stream& s, size_t element) {
     d.value(s.count);
     const auto* base = static_cast<const unsigned char*>(s.data);
+    if (s.stride == element) {
+        d.bytes(base, s.count * element);
+        return;
+    }
     for (size_t i = 0; i < s.count; ++i) d.bytes(base + i * s.stride, element);
+}
+
+// The identity digest of a mesh from its vertex-stream digest state and its
+// triangle streams (the order host_mesh hashes them in).
+void digest_triangles(Digest& d, const uint32_t* indices, size_t index_count, const uint16_t* materials,
+                      size_t material_count, const uint8_t* double_sided, size_t double_sided_count) {
+    d.value(index_count);
+    d.bytes(indices, index_count * 4);
+    d.value(material_count);
+    d.bytes(materials, material_count * 2);
+    d.value(double_sided_count);
+    d.bytes(double_sided, double_sided_count);
 }
 
 HostMesh host_mesh(const blitz_mesh* m) {
@@ -144,12 +168,10 @@ HostMesh host_mesh(const blitz_mesh* m) {
     digest_stream(d, m->positions, 12);
     digest_stream(d, m->normals, 12);
     digest_stream(d, m->colors, 4);
-    d.value(m->index_count);
-    d.bytes(m->indices, m->index_count * 4);
-    d.value(m->material_count);
-    d.bytes(m->materials, m->material_count * 2);
-    d.value(m->double_sided_count);
-    d.bytes(m->double_sided, m->double_sided_count);
+    h.vertex_digest[0] = d.a;
+    h.vertex_digest[1] = d.b;
+    digest_triangles(d, m->indices, m->index_count, m->materials, m->material_count, m->double_sided,
+                     m->double_sided_count);
     h.digest[0] = d.a;
     h.digest[1] = d.b;
     Digest pd;
@@ -559,14 +581,17 @@ blitz_status blitz_hip_execute_actions(blitz_hip_audit* a, const blitz_mesh* inp
         }
         auto reference = audit.upload(source);
         uint64_t audits = 0;
-        // The gate: an exact audit of the candidate against the input.
+        // The gate: an exact audit of the candidate (the input's vertex
+        // streams, already on the device, with the trial's triangles)
+        // against the input.
         const ActionGate accept = [&](const std::vector<uint32_t>& idx, const std::vector<uint16_t>& mats) {
-            blitz_mesh d = *input;
-            d.indices = idx.data();
-            d.index_count = idx.size();
-            d.materials = mats.empty() ? nullptr : mats.data();
-            d.material_count = mats.size();
-            auto candidate = audit.upload(host_mesh(&d));
+            Digest d;
+            d.a = reference->vertex_digest[0];
+            d.b = reference->vertex_digest[1];
+            digest_triangles(d, idx.data(), idx.size(), mats.data(), mats.size(), source.double_sided.data(),
+                             source.double_sided.size());
+            const uint64_t digest[2] = {d.a, d.b};
+            auto candidate = audit.derive(*reference, idx, mats, source.double_sided, digest);
             ++audits;
             return audit.evaluate(*reference, *candidate, bounds, settings).passed;
         };
