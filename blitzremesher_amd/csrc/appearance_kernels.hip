@@ -344,7 +344,7 @@ __device__ __forceinline__ double load_max(const unsigned long long* p) {
     return __longlong_as_double((long long)__hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
 }
 
-constexpr uint32_t kReduceBlocks = 256;  // blocks per raster pair of the reductions
+constexpr uint32_t kReduceBlocks = 2048;  // blocks per raster pair of the streaming reductions
 
 // Grid (blocks, pairs); each block strides over the pair's samples.
 __global__ __launch_bounds__(256) void normal_angle_kernel(const AttrPair* __restrict__ pairs) {
@@ -380,16 +380,19 @@ __device__ __forceinline__ double sample_cost(float4 na, float4 ca, float4 nb, f
     return cost;
 }
 
-// Grid (blocks, pairs, 2 directions); each thread strides over samples of
-// the source raster and, for each visible one, finds the reference's
-// windowed minimum cost (src/evaluate.cpp:229-262). out[1] holds the largest
+// Grid (blocks, pairs, 2 directions), launched twice (phase 0: samples whose
+// own position is not visible in the other raster; phase 1: the others);
+// each thread strides over samples of the source raster and, for each
+// visible one, finds the reference's windowed minimum cost
+// (src/evaluate.cpp:229-262). out[1] holds the largest
 // minimum found so far, shared by both directions as the reference's
 // `maximum` is. A sample whose running minimum is already <= that value
 // cannot raise it and stops (the early break of Taha and Hanbury, "An
 // efficient algorithm for calculating the exact Hausdorff distance", IEEE
 // TPAMI 2015): out[1] ends at the reference's maximum whenever that is within
 // limit^2, and above limit^2 (an infinite distance) otherwise.
-__global__ __launch_bounds__(256) void attributed_kernel(const AttrPair* __restrict__ pairs, CostSettings s) {
+__global__ __launch_bounds__(256) void attributed_kernel(const AttrPair* __restrict__ pairs, CostSettings s,
+                                                         uint32_t phase) {
     const AttrPair& p = pairs[blockIdx.y];
     const bool reverse = blockIdx.z != 0;
     const float4* fn = reverse ? p.normal_c : p.normal_a;
@@ -406,9 +409,13 @@ __global__ __launch_bounds__(256) void attributed_kernel(const AttrPair* __restr
     const float4 none = make_float4(0, 0, 0, 0);
     for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
         if (fn[i].w == 0) continue;
+        // Phase 0 takes the samples whose own position is not visible in the
+        // other raster (the likeliest largest minima), phase 1 the rest.
+        const bool seen = tn[i].w != 0;
+        if (seen != (phase != 0)) continue;
         const float4 an = fn[i], ac = colored ? fc[i] : none;
         double best = INFINITY;
-        if (tn[i].w != 0) best = sample_cost(an, ac, tn[i], colored ? tc[i] : none, 0, s);
+        if (seen) best = sample_cost(an, ac, tn[i], colored ? tc[i] : none, 0, s);
         if (best <= 1e-18) continue;
         double known = load_max(slot);
         if (known > limit2) return;  // the distance is already infinite
@@ -458,7 +465,9 @@ void launch_attribute_raster(const AttrTask* tasks, uint32_t count, uint32_t til
     attribute_kernel<<<dim3(blocks, count), dim3(64 * kWaves), 0, stream>>>(tasks);
 }
 
-uint32_t reduce_blocks(uint32_t tiles) { return std::min(kReduceBlocks, (tiles * tiles * 64 + 255) / 256); }
+// About eight samples per thread, at most 2048 blocks per raster pair, so a
+// few views still fill the chip.
+uint32_t reduce_blocks(uint32_t tiles) { return std::min(kReduceBlocks, (tiles * tiles * 64 + 2047) / 2048); }
 
 void launch_overlap(const AttrTask* tasks, uint32_t count, uint32_t tiles, unsigned long long* sums,
                     hipStream_t stream) {
@@ -475,7 +484,11 @@ void launch_normal_angle(const AttrPair* pairs, uint32_t count, uint32_t tiles, 
 void launch_attributed(const AttrPair* pairs, uint32_t count, uint32_t tiles, const CostSettings& s,
                        hipStream_t stream) {
     if (!count) return;
-    attributed_kernel<<<dim3(reduce_blocks(tiles), count, 2), dim3(256), 0, stream>>>(pairs, s);
+    // One thread per 4 samples: the windowed searches are latency-bound
+    // gathers, so the launch keeps many waves resident.
+    const uint32_t blocks = std::min<uint32_t>(16384, (tiles * tiles * 64 + 1023) / 1024);
+    for (uint32_t phase = 0; phase < 2; ++phase)
+        attributed_kernel<<<dim3(blocks, count, 2), dim3(256), 0, stream>>>(pairs, s, phase);
 }
 
 }  // namespace blitz_hip

@@ -382,10 +382,21 @@ std::vector<Measurement> Audit::evaluate_batch(const DeviceMesh& reference,
         for (auto& st : state[c]) st.ss = s.supersample;
     }
     DeviceGuard guard(device_);
+    // Views per candidate and round. The reference walks its views in order
+    // and stops at the first failing one (src/evaluate.cpp:319-350). Under
+    // the appearance profiles (attribute rasters and windowed searches) a
+    // round takes the first 4 pending views, so work after an early failure
+    // is skipped; a fully passing audit then takes more, smaller rounds.
+    // The coverage profile takes every pending view at once. Results do not
+    // depend on the window.
+    const uint32_t window = attributed ? 4u : 0u;
     for (;;) {
+        // Per candidate, the first `window` pending views in view order (all
+        // of them for window 0), stopping at a settled failing view.
         std::vector<Item> items;
         for (size_t c = 0; c < n; ++c) {
             if (identical[c]) continue;
+            uint32_t taken = 0;
             for (uint32_t v = 0; v < views; ++v) {
                 State& st = state[c][v];
                 if (st.final) {
@@ -398,25 +409,32 @@ std::vector<Measurement> Audit::evaluate_batch(const DeviceMesh& reference,
                     break;
                 }
                 items.push_back({uint32_t(c), v, st.ss});
+                if (window && ++taken == window) break;
             }
         }
         if (items.empty()) break;
-        // All pending views of one round share one supersampling: every view
-        // starts at s.supersample and refines by the same doubling rule.
-        std::vector<ViewResult> results;
-        run_items(reference, candidates, cams, b, s, items, results);
-        ++timing_.passes;
-        for (size_t i = 0; i < items.size(); ++i) {
-            const Item& it = items[i];
-            State& st = state[it.candidate][it.view];
-            const ViewResult& res = results[i];
-            st.m = attributed ? finish_view_attributed(res.coverage, res.changed, res.total, res.clipped,
-                                                       res.normal_acos, res.attributed, st.m.normal_degrees, s, it.ss)
-                              : finish_view(res.coverage, res.changed, res.total, res.clipped, s, it.ss);
-            if (settled(st.m, s, it.ss))
-                st.final = true;
-            else
-                st.ss = uint8_t(std::min<unsigned>(s.max_supersample, unsigned(it.ss) * 2));
+        // One pass per supersampling among the round's items (with a window,
+        // views refined earlier run beside views seen for the first time).
+        std::map<uint8_t, std::vector<Item>> by_ss;
+        for (const Item& it : items) by_ss[it.ss].push_back(it);
+        for (const auto& group : by_ss) {
+            const std::vector<Item>& pass = group.second;
+            std::vector<ViewResult> results;
+            run_items(reference, candidates, cams, b, s, pass, results);
+            ++timing_.passes;
+            for (size_t i = 0; i < pass.size(); ++i) {
+                const Item& it = pass[i];
+                State& st = state[it.candidate][it.view];
+                const ViewResult& res = results[i];
+                st.m = attributed ? finish_view_attributed(res.coverage, res.changed, res.total, res.clipped,
+                                                           res.normal_acos, res.attributed, st.m.normal_degrees, s,
+                                                           it.ss)
+                                  : finish_view(res.coverage, res.changed, res.total, res.clipped, s, it.ss);
+                if (settled(st.m, s, it.ss))
+                    st.final = true;
+                else
+                    st.ss = uint8_t(std::min<unsigned>(s.max_supersample, unsigned(it.ss) * 2));
+            }
         }
     }
     // Sequential assembly, exactly as the reference's view loop.

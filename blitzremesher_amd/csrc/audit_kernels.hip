@@ -49,7 +49,7 @@ __global__ __launch_bounds__(256) void project_kernel(const ProjectTask* __restr
 // undecided (classify_rect), then produces each undecided tile's word with a
 // ballot.
 #ifndef BLITZ_LANE_SAMPLES
-#define BLITZ_LANE_SAMPLES 24  // build-time tuning override
+#define BLITZ_LANE_SAMPLES 40  // build-time tuning override
 #endif
 constexpr int kLaneSamples = BLITZ_LANE_SAMPLES;
 

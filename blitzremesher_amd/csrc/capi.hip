@@ -581,6 +581,7 @@ blitz_status blitz_hip_execute_actions(blitz_hip_audit* a, const blitz_mesh* inp
         }
         auto reference = audit.upload(source);
         uint64_t audits = 0;
+
         // The gate: an exact audit of the candidate (the input's vertex
         // streams, already on the device, with the trial's triangles)
         // against the input.
