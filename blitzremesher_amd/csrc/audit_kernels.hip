@@ -246,12 +246,25 @@ __global__ __launch_bounds__(256) void compare_kernel(const ViewPair* __restrict
     cc = wave_sum(cc);
     changed = wave_sum(changed);
     total = wave_sum(total);
-    if ((threadIdx.x & 63) == 0 && (ca | cc | changed | total)) {
+    // One set of atomics per block: the four waves' sums meet in LDS.
+    __shared__ unsigned long long part[4][4];
+    const uint32_t wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        part[wave][0] = ca;
+        part[wave][1] = cc;
+        part[wave][2] = changed;
+        part[wave][3] = total;
+    }
+    __syncthreads();
+    if (threadIdx.x < 4) {
+        const unsigned long long v = part[0][threadIdx.x] + part[1][threadIdx.x] + part[2][threadIdx.x] +
+                                     part[3][threadIdx.x];
         ViewSummary& s = summaries[blockIdx.y];
-        atomicAdd(&s.count_a, ca);
-        atomicAdd(&s.count_c, cc);
-        atomicAdd(&s.changed, changed);
-        atomicAdd(&s.total, total);
+        unsigned long long* field = threadIdx.x == 0   ? &s.count_a
+                                    : threadIdx.x == 1 ? &s.count_c
+                                    : threadIdx.x == 2 ? &s.changed
+                                                       : &s.total;
+        if (v) atomicAdd(field, v);
     }
 }
 
@@ -483,7 +496,7 @@ void launch_compare(const ViewPair* pairs, ViewSummary* summaries, uint32_t coun
                     hipStream_t stream) {
     if (!count) return;
     const size_t words = size_t(tiles) * tiles;
-    const uint32_t per_block = 256 * 16;
+    const uint32_t per_block = 256 * 4;
     const uint32_t blocks = uint32_t((words + per_block - 1) / per_block);
     compare_kernel<<<dim3(blocks, count), dim3(256), 0, stream>>>(pairs, summaries, per_block);
 }
