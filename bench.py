@@ -25,6 +25,7 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 vector (and matrix) peak
 BF16_MFMA_PEAK_TFLOPS = 2500.0  # dense
 FP32_PEAK_TFLOPS = 157.3
 METRIC = "audited collapses/s & ms per LOD chain at ≤1px error; raster audit % HBM roof"
@@ -57,6 +58,39 @@ def algorithmic_bytes(ref, cand, views: int, samples: int, distance_views: int) 
     project = 32 * views * (ref.positions.shape[0] + (0 if shared else cand.positions.shape[0]))
     return {"geometry": geometry, "masks": masks, "raster": geometry + masks, "project": project,
             "distance": distance_views * 8 * samples, "audit": geometry + 2 * masks + distance_views * 8 * samples}
+
+
+def raster_fp64_ops(meshes_, center, radius, settings) -> int:
+    """FP64 operations the reference's coverage raster performs for one audit
+    (src/evaluate.cpp:144-170), both meshes over every view: per drawn face 19
+    (signed area 7, three conservative radii 4 each) and per sample of its
+    clipped bounding box 26 (sample centre 2, three edge functions 7 each,
+    three radius additions); comparisons are not counted."""
+    import numpy as np
+    from blitzremesher_amd.audit import cameras
+    cams = cameras(center, radius, settings.screen_size, settings.orthographic, settings.perspective,
+                   settings.rotation_seed)
+    size = int(-(-(settings.screen_size + 8) // 1)) * settings.supersample
+    total = 0
+    for mesh in meshes_:
+        p = mesh.positions.astype(np.float64) - np.asarray(center, np.float64)
+        idx = mesh.indices.reshape(-1, 3).astype(np.int64)
+        for cam in cams:
+            r, u = np.asarray(cam.right, np.float64), np.asarray(cam.up, np.float64)
+            x = (p[:, 0] * r[0] + p[:, 1] * r[1] + p[:, 2] * r[2]) * cam.scale * settings.supersample + size * .5
+            y = (p[:, 0] * u[0] + p[:, 1] * u[1] + p[:, 2] * u[2]) * cam.scale * settings.supersample + size * .5
+            fx, fy = x[idx], y[idx]
+            area = (fx[:, 1] - fx[:, 0]) * (fy[:, 2] - fy[:, 0]) - (fy[:, 1] - fy[:, 0]) * (fx[:, 2] - fx[:, 0])
+            drawn = np.abs(area) >= 1e-16
+            if not settings.force_two_sided:
+                drawn &= area > 0
+            x0 = np.clip(np.floor(fx.min(1)), 0, size - 1)
+            x1 = np.clip(np.floor(fx.max(1)), 0, size - 1)
+            y0 = np.clip(np.floor(fy.min(1)), 0, size - 1)
+            y1 = np.clip(np.floor(fy.max(1)), 0, size - 1)
+            box = (x1 - x0 + 1) * (y1 - y0 + 1)
+            total += int(19 * drawn.sum() + 26 * box[drawn].sum())
+    return total
 
 
 def cpu_baseline(ref, cand, center, radius, settings, sample_views: int):
@@ -429,6 +463,10 @@ def main() -> int:
         kernels = {"raster_band_kernel": band_ms / band_launches, "project_kernel": project_ms / band_launches,
                    "compare_kernel": compare_ms / launches, "distance_kernels": distance_ms / launches}
         dominant = max(kernels, key=kernels.get)
+        # The raster's secondary (compute) roof: the reference algorithm's FP64
+        # operations per launch over the FP64 vector peak.
+        fp64_ops = raster_fp64_ops((ref, cand), center, radius, settings)
+        fp64_tflops = fp64_ops / (kernels["raster_band_kernel"] * 1e-3) / 1e12
         per_launch_bytes = {"raster_band_kernel": b["raster"], "project_kernel": b["project"],
                             "compare_kernel": 2 * b["masks"], "distance_kernels": b["distance"]}[dominant]
         achieved = per_launch_bytes / (kernels[dominant] * 1e-3) / 1e9
@@ -463,7 +501,10 @@ def main() -> int:
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": measured_traffic(),
                      "algorithmic_bytes_per_launch": per_launch_bytes,
                      "kernel_ms_per_launch": {k: round(v, 4) for k, v in kernels.items()},
-                     "raster_pass_ms": round(raster_ms / launches, 4)},
+                     "raster_pass_ms": round(raster_ms / launches, 4),
+                     "fp64_vector": {"kernel": "raster_band_kernel", "ops_per_launch": fp64_ops,
+                                     "achieved": round(fp64_tflops, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                                     "frac": round(fp64_tflops / FP64_PEAK_TFLOPS, 4)}},
         # Each collapse of the candidate removes one vertex from use.
         "collapses_per_audit": collapses_full,
         "audited_collapses_per_s": round(collapses_full * 1000.0 / ms_per_step, 1),
