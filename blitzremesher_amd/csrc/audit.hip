@@ -170,6 +170,7 @@ Audit::~Audit() {
     for (auto& s : slots_) (void)hipFree(s.first);
     for (auto& e : events_) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(stream_);
+    (void)hipHostFree(pinned_status_);
 }
 
 void* Audit::scratch(size_t slot, size_t bytes) {
@@ -480,9 +481,17 @@ std::vector<Measurement> Audit::evaluate_batch(const DeviceMesh& reference,
     return out;
 }
 
-void Audit::rasterize(const std::vector<RasterJob>& jobs, const std::vector<Camera>& cams, const Bounds& b,
-                      uint8_t ss, uint32_t size, bool force_two, std::vector<uint32_t>& status_out,
-                      std::vector<RasterTask>* drawn, std::vector<const double*>* depth) {
+void Audit::launch_tasks(const std::vector<RasterTask>& list, size_t slot, uint32_t tiles) {
+    RasterTask* d = scratch_as<RasterTask>(slot, std::max<size_t>(1, list.size()));
+    hip_check(hipMemcpyAsync(d, list.data(), list.size() * sizeof(RasterTask), hipMemcpyHostToDevice, stream_),
+              "upload raster tasks");
+    uint32_t faces = 0;
+    for (const auto& t : list) faces = std::max(faces, t.faces);
+    launch_raster(d, make_raster_grid(uint32_t(list.size()), tiles, faces), tiles, stream_);
+}
+
+void Audit::raster_launch(const std::vector<RasterJob>& jobs, const std::vector<Camera>& cams, const Bounds& b,
+                          uint8_t ss, uint32_t size, bool force_two, bool want_depth, RasterLaunch& out) {
     const uint32_t tiles = (size + kTile - 1) / kTile;
     const size_t words = size_t(tiles) * tiles;
     // One projection per (vertex stream contents, view).
@@ -516,14 +525,14 @@ void Audit::rasterize(const std::vector<RasterJob>& jobs, const std::vector<Came
     double2* xy = scratch_as<double2>(9, std::max<size_t>(1, offset.back()));
     // View depths: perspective views divide by them; attribute rasters
     // interpolate them in every view.
-    double* zs = perspective || drawn ? scratch_as<double>(10, std::max<size_t>(1, offset.back())) : nullptr;
+    double* zs = perspective || want_depth ? scratch_as<double>(10, std::max<size_t>(1, offset.back())) : nullptr;
     std::vector<ProjectTask> projections(sets.size());
     for (size_t i = 0; i < sets.size(); ++i) {
         ProjectTask& p = projections[i];
         p = {};
         p.positions = sets[i].first->positions;
         p.xy = xy + offset[i];
-        p.z = cams[sets[i].second].perspective || drawn ? zs + offset[i] : nullptr;
+        p.z = cams[sets[i].second].perspective || want_depth ? zs + offset[i] : nullptr;
         p.camera = cams[sets[i].second];
         p.center = b.center;
         p.vertices = sets[i].first->vertices;
@@ -554,14 +563,6 @@ void Audit::rasterize(const std::vector<RasterJob>& jobs, const std::vector<Came
         init[2 * j] = kNone;
         init[2 * j + 1] = 0;
     }
-    auto launch = [&](const std::vector<RasterTask>& list, size_t slot_id) {
-        RasterTask* d = scratch_as<RasterTask>(slot_id, std::max<size_t>(1, list.size()));
-        hip_check(hipMemcpyAsync(d, list.data(), list.size() * sizeof(RasterTask), hipMemcpyHostToDevice, stream_),
-                  "upload raster tasks");
-        uint32_t faces = 0;
-        for (const auto& t : list) faces = std::max(faces, t.faces);
-        launch_raster(d, make_raster_grid(uint32_t(list.size()), tiles, faces), tiles, stream_);
-    };
     hip_check(hipMemcpyAsync(status, init.data(), init.size() * 4, hipMemcpyHostToDevice, stream_), "upload status");
     ProjectTask* dp = scratch_as<ProjectTask>(8, std::max<size_t>(1, projections.size()));
     hip_check(hipMemcpyAsync(dp, projections.data(), projections.size() * sizeof(ProjectTask), hipMemcpyHostToDevice,
@@ -570,13 +571,35 @@ void Audit::rasterize(const std::vector<RasterJob>& jobs, const std::vector<Came
     hip_check(hipEventRecord(events_[5], stream_), "event");
     launch_project(dp, uint32_t(projections.size()), max_vertices, stream_);
     hip_check(hipEventRecord(events_[6], stream_), "event");
-    launch(tasks, 4);
+    launch_tasks(tasks, 4, tiles);
     hip_check(hipEventRecord(events_[7], stream_), "event");
     hip_check(hipGetLastError(), "raster launch");
-    status_out.assign(jobs.size() * 2, 0);
-    hip_check(hipMemcpyAsync(status_out.data(), status, status_out.size() * 4, hipMemcpyDeviceToHost, stream_),
+    // Status words go to pinned memory without a wait: the caller may queue
+    // more work on the masks before raster_finish() reads them.
+    if (pinned_status_words_ < init.size()) {
+        hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+        (void)hipHostFree(pinned_status_);
+        pinned_status_ = nullptr;
+        pinned_status_words_ = 0;
+        hip_check(hipHostMalloc(reinterpret_cast<void**>(&pinned_status_), std::max<size_t>(64, init.size()) * 4,
+                                hipHostMallocDefault),
+                  "hipHostMalloc");
+        pinned_status_words_ = std::max<size_t>(64, init.size());
+    }
+    hip_check(hipMemcpyAsync(pinned_status_, status, init.size() * 4, hipMemcpyDeviceToHost, stream_),
               "download status");
-    hip_check(hipStreamSynchronize(stream_), "raster");
+    out.tasks = std::move(tasks);
+    out.words = words;
+    out.tiles = tiles;
+    out.depth.assign(jobs.size(), nullptr);
+    if (zs)
+        for (size_t j = 0; j < jobs.size(); ++j) out.depth[j] = zs + offset[job_slot[j]];
+}
+
+bool Audit::raster_finish(RasterLaunch& l, std::vector<uint32_t>& status_out, std::vector<RasterTask>* drawn,
+                          std::vector<const double*>* depth) {
+    const size_t n = l.tasks.size();
+    status_out.assign(pinned_status_, pinned_status_ + 2 * n);
     {
         float t_p = 0, t_b = 0;
         hip_check(hipEventElapsedTime(&t_p, events_[5], events_[6]), "elapsed");
@@ -585,33 +608,40 @@ void Audit::rasterize(const std::vector<RasterJob>& jobs, const std::vector<Came
         timing_.band_ms += t_b;
         ++timing_.band_launches;
     }
-    timing_.raster_tasks += uint32_t(jobs.size());
-    for (const auto& t : tasks) timing_.raster_triangles += t.faces;
-    timing_.mask_bytes += uint64_t(jobs.size()) * words * 8;
+    timing_.raster_tasks += uint32_t(n);
+    for (const auto& t : l.tasks) timing_.raster_triangles += t.faces;
+    timing_.mask_bytes += uint64_t(n) * l.words * 8;
     // The reference stops rasterizing at the first out-of-frame face
     // (src/evaluate.cpp:159-160): redraw such masks from the faces before it.
     std::vector<RasterTask> redo;
-    for (size_t j = 0; j < jobs.size(); ++j)
+    for (size_t j = 0; j < n; ++j)
         if (status_out[2 * j] != kNone) {
-            RasterTask t = tasks[j];
+            RasterTask t = l.tasks[j];
             t.faces = status_out[2 * j];
             redo.push_back(t);
-            launch_fill(t.mask, words, 0, stream_);
+            launch_fill(t.mask, l.words, 0, stream_);
         }
     if (!redo.empty()) {
-        launch(redo, 7);
+        launch_tasks(redo, 7, l.tiles);
         hip_check(hipGetLastError(), "raster redo launch");
         hip_check(hipStreamSynchronize(stream_), "raster redo");
     }
     if (drawn) {
-        *drawn = tasks;
-        for (size_t j = 0; j < jobs.size(); ++j)
+        *drawn = l.tasks;
+        for (size_t j = 0; j < n; ++j)
             if (status_out[2 * j] != kNone) (*drawn)[j].faces = status_out[2 * j];
     }
-    if (depth) {
-        depth->resize(jobs.size());
-        for (size_t j = 0; j < jobs.size(); ++j) (*depth)[j] = zs + offset[job_slot[j]];
-    }
+    if (depth) *depth = l.depth;
+    return !redo.empty();
+}
+
+void Audit::rasterize(const std::vector<RasterJob>& jobs, const std::vector<Camera>& cams, const Bounds& b,
+                      uint8_t ss, uint32_t size, bool force_two, std::vector<uint32_t>& status_out,
+                      std::vector<RasterTask>* drawn, std::vector<const double*>* depth) {
+    RasterLaunch l;
+    raster_launch(jobs, cams, b, ss, size, force_two, drawn != nullptr || depth != nullptr, l);
+    hip_check(hipStreamSynchronize(stream_), "raster");
+    raster_finish(l, status_out, drawn, depth);
 }
 
 void Audit::run_items(const DeviceMesh& reference, const std::vector<const DeviceMesh*>& candidates,
@@ -690,14 +720,27 @@ void Audit::run_items(const DeviceMesh& reference, const std::vector<const Devic
         std::vector<uint32_t> st;
         std::vector<RasterTask> drawn;
         std::vector<const double*> zdepth;
-        rasterize(jobs, cams, b, ss, size, s.force_two_sided, st, attributed ? &drawn : nullptr,
-                  attributed ? &zdepth : nullptr);
+        // Plain coverage audits queue the summaries and distances right
+        // behind the raster and read its status words with them (one wait);
+        // masks cut at an out-of-frame face are then redrawn and measured
+        // again. Appearance rasters and the reference cache need the final
+        // masks first.
+        const bool deferred = !attributed && !cached;
+        RasterLaunch rl;
+        if (deferred)
+            raster_launch(jobs, cams, b, ss, size, s.force_two_sided, false, rl);
+        else
+            rasterize(jobs, cams, b, ss, size, s.force_two_sided, st, attributed ? &drawn : nullptr,
+                      attributed ? &zdepth : nullptr);
         // Reference clip flags, and new reference masks into the cache while
         // it stays within a quarter of the memory budget.
         std::vector<uint8_t> ref_clipped(nref, 0);
-        for (uint32_t r = 0; r < nref; ++r)
-            ref_clipped[r] = hit[r] ? hit[r]->clipped
-                                    : (st[2 * ref_job[r]] != kNone || st[2 * ref_job[r] + 1] != 0);
+        const auto clip_flags = [&] {
+            for (uint32_t r = 0; r < nref; ++r)
+                ref_clipped[r] = hit[r] ? hit[r]->clipped
+                                        : (st[2 * ref_job[r]] != kNone || st[2 * ref_job[r] + 1] != 0);
+        };
+        if (!deferred) clip_flags();
         if (cached)
             for (auto& kv : ref_slot) {
                 const uint32_t r = kv.second;
@@ -713,12 +756,15 @@ void Audit::run_items(const DeviceMesh& reference, const std::vector<const Devic
         hip_check(hipEventRecord(events_[1], stream_), "event");
         hip_check(hipMemcpyAsync(dpairs, pairs.data(), pairs.size() * sizeof(ViewPair), hipMemcpyHostToDevice, stream_),
                   "upload pairs");
-        launch_fill(reinterpret_cast<uint64_t*>(summaries), ncand * sizeof(ViewSummary) / 8, 0, stream_);
-        launch_compare(dpairs, summaries, ncand, tiles, stream_);
-        hip_check(hipEventRecord(events_[2], stream_), "event");
-        launch_distance(dpairs, summaries, ncand, size, tiles, stream_);
-        hip_check(hipEventRecord(events_[3], stream_), "event");
-        hip_check(hipGetLastError(), "audit launch");
+        const auto measure = [&] {
+            launch_fill(reinterpret_cast<uint64_t*>(summaries), ncand * sizeof(ViewSummary) / 8, 0, stream_);
+            launch_compare(dpairs, summaries, ncand, tiles, stream_);
+            hip_check(hipEventRecord(events_[2], stream_), "event");
+            launch_distance(dpairs, summaries, ncand, size, tiles, stream_);
+            hip_check(hipEventRecord(events_[3], stream_), "event");
+            hip_check(hipGetLastError(), "audit launch");
+        };
+        measure();
         // Appearance profiles: attribute rasters of every job, then per item
         // the normal angle (src/evaluate.cpp:282-290).
         const size_t per = words * 64;
@@ -753,6 +799,16 @@ void Audit::run_items(const DeviceMesh& reference, const std::vector<const Devic
         hip_check(hipMemcpyAsync(sum.data(), summaries, ncand * sizeof(ViewSummary), hipMemcpyDeviceToHost, stream_),
                   "download summaries");
         hip_check(hipStreamSynchronize(stream_), "audit");
+        if (deferred) {
+            if (raster_finish(rl, st)) {
+                measure();
+                hip_check(hipMemcpyAsync(sum.data(), summaries, ncand * sizeof(ViewSummary), hipMemcpyDeviceToHost,
+                                         stream_),
+                          "download summaries");
+                hip_check(hipStreamSynchronize(stream_), "audit");
+            }
+            clip_flags();
+        }
         float t_r = 0, t_c = 0, t_d = 0;
         hip_check(hipEventElapsedTime(&t_r, events_[0], events_[1]), "elapsed");
         hip_check(hipEventElapsedTime(&t_c, events_[1], events_[2]), "elapsed");

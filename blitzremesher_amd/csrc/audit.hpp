@@ -210,6 +210,22 @@ class Audit {
     void rasterize(const std::vector<RasterJob>& jobs, const std::vector<Camera>& cams, const Bounds& b, uint8_t ss,
                    uint32_t size, bool force_two, std::vector<uint32_t>& status,
                    std::vector<RasterTask>* drawn = nullptr, std::vector<const double*>* depth = nullptr);
+    // rasterize() in two halves. raster_launch queues the projections, the
+    // band raster and the copy of the status words into pinned host memory,
+    // and waits for nothing; raster_finish, called once the stream has been
+    // synchronised, reads the status words, redraws the masks cut at an
+    // out-of-frame face (launch and wait) and says whether there were any.
+    struct RasterLaunch {
+        std::vector<RasterTask> tasks;
+        std::vector<const double*> depth;  // per job: its vertex set's view depths, when requested
+        size_t words = 0;
+        uint32_t tiles = 0;
+    };
+    void raster_launch(const std::vector<RasterJob>& jobs, const std::vector<Camera>& cams, const Bounds& b,
+                       uint8_t ss, uint32_t size, bool force_two, bool want_depth, RasterLaunch& out);
+    bool raster_finish(RasterLaunch& l, std::vector<uint32_t>& status, std::vector<RasterTask>* drawn = nullptr,
+                       std::vector<const double*>* depth = nullptr);
+    void launch_tasks(const std::vector<RasterTask>& list, size_t slot, uint32_t tiles);
     // Per (job, tile) bins of the faces each drawn job draws, face ids into
     // `faces`; job j's cells start at offsets + j*tiles*tiles.
     void build_bins(const std::vector<RasterTask>& drawn, const unsigned long long*& offsets,
@@ -249,6 +265,8 @@ class Audit {
     int device_;
     uint64_t memory_bytes_;
     bool cache_rasters_ = false;
+    uint32_t* pinned_status_ = nullptr;  // status words of the last raster launch (pinned host memory)
+    size_t pinned_status_words_ = 0;
     RasterCache cache_;
     hipStream_t stream_{};
     hipEvent_t events_[10]{};
