@@ -720,12 +720,12 @@ void Audit::run_items(const DeviceMesh& reference, const std::vector<const Devic
         std::vector<uint32_t> st;
         std::vector<RasterTask> drawn;
         std::vector<const double*> zdepth;
-        // Plain coverage audits queue the summaries and distances right
-        // behind the raster and read its status words with them (one wait);
-        // masks cut at an out-of-frame face are then redrawn and measured
-        // again. Appearance rasters and the reference cache need the final
-        // masks first.
-        const bool deferred = !attributed && !cached;
+        // Coverage audits queue the summaries and distances (and the copies
+        // of new reference masks into the cache) right behind the raster and
+        // read its status words with them (one wait); masks cut at an
+        // out-of-frame face are then redrawn, copied and measured again.
+        // Appearance rasters need the final masks first.
+        const bool deferred = !attributed;
         RasterLaunch rl;
         if (deferred)
             raster_launch(jobs, cams, b, ss, size, s.force_two_sided, false, rl);
@@ -741,18 +741,28 @@ void Audit::run_items(const DeviceMesh& reference, const std::vector<const Devic
                                         : (st[2 * ref_job[r]] != kNone || st[2 * ref_job[r] + 1] != 0);
         };
         if (!deferred) clip_flags();
-        if (cached)
+        // Cache entries made by this pass (reference slot, entry); their clip
+        // flags are set once the status words are read.
+        std::vector<std::pair<uint32_t, CachedMask*>> fresh;
+        const auto copy_fresh = [&] {
+            for (const auto& e : fresh)
+                hip_check(hipMemcpyAsync(e.second->mask, ref_masks + size_t(e.first) * words, words * 8,
+                                         hipMemcpyDeviceToDevice, stream_),
+                          "cache reference mask");
+        };
+        if (cached) {
             for (auto& kv : ref_slot) {
                 const uint32_t r = kv.second;
                 if (hit[r] || cache_.bytes + words * 8 > budget / 4) continue;
                 uint64_t* keep = nullptr;
                 hip_check(hipMalloc(&keep, words * 8), "hipMalloc");
-                hip_check(hipMemcpyAsync(keep, ref_masks + size_t(r) * words, words * 8, hipMemcpyDeviceToDevice,
-                                         stream_),
-                          "cache reference mask");
-                cache_.masks[{ss, kv.first}] = {keep, ref_clipped[r] != 0};
+                CachedMask& entry = cache_.masks[{ss, kv.first}];
+                entry = {keep, false};
+                fresh.push_back({r, &entry});
                 cache_.bytes += words * 8;
             }
+            copy_fresh();
+        }
         hip_check(hipEventRecord(events_[1], stream_), "event");
         hip_check(hipMemcpyAsync(dpairs, pairs.data(), pairs.size() * sizeof(ViewPair), hipMemcpyHostToDevice, stream_),
                   "upload pairs");
@@ -801,6 +811,7 @@ void Audit::run_items(const DeviceMesh& reference, const std::vector<const Devic
         hip_check(hipStreamSynchronize(stream_), "audit");
         if (deferred) {
             if (raster_finish(rl, st)) {
+                copy_fresh();
                 measure();
                 hip_check(hipMemcpyAsync(sum.data(), summaries, ncand * sizeof(ViewSummary), hipMemcpyDeviceToHost,
                                          stream_),
@@ -808,6 +819,7 @@ void Audit::run_items(const DeviceMesh& reference, const std::vector<const Devic
                 hip_check(hipStreamSynchronize(stream_), "audit");
             }
             clip_flags();
+            for (const auto& e : fresh) e.second->clipped = ref_clipped[e.first] != 0;
         }
         float t_r = 0, t_c = 0, t_d = 0;
         hip_check(hipEventElapsedTime(&t_r, events_[0], events_[1]), "elapsed");
