@@ -200,12 +200,22 @@ __global__ __launch_bounds__(256) void patch_kernel(PatchInput in) {
             }
             return;
         }
+        // Larger faces: each tile of the box within the dirty rectangle is
+        // classified exactly first (classify_rect); only undecided tiles
+        // are tested sample by sample.
         for (int ty = max(s.y0 >> 3, by0); ty <= min(s.y1 >> 3, by1); ++ty)
             for (int tx = max(s.x0 >> 3, bx0); tx <= min(s.x1 >> 3, bx1); ++tx) {
+                const int xa = max(tx * 8, s.x0), xb = min(tx * 8 + 7, s.x1);
+                const int ya = max(ty * 8, s.y0), yb = min(ty * 8 + 7, s.y1);
+                const int kind = classify_rect(s, xa, xb, ya, yb);
+                if (kind == 0) continue;
                 uint64_t word = 0;
-                for (int b = 0; b < 64; ++b) {
-                    const int x = tx * 8 + (b & 7), y = ty * 8 + (b >> 3);
-                    if (x >= s.x0 && x <= s.x1 && y >= s.y0 && y <= s.y1 && covers(s, x, y)) word |= 1ull << b;
+                if (kind == 1) {
+                    word = rect_word(xa & 7, xb & 7, ya & 7, yb & 7);
+                } else {
+                    for (int y = ya; y <= yb; ++y)
+                        for (int x = xa; x <= xb; ++x)
+                            if (covers(s, x, y)) word |= 1ull << ((y & 7) * 8 + (x & 7));
                 }
                 if (word)
                     atomicOr(&rect[uint32_t(ty - by0) * width + uint32_t(tx - bx0)],
