@@ -62,3 +62,29 @@ def pixel_digest(a: np.ndarray) -> str:
 
 def unpack_mask(bits: list[int], size: int) -> np.ndarray:
     return np.unpackbits(np.array(bits, np.uint8))[: size * size].reshape(size, size).astype(bool)
+
+
+def nonfinite_meshes():
+    """Meshes whose position streams hold NaN or infinite coordinates (the
+    reference's evaluate does not reject them), with the bounds of their
+    finite part: name -> (mesh, center, radius). A NaN first corner makes the
+    bounding box non-finite (the face counts as out of frame); a NaN second
+    or third corner leaves a finite box whose edge tests are NaN (every
+    sample of the box passes); an infinite corner is out of frame."""
+    import numpy as np
+    from blitzremesher_amd import Mesh, meshes
+    g = meshes.grid(6)
+    out = {}
+    for name, vertex, value in (("nan_first", None, np.nan), ("nan_inner", 20, np.nan), ("inf", 9, np.inf),
+                                ("neg_inf", 30, -np.inf)):
+        pos = g.positions.copy()
+        idx = g.indices.copy()
+        if vertex is None:
+            vertex = int(idx[len(idx) // 2, 0])  # the first corner of a middle face
+        pos[vertex, 0] = value
+        finite = np.isfinite(pos).all(1)
+        lo, hi = pos[finite].min(0), pos[finite].max(0)
+        center = ((lo + hi) * .5).astype(np.float32)
+        radius = float(np.linalg.norm(hi - lo) * .5)
+        out[name] = (Mesh(pos, idx), center, radius)
+    return out

@@ -118,6 +118,20 @@ def test_raster_large_triangles(audit, oracle):
             assert g[1:] == c[1:] and np.array_equal(g[0], c[0])
 
 
+def test_raster_nonfinite_positions(audit, oracle):
+    # NaN and infinite coordinates: out-of-frame stops, finite boxes with NaN
+    # edge tests, both rasters exactly as the oracle's.
+    from golden_data import nonfinite_meshes
+    for name, (m, c, r) in nonfinite_meshes().items():
+        d = audit.upload(m)
+        for cam in oracle.cameras(c, r, 24, 3, 2, 77):
+            for ss, two in ((1, False), (4, True)):
+                g = audit.raster_coverage(d, c, r, cam, 24, ss, two)
+                o = oracle.raster(m, c, r, cam, 24, ss, two)
+                assert g[1:] == o[1:] and np.array_equal(g[0], o[0]), (name, ss, two)
+        d.release()
+
+
 def test_coverage_distance_random_masks(audit, oracle):
     # tests/contracts.cpp:115-121 and tests/precision.cpp random masks (square here).
     rng = np.random.default_rng(871)

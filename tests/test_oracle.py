@@ -85,6 +85,20 @@ def test_oracle_matches_reference_build_on_fresh_inputs(oracle):
     assert oracle.coverage_distance(wa, wb, size, 1) == ref.coverage_distance(wa, wb, size, 1)
 
 
+@pytest.mark.skipif(not available("ref"), reason="reference build not present")
+def test_oracle_matches_reference_build_on_nonfinite_positions(oracle):
+    # NaN and infinite coordinates reach the raster unfiltered; the oracle's
+    # min/max and edge-test semantics must be the reference's.
+    from golden_data import nonfinite_meshes
+    ref = Oracle("ref")
+    for name, (m, c, r) in nonfinite_meshes().items():
+        for cam in oracle.cameras(c, r, 24, 3, 2, 77):
+            for ss, two in ((1, False), (4, True)):
+                got = oracle.raster(m, c, r, cam, 24, ss, two)
+                want = ref.raster(m, c, r, cam, 24, ss, two)
+                assert got[1:] == want[1:] and np.array_equal(got[0], want[0]), (name, ss, two)
+
+
 def brute_hausdorff(a: np.ndarray, b: np.ndarray) -> float:
     pa = np.argwhere(a).astype(np.float64)
     pb = np.argwhere(b).astype(np.float64)
