@@ -105,7 +105,6 @@ __global__ __launch_bounds__(kRasterThreads) void raster_band_kernel(const Raste
         next = load_corners(t, load_face(t, clamp_face(f0 + threadIdx.x)), clamp_face(f0 + threadIdx.x));
         tri_next = load_face(t, clamp_face(f0 + step + threadIdx.x));
     }
-#pragma unroll 2
     for (uint32_t base = f0; base < f1; base += step) {
         const uint32_t f = base + threadIdx.x;
         const Corners corners = next;
