@@ -417,43 +417,77 @@ __device__ __forceinline__ void store_used(uint32_t* p) {
     __hip_atomic_store(p, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// src/neural/action.cpp:261-304 on one wavefront: out[0] = picked count,
-// out[1] = 1 when a strict batch (a trial) is not independent, out[2] = the
-// position in `rows` of the first pick, out[3..] the picked rows.
+// src/neural/action.cpp:261-304 on one wavefront, 64 candidates at a time:
+// each lane tests one candidate's footprint (both endpoints and their
+// neighbours) against the footprints already claimed, the chunk's footprints
+// are intersected pairwise through per-vertex lane masks (`touch`, all zero
+// between chunks), and the sequential greedy choice is replayed over those
+// 64-bit masks in rank order, so the picks are exactly the sequential ones.
+// out[0] = picked count, out[1] = 1 when a strict batch (a trial) is not
+// independent, out[2] = the position in `rows` of the first pick, out[3..]
+// the picked rows.
 __global__ __launch_bounds__(64) void claim_kernel(Topo t, const LegalAction* __restrict__ legal,
                                                    const uint32_t* __restrict__ rows, uint32_t n, uint32_t maximum,
                                                    unsigned long long removable, uint32_t strict, uint32_t* used,
-                                                   uint32_t* out) {
+                                                   unsigned long long* touch, uint32_t* out) {
     const uint32_t lane = threadIdx.x;
+    const unsigned long long bit = 1ull << lane;
     uint32_t picked = 0, error = 0, first = kNone;
-    for (uint32_t k = 0; k < n; ++k) {
-        const LegalAction a = legal[rows[k]];
-        if (a.removed > removable) {
-            error |= strict;
-            continue;
-        }
+    bool done = false;
+    for (uint32_t k0 = 0; k0 < n && !done; k0 += 64) {
+        const uint32_t k = k0 + lane;
+        const bool valid = k < n;
+        LegalAction a{};
+        if (valid) a = legal[rows[k]];
+        const auto footprint = [&](auto&& visit) {
+            for (uint32_t e = 0; e < 2; ++e) {
+                const uint32_t u = e ? a.to : a.from;
+                visit(u);
+                const uint32_t j1 = t.offsets[u + 1];
+                for (uint32_t j = t.offsets[u]; j < j1; ++j) visit(t.neighbors[j]);
+            }
+        };
         bool hit = false;
-        for (uint32_t e = 0; e < 2; ++e) {
-            const uint32_t u = e ? a.to : a.from;
-            if (lane == 0) hit |= load_used(&used[u]) != 0;
-            for (uint32_t j = t.offsets[u] + lane; j < t.offsets[u + 1]; j += 64)
-                hit |= load_used(&used[t.neighbors[j]]) != 0;
+        unsigned long long conflict = 0;
+        if (valid) {
+            footprint([&](uint32_t v) {
+                hit |= load_used(&used[v]) != 0;
+                atomicOr(&touch[v], bit);
+            });
         }
-        if (__ballot(hit)) {
-            error |= strict;
-            continue;
+        __threadfence();  // every lane's touches land before any is read back
+        if (valid)
+            footprint([&](uint32_t v) { conflict |= __hip_atomic_load(&touch[v], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); });
+        conflict &= ~bit;
+        // Sequential replay (wave-uniform): candidate j is taken when its
+        // removal fits, it meets no earlier claim and no footprint taken
+        // earlier in this chunk.
+        const unsigned long long hits = __ballot(hit), valids = __ballot(valid);
+        const uint32_t before = picked;
+        unsigned long long taken = 0;
+        for (uint32_t j = 0; j < 64; ++j) {
+            if (!((valids >> j) & 1)) break;
+            const uint32_t removed = uint32_t(__shfl(int(a.removed), int(j)));
+            const unsigned long long meets = __shfl(conflict, int(j));
+            if (removed > removable || ((hits >> j) & 1) || (meets & taken)) {
+                error |= strict;
+                continue;
+            }
+            taken |= 1ull << j;
+            if (!picked) first = k0 + j;
+            ++picked;
+            removable -= removed;
+            if (picked == maximum || !removable) {
+                done = true;
+                break;
+            }
         }
-        for (uint32_t e = 0; e < 2; ++e) {
-            const uint32_t u = e ? a.to : a.from;
-            if (lane == 0) store_used(&used[u]);
-            for (uint32_t j = t.offsets[u] + lane; j < t.offsets[u + 1]; j += 64) store_used(&used[t.neighbors[j]]);
+        if (taken & bit) {
+            out[3 + before + uint32_t(__popcll(taken & (bit - 1)))] = rows[k];
+            footprint([&](uint32_t v) { store_used(&used[v]); });
         }
-        __threadfence();
-        if (lane == 0) out[3 + picked] = rows[k];
-        if (!picked) first = k;
-        ++picked;
-        removable -= a.removed;
-        if (picked == maximum || !removable) break;
+        if (valid) footprint([&](uint32_t v) { __hip_atomic_store(&touch[v], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); });
+        __threadfence();  // claims and cleared masks are seen by the next chunk
     }
     if (lane == 0) {
         out[0] = picked;
@@ -811,9 +845,17 @@ uint32_t ActionSet::claim_device(const uint32_t* drows, size_t n, uint32_t maxim
     const hipStream_t st = stream_;
     uint32_t* used = grow<uint32_t>(used_, vertices_);
     uint32_t* out = grow<uint32_t>(claim_out_, 3 + 64);
+    // Lane masks per vertex, zero between chunks; the claim kernel clears
+    // what it sets, so they are zeroed once per state.
+    if (!touch_ready_) {
+        unsigned long long* touch = grow<unsigned long long>(touch_, vertices_);
+        fill_u32_kernel<<<std::min<uint32_t>(1024, blocks_for(2 * vertices_)), 256, 0, st>>>(
+            reinterpret_cast<uint32_t*>(touch), 2 * vertices_, 0);
+        touch_ready_ = true;
+    }
     fill_u32_kernel<<<std::min<uint32_t>(1024, blocks_for(vertices_)), 256, 0, st>>>(used, vertices_, 0);
     claim_kernel<<<1, 64, 0, st>>>(topo(), legal_.as<LegalAction>(), drows, uint32_t(n), maximum, removable,
-                                   strict ? 1u : 0u, used, out);
+                                   strict ? 1u : 0u, used, touch_.as<unsigned long long>(), out);
     hip_check(hipGetLastError(), "claim launch");
     uint32_t host[3 + 64];
     hip_check(hipMemcpyAsync(host, out, sizeof host, hipMemcpyDeviceToHost, st), "download selection");

@@ -100,6 +100,8 @@ class ActionSet {
     DeviceBuffer used_, selection_, scratch_a_, scratch_b_, scratch_c_, temp_;
     DeviceBuffer order_, rank_keys_, rank_idx_, claim_out_;
     void* staging_ = nullptr;  // pinned host memory for trial downloads
+    DeviceBuffer touch_;       // per-vertex lane masks of the claim kernel
+    bool touch_ready_ = false;
 };
 
 // The gate of the executor: true when the candidate (index buffer, kept
