@@ -344,6 +344,29 @@ bool attributed_weights(const Settings& s) {
 }
 
 // Refinement stop rule (src/evaluate.cpp:336-337).
+// Verdict-only audits: the smallest squared sample distance whose coverage
+// (sqrt(float(d)) / ss, as the host rounds it) settles a coverage view as
+// failing at this supersampling, by the rules of finish_view and settled():
+// below the largest supersampling, coverage - 2*sqrt(2)/ss > limit (no
+// refinement can pass it); at the largest, coverage_upper > limit.
+static uint32_t failing_distance(const Settings& s, unsigned ss) {
+    const auto fails = [&](uint32_t d) {
+        const double c = std::sqrt(double(float(d))) / ss;
+        return ss >= s.max_supersample ? c + 2 * std::sqrt(2.0) / ss + 1e-6 > s.limit
+                                       : c - 2 * std::sqrt(2.0) / ss > s.limit;
+    };
+    uint32_t lo = 0, hi = 0xFFFFFFFEu;  // fails(hi) holds for any finite limit
+    if (!fails(hi)) return 0xFFFFFFFFu;
+    while (lo < hi) {
+        const uint32_t mid = lo + (hi - lo) / 2;
+        if (fails(mid))
+            hi = mid;
+        else
+            lo = mid + 1;
+    }
+    return lo;
+}
+
 bool settled(const Measurement& cur, const Settings& s, unsigned ss) {
     return s.conservative_screen || cur.passed || ss >= s.max_supersample ||
            cur.coverage - 2 * std::sqrt(2.0) / ss > s.limit || !std::isfinite(cur.error) ||
@@ -702,6 +725,7 @@ void Audit::run_items(const DeviceMesh& reference, const std::vector<const Devic
                 }
             }
         }
+        const uint32_t fail_sq = failing_distance(s, ss);
         std::vector<ViewPair> pairs(ncand);
         for (uint32_t i = 0; i < ncand; ++i) {
             const Item& it = all[begin + i];
@@ -713,6 +737,7 @@ void Audit::run_items(const DeviceMesh& reference, const std::vector<const Devic
             p.dist_c = p.dist_a + samples;
             p.size = size;
             p.tiles = tiles;
+            p.fail_sq = s.verdict_only && !attributed ? fail_sq : 0xFFFFFFFFu;
         }
         hip_check(hipEventRecord(events_[0], stream_), "event");
         launch_fill(ref_masks, size_t(nref) * words, 0, stream_);

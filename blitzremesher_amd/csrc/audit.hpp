@@ -33,6 +33,10 @@ struct Settings {
     double screen_size = 512, limit = 2, max_changed_area = 1;
     double normal_weight = 180.0 / (10 * 3.14159265358979323846), color_weight = 4, material_weight = 4;
     std::function<bool()> cancelled;
+    // Only `passed` is wanted (an executor's gate): the coverage distance of
+    // a view may stop at a lower bound once it settles the view as failing.
+    // Every other Measurement field of a failing view is then a bound.
+    bool verdict_only = false;
 };
 
 struct Measurement {

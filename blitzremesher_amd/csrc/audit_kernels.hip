@@ -391,7 +391,12 @@ __global__ __launch_bounds__(256) void hausdorff_kernel(const ViewPair* __restri
     const uint32_t total = uint32_t(__shfl(int(inclusive), 63));
     const uint32_t size = p.size;
     const bool dense = dense_rows(s, size);
-    const uint32_t known = __hip_atomic_load(target, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    // Wave-uniform copy of the maximum found so far.
+    const uint32_t known = uint32_t(__shfl(int(__hip_atomic_load(target, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)), 0));
+    // Verdict-only audits: once the maximum reaches fail_sq the view fails
+    // whatever the exact value, and queries stop at that lower bound.
+    const uint32_t fail = p.fail_sq;
+    if (known >= fail) return;  // whole waves
     uint32_t local = 0;
     for (uint32_t base = 0; base < total; base += 64) {
         const uint32_t g = base + lane;
@@ -406,7 +411,7 @@ __global__ __launch_bounds__(256) void hausdorff_kernel(const ViewPair* __restri
         const uint64_t bits = __shfl(my_bits, int(w));
         const uint32_t rank = g - uint32_t(__shfl(int(exclusive), int(w)));
         const uint32_t floor_value = max(known, local);
-        if (g < total) {
+        if (g < total && floor_value < fail) {
             const uint32_t bit = nth_bit(bits, rank);
             const size_t i = first + w;
             const uint32_t tx = uint32_t(i % p.tiles), ty = uint32_t(i / p.tiles);
@@ -429,6 +434,11 @@ __global__ __launch_bounds__(256) void hausdorff_kernel(const ViewPair* __restri
             };
             uint32_t best = row_term(y, kFar, 0);
             for (uint32_t k = 1; best > floor_value && uint64_t(k) * k < best; ++k) {
+                // Rows within k-1 are done: the minimum is >= min(best, k^2).
+                if (uint64_t(k) * k >= fail && best >= fail) {
+                    best = fail;
+                    break;
+                }
                 const uint32_t kk = k * k;
                 const bool up = y >= k, down = y + k < size;
                 if (!up && !down) break;

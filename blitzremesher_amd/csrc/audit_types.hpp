@@ -67,6 +67,10 @@ struct ViewPair {
     uint16_t* dist_a;  // row distances to the nearest A sample (queries from C)
     uint16_t* dist_c;  // row distances to the nearest C sample (queries from A)
     uint32_t size, tiles;
+    // Verdict-only audits: a squared distance at or above this settles the
+    // view as failing, so the Hausdorff pass may stop there with a lower
+    // bound. 0xFFFFFFFF: exact maxima.
+    uint32_t fail_sq = 0xFFFFFFFFu;
 };
 
 // Written by the compare and distance kernels, read back by the host.

@@ -581,6 +581,9 @@ blitz_status blitz_hip_execute_actions(blitz_hip_audit* a, const blitz_mesh* inp
         }
         auto reference = audit.upload(source);
         uint64_t audits = 0;
+        // The gate reads only `passed`.
+        Settings verdict = settings;
+        verdict.verdict_only = true;
 
         // The gate: an exact audit of the candidate (the input's vertex
         // streams, already on the device, with the trial's triangles)
@@ -594,7 +597,7 @@ blitz_status blitz_hip_execute_actions(blitz_hip_audit* a, const blitz_mesh* inp
             const uint64_t digest[2] = {d.a, d.b};
             auto candidate = audit.derive(*reference, idx, mats, source.double_sided, digest);
             ++audits;
-            return audit.evaluate(*reference, *candidate, bounds, settings).passed;
+            return audit.evaluate(*reference, *candidate, bounds, verdict).passed;
         };
         ExecuteStats st;
         if (ranking == 0 && (!scorer || !scorer->impl)) throw std::invalid_argument("learned ranking needs a scorer");
