@@ -331,15 +331,23 @@ __global__ __launch_bounds__(1024) void row_distance_kernel(const ViewPair* __re
 
 // Column pass restricted to the samples that can set the directed Hausdorff
 // maximum: every covered sample of `from` that is not covered in `to`. Each
-// wavefront reads 64 tile words (one per lane), numbers the wave's query
-// samples with a prefix sum of the words' popcounts and hands them out 64 at a
-// time, one per lane, so a wave with many sparse words (a silhouette running
-// along a tile row) does not walk its words one after another. Each query's
+// wavefront reads hausdorff_words() tile words (one per lane), numbers the
+// wave's query samples with a prefix sum of the words' popcounts and hands
+// them out 64 at a time, one per lane, so a wave with many sparse words (a
+// silhouette running along a tile row) does not walk its words one after
+// another. Each query's
 // exact squared distance is min over rows y' of (y-y')^2 + g(y'), g = rounded
 // row distance (the reference's lower envelope yields the same minimum; its
 // float output is a monotone rounding of it, applied once on the host). The
 // scan stops once (y-y')^2 reaches the best value or the best value cannot
 // exceed the maximum already found.
+// Tile words per wavefront. A silhouette band can put up to 64 queries in
+// every word, and each query's scan is a chain of dependent loads, so on
+// large rasters (long, wide bands) a wave takes 8 words (at most 8 rounds of
+// 64 queries) and the heavy part of a view spreads over many waves; small
+// rasters keep 64 words per wave (fewer, fuller waves).
+__host__ __device__ constexpr uint32_t hausdorff_words(uint32_t tiles) { return tiles >= 256 ? 8u : 64u; }
+
 __device__ __forceinline__ uint32_t wave_max(uint32_t v) {
     for (int o = 32; o > 0; o >>= 1) v = max(v, uint32_t(__shfl_xor(int(v), o)));
     return v;
@@ -375,10 +383,11 @@ __global__ __launch_bounds__(256) void hausdorff_kernel(const ViewPair* __restri
     uint32_t* target = (job & 1) ? &s.max_ca : &s.max_ac;
     const size_t words = size_t(p.tiles) * p.tiles;
     const uint32_t lane = threadIdx.x & 63;
-    const size_t first = (size_t(blockIdx.x) * 4 + (threadIdx.x >> 6)) * 64;
+    const uint32_t per_wave = hausdorff_words(p.tiles);
+    const size_t first = (size_t(blockIdx.x) * 4 + (threadIdx.x >> 6)) * per_wave;
     if (first >= words) return;  // whole waves
     const size_t mine = first + lane;
-    const uint64_t my_bits = mine < words ? from[mine] & ~to[mine] : 0;
+    const uint64_t my_bits = lane < per_wave && mine < words ? from[mine] & ~to[mine] : 0;
     if (!__ballot(my_bits != 0)) return;  // whole waves
     // Inclusive prefix sum of the words' query counts over the wave.
     const uint32_t count = uint32_t(__popcll(my_bits));
@@ -391,8 +400,8 @@ __global__ __launch_bounds__(256) void hausdorff_kernel(const ViewPair* __restri
     const uint32_t total = uint32_t(__shfl(int(inclusive), 63));
     const uint32_t size = p.size;
     const bool dense = dense_rows(s, size);
-    // Wave-uniform copy of the maximum found so far.
-    const uint32_t known = uint32_t(__shfl(int(__hip_atomic_load(target, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)), 0));
+    // Wave-uniform copy of the maximum found so far (refreshed every round).
+    uint32_t known = uint32_t(__shfl(int(__hip_atomic_load(target, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)), 0));
     // Verdict-only audits: once the maximum reaches fail_sq the view fails
     // whatever the exact value, and queries stop at that lower bound.
     const uint32_t fail = p.fail_sq;
@@ -447,8 +456,15 @@ __global__ __launch_bounds__(256) void hausdorff_kernel(const ViewPair* __restri
             }
             if (best > floor_value && best != kFar) local = best;
         }
-        // Share the round's maximum so later rounds stop earlier.
+        // Share the round's maximum with the wave and the other waves, and
+        // take theirs, so later queries stop earlier.
         local = wave_max(local);
+        if (base + 64 < total) {
+            if (lane == 0 && local > known) atomicMax(target, local);
+            known = max(known, uint32_t(__shfl(int(__hip_atomic_load(target, __ATOMIC_RELAXED,
+                                                                       __HIP_MEMORY_SCOPE_AGENT)), 0)));
+            if (known >= fail) return;  // whole waves
+        }
     }
     if (lane == 0 && local > known) atomicMax(target, local);
 }
@@ -516,7 +532,9 @@ void launch_distance(const ViewPair* pairs, ViewSummary* summaries, uint32_t cou
     const uint32_t threads = std::max<uint32_t>(64, (tiles + 63) / 64 * 64);
     row_distance_kernel<<<dim3(size, count * 2), dim3(threads), 0, stream>>>(pairs, summaries);
     const size_t words = size_t(tiles) * tiles;
-    hausdorff_kernel<<<dim3(uint32_t((words + 255) / 256), count * 2), dim3(256), 0, stream>>>(pairs, summaries);
+    const uint32_t per_block = 4 * hausdorff_words(tiles);
+    hausdorff_kernel<<<dim3(uint32_t((words + per_block - 1) / per_block), count * 2), dim3(256), 0, stream>>>(
+        pairs, summaries);
 }
 
 }  // namespace blitz_hip
