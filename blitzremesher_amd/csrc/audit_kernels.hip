@@ -42,16 +42,12 @@ __global__ __launch_bounds__(256) void project_kernel(const ProjectTask* __restr
     if (t.z) t.z[i] = z;
 }
 
-// Faces whose bounding box is at most 8x8 samples with few samples are
-// rasterized by their own lane; other faces within 8x8 samples take one
-// whole-wave pass (lane = sample, ballot = bitmap); for larger ones the
-// wavefront first sorts the face's tiles into empty, fully covered and
-// undecided (classify_rect), then produces each undecided tile's word with a
-// ballot.
-#ifndef BLITZ_LANE_SAMPLES
-#define BLITZ_LANE_SAMPLES 40  // build-time tuning override
-#endif
-constexpr int kLaneSamples = BLITZ_LANE_SAMPLES;
+// Faces whose bounding box is at most 4 samples wide and 8 high are
+// rasterized by their own lane, a row per step (lane_bitmap4); other faces
+// within 8x8 samples take one whole-wave pass (lane = sample, ballot =
+// bitmap); for larger ones the wavefront first sorts the face's tiles into
+// empty, fully covered and undecided (classify_rect), then produces each
+// undecided tile's word with a ballot.
 
 // Workgroups a raster launch should reach (four per CU of the 256-CU part).
 constexpr uint32_t kFillBlocks = 1024;
@@ -119,12 +115,12 @@ __global__ __launch_bounds__(kRasterThreads) void raster_band_kernel(const Raste
                 atomicMin(&t.status[0], f);
             } else if (kind == 1 && s.y1 >= ymin && s.y0 <= ymax) {
                 const int w = s.x1 - s.x0 + 1, h = s.y1 - s.y0 + 1;
-                if (w <= 8 && h <= 8 && w * h <= kLaneSamples) {
+                if (w <= 4 && h <= 8) {
                     // At most 2x2 tiles: an 8x8 local bitmap, then four words.
                     const int tx0 = s.x0 >> 3, ty = s.y0 >> 3;
                     const int ya = max(s.y0, ymin), yb = min(s.y1, ymax);
                     uint64_t word[4];
-                    scatter_bitmap(lane_bitmap(s, ya, yb), s.x0, s.y0, word);
+                    scatter_bitmap(lane_bitmap4(s, ya, yb), s.x0, s.y0, word);
 #pragma unroll
                     for (int k = 0; k < 4; ++k)
                         if (word[k]) {

@@ -210,6 +210,42 @@ __device__ __forceinline__ uint64_t lane_bitmap(const Setup& s, int ya, int yb) 
     return bits;
 }
 
+// Lane path for boxes at most 4 samples wide: one row of the box per step
+// (a wave takes max(rows) steps). The x-dependent products of the four
+// columns are computed once per face and the y-dependent ones once per row;
+// every value is the reference expression's own rounding,
+// u = (cx-bx)*(py-by) - (cy-by)*(px-bx), covered iff !(u+r0 < 0 || ...).
+// Bitmap relative to (x0, y0), bit 8*(y-y0) + (x-x0), rows [ya, yb].
+__device__ __forceinline__ uint64_t lane_bitmap4(const Setup& s, int ya, int yb) {
+    const double au = s.cx - s.bx, bu = s.cy - s.by;
+    const double av = s.ax - s.cx, bv = s.ay - s.cy;
+    const double aw = s.bx - s.ax, bw = s.by - s.ay;
+    const int w = s.x1 - s.x0 + 1;
+    double cu[4], cv[4], cw[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        const double px = double(s.x0 + c) + .5;
+        cu[c] = bu * (px - s.bx);
+        cv[c] = bv * (px - s.cx);
+        cw[c] = bw * (px - s.ax);
+    }
+    uint64_t bits = 0;
+    for (int y = ya; y <= yb; ++y) {
+        const double py = double(y) + .5;
+        const double ru = au * (py - s.by), rv = av * (py - s.cy), rw = aw * (py - s.ay);
+        uint32_t row = 0;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            // Bitwise | so that all three tests issue without branches.
+            const bool in = (c < w) & !(((ru - cu[c]) + s.r0 < 0) | ((rv - cv[c]) + s.r1 < 0) |
+                                        ((rw - cw[c]) + s.r2 < 0));
+            row |= uint32_t(in) << c;
+        }
+        bits |= uint64_t(row) << (8 * (y - s.y0));
+    }
+    return bits;
+}
+
 // Moves an 8x8 bitmap anchored at sample (x0, y0) into the up-to-four 8x8
 // tile words it overlaps: word[0] tile (x0>>3, y0>>3), word[1] the tile to
 // its right, word[2] below, word[3] below-right.

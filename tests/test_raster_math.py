@@ -1,7 +1,7 @@
 """CPU checks of the raster's device arithmetic (blitzremesher_amd/csrc/
 raster_device.hpp): the whole-rectangle coverage classifier used for large
-faces and the lane path's two-samples-per-step bitmap must agree with the
-per-sample predicate (src/evaluate.cpp:166-168) on every sample. The
+faces and the lane paths' bitmaps (two samples per step; one row of up to
+four samples per step) must agree with the per-sample predicate (src/evaluate.cpp:166-168) on every sample. The
 functions are taken verbatim from the header, compiled for the host with
 g++ -ffp-contract=off and checked by brute force over seeded triangles,
 including corners on sample centres and strict (r = 0) edges."""
@@ -103,6 +103,7 @@ int main(int argc, char** argv) {
                 if (covers(s, x, y)) want |= 1ull << (8 * (y - s.y0) + (x - s.x0));
         ++rects;
         bad += lane_bitmap(s, ya, yb) != want;
+        if (s.x1 - s.x0 < 4) bad += lane_bitmap4(s, ya, yb) != want;
     }
     printf("%ld %ld %ld %ld\n", rects, full, empty, bad);
     return 0;
@@ -138,7 +139,8 @@ def _source(plant: tuple[str, str] | None = None) -> str:
         assert plant[0] in text
         text = text.replace(plant[0], plant[1])
     parts = [_function(text, "double edge("), _function(text, "bool covers("), _function(text, "int classify_rect("),
-             _function(text, "uint64_t rect_word("), _function(text, "uint64_t lane_bitmap(")]
+             _function(text, "uint64_t rect_word("), _function(text, "uint64_t lane_bitmap("),
+             _function(text, "uint64_t lane_bitmap4(")]
     return PRELUDE + "\n\n".join(parts) + CHECK
 
 
@@ -162,6 +164,10 @@ def test_device_raster_math_agrees_with_every_sample(tmp_path):
     ("const bool in1 = (x < s.x1) &", "const bool in1 = (x <= s.x1) &"),
     # The second sample tested at the first one's position.
     ("qx = px + 1.0;", "qx = px;"),
+    # Columns past the box's width kept in the four-wide row.
+    ("const bool in = (c < w) &", "const bool in = (c <= w) &"),
+    # A row's bits shifted by the band row instead of the box row.
+    ("bits |= uint64_t(row) << (8 * (y - s.y0));", "bits |= uint64_t(row) << (8 * (y - ya));"),
 ])
 def test_check_catches_planted_mistakes(tmp_path, plant):
     _, _, _, bad = _run(tmp_path, _source(plant), 4000)
