@@ -112,12 +112,16 @@ __device__ __forceinline__ int setup_triangle(const RasterTask& t, const Corners
 }
 
 // Conservative coverage predicate at sample (x, y) (src/evaluate.cpp:166-168).
+// The reference's test fl(u + r) < 0 is evaluated as u < -r: the rounded sum
+// of two doubles has the sign of the exact sum (a nonzero sum too small to
+// round away is exact, by Sterbenz's lemma), overflow keeps the sign, and
+// NaN fails both forms; -r is exact.
 __device__ __forceinline__ bool covers(const Setup& s, int x, int y) {
     const double px = x + .5, py = y + .5;
     const double u = edge(s.bx, s.by, s.cx, s.cy, px, py);
     const double v = edge(s.cx, s.cy, s.ax, s.ay, px, py);
     const double w = edge(s.ax, s.ay, s.bx, s.by, px, py);
-    return !(u + s.r0 < 0 || v + s.r1 < 0 || w + s.r2 < 0);
+    return !(u < -s.r0 || v < -s.r1 || w < -s.r2);
 }
 
 // Whole-rectangle form of covers() over the samples [xa, xb] x [ya, yb]:
@@ -229,6 +233,8 @@ __device__ __forceinline__ uint64_t lane_bitmap4(const Setup& s, int ya, int yb)
         cv[c] = bv * (px - s.cx);
         cw[c] = bw * (px - s.ax);
     }
+    // fl(u + r) < 0 tested as u < -r (see covers()).
+    const double nu = -s.r0, nv = -s.r1, nw = -s.r2;
     uint64_t bits = 0;
     for (int y = ya; y <= yb; ++y) {
         const double py = double(y) + .5;
@@ -237,8 +243,7 @@ __device__ __forceinline__ uint64_t lane_bitmap4(const Setup& s, int ya, int yb)
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
             // Bitwise | so that all three tests issue without branches.
-            const bool in = (c < w) & !(((ru - cu[c]) + s.r0 < 0) | ((rv - cv[c]) + s.r1 < 0) |
-                                        ((rw - cw[c]) + s.r2 < 0));
+            const bool in = (c < w) & !(((ru - cu[c]) < nu) | ((rv - cv[c]) < nv) | ((rw - cw[c]) < nw));
             row |= uint32_t(in) << c;
         }
         bits |= uint64_t(row) << (8 * (y - s.y0));

@@ -21,6 +21,7 @@ PRELUDE = """#include <algorithm>
 #include <cstdint>
 #include <cstdio>
 #include <random>
+#include <cstdlib>
 using std::fabs; using std::isfinite; using std::max; using std::min;
 #define __device__
 #define __forceinline__ inline
@@ -31,6 +32,16 @@ struct Setup {
 """
 
 CHECK = r"""
+// The reference's predicate as written (src/evaluate.cpp:166-168), the
+// ground truth for every device routine below.
+static bool covers_ref(const Setup& s, int x, int y) {
+    const double px = x + .5, py = y + .5;
+    const double u = (s.cx - s.bx) * (py - s.by) - (s.cy - s.by) * (px - s.bx);
+    const double v = (s.ax - s.cx) * (py - s.cy) - (s.ay - s.cy) * (px - s.cx);
+    const double w = (s.bx - s.ax) * (py - s.ay) - (s.by - s.ay) * (px - s.ax);
+    return !(u + s.r0 < 0 || v + s.r1 < 0 || w + s.r2 < 0);
+}
+
 int main(int argc, char** argv) {
     const int iterations = atoi(argv[1]);
     std::mt19937_64 rng(11);
@@ -67,7 +78,8 @@ int main(int argc, char** argv) {
                 for (int y = ya; y <= yb; ++y)
                     for (int x = xa; x <= xb; ++x) {
                         ++n;
-                        in += covers(s, x, y);
+                        in += covers_ref(s, x, y);
+                        bad += covers(s, x, y) != covers_ref(s, x, y);
                         box |= 1ull << (8 * (y & 7) + (x & 7));
                     }
                 ++rects;
@@ -100,10 +112,25 @@ int main(int argc, char** argv) {
         uint64_t want = 0;
         for (int y = ya; y <= yb; ++y)
             for (int x = s.x0; x <= s.x1; ++x)
-                if (covers(s, x, y)) want |= 1ull << (8 * (y - s.y0) + (x - s.x0));
+                if (covers_ref(s, x, y)) want |= 1ull << (8 * (y - s.y0) + (x - s.x0));
         ++rects;
         bad += lane_bitmap(s, ya, yb) != want;
         if (s.x1 - s.x0 < 4) bad += lane_bitmap4(s, ya, yb) != want;
+    }
+    // fl(d + r) < 0 against d < -r over special and extreme values (r >= 0
+    // or NaN, as conservative radii are).
+    {
+        const double specials[] = {0.0, -0.0, 1.0, -1.0, 0.5, 4.9e-324, -4.9e-324, 2.2e-308, -2.2e-308,
+                                   1e300, -1e300, 1.7976931348623157e308, -1.7976931348623157e308,
+                                   INFINITY, -INFINITY, NAN, 3.0000000000000004, -3.0};
+        std::uniform_int_distribution<int> pick(0, int(sizeof specials / sizeof specials[0]) - 1);
+        for (int it = 0; it < iterations * 4; ++it) {
+            double d = specials[pick(rng)], r = std::fabs(specials[pick(rng)]);
+            if (it % 3 == 0) d = std::ldexp(std::uniform_real_distribution<double>(-1, 1)(rng), int(rng() % 2100) - 1075);
+            if (it % 5 == 0) r = std::ldexp(std::uniform_real_distribution<double>(0, 1)(rng), int(rng() % 2100) - 1075);
+            if (it % 7 == 0) d = -r * (1 + std::uniform_int_distribution<int>(-3, 3)(rng) * 2.2e-16);
+            bad += ((d + r) < 0) != (d < -r);
+        }
     }
     printf("%ld %ld %ld %ld\n", rects, full, empty, bad);
     return 0;
