@@ -128,11 +128,10 @@ __device__ __forceinline__ bool covers(const Setup& s, int x, int y) {
 // 0 = no sample covered, 1 = every sample covered, 2 = undecided (test each
 // sample). Exact, with no tolerance: an edge value is
 // fl(fl(A*fl(py-Py)) - fl(B*fl(px-Px))) and every rounded operation is
-// monotone in each operand, so over a rectangle the rounded value (and
-// fl(value + r)) is smallest at the row end chosen by sign(A) combined with
-// the column end chosen by sign(B), and largest at the opposite pair, both
-// of which are samples of the rectangle. Non-finite corners are left
-// undecided.
+// monotone in each operand, so over a rectangle the rounded value is
+// smallest at the row end chosen by sign(A) combined with the column end
+// chosen by sign(B), and largest at the opposite pair, both of which are
+// samples of the rectangle. Non-finite corners are left undecided.
 __device__ __forceinline__ int classify_rect(const Setup& s, int xa, int xb, int ya, int yb) {
     const double pxa = xa + .5, pxb = xb + .5, pya = ya + .5, pyb = yb + .5;
     bool all = true, none = false;
@@ -142,9 +141,10 @@ __device__ __forceinline__ int classify_rect(const Setup& s, int xa, int xb, int
         const double t2a = b * (pxa - ax), t2b = b * (pxb - ax);
         const double t1min = a >= 0 ? t1a : t1b, t1max = a >= 0 ? t1b : t1a;
         const double t2min = b >= 0 ? t2a : t2b, t2max = b >= 0 ? t2b : t2a;
-        const double lo = (t1min - t2max) + r, hi = (t1max - t2min) + r;
-        all = all && !(lo < 0);
-        none = none || hi < 0;
+        // fl(value + r) < 0 tested as value < -r (see covers()).
+        const double lo = t1min - t2max, hi = t1max - t2min;
+        all = all && !(lo < -r);
+        none = none || hi < -r;
     };
     edge_range(s.bx, s.by, s.cx, s.cy, s.r0);
     edge_range(s.cx, s.cy, s.ax, s.ay, s.r1);
@@ -200,10 +200,11 @@ __device__ __forceinline__ uint64_t lane_bitmap(const Setup& s, int ya, int yb) 
         const double ru = au * (py - s.by), rv = av * (py - s.cy), rw = aw * (py - s.ay);
         const double px = double(x) + .5, qx = px + 1.0;
         // Bitwise | so that all three tests issue without branches.
-        const bool in0 = !(((ru - bu * (px - s.bx)) + s.r0 < 0) | ((rv - bv * (px - s.cx)) + s.r1 < 0) |
-                           ((rw - bw * (px - s.ax)) + s.r2 < 0));
-        const bool in1 = (x < s.x1) & !(((ru - bu * (qx - s.bx)) + s.r0 < 0) | ((rv - bv * (qx - s.cx)) + s.r1 < 0) |
-                                         ((rw - bw * (qx - s.ax)) + s.r2 < 0));
+        // fl(u + r) < 0 tested as u < -r (see covers()).
+        const bool in0 = !(((ru - bu * (px - s.bx)) < -s.r0) | ((rv - bv * (px - s.cx)) < -s.r1) |
+                           ((rw - bw * (px - s.ax)) < -s.r2));
+        const bool in1 = (x < s.x1) & !(((ru - bu * (qx - s.bx)) < -s.r0) | ((rv - bv * (qx - s.cx)) < -s.r1) |
+                                         ((rw - bw * (qx - s.ax)) < -s.r2));
         bits |= uint64_t(uint32_t(in0) | (uint32_t(in1) << 1)) << (8 * (y - s.y0) + (x - s.x0));
         x += 2;
         if (x > s.x1) {
