@@ -190,7 +190,7 @@ __global__ __launch_bounds__(256) void patch_kernel(PatchInput in) {
         const int w = s.x1 - s.x0 + 1, h = s.y1 - s.y0 + 1;
         if (w <= 8 && h <= 8) {
             uint64_t word[4];
-            scatter_bitmap(lane_bitmap(s, ya, yb), s.x0, s.y0, word);
+            scatter_bitmap(w <= 4 ? lane_bitmap4(s, ya, yb) : lane_bitmap(s, ya, yb), s.x0, s.y0, word);
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
                 const int tx = (s.x0 >> 3) + (k & 1), ty = (s.y0 >> 3) + (k >> 1);
