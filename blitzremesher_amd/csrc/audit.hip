@@ -408,13 +408,14 @@ std::vector<Measurement> Audit::evaluate_batch(const DeviceMesh& reference,
     DeviceGuard guard(device_);
     // Views per candidate and round. The reference walks its views in order
     // and stops at the first failing one (src/evaluate.cpp:319-350). Under
-    // the appearance profiles (attribute rasters and windowed searches) a
-    // round takes the first 4 pending views, so work after an early failure
-    // is skipped; a fully passing audit then takes more, smaller rounds.
-    // The coverage profile takes every pending view at once. Results do not
-    // depend on the window.
-    const uint32_t window = attributed ? 4u : 0u;
-    for (;;) {
+    // the appearance profiles (attribute rasters and windowed searches) the
+    // first round takes the first pending view and each later round twice
+    // as many, so work after an early failure is skipped while a fully
+    // passing audit needs only about log2(views) more rounds. The coverage
+    // profile takes every pending view at once. Results do not depend on the
+    // window.
+    uint32_t window = attributed ? 1u : 0u;
+    for (;; window = window ? std::min(window * 2, 1u << 16) : 0u) {
         // Per candidate, the first `window` pending views in view order (all
         // of them for window 0), stopping at a settled failing view.
         std::vector<Item> items;
