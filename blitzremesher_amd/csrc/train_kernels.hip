@@ -265,7 +265,9 @@ void launch_column_sum(const float* a, const float* a_mask, float* out, uint32_t
                        hipStream_t stream) {
     if (!rows || !cols) return;
     if (cols > 256 || 256 % cols) throw std::invalid_argument("column_sum: cols must divide 256");
-    const uint32_t per = 1024;
+    // About a thousand blocks of at least 64 rows: enough to fill the chip,
+    // few enough atomics per column.
+    const uint32_t per = std::max<uint32_t>(64, (rows + 1023) / 1024);
     column_sum<<<dim3((rows + per - 1) / per), dim3(256), 0, stream>>>(a, a_mask, out, rows, cols, per);
 }
 
