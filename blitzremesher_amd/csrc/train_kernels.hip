@@ -66,78 +66,170 @@ __global__ __launch_bounds__(256) void loss_kernel(const float* __restrict__ y, 
     if ((threadIdx.x & 63) == 0 && l != 0.f) atomicAdd(loss, l);
 }
 
-// C[M][N] += sum_r A(r, m) * B(r, n) over rows [r0, r1) of this block's
-// slice (split-R with atomic accumulation). A and B are row-major with row
-// strides lda/ldb; a_mask (optional) zeroes A(r, m) where mask(r, m) <= 0
-// (ReLU derivative), reading the mask with A's layout.
+// v_mfma_f32_16x16x4f32 (FP32 products and sums) operand layout, wave64:
+// lane l supplies A(i = l%16, k = l/16) and B(k = l/16, j = l%16); element q
+// of lane l's accumulator is D(i = 4*(l/16) + q, j = l%16).
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ floatx4 mfma4(float a, float b, floatx4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+__device__ __forceinline__ float4 ld4(const float* p) { return *reinterpret_cast<const float4*>(p); }
+
+__device__ __forceinline__ float part_of(const float4& v, int s) {
+    return s == 0 ? v.x : s == 1 ? v.y : s == 2 ? v.z : v.w;
+}
+
+// An operand value: zero out of range and, masked, where mask <= 0 (the
+// ReLU derivative). Loads are issued unconditionally (out-of-range lanes read
+// element 0) one step before this select, so no load waits behind a branch.
+__device__ __forceinline__ float keep(bool in, float v, float mask) { return in && mask > 0.f ? v : 0.f; }
+
+// C[M][N] += sum_r A(r, m) * B(r, n) on the matrix cores. A and B are
+// row-major with row strides lda/ldb; kMasked zeroes A(r, m) where
+// a_mask(r, m) <= 0, read with A's layout. A block owns a 64 x 64 tile of C
+// and a slice of rows, a quarter per wave; four rows make one MFMA k-step and
+// sixteen rows one iteration, whose loads are issued an iteration ahead.
+// Lane l loads four consecutive columns 4*(l%16) .. +3 of row l/16 of A and
+// of B and feeds component s to sub-tile s, so sub-tile s holds tile columns
+// 4*i + s (rows of C) and 4*j + s (columns of C): a fixed permutation undone
+// at the write. The waves' tiles are combined in LDS and added atomically.
+template <bool kMasked>
 __global__ __launch_bounds__(256) void gemm_at_b(const float* __restrict__ a, uint32_t lda,
                                                  const float* __restrict__ a_mask, const float* __restrict__ b,
                                                  uint32_t ldb, float* __restrict__ c, uint32_t rows, uint32_t m,
                                                  uint32_t n, uint32_t rows_per_slice) {
-    __shared__ float as[16][64 + 1], bs[16][64 + 1];
+    __shared__ floatx4 part[3][16][64];
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const uint32_t tm = blockIdx.x * 64, tn = blockIdx.y * 64;
-    const uint32_t r0 = blockIdx.z * rows_per_slice, r1 = min(rows, r0 + rows_per_slice);
-    const uint32_t tx = threadIdx.x % 16, ty = threadIdx.x / 16;  // 4x4 outputs each
-    float acc[4][4] = {};
-    for (uint32_t r = r0; r < r1; r += 16) {
-        for (uint32_t i = threadIdx.x; i < 16 * 64; i += 256) {
-            const uint32_t rr = i / 64, cc = i % 64, row = r + rr;
-            float va = 0.f, vb = 0.f;
-            if (row < r1) {
-                if (tm + cc < m) {
-                    va = a[size_t(row) * lda + tm + cc];
-                    if (a_mask && !(a_mask[size_t(row) * lda + tm + cc] > 0.f)) va = 0.f;
-                }
-                if (tn + cc < n) vb = b[size_t(row) * ldb + tn + cc];
-            }
-            as[rr][cc] = va;
-            bs[rr][cc] = vb;
+    const uint32_t s0 = blockIdx.z * rows_per_slice, s1 = min(rows, s0 + rows_per_slice);
+    const uint32_t quarter = ((rows_per_slice + 63) / 64) * 16;  // rows per wave, a multiple of 16
+    const uint32_t r0 = min(s1, s0 + wave * quarter), r1 = min(s1, r0 + quarter);
+    const uint32_t ma = tm + 4 * (lane % 16), nb = tn + 4 * (lane % 16);
+    const bool a_in = ma < m, b_in = nb < n;
+    struct Rows {
+        float4 a[4], mask[4], b[4];
+    };
+    auto fetch = [&](uint32_t base, Rows& x) {
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            const uint32_t row = base + 4 * s + lane / 16;
+            const bool in = row < r1;
+            const size_t at = in && a_in ? size_t(row) * lda + ma : 0;
+            x.a[s] = ld4(a + at);
+            if (kMasked) x.mask[s] = ld4(a_mask + at);
+            x.b[s] = ld4(b + (in && b_in ? size_t(row) * ldb + nb : 0));
         }
-        __syncthreads();
-        for (int k = 0; k < 16; ++k)
-            for (int i = 0; i < 4; ++i)
-                for (int jj = 0; jj < 4; ++jj) acc[i][jj] += as[k][ty * 4 + i] * bs[k][tx * 4 + jj];
-        __syncthreads();
-    }
+    };
+    floatx4 acc[4][4];
     for (int i = 0; i < 4; ++i)
-        for (int jj = 0; jj < 4; ++jj) {
-            const uint32_t mi = tm + ty * 4 + i, nj = tn + tx * 4 + jj;
-            if (mi < m && nj < n && acc[i][jj] != 0.f) atomicAdd(&c[size_t(mi) * n + nj], acc[i][jj]);
+        for (int j = 0; j < 4; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+    Rows cur, next;
+    fetch(r0, cur);
+    for (uint32_t r = r0; r < r1; r += 16) {
+        fetch(r + 16, next);
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            const uint32_t row = r + 4 * s + lane / 16;
+            const bool ia = row < r1 && a_in, ib = row < r1 && b_in;
+            float av[4], bv[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                av[i] = keep(ia, part_of(cur.a[s], i), kMasked ? part_of(cur.mask[s], i) : 1.f);
+                bv[i] = ib ? part_of(cur.b[s], i) : 0.f;
+            }
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) acc[i][j] = mfma4(av[i], bv[j], acc[i][j]);
+        }
+        cur = next;
+    }
+    if (wave)
+        for (int i = 0; i < 4; ++i)
+            for (int j = 0; j < 4; ++j) part[wave - 1][i * 4 + j][lane] = acc[i][j];
+    __syncthreads();
+    if (wave) return;
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) {
+            const floatx4 v = acc[i][j] + part[0][i * 4 + j][lane] + part[1][i * 4 + j][lane] +
+                              part[2][i * 4 + j][lane];
+            const uint32_t nj = tn + 4 * (lane % 16) + j;
+            for (int q = 0; q < 4; ++q) {
+                const uint32_t mi = tm + 4 * (4 * (lane / 16) + q) + i;
+                if (mi < m && nj < n && v[q] != 0.f) atomicAdd(&c[size_t(mi) * n + nj], v[q]);
+            }
         }
 }
 
-// T[R][N] = (A masked)[R][K] x W[K][N] (W row-major with K rows of N).
+// T[R][N] = (A masked)[R][K] x W[K][N] (W row-major with K rows of N) on the
+// matrix cores. A wave owns 64 rows x 64 columns of T; a block's waves take
+// consecutive column tiles of the same 64 rows, so they share the reads of A
+// through the CU's caches. Per 16 columns of K,
+// lane l loads four consecutive A columns 4*(l/16) .. +3 of row l%16 of each
+// 16-row sub-tile and feeds component s to k-step s, so k-step s covers K
+// columns 4*g + s (g = l/16) — the same permutation applied to W's rows. The
+// next 16 columns' loads are issued before this block's 64 MFMAs.
+template <bool kMasked>
 __global__ __launch_bounds__(256) void gemm_a_w(const float* __restrict__ a, const float* __restrict__ a_mask,
                                                 const float* __restrict__ w, float* __restrict__ t, uint32_t rows,
                                                 uint32_t k_dim, uint32_t n) {
-    __shared__ float as[64][16 + 1], ws[16][64 + 1];
-    const uint32_t tr = blockIdx.x * 64, tn = blockIdx.y * 64;
-    const uint32_t tx = threadIdx.x % 16, ty = threadIdx.x / 16;
-    float acc[4][4] = {};
-    for (uint32_t k0 = 0; k0 < k_dim; k0 += 16) {
-        for (uint32_t i = threadIdx.x; i < 64 * 16; i += 256) {
-            const uint32_t rr = i / 16, kk = i % 16;
-            float v = 0.f;
-            if (tr + rr < rows && k0 + kk < k_dim) {
-                v = a[size_t(tr + rr) * k_dim + k0 + kk];
-                if (a_mask && !(a_mask[size_t(tr + rr) * k_dim + k0 + kk] > 0.f)) v = 0.f;
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t tr = blockIdx.x * 64, tn = (blockIdx.y * (blockDim.x / 64) + threadIdx.x / 64) * 64;
+    if (tn >= n) return;
+    const uint32_t g = lane / 16, col = lane % 16;
+    struct Block {
+        float4 a[4], mask[4];
+        float w[4][4];
+    };
+    auto fetch = [&](uint32_t k0, Block& x) {
+        const uint32_t ka = k0 + 4 * g;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const uint32_t row = tr + 16 * i + col;
+            const size_t at = row < rows && ka < k_dim ? size_t(row) * k_dim + ka : 0;
+            x.a[i] = ld4(a + at);
+            if (kMasked) x.mask[i] = ld4(a_mask + at);
+        }
+#pragma unroll
+        for (int s = 0; s < 4; ++s)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const uint32_t nj = tn + 16 * j + col;
+                x.w[s][j] = w[ka < k_dim && nj < n ? size_t(ka + s) * n + nj : 0];
             }
-            as[rr][kk] = v;
+    };
+    floatx4 acc[4][4];
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+    Block cur, next;
+    fetch(0, cur);
+    for (uint32_t k0 = 0; k0 < k_dim; k0 += 16) {
+        fetch(k0 + 16, next);
+        const uint32_t ka = k0 + 4 * g;
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            float wv[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) wv[j] = ka < k_dim && tn + 16 * j + col < n ? cur.w[s][j] : 0.f;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const bool in = tr + 16 * i + col < rows && ka < k_dim;
+                const float x = keep(in, part_of(cur.a[i], s), kMasked ? part_of(cur.mask[i], s) : 1.f);
+#pragma unroll
+                for (int j = 0; j < 4; ++j) acc[i][j] = mfma4(x, wv[j], acc[i][j]);
+            }
         }
-        for (uint32_t i = threadIdx.x; i < 16 * 64; i += 256) {
-            const uint32_t kk = i / 64, cc = i % 64;
-            ws[kk][cc] = (k0 + kk < k_dim && tn + cc < n) ? w[size_t(k0 + kk) * n + tn + cc] : 0.f;
-        }
-        __syncthreads();
-        for (int k = 0; k < 16; ++k)
-            for (int i = 0; i < 4; ++i)
-                for (int jj = 0; jj < 4; ++jj) acc[i][jj] += as[ty * 4 + i][k] * ws[k][tx * 4 + jj];
-        __syncthreads();
+        cur = next;
     }
     for (int i = 0; i < 4; ++i)
-        for (int jj = 0; jj < 4; ++jj) {
-            const uint32_t ri = tr + ty * 4 + i, nj = tn + tx * 4 + jj;
-            if (ri < rows && nj < n) t[size_t(ri) * n + nj] = acc[i][jj];
+        for (int j = 0; j < 4; ++j) {
+            const uint32_t nj = tn + 16 * j + col;
+            for (int q = 0; q < 4; ++q) {
+                const uint32_t ri = tr + 16 * i + 4 * g + q;
+                if (ri < rows && nj < n) t[size_t(ri) * n + nj] = acc[i][j][q];
+            }
         }
 }
 
@@ -247,18 +339,29 @@ void launch_loss(const float* y, const uint8_t* labels, const float* targets, fl
 
 void launch_gemm_at_b(const float* a, uint32_t lda, const float* a_mask, const float* b, uint32_t ldb, float* c,
                       uint32_t rows, uint32_t m, uint32_t n, hipStream_t stream) {
-    if (!rows) return;
+    if (!rows || !m || !n) return;
+    if ((m | n | lda | ldb) % 4) throw std::invalid_argument("gemm_at_b: m, n and row strides must be multiples of 4");
+    // About 512 blocks (two per CU) of at least 256 rows each.
     const uint32_t tiles = ((m + 63) / 64) * ((n + 63) / 64);
-    const uint32_t slices = std::max<uint32_t>(1, std::min<uint32_t>((rows + 255) / 256, 1024 / std::max(1u, tiles)));
+    const uint32_t slices = std::max<uint32_t>(1, std::min<uint32_t>((rows + 255) / 256, 512 / tiles));
     const uint32_t per = (rows + slices - 1) / slices;
-    gemm_at_b<<<dim3((m + 63) / 64, (n + 63) / 64, slices), dim3(256), 0, stream>>>(a, lda, a_mask, b, ldb, c, rows, m,
-                                                                                      n, per);
+    const dim3 grid((m + 63) / 64, (n + 63) / 64, slices);
+    if (a_mask)
+        gemm_at_b<true><<<grid, dim3(256), 0, stream>>>(a, lda, a_mask, b, ldb, c, rows, m, n, per);
+    else
+        gemm_at_b<false><<<grid, dim3(256), 0, stream>>>(a, lda, a_mask, b, ldb, c, rows, m, n, per);
 }
 
 void launch_gemm_a_w(const float* a, const float* a_mask, const float* w, float* t, uint32_t rows, uint32_t k,
                      uint32_t n, hipStream_t stream) {
-    if (!rows) return;
-    gemm_a_w<<<dim3((rows + 63) / 64, (n + 63) / 64), dim3(256), 0, stream>>>(a, a_mask, w, t, rows, k, n);
+    if (!rows || !n) return;
+    if (k % 4) throw std::invalid_argument("gemm_a_w: k must be a multiple of 4");
+    const uint32_t tiles = (n + 63) / 64, waves = std::min(4u, tiles);
+    const dim3 grid((rows + 63) / 64, (tiles + waves - 1) / waves), block(64 * waves);
+    if (a_mask)
+        gemm_a_w<true><<<grid, block, 0, stream>>>(a, a_mask, w, t, rows, k, n);
+    else
+        gemm_a_w<false><<<grid, block, 0, stream>>>(a, a_mask, w, t, rows, k, n);
 }
 
 void launch_column_sum(const float* a, const float* a_mask, float* out, uint32_t rows, uint32_t cols,
