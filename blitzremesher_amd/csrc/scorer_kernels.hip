@@ -137,17 +137,39 @@ __global__ __launch_bounds__(256) void features_kernel(FeatureInput in) {
 
 // ------------------------------------------------------------ FP32 (exact)
 
-// z[v] = [h[v], mean of h over N(v)], neighbours summed in CSR order.
+// z[v] = [h[v], mean of h over N(v)], neighbours summed in CSR order. Four
+// channels per thread (width is a multiple of 4); neighbour indices are read
+// four at a time so their rows' loads are in flight together, and the adds
+// still run in CSR order, as with one channel per thread.
+__device__ __forceinline__ float4 add4(float4 a, float4 b) {
+    return make_float4(a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w);
+}
+
+__device__ __forceinline__ float4 row4(const float* __restrict__ p, size_t at) {
+    return *reinterpret_cast<const float4*>(p + at);
+}
+
 __global__ __launch_bounds__(256) void aggregate_fp32(const float* __restrict__ h, GraphView g, uint32_t width,
                                                       float* __restrict__ z) {
+    const uint32_t quads = width / 4;
     const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= size_t(g.vertices) * width) return;
-    const uint32_t v = uint32_t(i / width), c = uint32_t(i % width);
+    if (i >= size_t(g.vertices) * quads) return;
+    const uint32_t v = uint32_t(i / quads), c = uint32_t(i % quads) * 4;
     const uint32_t b = g.offsets[v], e = g.offsets[v + 1];
-    float sum = 0.f;
-    for (uint32_t k = b; k < e; ++k) sum = sum + h[size_t(g.neighbors[k]) * width + c];
-    z[size_t(v) * 2 * width + c] = h[i];
-    z[size_t(v) * 2 * width + width + c] = sum / float(e - b > 0 ? e - b : 1u);
+    float4 sum = make_float4(0.f, 0.f, 0.f, 0.f);
+    uint32_t k = b;
+    for (; k + 4 <= e; k += 4) {
+        const uint32_t n0 = g.neighbors[k], n1 = g.neighbors[k + 1], n2 = g.neighbors[k + 2],
+                       n3 = g.neighbors[k + 3];
+        const float4 a0 = row4(h, size_t(n0) * width + c), a1 = row4(h, size_t(n1) * width + c),
+                     a2 = row4(h, size_t(n2) * width + c), a3 = row4(h, size_t(n3) * width + c);
+        sum = add4(add4(add4(add4(sum, a0), a1), a2), a3);
+    }
+    for (; k < e; ++k) sum = add4(sum, row4(h, size_t(g.neighbors[k]) * width + c));
+    const float d = float(e - b > 0 ? e - b : 1u);
+    *reinterpret_cast<float4*>(z + size_t(v) * 2 * width + c) = row4(h, size_t(v) * width + c);
+    *reinterpret_cast<float4*>(z + size_t(v) * 2 * width + width + c) =
+        make_float4(sum.x / d, sum.y / d, sum.z / d, sum.w / d);
 }
 
 // Compensated linear layer (src/neural/inference.cu:75-108). Eight lanes per
@@ -503,7 +525,7 @@ void launch_forward_fp32(const float* x, GraphView g, const float* weights, floa
     uint32_t width = kFeatures;
     float* targets[3] = {h_a, h_b, h_a};
     for (int l = 0; l < 3; ++l) {
-        const size_t n = size_t(g.vertices) * width;
+        const size_t n = size_t(g.vertices) * width / 4;
         if (n) aggregate_fp32<<<dim3(uint32_t((n + 255) / 256)), dim3(256), 0, stream>>>(h, g, width, z);
         const size_t threads = size_t(g.vertices) * kHidden * 8;
         if (threads)
@@ -534,7 +556,8 @@ void launch_convert_bf16(const float* x, __bf16* y, size_t n, hipStream_t stream
 }
 
 void launch_aggregate_fp32(const float* h, GraphView g, uint32_t width, float* z, hipStream_t stream) {
-    const size_t n = size_t(g.vertices) * width;
+    if (width % 4) throw std::invalid_argument("aggregate_fp32: width must be a multiple of 4");
+    const size_t n = size_t(g.vertices) * width / 4;
     if (n) aggregate_fp32<<<dim3(uint32_t((n + 255) / 256)), dim3(256), 0, stream>>>(h, g, width, z);
 }
 

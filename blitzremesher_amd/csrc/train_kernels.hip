@@ -77,6 +77,10 @@ __device__ __forceinline__ floatx4 mfma4(float a, float b, floatx4 c) {
 
 __device__ __forceinline__ float4 ld4(const float* p) { return *reinterpret_cast<const float4*>(p); }
 
+__device__ __forceinline__ float4 add4(float4 a, float4 b) {
+    return make_float4(a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w);
+}
+
 __device__ __forceinline__ float part_of(const float4& v, int s) {
     return s == 0 ? v.x : s == 1 ? v.y : s == 2 ? v.z : v.w;
 }
@@ -287,18 +291,30 @@ __global__ void scatter_edge(const float* __restrict__ dz, const uint32_t* __res
 }
 
 // dh[v] = t[v][0:d] + sum over u in N(v) of t[u][d:2d] / deg(u): the
-// transpose of the mean aggregation on a symmetric neighbour graph.
-__global__ void aggregate_backward(const float* __restrict__ t, GraphView g, uint32_t width, float* __restrict__ dh) {
+// transpose of the mean aggregation on a symmetric neighbour graph. Four
+// channels per thread, neighbours four at a time, adds in CSR order.
+__global__ __launch_bounds__(256) void aggregate_backward(const float* __restrict__ t, GraphView g, uint32_t width,
+                                                          float* __restrict__ dh) {
+    const uint32_t quads = width / 4;
     const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= size_t(g.vertices) * width) return;
-    const uint32_t v = uint32_t(i / width), c = uint32_t(i % width);
-    float s = t[size_t(v) * 2 * width + c];
-    for (uint32_t k = g.offsets[v]; k < g.offsets[v + 1]; ++k) {
-        const uint32_t u = g.neighbors[k];
+    if (i >= size_t(g.vertices) * quads) return;
+    const uint32_t v = uint32_t(i / quads), c = uint32_t(i % quads) * 4;
+    auto part = [&](uint32_t u) {
         const uint32_t deg = g.offsets[u + 1] - g.offsets[u];
-        s += t[size_t(u) * 2 * width + width + c] / float(deg > 0 ? deg : 1u);
+        const float d = float(deg > 0 ? deg : 1u);
+        const float4 x = *reinterpret_cast<const float4*>(t + size_t(u) * 2 * width + width + c);
+        return make_float4(x.x / d, x.y / d, x.z / d, x.w / d);
+    };
+    float4 s = *reinterpret_cast<const float4*>(t + size_t(v) * 2 * width + c);
+    const uint32_t e = g.offsets[v + 1];
+    uint32_t k = g.offsets[v];
+    for (; k + 4 <= e; k += 4) {
+        const float4 a0 = part(g.neighbors[k]), a1 = part(g.neighbors[k + 1]), a2 = part(g.neighbors[k + 2]),
+                     a3 = part(g.neighbors[k + 3]);
+        s = add4(add4(add4(add4(s, a0), a1), a2), a3);
     }
-    dh[i] = s;
+    for (; k < e; ++k) s = add4(s, part(g.neighbors[k]));
+    *reinterpret_cast<float4*>(dh + size_t(v) * width + c) = s;
 }
 
 __global__ void norm_kernel(const float* __restrict__ g, uint32_t n, float* __restrict__ sum) {
@@ -395,7 +411,8 @@ void launch_scatter_edge(const float* dz, const uint32_t* src, const uint32_t* d
 }
 
 void launch_aggregate_backward(const float* t, GraphView g, uint32_t width, float* dh, hipStream_t stream) {
-    const size_t n = size_t(g.vertices) * width;
+    if (width % 4) throw std::invalid_argument("aggregate_backward: width must be a multiple of 4");
+    const size_t n = size_t(g.vertices) * width / 4;
     if (!n) return;
     aggregate_backward<<<dim3(uint32_t((n + 255) / 256)), dim3(256), 0, stream>>>(t, g, width, dh);
 }
