@@ -173,45 +173,59 @@ __global__ __launch_bounds__(256) void aggregate_fp32(const float* __restrict__ 
 }
 
 // Compensated linear layer (src/neural/inference.cu:75-108). Eight lanes per
-// output; with `src` set, row r's input is [h[src[r]] ; h[dst[r]]].
+// four outputs of one row: lane j takes k = j, j + 8, ... and keeps one
+// compensated (Kahan) sum per output, in the order eight lanes per output
+// would, so each output has the same bits; the input value is loaded once
+// for the four outputs, whose sums are independent dependency chains. The
+// eight partial sums and residuals are then folded per output in lane order.
+// With `src` set, row r's input is [h[src[r]] ; h[dst[r]]]. out % 4 == 0.
 __global__ __launch_bounds__(256) void compensated_linear(const float* __restrict__ input, const uint32_t* src,
                                                           const uint32_t* dst, const float* __restrict__ weights,
                                                           const float* __restrict__ bias, float* __restrict__ output,
                                                           uint32_t rows, uint32_t in, uint32_t out, bool relu) {
+    const uint32_t quads = out / 4;
     const size_t i = (size_t(blockIdx.x) * blockDim.x + threadIdx.x) / 8;
     const uint32_t lane = threadIdx.x & 7;
-    const bool live = i < size_t(rows) * out;
-    const size_t row = live ? i / out : 0;
-    const uint32_t channel = live ? uint32_t(i % out) : 0;
-    const float* w = weights + size_t(channel) * in;
-    const uint32_t half = in / 2;
-    float sum = 0.f, correction = 0.f;
+    const bool live = i < size_t(rows) * quads;
+    const size_t row = live ? i / quads : 0;
+    const uint32_t c0 = live ? uint32_t(i % quads) * 4 : 0;
+    // Input of this row: k < split from x0[k], the rest from x1[k - split].
+    const uint32_t split = src ? in / 2 : in;
+    const float* x0 = input + row * in;
+    const float* x1 = x0;
+    if (src && live) {
+        x0 = input + size_t(src[row]) * split;
+        x1 = input + size_t(dst[row]) * split;
+    }
+    float sum[4] = {0.f, 0.f, 0.f, 0.f}, correction[4] = {0.f, 0.f, 0.f, 0.f};
     if (live)
         for (uint32_t k = lane; k < in; k += 8) {
-            float x;
-            if (src)
-                x = k < half ? input[size_t(src[row]) * half + k] : input[size_t(dst[row]) * half + (k - half)];
-            else
-                x = input[row * in + k];
-            const float term = __fmul_rn(x, w[k]) - correction;
-            const float next = __fadd_rn(sum, term);
-            correction = __fsub_rn(__fsub_rn(next, sum), term);
-            sum = next;
+            const float x = k < split ? x0[k] : x1[k - split];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const float term = __fmul_rn(x, weights[size_t(c0 + q) * in + k]) - correction[q];
+                const float next = __fadd_rn(sum[q], term);
+                correction[q] = __fsub_rn(__fsub_rn(next, sum[q]), term);
+                sum[q] = next;
+            }
         }
-    float total = live ? bias[channel] : 0.f, error = 0.f;
-    for (int j = 0; j < 8; ++j) {
-        const float part = __shfl(sum, j, 8), residual = __shfl(correction, j, 8);
-        if (lane == 0) {
-            float term = __fsub_rn(part, error), next = __fadd_rn(total, term);
-            error = __fsub_rn(__fsub_rn(next, total), term);
-            total = next;
-            term = __fsub_rn(-residual, error);
-            next = __fadd_rn(total, term);
-            error = __fsub_rn(__fsub_rn(next, total), term);
-            total = next;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        float total = live ? bias[c0 + q] : 0.f, error = 0.f;
+        for (int j = 0; j < 8; ++j) {
+            const float part = __shfl(sum[q], j, 8), residual = __shfl(correction[q], j, 8);
+            if (lane == 0) {
+                float term = __fsub_rn(part, error), next = __fadd_rn(total, term);
+                error = __fsub_rn(__fsub_rn(next, total), term);
+                total = next;
+                term = __fsub_rn(-residual, error);
+                next = __fadd_rn(total, term);
+                error = __fsub_rn(__fsub_rn(next, total), term);
+                total = next;
+            }
         }
+        if (live && lane == 0) output[row * out + c0 + q] = relu && total < 0 ? 0.f : total;
     }
-    if (live && lane == 0) output[i] = relu && total < 0 ? 0.f : total;
 }
 
 // ------------------------------------------------------------- BF16 / MFMA
@@ -527,7 +541,7 @@ void launch_forward_fp32(const float* x, GraphView g, const float* weights, floa
     for (int l = 0; l < 3; ++l) {
         const size_t n = size_t(g.vertices) * width / 4;
         if (n) aggregate_fp32<<<dim3(uint32_t((n + 255) / 256)), dim3(256), 0, stream>>>(h, g, width, z);
-        const size_t threads = size_t(g.vertices) * kHidden * 8;
+        const size_t threads = size_t(g.vertices) * (kHidden / 4) * 8;
         if (threads)
             compensated_linear<<<dim3(uint32_t((threads + 255) / 256)), dim3(256), 0, stream>>>(
                 z, nullptr, nullptr, weights + layer_offset(l), weights + layer_offset(l) + kHidden * kLayerIn[l],
@@ -535,12 +549,12 @@ void launch_forward_fp32(const float* x, GraphView g, const float* weights, floa
         h = targets[l];
         width = kHidden;
     }
-    const size_t t1 = size_t(g.actions) * kHidden * 8;
+    const size_t t1 = size_t(g.actions) * (kHidden / 4) * 8;
     if (t1)
         compensated_linear<<<dim3(uint32_t((t1 + 255) / 256)), dim3(256), 0, stream>>>(
             h, g.src, g.neighbors, weights + layer_offset(3), weights + layer_offset(3) + kHidden * kEdgeIn, z,
             g.actions, kEdgeIn, kHidden, true);
-    const size_t t2 = size_t(g.actions) * kOutputs * 8;
+    const size_t t2 = size_t(g.actions) * (kOutputs / 4) * 8;
     if (t2)
         compensated_linear<<<dim3(uint32_t((t2 + 255) / 256)), dim3(256), 0, stream>>>(
             z, nullptr, nullptr, weights + layer_offset(4), weights + layer_offset(4) + kOutputs * kHidden, out,
@@ -564,7 +578,8 @@ void launch_aggregate_fp32(const float* h, GraphView g, uint32_t width, float* z
 void launch_compensated(const float* input, const uint32_t* src, const uint32_t* dst, const float* weights,
                         const float* bias, float* output, uint32_t rows, uint32_t in, uint32_t out, bool relu,
                         hipStream_t stream) {
-    const size_t threads = size_t(rows) * out * 8;
+    if (out % 4) throw std::invalid_argument("compensated_linear: out must be a multiple of 4");
+    const size_t threads = size_t(rows) * (out / 4) * 8;
     if (threads)
         compensated_linear<<<dim3(uint32_t((threads + 255) / 256)), dim3(256), 0, stream>>>(
             input, src, dst, weights, bias, output, rows, in, out, relu);
