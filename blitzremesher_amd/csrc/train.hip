@@ -139,13 +139,13 @@ TrainResult Scorer::train_step(Graph& g, const uint32_t* actions, const uint8_t*
     float* t_e = grow<float>(g.t_e, size_t(nb) * kHidden);
     float* z_e = grow<float>(g.z_e, size_t(nb) * kEdgeIn);
     float* dz_e = grow<float>(g.dz_e, size_t(nb) * kEdgeIn);
-    launch_gemm_at_b(dy, kOutputs, nullptr, hidden, kHidden, gr + layer_offset(4), nb, kOutputs, kHidden, st);
-    launch_column_sum(dy, nullptr, gr + layer_offset(4) + kOutputs * kHidden, nb, kOutputs, st);
+    launch_gemm_at_b(dy, kOutputs, nullptr, hidden, kHidden, gr + layer_offset(4),
+                     gr + layer_offset(4) + kOutputs * kHidden, nb, kOutputs, kHidden, st);
     launch_gemm_a_w(dy, nullptr, w + layer_offset(4), t_e, nb, kOutputs, kHidden, st);
     // z_e = [h3(src) ; h3(dst)] for the edge weight gradient.
     launch_gather_edge(h[3], d_src, d_dst, z_e, nb, st);
-    launch_gemm_at_b(t_e, kHidden, hidden, z_e, kEdgeIn, gr + layer_offset(3), nb, kHidden, kEdgeIn, st);
-    launch_column_sum(t_e, hidden, gr + layer_offset(3) + kHidden * kEdgeIn, nb, kHidden, st);
+    launch_gemm_at_b(t_e, kHidden, hidden, z_e, kEdgeIn, gr + layer_offset(3), gr + layer_offset(3) + kHidden * kEdgeIn,
+                     nb, kHidden, kEdgeIn, st);
     launch_gemm_a_w(t_e, hidden, w + layer_offset(3), dz_e, nb, kHidden, kEdgeIn, st);
     float* dh = grow<float>(g.dh_a, size_t(nv) * kHidden);
     float* dh_next = grow<float>(g.dh_b, size_t(nv) * kHidden);
@@ -155,8 +155,8 @@ TrainResult Scorer::train_step(Graph& g, const uint32_t* actions, const uint8_t*
     float* t_v = grow<float>(g.t_v, size_t(nv) * 2 * kHidden);
     for (int l = 2; l >= 0; --l) {
         const uint32_t in = kLayerIn[l];
-        launch_gemm_at_b(dh, kHidden, h[l + 1], z[l], in, gr + layer_offset(l), nv, kHidden, in, st);
-        launch_column_sum(dh, h[l + 1], gr + layer_offset(l) + kHidden * in, nv, kHidden, st);
+        launch_gemm_at_b(dh, kHidden, h[l + 1], z[l], in, gr + layer_offset(l), gr + layer_offset(l) + kHidden * in, nv,
+                         kHidden, in, st);
         if (l > 0) {
             launch_gemm_a_w(dh, h[l + 1], w + layer_offset(l), t_v, nv, kHidden, in, st);
             launch_aggregate_backward(t_v, gv, in / 2, dh_next, st);

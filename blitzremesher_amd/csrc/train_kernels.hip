@@ -99,12 +99,16 @@ __device__ __forceinline__ float keep(bool in, float v, float mask) { return in 
 // of B and feeds component s to sub-tile s, so sub-tile s holds tile columns
 // 4*i + s (rows of C) and 4*j + s (columns of C): a fixed permutation undone
 // at the write. The waves' tiles are combined in LDS and added atomically.
+// With `sums`, the column sums of (A masked) (the bias gradients) are
+// accumulated from the same loaded values and added by the blocks of the
+// first column tile of C.
 template <bool kMasked>
 __global__ __launch_bounds__(256) void gemm_at_b(const float* __restrict__ a, uint32_t lda,
                                                  const float* __restrict__ a_mask, const float* __restrict__ b,
-                                                 uint32_t ldb, float* __restrict__ c, uint32_t rows, uint32_t m,
-                                                 uint32_t n, uint32_t rows_per_slice) {
+                                                 uint32_t ldb, float* __restrict__ c, float* __restrict__ sums,
+                                                 uint32_t rows, uint32_t m, uint32_t n, uint32_t rows_per_slice) {
     __shared__ floatx4 part[3][16][64];
+    __shared__ float part_sums[3][64];
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const uint32_t tm = blockIdx.x * 64, tn = blockIdx.y * 64;
     const uint32_t s0 = blockIdx.z * rows_per_slice, s1 = min(rows, s0 + rows_per_slice);
@@ -129,6 +133,7 @@ __global__ __launch_bounds__(256) void gemm_at_b(const float* __restrict__ a, ui
     floatx4 acc[4][4];
     for (int i = 0; i < 4; ++i)
         for (int j = 0; j < 4; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+    float col[4] = {0.f, 0.f, 0.f, 0.f};  // column sums of (A masked): bias gradients
     Rows cur, next;
     fetch(r0, cur);
     for (uint32_t r = r0; r < r1; r += 16) {
@@ -144,17 +149,34 @@ __global__ __launch_bounds__(256) void gemm_at_b(const float* __restrict__ a, ui
                 bv[i] = ib ? part_of(cur.b[s], i) : 0.f;
             }
 #pragma unroll
-            for (int i = 0; i < 4; ++i)
+            for (int i = 0; i < 4; ++i) {
+                col[i] += av[i];
 #pragma unroll
                 for (int j = 0; j < 4; ++j) acc[i][j] = mfma4(av[i], bv[j], acc[i][j]);
+            }
         }
         cur = next;
     }
-    if (wave)
+    // Lanes l, l+16, l+32, l+48 hold the same four columns of A.
+    for (int i = 0; i < 4; ++i) {
+        col[i] += __shfl_xor(col[i], 16);
+        col[i] += __shfl_xor(col[i], 32);
+    }
+    if (wave) {
         for (int i = 0; i < 4; ++i)
             for (int j = 0; j < 4; ++j) part[wave - 1][i * 4 + j][lane] = acc[i][j];
+        if (lane < 16)
+            for (int i = 0; i < 4; ++i) part_sums[wave - 1][4 * lane + i] = col[i];
+    }
     __syncthreads();
     if (wave) return;
+    // The column sums of A are added once, by the first column tile of C.
+    if (sums && blockIdx.y == 0 && lane < 16 && a_in)
+        for (int i = 0; i < 4; ++i) {
+            const float v = col[i] + part_sums[0][4 * lane + i] + part_sums[1][4 * lane + i] +
+                            part_sums[2][4 * lane + i];
+            if (v != 0.f) atomicAdd(&sums[ma + i], v);
+        }
     for (int i = 0; i < 4; ++i)
         for (int j = 0; j < 4; ++j) {
             const floatx4 v = acc[i][j] + part[0][i * 4 + j][lane] + part[1][i * 4 + j][lane] +
@@ -235,30 +257,6 @@ __global__ __launch_bounds__(256) void gemm_a_w(const float* __restrict__ a, con
                 if (ri < rows && nj < n) t[size_t(ri) * n + nj] = acc[i][j][q];
             }
         }
-}
-
-// Column sums of (A masked): bias gradients. 256 threads cover 256 / cols
-// rows of `cols` consecutive floats per step (coalesced); each block folds a
-// slice of rows and adds its column partials atomically.
-__global__ __launch_bounds__(256) void column_sum(const float* __restrict__ a, const float* __restrict__ a_mask,
-                                                  float* __restrict__ out, uint32_t rows, uint32_t cols,
-                                                  uint32_t rows_per_block) {
-    const uint32_t c = threadIdx.x % cols, step = blockDim.x / cols;
-    const uint32_t r0 = blockIdx.x * rows_per_block, r1 = min(rows, r0 + rows_per_block);
-    float s = 0.f;
-    for (uint32_t r = r0 + threadIdx.x / cols; r < r1; r += step) {
-        float v = a[size_t(r) * cols + c];
-        if (a_mask && !(a_mask[size_t(r) * cols + c] > 0.f)) v = 0.f;
-        s += v;
-    }
-    __shared__ float part[256];
-    part[threadIdx.x] = s;
-    __syncthreads();
-    if (threadIdx.x < cols) {
-        float t = 0.f;
-        for (uint32_t k = threadIdx.x; k < blockDim.x; k += cols) t += part[k];
-        atomicAdd(&out[threadIdx.x], t);
-    }
 }
 
 // Endpoints of the batch's actions.
@@ -354,7 +352,7 @@ void launch_loss(const float* y, const uint8_t* labels, const float* targets, fl
 }
 
 void launch_gemm_at_b(const float* a, uint32_t lda, const float* a_mask, const float* b, uint32_t ldb, float* c,
-                      uint32_t rows, uint32_t m, uint32_t n, hipStream_t stream) {
+                      float* sums, uint32_t rows, uint32_t m, uint32_t n, hipStream_t stream) {
     if (!rows || !m || !n) return;
     if ((m | n | lda | ldb) % 4) throw std::invalid_argument("gemm_at_b: m, n and row strides must be multiples of 4");
     // About 512 blocks (two per CU) of at least 256 rows each.
@@ -363,9 +361,9 @@ void launch_gemm_at_b(const float* a, uint32_t lda, const float* a_mask, const f
     const uint32_t per = (rows + slices - 1) / slices;
     const dim3 grid((m + 63) / 64, (n + 63) / 64, slices);
     if (a_mask)
-        gemm_at_b<true><<<grid, dim3(256), 0, stream>>>(a, lda, a_mask, b, ldb, c, rows, m, n, per);
+        gemm_at_b<true><<<grid, dim3(256), 0, stream>>>(a, lda, a_mask, b, ldb, c, sums, rows, m, n, per);
     else
-        gemm_at_b<false><<<grid, dim3(256), 0, stream>>>(a, lda, a_mask, b, ldb, c, rows, m, n, per);
+        gemm_at_b<false><<<grid, dim3(256), 0, stream>>>(a, lda, a_mask, b, ldb, c, sums, rows, m, n, per);
 }
 
 void launch_gemm_a_w(const float* a, const float* a_mask, const float* w, float* t, uint32_t rows, uint32_t k,
@@ -378,16 +376,6 @@ void launch_gemm_a_w(const float* a, const float* a_mask, const float* w, float*
         gemm_a_w<true><<<grid, block, 0, stream>>>(a, a_mask, w, t, rows, k, n);
     else
         gemm_a_w<false><<<grid, block, 0, stream>>>(a, a_mask, w, t, rows, k, n);
-}
-
-void launch_column_sum(const float* a, const float* a_mask, float* out, uint32_t rows, uint32_t cols,
-                       hipStream_t stream) {
-    if (!rows || !cols) return;
-    if (cols > 256 || 256 % cols) throw std::invalid_argument("column_sum: cols must divide 256");
-    // About a thousand blocks of at least 64 rows: enough to fill the chip,
-    // few enough atomics per column.
-    const uint32_t per = std::max<uint32_t>(64, (rows + 1023) / 1024);
-    column_sum<<<dim3((rows + per - 1) / per), dim3(256), 0, stream>>>(a, a_mask, out, rows, cols, per);
 }
 
 void launch_batch_endpoints(const uint32_t* actions, GraphView g, uint32_t* src, uint32_t* dst, uint32_t count,

@@ -24,14 +24,13 @@ struct AdamStep {
 
 void launch_loss(const float* y, const uint8_t* labels, const float* targets, float* dy, float* loss,
                  const LossInput& in, hipStream_t stream);
-// c[m][n] += sum over rows of (a masked)[r][m] * b[r][n].
+// c[m][n] += sum over rows of (a masked)[r][m] * b[r][n] and, when `sums` is
+// set, sums[m] += sum over rows of (a masked)[r][m].
 void launch_gemm_at_b(const float* a, uint32_t lda, const float* a_mask, const float* b, uint32_t ldb, float* c,
-                      uint32_t rows, uint32_t m, uint32_t n, hipStream_t stream);
+                      float* sums, uint32_t rows, uint32_t m, uint32_t n, hipStream_t stream);
 // t[r][n] = sum_k (a masked)[r][k] * w[k][n].
 void launch_gemm_a_w(const float* a, const float* a_mask, const float* w, float* t, uint32_t rows, uint32_t k,
                      uint32_t n, hipStream_t stream);
-void launch_column_sum(const float* a, const float* a_mask, float* out, uint32_t rows, uint32_t cols,
-                       hipStream_t stream);
 void launch_batch_endpoints(const uint32_t* actions, GraphView g, uint32_t* src, uint32_t* dst, uint32_t count,
                             hipStream_t stream);
 void launch_gather_edge(const float* h, const uint32_t* src, const uint32_t* dst, float* z, uint32_t count,
