@@ -173,22 +173,24 @@ __global__ __launch_bounds__(256) void aggregate_fp32(const float* __restrict__ 
 }
 
 // Compensated linear layer (src/neural/inference.cu:75-108). Eight lanes per
-// four outputs of one row: lane j takes k = j, j + 8, ... and keeps one
-// compensated (Kahan) sum per output, in the order eight lanes per output
-// would, so each output has the same bits; the input value is loaded once
-// for the four outputs, whose sums are independent dependency chains. The
-// eight partial sums and residuals are then folded per output in lane order.
-// With `src` set, row r's input is [h[src[r]] ; h[dst[r]]]. out % 4 == 0.
+// G outputs of one row (G = 4 or 8): lane j takes k = j, j + 8, ... and keeps
+// one compensated (Kahan) sum per output, in the order eight lanes per output
+// would, so each output has the same bits; the input value is loaded once for
+// the G outputs, whose sums are independent dependency chains. Output q's
+// eight partial sums and residuals are then folded in lane order by lane q,
+// the G folds running side by side. With `src` set, row r's input is
+// [h[src[r]] ; h[dst[r]]]. out % G == 0.
+template <int G>
 __global__ __launch_bounds__(256) void compensated_linear(const float* __restrict__ input, const uint32_t* src,
                                                           const uint32_t* dst, const float* __restrict__ weights,
                                                           const float* __restrict__ bias, float* __restrict__ output,
                                                           uint32_t rows, uint32_t in, uint32_t out, bool relu) {
-    const uint32_t quads = out / 4;
+    const uint32_t groups = out / G;
     const size_t i = (size_t(blockIdx.x) * blockDim.x + threadIdx.x) / 8;
     const uint32_t lane = threadIdx.x & 7;
-    const bool live = i < size_t(rows) * quads;
-    const size_t row = live ? i / quads : 0;
-    const uint32_t c0 = live ? uint32_t(i % quads) * 4 : 0;
+    const bool live = i < size_t(rows) * groups;
+    const size_t row = live ? i / groups : 0;
+    const uint32_t c0 = live ? uint32_t(i % groups) * G : 0;
     // Input of this row: k < split from x0[k], the rest from x1[k - split].
     const uint32_t split = src ? in / 2 : in;
     const float* x0 = input + row * in;
@@ -197,35 +199,43 @@ __global__ __launch_bounds__(256) void compensated_linear(const float* __restric
         x0 = input + size_t(src[row]) * split;
         x1 = input + size_t(dst[row]) * split;
     }
-    float sum[4] = {0.f, 0.f, 0.f, 0.f}, correction[4] = {0.f, 0.f, 0.f, 0.f};
+    float sum[G], correction[G];
+#pragma unroll
+    for (int q = 0; q < G; ++q) sum[q] = correction[q] = 0.f;
     if (live)
         for (uint32_t k = lane; k < in; k += 8) {
             const float x = k < split ? x0[k] : x1[k - split];
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
+            for (int q = 0; q < G; ++q) {
                 const float term = __fmul_rn(x, weights[size_t(c0 + q) * in + k]) - correction[q];
                 const float next = __fadd_rn(sum[q], term);
                 correction[q] = __fsub_rn(__fsub_rn(next, sum[q]), term);
                 sum[q] = next;
             }
         }
+    const bool folds = live && lane < G;
+    float total = folds ? bias[c0 + lane] : 0.f, error = 0.f;
+    for (int j = 0; j < 8; ++j) {
+        float part = 0.f, residual = 0.f;
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        float total = live ? bias[c0 + q] : 0.f, error = 0.f;
-        for (int j = 0; j < 8; ++j) {
-            const float part = __shfl(sum[q], j, 8), residual = __shfl(correction[q], j, 8);
-            if (lane == 0) {
-                float term = __fsub_rn(part, error), next = __fadd_rn(total, term);
-                error = __fsub_rn(__fsub_rn(next, total), term);
-                total = next;
-                term = __fsub_rn(-residual, error);
-                next = __fadd_rn(total, term);
-                error = __fsub_rn(__fsub_rn(next, total), term);
-                total = next;
+        for (int q = 0; q < G; ++q) {
+            const float p = __shfl(sum[q], j, 8), r = __shfl(correction[q], j, 8);
+            if (lane == uint32_t(q)) {
+                part = p;
+                residual = r;
             }
         }
-        if (live && lane == 0) output[row * out + c0 + q] = relu && total < 0 ? 0.f : total;
+        if (folds) {
+            float term = __fsub_rn(part, error), next = __fadd_rn(total, term);
+            error = __fsub_rn(__fsub_rn(next, total), term);
+            total = next;
+            term = __fsub_rn(-residual, error);
+            next = __fadd_rn(total, term);
+            error = __fsub_rn(__fsub_rn(next, total), term);
+            total = next;
+        }
     }
+    if (folds) output[row * out + c0 + lane] = relu && total < 0 ? 0.f : total;
 }
 
 // ------------------------------------------------------------- BF16 / MFMA
@@ -541,22 +551,22 @@ void launch_forward_fp32(const float* x, GraphView g, const float* weights, floa
     for (int l = 0; l < 3; ++l) {
         const size_t n = size_t(g.vertices) * width / 4;
         if (n) aggregate_fp32<<<dim3(uint32_t((n + 255) / 256)), dim3(256), 0, stream>>>(h, g, width, z);
-        const size_t threads = size_t(g.vertices) * (kHidden / 4) * 8;
+        const size_t threads = size_t(g.vertices) * (kHidden / 8) * 8;
         if (threads)
-            compensated_linear<<<dim3(uint32_t((threads + 255) / 256)), dim3(256), 0, stream>>>(
+            compensated_linear<8><<<dim3(uint32_t((threads + 255) / 256)), dim3(256), 0, stream>>>(
                 z, nullptr, nullptr, weights + layer_offset(l), weights + layer_offset(l) + kHidden * kLayerIn[l],
                 targets[l], g.vertices, kLayerIn[l], kHidden, true);
         h = targets[l];
         width = kHidden;
     }
-    const size_t t1 = size_t(g.actions) * (kHidden / 4) * 8;
+    const size_t t1 = size_t(g.actions) * (kHidden / 8) * 8;
     if (t1)
-        compensated_linear<<<dim3(uint32_t((t1 + 255) / 256)), dim3(256), 0, stream>>>(
+        compensated_linear<8><<<dim3(uint32_t((t1 + 255) / 256)), dim3(256), 0, stream>>>(
             h, g.src, g.neighbors, weights + layer_offset(3), weights + layer_offset(3) + kHidden * kEdgeIn, z,
             g.actions, kEdgeIn, kHidden, true);
     const size_t t2 = size_t(g.actions) * (kOutputs / 4) * 8;
     if (t2)
-        compensated_linear<<<dim3(uint32_t((t2 + 255) / 256)), dim3(256), 0, stream>>>(
+        compensated_linear<4><<<dim3(uint32_t((t2 + 255) / 256)), dim3(256), 0, stream>>>(
             z, nullptr, nullptr, weights + layer_offset(4), weights + layer_offset(4) + kOutputs * kHidden, out,
             g.actions, kHidden, kOutputs, false);
 }
@@ -579,10 +589,14 @@ void launch_compensated(const float* input, const uint32_t* src, const uint32_t*
                         const float* bias, float* output, uint32_t rows, uint32_t in, uint32_t out, bool relu,
                         hipStream_t stream) {
     if (out % 4) throw std::invalid_argument("compensated_linear: out must be a multiple of 4");
-    const size_t threads = size_t(rows) * (out / 4) * 8;
-    if (threads)
-        compensated_linear<<<dim3(uint32_t((threads + 255) / 256)), dim3(256), 0, stream>>>(
-            input, src, dst, weights, bias, output, rows, in, out, relu);
+    const int per = out % 8 ? 4 : 8;
+    const size_t threads = size_t(rows) * (out / per) * 8;
+    if (!threads) return;
+    const dim3 grid(uint32_t((threads + 255) / 256));
+    if (per == 8)
+        compensated_linear<8><<<grid, dim3(256), 0, stream>>>(input, src, dst, weights, bias, output, rows, in, out, relu);
+    else
+        compensated_linear<4><<<grid, dim3(256), 0, stream>>>(input, src, dst, weights, bias, output, rows, in, out, relu);
 }
 
 static void configure_bf16() {
