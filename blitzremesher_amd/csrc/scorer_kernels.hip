@@ -178,64 +178,80 @@ __global__ __launch_bounds__(256) void aggregate_fp32(const float* __restrict__ 
 // would, so each output has the same bits; the input value is loaded once for
 // the G outputs, whose sums are independent dependency chains. Output q's
 // eight partial sums and residuals are then folded in lane order by lane q,
-// the G folds running side by side. With `src` set, row r's input is
-// [h[src[r]] ; h[dst[r]]]. out % G == 0.
+// the G folds running side by side. A block takes one group of G outputs and
+// kCompRows rows per eight-lane group; the G weight rows are staged in LDS
+// once per block. With `src` set, row r's input is [h[src[r]] ; h[dst[r]]].
+// out % G == 0, in <= kCompMaxIn.
+constexpr uint32_t kCompMaxIn = 256, kCompRows = 4, kCompRowsPerBlock = 32 * kCompRows;
+static_assert(kEdgeIn <= kCompMaxIn && 2 * kHidden <= kCompMaxIn && 2 * kFeatures <= kCompMaxIn,
+              "layer inputs must fit the staged weight rows");
+
 template <int G>
 __global__ __launch_bounds__(256) void compensated_linear(const float* __restrict__ input, const uint32_t* src,
                                                           const uint32_t* dst, const float* __restrict__ weights,
                                                           const float* __restrict__ bias, float* __restrict__ output,
                                                           uint32_t rows, uint32_t in, uint32_t out, bool relu) {
+    __shared__ float w[G * kCompMaxIn];
     const uint32_t groups = out / G;
-    const size_t i = (size_t(blockIdx.x) * blockDim.x + threadIdx.x) / 8;
+    const uint32_t c0 = (blockIdx.x % groups) * G;
+    for (uint32_t e = threadIdx.x; e < G * in; e += blockDim.x) w[e] = weights[size_t(c0) * in + e];
+    __syncthreads();
     const uint32_t lane = threadIdx.x & 7;
-    const bool live = i < size_t(rows) * groups;
-    const size_t row = live ? i / groups : 0;
-    const uint32_t c0 = live ? uint32_t(i % groups) * G : 0;
-    // Input of this row: k < split from x0[k], the rest from x1[k - split].
     const uint32_t split = src ? in / 2 : in;
-    const float* x0 = input + row * in;
-    const float* x1 = x0;
-    if (src && live) {
-        x0 = input + size_t(src[row]) * split;
-        x1 = input + size_t(dst[row]) * split;
-    }
-    float sum[G], correction[G];
+    for (uint32_t rr = 0; rr < kCompRows; ++rr) {
+        const size_t row = size_t(blockIdx.x / groups) * kCompRowsPerBlock + rr * 32 + threadIdx.x / 8;
+        const bool live = row < rows;
+        // Input of this row: k < split from x0[k], the rest from x1[k - split].
+        const float* x0 = input + (live ? row : 0) * in;
+        const float* x1 = x0;
+        if (src && live) {
+            x0 = input + size_t(src[row]) * split;
+            x1 = input + size_t(dst[row]) * split;
+        }
+        float sum[G], correction[G];
 #pragma unroll
-    for (int q = 0; q < G; ++q) sum[q] = correction[q] = 0.f;
-    if (live)
-        for (uint32_t k = lane; k < in; k += 8) {
-            const float x = k < split ? x0[k] : x1[k - split];
+        for (int q = 0; q < G; ++q) sum[q] = correction[q] = 0.f;
+        if (live)
+            for (uint32_t k = lane; k < in; k += 8) {
+                const float x = k < split ? x0[k] : x1[k - split];
+#pragma unroll
+                for (int q = 0; q < G; ++q) {
+                    const float term = __fmul_rn(x, w[q * in + k]) - correction[q];
+                    const float next = __fadd_rn(sum[q], term);
+                    correction[q] = __fsub_rn(__fsub_rn(next, sum[q]), term);
+                    sum[q] = next;
+                }
+            }
+        const bool folds = live && lane < G;
+        float total = folds ? bias[c0 + lane] : 0.f, error = 0.f;
+        for (int j = 0; j < 8; ++j) {
+            float part = 0.f, residual = 0.f;
 #pragma unroll
             for (int q = 0; q < G; ++q) {
-                const float term = __fmul_rn(x, weights[size_t(c0 + q) * in + k]) - correction[q];
-                const float next = __fadd_rn(sum[q], term);
-                correction[q] = __fsub_rn(__fsub_rn(next, sum[q]), term);
-                sum[q] = next;
+                const float p = __shfl(sum[q], j, 8), r = __shfl(correction[q], j, 8);
+                if (lane == uint32_t(q)) {
+                    part = p;
+                    residual = r;
+                }
+            }
+            if (folds) {
+                float term = __fsub_rn(part, error), next = __fadd_rn(total, term);
+                error = __fsub_rn(__fsub_rn(next, total), term);
+                total = next;
+                term = __fsub_rn(-residual, error);
+                next = __fadd_rn(total, term);
+                error = __fsub_rn(__fsub_rn(next, total), term);
+                total = next;
             }
         }
-    const bool folds = live && lane < G;
-    float total = folds ? bias[c0 + lane] : 0.f, error = 0.f;
-    for (int j = 0; j < 8; ++j) {
-        float part = 0.f, residual = 0.f;
-#pragma unroll
-        for (int q = 0; q < G; ++q) {
-            const float p = __shfl(sum[q], j, 8), r = __shfl(correction[q], j, 8);
-            if (lane == uint32_t(q)) {
-                part = p;
-                residual = r;
-            }
-        }
-        if (folds) {
-            float term = __fsub_rn(part, error), next = __fadd_rn(total, term);
-            error = __fsub_rn(__fsub_rn(next, total), term);
-            total = next;
-            term = __fsub_rn(-residual, error);
-            next = __fadd_rn(total, term);
-            error = __fsub_rn(__fsub_rn(next, total), term);
-            total = next;
-        }
+        if (folds) output[row * out + c0 + lane] = relu && total < 0 ? 0.f : total;
     }
-    if (folds) output[row * out + c0 + lane] = relu && total < 0 ? 0.f : total;
+}
+
+// Grid of compensated_linear<G>: one block per output group and
+// kCompRowsPerBlock rows.
+inline dim3 comp_grid(uint32_t rows, uint32_t out, uint32_t g) {
+    return dim3((out / g) * ((rows + kCompRowsPerBlock - 1) / kCompRowsPerBlock));
 }
 
 // ------------------------------------------------------------- BF16 / MFMA
@@ -551,22 +567,19 @@ void launch_forward_fp32(const float* x, GraphView g, const float* weights, floa
     for (int l = 0; l < 3; ++l) {
         const size_t n = size_t(g.vertices) * width / 4;
         if (n) aggregate_fp32<<<dim3(uint32_t((n + 255) / 256)), dim3(256), 0, stream>>>(h, g, width, z);
-        const size_t threads = size_t(g.vertices) * (kHidden / 8) * 8;
-        if (threads)
-            compensated_linear<8><<<dim3(uint32_t((threads + 255) / 256)), dim3(256), 0, stream>>>(
+        if (g.vertices)
+            compensated_linear<8><<<comp_grid(g.vertices, kHidden, 8), dim3(256), 0, stream>>>(
                 z, nullptr, nullptr, weights + layer_offset(l), weights + layer_offset(l) + kHidden * kLayerIn[l],
                 targets[l], g.vertices, kLayerIn[l], kHidden, true);
         h = targets[l];
         width = kHidden;
     }
-    const size_t t1 = size_t(g.actions) * (kHidden / 8) * 8;
-    if (t1)
-        compensated_linear<8><<<dim3(uint32_t((t1 + 255) / 256)), dim3(256), 0, stream>>>(
+    if (g.actions)
+        compensated_linear<8><<<comp_grid(g.actions, kHidden, 8), dim3(256), 0, stream>>>(
             h, g.src, g.neighbors, weights + layer_offset(3), weights + layer_offset(3) + kHidden * kEdgeIn, z,
             g.actions, kEdgeIn, kHidden, true);
-    const size_t t2 = size_t(g.actions) * (kOutputs / 4) * 8;
-    if (t2)
-        compensated_linear<4><<<dim3(uint32_t((t2 + 255) / 256)), dim3(256), 0, stream>>>(
+    if (g.actions)
+        compensated_linear<4><<<comp_grid(g.actions, kOutputs, 4), dim3(256), 0, stream>>>(
             z, nullptr, nullptr, weights + layer_offset(4), weights + layer_offset(4) + kOutputs * kHidden, out,
             g.actions, kHidden, kOutputs, false);
 }
@@ -589,10 +602,10 @@ void launch_compensated(const float* input, const uint32_t* src, const uint32_t*
                         const float* bias, float* output, uint32_t rows, uint32_t in, uint32_t out, bool relu,
                         hipStream_t stream) {
     if (out % 4) throw std::invalid_argument("compensated_linear: out must be a multiple of 4");
+    if (in > kCompMaxIn) throw std::invalid_argument("compensated_linear: in must be at most 256");
     const int per = out % 8 ? 4 : 8;
-    const size_t threads = size_t(rows) * (out / per) * 8;
-    if (!threads) return;
-    const dim3 grid(uint32_t((threads + 255) / 256));
+    if (!rows || !out) return;
+    const dim3 grid = comp_grid(rows, out, per);
     if (per == 8)
         compensated_linear<8><<<grid, dim3(256), 0, stream>>>(input, src, dst, weights, bias, output, rows, in, out, relu);
     else
