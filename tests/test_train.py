@@ -174,10 +174,21 @@ TRAIN_CASES = {
 @pytest.mark.parametrize("case", sorted(TRAIN_CASES))
 @pytest.mark.parametrize("precision", [0, 1])
 def test_gpu_gradients_match_fp64(scorer, case, precision):
+    check_gradients(scorer, case, precision, 512)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("count", [1, 17, 509])
+def test_gpu_gradients_match_fp64_ragged_batches(scorer, count):
+    # Batch rows that fill neither a 16-row GEMM step nor a pool of 32.
+    check_gradients(scorer, "torus", 0, count)
+
+
+def check_gradients(scorer, case, precision, count):
     from blitzremesher_amd.scorer import TrainSettings, he_weights
     w = he_weights(4)
     scorer.set_weights(w)
-    g, offsets, neighbors, acts, ids, labels, targets = gpu_case(scorer, TRAIN_CASES[case](), 512, 11)
+    g, offsets, neighbors, acts, ids, labels, targets = gpu_case(scorer, TRAIN_CASES[case](), count, 11)
     s = TrainSettings(pool=32, precision=precision)
     r = scorer.train_step(g, ids, labels, targets, s)
     grad = scorer.gradients()
