@@ -339,13 +339,17 @@ __global__ __launch_bounds__(256) void gemm_bf16(const __bf16* __restrict__ a, u
 #pragma unroll
         for (int nb = 0; nb < 8; ++nb) acc[nb] = floatx4{0, 0, 0, 0};
 #pragma unroll
-        for (uint32_t s = 0; s < steps; ++s)
+        for (uint32_t s = 0; s < steps; ++s) {
 #pragma unroll
             for (int nb = 0; nb < 8; ++nb) {
                 const bf16x8 b =
                     *reinterpret_cast<const bf16x8*>(smem + (nb * 16 + (lane & 15)) * (K + kPad) + s * 32 + kq);
                 acc[nb] = mfma(cur[s], b, acc[nb]);
             }
+            // Keeps the next step's B-fragment reads below this step's
+            // MFMAs: hoisting all K/4 of them takes two waves' registers.
+            __builtin_amdgcn_sched_barrier(0);
+        }
 #pragma unroll
         for (int nb = 0; nb < 8; ++nb) {
             const uint32_t col = nb * 16 + (lane & 15);
