@@ -110,8 +110,14 @@ __global__ __launch_bounds__(256) void gemm_at_b(const float* __restrict__ a, ui
     __shared__ floatx4 part[3][16][64];
     __shared__ float part_sums[3][64];
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const uint32_t tm = blockIdx.x * 64, tn = blockIdx.y * 64;
-    const uint32_t s0 = blockIdx.z * rows_per_slice, s1 = min(rows, s0 + rows_per_slice);
+    // Blocks are dealt to the eight XCDs round-robin (block b on XCD b % 8);
+    // all tiles of one row slice go to the same XCD, one after another, so
+    // their waves walk the same rows together through that XCD's L2.
+    const uint32_t n_tiles = (n + 63) / 64, tiles = ((m + 63) / 64) * n_tiles;
+    const uint32_t local = blockIdx.x / 8, tile = local % tiles;
+    const uint32_t slice = (local / tiles) * 8 + blockIdx.x % 8;
+    const uint32_t tm = tile / n_tiles * 64, tn = tile % n_tiles * 64;
+    const uint32_t s0 = min(rows, slice * rows_per_slice), s1 = min(rows, s0 + rows_per_slice);
     const uint32_t quarter = ((rows_per_slice + 63) / 64) * 16;  // rows per wave, a multiple of 16
     const uint32_t r0 = min(s1, s0 + wave * quarter), r1 = min(s1, r0 + quarter);
     const uint32_t ma = tm + 4 * (lane % 16), nb = tn + 4 * (lane % 16);
@@ -171,7 +177,7 @@ __global__ __launch_bounds__(256) void gemm_at_b(const float* __restrict__ a, ui
     __syncthreads();
     if (wave) return;
     // The column sums of A are added once, by the first column tile of C.
-    if (sums && blockIdx.y == 0 && lane < 16 && a_in)
+    if (sums && tn == 0 && lane < 16 && a_in)
         for (int i = 0; i < 4; ++i) {
             const float v = col[i] + part_sums[0][4 * lane + i] + part_sums[1][4 * lane + i] +
                             part_sums[2][4 * lane + i];
@@ -355,11 +361,13 @@ void launch_gemm_at_b(const float* a, uint32_t lda, const float* a_mask, const f
                       float* sums, uint32_t rows, uint32_t m, uint32_t n, hipStream_t stream) {
     if (!rows || !m || !n) return;
     if ((m | n | lda | ldb) % 4) throw std::invalid_argument("gemm_at_b: m, n and row strides must be multiples of 4");
-    // About 512 blocks (two per CU) of at least 256 rows each.
+    // About 512 blocks (two per CU) of at least 256 rows each; the slice
+    // count is a multiple of 8 (one share per XCD).
     const uint32_t tiles = ((m + 63) / 64) * ((n + 63) / 64);
-    const uint32_t slices = std::max<uint32_t>(1, std::min<uint32_t>((rows + 255) / 256, 512 / tiles));
+    const uint32_t wanted = std::max<uint32_t>(1, std::min<uint32_t>((rows + 255) / 256, 512 / tiles));
+    const uint32_t slices = (wanted + 7) / 8 * 8;
     const uint32_t per = (rows + slices - 1) / slices;
-    const dim3 grid((m + 63) / 64, (n + 63) / 64, slices);
+    const dim3 grid(tiles * slices);
     if (a_mask)
         gemm_at_b<true><<<grid, dim3(256), 0, stream>>>(a, lda, a_mask, b, ldb, c, sums, rows, m, n, per);
     else
