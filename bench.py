@@ -22,6 +22,7 @@ import time
 from pathlib import Path
 
 ROOT = Path(__file__).resolve().parent
+COPY_PROBE = ROOT / "bench_calib" / "libblitz_copy_probe.so"
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
@@ -367,7 +368,6 @@ def copy_probe(mib: int) -> int:
     """Runs the counter-calibration copy kernel (see profiles/) and prints its
     bandwidth."""
     import ctypes as C
-    from blitzremesher_amd.build import COPY_PROBE
     lib = C.CDLL(str(COPY_PROBE))
     ms = C.c_double()
     rc = lib.blitz_copy_probe(0, C.c_size_t(mib << 20), 5, C.byref(ms))

@@ -1,7 +1,10 @@
 """In-tree build of the gfx950 library (libblitz_hip.so) and the test oracle.
 
-The library is one hipcc invocation over csrc/*.hip (kernels, host controller
-and the extern "C" layer) plus csrc/*.cpp (host-only code).
+Every csrc/*.hip (kernels, host controller, the extern "C" layer) and
+csrc/*.cpp (host-only code) is compiled to its own object under _build/ in
+parallel, then hipcc links them into libblitz_hip.so. Kernels are launched
+only from the translation unit that defines them, so no relocatable device
+code is needed.
 """
 from __future__ import annotations
 
@@ -9,13 +12,13 @@ import os
 import shutil
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 from pathlib import Path
 
 PACKAGE = Path(__file__).resolve().parent
 ROOT = PACKAGE.parent
 LIBRARY = PACKAGE / "libblitz_hip.so"
-# Counter calibration probe (a plain copy kernel), loaded only by bench.py.
-COPY_PROBE = PACKAGE / "calib" / "libblitz_copy_probe.so"
+OBJECTS = ROOT / "_build"
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
@@ -23,17 +26,17 @@ ARCH = "gfx950"
 # -ffp-contract=off keeps every FP64 multiply and add separately rounded, as
 # in the reference CPU audit (built with -ffp-contract=off). Packed FP32 is
 # disabled for all device code.
-HIP_FLAGS = [
+COMPILE_FLAGS = [
     f"--offload-arch={ARCH}",
     "-O3",
     "-std=c++20",
     "-fPIC",
-    "-shared",
     "-ffp-contract=off",
     "-fno-fast-math",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-command-line-argument",
 ]
+HIP_FLAGS = [*COMPILE_FLAGS, "-shared"]
 
 
 def sources() -> list[Path]:
@@ -41,18 +44,30 @@ def sources() -> list[Path]:
     return sorted(csrc.glob("*.hip")) + sorted(csrc.glob("*.cpp"))
 
 
+def _jobs() -> int:
+    return max(1, min(16, os.cpu_count() or 1))
+
+
 def build_library(verbose: bool = False) -> Path:
-    cmd = [HIPCC, *HIP_FLAGS, "-I", str(ROOT / "include"), "-o", str(LIBRARY) + ".tmp",
-           *[str(s) for s in sources()]]
+    OBJECTS.mkdir(exist_ok=True)
+    srcs = sources()
+    objs = [OBJECTS / (s.name + ".o") for s in srcs]
+
+    def compile_one(pair: tuple[Path, Path]) -> None:
+        src, obj = pair
+        cmd = [HIPCC, *COMPILE_FLAGS, "-I", str(ROOT / "include"), "-c", str(src), "-o", str(obj) + ".tmp"]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.run(cmd, check=True)
+        os.replace(str(obj) + ".tmp", obj)
+
+    with ThreadPoolExecutor(_jobs()) as pool:
+        list(pool.map(compile_one, zip(srcs, objs)))
+    link = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(LIBRARY) + ".tmp", *map(str, objs)]
     if verbose:
-        print(" ".join(cmd), flush=True)
-    subprocess.run(cmd, check=True)
+        print(" ".join(link), flush=True)
+    subprocess.run(link, check=True)
     os.replace(str(LIBRARY) + ".tmp", LIBRARY)
-    probe = [HIPCC, *HIP_FLAGS, "-o", str(COPY_PROBE) + ".tmp", str(PACKAGE / "calib" / "copy_probe.hip")]
-    if verbose:
-        print(" ".join(probe), flush=True)
-    subprocess.run(probe, check=True)
-    os.replace(str(COPY_PROBE) + ".tmp", COPY_PROBE)
     return LIBRARY
 
 

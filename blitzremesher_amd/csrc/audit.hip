@@ -348,10 +348,14 @@ bool attributed_weights(const Settings& s) {
 // (sqrt(float(d)) / ss, as the host rounds it) settles a coverage view as
 // failing at this supersampling, by the rules of finish_view and settled():
 // below the largest supersampling, coverage - 2*sqrt(2)/ss > limit (no
-// refinement can pass it); at the largest, coverage_upper > limit.
+// refinement can pass it); at the largest, coverage_upper > limit. A
+// conservative screen never refines and fails a view only on
+// max(0, coverage - 2*sqrt(2)/ss - 1e-6) > limit (a clipped view passes
+// whatever its distance), at every supersampling.
 static uint32_t failing_distance(const Settings& s, unsigned ss) {
     const auto fails = [&](uint32_t d) {
         const double c = std::sqrt(double(float(d))) / ss;
+        if (s.conservative_screen) return std::max(0.0, c - 2 * std::sqrt(2.0) / ss - 1e-6) > s.limit;
         return ss >= s.max_supersample ? c + 2 * std::sqrt(2.0) / ss + 1e-6 > s.limit
                                        : c - 2 * std::sqrt(2.0) / ss > s.limit;
     };

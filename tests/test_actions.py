@@ -289,11 +289,12 @@ def test_gpu_learned_rank_orders_scorer_logits():
 
 
 @pytest.mark.gpu
-def test_gpu_executor_matches_golden():
+@pytest.mark.parametrize("cache", [False, True])
+def test_gpu_executor_matches_golden(cache):
     from blitzremesher_amd import Audit
     from blitzremesher_amd.actions import execute_actions as gpu_execute
     ms = golden_meshes()
-    with Audit(0) as audit:
+    with Audit(0, cache_rasters=cache) as audit:
         for r in golden()["executions"]:
             mesh, _ = ms[r["mesh"]]
             out, stats = gpu_execute(audit, mesh, r["center"], r["radius"], EvalSettings(**r["gate"]), r["target"],
@@ -303,13 +304,14 @@ def test_gpu_executor_matches_golden():
 
 
 @pytest.mark.gpu
-def test_gpu_executor_matches_oracle():
+@pytest.mark.parametrize("cache", [False, True])
+def test_gpu_executor_matches_oracle(cache):
     from blitzremesher_amd import Audit
     from blitzremesher_amd.actions import execute_actions as gpu_execute
     gate = EvalSettings(profile=0, screen_size=32, limit=1.25, supersample=2, max_supersample=8, orthographic=4,
                         perspective=2, rotation_seed=5)
     o = Oracle("oracle")
-    with Audit(0) as audit:
+    with Audit(0, cache_rasters=cache) as audit:
         for mesh in (meshes.icosphere(3, 0.03, 11), meshes.jitter(meshes.heightfield(16, 0.3, 11), 0.01, 11, 0.5)):
             center, radius = o.bounds(mesh)
             for ranking, batch in ((3, 16), (4, 64), (1, 4)):
@@ -318,6 +320,30 @@ def test_gpu_executor_matches_oracle():
                 out, stats = gpu_execute(audit, mesh, center, radius, gate, mesh.triangles // 3, 50, ranking, batch)
                 got = {k: stats[k] for k in ("ranked", "trials", "accepted", "rejected")}
                 assert np.array_equal(out, want[0]) and got == want[1], (ranking, batch, got, want[1])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("cache", [False, True])
+def test_gpu_executor_conservative_gate_matches_oracle(cache):
+    # A conservative screen fails a view only on a proved coverage violation
+    # (coverage - 2*sqrt(2)/ss - 1e-6 > limit) and never refines; the gate's
+    # verdict-only Hausdorff search must stop on that rule, not the full one.
+    # supersample == max_supersample puts every view on the last rung.
+    from blitzremesher_amd import Audit
+    from blitzremesher_amd.actions import execute_actions as gpu_execute
+    o = Oracle("oracle")
+    with Audit(0, cache_rasters=cache) as audit:
+        for mesh, ss, limit in ((meshes.icosphere(3, 0.03, 11), 4, 0.75), (meshes.icosphere(3, 0.03, 13), 4, 0.0),
+                                (meshes.jitter(meshes.heightfield(16, 0.3, 11), 0.01, 11, 0.5), 4, 0.2)):
+            gate = EvalSettings(profile=0, screen_size=32, limit=limit, supersample=ss, max_supersample=ss,
+                                orthographic=4, perspective=2, rotation_seed=5, conservative_screen=True)
+            center, radius = o.bounds(mesh)
+            for ranking, batch in ((4, 64), (3, 8)):
+                want = execute_actions(mesh, mesh.triangles // 4, 40, ranking, batch,
+                                       oracle_gate(mesh, center, radius, gate))
+                out, stats = gpu_execute(audit, mesh, center, radius, gate, mesh.triangles // 4, 40, ranking, batch)
+                got = {k: stats[k] for k in ("ranked", "trials", "accepted", "rejected")}
+                assert np.array_equal(out, want[0]) and got == want[1], (ss, limit, ranking, batch, got, want[1])
 
 
 @pytest.mark.gpu
