@@ -114,6 +114,75 @@ class ActionStats(C.Structure):
                 ("rank_ms", C.c_double), ("select_ms", C.c_double), ("trial_ms", C.c_double), ("gate_ms", C.c_double)]
 
 
+class CurvePoint(C.Structure):
+    _fields_ = [("x", C.c_float), ("y", C.c_float)]
+
+
+class BlitzSettings(C.Structure):
+    """blitz_settings (reference include/blitz/blitz.h:41-63)."""
+    _fields_ = [
+        ("struct_size", C.c_uint32), ("abi_version", C.c_uint32),
+        ("levels", C.c_uint8), ("profile", C.c_uint8), ("triangle_overhead_bps", C.c_uint16),
+        ("max_added_vertex_bytes_bps", C.c_uint32),
+        ("objective", C.c_uint8), ("beam_width", C.c_uint8), ("search_supersample", C.c_uint8),
+        ("audit_supersample", C.c_uint8),
+        ("max_supersample", C.c_uint8), ("prune", C.c_uint8), ("force_scalar", C.c_uint8),
+        ("coupled_wedges", C.c_uint8),
+        ("candidate_budget", C.c_uint16), ("search_ortho", C.c_uint16), ("search_perspective", C.c_uint16),
+        ("audit_ortho", C.c_uint16), ("audit_perspective", C.c_uint16),
+        ("search_seed", C.c_uint32), ("audit_seed", C.c_uint32),
+        ("pixels_per_meter", C.c_double), ("meters_per_unit", C.c_double), ("base_pixels", C.c_double),
+        ("last_pixels", C.c_double), ("max_lod0_delta_px", C.c_double),
+        ("normal_weight", C.c_double), ("color_weight", C.c_double), ("material_weight", C.c_double),
+        ("max_changed_area", C.c_double),
+        ("transition", C.POINTER(CurvePoint)), ("transition_count", C.c_size_t),
+        ("normal_importance", C.POINTER(CurvePoint)), ("normal_importance_count", C.c_size_t),
+        ("attribute_importance", C.POINTER(CurvePoint)), ("attribute_importance_count", C.c_size_t),
+        ("cancelled", CANCEL_FN), ("user_data", C.c_void_p),
+    ]
+
+
+class ChainOptionsC(C.Structure):
+    _fields_ = [
+        ("struct_size", C.c_uint32), ("version", C.c_uint32), ("action_trials", C.c_uint32),
+        ("action_batch", C.c_uint8), ("ranking", C.c_uint8), ("origin", C.c_uint8), ("output", C.c_uint8),
+        ("confirmation", C.c_uint8), ("overdraw_tiebreak", C.c_uint8), ("reserved", C.c_uint8 * 6),
+    ]
+
+
+class ChainLodC(C.Structure):
+    _fields_ = [
+        ("struct_size", C.c_uint32), ("indices", C.POINTER(C.c_uint32)), ("index_count", C.c_size_t),
+        ("materials", C.POINTER(C.c_uint16)), ("material_count", C.c_size_t),
+        ("screen_pixels", C.c_double), ("transition_limit", C.c_double), ("source_limit", C.c_double),
+        ("adjacent", MeasurementC), ("source", MeasurementC),
+    ]
+
+
+class ChainInfoC(C.Structure):
+    _fields_ = [
+        ("struct_size", C.c_uint32), ("levels", C.c_uint32), ("status", C.c_uint32), ("fallback_levels", C.c_uint32),
+        ("candidate_evaluations", C.c_uint64), ("audit_evaluations", C.c_uint64 * 4),
+        ("rejected_gates", C.c_uint64 * 4), ("area_rejected_gates", C.c_uint64 * 4),
+        ("decoded", C.c_uint64), ("legal_collapses", C.c_uint64), ("rejected_collapses", C.c_uint64),
+        ("action_ranked", C.c_uint64), ("action_trials", C.c_uint64), ("reference_rejections", C.c_uint64),
+        ("bounded_audits", C.c_uint32), ("finalists", C.c_uint32), ("selected", C.c_uint32),
+        ("proposals", C.c_uint32), ("gate_audits", C.c_uint64), ("confirm_audits", C.c_uint64),
+        ("ms", C.c_double), ("executor_ms", C.c_double), ("gate_ms", C.c_double), ("offer_ms", C.c_double),
+        ("confirm_ms", C.c_double), ("tiebreak_ms", C.c_double),
+    ]
+
+
+class ChainProposalC(C.Structure):
+    _fields_ = [
+        ("struct_size", C.c_uint32), ("level", C.c_uint32), ("parent_triangles", C.c_uint32),
+        ("start_triangles", C.c_uint32), ("target_triangles", C.c_uint32), ("final_triangles", C.c_uint32),
+        ("trials", C.c_uint32), ("accepted_batches", C.c_uint32), ("accepted", C.c_uint64),
+        ("rejected", C.c_uint64), ("screen_pixels", C.c_double), ("stop_reason", C.c_uint8), ("gate", C.c_uint8),
+        ("reserved", C.c_uint8 * 6),
+    ]
+
+
 _P = C.c_void_p
 _ERR = [C.c_char_p, C.c_size_t]
 
@@ -174,6 +243,16 @@ PROTOTYPES = {
                                             C.POINTER(C.c_uint32), C.c_size_t, C.POINTER(C.c_uint32),
                                             C.POINTER(C.c_uint32), C.c_size_t, C.POINTER(C.c_size_t),
                                             C.POINTER(C.c_size_t), *_ERR]),
+    "blitz_hip_chain_options_init": (C.c_int, [C.POINTER(ChainOptionsC), C.c_size_t]),
+    "blitz_hip_generate": (C.c_int, [_P, C.POINTER(BlitzMesh), C.POINTER(BlitzSettings), C.POINTER(ChainOptionsC),
+                                     _P, C.POINTER(_P), *_ERR]),
+    "blitz_hip_chain_destroy": (None, [_P]),
+    "blitz_hip_schedule": (C.c_int, [C.POINTER(BlitzSettings), C.POINTER(Bounds), C.POINTER(C.c_double), C.c_size_t,
+                                     *_ERR]),
+    "blitz_hip_chain_info_of": (C.c_int, [_P, C.POINTER(ChainInfoC)]),
+    "blitz_hip_chain_lod_of": (C.c_int, [_P, C.c_size_t, C.POINTER(ChainLodC)]),
+    "blitz_hip_chain_finalist": (C.c_int, [_P, C.c_size_t, C.POINTER(C.c_uint32), C.c_size_t]),
+    "blitz_hip_chain_proposal_of": (C.c_int, [_P, C.c_size_t, C.POINTER(ChainProposalC)]),
     "blitz_hip_coverage_distance": (C.c_int, [_P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.c_uint32,
                                               C.c_uint8, C.POINTER(C.c_double), *_ERR]),
     "blitz_hip_audit_last_timing": (C.c_int, [_P, C.POINTER(AuditTiming)]),

@@ -1008,13 +1008,17 @@ std::vector<uint32_t> execute_actions(int device, const HostMesh& input, const f
     auto since = [](std::chrono::steady_clock::time_point t) {
         return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t).count();
     };
+    bool no_legal = false;
     while (state.indices.size() / 3 > target && stats.trials < budget) {
         auto t = std::chrono::steady_clock::now();
         ActionSet set(device, state, uv);
         stats.build_ms += since(t);
         ++stats.states;
         const size_t n = set.legal_count();
-        if (!n) break;
+        if (!n) {
+            no_legal = true;
+            break;
+        }
         t = std::chrono::steady_clock::now();
         std::unique_ptr<Graph> graph;
         if (ranking == 0) {
@@ -1046,6 +1050,7 @@ std::vector<uint32_t> execute_actions(int device, const HostMesh& input, const f
                     state.indices = std::move(candidate);
                     state.materials = std::move(materials);
                     stats.accepted += picked.size();
+                    ++stats.accepted_batches;
                     accepted = true;
                     break;
                 }
@@ -1058,6 +1063,7 @@ std::vector<uint32_t> execute_actions(int device, const HostMesh& input, const f
         }
         if (!accepted) break;
     }
+    stats.stop_reason = state.indices.size() / 3 <= target ? 0 : stats.trials >= budget ? 1 : no_legal ? 2 : 3;
     stats.ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - clock0).count();
     if (materials_out) *materials_out = state.materials;
     return state.indices;

@@ -114,6 +114,10 @@ struct ExecuteStats {
     // Wall time of each stage: state builds, ranking, batch selection,
     // trials, gate calls.
     double build_ms = 0, rank_ms = 0, select_ms = 0, trial_ms = 0, gate_ms = 0;
+    uint32_t accepted_batches = 0;
+    // blitz::NeuralActionStop (include/blitz/neural.hpp:22-30): 0 target
+    // reached, 1 trial budget, 2 no legal actions, 3 no accepted action.
+    uint8_t stop_reason = 3;
 };
 // neural::execute_actions (src/neural/action.cpp:453-525) on the device
 // action state: rank (0 learned: BF16 logits of `scorer` on each state's

@@ -59,7 +59,12 @@ __device__ __forceinline__ V3 normalized3(V3 a) {
 __device__ __forceinline__ double length3(V3 a) { return sqrt(dot3(a, a)); }
 __device__ __forceinline__ V3 xyz(float4 v) { return {v.x, v.y, v.z}; }
 
-// fdlibm e_acos.c (Sun Microsystems, 1993; freely redistributable): rational
+// fdlibm e_acos.c notice, preserved as its licence requires:
+// Copyright (C) 1993 by Sun Microsystems, Inc. All rights reserved.
+// Developed at SunSoft, a Sun Microsystems, Inc. business.
+// Permission to use, copy, modify, and distribute this software is freely
+// granted, provided that this notice is preserved.
+// The routine: a rational
 // approximation of asin about 0, half-angle forms near +-1 with the split
 // square root for x > 0.5, in the operation order of src/metric_angle.hpp:21-40
 // so that every rounding is the reference's.

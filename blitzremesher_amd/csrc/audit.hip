@@ -200,6 +200,10 @@ std::unique_ptr<DeviceMesh> Audit::upload(const HostMesh& m) {
     d->vertex_digest[0] = m.vertex_digest[0];
     d->vertex_digest[1] = m.vertex_digest[1];
     d->serial = next_serial_++;
+    d->positions_serial = d->serial;
+    if (!m.normals.empty()) d->raw_normals = std::make_shared<const std::vector<float>>(m.normals);
+    if (!m.colors.empty()) d->raw_colors = std::make_shared<const std::vector<uint8_t>>(m.colors);
+    d->raw_double_sided = m.double_sided;
     std::vector<float4> pos(d->vertices);
     for (uint32_t i = 0; i < d->vertices; ++i)
         pos[i] = make_float4(m.positions[3 * i], m.positions[3 * i + 1], m.positions[3 * i + 2], 0.f);
@@ -283,6 +287,10 @@ std::unique_ptr<DeviceMesh> Audit::derive(const DeviceMesh& base, const std::vec
     d->vertex_digest[0] = base.vertex_digest[0];
     d->vertex_digest[1] = base.vertex_digest[1];
     d->serial = next_serial_++;
+    d->positions_serial = base.positions_serial;
+    d->raw_normals = base.raw_normals;
+    d->raw_colors = base.raw_colors;
+    d->raw_double_sided = double_sided;
     hip_check(hipMalloc(&d->indices, std::max<size_t>(1, indices.size()) * sizeof(uint32_t)), "hipMalloc");
     if (!indices.empty())
         hip_check(hipMemcpyAsync(d->indices, indices.data(), indices.size() * 4, hipMemcpyHostToDevice, stream_),
@@ -311,6 +319,42 @@ std::unique_ptr<DeviceMesh> Audit::derive(const DeviceMesh& base, const std::vec
 }
 
 void Audit::release(DeviceMesh*) {}
+
+bool Audit::device_equal(const void* a, const void* b, size_t bytes) {
+    if (a == b || !bytes) return true;
+    DeviceGuard guard(device_);
+    uint64_t* flag = scratch_as<uint64_t>(48, 1);  // a slot no other pass uses
+    launch_fill(flag, 1, 0, stream_);
+    launch_differ(a, b, bytes, flag, stream_);
+    uint64_t differs = 1;
+    hip_check(hipMemcpyAsync(&differs, flag, 8, hipMemcpyDeviceToHost, stream_), "download compare flag");
+    hip_check(hipStreamSynchronize(stream_), "compare streams");
+    return differs == 0;
+}
+
+bool Audit::same_positions(const DeviceMesh& a, const DeviceMesh& b) {
+    if (a.positions == b.positions || a.positions_serial == b.positions_serial) return a.vertices == b.vertices;
+    if (a.vertices != b.vertices || a.pos_digest[0] != b.pos_digest[0] || a.pos_digest[1] != b.pos_digest[1])
+        return false;
+    const auto key = std::minmax(a.positions_serial, b.positions_serial);
+    auto it = position_pairs_.find(key);
+    if (it == position_pairs_.end())
+        it = position_pairs_.emplace(key, device_equal(a.positions, b.positions, size_t(a.vertices) * sizeof(float4)))
+                 .first;
+    return it->second;
+}
+
+bool Audit::identical(const DeviceMesh& a, const DeviceMesh& b) {
+    if (&a == &b) return true;
+    if (a.digest[0] != b.digest[0] || a.digest[1] != b.digest[1] || a.vertices != b.vertices || a.faces != b.faces)
+        return false;
+    const auto same_host = [](const auto& x, const auto& y) { return x == y || (x && y && *x == *y); };
+    if (a.raw_double_sided != b.raw_double_sided || !same_host(a.raw_normals, b.raw_normals) ||
+        !same_host(a.raw_colors, b.raw_colors) || !a.materials != !b.materials)
+        return false;
+    return same_positions(a, b) && device_equal(a.indices, b.indices, size_t(a.faces) * 12) &&
+           (!a.materials || device_equal(a.materials, b.materials, size_t(a.faces) * 2));
+}
 
 // Coverage-profile view finish (src/evaluate.cpp:271-298).
 Measurement finish_view(double coverage, uint64_t changed, uint64_t total, bool clipped, const Settings& s, uint8_t ss) {
@@ -404,9 +448,7 @@ std::vector<Measurement> Audit::evaluate_batch(const DeviceMesh& reference,
     for (size_t c = 0; c < n; ++c) {
         const DeviceMesh& m = *candidates[c];
         // src/evaluate.cpp:77-82,317: byte-identical streams skip every view.
-        identical[c] = &m == &reference ||
-                       (m.digest[0] == reference.digest[0] && m.digest[1] == reference.digest[1] &&
-                        m.vertices == reference.vertices && m.faces == reference.faces);
+        identical[c] = this->identical(m, reference);
         for (auto& st : state[c]) st.ss = s.supersample;
     }
     DeviceGuard guard(device_);
@@ -530,17 +572,25 @@ void Audit::raster_launch(const std::vector<RasterJob>& jobs, const std::vector<
             return std::tie(d0, d1, vertices, view) < std::tie(o.d0, o.d1, o.vertices, o.view);
         }
     };
-    std::map<Key, uint32_t> slot;
+    // Equal digests only propose sharing; same_positions() confirms it.
+    std::map<Key, std::vector<uint32_t>> slot;
     std::vector<std::pair<const DeviceMesh*, uint32_t>> sets;
     std::vector<uint32_t> job_slot(jobs.size());
     for (size_t j = 0; j < jobs.size(); ++j) {
         const Key k{jobs[j].mesh->pos_digest[0], jobs[j].mesh->pos_digest[1], jobs[j].mesh->vertices, jobs[j].view};
-        auto it = slot.find(k);
-        if (it == slot.end()) {
-            it = slot.emplace(k, uint32_t(sets.size())).first;
+        auto& list = slot[k];
+        uint32_t found = kNone;
+        for (uint32_t si : list)
+            if (same_positions(*sets[si].first, *jobs[j].mesh)) {
+                found = si;
+                break;
+            }
+        if (found == kNone) {
+            found = uint32_t(sets.size());
+            list.push_back(found);
             sets.push_back({jobs[j].mesh, jobs[j].view});
         }
-        job_slot[j] = it->second;
+        job_slot[j] = found;
     }
     std::vector<size_t> offset(sets.size() + 1, 0);
     uint32_t max_vertices = 0;

@@ -77,6 +77,15 @@ struct DeviceMesh {
     uint64_t pos_digest[2] = {0, 0};
     uint64_t vertex_digest[2] = {0, 0};
     uint64_t serial = 0;
+    // Serial of the device mesh that owns the position buffer (this mesh's
+    // own serial unless derived).
+    uint64_t positions_serial = 0;
+    // Host copies of the raw streams that identity compares and the device
+    // does not hold as given: normals and colours (the device holds them
+    // normalised / scaled) and the per-material sidedness table.
+    std::shared_ptr<const std::vector<float>> raw_normals;
+    std::shared_ptr<const std::vector<uint8_t>> raw_colors;
+    std::vector<uint8_t> raw_double_sided;
     // Vertex streams (positions, normals, colours) owned by another device
     // mesh of the same audit (derive()): not freed by this one.
     bool borrowed_vertices = false;
@@ -183,6 +192,13 @@ class Audit {
     // took the full audit path.
     std::vector<Measurement> evaluate_patches(Binding& binding, const PatchSet& patches, const Settings& s,
                                               uint32_t* fallbacks);
+    // blitz::identical (src/evaluate.cpp:71-82): positions, normals, colours,
+    // indices, materials and the sidedness table byte-equal. Digests decide
+    // only when they differ; equal digests are confirmed stream by stream.
+    bool identical(const DeviceMesh& a, const DeviceMesh& b);
+    // Position streams byte-equal (same owner, or compared once per pair of
+    // owners and remembered).
+    bool same_positions(const DeviceMesh& a, const DeviceMesh& b);
     const Timing& last_timing() const { return timing_; }
     int device() const { return device_; }
     hipStream_t stream() const { return stream_; }
@@ -272,6 +288,8 @@ class Audit {
     uint32_t* pinned_status_ = nullptr;  // status words of the last raster launch (pinned host memory)
     size_t pinned_status_words_ = 0;
     RasterCache cache_;
+    std::map<std::pair<uint64_t, uint64_t>, bool> position_pairs_;  // owner serials -> equal
+    bool device_equal(const void* a, const void* b, size_t bytes);
     hipStream_t stream_{};
     hipEvent_t events_[10]{};
     std::vector<std::pair<void*, size_t>> slots_;

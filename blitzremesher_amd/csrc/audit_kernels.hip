@@ -11,6 +11,7 @@
 // workgroup own a band of tile rows of one view's mask in LDS, walk a chunk
 // of the mesh's faces, OR coverage into LDS and flush the band once.
 #include <hip/hip_runtime.h>
+#include <algorithm>
 
 #include "audit_kernels.hpp"
 #include "raster_device.hpp"
@@ -207,6 +208,16 @@ __global__ __launch_bounds__(kRasterThreads) void raster_band_kernel(const Raste
             if (v) atomicOr(reinterpret_cast<unsigned long long*>(out + i), v);
         }
     }
+}
+
+// Sets *differs when any byte of a[0..n) and b[0..n) differs (every writer
+// stores the same value).
+__global__ __launch_bounds__(256) void differ_kernel(const uint8_t* __restrict__ a, const uint8_t* __restrict__ b,
+                                                     size_t n, uint64_t* differs) {
+    const size_t stride = size_t(gridDim.x) * blockDim.x;
+    bool d = false;
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) d |= a[i] != b[i];
+    if (d) *differs = 1;
 }
 
 __global__ void fill_kernel(uint64_t* p, size_t n, uint64_t value) {
@@ -466,6 +477,13 @@ __global__ __launch_bounds__(256) void hausdorff_kernel(const ViewPair* __restri
 }
 
 }  // namespace
+
+void launch_differ(const void* a, const void* b, size_t bytes, uint64_t* differs, hipStream_t stream) {
+    if (!bytes) return;
+    const unsigned blocks = unsigned(std::min<size_t>(1024, (bytes + 255) / 256));
+    differ_kernel<<<blocks, 256, 0, stream>>>(static_cast<const uint8_t*>(a), static_cast<const uint8_t*>(b), bytes,
+                                               differs);
+}
 
 void launch_fill(uint64_t* p, size_t n, uint64_t value, hipStream_t stream) {
     if (!n) return;

@@ -9,6 +9,8 @@
 
 namespace blitz_hip {
 void launch_fill(uint64_t* p, size_t n, uint64_t value, hipStream_t stream);
+// *differs |= any byte of a[0..bytes) != b[0..bytes) (the caller zeroes it).
+void launch_differ(const void* a, const void* b, size_t bytes, uint64_t* differs, hipStream_t stream);
 // tasks: device array of `count` projections; grid covers max_vertices each.
 void launch_project(const ProjectTask* tasks, uint32_t count, uint32_t max_vertices, hipStream_t stream);
 // tasks: device array of grid.tasks raster tasks sharing one raster side.

@@ -13,6 +13,8 @@
 
 #include "../../include/blitz_hip.h"
 #include "actions.hpp"
+#include "chain.hpp"
+#include "digest.hpp"
 #include "audit.hpp"
 #include "patch_audit.hpp"
 #include "patches.hpp"
@@ -37,6 +39,9 @@ struct blitz_hip_binding {
 };
 struct blitz_hip_actions {
     std::unique_ptr<ActionSet> impl;
+};
+struct blitz_hip_chain {
+    ChainResult result;
 };
 struct blitz_hip_graph {
     blitz_hip_scorer* owner;
@@ -76,29 +81,6 @@ template <class F> blitz_status guarded(char* error, size_t capacity, F&& body) 
     }
 }
 
-// Two independent 64-bit digests (FNV-1a style and a multiplicative mix, over
-// 8-byte words, then the remaining bytes) of the streams blitz's identical()
-// compares (src/evaluate.cpp:77-82).
-struct Digest {
-    uint64_t a = 1469598103934665603ull, b = 0x9E3779B97F4A7C15ull;
-    void word(uint64_t w) {
-        a = (a ^ w) * 1099511628211ull;
-        b = (b ^ w) * 0xff51afd7ed558ccdull;
-        b ^= b >> 29;
-    }
-    void bytes(const void* p, size_t n) {
-        const auto* c = static_cast<const unsigned char*>(p);
-        size_t i = 0;
-        for (; i + 8 <= n; i += 8) {
-            uint64_t w;
-            std::memcpy(&w, c + i, 8);
-            word(w);
-        }
-        for (; i < n; ++i) word(c[i]);
-    }
-    void value(uint64_t v) { word(v); }
-};
-
 bool valid_stream(const blitz_stream& s, size_t n, size_t element, bool required) {
     if (!s.count) return !required;
     if (!s.data) return false;
@@ -113,18 +95,6 @@ void digest_stream(Digest& d, const blitz_stream& s, size_t element) {
         return;
     }
     for (size_t i = 0; i < s.count; ++i) d.bytes(base + i * s.stride, element);
-}
-
-// The identity digest of a mesh from its vertex-stream digest state and its
-// triangle streams (the order host_mesh hashes them in).
-void digest_triangles(Digest& d, const uint32_t* indices, size_t index_count, const uint16_t* materials,
-                      size_t material_count, const uint8_t* double_sided, size_t double_sided_count) {
-    d.value(index_count);
-    d.bytes(indices, index_count * 4);
-    d.value(material_count);
-    d.bytes(materials, material_count * 2);
-    d.value(double_sided_count);
-    d.bytes(double_sided, double_sided_count);
 }
 
 HostMesh host_mesh(const blitz_mesh* m) {
@@ -243,6 +213,61 @@ const DeviceMesh& mesh_of(const blitz_hip_audit* a, const blitz_hip_device_mesh*
 ActionSet& actions_of(blitz_hip_actions* a) {
     if (!a || !a->impl) throw std::invalid_argument("null action state");
     return *a->impl;
+}
+// blitz::Curve from a C curve (src/c_api.cpp:33-42): an empty curve keeps
+// the default.
+Curve curve_of(const blitz_curve_point* p, size_t n, Curve fallback) {
+    if (!n) return fallback;
+    if (!p || n > 1024) throw std::invalid_argument("invalid curve");
+    fallback.points.clear();
+    for (size_t i = 0; i < n; ++i) fallback.points.push_back({p[i].x, p[i].y});
+    return fallback;
+}
+
+// blitz::Settings from the ABI-5 descriptor (src/c_api.cpp:141-178), the
+// fields neural generation reads.
+ChainSettings chain_settings_of(const blitz_settings* c) {
+    if (!c || c->struct_size != sizeof(blitz_settings) || c->abi_version != BLITZ_ABI_VERSION)
+        throw std::invalid_argument("null pointer, descriptor size or ABI version mismatch");
+    ChainSettings s;
+    s.levels = c->levels;
+    s.triangle_overhead_bps = c->triangle_overhead_bps;
+    s.profile = c->profile;
+    s.objective = c->objective;
+    s.beam_width = c->beam_width;
+    s.candidate_budget = c->candidate_budget;
+    s.search_supersample = c->search_supersample;
+    s.audit_supersample = c->audit_supersample;
+    s.max_supersample = c->max_supersample;
+    s.search_orthographic = c->search_ortho;
+    s.search_perspective = c->search_perspective;
+    s.search_seed = c->search_seed;
+    s.audit_orthographic = c->audit_ortho;
+    s.audit_perspective = c->audit_perspective;
+    s.audit_seed = c->audit_seed;
+    s.pixels_per_meter = c->pixels_per_meter;
+    s.meters_per_unit = c->meters_per_unit;
+    s.last_pixels = c->last_pixels;
+    s.base_pixels = c->base_pixels;
+    s.max_lod0_delta_px = c->max_lod0_delta_px;
+    s.normal_weight = c->normal_weight;
+    s.color_weight = c->color_weight;
+    s.material_weight = c->material_weight;
+    s.max_changed_area = c->max_changed_area;
+    s.transition = curve_of(c->transition, c->transition_count, s.transition);
+    s.normal_importance = curve_of(c->normal_importance, c->normal_importance_count, s.normal_importance);
+    s.attribute_importance = curve_of(c->attribute_importance, c->attribute_importance_count, s.attribute_importance);
+    if (c->cancelled) {
+        auto fn = c->cancelled;
+        void* user = c->user_data;
+        s.cancelled = [fn, user] { return fn(user) != 0; };
+    }
+    return s;
+}
+
+const ChainResult& chain_of(const blitz_hip_chain* c) {
+    if (!c) throw std::invalid_argument("null chain");
+    return c->result;
 }
 }  // namespace
 
@@ -965,5 +990,167 @@ blitz_status blitz_hip_scorer_last_step_ms(const blitz_hip_scorer* s, double* ms
     if (!s || !s->impl || !ms) return BLITZ_INVALID_ARGUMENT;
     *ms = s->impl->last_step_ms();
     return BLITZ_OK;
+}
+
+blitz_status blitz_hip_chain_options_init(blitz_hip_chain_options* o, size_t size) {
+    if (!o || size != sizeof(*o)) return BLITZ_INVALID_ARGUMENT;
+    *o = {};
+    o->struct_size = sizeof(*o);
+    o->version = BLITZ_HIP_API_VERSION;
+    o->action_trials = 64;
+    o->action_batch = 32;
+    o->ranking = 4;
+    o->origin = 0;
+    o->output = 1;
+    o->confirmation = 1;
+    o->overdraw_tiebreak = 1;
+    return BLITZ_OK;
+}
+
+blitz_status blitz_hip_generate(blitz_hip_audit* a, const blitz_mesh* source, const blitz_settings* settings,
+                                const blitz_hip_chain_options* options, blitz_hip_scorer* scorer,
+                                blitz_hip_chain** out, char* error, size_t capacity) {
+    if (out) *out = nullptr;
+    uint8_t status = 0;
+    const blitz_status rc = guarded(error, capacity, [&] {
+        if (!out) throw std::invalid_argument("null output");
+        if (!options || options->struct_size != sizeof(*options) || options->version != BLITZ_HIP_API_VERSION)
+            throw std::invalid_argument("null pointer, options size or version mismatch");
+        if (options->origin > 2 || options->output > 1 || options->confirmation > 2)
+            throw std::invalid_argument("invalid neural proposal origin, output or confirmation");
+        if (options->origin != 0) throw Unavailable("the HIP chain proposes from the source only");
+        if (options->output != 1) throw Unavailable("the HIP chain emits reuse (shared-vertex) LODs only");
+        if (options->confirmation != 1) throw Unavailable("the HIP chain confirms on the GPU only");
+        Audit& audit = audit_of(a);
+        const ChainSettings s = chain_settings_of(settings);
+        const HostMesh m = host_mesh(source);
+        std::vector<float> uv;
+        if (source->uv.count) {
+            const size_t n = m.positions.size() / 3;
+            uv.resize(2 * n);
+            const auto* base = static_cast<const unsigned char*>(source->uv.data);
+            for (size_t i = 0; i < n; ++i) std::memcpy(&uv[2 * i], base + i * source->uv.stride, 8);
+        }
+        ChainOptions o;
+        o.action_trials = options->action_trials;
+        o.action_batch = options->action_batch;
+        o.ranking = options->ranking;
+        o.overdraw_tiebreak = options->overdraw_tiebreak != 0;
+        Scorer* sc = nullptr;
+        if (scorer) {
+            if (!scorer->impl) throw std::invalid_argument("null scorer");
+            if (scorer->impl->device() != audit.device()) throw std::invalid_argument("scorer on another device");
+            sc = scorer->impl.get();
+        }
+        auto chain = std::make_unique<blitz_hip_chain>();
+        chain->result = generate_chain(audit, m, uv.empty() ? nullptr : uv.data(), s, o, sc);
+        status = chain->result.status;
+        *out = chain.release();
+    });
+    if (rc != BLITZ_OK) return rc;
+    return status == 2 ? BLITZ_CANCELLED : status == 1 ? BLITZ_BUDGET_LIMITED : BLITZ_OK;
+}
+
+void blitz_hip_chain_destroy(blitz_hip_chain* c) { delete c; }
+
+blitz_status blitz_hip_schedule(const blitz_settings* settings, const blitz_hip_bounds* b, double* out,
+                                size_t capacity, char* error, size_t error_capacity) {
+    return guarded(error, error_capacity, [&] {
+        const auto steps = schedule(bounds_from(b), chain_settings_of(settings));
+        if (!out || capacity < 3 * steps.size()) throw std::invalid_argument("schedule capacity too small");
+        for (size_t i = 0; i < steps.size(); ++i) {
+            out[3 * i] = steps[i].pixels;
+            out[3 * i + 1] = steps[i].transition;
+            out[3 * i + 2] = steps[i].source;
+        }
+    });
+}
+
+blitz_status blitz_hip_chain_info_of(const blitz_hip_chain* c, blitz_hip_chain_info* out) {
+    return guarded(nullptr, 0, [&] {
+        const ChainResult& r = chain_of(c);
+        if (!out || out->struct_size != sizeof(*out)) throw std::invalid_argument("info size mismatch");
+        *out = {};
+        out->struct_size = sizeof(*out);
+        out->levels = uint32_t(r.lods.size());
+        out->status = r.status;
+        out->fallback_levels = r.fallback_levels;
+        out->candidate_evaluations = r.candidate_evaluations;
+        for (int i = 0; i < 4; ++i) {
+            out->audit_evaluations[i] = r.audit_evaluations[i];
+            out->rejected_gates[i] = r.rejected_gates[i];
+            out->area_rejected_gates[i] = r.area_rejected_gates[i];
+        }
+        out->decoded = r.decoded;
+        out->legal_collapses = r.legal_collapses;
+        out->rejected_collapses = r.rejected_collapses;
+        out->action_ranked = r.action_ranked;
+        out->action_trials = r.action_trials;
+        out->reference_rejections = r.reference_rejections;
+        out->bounded_audits = r.bounded_audits;
+        out->finalists = uint32_t(r.candidates.size());
+        out->selected = uint32_t(r.selected);
+        out->proposals = uint32_t(r.proposals.size());
+        out->gate_audits = r.gate_audits;
+        out->confirm_audits = r.confirm_audits;
+        out->ms = r.ms;
+        out->executor_ms = r.executor_ms;
+        out->gate_ms = r.gate_ms;
+        out->offer_ms = r.offer_ms;
+        out->confirm_ms = r.confirm_ms;
+        out->tiebreak_ms = r.tiebreak_ms;
+    });
+}
+
+blitz_status blitz_hip_chain_lod_of(const blitz_hip_chain* c, size_t level, blitz_hip_chain_lod* out) {
+    return guarded(nullptr, 0, [&] {
+        const ChainResult& r = chain_of(c);
+        if (!out || out->struct_size != sizeof(*out)) throw std::invalid_argument("lod size mismatch");
+        if (level >= r.lods.size()) throw std::invalid_argument("level out of range");
+        const ChainLod& l = r.lods[level];
+        *out = {};
+        out->struct_size = sizeof(*out);
+        out->indices = l.indices.data();
+        out->index_count = l.indices.size();
+        out->materials = l.materials.empty() ? nullptr : l.materials.data();
+        out->material_count = l.materials.size();
+        out->screen_pixels = l.schedule.pixels;
+        out->transition_limit = l.schedule.transition;
+        out->source_limit = l.schedule.source;
+        to_c(l.adjacent, &out->adjacent);
+        to_c(l.source_error, &out->source);
+    });
+}
+
+blitz_status blitz_hip_chain_finalist(const blitz_hip_chain* c, size_t i, uint32_t* triangles, size_t capacity) {
+    return guarded(nullptr, 0, [&] {
+        const ChainResult& r = chain_of(c);
+        if (i >= r.candidates.size()) throw std::invalid_argument("finalist out of range");
+        if (!triangles || capacity < r.candidates[i].size()) throw std::invalid_argument("capacity too small");
+        std::memcpy(triangles, r.candidates[i].data(), r.candidates[i].size() * 4);
+    });
+}
+
+blitz_status blitz_hip_chain_proposal_of(const blitz_hip_chain* c, size_t i, blitz_hip_chain_proposal* out) {
+    return guarded(nullptr, 0, [&] {
+        const ChainResult& r = chain_of(c);
+        if (!out || out->struct_size != sizeof(*out)) throw std::invalid_argument("proposal size mismatch");
+        if (i >= r.proposals.size()) throw std::invalid_argument("proposal out of range");
+        const ChainProposal& p = r.proposals[i];
+        *out = {};
+        out->struct_size = sizeof(*out);
+        out->level = p.level;
+        out->parent_triangles = p.parent_triangles;
+        out->start_triangles = p.start_triangles;
+        out->target_triangles = p.target_triangles;
+        out->final_triangles = p.final_triangles;
+        out->trials = p.trials;
+        out->accepted_batches = p.accepted_batches;
+        out->accepted = p.accepted;
+        out->rejected = p.rejected;
+        out->screen_pixels = p.screen_pixels;
+        out->stop_reason = p.stop_reason;
+        out->gate = p.gate;
+    });
 }
 }

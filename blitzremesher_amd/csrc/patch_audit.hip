@@ -13,6 +13,7 @@
 
 #include "audit_kernels.hpp"
 #include "patch_audit.hpp"
+#include "digest.hpp"
 #include "patch_kernels.hpp"
 #include "patches.hpp"
 
@@ -32,12 +33,6 @@ struct Clock {
         return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     }
 };
-
-uint64_t fnv(uint64_t h, const void* p, size_t n) {
-    const auto* c = static_cast<const unsigned char*>(p);
-    for (size_t i = 0; i < n; ++i) h = (h ^ c[i]) * 1099511628211ull;
-    return h;
-}
 
 }  // namespace
 
@@ -352,11 +347,21 @@ std::vector<Measurement> Audit::evaluate_patches(Binding& bd, const PatchSet& P,
             c.double_sided = bd.host.double_sided;
             c.indices = patch_candidate(ref_idx, faces, P.removed + r0, P.removed_offsets[i + 1] - r0,
                                         P.added ? P.added + 3 * size_t(a0) : nullptr, P.added_offsets[i + 1] - a0);
+            c.normals = bd.host.normals;
+            c.colors = bd.host.colors;
             c.pos_digest[0] = bd.host.pos_digest[0];
             c.pos_digest[1] = bd.host.pos_digest[1];
-            // Differs from the reference's digest: the index streams differ.
-            c.digest[0] = fnv(bd.host.digest[0] ^ 0x5851F42D4C957F2Dull, c.indices.data(), c.indices.size() * 4);
-            c.digest[1] = fnv(bd.host.digest[1] + 1, c.indices.data(), c.indices.size() * 4);
+            c.vertex_digest[0] = bd.host.vertex_digest[0];
+            c.vertex_digest[1] = bd.host.vertex_digest[1];
+            // The identity digest of the candidate's own streams, as a fresh
+            // upload of the same mesh would compute it.
+            Digest d;
+            d.a = c.vertex_digest[0];
+            d.b = c.vertex_digest[1];
+            digest_triangles(d, c.indices.data(), c.indices.size(), nullptr, 0, c.double_sided.data(),
+                             c.double_sided.size());
+            c.digest[0] = d.a;
+            c.digest[1] = d.b;
             meshes.push_back(upload(c));
             cands.push_back(meshes.back().get());
         }

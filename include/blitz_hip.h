@@ -48,6 +48,31 @@ typedef struct blitz_mesh {
     const uint8_t* double_sided;
     size_t double_sided_count;
 } blitz_mesh;
+typedef struct blitz_curve_point {
+    float x, y;
+} blitz_curve_point;
+/* ABI-5 generation settings (reference include/blitz/blitz.h:41-63). */
+typedef struct blitz_settings {
+    uint32_t struct_size, abi_version;
+    uint8_t levels, profile;
+    uint16_t triangle_overhead_bps;
+    uint32_t max_added_vertex_bytes_bps;
+    uint8_t objective, beam_width, search_supersample, audit_supersample;
+    uint8_t max_supersample, prune, force_scalar, coupled_wedges;
+    uint16_t candidate_budget, search_ortho, search_perspective, audit_ortho, audit_perspective;
+    uint32_t search_seed, audit_seed;
+    double pixels_per_meter, meters_per_unit, base_pixels, last_pixels, max_lod0_delta_px;
+    double normal_weight, color_weight, material_weight;
+    double max_changed_area;
+    const blitz_curve_point* transition;
+    size_t transition_count;
+    const blitz_curve_point* normal_importance;
+    size_t normal_importance_count;
+    const blitz_curve_point* attribute_importance;
+    size_t attribute_importance_count;
+    int (*cancelled)(void*);
+    void* user_data;
+} blitz_settings;
 #endif
 
 /* New enumerator for blitz_neural_options_v2.raster_backend (CUDA = 0,
@@ -71,9 +96,10 @@ typedef struct blitz_hip_camera {
 } blitz_hip_camera;
 
 /* blitz::EvalSettings (include/blitz/evaluate.hpp:24-33) as a versioned POD.
- * profile: 0 coverage (the HIP backend implements coverage; 1 normals and
- * 2 attributes return BLITZ_UNAVAILABLE). cancelled is polled once per view
- * in view order, as the reference does (src/evaluate.cpp:319). */
+ * profile: 0 coverage, 1 normals, 2 attributes; every audit entry point
+ * implements all three except the bound-reference patch audits, which take
+ * coverage only (others return BLITZ_UNAVAILABLE). cancelled is polled once
+ * per view in view order, as the reference does (src/evaluate.cpp:319). */
 typedef struct blitz_hip_eval_settings {
     uint32_t struct_size, version;
     uint8_t profile, supersample, max_supersample, force_two_sided;
@@ -288,6 +314,88 @@ blitz_status blitz_hip_execute_actions(blitz_hip_audit*, const blitz_mesh* input
                                        uint8_t ranking, blitz_hip_scorer* scorer, uint8_t batch, uint32_t* indices,
                                        size_t index_capacity, size_t* index_count, blitz_hip_action_stats* stats,
                                        char* error, size_t capacity);
+
+/* ---- LOD chain ------------------------------------------------------------- */
+
+/* The fields of blitz_neural_options_v2 (include/blitz/blitz.h:127-146) that
+ * the HIP chain reads, plus the explicit output mode. Supported: origin
+ * BLITZ_NEURAL_SOURCE (0), output reuse (1: every LOD indexes the source's
+ * vertices; research.output = Reuse), GPU confirmation (1). Other values
+ * return BLITZ_UNAVAILABLE. */
+typedef struct blitz_hip_chain_options {
+    uint32_t struct_size, version;
+    uint32_t action_trials;     /* 64: trials per proposal */
+    uint8_t action_batch;       /* 32: independent collapses per trial, 1..64 */
+    uint8_t ranking;            /* 4: 0 learned (scorer), 1 constant, 3 shortest edge, 4 current plane */
+    uint8_t origin;             /* 0: source */
+    uint8_t output;             /* 1: reuse */
+    uint8_t confirmation;       /* 1: GPU */
+    uint8_t overdraw_tiebreak;  /* 1: equal-cost finalists ordered by the top-left overlap */
+    uint8_t reserved[6];
+} blitz_hip_chain_options;
+
+/* One scheduled level (blitz_lod_info, include/blitz/blitz.h:64-72): the
+ * triangles as indices into the source's vertices (borrowed from the chain,
+ * valid until it is destroyed), the schedule and both audits. */
+typedef struct blitz_hip_chain_lod {
+    uint32_t struct_size;
+    const uint32_t* indices;
+    size_t index_count;
+    const uint16_t* materials;
+    size_t material_count;
+    double screen_pixels, transition_limit, source_limit;
+    blitz_hip_measurement adjacent, source;
+} blitz_hip_chain_lod;
+
+/* Counters of the generation (blitz::Result, include/blitz/remesher.hpp:119-151,
+ * and blitz_neural_info, include/blitz/blitz.h:153-158). legal_collapses
+ * counts collapses that passed all four gates (source search, adjacent
+ * search, source audit, adjacent audit). */
+typedef struct blitz_hip_chain_info {
+    uint32_t struct_size;
+    uint32_t levels, status, fallback_levels;
+    uint64_t candidate_evaluations;
+    uint64_t audit_evaluations[4], rejected_gates[4], area_rejected_gates[4];
+    uint64_t decoded, legal_collapses, rejected_collapses, action_ranked, action_trials, reference_rejections;
+    uint32_t bounded_audits, finalists, selected, proposals;
+    uint64_t gate_audits, confirm_audits;
+    double ms, executor_ms, gate_ms, offer_ms, confirm_ms, tiebreak_ms;
+} blitz_hip_chain_info;
+
+/* One proposal (blitz_neural_action_proposal, include/blitz/blitz.h:147-152)
+ * and the gate its offer stopped at (0 accepted, 1..4 the four gates,
+ * 5 invalid, 6 growth). */
+typedef struct blitz_hip_chain_proposal {
+    uint32_t struct_size;
+    uint32_t level, parent_triangles, start_triangles, target_triangles, final_triangles, trials,
+        accepted_batches;
+    uint64_t accepted, rejected;
+    double screen_pixels;
+    uint8_t stop_reason, gate, reserved[6];
+} blitz_hip_chain_proposal;
+
+typedef struct blitz_hip_chain blitz_hip_chain;
+
+blitz_status blitz_hip_chain_options_init(blitz_hip_chain_options*, size_t);
+/* blitz_generate_neural (include/blitz/blitz.h:165-167; src/neural/generate.cpp:126-577
+ * with detail::generate_with_hooks, src/chain.cpp:1175): the LOD chain of
+ * `source` under the ABI-5 settings, every proposal made by the device
+ * collapse executor (scorer needed for ranking 0) and audited on the GPU.
+ * Returns BLITZ_BUDGET_LIMITED or BLITZ_CANCELLED with a valid chain, as the
+ * reference's C shim does (src/c_api.cpp:180-186). */
+blitz_status blitz_hip_generate(blitz_hip_audit*, const blitz_mesh* source, const blitz_settings*,
+                                const blitz_hip_chain_options*, blitz_hip_scorer* scorer, blitz_hip_chain** out,
+                                char* error, size_t capacity);
+void blitz_hip_chain_destroy(blitz_hip_chain*);
+/* blitz::schedule (src/chain.cpp:101-118), host only: per level the screen
+ * size, transition limit and source limit (3 doubles each, `levels` rows). */
+blitz_status blitz_hip_schedule(const blitz_settings*, const blitz_hip_bounds*, double* out, size_t capacity,
+                                char* error, size_t error_capacity);
+blitz_status blitz_hip_chain_info_of(const blitz_hip_chain*, blitz_hip_chain_info*);
+blitz_status blitz_hip_chain_lod_of(const blitz_hip_chain*, size_t level, blitz_hip_chain_lod*);
+/* Triangles per level of finalist i (levels entries). */
+blitz_status blitz_hip_chain_finalist(const blitz_hip_chain*, size_t i, uint32_t* triangles, size_t capacity);
+blitz_status blitz_hip_chain_proposal_of(const blitz_hip_chain*, size_t i, blitz_hip_chain_proposal*);
 
 /* blitz::detail::rasterize_coverage (src/coverage.hpp:12, src/evaluate.cpp:125-189)
  * for one camera: row-major packed words (index = y*size + x), padding bits

@@ -416,7 +416,12 @@ Mask coverage_of(const Pixels& r) {
     return m;
 }
 
-// fdlibm e_acos.c (Sun Microsystems, 1993; freely redistributable), in the
+// fdlibm e_acos.c notice, preserved as its licence requires:
+// Copyright (C) 1993 by Sun Microsystems, Inc. All rights reserved.
+// Developed at SunSoft, a Sun Microsystems, Inc. business.
+// Permission to use, copy, modify, and distribute this software is freely
+// granted, provided that this notice is preserved.
+// fdlibm e_acos.c, in the
 // operation sequence of src/metric_angle.hpp:20-41: rational approximation of
 // asin about 0 and the half-angle forms near +-1, with the split square root
 // for x > 0.5.

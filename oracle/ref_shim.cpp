@@ -9,9 +9,13 @@
 // Signatures mirror oracle/audit_oracle.cpp so tests can swap one for the other.
 #include "blitz/evaluate.hpp"
 #include "blitz/remesher.hpp"
+#include "chain_hooks.hpp"
 #include "coverage.hpp"
 #include "neural/action.hpp"
+#include <cmath>
+#include <cstdio>
 #include <cstring>
+#include <span>
 #include <memory>
 #include <stdexcept>
 #include <vector>
@@ -382,6 +386,197 @@ int64_t ref_execute_actions(const float* pos, uint64_t nv, const uint32_t* idx, 
         stats[2] = st.accepted;
         stats[3] = st.rejected;
         return int64_t(lod.data.indices.size());
+    } catch (...) {
+        return -1;
+    }
+}
+
+// detail::generate_with_hooks (src/chain.cpp:1175) driven by CPU hooks that
+// follow src/neural/generate.cpp for the explicit-reuse, source-origin,
+// action-policy configuration: hooks.evaluate is blitz::evaluate with the
+// refinement bounded by neural::bounded_refinement (:215-224); the guarded
+// proposal is neural::execute_actions from the input towards the requested
+// target, ranked by shortest edge (3) or current plane (4) as in
+// ref_execute_actions, each trial passing four bounded evaluates (source
+// search, adjacent search, source audit, adjacent audit; :352-388); confirm
+// is the CPU confirmation of :456-548; no overdraw tie-break.
+struct HookSettings {
+    uint8_t levels, beam_width, search_supersample, audit_supersample, max_supersample, pad[3];
+    uint16_t candidate_budget, search_ortho, search_persp, audit_ortho, audit_persp, pad2[3];
+    uint32_t search_seed, audit_seed, action_trials, action_batch, ranking, transition_count;
+    double base_pixels, last_pixels, max_lod0_delta_px, max_changed_area;
+    const float* transition;  // x, y pairs
+};
+// counters: status, candidate_evaluations, audit_evaluations[4],
+// rejected_gates[4], legal_collapses, rejected_collapses, action_trials,
+// action_ranked, decoded, finalists, fallback_levels (17 values).
+int ref_generate_hooks(const float* pos, uint64_t nv, const uint32_t* idx, uint64_t ni, const HookSettings* c,
+                       uint32_t* indices_out, uint64_t capacity, uint64_t* offsets, Measure* source_out,
+                       Measure* adjacent_out, double* schedule_out, uint64_t* counters, uint32_t* finalists,
+                       uint64_t finalist_capacity) {
+    try {
+        MeshView source = view_of(pos, nv, idx, ni, nullptr, 0, nullptr, 0);
+        Settings s;
+        s.levels = c->levels;
+        s.profile = Profile::Coverage;
+        s.beam_width = c->beam_width;
+        s.search_supersample = c->search_supersample;
+        s.audit_supersample = c->audit_supersample;
+        s.max_supersample = c->max_supersample;
+        s.candidate_budget = c->candidate_budget;
+        s.search_views = {c->search_ortho, c->search_persp, c->search_seed};
+        s.audit_views = {c->audit_ortho, c->audit_persp, c->audit_seed};
+        s.base_pixels = c->base_pixels;
+        s.last_pixels = c->last_pixels;
+        if (c->max_lod0_delta_px > 0) s.max_lod0_delta_px = c->max_lod0_delta_px;
+        s.transition.points.clear();
+        for (uint32_t i = 0; i < c->transition_count; ++i)
+            s.transition.points.push_back({c->transition[2 * i], c->transition[2 * i + 1]});
+        s.max_changed_area = c->max_changed_area;
+        s.research.output = OutputMode::Reuse;
+        s.research.chain = ChainMode::Direct;
+        uint64_t legal = 0, rejected = 0, trials = 0, ranked = 0, decoded = 0;
+        auto bounded = [](const EvalSettings& e) {
+            auto b = e;
+            b.max_supersample = neural::bounded_refinement(e);
+            return b;
+        };
+        detail::GenerationHooks hooks;
+        hooks.evaluate = [&](MeshView a, MeshView b, const Bounds& bounds, const EvalSettings& e) {
+            return evaluate(a, b, bounds, bounded(e));
+        };
+        hooks.propose_guarded = [&](MeshView input, MeshView fixed_source, MeshView previous, const Bounds& bounds,
+                                    const ReduceSettings& rs, const EvalSettings& source_eval,
+                                    const EvalSettings& adjacent_eval, const EvalSettings& source_search,
+                                    const EvalSettings& adjacent_search) {
+            auto pass = [&](MeshView reference, MeshView candidate, const EvalSettings& e) {
+                return evaluate(reference, candidate, bounds, bounded(e)).passed;
+            };
+            auto gate = [&](MeshView candidate) {
+                return pass(fixed_source, candidate, source_search) && pass(previous, candidate, adjacent_search) &&
+                       pass(fixed_source, candidate, source_eval) && pass(previous, candidate, adjacent_eval);
+            };
+            auto rank = [&](const neural::ActionState& state, std::span<const neural::ActionRecord> rows) {
+                std::vector<float> scores(rows.size());
+                for (size_t i = 0; i < rows.size(); ++i)
+                    if (c->ranking == 3)
+                        scores[i] = -rows[i].x[59];
+                    else if (c->ranking == 4)
+                        scores[i] = float(-state.teacher_cost(rows[i].action));
+                return scores;
+            };
+            neural::ActionStats st;
+            Lod lod = neural::execute_actions(input, {}, rs.target_triangles, c->action_trials, rank, gate, &st, {},
+                                              c->action_batch);
+            ++decoded;
+            legal += st.accepted;
+            rejected += st.rejected;
+            trials += st.trials;
+            ranked += st.ranked;
+            return lod;
+        };
+        hooks.confirm = [&](Result& r) {
+            bool good = true;
+            auto confirm = [&](MeshView reference, MeshView candidate, const EvalSettings& e, Measurement& output) {
+                output = evaluate(reference, candidate, r.reference_bounds, e);
+                bool nonfinite = std::isnan(output.error) || std::isnan(output.coverage) ||
+                                 std::isnan(output.coverage_upper) || !std::isfinite(output.changed_area) ||
+                                 !std::isfinite(output.normal_degrees) ||
+                                 (output.passed && (!std::isfinite(output.error) || !std::isfinite(output.coverage) ||
+                                                    !std::isfinite(output.coverage_upper)));
+                if (!output.cancelled && !output.resource_limited && !nonfinite && output.complete && output.passed)
+                    return true;
+                output.complete = false;
+                output.passed = false;
+                return false;
+            };
+            for (size_t i = 1; i < r.lods.size() && good; ++i) {
+                auto& lod = r.lods[i];
+                EvalSettings e;
+                e.profile = s.profile;
+                e.weights = s.weights;
+                e.views = s.audit_views;
+                e.supersample = s.audit_supersample;
+                e.max_supersample = s.max_supersample;
+                e.screen_size = lod.schedule.pixels;
+                e.max_changed_area = s.max_changed_area;
+                double t = s.levels == 2 ? 0 : double(i - 1) / (s.levels - 2);
+                e.weights.normal *= s.normal_importance.at(t);
+                e.weights.color *= s.attribute_importance.at(t);
+                e.weights.material *= s.attribute_importance.at(t);
+                e.limit = lod.schedule.source;
+                good = confirm(source, lod.view(source), e, lod.source_error);
+                if (good) {
+                    e.limit = lod.schedule.transition;
+                    good = confirm(r.lods[i - 1].view(source), lod.view(source), e, lod.adjacent);
+                }
+            }
+            return good;
+        };
+        auto r = detail::generate_with_hooks(source, s, {}, &hooks);
+        uint64_t total = 0;
+        offsets[0] = 0;
+        for (size_t i = 0; i < r.lods.size(); ++i) {
+            auto v = r.lods[i].view(source);
+            if (r.lods[i].shared_vertices == false) return -2;
+            if (total + v.indices.size() > capacity) return -3;
+            std::memcpy(indices_out + total, v.indices.data(), v.indices.size() * 4);
+            total += v.indices.size();
+            offsets[i + 1] = total;
+            source_out[i] = out_of(r.lods[i].source_error);
+            adjacent_out[i] = out_of(r.lods[i].adjacent);
+            schedule_out[3 * i] = r.lods[i].schedule.pixels;
+            schedule_out[3 * i + 1] = r.lods[i].schedule.transition;
+            schedule_out[3 * i + 2] = r.lods[i].schedule.source;
+        }
+        uint32_t fallback_levels = 0;
+        for (size_t i = 1; i < r.lods.size(); ++i)
+            fallback_levels += same_mesh_data(r.lods[i].view(source), r.lods[i - 1].view(source));
+        uint64_t k = 0;
+        counters[k++] = uint64_t(r.status);
+        counters[k++] = r.candidate_evaluations;
+        for (auto a : r.audit_evaluations) counters[k++] = a;
+        for (auto a : r.rejected_gates) counters[k++] = a;
+        counters[k++] = legal;
+        counters[k++] = rejected;
+        counters[k++] = trials;
+        counters[k++] = ranked;
+        counters[k++] = decoded;
+        counters[k++] = r.candidates.size();
+        counters[k++] = fallback_levels;
+        uint64_t w = 0;
+        for (auto& cand : r.candidates)
+            for (auto t : cand.triangles)
+                if (w < finalist_capacity) finalists[w++] = t;
+        return int(r.lods.size());
+    } catch (const std::exception& e) {
+        std::fprintf(stderr, "ref_generate_hooks: %s\n", e.what());
+        return -1;
+    }
+}
+
+// blitz::schedule (src/chain.cpp:101-118) of the settings of ref_generate_hooks
+// plus pixels_per_meter / meters_per_unit (base_pixels 0: unset).
+int ref_schedule(const HookSettings* c, double pixels_per_meter, double meters_per_unit, const float* center,
+                 double radius, double* out) {
+    try {
+        Settings s;
+        s.levels = c->levels;
+        s.pixels_per_meter = pixels_per_meter;
+        s.meters_per_unit = meters_per_unit;
+        if (c->base_pixels > 0) s.base_pixels = c->base_pixels;
+        s.last_pixels = c->last_pixels;
+        if (c->max_lod0_delta_px > 0) s.max_lod0_delta_px = c->max_lod0_delta_px;
+        s.transition.points.clear();
+        for (uint32_t i = 0; i < c->transition_count; ++i)
+            s.transition.points.push_back({c->transition[2 * i], c->transition[2 * i + 1]});
+        auto steps = schedule({{center[0], center[1], center[2]}, radius}, s);
+        for (size_t i = 0; i < steps.size(); ++i) {
+            out[3 * i] = steps[i].pixels;
+            out[3 * i + 1] = steps[i].transition;
+            out[3 * i + 2] = steps[i].source;
+        }
+        return int(steps.size());
     } catch (...) {
         return -1;
     }

@@ -52,6 +52,23 @@ class ChainSettings(C.Structure):
     ]
 
 
+class HookSettings(C.Structure):
+    _fields_ = [
+        ("levels", C.c_uint8), ("beam_width", C.c_uint8), ("search_supersample", C.c_uint8),
+        ("audit_supersample", C.c_uint8), ("max_supersample", C.c_uint8), ("pad", C.c_uint8 * 3),
+        ("candidate_budget", C.c_uint16), ("search_ortho", C.c_uint16), ("search_persp", C.c_uint16),
+        ("audit_ortho", C.c_uint16), ("audit_persp", C.c_uint16), ("pad2", C.c_uint16 * 3),
+        ("search_seed", C.c_uint32), ("audit_seed", C.c_uint32), ("action_trials", C.c_uint32),
+        ("action_batch", C.c_uint32), ("ranking", C.c_uint32), ("transition_count", C.c_uint32),
+        ("base_pixels", C.c_double), ("last_pixels", C.c_double), ("max_lod0_delta_px", C.c_double),
+        ("max_changed_area", C.c_double), ("transition", C.POINTER(C.c_float)),
+    ]
+
+
+HOOK_COUNTERS = ("status", "candidate_evaluations", "audit_evaluations", "rejected_gates", "legal_collapses",
+                 "rejected_collapses", "action_trials", "action_ranked", "decoded", "finalists", "fallback_levels")
+
+
 def available(kind: str) -> bool:
     return PATHS[kind].exists()
 
@@ -253,6 +270,55 @@ class Oracle:
               C.byref(status))
         assert n >= 0
         return tri[:n].tolist(), se[:n].tolist(), ae[:n].tolist(), int(audits.value), int(status.value)
+
+
+def ref_generate_hooks(mesh: Mesh, chain) -> dict:
+    """The reference's detail::generate_with_hooks with the CPU hooks of
+    oracle/ref_shim.cpp (explicit reuse, source origin, shortest-edge or
+    current-plane executor, four-gate trials, CPU confirmation). `chain` is a
+    blitzremesher_amd.chain.ChainSettings (coverage profile, default curves)
+    with `action_trials`, `action_batch` and `ranking` attributes added by the
+    caller via a ChainOptions-like object: pass (settings, options)."""
+    settings, options = chain
+    lib = Oracle("ref").lib
+    hs = HookSettings()
+    for name in ("levels", "beam_width", "search_supersample", "audit_supersample", "max_supersample",
+                 "candidate_budget", "search_seed", "audit_seed", "base_pixels", "last_pixels", "max_lod0_delta_px",
+                 "max_changed_area"):
+        setattr(hs, name, getattr(settings, name))
+    hs.search_ortho, hs.search_persp = settings.search_orthographic, settings.search_perspective
+    hs.audit_ortho, hs.audit_persp = settings.audit_orthographic, settings.audit_perspective
+    hs.action_trials, hs.action_batch, hs.ranking = options.action_trials, options.action_batch, options.ranking
+    tr = np.asarray(settings.transition, np.float32).reshape(-1)
+    hs.transition_count = tr.size // 2
+    hs.transition = tr.ctypes.data_as(C.POINTER(C.c_float))
+    levels = settings.levels
+    cap = mesh.indices.size * levels
+    idx = np.zeros(cap, np.uint32)
+    offsets = np.zeros(levels + 1, np.uint64)
+    src = (OracleMeasurement * levels)()
+    adj = (OracleMeasurement * levels)()
+    sched = np.zeros(3 * levels, np.float64)
+    counters = np.zeros(32, np.uint64)
+    fin = np.zeros(64 * levels, np.uint32)
+    f = lib.ref_generate_hooks
+    f.restype = C.c_int
+    n = f(_ptr(mesh.positions, C.c_float), C.c_uint64(mesh.positions.shape[0]), _ptr(mesh.indices, C.c_uint32),
+          C.c_uint64(mesh.indices.size), C.byref(hs), idx.ctypes.data_as(C.POINTER(C.c_uint32)), C.c_uint64(cap),
+          offsets.ctypes.data_as(C.POINTER(C.c_uint64)), src, adj, sched.ctypes.data_as(C.POINTER(C.c_double)),
+          counters.ctypes.data_as(C.POINTER(C.c_uint64)), fin.ctypes.data_as(C.POINTER(C.c_uint32)),
+          C.c_uint64(fin.size))
+    assert n == levels, n
+    c = [int(x) for x in counters]
+    out = {"status": c[0], "candidate_evaluations": c[1], "audit_evaluations": c[2:6], "rejected_gates": c[6:10],
+           "legal_collapses": c[10], "rejected_collapses": c[11], "action_trials": c[12], "action_ranked": c[13],
+           "decoded": c[14], "finalists": c[15], "fallback_levels": c[16]}
+    out["lods"] = [idx[int(offsets[i]):int(offsets[i + 1])].reshape(-1, 3).copy() for i in range(levels)]
+    out["source"] = [Measurement.from_c(src[i]) for i in range(levels)]
+    out["adjacent"] = [Measurement.from_c(adj[i]) for i in range(levels)]
+    out["schedule"] = sched.reshape(-1, 3).tolist()
+    out["finalist_triangles"] = fin[: out["finalists"] * levels].reshape(-1, levels).tolist()
+    return out
 
 
 def _stream_args(mesh: Mesh):
