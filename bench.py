@@ -204,7 +204,9 @@ def patch_leg(ref, center, radius, settings, count: int, repeats: int) -> dict:
             "patches": count, "removed_faces": removed, "added_faces": added, "collapses": collapses,
             "bind_ms": round(bind_ms, 2), "batch_ms": round(1000 * t, 3),
             "patch_kernel_ms": round(float(np.median(kernel)), 3),
-            "patch_audits_per_s": round(count / t, 1), "audited_collapses_per_s": round(collapses / t, 1),
+            "patch_audits_per_s": round(count / t, 1),
+            # One audit against the source per patch: not the four-gate count.
+            "single_audit_collapses_per_s": round(collapses / t, 1),
             "full_path_fallbacks": fallbacks, "passed": int(sum(m.passed for m in ms)),
             "generate_s": round(generate_s, 2)}
 
@@ -279,7 +281,8 @@ def executor_leg(budget: int, cpu_budget: int) -> dict:
            "ranked": st["ranked"], "states": st["states"], "triangles": int(len(out)), "ms": round(st["ms"], 2),
            "ms_per_trial": round(st["ms"] / max(1, st["trials"]), 3),
            "stage_ms": {k: round(st[f"{k}_ms"], 2) for k in ("build", "rank", "select", "trial", "gate")},
-           "audited_collapses_per_s": round(st["accepted"] / seconds, 1) if seconds else None}
+           # One gate (an audit against the source) per trial: not the four-gate count of the chain leg.
+           "one_gate_collapses_per_s": round(st["accepted"] / seconds, 1) if seconds else None}
     try:
         sys.path.insert(0, str(ROOT / "tests"))
         from oracle_lib import available, ref_execute_actions
@@ -290,7 +293,7 @@ def executor_leg(budget: int, cpu_budget: int) -> dict:
             same = (len(cpu_out) == len(sample_out) and (cpu_out == sample_out).all() and
                     all(cpu_st[k] == sample_st[k] for k in ("ranked", "trials", "accepted", "rejected")))
             leg["cpu_reference"] = {"trials": cpu_st["trials"], "accepted": cpu_st["accepted"], "s": round(dt, 2),
-                                    "audited_collapses_per_s": round(cpu_st["accepted"] / dt, 1), "cores": 1,
+                                    "one_gate_collapses_per_s": round(cpu_st["accepted"] / dt, 1), "cores": 1,
                                     "gpu_same_trials_ms": round(sample_st["ms"], 2),
                                     "identical_output_and_counters": bool(same)}
     except Exception as e:  # the CPU comparison is optional
@@ -360,8 +363,111 @@ def neural_leg(train_steps: int, budget: int) -> dict:
             out[name] = {"trials": st["trials"], "accepted": st["accepted"], "rejected": st["rejected"],
                          "triangles": int(len(o)), "ms": round(st["ms"], 1),
                          "accepted_per_trial": round(st["accepted"] / max(1, st["trials"]), 3),
-                         "audited_collapses_per_s": round(st["accepted"] / (st["ms"] / 1000.0), 1)}
+                         "one_gate_collapses_per_s": round(st["accepted"] / (st["ms"] / 1000.0), 1)}
     return out
+
+
+def c1_chain_settings():
+    """The CPU-runnable chain configuration: 4 levels from 256 px to 32 px,
+    transition and LOD0 bounds 1 px, coverage profile, 16 orthographic search
+    and audit views (reference default seeds), ss 4 search / 8 audit / 16 max,
+    candidate budget 8, beam 2, max changed area 1."""
+    from blitzremesher_amd.chain import ChainSettings
+    return ChainSettings(levels=4, profile=0, base_pixels=256, last_pixels=32, max_lod0_delta_px=1.0,
+                         transition=((0, 1), (1, 1)), search_orthographic=16, search_perspective=0,
+                         search_seed=0xB1172024, audit_orthographic=16, audit_perspective=0, audit_seed=0xB1172026,
+                         search_supersample=4, audit_supersample=8, max_supersample=16, candidate_budget=8,
+                         beam_width=2, max_changed_area=1.0)
+
+
+def _cpu_classical_chain(_=None) -> float:
+    """One reference classical chain (oracle/_ref generate) of the icosphere;
+    returns its wall seconds."""
+    sys.path.insert(0, str(ROOT / "tests"))
+    from oracle_lib import ChainSettings as RefChain
+    from oracle_lib import Oracle
+    from blitzremesher_amd import meshes
+    s = c1_chain_settings()
+    cs = RefChain(levels=s.levels, beam_width=s.beam_width, search_supersample=s.search_supersample,
+                  audit_supersample=s.audit_supersample, max_supersample=s.max_supersample,
+                  candidate_budget=s.candidate_budget, search_ortho=s.search_orthographic, search_persp=0,
+                  audit_ortho=s.audit_orthographic, audit_persp=0, search_seed=s.search_seed, audit_seed=s.audit_seed,
+                  base_pixels=s.base_pixels, last_pixels=s.last_pixels, max_lod0_delta_px=s.max_lod0_delta_px,
+                  transition0=1.0, transition1=1.0, max_changed_area=s.max_changed_area)
+    mesh = meshes.icosphere(5, 0.02, 1)
+    t0 = time.perf_counter()
+    Oracle("ref").generate(mesh, cs)
+    return time.perf_counter() - t0
+
+
+def cpu_model() -> str:
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def chain_cpu_baseline() -> dict | None:
+    """The reference's classical CPU generator (QEM reduce proposals, CPU
+    coverage audits; oracle/_ref generate) on the chain leg's mesh and
+    settings: one chain on one core, then one chain per core on all cores at
+    once (worker processes; run before the GPU is touched)."""
+    sys.path.insert(0, str(ROOT / "tests"))
+    from oracle_lib import available
+    if not available("ref"):
+        return None
+    import multiprocessing as mp
+    one = _cpu_classical_chain()
+    cores = max(1, min(16, len(os.sched_getaffinity(0))))
+    t0 = time.perf_counter()
+    with mp.get_context("spawn").Pool(cores) as pool:
+        pool.map(_cpu_classical_chain, range(cores))
+    wall = time.perf_counter() - t0
+    return {"ms_per_chain_1_core": round(1000 * one, 1), "chains_per_s_all_cores": round(cores / wall, 3),
+            "ms_per_chain_all_cores": round(1000 * wall / cores, 1), "cores": cores, "cpu_model": cpu_model(),
+            "nproc": os.cpu_count(),
+            "note": "reference generate(): QEM reduce proposals + CPU coverage audits, same mesh and settings"}
+
+
+def chain_leg(repeats: int, cpu_reference: dict | None) -> dict:
+    """LOD chains of the CPU-runnable icosphere (10,242 V / 20,480 F, radial
+    noise 0.02, seed 1) under c1_chain_settings: explicit reuse output,
+    proposals from the source by the collapse executor in the current-plane
+    order (the plain classical order on the device; 64 trials per proposal,
+    batches of 32), every trial certified by the four gates (source search,
+    adjacent search, source audit, adjacent audit), every offer measured by
+    the four gates and the selected chain confirmed by full audits, overdraw
+    tie-break on. `four_gate_audited_collapses` counts collapses that passed
+    all four gates."""
+    import numpy as np
+    from blitzremesher_amd import Audit, meshes
+    from blitzremesher_amd.chain import ChainOptions, generate
+    mesh = meshes.icosphere(5, 0.02, 1)
+    settings = c1_chain_settings()
+    options = ChainOptions(action_trials=64, action_batch=32, ranking=4, overdraw_tiebreak=True)
+    with Audit(0, cache_rasters=True) as audit:
+        generate(audit, mesh, settings, options)  # warm-up
+        runs = [generate(audit, mesh, settings, options) for _ in range(repeats)]
+    r = runs[-1]
+    ms = float(np.median([x.info["ms"] for x in runs]))
+    i = r.info
+    return {"workload": "4-level LOD chain (256 -> 32 px, transition and LOD0 bounds 1 px, 16 views, ss 4/8/16, "
+                        "budget 8, beam 2) of the 10,242-vertex icosphere; device collapse executor in "
+                        "current-plane order (64 trials x batch 32 per proposal), four-gate certified trials, "
+                        "GPU audits",
+            "ms_per_chain": round(ms, 1), "chains": repeats,
+            "triangles": [l.triangles for l in r.lods],
+            "four_gate_audited_collapses": i["legal_collapses"],
+            "four_gate_audited_collapses_per_s": round(i["legal_collapses"] / (ms * 1e-3), 1),
+            "rejected_collapses": i["rejected_collapses"], "trials": i["action_trials"], "proposals": i["decoded"],
+            "gate_audits": i["gate_audits"], "offer_audits": int(sum(i["audit_evaluations"])),
+            "confirm_audits": i["confirm_audits"], "finalists": len(r.finalists), "status": r.status,
+            "stage_ms": {k: round(i[f"{k}_ms"], 1) for k in ("executor", "gate", "offer", "confirm", "tiebreak")},
+            "levels_passed": all(l.source.passed and l.adjacent.passed for l in r.lods[1:]),
+            "cpu_reference_classical": cpu_reference}
 
 
 def copy_probe(mib: int) -> int:
@@ -404,6 +510,8 @@ def main() -> int:
     ap.add_argument("--no-executor", action="store_true")
     ap.add_argument("--no-appearance", action="store_true")
     ap.add_argument("--no-neural", action="store_true")
+    ap.add_argument("--no-chain", action="store_true")
+    ap.add_argument("--chain-repeats", type=int, default=2)
     ap.add_argument("--neural-train-steps", type=int, default=300)
     ap.add_argument("--executor-budget", type=int, default=400)
     ap.add_argument("--executor-cpu-budget", type=int, default=3)
@@ -420,8 +528,9 @@ def main() -> int:
 
     from blitzremesher_amd import Audit, _lib
     import numpy as np
+    # CPU-only baselines run first, before this process touches the GPU.
+    chain_cpu = None if (args.no_cpu or args.no_chain) else chain_cpu_baseline()
     ref, cand, center, radius, settings = workload()
-    collapses_full = int(np.unique(ref.indices).size - np.unique(cand.indices).size)
     # torch is not imported: its wheel bundles a second HIP runtime, and two
     # HIP runtimes in one process cannot share the device. The library's own
     # hipDeviceSynchronize brackets the timed region instead.
@@ -482,6 +591,7 @@ def main() -> int:
                                                        max(3, min(args.steps, 10)))
     appearance = None if args.no_appearance else appearance_leg(max(3, min(args.steps, 10)))
     neural = None if args.no_neural else neural_leg(args.neural_train_steps, args.executor_budget)
+    chain = None if args.no_chain else chain_leg(args.chain_repeats, chain_cpu)
     executor = None if args.no_executor else executor_leg(args.executor_budget,
                                                           0 if args.no_cpu else args.executor_cpu_budget)
 
@@ -505,15 +615,13 @@ def main() -> int:
                      "fp64_vector": {"kernel": "raster_band_kernel", "ops_per_launch": fp64_ops,
                                      "achieved": round(fp64_tflops, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                                      "frac": round(fp64_tflops / FP64_PEAK_TFLOPS, 4)}},
-        # Each collapse of the candidate removes one vertex from use.
-        "collapses_per_audit": collapses_full,
-        "audited_collapses_per_s": round(collapses_full * 1000.0 / ms_per_step, 1),
         "audit_roofline": {"algorithmic_bytes": b["audit"],
                            "achieved_gbs": round(b["audit"] / (ms_per_step * 1e-3) / 1e9, 2),
                            "frac": round(b["audit"] / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 5)},
         "measurement": m.as_tuple() if m else None,
         "cpu_baseline": cpu,
         "parity_on_cpu_sample": parity,
+        "chain": chain,
         "patches": patches,
         "executor": executor,
         "appearance": appearance,
