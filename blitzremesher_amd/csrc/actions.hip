@@ -651,17 +651,38 @@ struct Clock {
 ActionSet::ActionSet(int device, const HostMesh& m, const float* uv) : device_(device) {
     DeviceGuard guard(device_);
     hip_check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
+    build(m, uv);
+}
+
+void ActionSet::rebuild(const HostMesh& m, const float* uv) {
+    DeviceGuard guard(device_);
+    hip_check(hipStreamSynchronize(stream_), "action state");
+    build(m, uv);
+}
+
+void ActionSet::build(const HostMesh& m, const float* uv) {
     Clock clock;
     const hipStream_t st = stream_;
+    legal_count_ = edges_ = ranked_ = 0;
+    touch_ready_ = false;
+    const uint32_t previous_faces = faces_;
     vertices_ = uint32_t(m.positions.size() / 3);
     faces_ = uint32_t(m.indices.size() / 3);
     if (!vertices_ || !faces_) throw std::invalid_argument("action state needs a non-empty mesh");
     if (uint64_t(faces_) * 6 >= uint64_t(INT32_MAX)) throw std::invalid_argument("mesh too large for 32-bit corners");
+    if (staging_ && faces_ > previous_faces) {
+        (void)hipHostFree(staging_);
+        staging_ = nullptr;
+    }
     const uint32_t n = vertices_, corners = 3 * faces_;
-    diameter_ = std::max(1e-20, bounds_of(m.positions.data(), n).radius * 2);
+    // The weld (geometric vertex of each index) depends on the positions
+    // only: a rebuild over the same positions keeps it.
+    const bool same_positions = weld_ready_ && weld_positions_ == m.positions;
+    if (!same_positions) diameter_ = std::max(1e-20, bounds_of(m.positions.data(), n).radius * 2);
     // Every buffer of the build is sized here, before any launch, so nothing
-    // is reallocated under queued work.
-    DeviceBuffer weld, weld2, corner_buf, key_buf, temp, all_buf, flag_buf;
+    // is reallocated under queued work; the buffers persist across rebuilds.
+    DeviceBuffer &weld = build_weld_, &weld2 = build_weld2_, &corner_buf = build_corners_, &key_buf = build_keys_,
+                 &temp = build_temp_, &all_buf = build_all_, &flag_buf = build_flags_;
     uint32_t* kx = grow<uint32_t>(weld, 6 * size_t(n));
     uint32_t* ky = kx + n;
     uint32_t* kz = ky + n;
@@ -711,7 +732,7 @@ ActionSet::ActionSet(int device, const HostMesh& m, const float* uv) : device_(d
         size_t b = bytes;
         hip_check(call(b), what);
     };
-    {
+    if (!same_positions) {
         std::vector<float4> pos(n);
         for (uint32_t i = 0; i < n; ++i)
             pos[i] = make_float4(m.positions[3 * i], m.positions[3 * i + 1], m.positions[3 * i + 2], 0.f);
@@ -729,23 +750,28 @@ ActionSet::ActionSet(int device, const HostMesh& m, const float* uv) : device_(d
                   "upload materials");
     if (uv)
         hip_check(hipMemcpyAsync(grow<float2>(uv_, n), uv, size_t(n) * 8, hipMemcpyHostToDevice, st), "upload uv");
-    // ---- weld: stable radix passes on the z, y, x keys carry the index order.
-    weld_keys_kernel<<<blocks_for(n), 256, 0, st>>>(positions_.as<float4>(), n, kx, ky, kz, ids);
-    auto sort_pass = [&](const uint32_t* key, const uint32_t* order_in, uint32_t* order_out) {
-        gather_kernel<<<blocks_for(n), 256, 0, st>>>(key, order_in, n, key_tmp);
-        run("weld sort", [&](size_t& b) {
-            return hipcub::DeviceRadixSort::SortPairs(work, b, key_tmp, key_sorted, order_in, order_out, int(n), 0, 32,
-                                                      st);
+    has_uv_ = uv != nullptr;
+    if (!same_positions) {
+        // ---- weld: stable radix passes on the z, y, x keys carry the index order.
+        weld_keys_kernel<<<blocks_for(n), 256, 0, st>>>(positions_.as<float4>(), n, kx, ky, kz, ids);
+        auto sort_pass = [&](const uint32_t* key, const uint32_t* order_in, uint32_t* order_out) {
+            gather_kernel<<<blocks_for(n), 256, 0, st>>>(key, order_in, n, key_tmp);
+            run("weld sort", [&](size_t& b) {
+                return hipcub::DeviceRadixSort::SortPairs(work, b, key_tmp, key_sorted, order_in, order_out, int(n), 0,
+                                                          32, st);
+            });
+        };
+        sort_pass(kz, ids, ord);
+        sort_pass(ky, ord, ord2);
+        sort_pass(kx, ord2, ord);
+        weld_heads_kernel<<<blocks_for(n), 256, 0, st>>>(ord, kx, ky, kz, n, ord2);
+        run("weld scan", [&](size_t& b) {
+            return hipcub::DeviceScan::InclusiveScan(work, b, ord2, key_sorted, hipcub::Max(), int(n), st);
         });
-    };
-    sort_pass(kz, ids, ord);
-    sort_pass(ky, ord, ord2);
-    sort_pass(kx, ord2, ord);
-    weld_heads_kernel<<<blocks_for(n), 256, 0, st>>>(ord, kx, ky, kz, n, ord2);
-    run("weld scan", [&](size_t& b) {
-        return hipcub::DeviceScan::InclusiveScan(work, b, ord2, key_sorted, hipcub::Max(), int(n), st);
-    });
-    weld_assign_kernel<<<blocks_for(n), 256, 0, st>>>(ord, key_sorted, n, geometry);
+        weld_assign_kernel<<<blocks_for(n), 256, 0, st>>>(ord, key_sorted, n, geometry);
+        weld_positions_ = m.positions;
+        weld_ready_ = true;
+    }
     // ---- corners: face lists and edge keys.
     hip_check(hipMemsetAsync(invalid, 0, n, st), "clear");
     hip_check(hipMemsetAsync(boundary, 0, n, st), "clear");
@@ -791,7 +817,7 @@ ActionSet::ActionSet(int device, const HostMesh& m, const float* uv) : device_(d
         });
         low_words_kernel<<<blocks_for(slots), 256, 0, st>>>(directed_sorted, slots, neighbors);
     }
-    Topo t{positions_.as<float4>(), uv ? uv_.as<float2>() : nullptr, indices_.as<uint32_t>(),
+    Topo t{positions_.as<float4>(), has_uv_ ? uv_.as<float2>() : nullptr, indices_.as<uint32_t>(),
            m.materials.empty() ? nullptr : materials_.as<uint16_t>(), geometry, invalid, boundary, offsets,
            neighbors, face_offsets, faces, n, faces_};
     fans_kernel<<<blocks_for(n), 256, 0, st>>>(t, invalid, visited);
@@ -834,7 +860,7 @@ void ActionSet::download_legal(uint32_t* pairs) const {
 }
 
 ActionTopo ActionSet::topo() const {
-    return {positions_.as<float4>(), uv_.p ? uv_.as<float2>() : nullptr, indices_.as<uint32_t>(),
+    return {positions_.as<float4>(), has_uv_ ? uv_.as<float2>() : nullptr, indices_.as<uint32_t>(),
             has_materials_ ? materials_.as<uint16_t>() : nullptr, geometry_.as<uint32_t>(), invalid_.as<uint8_t>(),
             boundary_.as<uint8_t>(), offsets_.as<uint32_t>(), neighbors_.as<uint32_t>(),
             face_offsets_.as<uint32_t>(), face_lists_.as<uint32_t>(), vertices_, faces_};
@@ -994,7 +1020,7 @@ std::vector<uint32_t> ActionSet::trial(const uint32_t* rows, size_t n, std::vect
 std::vector<uint32_t> execute_actions(int device, const HostMesh& input, const float* uv, uint64_t target,
                                       uint32_t budget, uint32_t ranking, uint32_t batch, const ActionGate& gate,
                                       ExecuteStats& stats, std::vector<uint16_t>* materials_out, Scorer* scorer,
-                                      const Bounds* bounds) {
+                                      const Bounds* bounds, ExecutorCache* cache) {
     if (!batch || batch > 64) throw std::invalid_argument("action batch outside 1..64");
     if (ranking == 2) throw Unavailable("shuffled ranking is not implemented by the HIP executor");
     if (ranking > 4) throw std::invalid_argument("unknown ranking");
@@ -1009,38 +1035,70 @@ std::vector<uint32_t> execute_actions(int device, const HostMesh& input, const f
         return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t).count();
     };
     bool no_legal = false;
+    ExecutorCache local;
+    ExecutorCache& c = cache ? *cache : local;
+    const std::vector<float> uv_values = uv ? std::vector<float>(uv, uv + input.positions.size() / 3 * 2)
+                                            : std::vector<float>{};
+    if (c.input_state && (c.input_state->device() != device || c.input_indices != input.indices ||
+                          c.input_materials != input.materials || c.input_uv != uv_values))
+        c = ExecutorCache{};
+    bool first = true;
     while (state.indices.size() / 3 > target && stats.trials < budget) {
         auto t = std::chrono::steady_clock::now();
-        ActionSet set(device, state, uv);
+        // The input's state (ranked once per ranking order that does not
+        // depend on a scorer), then one reused state for the later meshes.
+        const bool cached_rank = first && ranking != 0 && c.input_state && c.input_ranking == int(ranking);
+        ActionSet* set;
+        if (first) {
+            if (!c.input_state) {
+                c.input_state = std::make_unique<ActionSet>(device, state, uv);
+                c.input_indices = input.indices;
+                c.input_materials = input.materials;
+                c.input_uv = uv_values;
+                c.input_ranking = -1;
+                ++stats.states;
+            }
+            set = c.input_state.get();
+        } else {
+            if (c.work)
+                c.work->rebuild(state, uv);
+            else
+                c.work = std::make_unique<ActionSet>(device, state, uv);
+            ++stats.states;
+            set = c.work.get();
+        }
+        first = false;
         stats.build_ms += since(t);
-        ++stats.states;
-        const size_t n = set.legal_count();
+        const size_t n = set->legal_count();
         if (!n) {
             no_legal = true;
             break;
         }
         t = std::chrono::steady_clock::now();
-        std::unique_ptr<Graph> graph;
-        if (ranking == 0) {
-            graph = scorer->graph(state, *bounds);
-            scorer->forward(*graph, Precision::Bf16);
+        if (!cached_rank) {
+            std::unique_ptr<Graph> graph;
+            if (ranking == 0) {
+                graph = scorer->graph(state, *bounds);
+                scorer->forward(*graph, Precision::Bf16);
+            }
+            set->rank(ranking, diameter, graph.get(), nullptr);
+            if (set == c.input_state.get()) c.input_ranking = ranking == 0 ? -1 : int(ranking);
         }
-        set.rank(ranking, diameter, graph.get(), nullptr);
         stats.rank_ms += since(t);
         stats.ranked += n;
         const uint64_t faces = state.indices.size() / 3;
         bool accepted = false;
         for (size_t position = 0; position < n && stats.trials < budget && !accepted;) {
-            size_t first = 0;
             t = std::chrono::steady_clock::now();
-            std::vector<uint32_t> picked = set.independent_ranked(position, batch, faces - target, &first);
+            size_t first_pick = 0;
+            std::vector<uint32_t> picked = set->independent_ranked(position, batch, faces - target, &first_pick);
             stats.select_ms += since(t);
             if (picked.empty()) break;
             for (;;) {
                 if (stats.trials == budget) break;
                 std::vector<uint16_t> materials;
                 t = std::chrono::steady_clock::now();
-                std::vector<uint32_t> candidate = set.trial(picked.data(), picked.size(), &materials);
+                std::vector<uint32_t> candidate = set->trial(picked.data(), picked.size(), &materials);
                 stats.trial_ms += since(t);
                 ++stats.trials;
                 t = std::chrono::steady_clock::now();
@@ -1059,7 +1117,7 @@ std::vector<uint32_t> execute_actions(int device, const HostMesh& input, const f
                 picked.resize(picked.size() / 2);
             }
             // A rejected singleton is not retried: continue after the first pick.
-            position = first + 1;
+            position = first_pick + 1;
         }
         if (!accepted) break;
     }

@@ -7,6 +7,7 @@
 
 #include <cstdint>
 #include <functional>
+#include <memory>
 #include <vector>
 
 #include "audit.hpp"
@@ -48,6 +49,9 @@ class ActionSet {
     // (src/neural/action.cpp:371-383).
     ActionSet(int device, const HostMesh& m, const float* uv);
     ~ActionSet();
+    // The same state of another mesh (of the next executor state), keeping
+    // the device buffers and, when the positions are unchanged, the weld.
+    void rebuild(const HostMesh& m, const float* uv);
     ActionSet(const ActionSet&) = delete;
     ActionSet& operator=(const ActionSet&) = delete;
 
@@ -81,6 +85,7 @@ class ActionSet {
     int device() const { return device_; }
 
   private:
+    void build(const HostMesh& m, const float* uv);
     void claim(const uint32_t* rows, size_t n, uint32_t maximum, uint64_t removable, bool strict,
                std::vector<uint32_t>& picked);
     uint32_t claim_device(const uint32_t* drows, size_t n, uint32_t maximum, uint64_t removable, bool strict,
@@ -102,6 +107,12 @@ class ActionSet {
     void* staging_ = nullptr;  // pinned host memory for trial downloads
     DeviceBuffer touch_;       // per-vertex lane masks of the claim kernel
     bool touch_ready_ = false;
+    bool has_uv_ = false;
+    // Build workspaces, kept across rebuilds.
+    DeviceBuffer build_weld_, build_weld2_, build_corners_, build_keys_, build_temp_, build_all_, build_flags_;
+    // Host positions the device weld was computed from.
+    std::vector<float> weld_positions_;
+    bool weld_ready_ = false;
 };
 
 // The gate of the executor: true when the candidate (index buffer, kept
@@ -119,6 +130,18 @@ struct ExecuteStats {
     // reached, 1 trial budget, 2 no legal actions, 3 no accepted action.
     uint8_t stop_reason = 3;
 };
+// State kept across execute_actions calls on one input (the reference keeps
+// its GpuActionState and resets it per proposal, src/neural/generate.cpp:
+// 338-346): the input's action state ranked once, and an action state whose
+// device buffers (and weld) are reused for every later state.
+struct ExecutorCache {
+    std::vector<uint32_t> input_indices;
+    std::vector<uint16_t> input_materials;
+    std::vector<float> input_uv;
+    std::unique_ptr<ActionSet> input_state;
+    int input_ranking = -1;
+    std::unique_ptr<ActionSet> work;
+};
 // neural::execute_actions (src/neural/action.cpp:453-525) on the device
 // action state: rank (0 learned: BF16 logits of `scorer` on each state's
 // graph, built with `bounds`; 1 constant, 3 shortest edge, 4 current plane),
@@ -128,6 +151,7 @@ struct ExecuteStats {
 std::vector<uint32_t> execute_actions(int device, const HostMesh& input, const float* uv, uint64_t target,
                                       uint32_t budget, uint32_t ranking, uint32_t batch, const ActionGate& gate,
                                       ExecuteStats& stats, std::vector<uint16_t>* materials_out = nullptr,
-                                      Scorer* scorer = nullptr, const Bounds* bounds = nullptr);
+                                      Scorer* scorer = nullptr, const Bounds* bounds = nullptr,
+                                      ExecutorCache* cache = nullptr);
 
 }  // namespace blitz_hip

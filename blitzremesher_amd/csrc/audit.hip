@@ -10,6 +10,7 @@
 #include "audit.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -41,6 +42,13 @@ F3 rotate(F3 v, uint32_t seed) {
     const double a = (seed % 65521) * .000731, b = ((seed >> 8) % 32749) * .000913;
     const double x = v.x * std::cos(a) - v.y * std::sin(a), y = v.x * std::sin(a) + v.y * std::cos(a);
     return {float(x), float(y * std::cos(b) - v.z * std::sin(b)), float(y * std::sin(b) + v.z * std::cos(b))};
+}
+
+// Serials identify device meshes across every audit object of the process
+// (caches and identity checks key on them), so one counter serves them all.
+uint64_t next_serial() {
+    static std::atomic<uint64_t> counter{1};
+    return counter.fetch_add(1);
 }
 
 struct Clock {
@@ -199,7 +207,7 @@ std::unique_ptr<DeviceMesh> Audit::upload(const HostMesh& m) {
     d->pos_digest[1] = m.pos_digest[1];
     d->vertex_digest[0] = m.vertex_digest[0];
     d->vertex_digest[1] = m.vertex_digest[1];
-    d->serial = next_serial_++;
+    d->serial = next_serial();
     d->positions_serial = d->serial;
     if (!m.normals.empty()) d->raw_normals = std::make_shared<const std::vector<float>>(m.normals);
     if (!m.colors.empty()) d->raw_colors = std::make_shared<const std::vector<uint8_t>>(m.colors);
@@ -286,7 +294,7 @@ std::unique_ptr<DeviceMesh> Audit::derive(const DeviceMesh& base, const std::vec
     d->pos_digest[1] = base.pos_digest[1];
     d->vertex_digest[0] = base.vertex_digest[0];
     d->vertex_digest[1] = base.vertex_digest[1];
-    d->serial = next_serial_++;
+    d->serial = next_serial();
     d->positions_serial = base.positions_serial;
     d->raw_normals = base.raw_normals;
     d->raw_colors = base.raw_colors;

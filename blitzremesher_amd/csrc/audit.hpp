@@ -201,6 +201,8 @@ class Audit {
     bool same_positions(const DeviceMesh& a, const DeviceMesh& b);
     const Timing& last_timing() const { return timing_; }
     int device() const { return device_; }
+    uint64_t memory_bytes() const { return memory_bytes_; }
+    bool cache_rasters() const { return cache_rasters_; }
     hipStream_t stream() const { return stream_; }
     void release(DeviceMesh* m);
 
@@ -293,7 +295,6 @@ class Audit {
     hipStream_t stream_{};
     hipEvent_t events_[10]{};
     std::vector<std::pair<void*, size_t>> slots_;
-    uint64_t next_serial_ = 1;
     Timing timing_;
 };
 

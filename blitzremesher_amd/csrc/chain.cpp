@@ -14,10 +14,14 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <condition_variable>
+#include <exception>
 #include <limits>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
+#include <thread>
 
 #include "actions.hpp"
 #include "digest.hpp"
@@ -144,6 +148,83 @@ Settings eval_config(const ChainSettings& s, const ScheduleEntry& step, unsigned
     return e;
 }
 
+// Runs a fixed number of tasks at once: task 0 on the calling thread, task
+// i > 0 on worker i - 1. The four gates of an executor trial are independent
+// audits (each a pure function of its reference, candidate and settings), so
+// they run side by side on four audit objects with their own HIP streams;
+// their verdicts are then read in the reference's order.
+class TaskPool {
+  public:
+    explicit TaskPool(size_t workers) {
+        for (size_t w = 0; w < workers; ++w) threads_.emplace_back([this, w] { loop(w); });
+    }
+    ~TaskPool() {
+        {
+            std::lock_guard<std::mutex> lock(m_);
+            stop_ = true;
+        }
+        wake_.notify_all();
+        for (auto& t : threads_) t.join();
+    }
+    TaskPool(const TaskPool&) = delete;
+    TaskPool& operator=(const TaskPool&) = delete;
+
+    void run(std::vector<std::function<void()>>& tasks) {
+        errors_.assign(tasks.size(), nullptr);
+        {
+            std::lock_guard<std::mutex> lock(m_);
+            tasks_ = &tasks;
+            pending_ = threads_.size();
+            ++generation_;
+        }
+        wake_.notify_all();
+        try {
+            if (!tasks.empty()) tasks[0]();
+        } catch (...) {
+            errors_[0] = std::current_exception();
+        }
+        {
+            std::unique_lock<std::mutex> lock(m_);
+            done_.wait(lock, [&] { return pending_ == 0; });
+            tasks_ = nullptr;
+        }
+        for (auto& e : errors_)
+            if (e) std::rethrow_exception(e);
+    }
+
+  private:
+    void loop(size_t w) {
+        uint64_t seen = 0;
+        for (;;) {
+            std::function<void()>* task = nullptr;
+            {
+                std::unique_lock<std::mutex> lock(m_);
+                wake_.wait(lock, [&] { return stop_ || generation_ != seen; });
+                if (stop_) return;
+                seen = generation_;
+                if (w + 1 < tasks_->size()) task = &(*tasks_)[w + 1];
+            }
+            if (task) try {
+                    (*task)();
+                } catch (...) {
+                    errors_[w + 1] = std::current_exception();
+                }
+            {
+                std::lock_guard<std::mutex> lock(m_);
+                if (--pending_ == 0) done_.notify_one();
+            }
+        }
+    }
+    std::vector<std::thread> threads_;
+    std::mutex m_;
+    std::condition_variable wake_, done_;
+    std::vector<std::function<void()>>* tasks_ = nullptr;
+    std::vector<std::exception_ptr> errors_;
+    uint64_t generation_ = 0;
+    size_t pending_ = 0;
+    bool stop_ = false;
+};
+
 // neural::audit_measurement_nonfinite (src/neural/audit_measurement.hpp:8-13).
 bool nonfinite(const Measurement& m) {
     return std::isnan(m.error) || std::isnan(m.coverage) || std::isnan(m.coverage_upper) ||
@@ -155,7 +236,13 @@ class ChainSearch {
   public:
     ChainSearch(Audit& audit, const HostMesh& source, const float* uv, const ChainSettings& s, const ChainOptions& o,
                 Scorer* scorer)
-        : audit_(audit), source_(source), uv_(uv), s_(s), o_(o), scorer_(scorer) {}
+        : audit_(audit), source_(source), uv_(uv), s_(s), o_(o), scorer_(scorer), pool_(3) {
+        gate_audit_[0] = &audit_;
+        for (int g = 1; g < 4; ++g) {
+            gate_owned_[g - 1] = std::make_unique<Audit>(audit.device(), audit.memory_bytes(), audit.cache_rasters());
+            gate_audit_[g] = gate_owned_[g - 1].get();
+        }
+    }
 
     ChainResult run();
 
@@ -183,6 +270,11 @@ class ChainSearch {
     std::shared_ptr<const DeviceMesh> source_dev_;
     std::vector<std::shared_ptr<Node>> finalists_;
     std::vector<uint64_t> finalist_storage_;
+    ExecutorCache executor_;
+    // One audit object per executor gate (the first is the caller's).
+    std::unique_ptr<Audit> gate_owned_[3];
+    Audit* gate_audit_[4] = {};
+    TaskPool pool_;
     // Per level: the four settings (source search, adjacent search, source
     // audit, adjacent audit).
     Settings gates_[4];
@@ -221,16 +313,36 @@ Measurement ChainSearch::measure(const DeviceMesh& reference, const DeviceMesh& 
 // search settings, then under the audit settings.
 ChainLod ChainSearch::propose(const std::shared_ptr<Node>& parent, size_t target, unsigned level) {
     const auto previous = device_of(parent);
+    // The four certifications of a trial in order: source search, adjacent
+    // search, source audit, adjacent audit, each verdict-only with the
+    // bounded refinement; cancellation is polled once per trial.
+    Settings bounded[4];
+    bool clamped[4];
+    for (int g = 0; g < 4; ++g) {
+        bounded[g] = gates_[g];
+        bounded[g].max_supersample = bounded_refinement(gates_[g]);
+        clamped[g] = bounded[g].max_supersample < gates_[g].max_supersample;
+        bounded[g].verdict_only = true;
+        bounded[g].cancelled = nullptr;
+    }
+    const DeviceMesh* refs[4] = {source_dev_.get(), previous.get(), source_dev_.get(), previous.get()};
     const auto gate = [&](const std::vector<uint32_t>& idx, const std::vector<uint16_t>& mats) {
         if (s_.cancelled && s_.cancelled()) return false;
         const auto t = Clock::now();
         ChainLod trial{idx, mats};
         const auto candidate = derive(trial);
+        bool passed[4];
+        std::vector<std::function<void()>> tasks;
+        for (int g = 0; g < 4; ++g)
+            tasks.push_back([&, g] {
+                passed[g] = gate_audit_[g]->evaluate(*refs[g], *candidate, bounds_, bounded[g]).passed;
+            });
+        pool_.run(tasks);
         bool pass = true;
-        const DeviceMesh* refs[4] = {source_dev_.get(), previous.get(), source_dev_.get(), previous.get()};
         for (int g = 0; g < 4 && pass; ++g) {
             ++r_.gate_audits;
-            pass = measure(*refs[g], *candidate, gates_[g], true).passed;
+            r_.bounded_audits += clamped[g];
+            pass = passed[g];
         }
         r_.gate_ms += since(t);
         return pass;
@@ -239,7 +351,7 @@ ChainLod ChainSearch::propose(const std::shared_ptr<Node>& parent, size_t target
     ChainLod out;
     const auto t = Clock::now();
     out.indices = execute_actions(audit_.device(), source_, uv_, target, o_.action_trials, o_.ranking, o_.action_batch,
-                                  gate, st, &out.materials, scorer_, &bounds_);
+                                  gate, st, &out.materials, scorer_, &bounds_, &executor_);
     r_.executor_ms += since(t);
     ++r_.decoded;
     r_.legal_collapses += st.accepted;
