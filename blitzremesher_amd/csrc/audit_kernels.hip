@@ -294,7 +294,7 @@ __global__ __launch_bounds__(1024) void row_distance_kernel(const ViewPair* __re
                                                             const ViewSummary* __restrict__ summaries) {
     const uint32_t job = blockIdx.y;  // view*2 + direction
     const ViewPair& p = pairs[job >> 1];
-    if (!dense_rows(summaries[job >> 1], p.size)) return;
+    if (!p.dist_a || !dense_rows(summaries[job >> 1], p.size)) return;
     const uint64_t* mask = (job & 1) ? p.a : p.c;
     uint16_t* out = (job & 1) ? p.dist_a : p.dist_c;
     const uint32_t y = blockIdx.x;
@@ -406,7 +406,8 @@ __global__ __launch_bounds__(256) void hausdorff_kernel(const ViewPair* __restri
     const uint32_t exclusive = inclusive - count;
     const uint32_t total = uint32_t(__shfl(int(inclusive), 63));
     const uint32_t size = p.size;
-    const bool dense = dense_rows(s, size);
+    // Without row-distance planes every view searches the mask directly.
+    const bool dense = p.dist_a != nullptr && dense_rows(s, size);
     // Wave-uniform copy of the maximum found so far (refreshed every round).
     uint32_t known = uint32_t(__shfl(int(__hip_atomic_load(target, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)), 0));
     // Verdict-only audits: once the maximum reaches fail_sq the view fails
@@ -545,6 +546,15 @@ void launch_distance(const ViewPair* pairs, ViewSummary* summaries, uint32_t cou
     if (!count) return;
     const uint32_t threads = std::max<uint32_t>(64, (tiles + 63) / 64 * 64);
     row_distance_kernel<<<dim3(size, count * 2), dim3(threads), 0, stream>>>(pairs, summaries);
+    const size_t words = size_t(tiles) * tiles;
+    const uint32_t per_block = 4 * hausdorff_words(tiles);
+    hausdorff_kernel<<<dim3(uint32_t((words + per_block - 1) / per_block), count * 2), dim3(256), 0, stream>>>(
+        pairs, summaries);
+}
+
+void launch_hausdorff(const ViewPair* pairs, ViewSummary* summaries, uint32_t count, uint32_t tiles,
+                      hipStream_t stream) {
+    if (!count) return;
     const size_t words = size_t(tiles) * tiles;
     const uint32_t per_block = 4 * hausdorff_words(tiles);
     hausdorff_kernel<<<dim3(uint32_t((words + per_block - 1) / per_block), count * 2), dim3(256), 0, stream>>>(

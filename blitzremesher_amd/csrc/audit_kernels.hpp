@@ -23,4 +23,7 @@ void launch_compare(const ViewPair* pairs, ViewSummary* summaries, uint32_t coun
                     hipStream_t stream);
 void launch_distance(const ViewPair* pairs, ViewSummary* summaries, uint32_t count, uint32_t size, uint32_t tiles,
                      hipStream_t stream);
+// The directed Hausdorff pass alone (pairs without row-distance planes).
+void launch_hausdorff(const ViewPair* pairs, ViewSummary* summaries, uint32_t count, uint32_t tiles,
+                      hipStream_t stream);
 }  // namespace blitz_hip

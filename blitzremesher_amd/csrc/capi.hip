@@ -15,6 +15,7 @@
 #include "actions.hpp"
 #include "chain.hpp"
 #include "digest.hpp"
+#include "noalloc.hpp"
 #include "audit.hpp"
 #include "patch_audit.hpp"
 #include "patches.hpp"
@@ -765,6 +766,14 @@ blitz_status blitz_hip_coverage_distance(blitz_hip_audit* a, const uint64_t* x, 
         const size_t n = (size_t(size) * size + 63) / 64;
         std::vector<uint64_t> va(x, x + n), vb(y, y + n);
         *out = audit_of(a).coverage_distance(va, vb, size, ss);
+    });
+}
+
+blitz_status blitz_hip_coverage_distance_noalloc(int32_t device, const uint64_t* a, const uint64_t* b,
+                                                 uint32_t size, uint8_t supersample, double* out, void* stream,
+                                                 char* error, size_t capacity) {
+    return guarded(error, capacity, [&] {
+        coverage_distance_noalloc(device, a, b, size, supersample, out, static_cast<hipStream_t>(stream));
     });
 }
 

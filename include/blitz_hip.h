@@ -433,6 +433,20 @@ blitz_status blitz_hip_coverage_distance(blitz_hip_audit*, const uint64_t* a, co
                                          uint32_t size, uint8_t supersample, double* out,
                                          char* error, size_t capacity);
 
+/* No counterpart in the reference. For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): blitz_hip_coverage_distance on masks already in device
+ * memory, with every temporary in one fixed array inside the library (16 MiB
+ * per GPU, held until the process ends). `a` and `b` are device pointers to
+ * row-major packed masks (bit y*size + x, (size*size + 63) / 64 words each);
+ * the distance is written to the device double `out`. Work is queued on
+ * `stream` (a hipStream_t; null for the default stream) and the call returns
+ * without waiting. size <= 8192. Calls that overlap on one GPU share the
+ * array and race; calls on one stream are ordered. */
+blitz_status blitz_hip_coverage_distance_noalloc(int32_t device, const uint64_t* a, const uint64_t* b,
+                                                 uint32_t size, uint8_t supersample, double* out, void* stream,
+                                                 char* error, size_t capacity);
+
 blitz_status blitz_hip_audit_last_timing(const blitz_hip_audit*, blitz_hip_audit_timing*);
 
 /* ---- classical proposer --------------------------------------------------- */
