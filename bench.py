@@ -166,6 +166,7 @@ def scorer_leg(steps: int) -> dict:
             s.train_step(g, ids, labels, targets, settings)
             ms.append(s.last_step_ms())
         t = float(np.median(ms))
+        out["error_vs_fp64"] = scorer_error()
         out["train_bf16"] = {"batch": batch, "ms_per_step": round(t, 4), "edges_per_s": round(batch / (t * 1e-3), 1),
                              "note": "full-graph forward (all 100,000 vertices), batch edge layer, FP32 backward, "
                                      "AdamW"}
@@ -483,6 +484,21 @@ def copy_probe(mib: int) -> int:
     print(json.dumps({"copy_bytes": mib << 20, "ms": ms.value,
                       "gbs": round(2 * (mib << 20) / (ms.value * 1e-3) / 1e9, 1)}), flush=True)
     return 0
+
+
+def scorer_error() -> dict | None:
+    """BF16 / FP32 forward error of the 100,000-vertex torus against the FP64
+    oracle, from the committed summary of the GPU test that measures it."""
+    path = ROOT / "profiles" / "scorer_torus100k_error.md"
+    if not path.exists():
+        return None
+    out = {}
+    for line in path.read_text().splitlines():
+        for key in ("bf16_max_rel", "bf16_rms_rel", "fp32_max_rel"):
+            if line.startswith(key + ":"):
+                out[key] = float(line.split(":", 1)[1])
+    out["source"] = "profiles/scorer_torus100k_error.md (tests/test_configs_gpu.py, all 600,000 actions)"
+    return out
 
 
 def measured_traffic() -> int | None:

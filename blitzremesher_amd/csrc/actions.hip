@@ -23,6 +23,7 @@
 // order, testing and claiming a footprint (both endpoints and their
 // neighbours) 64 neighbour slots at a time.
 #include "actions.hpp"
+#include "vec_device.hpp"
 
 #include <algorithm>
 #include <chrono>
@@ -184,14 +185,6 @@ struct F3 {
     float x, y, z;
 };
 // include/blitz/mesh.hpp:22-26
-__device__ __forceinline__ F3 sub3(float4 a, float4 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
-__device__ __forceinline__ F3 cross3(F3 a, F3 b) {
-    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
-}
-__device__ __forceinline__ double dot3(F3 a, F3 b) {
-    return double(a.x) * b.x + double(a.y) * b.y + double(a.z) * b.z;
-}
-__device__ __forceinline__ double len3(F3 a) { return sqrt(dot3(a, a)); }
 
 __device__ __forceinline__ uint16_t material_of(const Topo& t, uint32_t f) {
     return t.materials ? t.materials[f] : uint16_t(0);
@@ -526,25 +519,9 @@ __global__ void rank_kernel(Topo t, RankInput in) {
     const double scale = fmax(1e-20, in.diameter);
     float score = 0.f;
     if (in.mode == 3) {
-        const F3 d = sub3(t.positions[a.to], t.positions[a.from]);
-        const double s = 1 / scale;
-        const F3 e{float(d.x * s), float(d.y * s), float(d.z * s)};
-        score = -float(len3(e));
+        score = shortest_edge_score(t.positions, a.from, a.to, scale);
     } else if (in.mode == 4) {
-        double cost = 0;
-        for (uint32_t k = t.face_offsets[a.from]; k < t.face_offsets[a.from + 1]; ++k) {
-            const uint32_t f = t.faces[k];
-            const float4 p = t.positions[t.indices[3 * f]], q = t.positions[t.indices[3 * f + 1]],
-                         r = t.positions[t.indices[3 * f + 2]];
-            const F3 normal = cross3(sub3(q, p), sub3(r, p));
-            const double weight = len3(normal);
-            const double l = len3(normal);
-            const F3 unit = l > 0 ? F3{float(normal.x * (1 / l)), float(normal.y * (1 / l)), float(normal.z * (1 / l))}
-                                  : F3{0, 0, 0};
-            const double distance = dot3(unit, sub3(t.positions[a.to], p)) / scale;
-            cost += distance * distance * weight / (scale * scale);
-        }
-        score = float(-cost);
+        score = current_plane_score(t.positions, t.indices, t.face_offsets, t.faces, a.from, a.to, scale);
     } else if (in.mode == 0) {
         uint32_t lo = in.graph_offsets[a.from], hi = in.graph_offsets[a.from + 1];
         const uint32_t end = hi;

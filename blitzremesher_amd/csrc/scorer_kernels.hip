@@ -39,6 +39,7 @@
 #include <stdexcept>
 
 #include "scorer_kernels.hpp"
+#include "vec_device.hpp"
 
 namespace blitz_hip {
 namespace {
@@ -412,31 +413,12 @@ __global__ void score_kernel(ScoreInput in) {
     } else if (in.mode == 3) {
         score = 0.f;  // constant ranking: action order
     } else {
+        // The reference's Vec3 arithmetic (float differences and cross
+        // products, double dot products), as the action state ranks.
+        const double scale = fmax(1e-20, in.scale);
         const uint32_t u = in.src[a], v = in.dst[a];
-        const float4 pu = in.positions[u], pv = in.positions[v];
-        if (in.mode == 1) {
-            const double dx = (double(pv.x) - pu.x) / in.scale, dy = (double(pv.y) - pu.y) / in.scale,
-                         dz = (double(pv.z) - pu.z) / in.scale;
-            score = -float(sqrt(dx * dx + dy * dy + dz * dz));
-        } else {
-            const double scale = fmax(1e-20, in.scale);
-            double cost = 0;
-            for (uint32_t k = in.face_offsets[u]; k < in.face_offsets[u + 1]; ++k) {
-                const uint32_t f = in.faces[k];
-                const float4 p = in.positions[in.indices[3 * f]], q = in.positions[in.indices[3 * f + 1]],
-                             r = in.positions[in.indices[3 * f + 2]];
-                const double e1[3] = {double(q.x) - p.x, double(q.y) - p.y, double(q.z) - p.z};
-                const double e2[3] = {double(r.x) - p.x, double(r.y) - p.y, double(r.z) - p.z};
-                const double n[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2],
-                                     e1[0] * e2[1] - e1[1] * e2[0]};
-                const double weight = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
-                if (!(weight > 0)) continue;
-                const double d = (n[0] * (double(pv.x) - p.x) + n[1] * (double(pv.y) - p.y) +
-                                  n[2] * (double(pv.z) - p.z)) / weight / scale;
-                cost += d * d * weight / (scale * scale);
-            }
-            score = float(-cost);
-        }
+        score = in.mode == 1 ? shortest_edge_score(in.positions, u, v, scale)
+                             : current_plane_score(in.positions, in.indices, in.face_offsets, in.faces, u, v, scale);
     }
     // -0.0 sorts as +0.0 so equal scores keep ascending action order.
     in.keys[a] = score + 0.f;

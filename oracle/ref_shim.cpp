@@ -12,6 +12,7 @@
 #include "chain_hooks.hpp"
 #include "coverage.hpp"
 #include "neural/action.hpp"
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -338,6 +339,35 @@ int64_t ref_actions_trial(void* h, const uint32_t* rows, uint64_t n, uint32_t* i
         if (lod.data.indices.size() > capacity) return -1;
         std::memcpy(indices, lod.data.indices.data(), lod.data.indices.size() * 4);
         return int64_t(lod.data.indices.size());
+    } catch (...) {
+        return -1;
+    }
+}
+
+// Ranking scores of the legal rows in enumeration order: 3 shortest edge
+// (-x[59] of the row), 4 current plane (-teacher_cost), as
+// tests/neural_action_gpu.cpp:155-160 ranks them.
+int64_t ref_actions_scores(void* h, int mode, float* out) {
+    try {
+        auto* r = static_cast<RefActions*>(h);
+        for (size_t i = 0; i < r->rows.size(); ++i)
+            out[i] = mode == 3 ? -r->rows[i].x[59] : float(-r->state->teacher_cost(r->rows[i].action));
+        return int64_t(r->rows.size());
+    } catch (...) {
+        return -1;
+    }
+}
+// The 80 action features of every legal row (ActionRecord::x,
+// src/neural/action.cpp:371-450) with the 8 conditions `c`.
+int64_t ref_actions_features(void* h, const float* c, float* out) {
+    try {
+        auto* r = static_cast<RefActions*>(h);
+        std::array<float, neural::conditions> cond{};
+        for (size_t i = 0; i < cond.size(); ++i) cond[i] = c[i];
+        auto rows = r->state->actions(cond);
+        for (size_t i = 0; i < rows.size(); ++i)
+            std::memcpy(out + i * neural::action_features, rows[i].x.data(), neural::action_features * 4);
+        return int64_t(rows.size());
     } catch (...) {
         return -1;
     }

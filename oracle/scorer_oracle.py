@@ -220,6 +220,47 @@ def forward_fp32(x: np.ndarray, offsets, neighbors, acts, weights: np.ndarray) -
     return _compensated_linear(g, W2, b2, False)
 
 
+def receptive_field(offsets, neighbors, vertices: np.ndarray) -> list[np.ndarray]:
+    """Vertex sets needed by a forward pass that only has to produce the
+    final embeddings of `vertices`: entry L holds the vertices whose layer
+    input is needed L hops out (entry 0 = `vertices`)."""
+    sets = [np.unique(vertices)]
+    for _ in range(LAYERS):
+        v = sets[-1]
+        nb = np.concatenate([neighbors[offsets[i]:offsets[i + 1]] for i in v]) if len(v) else v
+        sets.append(np.union1d(v, nb))
+    return sets
+
+
+def forward_fp32_subset(x: np.ndarray, offsets, neighbors, acts, weights: np.ndarray) -> np.ndarray:
+    """forward_fp32 for the actions `acts` only: each message-passing layer is
+    evaluated on the rows its successors need (the receptive field of the
+    actions' endpoints). Every row's arithmetic is the one forward_fp32
+    performs, so the outputs are the same bits."""
+    f32 = np.float32
+    layers = split_weights(weights.astype(f32))
+    sets = receptive_field(offsets, neighbors, acts.reshape(-1))
+    deg = np.diff(offsets)
+    h = x.astype(f32).copy()
+    for i, (W, b) in enumerate(layers[:LAYERS]):
+        rows = sets[LAYERS - 1 - i]
+        agg = np.zeros((len(rows), h.shape[1]), f32)
+        d = deg[rows]
+        for slot in range(int(d.max(initial=0))):
+            sel = np.nonzero(d > slot)[0]
+            agg[sel] = agg[sel] + h[neighbors[offsets[rows[sel]] + slot]]
+        agg = agg / np.maximum(d, 1).astype(f32)[:, None]
+        z = np.concatenate([h[rows], agg], 1)
+        out = _compensated_linear(z, W, b, True)
+        h = np.zeros((len(deg), out.shape[1]), f32)
+        h[rows] = out
+    W1, b1 = layers[3]
+    W2, b2 = layers[4]
+    z = np.concatenate([h[acts[:, 0]], h[acts[:, 1]]], 1)
+    g = _compensated_linear(z, W1, b1, True)
+    return _compensated_linear(g, W2, b2, False)
+
+
 def rank(logits: np.ndarray) -> np.ndarray:
     """Descending score order, ties by ascending action index."""
     return np.lexsort((np.arange(len(logits)), -logits.astype(np.float64)))
@@ -332,38 +373,55 @@ def teacher_placements(positions: np.ndarray, indices: np.ndarray, face_offsets,
     return out, tgt
 
 
+def _cross32(a: np.ndarray, b: np.ndarray) -> np.ndarray:
+    """Vec3 cross product in float32, every product and difference rounded."""
+    f = np.float32
+    return np.stack([f(a[..., 1] * b[..., 2]) - f(a[..., 2] * b[..., 1]),
+                     f(a[..., 2] * b[..., 0]) - f(a[..., 0] * b[..., 2]),
+                     f(a[..., 0] * b[..., 1]) - f(a[..., 1] * b[..., 0])], -1).astype(np.float32)
+
+
+def _dot64(a: np.ndarray, b: np.ndarray) -> np.ndarray:
+    """Vec3 dot product in double, summed x, y, z in order."""
+    a, b = a.astype(np.float64), b.astype(np.float64)
+    return (a[..., 0] * b[..., 0] + a[..., 1] * b[..., 1]) + a[..., 2] * b[..., 2]
+
+
 def scores(positions: np.ndarray, indices: np.ndarray, acts: np.ndarray, mode: int, outputs=None,
            diameter: float = 1.0) -> np.ndarray:
-    """Ranking scores of src/neural/action_gpu.cu:809-844 as float32: 0 learned
-    logit, 1 constant, 3 -|dst - src| / diameter, 4 current plane: minus the
-    sum over faces around src of (plane distance of dst / diameter)^2 times
-    the face's |cross| / diameter^2, FP64 in face order."""
+    """Ranking scores of src/neural/action_gpu.cu:809-844 as float32, in the
+    reference's Vec3 arithmetic (float differences and cross products, double
+    dot products and lengths; CPU ActionState::teacher_cost and the x[59]
+    edge length of src/neural/action.cpp:355-395): 0 learned logit,
+    1 constant, 3 -|dst - src| / diameter, 4 current plane: minus the sum over
+    faces around src of (plane distance of dst / diameter)^2 times the face's
+    |cross| / diameter^2, accumulated in face order."""
     n = len(acts)
     if mode == 0:
         return outputs[:, 0].astype(np.float32) + np.float32(0)
     if mode == 1:
         return np.zeros(n, np.float32)
-    p = positions.astype(np.float64)
+    p = positions.astype(np.float32)
+    scale = max(1e-20, diameter)
+    src, dst = acts[:, 0].astype(np.int64), acts[:, 1].astype(np.int64)
     if mode == 3:
-        d = (p[acts[:, 1]] - p[acts[:, 0]]) / diameter
-        return -np.sqrt((d * d).sum(1)).astype(np.float32)
+        d = (p[dst] - p[src]).astype(np.float32)
+        e = (d.astype(np.float64) * (1 / scale)).astype(np.float32)
+        return (-np.sqrt(_dot64(e, e)).astype(np.float32)) + np.float32(0)
     f = indices.astype(np.int64)
     _, _, face_offsets, faces, _ = topology(indices, len(positions))
-    scale = max(1e-20, diameter)
-    out = np.zeros(n, np.float32)
-    for a in range(n):
-        u, v = acts[a]
-        cost = 0.0
-        for fi in faces[face_offsets[u]:face_offsets[u + 1]]:
-            P, Q, R = p[f[fi, 0]], p[f[fi, 1]], p[f[fi, 2]]
-            e1, e2 = Q - P, R - P
-            nrm = np.array([e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2],
-                            e1[0] * e2[1] - e1[1] * e2[0]])
-            w = np.sqrt(nrm[0] * nrm[0] + nrm[1] * nrm[1] + nrm[2] * nrm[2])
-            if not w > 0:
-                continue
-            q = p[v] - P
-            dist = (nrm[0] * q[0] + nrm[1] * q[1] + nrm[2] * q[2]) / w / scale
-            cost += dist * dist * w / (scale * scale)
-        out[a] = np.float32(-cost) + np.float32(0)
-    return out
+    count = face_offsets[src + 1] - face_offsets[src]
+    cost = np.zeros(n)
+    for slot in range(int(count.max(initial=0))):
+        a = np.nonzero(count > slot)[0]
+        fi = faces[face_offsets[src[a]] + slot]
+        P, Q, R = p[f[fi, 0]], p[f[fi, 1]], p[f[fi, 2]]
+        normal = _cross32((Q - P).astype(np.float32), (R - P).astype(np.float32))
+        weight = np.sqrt(_dot64(normal, normal))
+        with np.errstate(divide="ignore", invalid="ignore"):
+            inv = np.where(weight > 0, 1 / np.where(weight > 0, weight, 1), 0.0)
+        unit = (normal.astype(np.float64) * inv[:, None]).astype(np.float32)
+        unit[~(weight > 0)] = 0
+        distance = _dot64(unit, (p[dst[a]] - P).astype(np.float32)) / scale
+        cost[a] += distance * distance * weight / (scale * scale)
+    return (-cost).astype(np.float32) + np.float32(0)

@@ -390,6 +390,29 @@ class Actions:
             raise ValueError("independent rejected its input")
         return out[:n].copy()
 
+    def ref_scores(self, ranking: int) -> np.ndarray:
+        """The reference build's own scores of the legal rows (3 shortest edge,
+        4 current plane)."""
+        assert self.oracle.kind == "ref"
+        out = np.zeros(max(1, len(self.legal)), np.float32)
+        f = self.oracle.fn("actions_scores")
+        f.restype = C.c_int64
+        f.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_float)]
+        assert f(self.handle, ranking, out.ctypes.data_as(C.POINTER(C.c_float))) == len(self.legal)
+        return out[: len(self.legal)]
+
+    def ref_features(self, conditions=None) -> np.ndarray:
+        """The reference build's 80 action features of every legal row."""
+        assert self.oracle.kind == "ref"
+        c = np.zeros(8, np.float32) if conditions is None else np.asarray(conditions, np.float32)
+        out = np.zeros((max(1, len(self.legal)), 80), np.float32)
+        f = self.oracle.fn("actions_features")
+        f.restype = C.c_int64
+        f.argtypes = [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float)]
+        assert f(self.handle, c.ctypes.data_as(C.POINTER(C.c_float)),
+                 out.ctypes.data_as(C.POINTER(C.c_float))) == len(self.legal)
+        return out[: len(self.legal)]
+
     def scores(self, ranking: int, diameter: float) -> np.ndarray:
         assert self.oracle.kind == "oracle"
         out = np.zeros(max(1, len(self.legal)), np.float32)

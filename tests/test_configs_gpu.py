@@ -1,0 +1,130 @@
+"""The benchmark configurations at their own sizes on the GPU.
+
+Raster audit: the 512 x 512 fBm heightfield (262,144 V / 522,242 F) against
+its 25 %-collapsed copy over 64 orthographic views at 1024 x 1024 samples, at
+limit 1e3 (every view measured) and 1 px (the decision), field for field
+against the CPU oracle; and 64 of its 4,096 local patches, audited in one
+batch on the GPU, against the oracle's full audit of each patched mesh (the
+oracle runs in worker processes, one audit each).
+
+Scorer: the noisy 500 x 200 torus (100,000 V / 600,000 directed actions):
+features bit-exact; the FP32 forward bit-exact on 1,024 sampled actions
+(oracle evaluated on their receptive field); the BF16 forward within its
+stated tolerance of the FP64 oracle on every action, max and RMS printed.
+"""
+from __future__ import annotations
+
+import multiprocessing as mp
+import os
+import sys
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+sys.path.insert(0, str(Path(__file__).resolve().parent.parent / "oracle"))
+import scorer_oracle as so  # noqa: E402
+from blitzremesher_amd import EvalSettings, Mesh, meshes  # noqa: E402
+from blitzremesher_amd.audit import bounds, local_patches, reduce_reuse  # noqa: E402
+from oracle_lib import Oracle  # noqa: E402
+
+# BF16 forward error against FP64, relative to the output scale (max |FP64|):
+# bound on the largest and on the root-mean-square difference over all
+# actions of the 100,000-vertex torus.
+BF16_MAX_TOLERANCE = 2e-2
+BF16_RMS_TOLERANCE = 5e-3
+
+
+def heightfield_case():
+    ref = meshes.heightfield(512, 0.1, 2)
+    cand = reduce_reuse(ref, int(round(ref.triangles * 0.75)))
+    center, radius = bounds(ref)
+    settings = EvalSettings(profile=0, orthographic=64, perspective=0, rotation_seed=0xB1172026, screen_size=120,
+                            supersample=8, max_supersample=8, limit=1e3, max_changed_area=1.0, force_two_sided=True)
+    return ref, cand, center, radius, settings
+
+
+_worker = {}
+
+
+def _init(positions, indices, center, radius, settings):
+    _worker.update(positions=positions, indices=indices, center=center, radius=radius, settings=settings)
+
+
+def _oracle_audit(indices: np.ndarray) -> tuple:
+    w = _worker
+    ref = Mesh(w["positions"], w["indices"])
+    return Oracle("oracle").evaluate(ref, Mesh(w["positions"], indices), w["center"], w["radius"],
+                                     w["settings"]).as_tuple()
+
+
+def workers() -> int:
+    return max(1, min(16, len(os.sched_getaffinity(0))))
+
+
+@pytest.mark.gpu
+def test_gpu_heightfield_audit_matches_oracle_at_full_size():
+    from blitzremesher_amd import Audit
+    ref, cand, center, radius, settings = heightfield_case()
+    o = Oracle("oracle")
+    with Audit(0) as audit:
+        dref, dcand = audit.upload(ref), audit.upload(cand)
+        for limit in (1e3, 1.0):
+            s = EvalSettings(**{**settings.__dict__, "limit": limit})
+            got = audit.evaluate(dref, dcand, center, radius, s)
+            want = o.evaluate(ref, cand, center, radius, s)
+            print(f"limit {limit}: {got}", flush=True)
+            assert got.as_tuple() == want.as_tuple(), (limit, got, want)
+            assert got.views_evaluated == (64 if limit > 1 else got.views_evaluated)
+
+
+@pytest.mark.gpu
+def test_gpu_heightfield_patches_match_oracle_at_full_size():
+    from blitzremesher_amd import Audit
+    ref, _, center, radius, settings = heightfield_case()
+    s = EvalSettings(**{**settings.__dict__, "limit": 1.0})
+    patches = local_patches(ref, 4096, 0xB1172026, 16, 200)
+    with Audit(0) as audit:
+        binding = audit.bind(ref, center, radius, s)
+        got, fallbacks = audit.evaluate_patches(binding, patches, s)
+    assert len(got) == 4096
+    sample = np.random.default_rng(7).choice(4096, 64, replace=False)
+    candidates = [patches.candidate(ref, int(i)).indices for i in sample]
+    ctx = mp.get_context("spawn")
+    with ctx.Pool(workers(), initializer=_init, initargs=(ref.positions, ref.indices, center, radius, s)) as pool:
+        want = pool.map(_oracle_audit, candidates)
+    passed = 0
+    for i, w in zip(sample, want):
+        assert got[i].as_tuple() == w, (int(i), got[i], w)
+        passed += got[i].passed
+    print(f"64 patches equal the oracle ({passed} pass, {fallbacks} full-path fallbacks of 4096)", flush=True)
+
+
+@pytest.mark.gpu
+def test_gpu_torus_scorer_at_full_size():
+    from blitzremesher_amd.scorer import BF16, FP32, Scorer, he_weights
+    mesh = meshes.torus(1.0, 0.3, 500, 200, 0.01, 3)
+    c, r = bounds(mesh)
+    w = he_weights(4)
+    offsets, neighbors, *_ = so.topology(mesh.indices, len(mesh.positions))
+    acts = so.actions(offsets, neighbors)
+    assert len(mesh.positions) == 100_000 and len(acts) == 600_000
+    with Scorer(w) as s:
+        g = s.graph(mesh, c, r)
+        x = g.features()
+        assert np.array_equal(x, so.features(mesh.positions, mesh.indices, c, 2 * r).astype(np.float32))
+        assert np.array_equal(g.actions_array(), acts.astype(np.uint32))
+        f32 = s.forward(g, FP32)
+        bf = s.forward(g, BF16)
+    pick = np.random.default_rng(11).choice(len(acts), 1024, replace=False)
+    want32 = so.forward_fp32_subset(x, offsets, neighbors, acts[pick], w)
+    assert np.array_equal(f32[pick], want32)
+    ref = so.forward_fp64(x, offsets, neighbors, acts, w)
+    scale = max(1.0, float(np.abs(ref).max()))
+    err_max = float(np.abs(bf - ref).max()) / scale
+    err_rms = float(np.sqrt(((bf - ref) ** 2).mean())) / scale
+    e32 = float(np.abs(f32 - ref).max()) / scale
+    print(f"torus 100k: BF16 vs FP64 max {err_max:.3e} rms {err_rms:.3e} (of output scale {scale:.3f}); "
+          f"FP32 vs FP64 max {e32:.3e}", flush=True)
+    assert err_max <= BF16_MAX_TOLERANCE and err_rms <= BF16_RMS_TOLERANCE
+    assert e32 <= 1e-5

@@ -19,8 +19,12 @@ ROOT = Path(__file__).resolve().parent.parent
 sys.path.insert(0, str(ROOT / "oracle"))
 
 import scorer_oracle as so  # noqa: E402
+from oracle_lib import available  # noqa: E402
 from blitzremesher_amd import meshes  # noqa: E402
 from blitzremesher_amd.audit import Mesh, bounds  # noqa: E402
+
+
+needs_ref = pytest.mark.skipif(not available("ref"), reason="reference CPU build (oracle/_ref) not present")
 
 
 def tiny_graph():
@@ -118,6 +122,72 @@ def test_features_closed_form_on_flat_grid():
     assert x[centre, 17] == 6 / 16 and x[centre, 18] == 0
 
 
+def bipyramid(k: int, r: float, h: float) -> Mesh:
+    """A regular k-gon of radius r in z = 0 (vertices 0..k-1, counter-clockwise
+    from above) joined to apexes (0, 0, h) (vertex k) and (0, 0, -h) (vertex
+    k+1); closed, faces wound outward."""
+    t = 2 * np.pi * np.arange(k) / k
+    p = np.zeros((k + 2, 3))
+    p[:k, 0], p[:k, 1] = r * np.cos(t), r * np.sin(t)
+    p[k, 2], p[k + 1, 2] = h, -h
+    f = [(k, i, (i + 1) % k) for i in range(k)] + [(k + 1, (i + 1) % k, i) for i in range(k)]
+    return Mesh(p.astype(np.float32), np.array(f, np.uint32))
+
+
+@pytest.mark.parametrize("k,r,h", [(6, 1.0, 0.5), (8, 1.0, 0.25), (10, 0.75, 1.5), (12, 2.0, 0.1)])
+def test_features_closed_form_on_curved_apexes(k, r, h):
+    # Each apex sits h above (below) a regular ring of radius r: its
+    # neighbour mean is the ring centre, so lap = (0, 0, -+h) / diameter, the
+    # area-weighted normal is (0, 0, +-1), every incident edge has length
+    # sqrt(r^2 + h^2), and every incident face is tilted by alpha with
+    # tan(alpha) = h / (r cos(pi/k)) and has |cross| = 2 r sin(pi/k)
+    # sqrt(r^2 cos^2(pi/k) + h^2). The bounds are centred at the origin with
+    # radius max(r, h).
+    m = bipyramid(k, r, h)
+    c, radius = bounds(m)
+    assert np.allclose(c, 0, atol=1e-6) and abs(radius - max(r, h)) < 1e-6 * max(r, h)
+    diameter = 2 * radius
+    x = so.features(m.positions, m.indices, c, diameter)
+    slant = np.sqrt(r * r + h * h)
+    mean_edge = slant / diameter
+    alpha = np.arctan2(h, r * np.cos(np.pi / k))
+    total = k * 2 * r * np.sin(np.pi / k) * np.sqrt((r * np.cos(np.pi / k)) ** 2 + h * h) / diameter ** 2
+    for apex, up in ((k, 1.0), (k + 1, -1.0)):
+        a = x[apex]
+        assert np.allclose(a[3:6], [0, 0, up], atol=1e-6)
+        assert a[16] == pytest.approx(-h / slant, rel=1e-6)                     # curvature
+        assert a[17] == k / 16 and a[21] == k / 8 and a[18] == 0 and a[22] == 0  # valence, faces, boundary
+        assert a[20] == pytest.approx(min(64 * mean_edge, 4.0), rel=1e-6)         # mean edge
+        assert a[30] == pytest.approx(min(np.sqrt(total) / mean_edge, 4.0), rel=1e-6)
+        q = a[6:16]  # xx xy xz xw yy yz yw zz zw ww
+        d = -np.cos(alpha) * h / diameter  # plane offset of every incident face
+        want = [np.sin(alpha) ** 2 / 2, 0, 0, 0, np.sin(alpha) ** 2 / 2, 0, 0, np.cos(alpha) ** 2,
+                up * np.cos(alpha) * d, d * d]
+        assert np.allclose(q, want, atol=1e-6), (q, want)
+    ring = x[:k]
+    assert np.all(ring[:, 17] == 4 / 16) and np.all(ring[:, 21] == 4 / 8)
+    assert np.allclose(ring[:, 3:6] @ np.array([0, 0, 1.0]), 0, atol=1e-6)  # ring normals horizontal
+
+
+@needs_ref
+def test_oracle_scores_match_reference_build():
+    # The shortest-edge and current-plane orders rank by the reference's own
+    # scores: x[59] of its action rows and ActionState::teacher_cost.
+    from oracle_lib import Actions, Oracle
+    o = Oracle("ref")
+    for mesh in (meshes.icosphere(2, 0.02, 1), meshes.torus(1.0, 0.3, 12, 8, 0.02, 3),
+                 meshes.jitter(meshes.heightfield(10, 0.3, 5), 0.01, 5, 0.5), bipyramid(7, 1.0, 0.6)):
+        _, radius = o.bounds(mesh)
+        acts = Actions("ref", mesh)
+        legal = acts.legal.astype(np.int64)
+        assert len(legal) > 0
+        for mode in (3, 4):
+            want = acts.ref_scores(mode) + np.float32(0)
+            got = so.scores(mesh.positions, mesh.indices, legal, mode, diameter=max(1e-20, 2 * radius))
+            assert np.array_equal(got, want), (mode, np.abs(got - want).max())
+        acts.close()
+
+
 def test_topology_is_symmetric_with_reverse():
     m = meshes.icosphere(2, 0.02, 1)
     offsets, neighbors, face_offsets, faces, boundary = so.topology(m.indices, len(m.positions))
@@ -177,9 +247,9 @@ def test_gpu_fp32_forward_is_bit_exact(scorer, case):
 
 # BF16 tolerance: activations and weights rounded to 8-bit mantissas
 # (relative 2^-9 each) through four layers of width <= 256 with FP32
-# accumulation; measured errors are about 1e-2 of the output scale. The
-# bound below is 2^-4 of the largest FP64 output magnitude.
-BF16_TOLERANCE = 2 ** -4
+# accumulation; measured errors are 6.7e-3 to 8.7e-3 of the output scale.
+# The bound below is 2e-2 of the largest FP64 output magnitude.
+BF16_TOLERANCE = 2e-2
 
 
 @pytest.mark.gpu
