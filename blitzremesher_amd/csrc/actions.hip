@@ -181,9 +181,6 @@ __global__ void fans_kernel(Topo t, uint8_t* invalid, uint8_t* visited) {
         }
 }
 
-struct F3 {
-    float x, y, z;
-};
 // include/blitz/mesh.hpp:22-26
 
 __device__ __forceinline__ uint16_t material_of(const Topo& t, uint32_t f) {

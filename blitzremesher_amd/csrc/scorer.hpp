@@ -34,7 +34,7 @@ struct Graph {
     // batch buffers and gradients.
     DeviceBuffer t_h[4], t_hq[4], t_z[3];
     DeviceBuffer batch_actions, batch_src, batch_dst, labels, targets, hidden, batch_out, dy, t_e, z_e, dz_e, dh_a, dh_b, t_v,
-        accum;
+        accum, reduce_work, edge_buckets;
     bool has_outputs = false;
     GraphView view() const;
 };

@@ -11,6 +11,7 @@
 #include "scorer.hpp"
 #include "scorer_kernels.hpp"
 #include "train_kernels.hpp"
+#include <algorithm>
 
 namespace blitz_hip {
 namespace {
@@ -130,33 +131,46 @@ TrainResult Scorer::train_step(Graph& g, const uint32_t* actions, const uint8_t*
     }
     // ---- loss
     float* accum = grow<float>(g.accum, 2);  // [0] loss, [1] squared gradient norm
-    hip_check(hipMemsetAsync(accum, 0, 8, st), "clear accum");
+    // Reductions without float atomics: per-wave loss and norm partials,
+    // per-slice weight-gradient tiles, per-vertex gathers of the edge
+    // gradients, each added in a fixed order (one workspace, sized here
+    // before any launch).
+    size_t gemm_work = 0;
+    for (int l = 0; l < 3; ++l) gemm_work = std::max(gemm_work, gemm_at_b_workspace(nv, kHidden, kLayerIn[l]));
+    gemm_work = std::max({gemm_work, gemm_at_b_workspace(nb, kOutputs, kHidden),
+                          gemm_at_b_workspace(nb, kHidden, kEdgeIn)});
+    const size_t parts = std::max<size_t>(loss_parts(li), 4096);
+    const size_t scan_bytes = gather_edge_scan_bytes(nv);
+    float* work = grow<float>(g.reduce_work, gemm_work + parts + (scan_bytes + 3) / 4);
+    float* part = work + gemm_work;
+    void* scan_temp = part + parts;
+    uint32_t* buckets = grow<uint32_t>(g.edge_buckets, 3 * size_t(nv) + 2 + 2 * size_t(nb));
     hip_check(hipMemsetAsync(grads_.p, 0, size_t(kWeightCount) * 4, st), "clear gradients");
     float* dy = grow<float>(g.dy, size_t(nb) * kOutputs);
-    launch_loss(out, d_labels, d_targets, dy, accum, li, st);
+    launch_loss(out, d_labels, d_targets, dy, part, accum, li, st);
     // ---- backward: head and edge layer
     float* gr = grads_.as<float>();
     float* t_e = grow<float>(g.t_e, size_t(nb) * kHidden);
     float* z_e = grow<float>(g.z_e, size_t(nb) * kEdgeIn);
     float* dz_e = grow<float>(g.dz_e, size_t(nb) * kEdgeIn);
     launch_gemm_at_b(dy, kOutputs, nullptr, hidden, kHidden, gr + layer_offset(4),
-                     gr + layer_offset(4) + kOutputs * kHidden, nb, kOutputs, kHidden, st);
+                     gr + layer_offset(4) + kOutputs * kHidden, work, nb, kOutputs, kHidden, st);
     launch_gemm_a_w(dy, nullptr, w + layer_offset(4), t_e, nb, kOutputs, kHidden, st);
     // z_e = [h3(src) ; h3(dst)] for the edge weight gradient.
     launch_gather_edge(h[3], d_src, d_dst, z_e, nb, st);
     launch_gemm_at_b(t_e, kHidden, hidden, z_e, kEdgeIn, gr + layer_offset(3), gr + layer_offset(3) + kHidden * kEdgeIn,
-                     nb, kHidden, kEdgeIn, st);
+                     work, nb, kHidden, kEdgeIn, st);
     launch_gemm_a_w(t_e, hidden, w + layer_offset(3), dz_e, nb, kHidden, kEdgeIn, st);
     float* dh = grow<float>(g.dh_a, size_t(nv) * kHidden);
     float* dh_next = grow<float>(g.dh_b, size_t(nv) * kHidden);
-    hip_check(hipMemsetAsync(dh, 0, size_t(nv) * kHidden * 4, st), "clear dh");
-    launch_scatter_edge(dz_e, d_src, d_dst, dh, nb, st);
+    launch_gather_edge_backward(dz_e, d_src, d_dst, nb, nv, buckets, buckets + nv + 1, buckets + 2 * size_t(nv) + 2,
+                                buckets + 3 * size_t(nv) + 2, scan_temp, scan_bytes, dh, st);
     // ---- backward: message-passing layers
     float* t_v = grow<float>(g.t_v, size_t(nv) * 2 * kHidden);
     for (int l = 2; l >= 0; --l) {
         const uint32_t in = kLayerIn[l];
-        launch_gemm_at_b(dh, kHidden, h[l + 1], z[l], in, gr + layer_offset(l), gr + layer_offset(l) + kHidden * in, nv,
-                         kHidden, in, st);
+        launch_gemm_at_b(dh, kHidden, h[l + 1], z[l], in, gr + layer_offset(l), gr + layer_offset(l) + kHidden * in, work,
+                         nv, kHidden, in, st);
         if (l > 0) {
             launch_gemm_a_w(dh, h[l + 1], w + layer_offset(l), t_v, nv, kHidden, in, st);
             launch_aggregate_backward(t_v, gv, in / 2, dh_next, st);
@@ -164,7 +178,7 @@ TrainResult Scorer::train_step(Graph& g, const uint32_t* actions, const uint8_t*
         }
     }
     // ---- clip and AdamW (training/update_cuda.cu:358-393)
-    launch_norm(gr, kWeightCount, accum + 1, st);
+    launch_norm(gr, kWeightCount, part, accum + 1, st);
     float host[2] = {0, 0};
     hip_check(hipMemcpyAsync(host, accum, 8, hipMemcpyDeviceToHost, st), "download loss");
     hip_check(hipStreamSynchronize(st), "train step");

@@ -11,6 +11,9 @@
 // Gradient clipping by global norm and AdamW restate
 // training/update_cuda.cu:338-393 operation for operation.
 #include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include <string>
 
 #include <stdexcept>
 
@@ -18,6 +21,20 @@
 
 namespace blitz_hip {
 namespace {
+
+// Zeroes a[0..na) and b[0..nb).
+__global__ void fill_u32(uint32_t* __restrict__ a, uint32_t na, uint32_t* __restrict__ b, uint32_t nb) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < na + nb; i += gridDim.x * blockDim.x) {
+        if (i < na)
+            a[i] = 0;
+        else
+            b[i - na] = 0;
+    }
+}
+
+void hip_check_scan(hipError_t e) {
+    if (e != hipSuccess) throw std::runtime_error(std::string("gradient scan: ") + hipGetErrorString(e));
+}
 
 // Label bits of one batch entry.
 constexpr uint8_t kValid = 1, kPreferred = 2, kPlaced = 4;
@@ -63,7 +80,24 @@ __global__ __launch_bounds__(256) void loss_kernel(const float* __restrict__ y, 
         for (int c = 0; c < 4; ++c) dy[size_t(j) * 4 + c] = g[c];
     }
     for (int o = 32; o > 0; o >>= 1) l += __shfl_xor(l, o);
-    if ((threadIdx.x & 63) == 0 && l != 0.f) atomicAdd(loss, l);
+    // One partial per wave, summed in a fixed order by reduce_parts.
+    if ((threadIdx.x & 63) == 0) loss[blockIdx.x * 4 + (threadIdx.x >> 6)] = l;
+}
+
+// out = the sum of parts[0..n) in a fixed order (thread t takes t, t + 256,
+// ... in turn, then a fixed tree), so the result does not depend on launch
+// timing.
+__global__ __launch_bounds__(256) void reduce_parts(const float* __restrict__ parts, uint32_t n, float* out) {
+    __shared__ float sum[256];
+    float s = 0.f;
+    for (uint32_t i = threadIdx.x; i < n; i += 256) s += parts[i];
+    sum[threadIdx.x] = s;
+    __syncthreads();
+    for (uint32_t o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) sum[threadIdx.x] += sum[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = sum[0];
 }
 
 // v_mfma_f32_16x16x4f32 (FP32 products and sums) operand layout, wave64:
@@ -98,14 +132,16 @@ __device__ __forceinline__ float keep(bool in, float v, float mask) { return in 
 // Lane l loads four consecutive columns 4*(l%16) .. +3 of row l/16 of A and
 // of B and feeds component s to sub-tile s, so sub-tile s holds tile columns
 // 4*i + s (rows of C) and 4*j + s (columns of C): a fixed permutation undone
-// at the write. The waves' tiles are combined in LDS and added atomically.
-// With `sums`, the column sums of (A masked) (the bias gradients) are
-// accumulated from the same loaded values and added by the blocks of the
-// first column tile of C.
+// at the write. The waves' tiles are combined in LDS in wave order and the
+// block writes its slice's tile to `partial`; gemm_at_b_reduce adds the
+// slices into C in slice order, so the result does not depend on which
+// block finishes first. With `sums`, the column sums of (A masked) (the bias
+// gradients) are accumulated from the same loaded values and written per
+// slice by the blocks of the first column tile of C.
 template <bool kMasked>
 __global__ __launch_bounds__(256) void gemm_at_b(const float* __restrict__ a, uint32_t lda,
                                                  const float* __restrict__ a_mask, const float* __restrict__ b,
-                                                 uint32_t ldb, float* __restrict__ c, float* __restrict__ sums,
+                                                 uint32_t ldb, float* __restrict__ partial, bool with_sums,
                                                  uint32_t rows, uint32_t m, uint32_t n, uint32_t rows_per_slice) {
     __shared__ floatx4 part[3][16][64];
     __shared__ float part_sums[3][64];
@@ -176,23 +212,48 @@ __global__ __launch_bounds__(256) void gemm_at_b(const float* __restrict__ a, ui
     }
     __syncthreads();
     if (wave) return;
-    // The column sums of A are added once, by the first column tile of C.
-    if (sums && tn == 0 && lane < 16 && a_in)
-        for (int i = 0; i < 4; ++i) {
-            const float v = col[i] + part_sums[0][4 * lane + i] + part_sums[1][4 * lane + i] +
-                            part_sums[2][4 * lane + i];
-            if (v != 0.f) atomicAdd(&sums[ma + i], v);
-        }
+    // The block's share: its whole 64 x 64 tile (local row mi - tm, column
+    // nj - tn) in its own slot of `partial`, then, for the first column tile
+    // of C, the 64 column sums of A after all tiles. gemm_at_b_reduce adds
+    // the slices in slice order, so the result is the same bits every run.
+    const uint32_t m_tiles = (m + 63) / 64;
+    float* mine = partial + (size_t(slice) * tiles + tile) * 4096;
     for (int i = 0; i < 4; ++i)
         for (int j = 0; j < 4; ++j) {
             const floatx4 v = acc[i][j] + part[0][i * 4 + j][lane] + part[1][i * 4 + j][lane] +
                               part[2][i * 4 + j][lane];
-            const uint32_t nj = tn + 4 * (lane % 16) + j;
-            for (int q = 0; q < 4; ++q) {
-                const uint32_t mi = tm + 4 * (4 * (lane / 16) + q) + i;
-                if (mi < m && nj < n && v[q] != 0.f) atomicAdd(&c[size_t(mi) * n + nj], v[q]);
-            }
+            const uint32_t lj = 4 * (lane % 16) + j;
+            for (int q = 0; q < 4; ++q) mine[(4 * (4 * (lane / 16) + q) + i) * 64 + lj] = v[q];
         }
+    if (with_sums && tn == 0 && lane < 16) {
+        // After the tiles of every slice (gridDim.x = tiles x slices blocks).
+        float* col_sums = partial + size_t(gridDim.x) * 4096 + (size_t(slice) * m_tiles + tm / 64) * 64;
+        for (int i = 0; i < 4; ++i)
+            col_sums[4 * lane + i] = col[i] + part_sums[0][4 * lane + i] + part_sums[1][4 * lane + i] +
+                                     part_sums[2][4 * lane + i];
+    }
+}
+
+// c[mi][nj] += the slices' partial tiles in slice order; sums[mi] += the
+// slices' column sums in slice order (when sums is set).
+__global__ __launch_bounds__(256) void gemm_at_b_reduce(const float* __restrict__ partial, float* __restrict__ c,
+                                                        float* __restrict__ sums, uint32_t m, uint32_t n,
+                                                        uint32_t slices) {
+    const uint32_t n_tiles = (n + 63) / 64, tiles = ((m + 63) / 64) * n_tiles, m_tiles = (m + 63) / 64;
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < size_t(m) * n) {
+        const uint32_t mi = uint32_t(i / n), nj = uint32_t(i % n);
+        const uint32_t tile = (mi / 64) * n_tiles + nj / 64, local = (mi % 64) * 64 + nj % 64;
+        float v = 0.f;
+        for (uint32_t sl = 0; sl < slices; ++sl) v += partial[(size_t(sl) * tiles + tile) * 4096 + local];
+        c[i] += v;
+    }
+    if (sums && i < m) {
+        const float* col_sums = partial + size_t(tiles) * 4096 * slices;
+        float v = 0.f;
+        for (uint32_t sl = 0; sl < slices; ++sl) v += col_sums[(size_t(sl) * m_tiles + i / 64) * 64 + i % 64];
+        sums[i] += v;
+    }
 }
 
 // T[R][N] = (A masked)[R][K] x W[K][N] (W row-major with K rows of N) on the
@@ -284,14 +345,68 @@ __global__ void gather_edge(const float* __restrict__ h, const uint32_t* __restr
     z[i] = h[size_t(v) * kHidden + (c % kHidden)];
 }
 
-// dz of the edge input [h(src) ; h(dst)] scattered into dh.
-__global__ void scatter_edge(const float* __restrict__ dz, const uint32_t* __restrict__ src,
-                             const uint32_t* __restrict__ dst, float* __restrict__ dh, uint32_t count) {
-    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= size_t(count) * kEdgeIn) return;
-    const uint32_t r = uint32_t(i / kEdgeIn), c = uint32_t(i % kEdgeIn);
-    const uint32_t v = c < kHidden ? src[r] : dst[r];
-    atomicAdd(&dh[size_t(v) * kHidden + (c % kHidden)], dz[i]);
+// The transpose of gather_edge without float atomics: entry e = 2 r + half
+// of the batch reads h(src[r]) (half 0) or h(dst[r]) (half 1); every vertex
+// gathers the gradients of its entries in ascending e. edge_count and
+// edge_fill bucket the entries by vertex (integer atomics; the order inside
+// a bucket is then put right by edge_gather).
+__global__ void edge_count(const uint32_t* __restrict__ src, const uint32_t* __restrict__ dst, uint32_t count,
+                           uint32_t* __restrict__ counts) {
+    const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= 2 * count) return;
+    atomicAdd(&counts[(e & 1) ? dst[e >> 1] : src[e >> 1]], 1u);
+}
+
+__global__ void edge_fill(const uint32_t* __restrict__ src, const uint32_t* __restrict__ dst, uint32_t count,
+                          const uint32_t* __restrict__ offsets, uint32_t* __restrict__ cursor,
+                          uint32_t* __restrict__ list) {
+    const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= 2 * count) return;
+    const uint32_t v = (e & 1) ? dst[e >> 1] : src[e >> 1];
+    list[offsets[v] + atomicAdd(&cursor[v], 1u)] = e;
+}
+
+// One wave per vertex: dh[v][c] = sum over its entries e in ascending order
+// of dz[e / 2][(e % 2) * 128 + c]; vertices without entries get zeros. Lane l
+// owns channels l and l + 64.
+__global__ __launch_bounds__(256) void edge_gather(const float* __restrict__ dz, const uint32_t* __restrict__ offsets,
+                                                   const uint32_t* __restrict__ list, uint32_t vertices,
+                                                   float* __restrict__ dh) {
+    const uint32_t lane = threadIdx.x & 63, v = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (v >= vertices) return;
+    const uint32_t b = offsets[v], len = offsets[v + 1] - b;
+    float a0 = 0.f, a1 = 0.f;
+    const auto take = [&](uint32_t e) {
+        const float* row = dz + size_t(e >> 1) * kEdgeIn + (e & 1) * kHidden;
+        a0 += row[lane];
+        a1 += row[lane + 64];
+    };
+    if (len <= 64) {
+        // Rank of each entry among the bucket's (all distinct), then the
+        // entries in rank order.
+        const uint32_t code = lane < len ? list[b + lane] : 0xFFFFFFFFu;
+        uint32_t rank = 0;
+        for (uint32_t k = 0; k < len; ++k) rank += uint32_t(__shfl(int(code), int(k))) < code;
+        for (uint32_t k = 0; k < len; ++k) {
+            const uint64_t owner = __ballot(lane < len && rank == k);
+            take(uint32_t(__shfl(int(code), __ffsll(static_cast<unsigned long long>(owner)) - 1)));
+        }
+    } else {
+        // Long buckets: repeatedly the smallest entry above the last one.
+        uint32_t last = 0;
+        for (uint32_t k = 0; k < len; ++k) {
+            uint32_t best = 0xFFFFFFFFu;
+            for (uint32_t j = lane; j < len; j += 64) {
+                const uint32_t e = list[b + j];
+                if ((k == 0 || e > last) && e < best) best = e;
+            }
+            for (int o = 32; o > 0; o >>= 1) best = min(best, uint32_t(__shfl_xor(int(best), o)));
+            take(best);
+            last = best;
+        }
+    }
+    dh[size_t(v) * kHidden + lane] = a0;
+    dh[size_t(v) * kHidden + lane + 64] = a1;
 }
 
 // dh[v] = t[v][0:d] + sum over u in N(v) of t[u][d:2d] / deg(u): the
@@ -328,7 +443,7 @@ __global__ void norm_kernel(const float* __restrict__ g, uint32_t n, float* __re
         s += isfinite(v) ? v * v : INFINITY;
     }
     for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
-    if ((threadIdx.x & 63) == 0) atomicAdd(sum, s);
+    if ((threadIdx.x & 63) == 0) sum[blockIdx.x * 4 + (threadIdx.x >> 6)] = s;  // one partial per wave
 }
 
 // training/update_cuda.cu:378-393 with the clip and bias corrections
@@ -351,27 +466,44 @@ __global__ void bf16_to_fp32(const __bf16* __restrict__ x, float* __restrict__ y
 
 }  // namespace
 
-void launch_loss(const float* y, const uint8_t* labels, const float* targets, float* dy, float* loss,
+uint32_t loss_parts(const LossInput& in) { return (in.count + in.pool - 1) / in.pool * 4; }
+
+void launch_loss(const float* y, const uint8_t* labels, const float* targets, float* dy, float* parts, float* loss,
                  const LossInput& in, hipStream_t stream) {
     if (!in.count) return;
-    loss_kernel<<<dim3((in.count + in.pool - 1) / in.pool), dim3(256), 0, stream>>>(y, labels, targets, dy, loss, in);
+    loss_kernel<<<dim3((in.count + in.pool - 1) / in.pool), dim3(256), 0, stream>>>(y, labels, targets, dy, parts, in);
+    reduce_parts<<<1, 256, 0, stream>>>(parts, loss_parts(in), loss);
+}
+
+namespace {
+// About 512 blocks (two per CU) of at least 256 rows each; the slice count is
+// a multiple of 8 (one share per XCD).
+uint32_t gemm_at_b_slices(uint32_t rows, uint32_t m, uint32_t n) {
+    const uint32_t tiles = ((m + 63) / 64) * ((n + 63) / 64);
+    const uint32_t wanted = std::max<uint32_t>(1, std::min<uint32_t>((rows + 255) / 256, 512 / tiles));
+    return (wanted + 7) / 8 * 8;
+}
+}  // namespace
+
+size_t gemm_at_b_workspace(uint32_t rows, uint32_t m, uint32_t n) {
+    const uint32_t tiles = ((m + 63) / 64) * ((n + 63) / 64), slices = gemm_at_b_slices(rows, m, n);
+    return size_t(slices) * tiles * 4096 + size_t(slices) * ((m + 63) / 64) * 64;
 }
 
 void launch_gemm_at_b(const float* a, uint32_t lda, const float* a_mask, const float* b, uint32_t ldb, float* c,
-                      float* sums, uint32_t rows, uint32_t m, uint32_t n, hipStream_t stream) {
+                      float* sums, float* work, uint32_t rows, uint32_t m, uint32_t n, hipStream_t stream) {
     if (!rows || !m || !n) return;
     if ((m | n | lda | ldb) % 4) throw std::invalid_argument("gemm_at_b: m, n and row strides must be multiples of 4");
-    // About 512 blocks (two per CU) of at least 256 rows each; the slice
-    // count is a multiple of 8 (one share per XCD).
     const uint32_t tiles = ((m + 63) / 64) * ((n + 63) / 64);
-    const uint32_t wanted = std::max<uint32_t>(1, std::min<uint32_t>((rows + 255) / 256, 512 / tiles));
-    const uint32_t slices = (wanted + 7) / 8 * 8;
+    const uint32_t slices = gemm_at_b_slices(rows, m, n);
     const uint32_t per = (rows + slices - 1) / slices;
     const dim3 grid(tiles * slices);
     if (a_mask)
-        gemm_at_b<true><<<grid, dim3(256), 0, stream>>>(a, lda, a_mask, b, ldb, c, sums, rows, m, n, per);
+        gemm_at_b<true><<<grid, dim3(256), 0, stream>>>(a, lda, a_mask, b, ldb, work, sums != nullptr, rows, m, n, per);
     else
-        gemm_at_b<false><<<grid, dim3(256), 0, stream>>>(a, lda, a_mask, b, ldb, c, sums, rows, m, n, per);
+        gemm_at_b<false><<<grid, dim3(256), 0, stream>>>(a, lda, a_mask, b, ldb, work, sums != nullptr, rows, m, n, per);
+    const size_t elements = std::max<size_t>(size_t(m) * n, m);
+    gemm_at_b_reduce<<<dim3(uint32_t((elements + 255) / 256)), dim3(256), 0, stream>>>(work, c, sums, m, n, slices);
 }
 
 void launch_gemm_a_w(const float* a, const float* a_mask, const float* w, float* t, uint32_t rows, uint32_t k,
@@ -399,11 +531,29 @@ void launch_gather_edge(const float* h, const uint32_t* src, const uint32_t* dst
     gather_edge<<<dim3(uint32_t((n + 255) / 256)), dim3(256), 0, stream>>>(h, src, dst, z, count);
 }
 
-void launch_scatter_edge(const float* dz, const uint32_t* src, const uint32_t* dst, float* dh, uint32_t count,
-                         hipStream_t stream) {
-    const size_t n = size_t(count) * kEdgeIn;
-    if (!n) return;
-    scatter_edge<<<dim3(uint32_t((n + 255) / 256)), dim3(256), 0, stream>>>(dz, src, dst, dh, count);
+void launch_gather_edge_backward(const float* dz, const uint32_t* src, const uint32_t* dst, uint32_t count,
+                                 uint32_t vertices, uint32_t* counts, uint32_t* offsets, uint32_t* cursor,
+                                 uint32_t* list, void* scan_temp, size_t scan_bytes, float* dh, hipStream_t stream) {
+    fill_u32<<<dim3(std::min<uint32_t>(1024, (2 * vertices + 2 + 255) / 256)), dim3(256), 0, stream>>>(
+        counts, vertices + 1, cursor, vertices);
+    const uint32_t entries = 2 * count;
+    if (entries) {
+        edge_count<<<dim3((entries + 255) / 256), dim3(256), 0, stream>>>(src, dst, count, counts);
+        hip_check_scan(hipcub::DeviceScan::ExclusiveSum(scan_temp, scan_bytes, counts, offsets, int(vertices + 1),
+                                                        stream));
+        edge_fill<<<dim3((entries + 255) / 256), dim3(256), 0, stream>>>(src, dst, count, offsets, cursor, list);
+    } else {
+        fill_u32<<<dim3(std::min<uint32_t>(1024, (vertices + 1 + 255) / 256)), dim3(256), 0, stream>>>(
+            offsets, vertices + 1, cursor, 0);
+    }
+    edge_gather<<<dim3((vertices + 3) / 4), dim3(256), 0, stream>>>(dz, offsets, list, vertices, dh);
+}
+
+size_t gather_edge_scan_bytes(uint32_t vertices) {
+    size_t bytes = 0;
+    (void)hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, static_cast<const uint32_t*>(nullptr),
+                                           static_cast<uint32_t*>(nullptr), int(vertices + 1));
+    return std::max<size_t>(bytes, 16);
 }
 
 void launch_aggregate_backward(const float* t, GraphView g, uint32_t width, float* dh, hipStream_t stream) {
@@ -413,8 +563,10 @@ void launch_aggregate_backward(const float* t, GraphView g, uint32_t width, floa
     aggregate_backward<<<dim3(uint32_t((n + 255) / 256)), dim3(256), 0, stream>>>(t, g, width, dh);
 }
 
-void launch_norm(const float* g, uint32_t n, float* sum, hipStream_t stream) {
-    norm_kernel<<<dim3(std::min<uint32_t>((n + 255) / 256, 1024)), dim3(256), 0, stream>>>(g, n, sum);
+void launch_norm(const float* g, uint32_t n, float* parts, float* sum, hipStream_t stream) {
+    const uint32_t blocks = std::min<uint32_t>((n + 255) / 256, 1024);
+    norm_kernel<<<dim3(blocks), dim3(256), 0, stream>>>(g, n, parts);
+    reduce_parts<<<1, 256, 0, stream>>>(parts, blocks * 4, sum);
 }
 
 void launch_adamw(float* value, const float* grad, float* mean, float* variance, uint32_t n, const AdamStep& s,

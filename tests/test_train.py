@@ -237,6 +237,30 @@ def test_gpu_training_reduces_loss(scorer, precision):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("precision", [0, 1])
+def test_gpu_training_is_deterministic(scorer, precision):
+    # Every reduction (loss, weight-gradient slices, edge-gradient gathers,
+    # gradient norm) adds its partials in a fixed order: two runs from the
+    # same weights and optimizer state end bit-identical. The torus has
+    # 12,000 vertices, so the vertex-layer weight gradients are summed over
+    # many row slices.
+    from blitzremesher_amd.scorer import TrainSettings, he_weights
+    g, offsets, neighbors, acts, ids, labels, targets = gpu_case(
+        scorer, meshes.torus(1.0, 0.3, 200, 60, 0.01, 9), 4096, 21)
+    s = TrainSettings(pool=32, precision=precision, lr=2e-3)
+    runs = []
+    for _ in range(2):
+        scorer.set_weights(he_weights(8))
+        scorer.reset_optimizer()
+        results = [scorer.train_step(g, ids, labels, targets, s) for _ in range(4)]
+        runs.append(([(r.loss, r.grad_norm, r.step) for r in results], scorer.gradients(), scorer.weights()))
+    (res_a, grad_a, w_a), (res_b, grad_b, w_b) = runs
+    assert res_a == res_b
+    assert np.array_equal(grad_a.view(np.uint32), grad_b.view(np.uint32))
+    assert np.array_equal(w_a.view(np.uint32), w_b.view(np.uint32))
+
+
+@pytest.mark.gpu
 def test_gpu_train_rejects_bad_batches(scorer):
     from blitzremesher_amd._lib import BlitzError
     from blitzremesher_amd.scorer import TrainSettings
