@@ -118,17 +118,17 @@ __global__ __launch_bounds__(kRasterThreads) void raster_band_kernel(const Raste
                 const int w = s.x1 - s.x0 + 1, h = s.y1 - s.y0 + 1;
                 if (w <= 4 && h <= 8) {
                     // At most 2x2 tiles: an 8x8 local bitmap, then four words.
-                    const int tx0 = s.x0 >> 3, ty = s.y0 >> 3;
                     const int ya = max(s.y0, ymin), yb = min(s.y1, ymax);
                     uint64_t word[4];
                     scatter_bitmap(lane_bitmap4(s, ya, yb), s.x0, s.y0, word);
-#pragma unroll
-                    for (int k = 0; k < 4; ++k)
-                        if (word[k]) {
-                            const uint32_t row = uint32_t(ty + (k >> 1)) - ty0;
-                            atomicOr(&band[row * t.tiles + uint32_t(tx0 + (k & 1))],
-                                     static_cast<unsigned long long>(word[k]));
-                        }
+                    // Word k goes to tile row (y0 >> 3) + (k >> 1), column
+                    // (x0 >> 3) + (k & 1); a nonzero word holds samples of
+                    // rows [ya, yb], which lie in the band.
+                    const uint32_t at = (uint32_t(s.y0 >> 3) - ty0) * t.tiles + uint32_t(s.x0 >> 3);
+                    if (word[0]) atomicOr(&band[at], static_cast<unsigned long long>(word[0]));
+                    if (word[1]) atomicOr(&band[at + 1], static_cast<unsigned long long>(word[1]));
+                    if (word[2]) atomicOr(&band[at + t.tiles], static_cast<unsigned long long>(word[2]));
+                    if (word[3]) atomicOr(&band[at + t.tiles + 1], static_cast<unsigned long long>(word[3]));
                 } else if (w <= 8 && h <= 8) {
                     medium = true;
                 } else {

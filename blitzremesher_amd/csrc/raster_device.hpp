@@ -73,6 +73,34 @@ __device__ __forceinline__ Corners load_corners(const RasterTask& t, uint3 tri, 
     return k;
 }
 
+// Bounding box and frame test of src/evaluate.cpp:155-160 for corners a, d, e
+// (d and e already swapped for a back face) whose signed area is `area`:
+// false when the face is out of frame. The reference takes std::min /
+// std::max over the corners in order and then tests isfinite and the frame.
+// Without a NaN coordinate the order does not matter (results can differ
+// only in the sign of a zero, which neither floor() nor the frame test
+// sees) and the isfinite tests are implied by the frame test (an infinite
+// minimum or maximum fails it), so min/max instructions do. A NaN
+// coordinate makes the signed area NaN (and leaves the corners unswapped);
+// such faces, and infinite ones whose area is NaN, take the reference's
+// ordered form.
+__device__ __forceinline__ bool box_in_frame(double2 a, double2 d, double2 e, double area, double size,
+                                             double& minx, double& maxx, double& miny, double& maxy) {
+    if (!isnan(area)) {
+        minx = fmin(fmin(a.x, d.x), e.x);
+        maxx = fmax(fmax(a.x, d.x), e.x);
+        miny = fmin(fmin(a.y, d.y), e.y);
+        maxy = fmax(fmax(a.y, d.y), e.y);
+        return !(minx < 0 || miny < 0 || maxx > size || maxy > size);
+    }
+    minx = min3(a.x, d.x, e.x);
+    maxx = max3(a.x, d.x, e.x);
+    miny = min3(a.y, d.y, e.y);
+    maxy = max3(a.y, d.y, e.y);
+    return !(!isfinite(minx) || !isfinite(maxx) || !isfinite(miny) || !isfinite(maxy) || minx < 0 || miny < 0 ||
+             maxx > size || maxy > size);
+}
+
 // Triangle setup of src/evaluate.cpp:144-165 on projected vertices.
 // Returns 0 = nothing to draw, 1 = draw, 2 = out of frame (the reference stops
 // at the first such face).
@@ -89,12 +117,8 @@ __device__ __forceinline__ int setup_triangle(const RasterTask& t, const Corners
         d = e;
         e = tmp;
     }
-    const double minx = min3(a.x, d.x, e.x), maxx = max3(a.x, d.x, e.x);
-    const double miny = min3(a.y, d.y, e.y), maxy = max3(a.y, d.y, e.y);
-    const double size = double(t.size);
-    if (!isfinite(minx) || !isfinite(maxx) || !isfinite(miny) || !isfinite(maxy) || minx < 0 || miny < 0 ||
-        maxx > size || maxy > size)
-        return 2;
+    double minx, maxx, miny, maxy;
+    if (!box_in_frame(a, d, e, area, double(t.size), minx, maxx, miny, maxy)) return 2;
     s.x0 = max(0, int(floor(minx)));
     s.x1 = min(int(t.size) - 1, int(floor(maxx)));
     s.y0 = max(0, int(floor(miny)));
@@ -220,46 +244,64 @@ __device__ __forceinline__ uint64_t lane_bitmap(const Setup& s, int ya, int yb) 
 // columns are computed once per face and the y-dependent ones once per row;
 // every value is the reference expression's own rounding,
 // u = (cx-bx)*(py-by) - (cy-by)*(px-bx), covered iff !(u+r0 < 0 || ...).
+// Sample centres x + .5 are exact in double for every int x, so they are
+// stepped by exact additions of 1. The row counter k is the same in every
+// active lane, so each row's bits land at wave-uniform positions (bit
+// 8*k + c for row k of [ya, yb]) and the bitmap is moved to the box's rows
+// once at the end.
 // Bitmap relative to (x0, y0), bit 8*(y-y0) + (x-x0), rows [ya, yb].
 __device__ __forceinline__ uint64_t lane_bitmap4(const Setup& s, int ya, int yb) {
     const double au = s.cx - s.bx, bu = s.cy - s.by;
     const double av = s.ax - s.cx, bv = s.ay - s.cy;
     const double aw = s.bx - s.ax, bw = s.by - s.ay;
     const int w = s.x1 - s.x0 + 1;
+    const double px0 = double(s.x0) + .5;
     double cu[4], cv[4], cw[4];
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
-        const double px = double(s.x0 + c) + .5;
+        const double px = px0 + double(c);
         cu[c] = bu * (px - s.bx);
         cv[c] = bv * (px - s.cx);
         cw[c] = bw * (px - s.ax);
     }
     // fl(u + r) < 0 tested as u < -r (see covers()).
     const double nu = -s.r0, nv = -s.r1, nw = -s.r2;
-    uint64_t bits = 0;
-    for (int y = ya; y <= yb; ++y) {
-        const double py = double(y) + .5;
+    const int rows = yb - ya + 1;
+    if (rows <= 0) return 0;
+    double py = double(ya) + .5;
+    uint32_t lo = 0, hi = 0;
+    for (int k = 0; k < rows; ++k) {
         const double ru = au * (py - s.by), rv = av * (py - s.cy), rw = aw * (py - s.ay);
         uint32_t row = 0;
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
             // Bitwise | so that all three tests issue without branches.
             const bool in = (c < w) & !(((ru - cu[c]) < nu) | ((rv - cv[c]) < nv) | ((rw - cw[c]) < nw));
-            row |= uint32_t(in) << c;
+            row |= in ? 1u << c : 0u;
         }
-        bits |= uint64_t(row) << (8 * (y - s.y0));
+        if (k < 4)
+            lo |= row << (8 * k);
+        else
+            hi |= row << (8 * (k - 4));
+        py += 1.0;
     }
-    return bits;
+    return ((uint64_t(hi) << 32) | lo) << (8 * (ya - s.y0));
 }
 
 // Moves an 8x8 bitmap anchored at sample (x0, y0) into the up-to-four 8x8
 // tile words it overlaps: word[0] tile (x0>>3, y0>>3), word[1] the tile to
-// its right, word[2] below, word[3] below-right.
+// its right, word[2] below, word[3] below-right. Shifting the bitmap left by
+// x0 & 7 puts each row's samples that stay in the left tile at columns
+// >= x0 & 7 (`keep`, a byte mask repeated in every row); the others are the
+// right tile's columns < x0 & 7, found by the complementary shift and mask.
 __device__ __forceinline__ void scatter_bitmap(uint64_t local, int x0, int y0, uint64_t word[4]) {
-    constexpr uint64_t kColumn0 = 0x0101010101010101ull;
-    const int ox = x0 & 7, oy = y0 & 7;
-    const uint64_t left = (local & (kColumn0 * ((1u << (8 - ox)) - 1))) << ox;
-    const uint64_t right = ox ? (local >> (8 - ox)) & (kColumn0 * ((1u << ox) - 1)) : 0;
+    const uint32_t ox = uint32_t(x0) & 7u, oy = uint32_t(y0) & 7u;
+    uint32_t keep = (0xFFu << ox) & 0xFFu;
+    keep |= keep << 8;
+    keep |= keep << 16;
+    const uint64_t keep64 = (uint64_t(keep) << 32) | keep;
+    const uint64_t left = (local << ox) & keep64;
+    const uint64_t right = (local >> (8 - ox)) & ~keep64;
     word[0] = left << (8 * oy);
     word[1] = right << (8 * oy);
     word[2] = oy ? left >> (64 - 8 * oy) : 0;

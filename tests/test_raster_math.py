@@ -22,9 +22,12 @@ PRELUDE = """#include <algorithm>
 #include <cstdio>
 #include <random>
 #include <cstdlib>
-using std::fabs; using std::isfinite; using std::max; using std::min;
+using std::fabs; using std::isfinite; using std::isnan; using std::fmax; using std::fmin; using std::max; using std::min;
 #define __device__
 #define __forceinline__ inline
+struct double2 {
+    double x, y;
+};
 struct Setup {
     double ax, ay, bx, by, cx, cy, r0, r1, r2;
     int x0, x1, y0, y1;
@@ -132,6 +135,57 @@ int main(int argc, char** argv) {
             bad += ((d + r) < 0) != (d < -r);
         }
     }
+    // scatter_bitmap against a per-bit move of 4- and 8-wide bitmaps.
+    {
+        std::uniform_int_distribution<int> origin(0, 4095);
+        for (int it = 0; it < iterations; ++it) {
+            const int cols = it % 2 ? 4 : 8;
+            uint64_t local = rng();
+            for (int r = 0; r < 8; ++r)
+                for (int c = cols; c < 8; ++c) local &= ~(1ull << (8 * r + c));
+            const int x0 = origin(rng), y0 = origin(rng);
+            uint64_t want[4] = {0, 0, 0, 0}, got[4];
+            for (int r = 0; r < 8; ++r)
+                for (int c = 0; c < 8; ++c)
+                    if (local >> (8 * r + c) & 1) {
+                        const int x = (x0 & 7) + c, y = (y0 & 7) + r;
+                        want[(y >= 8) * 2 + (x >= 8)] |= 1ull << (8 * (y & 7) + (x & 7));
+                    }
+            scatter_bitmap(local, x0, y0, got);
+            for (int k = 0; k < 4; ++k) bad += got[k] != want[k];
+        }
+    }
+    // box_in_frame against the reference's ordered std::min / std::max,
+    // isfinite and frame tests (src/evaluate.cpp:155-160), corners swapped
+    // for back faces as the setup does, over special coordinates.
+    {
+        const double size = 1024;
+        const double specials[] = {0.0, -0.0, 0.25, 3.5, 511.75, 1023.999, 1024.0, 1024.0000000000002, -1e-300,
+                                   1e-300, 1e308, -1e308, INFINITY, -INFINITY, NAN, -NAN, 7.0, 1000.5};
+        std::uniform_int_distribution<int> pick(0, int(sizeof specials / sizeof specials[0]) - 1);
+        std::uniform_real_distribution<double> any(-8, 1032);
+        const auto ref_min = [](double a, double b, double c) { return std::min({a, b, c}); };
+        const auto ref_max = [](double a, double b, double c) { return std::max({a, b, c}); };
+        for (int it = 0; it < iterations * 4; ++it) {
+            double v[6];
+            for (double& x : v) x = it % 3 == 0 ? any(rng) : specials[pick(rng)];
+            if (it % 5 == 1)
+                for (double& x : v) x = any(rng);
+            double2 a{v[0], v[1]}, d{v[2], v[3]}, e{v[4], v[5]};
+            const double area = edge(a.x, a.y, d.x, d.y, e.x, e.y);
+            if (area < 0) std::swap(d, e);
+            const double rminx = ref_min(a.x, d.x, e.x), rmaxx = ref_max(a.x, d.x, e.x);
+            const double rminy = ref_min(a.y, d.y, e.y), rmaxy = ref_max(a.y, d.y, e.y);
+            const bool want = !(!isfinite(rminx) || !isfinite(rmaxx) || !isfinite(rminy) || !isfinite(rmaxy) ||
+                                rminx < 0 || rminy < 0 || rmaxx > size || rmaxy > size);
+            double minx, maxx, miny, maxy;
+            const bool got = box_in_frame(a, d, e, area, size, minx, maxx, miny, maxy);
+            bad += got != want;
+            if (got && want)
+                bad += std::floor(minx) != std::floor(rminx) || std::floor(maxx) != std::floor(rmaxx) ||
+                       std::floor(miny) != std::floor(rminy) || std::floor(maxy) != std::floor(rmaxy);
+        }
+    }
     printf("%ld %ld %ld %ld\n", rects, full, empty, bad);
     return 0;
 }
@@ -165,9 +219,11 @@ def _source(plant: tuple[str, str] | None = None) -> str:
     if plant:
         assert plant[0] in text
         text = text.replace(plant[0], plant[1])
-    parts = [_function(text, "double edge("), _function(text, "bool covers("), _function(text, "int classify_rect("),
-             _function(text, "uint64_t rect_word("), _function(text, "uint64_t lane_bitmap("),
-             _function(text, "uint64_t lane_bitmap4(")]
+    parts = [_function(text, "double edge("), _function(text, "double min3("), _function(text, "double max3("),
+             _function(text, "bool box_in_frame("), _function(text, "bool covers("),
+             _function(text, "int classify_rect("), _function(text, "uint64_t rect_word("),
+             _function(text, "uint64_t lane_bitmap("), _function(text, "uint64_t lane_bitmap4("),
+             _function(text, "void scatter_bitmap(")]
     return PRELUDE + "\n\n".join(parts) + CHECK
 
 
@@ -194,7 +250,16 @@ def test_device_raster_math_agrees_with_every_sample(tmp_path):
     # Columns past the box's width kept in the four-wide row.
     ("const bool in = (c < w) &", "const bool in = (c <= w) &"),
     # A row's bits shifted by the band row instead of the box row.
-    ("bits |= uint64_t(row) << (8 * (y - s.y0));", "bits |= uint64_t(row) << (8 * (y - ya));"),
+    ("| lo) << (8 * (ya - s.y0));", "| lo);"),
+    # Row centres not stepped between rows of the four-wide path.
+    ("        py += 1.0;\n    }\n    return ((uint64_t(hi)", "    }\n    return ((uint64_t(hi)"),
+    # The right tile's columns taken without the complementary mask.
+    ("const uint64_t right = (local >> (8 - ox)) & ~keep64;", "const uint64_t right = (local >> (8 - ox));"),
+    # NaN corners sent down the unordered min/max path.
+    ("    if (!isnan(area)) {\n        minx = fmin", "    if (true) {\n        minx = fmin"),
+    # The frame test's upper bound taken as inclusive.
+    ("return !(minx < 0 || miny < 0 || maxx > size || maxy > size);",
+     "return !(minx < 0 || miny < 0 || maxx >= size || maxy >= size);"),
 ])
 def test_check_catches_planted_mistakes(tmp_path, plant):
     _, _, _, bad = _run(tmp_path, _source(plant), 4000)
