@@ -236,6 +236,16 @@ PROTOTYPES = {
     "blitz_hip_actions_trial": (C.c_int, [_P, C.POINTER(C.c_uint32), C.c_size_t, C.POINTER(C.c_uint32), C.c_size_t,
                                           C.POINTER(C.c_size_t), *_ERR]),
     "blitz_hip_actions_rank": (C.c_int, [_P, C.c_uint8, _P, _P, C.POINTER(C.c_uint32), C.c_size_t, *_ERR]),
+    "blitz_hip_actions_vertex_features": (C.c_int, [_P, C.POINTER(C.c_float), C.c_size_t, *_ERR]),
+    "blitz_hip_policy_create": (C.c_int, [C.c_int32, C.c_uint32, C.c_uint32, C.POINTER(C.c_float), C.c_size_t,
+                                          C.POINTER(_P), *_ERR]),
+    "blitz_hip_policy_destroy": (None, [_P]),
+    "blitz_hip_policy_predict": (C.c_int, [_P, C.POINTER(C.c_float), C.c_size_t, C.POINTER(C.c_float), C.c_size_t,
+                                           *_ERR]),
+    "blitz_hip_actions_rank_policy": (C.c_int, [_P, _P, C.POINTER(C.c_float), C.c_uint64, C.c_int32,
+                                                C.POINTER(C.c_uint32), C.c_size_t, *_ERR]),
+    "blitz_hip_actions_features": (C.c_int, [_P, C.POINTER(C.c_float), C.c_uint32, C.c_uint64, C.POINTER(C.c_float),
+                                             C.c_size_t, *_ERR]),
     "blitz_hip_execute_actions": (C.c_int, [_P, C.POINTER(BlitzMesh), C.POINTER(Bounds), C.POINTER(EvalSettingsC),
                                             C.c_uint64, C.c_uint32, C.c_uint8, _P, C.c_uint8, C.POINTER(C.c_uint32),
                                             C.c_size_t, C.POINTER(C.c_size_t), C.POINTER(ActionStats), *_ERR]),

@@ -20,13 +20,17 @@ def _u32p(a: np.ndarray):
 
 
 class ActionSet:
-    def __init__(self, mesh: Mesh, uv: np.ndarray | None = None, device: int = 0):
+    def __init__(self, mesh: Mesh, uv: np.ndarray | None = None, device: int = 0,
+                 tangents: np.ndarray | None = None):
         self.lib = _lib.load()
         self.mesh = mesh
         self.uv = None if uv is None else np.ascontiguousarray(uv, np.float32).reshape(-1, 2)
+        self.tangents = None if tangents is None else np.ascontiguousarray(tangents, np.float32).reshape(-1, 4)
         desc = mesh.descriptor()
         if self.uv is not None:
             desc.uv = _lib.BlitzStream(self.uv.ctypes.data, self.uv.shape[0], 8)
+        if self.tangents is not None:
+            desc.tangents = _lib.BlitzStream(self.tangents.ctypes.data, self.tangents.shape[0], 16)
         handle = C.c_void_p()
         err = _lib.error_buffer()
         _lib.check(self.lib.blitz_hip_actions_create(device, C.byref(desc), C.byref(handle), err, len(err)), err)
@@ -63,6 +67,37 @@ class ActionSet:
         _lib.check(self.lib.blitz_hip_actions_rank(self.handle, ranking, scorer.handle if scorer else None,
                                                    graph.handle if graph else None, _u32p(order), order.size, err,
                                                    len(err)), err)
+        return order[: len(self.legal)].copy()
+
+    def vertex_features(self) -> np.ndarray:
+        """blitz::neural::graph's 24 channels per vertex (n, 24) of this state's mesh."""
+        out = np.zeros((self.mesh.positions.shape[0], 24), np.float32)
+        err = _lib.error_buffer()
+        _lib.check(self.lib.blitz_hip_actions_vertex_features(self.handle, out.ctypes.data_as(C.POINTER(C.c_float)),
+                                                              out.size, err, len(err)), err)
+        return out
+
+    def features(self, conditions=None, width: int = 80, input_triangles: int = 0) -> np.ndarray:
+        """ActionState::actions' feature rows (legal, width) for the eight conditions."""
+        c = np.zeros(8, np.float32) if conditions is None else np.ascontiguousarray(conditions, np.float32)
+        out = np.zeros((max(1, len(self.legal)), width), np.float32)
+        err = _lib.error_buffer()
+        _lib.check(self.lib.blitz_hip_actions_features(self.handle, c.ctypes.data_as(C.POINTER(C.c_float)), width,
+                                                       input_triangles, out.ctypes.data_as(C.POINTER(C.c_float)),
+                                                       out.size, err, len(err)), err)
+        return out[: len(self.legal)]
+
+    def rank_policy(self, policy: "Policy", conditions=None, input_triangles: int = 0,
+                    preserve_uv: bool = False) -> np.ndarray:
+        """Learned order of the reference's GPU action state: the policy's
+        first output on each legal action's feature row, descending."""
+        c = np.zeros(8, np.float32) if conditions is None else np.ascontiguousarray(conditions, np.float32)
+        order = np.zeros(max(1, len(self.legal)), np.uint32)
+        err = _lib.error_buffer()
+        _lib.check(self.lib.blitz_hip_actions_rank_policy(self.handle, policy.handle,
+                                                          c.ctypes.data_as(C.POINTER(C.c_float)), input_triangles,
+                                                          int(preserve_uv), _u32p(order), order.size, err, len(err)),
+                   err)
         return order[: len(self.legal)].copy()
 
     def independent(self, order: np.ndarray, maximum: int, maximum_removed: int) -> np.ndarray:
@@ -132,3 +167,55 @@ def execute_actions(audit, mesh: Mesh, center, radius, gate, target: int, budget
 
 
 __all__ = ["ActionSet", "execute_actions"]
+
+
+# Inputs and outputs of the policy architectures (src/neural/placement.hpp:17-26).
+POLICY_SHAPES = {2: (80, 3), 3: (128, 12), 4: (128, 12)}
+
+
+def policy_weight_count(architecture: int, hidden: int = 64) -> int:
+    i, o = POLICY_SHAPES[architecture]
+    return hidden * (i + 1) + hidden * (hidden + 1) + o * (hidden + 1)
+
+
+class Policy:
+    """ActionCuda (src/neural/action.hpp:70-87): the action (architecture 2)
+    or placement (3, 4) policy on the device, three compensated FP32 layers."""
+
+    def __init__(self, weights: np.ndarray, architecture: int = 2, hidden: int = 64, device: int = 0):
+        self.lib = _lib.load()
+        self.architecture, self.hidden = architecture, hidden
+        self.inputs, self.outputs = POLICY_SHAPES.get(architecture, (0, 0))
+        w = np.ascontiguousarray(weights, np.float32).reshape(-1)
+        handle = C.c_void_p()
+        err = _lib.error_buffer()
+        _lib.check(self.lib.blitz_hip_policy_create(device, architecture, hidden,
+                                                    w.ctypes.data_as(C.POINTER(C.c_float)), w.size,
+                                                    C.byref(handle), err, len(err)), err)
+        self.handle = handle
+
+    def predict(self, x: np.ndarray) -> np.ndarray:
+        x = np.ascontiguousarray(x, np.float32).reshape(-1, self.inputs)
+        out = np.zeros((max(1, x.shape[0]), self.outputs), np.float32)
+        err = _lib.error_buffer()
+        _lib.check(self.lib.blitz_hip_policy_predict(self.handle, x.ctypes.data_as(C.POINTER(C.c_float)), x.shape[0],
+                                                     out.ctypes.data_as(C.POINTER(C.c_float)), out.size, err,
+                                                     len(err)), err)
+        return out[: x.shape[0]]
+
+    def close(self) -> None:
+        if self.handle:
+            self.lib.blitz_hip_policy_destroy(self.handle)
+            self.handle = None
+
+    def __enter__(self) -> "Policy":
+        return self
+
+    def __exit__(self, *exc) -> None:
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -493,6 +493,9 @@ __global__ __launch_bounds__(64) void claim_kernel(Topo t, const LegalAction* __
 // neighbour lists). A non-finite score is an error, as in
 // src/neural/action.cpp:477-479. -0 sorts as +0 so
 // that equal scores keep ascending action order, as a stable sort does.
+// rank_kernel mode of per-action policy outputs (rank_policy).
+constexpr uint32_t kPolicyScores = 5;
+
 struct RankInput {
     const LegalAction* legal;
     uint32_t count;
@@ -530,6 +533,10 @@ __global__ void rank_kernel(Topo t, RankInput in) {
                 hi = mid;
         }
         score = lo < end && in.graph_neighbors[lo] == a.to ? in.outputs[size_t(lo) * in.outputs_per_action] : NAN;
+    } else if (in.mode == kPolicyScores) {
+        // The policy's first output for this legal action
+        // (rank_actions, src/neural/action_gpu.cu:821-822).
+        score = in.outputs[size_t(i) * in.outputs_per_action];
     }
     if (!isfinite(score)) atomicExch(in.bad, 1u);
     in.keys[i] = score + 0.f;
@@ -880,15 +887,9 @@ void ActionSet::rank(uint32_t mode, double diameter, const Graph* graph, uint32_
     if (mode > 4) throw std::invalid_argument("unknown ranking");
     if (mode == 0 && (!graph || !graph->has_outputs)) throw std::invalid_argument("learned ranking needs scorer outputs");
     DeviceGuard guard(device_);
-    const hipStream_t st = stream_;
     const uint32_t n = legal_count_;
     ranked_ = 0;
     if (!n) return;
-    float* keys = grow<float>(rank_keys_, 2 * size_t(n));
-    uint32_t* idx = grow<uint32_t>(rank_idx_, n);
-    uint32_t* order = grow<uint32_t>(order_, n);
-    uint32_t* bad = grow<uint32_t>(claim_out_, 3 + 64);
-    hip_check(hipMemsetAsync(bad, 0, 4, st), "clear");
     RankInput in{};
     in.legal = legal_.as<LegalAction>();
     in.count = n;
@@ -901,6 +902,34 @@ void ActionSet::rank(uint32_t mode, double diameter, const Graph* graph, uint32_
         in.outputs_per_action = kOutputs;
         if (graph->vertices != vertices_) throw std::invalid_argument("scorer graph of another mesh");
     }
+    run_rank(&in, order_out);
+}
+
+void ActionSet::rank_scores(const float* scores, uint32_t stride, uint32_t* order_out) {
+    ranked_ = 0;
+    if (!legal_count_) return;
+    RankInput in{};
+    in.legal = legal_.as<LegalAction>();
+    in.count = legal_count_;
+    in.mode = kPolicyScores;
+    in.diameter = diameter_;
+    in.outputs = scores;
+    in.outputs_per_action = stride;
+    run_rank(&in, order_out);
+}
+
+// Scores every legal action with `in` (mode, inputs), sorts the order by
+// descending score (stable: ties keep enumeration order) and keeps it on the
+// device; a score that is not finite is an error.
+void ActionSet::run_rank(const void* input, uint32_t* order_out) {
+    RankInput in = *static_cast<const RankInput*>(input);
+    const hipStream_t st = stream_;
+    const uint32_t n = legal_count_;
+    float* keys = grow<float>(rank_keys_, 2 * size_t(n));
+    uint32_t* idx = grow<uint32_t>(rank_idx_, n);
+    uint32_t* order = grow<uint32_t>(order_, n);
+    uint32_t* bad = grow<uint32_t>(claim_out_, 3 + 64);
+    hip_check(hipMemsetAsync(bad, 0, 4, st), "clear");
     in.keys = keys;
     in.order = idx;
     in.bad = bad;

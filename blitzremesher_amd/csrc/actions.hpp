@@ -16,6 +16,8 @@
 
 namespace blitz_hip {
 
+class Policy;
+
 // One legal directed half-edge collapse `from` -> `to` (geometric vertices:
 // the smallest index among bit-equal positions) and its wedge mapping: in
 // every face holding both, index wedge_from[i] becomes wedge_to[i].
@@ -40,6 +42,14 @@ struct ActionTopo {
     const uint32_t* face_offsets;  // faces around each geometric vertex (ascending)
     const uint32_t* faces;
     uint32_t vertices, triangles;
+};
+
+// Host attribute streams of a state's input for the graph features (null
+// when absent): normals xyz, colours RGBA8 and tangents xyzw per vertex.
+struct VertexStreams {
+    const float* normals = nullptr;
+    const uint8_t* colors = nullptr;
+    const float* tangents = nullptr;
 };
 
 class ActionSet {
@@ -78,6 +88,22 @@ class ActionSet {
     void patches(const uint32_t* rows, size_t n, std::vector<uint32_t>& removed_offsets,
                  std::vector<uint32_t>& removed, std::vector<uint32_t>& added_offsets,
                  std::vector<uint32_t>& added);
+    // blitz::neural::graph (src/neural/graph.cpp:35-187) of the state's
+    // mesh: 24 channels per vertex, vertex-major.
+    void vertex_features(const VertexStreams& streams, std::vector<float>& out);
+    // ActionState::actions (src/neural/action.cpp:371-450): `width` (80, or
+    // 128 with the second wedge's vertex channels in 80..127,
+    // src/neural/action_gpu.cu:725-729) floats per legal action, in legal
+    // order, for the eight conditions; input_faces is the triangle count of
+    // the input the state descends from (0: this state's own).
+    void action_features(const VertexStreams& streams, const float* conditions, uint32_t width, uint64_t input_faces,
+                         std::vector<float>& rows);
+    // The learned order of the reference's GPU action state
+    // (src/neural/action_gpu.cu:1751-1798, 809-844): the policy's first
+    // output on every legal action's feature row (policy.inputs() wide;
+    // architecture 4 carries preserve_uv in feature 79), ranked as rank().
+    void rank_policy(Policy& policy, const VertexStreams& streams, const float* conditions, uint64_t input_faces,
+                     bool preserve_uv, uint32_t* order_out);
     double build_ms() const { return build_ms_; }
     // max(1e-20, diameter of the input's bounds), the scale of the
     // shortest-edge and current-plane scores (src/neural/action.cpp:29).
@@ -91,6 +117,14 @@ class ActionSet {
     uint32_t claim_device(const uint32_t* drows, size_t n, uint32_t maximum, uint64_t removable, bool strict,
                           std::vector<uint32_t>& picked);
     ActionTopo topo() const;
+    void graph_features_device(const VertexStreams& streams, float* device_out);
+    float* grow_vertex_features();
+    // Feature rows of every legal action on the device (feature_rows_);
+    // x79 < 0 keeps the target's channel 23 in feature 79.
+    float* action_rows_device(const VertexStreams& streams, const float* conditions, uint32_t width,
+                              uint64_t input_faces, float x79);
+    void rank_scores(const float* scores, uint32_t stride, uint32_t* order_out);
+    void run_rank(const void* input, uint32_t* order_out);
 
     int device_;
     hipStream_t stream_{};
@@ -113,6 +147,9 @@ class ActionSet {
     // Host positions the device weld was computed from.
     std::vector<float> weld_positions_;
     bool weld_ready_ = false;
+    // Graph and action feature workspaces.
+    DeviceBuffer feature_attr_, feature_work_, feature_vertex_, feature_rows_, policy_out_;
+    const uint32_t* feature_colors_ = nullptr;
 };
 
 // The gate of the executor: true when the candidate (index buffer, kept

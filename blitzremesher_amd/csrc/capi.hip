@@ -4,6 +4,7 @@
 // messages (src/c_api.cpp:26-32). No exception leaves this file.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstring>
 #include <memory>
 #include <new>
@@ -13,6 +14,7 @@
 
 #include "../../include/blitz_hip.h"
 #include "actions.hpp"
+#include "policy.hpp"
 #include "chain.hpp"
 #include "digest.hpp"
 #include "noalloc.hpp"
@@ -40,6 +42,16 @@ struct blitz_hip_binding {
 };
 struct blitz_hip_actions {
     std::unique_ptr<ActionSet> impl;
+    // Host copies of the input's attribute streams for the graph features.
+    std::vector<float> normals, tangents;
+    std::vector<uint8_t> colors;
+    VertexStreams streams() const {
+        return {normals.empty() ? nullptr : normals.data(), colors.empty() ? nullptr : colors.data(),
+                tangents.empty() ? nullptr : tangents.data()};
+    }
+};
+struct blitz_hip_policy {
+    std::unique_ptr<Policy> impl;
 };
 struct blitz_hip_chain {
     ChainResult result;
@@ -521,6 +533,14 @@ blitz_status blitz_hip_actions_create(int32_t device, const blitz_mesh* m, blitz
         }
         auto a = std::make_unique<blitz_hip_actions>();
         a->impl = std::make_unique<ActionSet>(device, h, uv.empty() ? nullptr : uv.data());
+        a->normals = h.normals;
+        a->colors = h.colors;
+        if (m->tangents.count) {
+            const size_t n = h.positions.size() / 3;
+            a->tangents.resize(4 * n);
+            const auto* base = static_cast<const unsigned char*>(m->tangents.data);
+            for (size_t i = 0; i < n; ++i) std::memcpy(&a->tangents[4 * i], base + i * m->tangents.stride, 16);
+        }
         *out = a.release();
     });
 }
@@ -541,6 +561,70 @@ blitz_status blitz_hip_actions_legal(blitz_hip_actions* a, uint32_t* pairs, size
         ActionSet& s = actions_of(a);
         if (!pairs || capacity < s.legal_count()) throw std::invalid_argument("legal action capacity too small");
         s.download_legal(pairs);
+    });
+}
+
+blitz_status blitz_hip_actions_vertex_features(blitz_hip_actions* a, float* out, size_t capacity, char* error,
+                                               size_t error_capacity) {
+    return guarded(error, error_capacity, [&] {
+        ActionSet& s = actions_of(a);
+        std::vector<float> x;
+        s.vertex_features(a->streams(), x);
+        if (!out || capacity < x.size()) throw std::invalid_argument("vertex feature capacity too small");
+        std::copy(x.begin(), x.end(), out);
+    });
+}
+
+blitz_status blitz_hip_actions_features(blitz_hip_actions* a, const float* conditions, uint32_t width,
+                                        uint64_t input_triangles, float* rows, size_t capacity, char* error,
+                                        size_t error_capacity) {
+    return guarded(error, error_capacity, [&] {
+        ActionSet& s = actions_of(a);
+        if (!conditions) throw std::invalid_argument("null conditions");
+        if (width != 80 && width != 128) throw std::invalid_argument("action feature width must be 80 or 128");
+        if (!rows || capacity < size_t(s.legal_count()) * width)
+            throw std::invalid_argument("action feature capacity too small");
+        std::vector<float> x;
+        s.action_features(a->streams(), conditions, width, input_triangles, x);
+        std::copy(x.begin(), x.end(), rows);
+    });
+}
+
+blitz_status blitz_hip_policy_create(int32_t device, uint32_t architecture, uint32_t hidden_width,
+                                     const float* weights, size_t count, blitz_hip_policy** out, char* error,
+                                     size_t capacity) {
+    if (out) *out = nullptr;
+    return guarded(error, capacity, [&] {
+        if (!out) throw std::invalid_argument("null policy output");
+        if (!blitz_hip_available(device)) throw Unavailable("no gfx950 HIP device at the requested index");
+        auto p = std::make_unique<blitz_hip_policy>();
+        p->impl = std::make_unique<Policy>(device, architecture, hidden_width, weights, count);
+        *out = p.release();
+    });
+}
+
+void blitz_hip_policy_destroy(blitz_hip_policy* p) { delete p; }
+
+blitz_status blitz_hip_policy_predict(blitz_hip_policy* p, const float* input, size_t rows, float* output,
+                                      size_t output_capacity, char* error, size_t capacity) {
+    return guarded(error, capacity, [&] {
+        if (!p || !p->impl) throw std::invalid_argument("null policy");
+        if ((!input && rows) || (!output && rows)) throw std::invalid_argument("null policy rows");
+        if (output_capacity < rows * p->impl->outputs()) throw std::invalid_argument("policy output capacity too small");
+        const auto out = p->impl->predict(input, rows);
+        std::copy(out.begin(), out.end(), output);
+    });
+}
+
+blitz_status blitz_hip_actions_rank_policy(blitz_hip_actions* a, blitz_hip_policy* p, const float* conditions,
+                                           uint64_t input_triangles, int32_t preserve_uv, uint32_t* order,
+                                           size_t capacity, char* error, size_t error_capacity) {
+    return guarded(error, error_capacity, [&] {
+        ActionSet& s = actions_of(a);
+        if (!p || !p->impl) throw std::invalid_argument("null policy");
+        if (!conditions) throw std::invalid_argument("null conditions");
+        if (order && capacity < s.legal_count()) throw std::invalid_argument("rank order capacity too small");
+        s.rank_policy(*p->impl, a->streams(), conditions, input_triangles, preserve_uv != 0, order);
     });
 }
 

@@ -182,7 +182,7 @@ __global__ __launch_bounds__(256) void aggregate_fp32(const float* __restrict__ 
 // the G folds running side by side. A block takes one group of G outputs and
 // kCompRows rows per eight-lane group; the G weight rows are staged in LDS
 // once per block. With `src` set, row r's input is [h[src[r]] ; h[dst[r]]].
-// out % G == 0, in <= kCompMaxIn.
+// out % G == 0, in <= kCompMaxIn (G = 1 takes any output count).
 constexpr uint32_t kCompMaxIn = 256, kCompRows = 4, kCompRowsPerBlock = 32 * kCompRows;
 static_assert(kEdgeIn <= kCompMaxIn && 2 * kHidden <= kCompMaxIn && 2 * kFeatures <= kCompMaxIn,
               "layer inputs must fit the staged weight rows");
@@ -598,15 +598,16 @@ void launch_aggregate_fp32(const float* h, GraphView g, uint32_t width, float* z
 void launch_compensated(const float* input, const uint32_t* src, const uint32_t* dst, const float* weights,
                         const float* bias, float* output, uint32_t rows, uint32_t in, uint32_t out, bool relu,
                         hipStream_t stream) {
-    if (out % 4) throw std::invalid_argument("compensated_linear: out must be a multiple of 4");
     if (in > kCompMaxIn) throw std::invalid_argument("compensated_linear: in must be at most 256");
-    const int per = out % 8 ? 4 : 8;
+    const int per = out % 8 == 0 ? 8 : out % 4 == 0 ? 4 : 1;
     if (!rows || !out) return;
     const dim3 grid = comp_grid(rows, out, per);
     if (per == 8)
         compensated_linear<8><<<grid, dim3(256), 0, stream>>>(input, src, dst, weights, bias, output, rows, in, out, relu);
-    else
+    else if (per == 4)
         compensated_linear<4><<<grid, dim3(256), 0, stream>>>(input, src, dst, weights, bias, output, rows, in, out, relu);
+    else
+        compensated_linear<1><<<grid, dim3(256), 0, stream>>>(input, src, dst, weights, bias, output, rows, in, out, relu);
 }
 
 static void configure_bf16() {

@@ -18,6 +18,16 @@ __device__ __forceinline__ double dot3(F3 a, F3 b) {
     return double(a.x) * b.x + double(a.y) * b.y + double(a.z) * b.z;
 }
 __device__ __forceinline__ double len3(F3 a) { return sqrt(dot3(a, a)); }
+__device__ __forceinline__ F3 sub3(F3 a, F3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ F3 add3(F3 a, F3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+// Vec3 * double: each product in double, rounded to float.
+__device__ __forceinline__ F3 scale3(F3 a, double s) { return {float(a.x * s), float(a.y * s), float(a.z * s)}; }
+// normalized(): a * (1 / length) when the length is positive, else zero.
+__device__ __forceinline__ F3 norm3(F3 a) {
+    const double l = len3(a);
+    return l > 0 ? scale3(a, 1 / l) : F3{0, 0, 0};
+}
+__device__ __forceinline__ F3 xyz(float4 p) { return {p.x, p.y, p.z}; }
 
 // Ranking scores of the plain orders (src/neural/action_gpu.cu:822-840,
 // CPU ActionState::teacher_cost src/neural/action.cpp:355-370): shortest

@@ -288,6 +288,56 @@ blitz_status blitz_hip_actions_patches(blitz_hip_actions*, const uint32_t* rows,
                                        uint32_t* added_offsets, uint32_t* added, size_t added_capacity,
                                        size_t* removed_count, size_t* added_count, char* error, size_t capacity);
 
+/* blitz::neural::graph (src/neural/graph.cpp:35-187) of the state's mesh:
+ * the 24 vertex channels (position over the bounds diameter, unit normal,
+ * colour, UV, tangent, stream presence, seam / boundary / material flags,
+ * log degree, log mean edge) per vertex, vertex-major; `capacity` in floats
+ * (at least 24 per vertex). The normals, colours and tangents streams are
+ * the ones given to blitz_hip_actions_create. */
+blitz_status blitz_hip_actions_vertex_features(blitz_hip_actions*, float* out, size_t capacity, char* error,
+                                               size_t error_capacity);
+/* ActionState::actions (src/neural/action.cpp:371-450; fill_action_features,
+ * src/neural/action_gpu.cu:644-730): the feature row of every legal action,
+ * in legal order, `width` floats each: 80 (the action scorer's input), or
+ * 128 for placement rows (channels 80..127: the second wedge's source and
+ * target vertex channels for two-wedge actions). `conditions` are the eight
+ * neural::condition values (src/neural/graph.cpp:253-263), which must be
+ * finite. `input_triangles` is the triangle count of the input the state
+ * descends from (channel 66 = current / input triangles); 0 takes the
+ * state's own. `capacity` in floats. */
+blitz_status blitz_hip_actions_features(blitz_hip_actions*, const float* conditions, uint32_t width,
+                                        uint64_t input_triangles, float* rows, size_t capacity, char* error,
+                                        size_t error_capacity);
+
+/* Action and placement policies: ActionCuda (src/neural/action.hpp:70-87,
+ * src/neural/inference.cu:183-290), three compensated FP32 layers (ReLU after
+ * the first two; the reference's compensated_linear_kernel arithmetic, so the
+ * outputs are bit-identical to it). architecture 2: 80 inputs, 3 outputs
+ * (rank, source, adjacent logits); 3 and 4: 128 inputs, 12 outputs
+ * (src/neural/placement.hpp:8-31). hidden_width 64, 128 or 256; weights
+ * float32 [out,in] then bias per layer, count = (in+1)*h + (h+1)*h +
+ * (h+1)*out, all finite. */
+typedef struct blitz_hip_policy blitz_hip_policy;
+blitz_status blitz_hip_policy_create(int32_t device, uint32_t architecture, uint32_t hidden_width,
+                                     const float* weights, size_t count, blitz_hip_policy**, char* error,
+                                     size_t capacity);
+void blitz_hip_policy_destroy(blitz_hip_policy*);
+/* ActionCuda::predict: `rows` input rows (host, finite) to rows x outputs
+ * floats; a non-finite output is BLITZ_INTERNAL_ERROR. */
+blitz_status blitz_hip_policy_predict(blitz_hip_policy*, const float* input, size_t rows, float* output,
+                                      size_t output_capacity, char* error, size_t capacity);
+/* Learned ranking of the reference's GPU action state (rank with a policy,
+ * src/neural/action_gpu.cu:1751-1798 and rank_actions 809-844): every legal
+ * action's feature row (blitz_hip_actions_features at the policy's input
+ * width; architecture 4 stores preserve_uv in feature 79) through the
+ * policy, ordered by its first output, descending, ties in enumeration
+ * order; the order stays on the device for later batch selection and is
+ * written to `order` (legal-action indices) when it is not null. One policy
+ * serves one thread at a time. */
+blitz_status blitz_hip_actions_rank_policy(blitz_hip_actions*, blitz_hip_policy*, const float* conditions,
+                                           uint64_t input_triangles, int32_t preserve_uv, uint32_t* order,
+                                           size_t capacity, char* error, size_t error_capacity);
+
 /* Work counters of blitz::neural::ActionStats (src/neural/action.hpp:62-67). */
 typedef struct blitz_hip_action_stats {
     uint32_t struct_size;

@@ -269,8 +269,10 @@ int ref_generate(const float* pos, uint64_t nv, const uint32_t* idx, uint64_t ni
 // ---- ActionState (src/neural/action.cpp): collapse legality, independent
 // batches and trial edits. The state owns copies of the streams it borrows.
 struct RefActions {
-    std::vector<Vec3> positions;
+    std::vector<Vec3> positions, normals;
     std::vector<Vec2> uv;
+    std::vector<ColorRGBA8> colors;
+    std::vector<Vec4> tangents;
     std::vector<uint32_t> indices;
     std::vector<uint16_t> materials;
     std::vector<uint8_t> double_sided;
@@ -299,6 +301,57 @@ void* ref_actions_create(const float* pos, uint64_t nv, const uint32_t* idx, uin
         return nullptr;
     }
 }
+// ActionState over a mesh with normal, colour and tangent streams too (null
+// when absent): its graph features (src/neural/graph.cpp) see seams,
+// colours and tangents.
+void* ref_actions_create_streams(const float* pos, uint64_t nv, const uint32_t* idx, uint64_t ni, const uint16_t* mat,
+                                 uint64_t nm, const float* uv, const float* normals, const uint8_t* colors,
+                                 const float* tangents) {
+    try {
+        auto r = std::make_unique<RefActions>();
+        r->positions.assign(reinterpret_cast<const Vec3*>(pos), reinterpret_cast<const Vec3*>(pos) + nv);
+        if (uv) r->uv.assign(reinterpret_cast<const Vec2*>(uv), reinterpret_cast<const Vec2*>(uv) + nv);
+        if (normals) r->normals.assign(reinterpret_cast<const Vec3*>(normals), reinterpret_cast<const Vec3*>(normals) + nv);
+        if (colors)
+            r->colors.assign(reinterpret_cast<const ColorRGBA8*>(colors), reinterpret_cast<const ColorRGBA8*>(colors) + nv);
+        if (tangents)
+            r->tangents.assign(reinterpret_cast<const Vec4*>(tangents), reinterpret_cast<const Vec4*>(tangents) + nv);
+        r->indices.assign(idx, idx + ni);
+        if (nm) r->materials.assign(mat, mat + nm);
+        MeshView v;
+        v.positions = Stream<Vec3>(r->positions.data(), nv);
+        if (uv) v.uv = Stream<Vec2>(r->uv.data(), nv);
+        if (normals) v.normals = Stream<Vec3>(r->normals.data(), nv);
+        if (colors) v.colors = Stream<ColorRGBA8>(r->colors.data(), nv);
+        if (tangents) v.tangents = Stream<Vec4>(r->tangents.data(), nv);
+        v.indices = {r->indices.data(), r->indices.size()};
+        v.materials = {r->materials.data(), r->materials.size()};
+        r->state = std::make_unique<neural::ActionState>(v);
+        r->rows = r->state->actions({});
+        return r.release();
+    } catch (...) {
+        return nullptr;
+    }
+}
+
+// neural::graph (src/neural/graph.cpp:35-187): 24 channels per vertex.
+int ref_graph_features(const float* pos, uint64_t nv, const uint32_t* idx, uint64_t ni, const uint16_t* mat,
+                       uint64_t nm, const float* uv, const float* normals, const uint8_t* colors,
+                       const float* tangents, float* out) {
+    try {
+        MeshView v = view_of(pos, nv, idx, ni, mat, nm, nullptr, 0);
+        if (uv) v.uv = Stream<Vec2>(reinterpret_cast<const Vec2*>(uv), nv);
+        if (normals) v.normals = Stream<Vec3>(reinterpret_cast<const Vec3*>(normals), nv);
+        if (colors) v.colors = Stream<ColorRGBA8>(reinterpret_cast<const ColorRGBA8*>(colors), nv);
+        if (tangents) v.tangents = Stream<Vec4>(reinterpret_cast<const Vec4*>(tangents), nv);
+        const neural::Graph g = neural::graph(v);
+        std::memcpy(out, g.x.data(), g.x.size() * 4);
+        return 0;
+    } catch (...) {
+        return 1;
+    }
+}
+
 void ref_actions_destroy(void* h) { delete static_cast<RefActions*>(h); }
 // Legal directed actions in the state's enumeration order (edges ascending,
 // a->b then b->a): from/to pairs.

@@ -197,6 +197,22 @@ def _compensated_linear(x: np.ndarray, W: np.ndarray, b: np.ndarray, relu: bool)
     return total
 
 
+def policy_forward(x: np.ndarray, weights: np.ndarray, hidden: int, outputs: int) -> np.ndarray:
+    """ActionCuda::predict (src/neural/inference.cu:236-290): three
+    compensated layers in[=x.shape[1]] -> hidden -> hidden -> outputs, ReLU
+    after the first two, weights [out,in] then bias per layer."""
+    x = np.ascontiguousarray(x, np.float32)
+    w = np.ascontiguousarray(weights, np.float32)
+    at = 0
+    for layer, (k_in, k_out) in enumerate(((x.shape[1], hidden), (hidden, hidden), (hidden, outputs))):
+        W = w[at:at + k_in * k_out].reshape(k_out, k_in)
+        b = w[at + k_in * k_out:at + (k_in + 1) * k_out]
+        at += (k_in + 1) * k_out
+        x = _compensated_linear(x, W, b, layer < 2)
+    assert at == w.size
+    return x
+
+
 def forward_fp32(x: np.ndarray, offsets, neighbors, acts, weights: np.ndarray) -> np.ndarray:
     """Bit-exact restatement of the library's deterministic FP32 mode."""
     f32 = np.float32

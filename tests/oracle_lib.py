@@ -347,15 +347,29 @@ class Actions:
     actions in enumeration order, greedy independent batches and trial edits.
     """
 
-    def __init__(self, kind: str, mesh: Mesh, uv: np.ndarray | None = None):
+    def __init__(self, kind: str, mesh: Mesh, uv: np.ndarray | None = None, tangents: np.ndarray | None = None):
         self.oracle = Oracle(kind)
         self.mesh = mesh
         self.uv = None if uv is None else np.ascontiguousarray(uv, np.float32).reshape(-1, 2)
-        f = self.oracle.fn("actions_create")
-        f.restype = C.c_void_p
-        f.argtypes = [C.POINTER(C.c_float), C.c_uint64, C.POINTER(C.c_uint32), C.c_uint64, C.POINTER(C.c_uint16),
-                      C.c_uint64, C.POINTER(C.c_uint8), C.c_uint64, C.POINTER(C.c_float)]
-        self.handle = f(*_mesh_args(mesh), _ptr(self.uv, C.c_float))
+        self.tangents = None if tangents is None else np.ascontiguousarray(tangents, np.float32).reshape(-1, 4)
+        if mesh.normals is not None or mesh.colors is not None or self.tangents is not None:
+            # The reference state over every attribute stream (the graph
+            # features see seams, colours and tangents).
+            assert kind == "ref", "attribute streams need the reference build"
+            f = self.oracle.fn("actions_create_streams")
+            f.restype = C.c_void_p
+            f.argtypes = [C.POINTER(C.c_float), C.c_uint64, C.POINTER(C.c_uint32), C.c_uint64,
+                          C.POINTER(C.c_uint16), C.c_uint64, C.POINTER(C.c_float), C.POINTER(C.c_float),
+                          C.POINTER(C.c_uint8), C.POINTER(C.c_float)]
+            self.handle = f(*_mesh_args(mesh)[:6], _ptr(self.uv, C.c_float), _ptr(mesh.normals, C.c_float),
+                            _ptr(mesh.colors, C.c_uint8), _ptr(self.tangents, C.c_float))
+        else:
+            f = self.oracle.fn("actions_create")
+            f.restype = C.c_void_p
+            f.argtypes = [C.POINTER(C.c_float), C.c_uint64, C.POINTER(C.c_uint32), C.c_uint64,
+                          C.POINTER(C.c_uint16), C.c_uint64, C.POINTER(C.c_uint8), C.c_uint64,
+                          C.POINTER(C.c_float)]
+            self.handle = f(*_mesh_args(mesh), _ptr(self.uv, C.c_float))
         if not self.handle:
             raise ValueError("action state rejected the mesh")
         n = 6 * mesh.triangles + 2
@@ -412,6 +426,21 @@ class Actions:
         assert f(self.handle, c.ctypes.data_as(C.POINTER(C.c_float)),
                  out.ctypes.data_as(C.POINTER(C.c_float))) == len(self.legal)
         return out[: len(self.legal)]
+
+    def ref_graph_features(self) -> np.ndarray:
+        """The reference build's neural::graph channels (n, 24) of this mesh."""
+        assert self.oracle.kind == "ref"
+        m = self.mesh
+        out = np.zeros((m.positions.shape[0], 24), np.float32)
+        f = self.oracle.fn("graph_features")
+        f.restype = C.c_int
+        f.argtypes = [C.POINTER(C.c_float), C.c_uint64, C.POINTER(C.c_uint32), C.c_uint64, C.POINTER(C.c_uint16),
+                      C.c_uint64, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_uint8),
+                      C.POINTER(C.c_float), C.POINTER(C.c_float)]
+        assert f(*_mesh_args(m)[:6], _ptr(self.uv, C.c_float), _ptr(m.normals, C.c_float),
+                 _ptr(m.colors, C.c_uint8), _ptr(self.tangents, C.c_float),
+                 out.ctypes.data_as(C.POINTER(C.c_float))) == 0
+        return out
 
     def scores(self, ranking: int, diameter: float) -> np.ndarray:
         assert self.oracle.kind == "oracle"
