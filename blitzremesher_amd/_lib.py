@@ -184,6 +184,12 @@ class ChainProposalC(C.Structure):
 
 
 _P = C.c_void_p
+
+
+class ModelInfoC(C.Structure):
+    _fields_ = [("struct_size", C.c_uint32), ("architecture", C.c_uint32), ("hidden_width", C.c_uint32),
+                ("use", C.c_uint32), ("weight_count", C.c_uint64), ("weights", C.POINTER(C.c_float)),
+                ("provenance_size", C.c_uint64), ("provenance", C.c_void_p)]
 _ERR = [C.c_char_p, C.c_size_t]
 
 # name -> (restype, argtypes). Every symbol include/blitz_hip.h declares.
@@ -242,6 +248,13 @@ PROTOTYPES = {
     "blitz_hip_policy_destroy": (None, [_P]),
     "blitz_hip_policy_predict": (C.c_int, [_P, C.POINTER(C.c_float), C.c_size_t, C.POINTER(C.c_float), C.c_size_t,
                                            *_ERR]),
+    "blitz_hip_model_load": (C.c_int, [C.c_char_p, C.POINTER(_P), *_ERR]),
+    "blitz_hip_model_destroy": (None, [_P]),
+    "blitz_hip_model_sha256": (C.c_char_p, [_P]),
+    "blitz_hip_model_info_of": (C.c_int, [_P, C.POINTER(ModelInfoC)]),
+    "blitz_hip_model_save": (C.c_int, [C.c_char_p, C.c_uint32, C.c_uint32, C.c_uint32, C.POINTER(C.c_float),
+                                       C.c_size_t, C.c_char_p, C.c_size_t, *_ERR]),
+    "blitz_hip_policy_from_model": (C.c_int, [C.c_int32, _P, C.POINTER(_P), *_ERR]),
     "blitz_hip_actions_rank_policy": (C.c_int, [_P, _P, C.POINTER(C.c_float), C.c_uint64, C.c_int32,
                                                 C.POINTER(C.c_uint32), C.c_size_t, *_ERR]),
     "blitz_hip_actions_features": (C.c_int, [_P, C.POINTER(C.c_float), C.c_uint32, C.c_uint64, C.POINTER(C.c_float),

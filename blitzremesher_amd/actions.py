@@ -194,6 +194,27 @@ class Policy:
                                                     C.byref(handle), err, len(err)), err)
         self.handle = handle
 
+    @classmethod
+    def from_model(cls, path: str, device: int = 0) -> "Policy":
+        """The policy of a BLZNET action or placement model file."""
+        self = cls.__new__(cls)
+        self.lib = _lib.load()
+        err = _lib.error_buffer()
+        model = C.c_void_p()
+        _lib.check(self.lib.blitz_hip_model_load(str(path).encode(), C.byref(model), err, len(err)), err)
+        try:
+            info = _lib.ModelInfoC()
+            info.struct_size = C.sizeof(info)
+            _lib.check(self.lib.blitz_hip_model_info_of(model, C.byref(info)))
+            self.architecture, self.hidden = int(info.architecture), int(info.hidden_width)
+            self.inputs, self.outputs = POLICY_SHAPES.get(self.architecture, (0, 0))
+            handle = C.c_void_p()
+            _lib.check(self.lib.blitz_hip_policy_from_model(device, model, C.byref(handle), err, len(err)), err)
+            self.handle = handle
+        finally:
+            self.lib.blitz_hip_model_destroy(model)
+        return self
+
     def predict(self, x: np.ndarray) -> np.ndarray:
         x = np.ascontiguousarray(x, np.float32).reshape(-1, self.inputs)
         out = np.zeros((max(1, x.shape[0]), self.outputs), np.float32)

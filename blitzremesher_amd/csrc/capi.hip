@@ -14,6 +14,7 @@
 
 #include "../../include/blitz_hip.h"
 #include "actions.hpp"
+#include "model_io.hpp"
 #include "policy.hpp"
 #include "chain.hpp"
 #include "digest.hpp"
@@ -52,6 +53,9 @@ struct blitz_hip_actions {
 };
 struct blitz_hip_policy {
     std::unique_ptr<Policy> impl;
+};
+struct blitz_hip_model {
+    ModelData data;
 };
 struct blitz_hip_chain {
     ChainResult result;
@@ -613,6 +617,62 @@ blitz_status blitz_hip_policy_predict(blitz_hip_policy* p, const float* input, s
         if (output_capacity < rows * p->impl->outputs()) throw std::invalid_argument("policy output capacity too small");
         const auto out = p->impl->predict(input, rows);
         std::copy(out.begin(), out.end(), output);
+    });
+}
+
+blitz_status blitz_hip_model_load(const char* path, blitz_hip_model** out, char* error, size_t capacity) {
+    if (out) *out = nullptr;
+    return guarded(error, capacity, [&] {
+        if (!path || !out) throw std::invalid_argument("null model path or destination");
+        auto m = std::make_unique<blitz_hip_model>();
+        m->data = load_model(path);
+        *out = m.release();
+    });
+}
+
+void blitz_hip_model_destroy(blitz_hip_model* m) { delete m; }
+
+const char* blitz_hip_model_sha256(const blitz_hip_model* m) { return m ? m->data.sha256.c_str() : nullptr; }
+
+blitz_status blitz_hip_model_info_of(const blitz_hip_model* m, blitz_hip_model_info* info) {
+    if (!m || !info || info->struct_size != sizeof(blitz_hip_model_info)) return BLITZ_INVALID_ARGUMENT;
+    info->architecture = m->data.architecture;
+    info->hidden_width = m->data.hidden_width;
+    info->use = m->data.use;
+    info->weight_count = m->data.values.size();
+    info->weights = m->data.values.data();
+    info->provenance_size = m->data.provenance.size();
+    info->provenance = m->data.provenance.data();
+    return BLITZ_OK;
+}
+
+blitz_status blitz_hip_model_save(const char* path, uint32_t architecture, uint32_t hidden_width, uint32_t use,
+                                  const float* weights, size_t count, const char* provenance,
+                                  size_t provenance_size, char* error, size_t capacity) {
+    return guarded(error, capacity, [&] {
+        if (!path || (!weights && count) || (!provenance && provenance_size))
+            throw std::invalid_argument("null model path, weights or provenance");
+        ModelData m;
+        m.architecture = architecture;
+        m.hidden_width = hidden_width;
+        m.use = use;
+        m.values.assign(weights, weights + count);
+        if (provenance_size) m.provenance.assign(provenance, provenance_size);
+        save_model(path, m);
+    });
+}
+
+blitz_status blitz_hip_policy_from_model(int32_t device, const blitz_hip_model* m, blitz_hip_policy** out,
+                                         char* error, size_t capacity) {
+    if (out) *out = nullptr;
+    return guarded(error, capacity, [&] {
+        if (!m || !out) throw std::invalid_argument("null model or policy destination");
+        if (!policy_inputs(m->data.architecture)) throw std::invalid_argument("model is not an action or placement policy");
+        if (!blitz_hip_available(device)) throw Unavailable("no gfx950 HIP device at the requested index");
+        auto p = std::make_unique<blitz_hip_policy>();
+        p->impl = std::make_unique<Policy>(device, m->data.architecture, m->data.hidden_width, m->data.values.data(),
+                                           m->data.values.size());
+        *out = p.release();
     });
 }
 

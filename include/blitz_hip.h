@@ -338,6 +338,38 @@ blitz_status blitz_hip_actions_rank_policy(blitz_hip_actions*, blitz_hip_policy*
                                            uint64_t input_triangles, int32_t preserve_uv, uint32_t* order,
                                            size_t capacity, char* error, size_t error_capacity);
 
+/* Neural model files: the reference's BLZNET01/02/03 format (load_weights /
+ * save_weights, src/neural/model_io.cpp:61-153): header, provenance text,
+ * float32 weights and a SHA-256 checksum trailer (64 hex digits). Loading
+ * checks size, magic, architecture / width / weight count, use scope,
+ * payload length and checksum, and rejects non-finite weights
+ * (BLITZ_INVALID_ARGUMENT, as the reference). */
+typedef struct blitz_hip_model blitz_hip_model;
+blitz_status blitz_hip_model_load(const char* path, blitz_hip_model**, char* error, size_t capacity);
+void blitz_hip_model_destroy(blitz_hip_model*);
+/* blitz_neural_model_sha256 (include/blitz/blitz.h:163-164): SHA-256 of the
+ * whole file, 64 lower-case hex digits, borrowed until destruction. */
+const char* blitz_hip_model_sha256(const blitz_hip_model*);
+typedef struct blitz_hip_model_info {
+    uint32_t struct_size;
+    uint32_t architecture;    /* 1 graph encoder, 2 action policy, 3 / 4 placement policies */
+    uint32_t hidden_width;    /* 64, 128 or 256 */
+    uint32_t use;             /* 0 unrestricted, 1 endpoint-reuse only (architecture 4) */
+    uint64_t weight_count;
+    const float* weights;     /* borrowed */
+    uint64_t provenance_size;
+    const char* provenance;   /* borrowed, not NUL-terminated */
+} blitz_hip_model_info;
+blitz_status blitz_hip_model_info_of(const blitz_hip_model*, blitz_hip_model_info*);
+/* save_weights: BLZNET01 unless the width is not 64 (BLZNET02) or the use
+ * is restricted (BLZNET03); written to a temporary file renamed over `path`. */
+blitz_status blitz_hip_model_save(const char* path, uint32_t architecture, uint32_t hidden_width, uint32_t use,
+                                  const float* weights, size_t count, const char* provenance,
+                                  size_t provenance_size, char* error, size_t capacity);
+/* The policy of a loaded action (2) or placement (3, 4) model. */
+blitz_status blitz_hip_policy_from_model(int32_t device, const blitz_hip_model*, blitz_hip_policy**, char* error,
+                                         size_t capacity);
+
 /* Work counters of blitz::neural::ActionStats (src/neural/action.hpp:62-67). */
 typedef struct blitz_hip_action_stats {
     uint32_t struct_size;
