@@ -186,6 +186,12 @@ class ChainProposalC(C.Structure):
 _P = C.c_void_p
 
 
+class PlacementOptionsC(C.Structure):
+    _fields_ = [("struct_size", C.c_uint32), ("preserve_uv", C.c_int32), ("quantized", C.c_int32),
+                ("low", C.c_float * 3), ("extent", C.c_float * 3), ("exact_bits", C.POINTER(C.c_uint32)),
+                ("exact_words", C.c_size_t)]
+
+
 class ModelInfoC(C.Structure):
     _fields_ = [("struct_size", C.c_uint32), ("architecture", C.c_uint32), ("hidden_width", C.c_uint32),
                 ("use", C.c_uint32), ("weight_count", C.c_uint64), ("weights", C.POINTER(C.c_float)),
@@ -248,6 +254,10 @@ PROTOTYPES = {
     "blitz_hip_policy_destroy": (None, [_P]),
     "blitz_hip_policy_predict": (C.c_int, [_P, C.POINTER(C.c_float), C.c_size_t, C.POINTER(C.c_float), C.c_size_t,
                                            *_ERR]),
+    "blitz_hip_placement_options_init": (C.c_int, [C.POINTER(PlacementOptionsC), C.c_size_t]),
+    "blitz_hip_actions_placements": (C.c_int, [_P, _P, C.POINTER(C.c_float), C.c_uint64,
+                                               C.POINTER(PlacementOptionsC), C.POINTER(C.c_float), C.c_size_t, *_ERR]),
+    "blitz_hip_actions_mapping": (C.c_int, [_P, C.POINTER(C.c_uint32), C.c_size_t, *_ERR]),
     "blitz_hip_model_load": (C.c_int, [C.c_char_p, C.POINTER(_P), *_ERR]),
     "blitz_hip_model_destroy": (None, [_P]),
     "blitz_hip_model_sha256": (C.c_char_p, [_P]),

@@ -100,6 +100,38 @@ class ActionSet:
                    err)
         return order[: len(self.legal)].copy()
 
+    def placements(self, policy: "Policy | None" = None, conditions=None, input_triangles: int = 0,
+                   preserve_uv: bool = False, quantization=None, exact_bits: np.ndarray | None = None) -> np.ndarray:
+        """Placed position and destination-wedge normals (legal, 9) of every legal
+        action; `quantization` = (low[3], extent[3]) snaps to its unorm16 grid."""
+        c = np.zeros(8, np.float32) if conditions is None else np.ascontiguousarray(conditions, np.float32)
+        o = _lib.PlacementOptionsC()
+        _lib.check(self.lib.blitz_hip_placement_options_init(C.byref(o), C.sizeof(o)))
+        o.preserve_uv = int(preserve_uv)
+        if quantization is not None:
+            o.quantized = 1
+            o.low[:] = [float(v) for v in quantization[0]]
+            o.extent[:] = [float(v) for v in quantization[1]]
+        bits = None
+        if exact_bits is not None:
+            bits = np.ascontiguousarray(exact_bits, np.uint32)
+            o.exact_bits = _u32p(bits)
+            o.exact_words = bits.size
+        out = np.zeros((max(1, len(self.legal)), 9), np.float32)
+        err = _lib.error_buffer()
+        _lib.check(self.lib.blitz_hip_actions_placements(self.handle, policy.handle if policy else None,
+                                                         c.ctypes.data_as(C.POINTER(C.c_float)), input_triangles,
+                                                         C.byref(o), out.ctypes.data_as(C.POINTER(C.c_float)),
+                                                         out.size, err, len(err)), err)
+        return out[: len(self.legal)]
+
+    def mapping(self) -> np.ndarray:
+        """(legal, 5): wedges, from0, to0, from1, to1 (0xFFFFFFFF when absent)."""
+        out = np.zeros((max(1, len(self.legal)), 5), np.uint32)
+        err = _lib.error_buffer()
+        _lib.check(self.lib.blitz_hip_actions_mapping(self.handle, _u32p(out), out.size, err, len(err)), err)
+        return out[: len(self.legal)]
+
     def independent(self, order: np.ndarray, maximum: int, maximum_removed: int) -> np.ndarray:
         """ActionState::independent over legal-action indices in rank order."""
         order = np.ascontiguousarray(order, np.uint32)

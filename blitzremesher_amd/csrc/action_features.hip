@@ -349,6 +349,100 @@ __global__ void action_rows_kernel(RowInput in) {
     }
 }
 
+// src/neural/vertex_storage.hpp:24-32: unorm16 position storage of a
+// quantization box (zero extent encodes as zero).
+__device__ __forceinline__ uint16_t pack_unorm16(float value, float low, float extent) {
+    if (extent == 0) return 0;
+    const double q = (double(value) - low) / extent * 65535.;
+    return uint16_t(q <= 0 ? 0 : q >= 65535 ? 65535 : floor(q + .5));
+}
+__device__ __forceinline__ float unpack_unorm16(uint16_t value, float low, float extent) {
+    return low + extent * (float(value) / 65535.f);
+}
+
+struct PlacementInput {
+    const LegalAction* legal;
+    uint32_t count;
+    const float4* positions;
+    const float* normals;     // xyz per vertex, or null
+    const float* prediction;  // `outputs` per action, or null (default placements)
+    uint32_t outputs;
+    bool supervise;           // add the predicted normal residuals
+    bool quantized;
+    float low[3], extent[3];
+    const uint32_t* exact;    // per-vertex bitmap, or null
+    float* out;               // position, normal 0, normal 1 per action
+    uint32_t* bad;
+};
+
+__device__ __forceinline__ bool exact_vertex(const uint32_t* bits, uint32_t i) {
+    return bits && ((bits[i / 32] >> (i % 32)) & 1u);
+}
+
+// decode_placements / default_placements (src/neural/action_gpu.cu:
+// 1157-1200) with grid_position and exact_edit (374-390): the edge midpoint
+// plus the clamped offset in edge lengths, snapped to the quantization grid
+// unless an endpoint wedge is exact; each destination wedge's unit normal,
+// plus the predicted residual when supervised. Without a prediction the
+// position is the destination vertex's.
+__global__ void placements_kernel(PlacementInput in) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= in.count) return;
+    const LegalAction a = in.legal[i];
+    float* o = in.out + size_t(i) * 9;
+    for (uint32_t j = 0; j < 9; ++j) o[j] = 0.f;
+    F3 position = xyz(in.positions[a.to]);
+    const float* y = in.prediction ? in.prediction + size_t(i) * in.outputs : nullptr;
+    if (y) {
+        for (uint32_t j = 0; j < in.outputs; ++j)
+            if (!isfinite(y[j])) {
+                atomicExch(in.bad, 1u);
+                return;
+            }
+        const F3 from = xyz(in.positions[a.from]), to = xyz(in.positions[a.to]);
+        const F3 middle = scale3(add3(from, to), .5);
+        const double length = len3(sub3(from, to));
+        const auto unit = [](float v) { return fminf(1.f, fmaxf(-1.f, v)); };
+        position = add3(middle, scale3(F3{unit(y[3]), unit(y[4]), unit(y[5])}, length));
+        bool exact = false;
+        for (uint32_t j = 0; j < a.wedges; ++j)
+            exact |= exact_vertex(in.exact, a.wedge_from[j]) || exact_vertex(in.exact, a.wedge_to[j]);
+        if (!exact && in.quantized) {
+            const bool inside = !(position.x < in.low[0] || position.y < in.low[1] || position.z < in.low[2] ||
+                                  position.x > in.low[0] + in.extent[0] || position.y > in.low[1] + in.extent[1] ||
+                                  position.z > in.low[2] + in.extent[2]);
+            if (inside)
+                position = {unpack_unorm16(pack_unorm16(position.x, in.low[0], in.extent[0]), in.low[0], in.extent[0]),
+                            unpack_unorm16(pack_unorm16(position.y, in.low[1], in.extent[1]), in.low[1], in.extent[1]),
+                            unpack_unorm16(pack_unorm16(position.z, in.low[2], in.extent[2]), in.low[2], in.extent[2])};
+        }
+    }
+    o[0] = position.x;
+    o[1] = position.y;
+    o[2] = position.z;
+    if (in.normals)
+        for (uint32_t j = 0; j < a.wedges; ++j) {
+            const float* n = in.normals + 3 * size_t(a.wedge_to[j]);
+            F3 normal = norm3(F3{n[0], n[1], n[2]});
+            if (y && in.supervise) normal = norm3(add3(normal, F3{y[6 + 3 * j], y[7 + 3 * j], y[8 + 3 * j]}));
+            o[3 + 3 * j] = normal.x;
+            o[4 + 3 * j] = normal.y;
+            o[5 + 3 * j] = normal.z;
+        }
+}
+
+__global__ void mapping_kernel(const LegalAction* __restrict__ legal, uint32_t count, uint32_t* out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    const LegalAction a = legal[i];
+    uint32_t* o = out + size_t(i) * 5;
+    o[0] = a.wedges;
+    o[1] = a.wedge_from[0];
+    o[2] = a.wedge_to[0];
+    o[3] = a.wedges > 1 ? a.wedge_from[1] : 0xFFFFFFFFu;
+    o[4] = a.wedges > 1 ? a.wedge_to[1] : 0xFFFFFFFFu;
+}
+
 // Sub-allocations of one workspace, each 256-byte aligned. With a null base
 // it only measures the layout.
 struct Carve {
@@ -363,31 +457,39 @@ struct Carve {
 
 }  // namespace
 
-void ActionSet::graph_features_device(const VertexStreams& streams, float* device_out) {
+void ActionSet::upload_attributes(const VertexStreams& streams) {
     const hipStream_t st = stream_;
-    const uint32_t n = vertices_, faces = faces_, corners = 3 * faces_;
-    // Attribute uploads.
+    const uint32_t n = vertices_;
     const size_t attr_bytes = (streams.normals ? 12 * size_t(n) : 0) + (streams.colors ? 4 * size_t(n) : 0) +
                               (streams.tangents ? 16 * size_t(n) : 0);
     feature_attr_.resize(std::max<size_t>(attr_bytes, 64));
     char* attr = static_cast<char*>(feature_attr_.p);
-    Attributes a{nullptr, has_uv_ ? uv_.as<float2>() : nullptr, nullptr, nullptr};
+    feature_tangents_ = nullptr;
+    feature_normals_ = nullptr;
+    feature_colors_ = nullptr;
     size_t at = 0;
     if (streams.tangents) {
         hip_check(hipMemcpyAsync(attr + at, streams.tangents, 16 * size_t(n), hipMemcpyHostToDevice, st), "upload tangents");
-        a.tangents = reinterpret_cast<const float4*>(attr + at);
+        feature_tangents_ = reinterpret_cast<const float*>(attr + at);
         at += 16 * size_t(n);
     }
     if (streams.normals) {
         hip_check(hipMemcpyAsync(attr + at, streams.normals, 12 * size_t(n), hipMemcpyHostToDevice, st), "upload normals");
-        a.normals = reinterpret_cast<const float*>(attr + at);
+        feature_normals_ = reinterpret_cast<const float*>(attr + at);
         at += 12 * size_t(n);
     }
     if (streams.colors) {
         hip_check(hipMemcpyAsync(attr + at, streams.colors, 4 * size_t(n), hipMemcpyHostToDevice, st), "upload colors");
-        a.colors = reinterpret_cast<const uint32_t*>(attr + at);
+        feature_colors_ = reinterpret_cast<const uint32_t*>(attr + at);
     }
-    feature_colors_ = a.colors;
+}
+
+void ActionSet::graph_features_device(const VertexStreams& streams, float* device_out) {
+    const hipStream_t st = stream_;
+    const uint32_t n = vertices_, faces = faces_, corners = 3 * faces_;
+    upload_attributes(streams);
+    const Attributes a{feature_normals_, has_uv_ ? uv_.as<float2>() : nullptr, feature_colors_,
+                       reinterpret_cast<const float4*>(feature_tangents_)};
     const Bounds b = bounds_of(weld_positions_.data(), n);
     const Frame frame{double(b.center.x), double(b.center.y), double(b.center.z), b.radius * 2};
 
@@ -570,6 +672,83 @@ void ActionSet::rank_policy(Policy& policy, const VertexStreams& streams, const 
     policy_out_.resize(size_t(legal_count_) * policy.outputs() * 4);
     policy.predict_device(rows, policy_out_.as<float>(), legal_count_, stream_);
     rank_scores(policy_out_.as<float>(), policy.outputs(), order_out);
+}
+
+void ActionSet::placements(Policy* policy, const VertexStreams& streams, const float* conditions,
+                           uint64_t input_faces, const PlacementOptions& options, std::vector<float>& out) {
+    if (policy && policy->device() != device_) throw std::invalid_argument("policy on another device");
+    if (policy && policy->architecture() != 3 && policy->architecture() != 4)
+        throw std::invalid_argument("placements need a placement policy (architecture 3 or 4)");
+    if (options.quantized)
+        for (int k = 0; k < 3; ++k)
+            if (!std::isfinite(options.low[k]) || !std::isfinite(options.extent[k]) || options.extent[k] < 0 ||
+                !std::isfinite(options.low[k] + options.extent[k]))
+                throw std::invalid_argument("invalid quantization box");
+    if (options.exact && options.exact_words < (size_t(vertices_) + 31) / 32)
+        throw std::invalid_argument("exact position bitmap shorter than the vertex count");
+    DeviceGuard guard(device_);
+    const hipStream_t st = stream_;
+    out.assign(size_t(legal_count_) * 9, 0.f);
+    PlacementInput in{};
+    if (policy) {
+        const float* rows = action_rows_device(streams, conditions, policy->inputs(), input_faces,
+                                               policy->architecture() == 4 ? (options.preserve_uv ? 1.f : 0.f) : -1.f);
+        policy_out_.resize(std::max<size_t>(size_t(legal_count_) * policy->outputs() * 4, 4));
+        if (legal_count_) policy->predict_device(rows, policy_out_.as<float>(), legal_count_, st);
+        in.prediction = policy_out_.as<float>();
+        in.outputs = policy->outputs();
+        // The reference supervises normals when any appearance condition is set
+        // (src/neural/action_gpu.cu:1780-1782).
+        in.supervise = conditions[3] != 0 || conditions[4] != 0 || conditions[5] != 0;
+    } else {
+        upload_attributes(streams);
+    }
+    if (!legal_count_) {
+        hip_check(hipStreamSynchronize(st), "placements");
+        return;
+    }
+    // Outputs, the exact bitmap and the error flag in one buffer.
+    const size_t exact_bytes = options.exact ? 4 * ((size_t(vertices_) + 31) / 32) : 0;
+    const size_t out_bytes = (out.size() * 4 + 255) & ~size_t(255);
+    placement_work_.resize(out_bytes + exact_bytes + 4);
+    char* base = static_cast<char*>(placement_work_.p);
+    in.out = reinterpret_cast<float*>(base);
+    uint32_t* exact = reinterpret_cast<uint32_t*>(base + out_bytes);
+    uint32_t* bad = reinterpret_cast<uint32_t*>(base + out_bytes + exact_bytes);
+    if (options.exact)
+        hip_check(hipMemcpyAsync(exact, options.exact, exact_bytes, hipMemcpyHostToDevice, st), "upload exact bits");
+    hip_check(hipMemsetAsync(bad, 0, 4, st), "clear");
+    in.legal = legal_.as<LegalAction>();
+    in.count = legal_count_;
+    in.positions = positions_.as<float4>();
+    in.normals = feature_normals_;
+    in.quantized = options.quantized;
+    for (int k = 0; k < 3; ++k) {
+        in.low[k] = options.low[k];
+        in.extent[k] = options.extent[k];
+    }
+    in.exact = options.exact ? exact : nullptr;
+    in.bad = bad;
+    placements_kernel<<<blocks(legal_count_), 256, 0, st>>>(in);
+    hip_check(hipGetLastError(), "placement launch");
+    uint32_t flag = 0;
+    hip_check(hipMemcpyAsync(&flag, bad, 4, hipMemcpyDeviceToHost, st), "download flag");
+    hip_check(hipMemcpyAsync(out.data(), in.out, out.size() * 4, hipMemcpyDeviceToHost, st), "download placements");
+    hip_check(hipStreamSynchronize(st), "placements");
+    if (flag) throw std::invalid_argument("nonfinite neural prediction");
+}
+
+void ActionSet::mapping(std::vector<uint32_t>& out) {
+    DeviceGuard guard(device_);
+    out.assign(size_t(legal_count_) * 5, 0);
+    if (!legal_count_) return;
+    placement_work_.resize(out.size() * 4);
+    mapping_kernel<<<blocks(legal_count_), 256, 0, stream_>>>(legal_.as<LegalAction>(), legal_count_,
+                                                             placement_work_.as<uint32_t>());
+    hip_check(hipGetLastError(), "mapping launch");
+    hip_check(hipMemcpyAsync(out.data(), placement_work_.p, out.size() * 4, hipMemcpyDeviceToHost, stream_),
+              "download mapping");
+    hip_check(hipStreamSynchronize(stream_), "mapping");
 }
 
 }  // namespace blitz_hip

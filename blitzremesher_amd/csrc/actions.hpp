@@ -52,6 +52,17 @@ struct VertexStreams {
     const float* tangents = nullptr;
 };
 
+// Placement decoding options (src/neural/device_view.hpp:18-33): the
+// trajectory's quantization box (positions snap to its unorm16 grid) and the
+// host bitmap of exact-position vertices (edits touching one never snap).
+struct PlacementOptions {
+    bool preserve_uv = false;  // architecture 4's feature 79
+    bool quantized = false;
+    float low[3] = {0, 0, 0}, extent[3] = {0, 0, 0};
+    const uint32_t* exact = nullptr;
+    size_t exact_words = 0;
+};
+
 class ActionSet {
   public:
     // Builds the topology of `m` (optional per-vertex UVs, 2 floats each) and
@@ -104,6 +115,16 @@ class ActionSet {
     // architecture 4 carries preserve_uv in feature 79), ranked as rank().
     void rank_policy(Policy& policy, const VertexStreams& streams, const float* conditions, uint64_t input_faces,
                      bool preserve_uv, uint32_t* order_out);
+    // decode_placements / default_placements (src/neural/action_gpu.cu:
+    // 1157-1200): 9 floats per legal action (position, then the unit normal
+    // of each destination wedge; zeros without normals or a second wedge),
+    // from a placement policy (architecture 3 or 4) on the action rows, or
+    // the destination vertex without one.
+    void placements(Policy* policy, const VertexStreams& streams, const float* conditions, uint64_t input_faces,
+                    const PlacementOptions& options, std::vector<float>& out);
+    // Wedge mapping of every legal action: wedges, then (from, to) index
+    // pairs in ascending order, 0xFFFFFFFF for an absent second pair.
+    void mapping(std::vector<uint32_t>& out);
     double build_ms() const { return build_ms_; }
     // max(1e-20, diameter of the input's bounds), the scale of the
     // shortest-edge and current-plane scores (src/neural/action.cpp:29).
@@ -117,6 +138,7 @@ class ActionSet {
     uint32_t claim_device(const uint32_t* drows, size_t n, uint32_t maximum, uint64_t removable, bool strict,
                           std::vector<uint32_t>& picked);
     ActionTopo topo() const;
+    void upload_attributes(const VertexStreams& streams);
     void graph_features_device(const VertexStreams& streams, float* device_out);
     float* grow_vertex_features();
     // Feature rows of every legal action on the device (feature_rows_);
@@ -148,8 +170,9 @@ class ActionSet {
     std::vector<float> weld_positions_;
     bool weld_ready_ = false;
     // Graph and action feature workspaces.
-    DeviceBuffer feature_attr_, feature_work_, feature_vertex_, feature_rows_, policy_out_;
+    DeviceBuffer feature_attr_, feature_work_, feature_vertex_, feature_rows_, policy_out_, placement_work_;
     const uint32_t* feature_colors_ = nullptr;
+    const float *feature_normals_ = nullptr, *feature_tangents_ = nullptr;
 };
 
 // The gate of the executor: true when the candidate (index buffer, kept

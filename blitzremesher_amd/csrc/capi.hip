@@ -620,6 +620,52 @@ blitz_status blitz_hip_policy_predict(blitz_hip_policy* p, const float* input, s
     });
 }
 
+blitz_status blitz_hip_placement_options_init(blitz_hip_placement_options* o, size_t size) {
+    if (!o || size != sizeof(*o)) return BLITZ_INVALID_ARGUMENT;
+    *o = blitz_hip_placement_options{};
+    o->struct_size = sizeof(*o);
+    return BLITZ_OK;
+}
+
+blitz_status blitz_hip_actions_placements(blitz_hip_actions* a, blitz_hip_policy* p, const float* conditions,
+                                          uint64_t input_triangles, const blitz_hip_placement_options* o,
+                                          float* out, size_t capacity, char* error, size_t error_capacity) {
+    return guarded(error, error_capacity, [&] {
+        ActionSet& s = actions_of(a);
+        if (p && !p->impl) throw std::invalid_argument("null policy");
+        if (p && !conditions) throw std::invalid_argument("null conditions");
+        if (o && o->struct_size != sizeof(*o)) throw std::invalid_argument("placement options size mismatch");
+        if (!out || capacity < size_t(s.legal_count()) * 9) throw std::invalid_argument("placement capacity too small");
+        PlacementOptions options;
+        if (o) {
+            options.preserve_uv = o->preserve_uv != 0;
+            options.quantized = o->quantized != 0;
+            for (int k = 0; k < 3; ++k) {
+                options.low[k] = o->low[k];
+                options.extent[k] = o->extent[k];
+            }
+            options.exact = o->exact_bits;
+            options.exact_words = o->exact_words;
+        }
+        const float zero[8] = {};
+        std::vector<float> x;
+        s.placements(p ? p->impl.get() : nullptr, a->streams(), conditions ? conditions : zero, input_triangles,
+                     options, x);
+        std::copy(x.begin(), x.end(), out);
+    });
+}
+
+blitz_status blitz_hip_actions_mapping(blitz_hip_actions* a, uint32_t* out, size_t capacity, char* error,
+                                       size_t error_capacity) {
+    return guarded(error, error_capacity, [&] {
+        ActionSet& s = actions_of(a);
+        if (!out || capacity < size_t(s.legal_count()) * 5) throw std::invalid_argument("mapping capacity too small");
+        std::vector<uint32_t> x;
+        s.mapping(x);
+        std::copy(x.begin(), x.end(), out);
+    });
+}
+
 blitz_status blitz_hip_model_load(const char* path, blitz_hip_model** out, char* error, size_t capacity) {
     if (out) *out = nullptr;
     return guarded(error, capacity, [&] {

@@ -338,6 +338,37 @@ blitz_status blitz_hip_actions_rank_policy(blitz_hip_actions*, blitz_hip_policy*
                                            uint64_t input_triangles, int32_t preserve_uv, uint32_t* order,
                                            size_t capacity, char* error, size_t error_capacity);
 
+/* Placement decoding (decode_placements / default_placements,
+ * src/neural/action_gpu.cu:1157-1200; grid_position 380-390): the
+ * trajectory's quantization box (positions inside it snap to its unorm16
+ * grid, src/neural/vertex_storage.hpp:24-32) and the bitmap of exact-position
+ * vertices (an edit touching one never snaps). */
+typedef struct blitz_hip_placement_options {
+    uint32_t struct_size;
+    int32_t preserve_uv;          /* architecture 4's feature 79 */
+    int32_t quantized;            /* snap to the grid of [low, low + extent] */
+    float low[3], extent[3];
+    const uint32_t* exact_bits;   /* bit i of word i / 32: vertex i is exact; or null */
+    size_t exact_words;
+} blitz_hip_placement_options;
+blitz_status blitz_hip_placement_options_init(blitz_hip_placement_options*, size_t size);
+/* 9 floats per legal action: the placed position, then the unit normal of
+ * each destination wedge (zeros without normals or a second wedge). With a
+ * placement policy (architecture 3 or 4): the edge midpoint plus the policy's
+ * clamped offset (outputs 3..5, in edge lengths), snapped; normals plus the
+ * predicted residuals (outputs 6..11) when any of conditions[3..5] is
+ * non-zero, as the reference supervises them. Without a policy
+ * (policy = NULL): the destination vertex and its wedges' unit normals. */
+blitz_status blitz_hip_actions_placements(blitz_hip_actions*, blitz_hip_policy*, const float* conditions,
+                                          uint64_t input_triangles, const blitz_hip_placement_options*,
+                                          float* out, size_t capacity, char* error, size_t error_capacity);
+/* Wedge mapping of every legal action (the remap of ActionState::mapping,
+ * src/neural/action.cpp:146-215): 5 words each, the wedge count then the
+ * (from, to) vertex index pairs in ascending order, 0xFFFFFFFF for an absent
+ * second pair. */
+blitz_status blitz_hip_actions_mapping(blitz_hip_actions*, uint32_t* out, size_t capacity, char* error,
+                                       size_t error_capacity);
+
 /* Neural model files: the reference's BLZNET01/02/03 format (load_weights /
  * save_weights, src/neural/model_io.cpp:61-153): header, provenance text,
  * float32 weights and a SHA-256 checksum trailer (64 hex digits). Loading

@@ -334,6 +334,64 @@ def decode_placements(positions: np.ndarray, acts: np.ndarray, outputs: np.ndarr
     return _add3(middle, _mul3(y, length[:, None]))
 
 
+def pack_unorm16(value, low, extent) -> int:
+    """src/neural/vertex_storage.hpp:24-29: (value - low) / extent * 65535 in
+    double, rounded half up and clamped; zero extent encodes as zero."""
+    if _f32(extent) == 0:
+        return 0
+    q = (float(_f32(value)) - float(_f32(low))) / float(_f32(extent)) * 65535.0
+    return 0 if q <= 0 else 65535 if q >= 65535 else int(np.floor(q + 0.5))
+
+
+def unpack_unorm16(v: int, low, extent) -> np.float32:
+    """src/neural/vertex_storage.hpp:30-32: low + extent * (v / 65535), float."""
+    return _f32(_f32(low) + _f32(extent) * (_f32(v) / _f32(65535)))
+
+
+def grid_position(p: np.ndarray, low, extent) -> np.ndarray:
+    """src/neural/action_gpu.cu:380-390: positions inside the box snap to its
+    unorm16 grid, others are kept."""
+    p = p.astype(_f32)
+    low, extent = np.asarray(low, _f32), np.asarray(extent, _f32)
+    if (p < low).any() or (p > (low + extent).astype(_f32)).any():
+        return p
+    return np.array([unpack_unorm16(pack_unorm16(p[k], low[k], extent[k]), low[k], extent[k]) for k in range(3)],
+                    _f32)
+
+
+def policy_placements(positions, normals, legal, mapping, y=None, supervise=False, quantization=None,
+                      exact_bits=None) -> np.ndarray:
+    """decode_placements / default_placements (src/neural/action_gpu.cu:
+    1157-1200) for every legal action (from, to) with its wedge mapping
+    (wedges, from0, to0, from1, to1): 9 floats, position then each
+    destination wedge's unit normal."""
+    p = positions.astype(_f32)
+    out = np.zeros((len(legal), 9), _f32)
+
+    def exact(v):
+        return exact_bits is not None and bool((int(exact_bits[v // 32]) >> (v % 32)) & 1)
+
+    for i, (u, v) in enumerate(legal):
+        w = int(mapping[i, 0])
+        pairs = [(int(mapping[i, 1]), int(mapping[i, 2])), (int(mapping[i, 3]), int(mapping[i, 4]))][:w]
+        pos = p[v]
+        if y is not None:
+            middle = _mul3(_add3(p[u], p[v]), 0.5)
+            length = _len3(_sub3(p[u], p[v]))
+            off = np.minimum(_f32(1), np.maximum(_f32(-1), y[i, 3:6].astype(_f32)))
+            pos = _add3(middle, _mul3(off, length))
+            if quantization is not None and not any(exact(s) or exact(t) for s, t in pairs):
+                pos = grid_position(pos, *quantization)
+        out[i, :3] = pos
+        if normals is not None:
+            for j, (_, t) in enumerate(pairs):
+                n = _norm3(normals[t].astype(_f32))
+                if y is not None and supervise:
+                    n = _norm3(_add3(n, y[i, 6 + 3 * j:9 + 3 * j].astype(_f32)))
+                out[i, 3 + 3 * j:6 + 3 * j] = n
+    return out
+
+
 def teacher_placements(positions: np.ndarray, indices: np.ndarray, face_offsets, faces, acts: np.ndarray):
     """src/neural/action_gpu.cu:1304-1373 (positions, no grid snapping, no
     normals) for each action: 10 candidate positions (from, to, middle, the
