@@ -467,6 +467,8 @@ def chain_leg(repeats: int, cpu_reference: dict | None) -> dict:
             "gate_audits": i["gate_audits"], "offer_audits": int(sum(i["audit_evaluations"])),
             "confirm_audits": i["confirm_audits"], "finalists": len(r.finalists), "status": r.status,
             "stage_ms": {k: round(i[f"{k}_ms"], 1) for k in ("executor", "gate", "offer", "confirm", "tiebreak")},
+            "executor_stage_ms": {k: round(i[f"{k}_ms"], 1) for k in ("build", "rank", "select", "trial",
+                                                                      "speculative")},
             "levels_passed": all(l.source.passed and l.adjacent.passed for l in r.lods[1:]),
             "cpu_reference_classical": cpu_reference}
 

@@ -169,7 +169,9 @@ class ChainInfoC(C.Structure):
         ("bounded_audits", C.c_uint32), ("finalists", C.c_uint32), ("selected", C.c_uint32),
         ("proposals", C.c_uint32), ("gate_audits", C.c_uint64), ("confirm_audits", C.c_uint64),
         ("ms", C.c_double), ("executor_ms", C.c_double), ("gate_ms", C.c_double), ("offer_ms", C.c_double),
-        ("confirm_ms", C.c_double), ("tiebreak_ms", C.c_double),
+        ("confirm_ms", C.c_double), ("tiebreak_ms", C.c_double), ("build_ms", C.c_double), ("rank_ms", C.c_double),
+        ("select_ms", C.c_double), ("trial_ms", C.c_double), ("speculative_ms", C.c_double),
+        ("speculative_build_ms", C.c_double),
     ]
 
 

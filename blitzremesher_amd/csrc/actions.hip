@@ -28,6 +28,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cstring>
+#include <future>
 #include <hipcub/device/device_radix_sort.hpp>
 #include <hipcub/device/device_run_length_encode.hpp>
 #include <hipcub/device/device_scan.hpp>
@@ -37,7 +38,6 @@ namespace blitz_hip {
 namespace {
 
 constexpr uint32_t kNone = 0xFFFFFFFFu;
-constexpr unsigned long long kNoEdge = ~0ull;
 
 using Topo = ActionTopo;
 
@@ -76,45 +76,177 @@ __global__ void weld_assign_kernel(const uint32_t* __restrict__ order, const uin
     if (k < n) geometry[order[k]] = order[head[k]];
 }
 
-// src/neural/action.cpp:58-76: per corner, its geometric vertex and face,
-// the face-count histogram, degenerate corners and undirected edge keys.
-__global__ void corners_kernel(const uint32_t* __restrict__ indices, const uint32_t* __restrict__ geometry,
-                               uint32_t corners, uint32_t* vertex, uint32_t* face, uint32_t* face_count,
-                               uint8_t* invalid, unsigned long long* keys) {
+// ---- Topology by counting (src/neural/action.cpp:58-104 builds the same
+// lists by sorting): per-vertex face and adjacency lists are filled by
+// atomic cursors and then each list is sorted by its own thread, which
+// gives the reference's ascending orders without a device-wide sort.
+
+// Each corner adds its face to its geometric vertex's face list and, when
+// its edge is not degenerate, each end to the other's adjacency list; a
+// degenerate corner marks the face's vertices invalid (action.cpp:63-68).
+__global__ void topo_count_kernel(const uint32_t* __restrict__ indices, const uint32_t* __restrict__ geometry,
+                                  uint32_t corners, uint32_t* face_count, uint32_t* adj_count, uint8_t* invalid) {
     const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= corners) return;
     const uint32_t f = c / 3, j = c % 3;
     const uint32_t a = geometry[indices[c]], b = geometry[indices[3 * f + (j + 1) % 3]];
-    vertex[c] = a;
-    face[c] = f;
     atomicAdd(&face_count[a], 1u);
     if (a == b) {
         for (uint32_t t = 0; t < 3; ++t) invalid[geometry[indices[3 * f + t]]] = 1;
-        keys[c] = kNoEdge;
         return;
     }
-    keys[c] = (static_cast<unsigned long long>(min(a, b)) << 32) | max(a, b);
+    atomicAdd(&adj_count[a], 1u);
+    atomicAdd(&adj_count[b], 1u);
 }
 
-// src/neural/action.cpp:77-92
-__global__ void edge_flags_kernel(const unsigned long long* __restrict__ keys, const uint32_t* __restrict__ counts,
-                                  uint32_t edges, uint8_t* boundary, uint8_t* invalid, uint32_t* degree,
-                                  unsigned long long* directed) {
-    const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= edges) return;
-    const uint32_t a = uint32_t(keys[e] >> 32), b = uint32_t(keys[e]);
-    if (counts[e] == 1) boundary[a] = boundary[b] = 1;
-    if (counts[e] > 2) invalid[a] = invalid[b] = 1;
-    atomicAdd(&degree[a], 1u);
-    atomicAdd(&degree[b], 1u);
-    directed[2 * e] = (static_cast<unsigned long long>(a) << 32) | b;
-    directed[2 * e + 1] = (static_cast<unsigned long long>(b) << 32) | a;
+__global__ void topo_fill_kernel(const uint32_t* __restrict__ indices, const uint32_t* __restrict__ geometry,
+                                 uint32_t corners, const uint32_t* __restrict__ face_offsets,
+                                 const uint32_t* __restrict__ adj_offsets, uint32_t* face_cursor, uint32_t* adj_cursor,
+                                 uint32_t* faces, uint32_t* adj) {
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= corners) return;
+    const uint32_t f = c / 3, j = c % 3;
+    const uint32_t a = geometry[indices[c]], b = geometry[indices[3 * f + (j + 1) % 3]];
+    faces[face_offsets[a] + atomicAdd(&face_cursor[a], 1u)] = f;
+    if (a == b) return;
+    adj[adj_offsets[a] + atomicAdd(&adj_cursor[a], 1u)] = b;
+    adj[adj_offsets[b] + atomicAdd(&adj_cursor[b], 1u)] = a;
 }
 
-__global__ void low_words_kernel(const unsigned long long* __restrict__ keys, uint32_t n, uint32_t* out) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) out[i] = uint32_t(keys[i]);
+// Ascending in place: insertion sort for short lists, heap sort otherwise
+// (a vertex shared by very many faces stays O(d log d)).
+__device__ void sort_list(uint32_t* p, uint32_t n) {
+    if (n <= 32) {
+        for (uint32_t i = 1; i < n; ++i) {
+            const uint32_t v = p[i];
+            uint32_t k = i;
+            for (; k > 0 && p[k - 1] > v; --k) p[k] = p[k - 1];
+            p[k] = v;
+        }
+        return;
+    }
+    const auto sift = [&](uint32_t root, uint32_t end) {
+        for (;;) {
+            uint32_t child = 2 * root + 1;
+            if (child >= end) return;
+            if (child + 1 < end && p[child] < p[child + 1]) ++child;
+            if (!(p[root] < p[child])) return;
+            const uint32_t t = p[root];
+            p[root] = p[child];
+            p[child] = t;
+            root = child;
+        }
+    };
+    for (uint32_t start = n / 2; start-- > 0;) sift(start, n);
+    for (uint32_t end = n - 1; end > 0; --end) {
+        const uint32_t t = p[0];
+        p[0] = p[end];
+        p[end] = t;
+        sift(0, end);
+    }
 }
+
+// Per vertex u: its face list ascending (the reference fills it in face
+// order), its adjacency sorted; each run of equal neighbours is one edge
+// whose length is the edge's face count (action.cpp:77-92: one face marks
+// both ends boundary, more than two marks them invalid; each end sets its
+// own flag here). degree = distinct neighbours, up = those above u.
+__global__ void topo_vertex_kernel(uint32_t n, const uint32_t* __restrict__ face_offsets, uint32_t* faces,
+                                   const uint32_t* __restrict__ adj_offsets, uint32_t* adj, uint32_t* degree,
+                                   uint32_t* up, uint8_t* boundary, uint8_t* invalid) {
+    const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= n) return;
+    sort_list(faces + face_offsets[u], face_offsets[u + 1] - face_offsets[u]);
+    uint32_t* a = adj + adj_offsets[u];
+    const uint32_t m = adj_offsets[u + 1] - adj_offsets[u];
+    sort_list(a, m);
+    uint32_t distinct = 0, above = 0;
+    bool open = false, bad = false;
+    for (uint32_t i = 0; i < m;) {
+        uint32_t k = i + 1;
+        while (k < m && a[k] == a[i]) ++k;
+        ++distinct;
+        above += a[i] > u;
+        open |= k - i == 1;
+        bad |= k - i > 2;
+        i = k;
+    }
+    degree[u] = distinct;
+    up[u] = above;
+    if (open) boundary[u] = 1;
+    if (bad) invalid[u] = 1;
+}
+
+// Distinct sorted neighbours into the CSR lists and the edges (u, v > u),
+// ascending, as keys (u << 32) | v.
+__global__ void topo_emit_kernel(uint32_t n, const uint32_t* __restrict__ adj_offsets, const uint32_t* __restrict__ adj,
+                                 const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ edge_offsets,
+                                 uint32_t* neighbors, unsigned long long* edges) {
+    const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= n) return;
+    const uint32_t* a = adj + adj_offsets[u];
+    const uint32_t m = adj_offsets[u + 1] - adj_offsets[u];
+    uint32_t w = offsets[u], e = edge_offsets[u];
+    for (uint32_t i = 0; i < m; ++i) {
+        if (i && a[i] == a[i - 1]) continue;
+        neighbors[w++] = a[i];
+        if (a[i] > u) edges[e++] = (static_cast<unsigned long long>(u) << 32) | a[i];
+    }
+}
+
+// Exclusive prefix sums of one or two uint32 arrays of `count` entries in
+// one workgroup: tiles of 4,096 entries, four per thread scanned in
+// registers, the waves' totals combined through LDS, the running total
+// carried from tile to tile. Sized for per-state topology arrays.
+__global__ __launch_bounds__(1024) void scan_kernel(const uint32_t* __restrict__ a_in, uint32_t* a_out,
+                                                    const uint32_t* __restrict__ b_in, uint32_t* b_out,
+                                                    uint32_t count) {
+    __shared__ uint32_t wave_total[2][16];
+    __shared__ uint32_t carry[2];
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (threadIdx.x < 2) carry[threadIdx.x] = 0;
+    __syncthreads();
+    const int arrays = b_in ? 2 : 1;
+    for (uint32_t base = 0; base < count; base += 4096) {
+        uint32_t v[2][4], lanes[2];
+        for (int r = 0; r < arrays; ++r) {
+            const uint32_t* in = r ? b_in : a_in;
+            uint32_t sum = 0;
+            for (int k = 0; k < 4; ++k) {
+                const uint32_t i = base + 4 * threadIdx.x + k;
+                v[r][k] = sum;
+                sum += i < count ? in[i] : 0u;
+            }
+            // Inclusive scan of the threads' sums within the wave.
+            uint32_t x = sum;
+            for (uint32_t d = 1; d < 64; d <<= 1) {
+                const uint32_t y = __shfl_up(x, d);
+                if (lane >= d) x += y;
+            }
+            lanes[r] = x - sum;
+            if (lane == 63) wave_total[r][wave] = x;
+        }
+        __syncthreads();
+        for (int r = 0; r < arrays; ++r) {
+            uint32_t before = carry[r];
+            for (uint32_t w = 0; w < wave; ++w) before += wave_total[r][w];
+            uint32_t* out = r ? b_out : a_out;
+            for (int k = 0; k < 4; ++k) {
+                const uint32_t i = base + 4 * threadIdx.x + k;
+                if (i < count) out[i] = before + lanes[r] + v[r][k];
+            }
+        }
+        __syncthreads();
+        if (threadIdx.x < uint32_t(arrays)) {
+            uint32_t total = 0;
+            for (uint32_t w = 0; w < 16; ++w) total += wave_total[threadIdx.x][w];
+            carry[threadIdx.x] += total;
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void set_u32_kernel(uint32_t* p, uint32_t v) { *p = v; }
 
 __device__ uint32_t find_neighbor(const Topo& t, uint32_t u, uint32_t v) {
     uint32_t lo = t.offsets[u], hi = t.offsets[u + 1];
@@ -662,7 +794,7 @@ void ActionSet::build(const HostMesh& m, const float* uv) {
     if (!same_positions) diameter_ = std::max(1e-20, bounds_of(m.positions.data(), n).radius * 2);
     // Every buffer of the build is sized here, before any launch, so nothing
     // is reallocated under queued work; the buffers persist across rebuilds.
-    DeviceBuffer &weld = build_weld_, &weld2 = build_weld2_, &corner_buf = build_corners_, &key_buf = build_keys_,
+    DeviceBuffer &weld = build_weld_, &weld2 = build_weld2_, &corner_buf = build_corners_,
                  &temp = build_temp_, &all_buf = build_all_, &flag_buf = build_flags_;
     uint32_t* kx = grow<uint32_t>(weld, 6 * size_t(n));
     uint32_t* ky = kx + n;
@@ -672,28 +804,37 @@ void ActionSet::build(const HostMesh& m, const float* uv) {
     uint32_t* ord = key_tmp + n;
     uint32_t* ord2 = grow<uint32_t>(weld2, 2 * size_t(n));
     uint32_t* key_sorted = ord2 + n;
-    uint32_t* cv = grow<uint32_t>(corner_buf, 3 * size_t(corners) + 2 * (size_t(n) + 1));
-    uint32_t* cf = cv + corners;
-    uint32_t* cv_sorted = cf + corners;
-    uint32_t* face_count = cv_sorted + corners;  // n + 1
-    uint32_t* degree = face_count + (n + 1);     // n + 1
-    // Corner keys and their sorted copy, then both directions of every edge
-    // (at most 2 * corners) and their sorted copy.
-    auto* keys = grow<unsigned long long>(key_buf, 4 * size_t(corners));
-    auto* keys_sorted = keys + corners;
-    auto* directed = keys;
-    auto* directed_sorted = keys + 2 * size_t(corners);
+    // Counting topology: per-vertex counts, offsets and cursors, the face
+    // and adjacency lists, distinct-neighbour counts and edge offsets.
+    const size_t n1 = size_t(n) + 1;
+    uint32_t* face_count = grow<uint32_t>(corner_buf, 8 * n1 + 2 * size_t(corners));
+    uint32_t* adj_count = face_count + n1;
+    uint32_t* adj_offsets = adj_count + n1;
+    uint32_t* face_cursor = adj_offsets + n1;  // n
+    uint32_t* adj_cursor = face_cursor + n1;   // n
+    uint32_t* degree = adj_cursor + n1;
+    uint32_t* up = degree + n1;
+    uint32_t* edge_offsets = up + n1;
+    uint32_t* adj = edge_offsets + n1;         // 2 * corners
+    // At most one edge per corner: every per-edge buffer is sized from the
+    // corner count, so nothing is reallocated under queued work.
     auto* unique = grow<unsigned long long>(edge_keys_, corners);
-    uint32_t* counts = grow<uint32_t>(edge_counts_, corners + 1);
-    uint32_t* runs = counts + corners;
     uint32_t* geometry = grow<uint32_t>(geometry_, n);
     uint8_t* invalid = grow<uint8_t>(invalid_, n);
     uint8_t* boundary = grow<uint8_t>(boundary_, n);
     uint32_t* faces = grow<uint32_t>(face_lists_, corners);
-    uint32_t* face_offsets = grow<uint32_t>(face_offsets_, n + 1);
-    uint32_t* offsets = grow<uint32_t>(offsets_, n + 1);
+    uint32_t* face_offsets = grow<uint32_t>(face_offsets_, n1);
+    uint32_t* offsets = grow<uint32_t>(offsets_, n1);
+    const size_t max_slots = 2 * size_t(corners);
+    uint32_t* neighbors = grow<uint32_t>(neighbors_, max_slots);
+    uint8_t* visited = grow<uint8_t>(used_, std::max<size_t>(max_slots, size_t(n) * 4));
+    auto* all = grow<LegalAction>(all_buf, max_slots);
+    uint32_t* flags = grow<uint32_t>(flag_buf, 2 * (max_slots + 1));
+    uint32_t* where = flags + max_slots + 1;
+    auto* legal = grow<LegalAction>(legal_, max_slots);
     size_t bytes = 16;
     {
+        // Library sorts remain only in the weld (run when positions change).
         size_t need = 0;
         auto take = [&](hipError_t e) {
             hip_check(e, "workspace query");
@@ -701,12 +842,6 @@ void ActionSet::build(const HostMesh& m, const float* uv) {
         };
         take(hipcub::DeviceRadixSort::SortPairs(nullptr, need, key_tmp, key_sorted, ids, ord, int(n), 0, 32, st));
         take(hipcub::DeviceScan::InclusiveScan(nullptr, need, ord2, key_sorted, hipcub::Max(), int(n), st));
-        take(hipcub::DeviceRadixSort::SortPairs(nullptr, need, cv, cv_sorted, cf, faces, int(corners), 0, 32, st));
-        take(hipcub::DeviceScan::ExclusiveSum(nullptr, need, face_count, face_offsets, int(n + 1), st));
-        take(hipcub::DeviceRadixSort::SortKeys(nullptr, need, keys, keys_sorted, int(corners), 0, 64, st));
-        take(hipcub::DeviceRunLengthEncode::Encode(nullptr, need, keys_sorted, unique, counts, runs, int(corners), st));
-        take(hipcub::DeviceRadixSort::SortKeys(nullptr, need, keys, keys_sorted, int(2 * corners), 0, 64, st));
-        take(hipcub::DeviceScan::ExclusiveSum(nullptr, need, face_count, face_offsets, int(2 * corners), st));
     }
     void* work = grow<char>(temp, bytes);
     auto run = [&](const char* what, auto call) {
@@ -753,68 +888,38 @@ void ActionSet::build(const HostMesh& m, const float* uv) {
         weld_positions_ = m.positions;
         weld_ready_ = true;
     }
-    // ---- corners: face lists and edge keys.
+    // ---- faces and edges around every geometric vertex, by counting.
     hip_check(hipMemsetAsync(invalid, 0, n, st), "clear");
     hip_check(hipMemsetAsync(boundary, 0, n, st), "clear");
-    hip_check(hipMemsetAsync(face_count, 0, size_t(n + 1) * 4 * 2, st), "clear");
-    corners_kernel<<<blocks_for(corners), 256, 0, st>>>(indices_.as<uint32_t>(), geometry, corners, cv, cf, face_count,
-                                                        invalid, keys);
-    run("face lists", [&](size_t& b) {
-        return hipcub::DeviceRadixSort::SortPairs(work, b, cv, cv_sorted, cf, faces, int(corners), 0, 32, st);
-    });
-    run("face offsets", [&](size_t& b) {
-        return hipcub::DeviceScan::ExclusiveSum(work, b, face_count, face_offsets, int(n + 1), st);
-    });
-    run("edge sort", [&](size_t& b) {
-        return hipcub::DeviceRadixSort::SortKeys(work, b, keys, keys_sorted, int(corners), 0, 64, st);
-    });
-    run("edge runs", [&](size_t& b) {
-        return hipcub::DeviceRunLengthEncode::Encode(work, b, keys_sorted, unique, counts, runs, int(corners), st);
-    });
-    uint32_t nruns = 0;
-    hip_check(hipMemcpyAsync(&nruns, runs, 4, hipMemcpyDeviceToHost, st), "download runs");
-    hip_check(hipStreamSynchronize(st), "edge runs");
-    if (nruns) {
-        unsigned long long last = 0;
-        hip_check(hipMemcpy(&last, unique + nruns - 1, 8, hipMemcpyDeviceToHost), "download last edge");
-        if (last == kNoEdge) --nruns;
-    }
-    edges_ = nruns;
-    // ---- edges: flags, degrees, sorted neighbour lists (both directions of
-    // every edge).
+    hip_check(hipMemsetAsync(face_count, 0, 5 * n1 * 4, st), "clear");  // counts and cursors
+    const uint32_t* idx = indices_.as<uint32_t>();
+    topo_count_kernel<<<blocks_for(corners), 256, 0, st>>>(idx, geometry, corners, face_count, adj_count, invalid);
+    scan_kernel<<<1, 1024, 0, st>>>(face_count, face_offsets, adj_count, adj_offsets, uint32_t(n1));
+    topo_fill_kernel<<<blocks_for(corners), 256, 0, st>>>(idx, geometry, corners, face_offsets, adj_offsets,
+                                                          face_cursor, adj_cursor, faces, adj);
+    topo_vertex_kernel<<<blocks_for(n), 256, 0, st>>>(n, face_offsets, faces, adj_offsets, adj, degree, up, boundary,
+                                                      invalid);
+    set_u32_kernel<<<1, 1, 0, st>>>(degree + n, 0);
+    set_u32_kernel<<<1, 1, 0, st>>>(up + n, 0);
+    scan_kernel<<<1, 1024, 0, st>>>(degree, offsets, up, edge_offsets, uint32_t(n1));
+    topo_emit_kernel<<<blocks_for(n), 256, 0, st>>>(n, adj_offsets, adj, offsets, edge_offsets, neighbors, unique);
+    uint32_t edge_total = 0;
+    hip_check(hipMemcpyAsync(&edge_total, edge_offsets + n, 4, hipMemcpyDeviceToHost, st), "download edges");
+    hip_check(hipStreamSynchronize(st), "edges");
+    edges_ = edge_total;
     const uint32_t slots = 2 * edges_;
-    uint32_t* neighbors = grow<uint32_t>(neighbors_, slots);
-    uint8_t* visited = grow<uint8_t>(used_, std::max<size_t>(slots, size_t(n) * 4));
-    auto* all = grow<LegalAction>(all_buf, slots);
-    uint32_t* flags = grow<uint32_t>(flag_buf, 2 * size_t(slots) + 1);
-    uint32_t* where = flags + slots;
-    edge_flags_kernel<<<blocks_for(edges_), 256, 0, st>>>(unique, counts, edges_, boundary, invalid, degree, directed);
-    run("neighbour offsets", [&](size_t& b) {
-        return hipcub::DeviceScan::ExclusiveSum(work, b, degree, offsets, int(n + 1), st);
-    });
-    if (slots) {
-        run("neighbour sort", [&](size_t& b) {
-            return hipcub::DeviceRadixSort::SortKeys(work, b, directed, directed_sorted, int(slots), 0, 64, st);
-        });
-        low_words_kernel<<<blocks_for(slots), 256, 0, st>>>(directed_sorted, slots, neighbors);
-    }
-    Topo t{positions_.as<float4>(), has_uv_ ? uv_.as<float2>() : nullptr, indices_.as<uint32_t>(),
+    Topo t{positions_.as<float4>(), has_uv_ ? uv_.as<float2>() : nullptr, idx,
            m.materials.empty() ? nullptr : materials_.as<uint16_t>(), geometry, invalid, boundary, offsets,
            neighbors, face_offsets, faces, n, faces_};
     fans_kernel<<<blocks_for(n), 256, 0, st>>>(t, invalid, visited);
     // ---- legality of every directed edge, compacted in enumeration order.
+    legal_count_ = 0;
     if (slots) {
         legal_kernel<<<blocks_for(slots), 256, 0, st>>>(t, unique, edges_, all, flags);
-        run("legal slots", [&](size_t& b) {
-            return hipcub::DeviceScan::ExclusiveSum(work, b, flags, where, int(slots), st);
-        });
-        uint32_t tail[2] = {0, 0};
-        hip_check(hipMemcpyAsync(&tail[0], where + slots - 1, 4, hipMemcpyDeviceToHost, st), "download count");
-        hip_check(hipMemcpyAsync(&tail[1], flags + slots - 1, 4, hipMemcpyDeviceToHost, st), "download count");
-        hip_check(hipStreamSynchronize(st), "legal count");
-        legal_count_ = tail[0] + tail[1];
-        auto* legal = grow<LegalAction>(legal_, legal_count_);
+        set_u32_kernel<<<1, 1, 0, st>>>(flags + slots, 0);
+        scan_kernel<<<1, 1024, 0, st>>>(flags, where, nullptr, nullptr, slots + 1);
         compact_kernel<<<blocks_for(slots), 256, 0, st>>>(all, flags, where, slots, legal);
+        hip_check(hipMemcpyAsync(&legal_count_, where + slots, 4, hipMemcpyDeviceToHost, st), "download count");
     }
     hip_check(hipGetLastError(), "action topology launch");
     hip_check(hipStreamSynchronize(st), "action topology");
@@ -1046,13 +1151,32 @@ std::vector<uint32_t> execute_actions(int device, const HostMesh& input, const f
                           c.input_materials != input.materials || c.input_uv != uv_values))
         c = ExecutorCache{};
     bool first = true;
+    // The next state, built and ranked from a trial's candidate while the
+    // gate audits it (the gate runs on another thread); kept when the gate
+    // accepts, dropped when it rejects. Positions do not change within a
+    // call, so the speculative mesh takes them once.
+    HostMesh next_mesh;
+    next_mesh.positions = state.positions;
+    bool next_ready = false;
+    const auto rank_state = [&](ActionSet* s, const HostMesh& mesh) {
+        std::unique_ptr<Graph> graph;
+        if (ranking == 0) {
+            graph = scorer->graph(mesh, *bounds);
+            scorer->forward(*graph, Precision::Bf16);
+        }
+        s->rank(ranking, diameter, graph.get(), nullptr);
+    };
     while (state.indices.size() / 3 > target && stats.trials < budget) {
         auto t = std::chrono::steady_clock::now();
         // The input's state (ranked once per ranking order that does not
-        // depend on a scorer), then one reused state for the later meshes.
-        const bool cached_rank = first && ranking != 0 && c.input_state && c.input_ranking == int(ranking);
+        // depend on a scorer), then reused states for the later meshes.
+        const bool cached_rank = next_ready || (first && ranking != 0 && c.input_state && c.input_ranking == int(ranking));
         ActionSet* set;
-        if (first) {
+        if (next_ready) {
+            std::swap(c.work, c.spec);
+            set = c.work.get();
+            next_ready = false;
+        } else if (first) {
             if (!c.input_state) {
                 c.input_state = std::make_unique<ActionSet>(device, state, uv);
                 c.input_indices = input.indices;
@@ -1079,12 +1203,7 @@ std::vector<uint32_t> execute_actions(int device, const HostMesh& input, const f
         }
         t = std::chrono::steady_clock::now();
         if (!cached_rank) {
-            std::unique_ptr<Graph> graph;
-            if (ranking == 0) {
-                graph = scorer->graph(state, *bounds);
-                scorer->forward(*graph, Precision::Bf16);
-            }
-            set->rank(ranking, diameter, graph.get(), nullptr);
+            rank_state(set, state);
             if (set == c.input_state.get()) c.input_ranking = ranking == 0 ? -1 : int(ranking);
         }
         stats.rank_ms += since(t);
@@ -1105,7 +1224,28 @@ std::vector<uint32_t> execute_actions(int device, const HostMesh& input, const f
                 stats.trial_ms += since(t);
                 ++stats.trials;
                 t = std::chrono::steady_clock::now();
-                const bool pass = gate(candidate, materials);
+                auto verdict = std::async(std::launch::async, [&] { return gate(candidate, materials); });
+                // Speculate only when an acceptance would continue the loop.
+                bool built = false;
+                if (candidate.size() / 3 > target && stats.trials < budget) {
+                    try {
+                        const auto tb = std::chrono::steady_clock::now();
+                        next_mesh.indices = candidate;
+                        next_mesh.materials = materials;
+                        if (c.spec)
+                            c.spec->rebuild(next_mesh, uv);
+                        else
+                            c.spec = std::make_unique<ActionSet>(device, next_mesh, uv);
+                        stats.speculative_build_ms += since(tb);
+                        if (c.spec->legal_count()) rank_state(c.spec.get(), next_mesh);
+                        stats.speculative_ms += since(tb);
+                        built = true;
+                    } catch (...) {
+                        verdict.wait();
+                        throw;
+                    }
+                }
+                const bool pass = verdict.get();
                 stats.gate_ms += since(t);
                 if (pass) {
                     state.indices = std::move(candidate);
@@ -1113,6 +1253,10 @@ std::vector<uint32_t> execute_actions(int device, const HostMesh& input, const f
                     stats.accepted += picked.size();
                     ++stats.accepted_batches;
                     accepted = true;
+                    if (built) {
+                        next_ready = true;
+                        ++stats.states;
+                    }
                     break;
                 }
                 stats.rejected += picked.size();

@@ -156,7 +156,7 @@ class ActionSet {
     DeviceBuffer positions_, uv_, indices_, materials_;
     DeviceBuffer geometry_, invalid_, boundary_;
     DeviceBuffer offsets_, neighbors_, face_offsets_, face_lists_;
-    DeviceBuffer edge_keys_, edge_counts_;
+    DeviceBuffer edge_keys_;
     DeviceBuffer legal_;  // LegalAction per legal action
     DeviceBuffer used_, selection_, scratch_a_, scratch_b_, scratch_c_, temp_;
     DeviceBuffer order_, rank_keys_, rank_idx_, claim_out_;
@@ -165,7 +165,7 @@ class ActionSet {
     bool touch_ready_ = false;
     bool has_uv_ = false;
     // Build workspaces, kept across rebuilds.
-    DeviceBuffer build_weld_, build_weld2_, build_corners_, build_keys_, build_temp_, build_all_, build_flags_;
+    DeviceBuffer build_weld_, build_weld2_, build_corners_, build_temp_, build_all_, build_flags_;
     // Host positions the device weld was computed from.
     std::vector<float> weld_positions_;
     bool weld_ready_ = false;
@@ -185,6 +185,9 @@ struct ExecuteStats {
     // Wall time of each stage: state builds, ranking, batch selection,
     // trials, gate calls.
     double build_ms = 0, rank_ms = 0, select_ms = 0, trial_ms = 0, gate_ms = 0;
+    // Of gate_ms: the next state's build and ranking, run while the gate
+    // audits the trial (kept when it passes).
+    double speculative_ms = 0, speculative_build_ms = 0;
     uint32_t accepted_batches = 0;
     // blitz::NeuralActionStop (include/blitz/neural.hpp:22-30): 0 target
     // reached, 1 trial budget, 2 no legal actions, 3 no accepted action.
@@ -201,6 +204,7 @@ struct ExecutorCache {
     std::unique_ptr<ActionSet> input_state;
     int input_ranking = -1;
     std::unique_ptr<ActionSet> work;
+    std::unique_ptr<ActionSet> spec;  // the speculative next state
 };
 // neural::execute_actions (src/neural/action.cpp:453-525) on the device
 // action state: rank (0 learned: BF16 logits of `scorer` on each state's

@@ -108,7 +108,7 @@ __global__ __launch_bounds__(kRasterThreads) void raster_band_kernel(const Raste
         next = load_corners(t, tri_next, clamp_face(f + step));
         tri_next = load_face(t, clamp_face(f + 2 * step));
         Setup s;
-        bool medium = false, large = false;
+        bool wide = false, small_box = false;
         if (f < f1) {
             const int kind = setup_triangle(t, corners, s);
             if (corners.behind) t.status[1] = 1u;
@@ -129,72 +129,89 @@ __global__ __launch_bounds__(kRasterThreads) void raster_band_kernel(const Raste
                     if (word[1]) atomicOr(&band[at + 1], static_cast<unsigned long long>(word[1]));
                     if (word[2]) atomicOr(&band[at + t.tiles], static_cast<unsigned long long>(word[2]));
                     if (word[3]) atomicOr(&band[at + t.tiles + 1], static_cast<unsigned long long>(word[3]));
-                } else if (w <= 8 && h <= 8) {
-                    medium = true;
                 } else {
-                    large = true;
+                    wide = true;
+                    small_box = w <= 8 && h <= 8;
                 }
             }
         }
-        // Medium faces (box within 8x8 samples): one wave pass each, lane l
+        // A few faces within 8x8 boxes: one whole-wave pass each, lane l
         // testing sample (x0 + l%8, y0 + l/8); the ballot is the face's local
-        // bitmap in the lane path's layout.
-        uint64_t pending = __ballot(medium);
-        while (pending) {
-            const int src = __ffsll(static_cast<unsigned long long>(pending)) - 1;
-            pending &= pending - 1;
-            const Setup q = shuffle_setup(s, src);
-            const int lx = lane & 7, ly = lane >> 3, x = q.x0 + lx, y = q.y0 + ly;
-            const bool in = x <= q.x1 && y <= q.y1 && y >= ymin && y <= ymax && covers(q, x, y);
-            const uint64_t local = __ballot(in);
-            if (lane < 4 && local) {
-                uint64_t word[4];
-                scatter_bitmap(local, q.x0, q.y0, word);
-                const uint64_t mine = lane == 0 ? word[0] : lane == 1 ? word[1] : lane == 2 ? word[2] : word[3];
-                if (mine) {
-                    const uint32_t row = uint32_t((q.y0 >> 3) + (lane >> 1)) - ty0;
-                    atomicOr(&band[row * t.tiles + uint32_t((q.x0 >> 3) + (lane & 1))],
-                             static_cast<unsigned long long>(mine));
+        // bitmap in the lane path's layout. (Same bits as the item queue
+        // below, which costs more for a handful of small faces.)
+        const uint64_t wide_lanes = __ballot(wide);
+        if (__popcll(wide_lanes) <= 4 && __ballot(wide && !small_box) == 0) {
+            uint64_t pending = wide_lanes;
+            while (pending) {
+                const int src = __ffsll(static_cast<unsigned long long>(pending)) - 1;
+                pending &= pending - 1;
+                const Setup q = shuffle_setup(s, src);
+                const int lx = lane & 7, ly = lane >> 3, x = q.x0 + lx, y = q.y0 + ly;
+                const bool in = x <= q.x1 && y <= q.y1 && y >= ymin && y <= ymax && covers(q, x, y);
+                const uint64_t local = __ballot(in);
+                if (lane < 4 && local) {
+                    uint64_t word[4];
+                    scatter_bitmap(local, q.x0, q.y0, word);
+                    const uint64_t mine = lane == 0 ? word[0] : lane == 1 ? word[1] : lane == 2 ? word[2] : word[3];
+                    if (mine) {
+                        const uint32_t row = uint32_t((q.y0 >> 3) + (lane >> 1)) - ty0;
+                        atomicOr(&band[row * t.tiles + uint32_t((q.x0 >> 3) + (lane & 1))],
+                                 static_cast<unsigned long long>(mine));
+                    }
                 }
             }
+            continue;
         }
-        // Large faces: the wave classifies up to 64 of the face's tiles at
-        // once (lane = tile) as empty, fully covered (within the box) or
-        // undecided; only undecided tiles get the per-sample pass.
-        pending = __ballot(large);
-        while (pending) {
-            const int src = __ffsll(static_cast<unsigned long long>(pending)) - 1;
-            pending &= pending - 1;
-            const Setup q = shuffle_setup(s, src);
-            const int lx = lane & 7, ly = lane >> 3;
-            const int tya = max(q.y0, ymin) >> 3, tyb = min(q.y1, ymax) >> 3;
-            const int txa = q.x0 >> 3, across = (q.x1 >> 3) - txa + 1;
-            const int count = across * (tyb - tya + 1);
-            for (int first = 0; first < count; first += 64) {
-                const int k = first + lane;
-                int kind = 0, tx = 0, ty = 0;
-                if (k < count) {
-                    tx = txa + k % across;
-                    ty = tya + k / across;
-                    const int xa = max(tx << 3, q.x0), xb = min((tx << 3) | 7, q.x1);
-                    const int ya = max(ty << 3, q.y0), yb = min((ty << 3) | 7, q.y1);
-                    kind = classify_rect(q, xa, xb, ya, yb);
-                    if (kind == 1)
-                        atomicOr(&band[uint32_t(ty - int(ty0)) * t.tiles + uint32_t(tx)],
-                                 static_cast<unsigned long long>(rect_word(xa & 7, xb & 7, ya & 7, yb & 7)));
-                }
-                uint64_t open = __ballot(kind == 2);
-                while (open) {
-                    const int at = __ffsll(static_cast<unsigned long long>(open)) - 1;
-                    open &= open - 1;
-                    const int ux = __shfl(tx, at), uy = __shfl(ty, at);
-                    const int x = (ux << 3) | lx, y = (uy << 3) | ly;
-                    const bool in = x >= q.x0 && x <= q.x1 && y >= q.y0 && y <= q.y1 && covers(q, x, y);
-                    const uint64_t word = __ballot(in);
-                    if (lane == 0 && word)
-                        atomicOr(&band[uint32_t(uy - int(ty0)) * t.tiles + uint32_t(ux)],
-                                 static_cast<unsigned long long>(word));
-                }
+        // Wider or taller faces: their boxes are cut into items, one tile
+        // row by one 4-column quad (within the band), and the wave's items
+        // are dealt to its lanes 64 at a time. A lane finds the face owning
+        // its item from the inclusive item counts (a fixed six-step search),
+        // takes that face's setup from the owner lane, classifies the item
+        // exactly (classify_rect: full, empty or undecided) and tests an
+        // undecided item's samples with the face's row and column products.
+        uint32_t items = 0, across = 0;
+        int quad0 = 0, row0 = 0, ya_face = 0, yb_face = 0;
+        if (wide) {
+            ya_face = max(s.y0, ymin);
+            yb_face = min(s.y1, ymax);
+            quad0 = s.x0 >> 2;
+            row0 = ya_face >> 3;
+            across = uint32_t((s.x1 >> 2) - quad0 + 1);
+            items = across * uint32_t((yb_face >> 3) - row0 + 1);
+        }
+        uint32_t inclusive = items;
+        for (int d = 1; d < 64; d <<= 1) {
+            const uint32_t v = __shfl_up(inclusive, d);
+            if (lane >= d) inclusive += v;
+        }
+        const uint32_t total = __shfl(inclusive, 63);
+        for (uint32_t base = 0; base < total; base += 64) {
+            const uint32_t k = base + uint32_t(lane);
+            const bool live = k < total;
+            const uint32_t key = live ? k : total - 1;
+            int lo = 0;
+            for (int half = 32; half > 0; half >>= 1)
+                if (__shfl(inclusive, lo + half - 1) <= key) lo += half;
+            const int owner = lo;
+            const uint32_t o_end = __shfl(inclusive, owner), o_items = __shfl(items, owner);
+            const uint32_t o_across = __shfl(across, owner);
+            const int o_quad = __shfl(quad0, owner), o_row = __shfl(row0, owner);
+            const int o_ya = __shfl(ya_face, owner), o_yb = __shfl(yb_face, owner);
+            const Setup q = shuffle_setup(s, owner);
+            const uint32_t j = key - (o_end - o_items);
+            const int qx = o_quad + int(j % o_across), ty = o_row + int(j / o_across);
+            const int xa = max(4 * qx, q.x0), xb = min(4 * qx + 3, q.x1);
+            const int ya = max(8 * ty, o_ya), yb = min(8 * ty + 7, o_yb);
+            if (live) {
+                const int kind = classify_rect(q, xa, xb, ya, yb);
+                uint64_t word = 0;
+                if (kind == 1)
+                    word = rect_word(xa & 7, xb & 7, ya & 7, yb & 7);
+                else if (kind == 2)
+                    word = quad_word(q, 4 * qx, xa, xb, ya, yb);
+                if (word)
+                    atomicOr(&band[uint32_t(ty - int(ty0)) * t.tiles + uint32_t(qx >> 1)],
+                             static_cast<unsigned long long>(word));
             }
         }
     }

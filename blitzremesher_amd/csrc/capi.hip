@@ -1295,6 +1295,12 @@ blitz_status blitz_hip_chain_info_of(const blitz_hip_chain* c, blitz_hip_chain_i
         out->ms = r.ms;
         out->executor_ms = r.executor_ms;
         out->gate_ms = r.gate_ms;
+        out->build_ms = r.build_ms;
+        out->rank_ms = r.rank_ms;
+        out->select_ms = r.select_ms;
+        out->trial_ms = r.trial_ms;
+        out->speculative_ms = r.speculative_ms;
+        out->speculative_build_ms = r.speculative_build_ms;
         out->offer_ms = r.offer_ms;
         out->confirm_ms = r.confirm_ms;
         out->tiebreak_ms = r.tiebreak_ms;

@@ -353,6 +353,12 @@ ChainLod ChainSearch::propose(const std::shared_ptr<Node>& parent, size_t target
     out.indices = execute_actions(audit_.device(), source_, uv_, target, o_.action_trials, o_.ranking, o_.action_batch,
                                   gate, st, &out.materials, scorer_, &bounds_, &executor_);
     r_.executor_ms += since(t);
+    r_.build_ms += st.build_ms;
+    r_.rank_ms += st.rank_ms;
+    r_.select_ms += st.select_ms;
+    r_.trial_ms += st.trial_ms;
+    r_.speculative_ms += st.speculative_ms;
+    r_.speculative_build_ms += st.speculative_build_ms;
     ++r_.decoded;
     r_.legal_collapses += st.accepted;
     r_.rejected_collapses += st.rejected;

@@ -105,6 +105,10 @@ struct ChainResult {
     std::vector<ChainProposal> proposals;
     // Wall time of the whole generation and of its parts (ms).
     double ms = 0, executor_ms = 0, gate_ms = 0, offer_ms = 0, confirm_ms = 0, tiebreak_ms = 0;
+    // Executor stages summed over proposals: state builds, rankings, batch
+    // selection, trials, and the speculative next-state builds that overlap
+    // the gates.
+    double build_ms = 0, rank_ms = 0, select_ms = 0, trial_ms = 0, speculative_ms = 0, speculative_build_ms = 0;
     uint64_t gate_audits = 0, confirm_audits = 0;  // offers: the sum of audit_evaluations
 };
 

@@ -288,6 +288,45 @@ __device__ __forceinline__ uint64_t lane_bitmap4(const Setup& s, int ya, int yb)
     return ((uint64_t(hi) << 32) | lo) << (8 * (ya - s.y0));
 }
 
+// Samples [xa, xb] x [ya, yb] of one 4-column quad of a tile (columns q ..
+// q + 3, q a multiple of 4; xa..xb within them; ya..yb within one tile row)
+// covered by the face, by one lane: the products of the quad's four columns
+// once, those of each row once per row, the reference expression's own
+// rounding as in lane_bitmap4. Returns the tile word (bit 8*(y&7) + (x&7)).
+__device__ __forceinline__ uint64_t quad_word(const Setup& s, int q, int xa, int xb, int ya, int yb) {
+    const double au = s.cx - s.bx, bu = s.cy - s.by;
+    const double av = s.ax - s.cx, bv = s.ay - s.cy;
+    const double aw = s.bx - s.ax, bw = s.by - s.ay;
+    const double px0 = double(q) + .5;
+    double cu[4], cv[4], cw[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        const double px = px0 + double(c);
+        cu[c] = bu * (px - s.bx);
+        cv[c] = bv * (px - s.cx);
+        cw[c] = bw * (px - s.ax);
+    }
+    // fl(u + r) < 0 tested as u < -r (see covers()).
+    const double nu = -s.r0, nv = -s.r1, nw = -s.r2;
+    const uint32_t columns = (0xFu >> (3 - (xb - q))) & (0xFu << (xa - q));
+    const int shift = q & 7;
+    uint64_t word = 0;
+    double py = double(ya) + .5;
+    for (int y = ya; y <= yb; ++y) {
+        const double ru = au * (py - s.by), rv = av * (py - s.cy), rw = aw * (py - s.ay);
+        uint32_t row = 0;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            // Bitwise | so that all three tests issue without branches.
+            const bool in = !(((ru - cu[c]) < nu) | ((rv - cv[c]) < nv) | ((rw - cw[c]) < nw));
+            row |= in ? 1u << c : 0u;
+        }
+        word |= uint64_t((row & columns) << shift) << (8 * (y & 7));
+        py += 1.0;
+    }
+    return word;
+}
+
 // Moves an 8x8 bitmap anchored at sample (x0, y0) into the up-to-four 8x8
 // tile words it overlaps: word[0] tile (x0>>3, y0>>3), word[1] the tile to
 // its right, word[2] below, word[3] below-right. Shifting the bitmap left by

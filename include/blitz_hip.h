@@ -473,6 +473,10 @@ typedef struct blitz_hip_chain_info {
     uint32_t bounded_audits, finalists, selected, proposals;
     uint64_t gate_audits, confirm_audits;
     double ms, executor_ms, gate_ms, offer_ms, confirm_ms, tiebreak_ms;
+    /* executor stages over all proposals: state builds, rankings, batch
+     * selection, trials, and next-state builds run while the gates audit */
+    double build_ms, rank_ms, select_ms, trial_ms, speculative_ms;
+    double speculative_build_ms; /* of speculative_ms: the state builds */
 } blitz_hip_chain_info;
 
 /* One proposal (blitz_neural_action_proposal, include/blitz/blitz.h:147-152)

@@ -135,6 +135,40 @@ int main(int argc, char** argv) {
             bad += ((d + r) < 0) != (d < -r);
         }
     }
+    // quad_word over random quads of boxes up to 24 x 24 samples, against
+    // the per-sample predicate.
+    {
+        for (int it = 0; it < iterations; ++it) {
+            const double span = it % 3 == 0 ? 8 : 24;
+            auto coord = [&]() {
+                double v = std::uniform_real_distribution<double>(0, span)(rng) + 40;
+                if (it % 4 == 1) v = std::round(v * 2) / 2;
+                if (it % 4 == 2) v = std::round(v) + .5;
+                return v;
+            };
+            Setup s;
+            s.ax = coord(); s.ay = coord(); s.bx = coord(); s.by = coord(); s.cx = coord(); s.cy = coord();
+            s.r0 = .5 * (fabs(s.cx - s.bx) + fabs(s.cy - s.by));
+            s.r1 = .5 * (fabs(s.ax - s.cx) + fabs(s.ay - s.cy));
+            s.r2 = .5 * (fabs(s.bx - s.ax) + fabs(s.by - s.ay));
+            if (it % 7 == 0) s.r0 = s.r1 = s.r2 = 0;
+            s.x0 = int(std::floor(std::min({s.ax, s.bx, s.cx})));
+            s.x1 = int(std::floor(std::max({s.ax, s.bx, s.cx})));
+            s.y0 = int(std::floor(std::min({s.ay, s.by, s.cy})));
+            s.y1 = int(std::floor(std::max({s.ay, s.by, s.cy})));
+            for (int q = s.x0 & ~3; q <= s.x1; q += 4)
+                for (int ty = s.y0 >> 3; ty <= s.y1 >> 3; ++ty) {
+                    const int xa = std::max(q, s.x0), xb = std::min(q + 3, s.x1);
+                    int ya = std::max(ty * 8, s.y0), yb = std::min(ty * 8 + 7, s.y1);
+                    if (it % 5 == 0 && ya < yb) ++ya;  // band-clipped rows
+                    uint64_t want = 0;
+                    for (int y = ya; y <= yb; ++y)
+                        for (int x = xa; x <= xb; ++x)
+                            if (covers_ref(s, x, y)) want |= 1ull << (8 * (y & 7) + (x & 7));
+                    bad += quad_word(s, q, xa, xb, ya, yb) != want;
+                }
+        }
+    }
     // scatter_bitmap against a per-bit move of 4- and 8-wide bitmaps.
     {
         std::uniform_int_distribution<int> origin(0, 4095);
@@ -223,7 +257,7 @@ def _source(plant: tuple[str, str] | None = None) -> str:
              _function(text, "bool box_in_frame("), _function(text, "bool covers("),
              _function(text, "int classify_rect("), _function(text, "uint64_t rect_word("),
              _function(text, "uint64_t lane_bitmap("), _function(text, "uint64_t lane_bitmap4("),
-             _function(text, "void scatter_bitmap(")]
+             _function(text, "uint64_t quad_word("), _function(text, "void scatter_bitmap(")]
     return PRELUDE + "\n\n".join(parts) + CHECK
 
 
@@ -255,6 +289,11 @@ def test_device_raster_math_agrees_with_every_sample(tmp_path):
     ("        py += 1.0;\n    }\n    return ((uint64_t(hi)", "    }\n    return ((uint64_t(hi)"),
     # The right tile's columns taken without the complementary mask.
     ("const uint64_t right = (local >> (8 - ox)) & ~keep64;", "const uint64_t right = (local >> (8 - ox));"),
+    # A quad's columns masked from the wrong end.
+    ("const uint32_t columns = (0xFu >> (3 - (xb - q))) & (0xFu << (xa - q));",
+     "const uint32_t columns = (0xFu >> (xb - q)) & (0xFu << (xa - q));"),
+    # The quad's right half placed in the tile's left half.
+    ("    const int shift = q & 7;\n    uint64_t word = 0;", "    const int shift = 0;\n    uint64_t word = 0;"),
     # NaN corners sent down the unordered min/max path.
     ("    if (!isnan(area)) {\n        minx = fmin", "    if (true) {\n        minx = fmin"),
     # The frame test's upper bound taken as inclusive.
