@@ -272,13 +272,76 @@ __device__ bool link_of(const Topo& t, uint32_t f, uint32_t u, uint32_t other[2]
     return count == 2;
 }
 
+// Sorted geometric vertices of every face, read by the link and legality
+// tests in place of three index and three weld lookups per face.
+__global__ void face_geo_kernel(const uint32_t* __restrict__ indices, const uint32_t* __restrict__ geometry,
+                                uint32_t faces, uint4* out) {
+    const uint32_t f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= faces) return;
+    uint32_t a = geometry[indices[3 * f]], b = geometry[indices[3 * f + 1]], c = geometry[indices[3 * f + 2]];
+    if (a > b) { const uint32_t x = a; a = b; b = x; }
+    if (b > c) { const uint32_t x = b; b = c; c = x; }
+    if (a > b) { const uint32_t x = a; a = b; b = x; }
+    out[f] = make_uint4(a, b, c, 0u);
+}
+
+// The two vertices of face f other than u from its sorted triple; false
+// unless u occurs exactly once (link_of's two-others condition).
+__device__ __forceinline__ bool link_from_geo(uint4 g, uint32_t u, uint32_t other[2]) {
+    const uint32_t hits = (g.x == u) + (g.y == u) + (g.z == u);
+    if (hits != 1) return false;
+    other[0] = g.x == u ? g.y : g.x;
+    other[1] = g.z == u ? g.y : g.z;
+    return true;
+}
+
 // src/neural/action.cpp:107-144: the link of u must be one connected fan
 // over all of u's neighbours.
-__global__ void fans_kernel(Topo t, uint8_t* invalid, uint8_t* visited) {
+constexpr uint32_t kFan = 32;
+
+__global__ __launch_bounds__(256) void fans_kernel(Topo t, uint8_t* invalid, uint8_t* visited) {
+    __shared__ uint8_t fan_slots[256][2 * kFan];
     const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
     if (u >= t.vertices) return;
     const uint32_t f0 = t.face_offsets[u], f1 = t.face_offsets[u + 1];
     if (f0 == f1 || invalid[u]) return;
+    if (f1 - f0 <= kFan && t.offsets[u + 1] - t.offsets[u] <= kFan) {
+        // Fans of up to 32 faces and neighbours: each face's link as a pair of
+        // neighbour slots (0xFF: not a neighbour), found once and kept in
+        // LDS; reach spreads over 32-bit slot masks.
+        uint8_t* pa = fan_slots[threadIdx.x];
+        uint8_t* pb = pa + kFan;
+        const uint32_t d = f1 - f0, n0 = t.offsets[u];
+        for (uint32_t k = 0; k < d; ++k) {
+            uint32_t other[2];
+            if (!link_from_geo(t.face_geo[t.faces[f0 + k]], u, other)) {
+                invalid[u] = 1;
+                return;
+            }
+            const uint32_t a = find_neighbor(t, u, other[0]), b = find_neighbor(t, u, other[1]);
+            pa[k] = a == kNone ? uint8_t(0xFF) : uint8_t(a - n0);
+            pb[k] = b == kNone ? uint8_t(0xFF) : uint8_t(b - n0);
+        }
+        if (pa[0] == 0xFF) {
+            invalid[u] = 1;
+            return;
+        }
+        uint32_t reached = 1u << pa[0];
+        for (bool changed = true; changed;) {
+            changed = false;
+            for (uint32_t k = 0; k < d; ++k) {
+                if (pa[k] == 0xFF || pb[k] == 0xFF) continue;
+                const uint32_t m = (1u << pa[k]) | (1u << pb[k]);
+                if ((reached & m) && (reached & m) != m) {
+                    reached |= m;
+                    changed = true;
+                }
+            }
+        }
+        const uint32_t slots = t.offsets[u + 1] - n0;
+        if (reached != (slots == 32 ? ~0u : (1u << slots) - 1)) invalid[u] = 1;
+        return;
+    }
     uint32_t other[2];
     for (uint32_t k = f0; k < f1; ++k)
         if (!link_of(t, t.faces[k], u, other)) {
@@ -341,14 +404,11 @@ __device__ __forceinline__ uint32_t map_id(const LegalAction& a, uint32_t id) {
     return id;
 }
 
-// Sorted geometric triple of face f after the mapping of `a` (identity when
-// `mapped` is false).
-__device__ __forceinline__ void mapped_triple(const Topo& t, const LegalAction& a, uint32_t f, bool mapped,
-                                              uint32_t out[3]) {
-    for (uint32_t j = 0; j < 3; ++j) {
-        const uint32_t id = t.indices[3 * f + j];
-        out[j] = t.geometry[mapped ? map_id(a, id) : id];
-    }
+// Sorted triple of a face's geometric vertices with `from` replaced by `to`.
+__device__ __forceinline__ void moved_triple(uint4 g, uint32_t from, uint32_t to, uint32_t out[3]) {
+    out[0] = g.x == from ? to : g.x;
+    out[1] = g.y == from ? to : g.y;
+    out[2] = g.z == from ? to : g.z;
     sort3(out);
 }
 
@@ -473,33 +533,31 @@ __device__ bool legal_action(const Topo& t, uint32_t from, uint32_t to, LegalAct
         }
     }
     // The moved faces: not degenerate, not duplicating a face around `to` or
-    // each other, not folded, not flipped in UV.
+    // each other, not folded, not flipped in UV. Every corner of `from` in
+    // these faces is one of the mapped wedges (checked above), so a moved
+    // face's geometric triple is its triple with `from` replaced by `to`.
     for (uint32_t k = f0; k < f1; ++k) {
         const uint32_t f = t.faces[k];
+        const uint4 geo = t.face_geo[f];
+        if (geo.x == to || geo.y == to || geo.z == to) continue;  // removed by the collapse
+        uint32_t ids[3];
+        moved_triple(geo, from, to, ids);
+        if (ids[0] == ids[1] || ids[1] == ids[2]) return false;
+        for (uint32_t l = g0; l < g1; ++l) {
+            const uint4 other = t.face_geo[t.faces[l]];
+            if (other.x == ids[0] && other.y == ids[1] && other.z == ids[2]) return false;
+        }
+        for (uint32_t l = f0; l < k; ++l) {
+            const uint4 g = t.face_geo[t.faces[l]];
+            if (g.x == to || g.y == to || g.z == to) continue;
+            uint32_t other[3];
+            moved_triple(g, from, to, other);
+            if (other[0] == ids[0] && other[1] == ids[1] && other[2] == ids[2]) return false;
+        }
         uint32_t old[3], next[3];
-        bool removed = false;
         for (uint32_t j = 0; j < 3; ++j) {
             old[j] = t.indices[3 * f + j];
             next[j] = map_id(a, old[j]);
-            removed |= t.geometry[old[j]] == to;
-        }
-        if (removed) continue;
-        uint32_t ids[3] = {t.geometry[next[0]], t.geometry[next[1]], t.geometry[next[2]]};
-        sort3(ids);
-        if (ids[0] == ids[1] || ids[1] == ids[2]) return false;
-        for (uint32_t l = g0; l < g1; ++l) {
-            uint32_t other[3];
-            mapped_triple(t, a, t.faces[l], false, other);
-            if (other[0] == ids[0] && other[1] == ids[1] && other[2] == ids[2]) return false;
-        }
-        for (uint32_t l = f0; l < k; ++l) {
-            const uint32_t g = t.faces[l];
-            bool skip = false;
-            for (uint32_t j = 0; j < 3; ++j) skip |= t.geometry[t.indices[3 * g + j]] == to;
-            if (skip) continue;
-            uint32_t other[3];
-            mapped_triple(t, a, g, true, other);
-            if (other[0] == ids[0] && other[1] == ids[1] && other[2] == ids[2]) return false;
         }
         const float4 p0 = t.positions[old[0]], p1 = t.positions[old[1]], p2 = t.positions[old[2]];
         const float4 q0 = t.positions[next[0]], q1 = t.positions[next[1]], q2 = t.positions[next[2]];
@@ -832,6 +890,7 @@ void ActionSet::build(const HostMesh& m, const float* uv) {
     uint32_t* flags = grow<uint32_t>(flag_buf, 2 * (max_slots + 1));
     uint32_t* where = flags + max_slots + 1;
     auto* legal = grow<LegalAction>(legal_, max_slots);
+    uint4* face_geo = grow<uint4>(face_geo_, faces_);
     size_t bytes = 16;
     {
         // Library sorts remain only in the weld (run when positions change).
@@ -894,6 +953,7 @@ void ActionSet::build(const HostMesh& m, const float* uv) {
     hip_check(hipMemsetAsync(face_count, 0, 5 * n1 * 4, st), "clear");  // counts and cursors
     const uint32_t* idx = indices_.as<uint32_t>();
     topo_count_kernel<<<blocks_for(corners), 256, 0, st>>>(idx, geometry, corners, face_count, adj_count, invalid);
+    face_geo_kernel<<<blocks_for(faces_), 256, 0, st>>>(idx, geometry, faces_, face_geo);
     scan_kernel<<<1, 1024, 0, st>>>(face_count, face_offsets, adj_count, adj_offsets, uint32_t(n1));
     topo_fill_kernel<<<blocks_for(corners), 256, 0, st>>>(idx, geometry, corners, face_offsets, adj_offsets,
                                                           face_cursor, adj_cursor, faces, adj);
@@ -910,7 +970,7 @@ void ActionSet::build(const HostMesh& m, const float* uv) {
     const uint32_t slots = 2 * edges_;
     Topo t{positions_.as<float4>(), has_uv_ ? uv_.as<float2>() : nullptr, idx,
            m.materials.empty() ? nullptr : materials_.as<uint16_t>(), geometry, invalid, boundary, offsets,
-           neighbors, face_offsets, faces, n, faces_};
+           neighbors, face_offsets, faces, n, faces_, face_geo};
     fans_kernel<<<blocks_for(n), 256, 0, st>>>(t, invalid, visited);
     // ---- legality of every directed edge, compacted in enumeration order.
     legal_count_ = 0;
@@ -949,7 +1009,7 @@ ActionTopo ActionSet::topo() const {
     return {positions_.as<float4>(), has_uv_ ? uv_.as<float2>() : nullptr, indices_.as<uint32_t>(),
             has_materials_ ? materials_.as<uint16_t>() : nullptr, geometry_.as<uint32_t>(), invalid_.as<uint8_t>(),
             boundary_.as<uint8_t>(), offsets_.as<uint32_t>(), neighbors_.as<uint32_t>(),
-            face_offsets_.as<uint32_t>(), face_lists_.as<uint32_t>(), vertices_, faces_};
+            face_offsets_.as<uint32_t>(), face_lists_.as<uint32_t>(), vertices_, faces_, face_geo_.as<uint4>()};
 }
 
 uint32_t ActionSet::claim_device(const uint32_t* drows, size_t n, uint32_t maximum, uint64_t removable, bool strict,

@@ -42,6 +42,7 @@ struct ActionTopo {
     const uint32_t* face_offsets;  // faces around each geometric vertex (ascending)
     const uint32_t* faces;
     uint32_t vertices, triangles;
+    const uint4* face_geo;  // sorted geometric vertices of each face
 };
 
 // Host attribute streams of a state's input for the graph features (null
@@ -155,7 +156,7 @@ class ActionSet {
     double build_ms_ = 0, diameter_ = 1;
     DeviceBuffer positions_, uv_, indices_, materials_;
     DeviceBuffer geometry_, invalid_, boundary_;
-    DeviceBuffer offsets_, neighbors_, face_offsets_, face_lists_;
+    DeviceBuffer offsets_, neighbors_, face_offsets_, face_lists_, face_geo_;
     DeviceBuffer edge_keys_;
     DeviceBuffer legal_;  // LegalAction per legal action
     DeviceBuffer used_, selection_, scratch_a_, scratch_b_, scratch_c_, temp_;
