@@ -368,6 +368,81 @@ def neural_leg(train_steps: int, budget: int) -> dict:
     return out
 
 
+def training_leg(steps: int, count: int, pools_per_mesh: int, holdout: int = 8) -> dict:
+    """Scorer training from method-generated labels: `count` seeded meshes of
+    uneven size (2k-100k triangles, log-uniform; spheres, tori, extruded
+    boxes, heightfields, seed 5); up to `pools_per_mesh` teacher pools of 16
+    collapses per mesh, half from the classical current-plane order and half
+    random, each collapse audited as a patch under the first chain level's
+    four gates (the chain leg's settings: 16 views at 256 px, ss 4 search /
+    8 audit, 1 px) and labelled with the reference's preferred sets; placement
+    targets from the teacher solve. All graphs are joined into one device
+    graph; each BF16 step (AdamW, lr 1e-3, clip 1) takes 8,192 rows = 512
+    whole pools drawn from the first count - holdout meshes, so a batch spans
+    many meshes. Reported: label time, the loss curve, step time, and the
+    share of pools whose top-scored valid collapse is preferred, before and
+    after training, on trained and held-out meshes."""
+    import numpy as np
+    from blitzremesher_amd import Audit, meshes
+    from blitzremesher_amd.scorer import BF16, Scorer, TrainSettings, he_weights
+    from blitzremesher_amd.teacher import build_dataset, sample_batch, top1_preferred
+    settings = c1_chain_settings()
+    t0 = time.perf_counter()
+    family_meshes = meshes.training_set(count, 2000, 100000, 5)
+    gen_s = time.perf_counter() - t0
+    ms = [m for _, m in family_meshes]
+    tris = np.array([m.triangles for m in ms])
+    out = {"workload": f"{count} procedural meshes (log-uniform 2k-100k triangles, seed 5; sphere, torus, extruded "
+                       f"box, heightfield), up to {pools_per_mesh} teacher pools of 16 per mesh audited under the "
+                       "level-1 four gates (16 views, S=256, ss 4/8, 1 px); BF16 training, batch 8,192 rows "
+                       f"(512 pools) spanning meshes, {steps} AdamW steps; last {holdout} meshes held out",
+           "meshes": {"count": count, "triangles_total": int(tris.sum()), "triangles_min": int(tris.min()),
+                      "triangles_max": int(tris.max()), "generate_s": round(gen_s, 3)}}
+    with Audit(0) as audit, Scorer(he_weights(4)) as scorer:
+        t0 = time.perf_counter()
+        data = build_dataset(audit, scorer, ms, settings, pools_per_mesh)
+        label_s = time.perf_counter() - t0
+        rows = len(data.actions)
+        valid = (data.bits & 1) != 0
+        out["labels"] = {"pools": data.pools, "rows": rows, "valid": int(valid.sum()),
+                         "preferred": int(((data.bits & 2) != 0).sum()), "placed": int(((data.bits & 4) != 0).sum()),
+                         "full_path_fallbacks": int(data.fallbacks), "s": round(label_s, 3),
+                         "row_audits_per_s": round(4 * rows / label_s, 1),
+                         "joined_graph": {"vertices": data.graph.vertices, "actions": data.graph.actions}}
+        train_pools = np.flatnonzero(data.pool_mesh < count - holdout)
+        held_pools = np.flatnonzero(data.pool_mesh >= count - holdout)
+        before = scorer.forward(data.graph, BF16)[:, 0]
+        ts = TrainSettings(pool=16, precision=BF16)
+        rng = np.random.default_rng(11)
+        losses, step_ms = [], []
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            a, b, t = sample_batch(data, train_pools, 8192, rng)
+            r = scorer.train_step(data.graph, a, b, t, ts)
+            losses.append(r.loss)
+            step_ms.append(scorer.last_step_ms())
+        train_s = time.perf_counter() - t0
+        after = scorer.forward(data.graph, BF16)[:, 0]
+        nv, nb = data.graph.vertices, 8192
+        # Forward flops of the formulation run (every vertex through the three
+        # message-passing layers, the batch through the edge MLP), backward
+        # counted as twice the forward.
+        flops = 3 * (2 * nv * (64 + 256 + 256) * 128 + 2 * nb * (256 * 128 + 128 * 4))
+        med = float(np.median(step_ms))
+        every = max(1, steps // 10)
+        out["training"] = {
+            "steps": steps, "s": round(train_s, 3), "step_ms_median": round(med, 3),
+            "rows_per_s": round(nb / (med * 1e-3), 1),
+            "flops_per_step": flops, "tflops": round(flops / (med * 1e-3) / 1e12, 2),
+            "loss_curve": [[i, round(float(np.mean(losses[i:i + every])), 4)] for i in range(0, steps, every)],
+            "top1_preferred": {"trained_before": round(top1_preferred(data, before, train_pools), 4),
+                               "trained_after": round(top1_preferred(data, after, train_pools), 4),
+                               "held_out_before": round(top1_preferred(data, before, held_pools), 4),
+                               "held_out_after": round(top1_preferred(data, after, held_pools), 4)}}
+        data.graph.release()
+    return out
+
+
 def c1_chain_settings():
     """The CPU-runnable chain configuration: 4 levels from 256 px to 32 px,
     transition and LOD0 bounds 1 px, coverage profile, 16 orthographic search
@@ -529,6 +604,10 @@ def main() -> int:
     ap.add_argument("--no-appearance", action="store_true")
     ap.add_argument("--no-neural", action="store_true")
     ap.add_argument("--no-chain", action="store_true")
+    ap.add_argument("--no-training", action="store_true")
+    ap.add_argument("--training-steps", type=int, default=200)
+    ap.add_argument("--training-meshes", type=int, default=64)
+    ap.add_argument("--training-pools", type=int, default=96)
     ap.add_argument("--chain-repeats", type=int, default=2)
     ap.add_argument("--neural-train-steps", type=int, default=300)
     ap.add_argument("--executor-budget", type=int, default=400)
@@ -610,6 +689,8 @@ def main() -> int:
     appearance = None if args.no_appearance else appearance_leg(max(3, min(args.steps, 10)))
     neural = None if args.no_neural else neural_leg(args.neural_train_steps, args.executor_budget)
     chain = None if args.no_chain else chain_leg(args.chain_repeats, chain_cpu)
+    training = None if args.no_training else training_leg(args.training_steps, args.training_meshes,
+                                                          args.training_pools)
     executor = None if args.no_executor else executor_leg(args.executor_budget,
                                                           0 if args.no_cpu else args.executor_cpu_budget)
 
@@ -644,6 +725,7 @@ def main() -> int:
         "executor": executor,
         "appearance": appearance,
         "neural": neural,
+        "training": training,
         "scorer": scorer,
     }
     print(json.dumps(line), flush=True)

@@ -300,6 +300,7 @@ PROTOTYPES = {
     "blitz_hip_scorer_destroy": (None, [_P]),
     "blitz_hip_scorer_set_weights": (C.c_int, [_P, C.POINTER(C.c_float), C.c_size_t, *_ERR]),
     "blitz_hip_graph_create": (C.c_int, [_P, C.POINTER(BlitzMesh), C.POINTER(Bounds), C.POINTER(_P), *_ERR]),
+    "blitz_hip_graph_concat": (C.c_int, [_P, C.POINTER(_P), C.c_size_t, C.POINTER(_P), *_ERR]),
     "blitz_hip_graph_destroy": (None, [_P]),
     "blitz_hip_graph_info": (C.c_int, [_P, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
     "blitz_hip_graph_features": (C.c_int, [_P, _P, C.POINTER(C.c_float), C.c_size_t]),

@@ -1054,6 +1054,30 @@ blitz_status blitz_hip_graph_create(blitz_hip_scorer* s, const blitz_mesh* m, co
     });
 }
 
+blitz_status blitz_hip_graph_concat(blitz_hip_scorer* s, blitz_hip_graph* const* parts, size_t count,
+                                    blitz_hip_graph** out, char* error, size_t capacity) {
+    if (out) *out = nullptr;
+    return guarded(error, capacity, [&] {
+        if (!out) throw std::invalid_argument("null graph destination");
+        if (!parts || !count) throw std::invalid_argument("no graphs to join");
+        Scorer& scorer = scorer_of(s);
+        std::vector<const Graph*> graphs(count);
+        for (size_t k = 0; k < count; ++k) graphs[k] = &graph_of(s, parts[k]);
+        auto g = std::make_unique<blitz_hip_graph>();
+        g->owner = s;
+        g->graph = scorer.concat(graphs.data(), count);
+        g->src.reserve(g->graph->actions);
+        g->dst.reserve(g->graph->actions);
+        uint32_t v0 = 0;
+        for (size_t k = 0; k < count; ++k) {
+            for (const uint32_t v : parts[k]->src) g->src.push_back(v + v0);
+            for (const uint32_t v : parts[k]->dst) g->dst.push_back(v + v0);
+            v0 += graphs[k]->vertices;
+        }
+        *out = g.release();
+    });
+}
+
 void blitz_hip_graph_destroy(blitz_hip_graph* g) {
     if (!g) return;
     if (g->owner && g->owner->impl) {

@@ -134,6 +134,17 @@ void Scorer::refresh_bf16() {
                             size_t(kHidden) * kLayerIn[l], stream_);
 }
 
+// Sizes the per-vertex and per-action buffers of the forward pass.
+static void size_forward(Graph& g) {
+    const size_t nv = g.vertices;
+    g.h_a.resize(nv * kHidden * 4);
+    g.h_b.resize(nv * kHidden * 4);
+    g.z.resize(std::max(nv * 2 * kHidden, size_t(g.actions) * kHidden) * 4);
+    g.hq_a.resize(nv * kHidden * 2);
+    g.hq_b.resize(nv * kHidden * 2);
+    g.outputs.resize(size_t(g.actions) * kOutputs * 4);
+}
+
 std::unique_ptr<Graph> Scorer::graph(const HostMesh& m, const Bounds& b) {
     if (!(b.radius > 0) || !std::isfinite(b.radius)) throw std::invalid_argument("invalid bounds");
     DeviceGuard guard(device_);
@@ -143,6 +154,7 @@ std::unique_ptr<Graph> Scorer::graph(const HostMesh& m, const Bounds& b) {
     g->vertices = uint32_t(nv);
     g->faces = uint32_t(nf);
     g->actions = uint32_t(t.neighbors.size());
+    g->corners = uint32_t(t.faces.size());
     g->diameter = b.radius * 2;
     std::vector<float> pos(nv * 4, 0.f);
     for (size_t i = 0; i < nv; ++i) std::memcpy(&pos[4 * i], &m.positions[3 * i], 12);
@@ -173,14 +185,71 @@ std::unique_ptr<Graph> Scorer::graph(const HostMesh& m, const Bounds& b) {
     launch_features(in, stream_);
     g->features_bf16.resize(nv * kFeatures * 2);
     launch_convert_bf16(g->features.as<float>(), g->features_bf16.as<__bf16>(), nv * kFeatures, stream_);
-    g->h_a.resize(nv * kHidden * 4);
-    g->h_b.resize(nv * kHidden * 4);
-    g->z.resize(std::max(nv * 2 * kHidden, size_t(g->actions) * kHidden) * 4);
-    g->hq_a.resize(nv * kHidden * 2);
-    g->hq_b.resize(nv * kHidden * 2);
-    g->outputs.resize(size_t(g->actions) * kOutputs * 4);
+    size_forward(*g);
     hip_check(hipGetLastError(), "features launch");
     hip_check(hipStreamSynchronize(stream_), "graph");
+    return g;
+}
+
+std::unique_ptr<Graph> Scorer::concat(const Graph* const* parts, size_t count) {
+    if (!parts || !count) throw std::invalid_argument("no graphs to join");
+    uint64_t nv = 0, na = 0, nf = 0, nc = 0;
+    for (size_t k = 0; k < count; ++k) {
+        if (!parts[k]) throw std::invalid_argument("null graph");
+        nv += parts[k]->vertices;
+        na += parts[k]->actions;
+        nf += parts[k]->faces;
+        nc += parts[k]->corners;
+    }
+    if (std::max({nv + 1, na, 3 * nf, nc}) > UINT32_MAX) throw std::invalid_argument("joined graph exceeds 32-bit ids");
+    DeviceGuard guard(device_);
+    auto g = std::make_unique<Graph>();
+    g->vertices = uint32_t(nv);
+    g->actions = uint32_t(na);
+    g->faces = uint32_t(nf);
+    g->corners = uint32_t(nc);
+    g->positions.resize(nv * 16);
+    g->indices.resize(nf * 12);
+    g->offsets.resize((nv + 1) * 4);
+    g->neighbors.resize(na * 4);
+    g->src.resize(na * 4);
+    g->reverse.resize(na * 4);
+    g->face_offsets.resize((nv + 1) * 4);
+    g->faces_of.resize(nc * 4);
+    g->boundary.resize(nv * 4);
+    g->features.resize(nv * kFeatures * 4);
+    g->features_bf16.resize(nv * kFeatures * 2);
+    uint32_t v0 = 0, a0 = 0, f0 = 0, c0 = 0;
+    const auto copy = [&](DeviceBuffer& to, size_t at, const DeviceBuffer& from, size_t bytes) {
+        if (bytes)
+            hip_check(hipMemcpyAsync(static_cast<char*>(to.p) + at, from.p, bytes, hipMemcpyDeviceToDevice, stream_),
+                      "join graphs");
+    };
+    for (size_t k = 0; k < count; ++k) {
+        const Graph& p = *parts[k];
+        copy(g->positions, size_t(v0) * 16, p.positions, size_t(p.vertices) * 16);
+        copy(g->boundary, size_t(v0) * 4, p.boundary, size_t(p.vertices) * 4);
+        copy(g->features, size_t(v0) * kFeatures * 4, p.features, size_t(p.vertices) * kFeatures * 4);
+        copy(g->features_bf16, size_t(v0) * kFeatures * 2, p.features_bf16, size_t(p.vertices) * kFeatures * 2);
+        launch_offset_copy(p.indices.as<uint32_t>(), g->indices.as<uint32_t>() + 3 * size_t(f0), 3 * size_t(p.faces),
+                           v0, stream_);
+        launch_offset_copy(p.offsets.as<uint32_t>(), g->offsets.as<uint32_t>() + v0, p.vertices, a0, stream_);
+        launch_offset_copy(p.neighbors.as<uint32_t>(), g->neighbors.as<uint32_t>() + a0, p.actions, v0, stream_);
+        launch_offset_copy(p.src.as<uint32_t>(), g->src.as<uint32_t>() + a0, p.actions, v0, stream_);
+        launch_offset_copy(p.reverse.as<uint32_t>(), g->reverse.as<uint32_t>() + a0, p.actions, a0, stream_);
+        launch_offset_copy(p.face_offsets.as<uint32_t>(), g->face_offsets.as<uint32_t>() + v0, p.vertices, c0, stream_);
+        launch_offset_copy(p.faces_of.as<uint32_t>(), g->faces_of.as<uint32_t>() + c0, p.corners, f0, stream_);
+        v0 += p.vertices;
+        a0 += p.actions;
+        f0 += p.faces;
+        c0 += p.corners;
+    }
+    const uint32_t ends[2] = {a0, c0};
+    hip_check(hipMemcpyAsync(g->offsets.as<uint32_t>() + v0, &ends[0], 4, hipMemcpyHostToDevice, stream_), "join");
+    hip_check(hipMemcpyAsync(g->face_offsets.as<uint32_t>() + v0, &ends[1], 4, hipMemcpyHostToDevice, stream_), "join");
+    size_forward(*g);
+    hip_check(hipGetLastError(), "join launch");
+    hip_check(hipStreamSynchronize(stream_), "join graphs");
     return g;
 }
 

@@ -26,7 +26,8 @@ Topology build_topology(const uint32_t* indices, size_t faces, size_t vertices);
 
 struct Graph {
     uint32_t vertices = 0, actions = 0, faces = 0;
-    double diameter = 0;
+    uint32_t corners = 0;  // entries of faces_of
+    double diameter = 0;   // of the bounds the features were computed with; 0 for a union of graphs
     DeviceBuffer positions, indices, offsets, neighbors, src, reverse, face_offsets, faces_of, boundary;
     DeviceBuffer features, features_bf16, outputs, h_a, h_b, z, hq_a, hq_b, zq, pq;
     DeviceBuffer keys, order, keys_out, order_out, sort_temp;
@@ -63,6 +64,11 @@ class Scorer {
     Scorer& operator=(const Scorer&) = delete;
     void set_weights(const float* weights, size_t count);
     std::unique_ptr<Graph> graph(const HostMesh& m, const Bounds& b);
+    // Disjoint union of graphs built by this scorer, in order: the vertex,
+    // action, face and face-list ids of each part are offset by the counts
+    // of the parts before it, and each part keeps the vertex features of its
+    // own bounds, so one training batch or forward pass spans many meshes.
+    std::unique_ptr<Graph> concat(const Graph* const* parts, size_t count);
     void forward(Graph& g, Precision p);
     void download_outputs(const Graph& g, float* out);
     void download_features(const Graph& g, float* out);

@@ -399,6 +399,11 @@ __global__ void to_bf16(const float* __restrict__ x, __bf16* __restrict__ y, siz
         y[i] = __bf16(x[i]);
 }
 
+__global__ void offset_copy(const uint32_t* __restrict__ x, uint32_t* __restrict__ y, size_t n, uint32_t add) {
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x)
+        y[i] = x[i] + add;
+}
+
 // ----------------------------------------------------------------- ranking
 
 // Scores of src/neural/action_gpu.cu:809-844: learned logit, shortest edge
@@ -587,6 +592,11 @@ size_t gemm_bf16_lds(uint32_t k) { return size_t(kHidden) * (k + kPad) * 2; }
 void launch_convert_bf16(const float* x, __bf16* y, size_t n, hipStream_t stream) {
     if (!n) return;
     to_bf16<<<dim3(uint32_t(std::min<size_t>((n + 255) / 256, 4096))), dim3(256), 0, stream>>>(x, y, n);
+}
+
+void launch_offset_copy(const uint32_t* x, uint32_t* y, size_t n, uint32_t add, hipStream_t stream) {
+    if (!n) return;
+    offset_copy<<<dim3(uint32_t(std::min<size_t>((n + 255) / 256, 4096))), dim3(256), 0, stream>>>(x, y, n, add);
 }
 
 void launch_aggregate_fp32(const float* h, GraphView g, uint32_t width, float* z, hipStream_t stream) {

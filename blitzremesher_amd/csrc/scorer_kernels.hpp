@@ -77,6 +77,8 @@ void launch_features(const FeatureInput& in, hipStream_t stream);
 void launch_forward_fp32(const float* x, GraphView g, const float* weights, float* h_a, float* h_b, float* z,
                          float* out, hipStream_t stream);
 void launch_convert_bf16(const float* x, __bf16* y, size_t n, hipStream_t stream);
+// y[i] = x[i] + add for i < n (id arrays of a graph placed after others).
+void launch_offset_copy(const uint32_t* x, uint32_t* y, size_t n, uint32_t add, hipStream_t stream);
 // x: vertices x kFeatures (BF16); h_a, h_b: vertices x kHidden (BF16);
 // z: vertices x 2 kHidden (BF16 scratch); p: vertices x 2 kHidden (BF16 scratch).
 void launch_forward_bf16(const __bf16* x, GraphView g, const __bf16* wq, const float* weights, __bf16* h_a,

@@ -237,3 +237,66 @@ def with_attributes(mesh: Mesh, seed: int, normal_noise: float = 0.05, colors: b
         rgb = 0.5 + 0.5 * np.sin(p * 3.0 + phase)
         col = np.concatenate([np.round(rgb * 255), np.full((count, 1), 255)], 1).astype(np.uint8)
     return Mesh(mesh.positions, mesh.indices, mesh.materials, mesh.double_sided, n.astype(np.float32), col)
+
+
+def uv_sphere(rings: int, segments: int, noise: float = 0.02, seed: int = 1) -> Mesh:
+    """Closed unit latitude-longitude sphere: `rings` latitude bands and
+    `segments` meridians (segments (rings - 1) + 2 vertices, 2 segments
+    (rings - 1) triangles, fans at both poles), each vertex scaled radially
+    by 1 + N(0, noise)."""
+    if rings < 2 or segments < 3:
+        raise ValueError("a sphere needs at least 2 rings and 3 segments")
+    theta = np.arange(1, rings) * np.pi / rings
+    phi = np.arange(segments) * 2 * np.pi / segments
+    t, p = np.meshgrid(theta, phi, indexing="ij")
+    body = np.stack([np.sin(t) * np.cos(p), np.sin(t) * np.sin(p), np.cos(t)], -1).reshape(-1, 3)
+    v = np.concatenate([body, [[0.0, 0.0, 1.0], [0.0, 0.0, -1.0]]])
+    north, south = len(body), len(body) + 1
+    r, s = np.meshgrid(np.arange(rings - 2), np.arange(segments), indexing="ij")
+    a = r * segments + s
+    b = r * segments + (s + 1) % segments
+    c, d = a + segments, b + segments
+    quads = np.stack([a, c, d, a, d, b], -1).reshape(-1, 3)
+    s = np.arange(segments)
+    top = np.stack([np.full(segments, north), s, (s + 1) % segments], -1)
+    last = (rings - 2) * segments
+    bottom = np.stack([np.full(segments, south), last + (s + 1) % segments, last + s], -1)
+    if noise:
+        v = v * (1.0 + noise * normal(seed, 0, len(v)))[:, None]
+    return Mesh(v.astype(np.float32), np.concatenate([top, quads, bottom]).astype(np.uint32))
+
+
+TRAINING_FAMILIES = ("sphere", "torus", "extruded_box", "heightfield")
+
+
+def training_mesh(family: str, triangles: int, seed: int) -> Mesh:
+    """One mesh of `family` with about `triangles` triangles (within a few
+    percent): a noisy latitude-longitude sphere (2 s (r - 1) triangles,
+    s = 2 r), a torus with normal noise (major = 2.5 minor), an extruded
+    box with a jittered top (4 n^2 + 8 n) or an fBm heightfield
+    (2 (n - 1)^2)."""
+    t = max(int(triangles), 64)
+    if family == "sphere":
+        r = max(2, int(round((1 + np.sqrt(1 + 4 * t / 4.0)) / 2)))
+        return uv_sphere(r, 2 * r, 0.02, seed)
+    if family == "torus":
+        minor = max(3, int(round(np.sqrt(t / 5.0))))
+        return torus(1.0, 0.3, max(3, int(round(2.5 * minor))), minor, 0.01, seed)
+    if family == "extruded_box":
+        return extruded_box(max(2, int(round(np.sqrt(1 + t / 4.0) - 1))), 0.5, seed)
+    if family == "heightfield":
+        return heightfield(max(3, int(round(np.sqrt(t / 2.0))) + 1), 0.1, seed)
+    raise ValueError(f"unknown mesh family {family!r}")
+
+
+def training_set(count: int = 64, low: int = 2000, high: int = 100000, seed: int = 5) -> list[tuple[str, Mesh]]:
+    """`count` meshes of uneven size for scorer training: triangle counts
+    log-uniform in [low, high] (splitmix stream 40 of `seed`), families in
+    turn sphere, torus, extruded box, heightfield, each mesh with its own
+    noise seed."""
+    if count < 1 or not 0 < low <= high:
+        raise ValueError("invalid training set size")
+    u = uniform(seed, 40, count)
+    sizes = np.exp(np.log(low) + u * (np.log(high) - np.log(low)))
+    return [(TRAINING_FAMILIES[i % 4], training_mesh(TRAINING_FAMILIES[i % 4], int(sizes[i]), seed * 1000 + i))
+            for i in range(count)]

@@ -104,6 +104,20 @@ class Scorer:
         self._graphs.append(g)
         return g
 
+    def concat(self, graphs: list[Graph]) -> Graph:
+        """Disjoint union of graphs of this scorer (blitz_hip_graph_concat):
+        part k's vertices and actions follow those of parts 0..k-1."""
+        parts = (C.c_void_p * len(graphs))(*[g.handle for g in graphs])
+        handle = C.c_void_p()
+        err = _lib.error_buffer()
+        _lib.check(self.lib.blitz_hip_graph_concat(self.handle, parts, len(graphs), C.byref(handle), err, len(err)),
+                   err)
+        nv, na = C.c_uint32(), C.c_uint32()
+        _lib.check(self.lib.blitz_hip_graph_info(handle, C.byref(nv), C.byref(na)))
+        g = Graph(self, handle, nv.value, na.value)
+        self._graphs.append(g)
+        return g
+
     def forward(self, graph: Graph, precision: int = FP32, download: bool = True) -> np.ndarray | None:
         out = np.zeros((graph.actions, OUTPUTS), np.float32) if download else None
         err = _lib.error_buffer()

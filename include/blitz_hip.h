@@ -612,6 +612,15 @@ blitz_status blitz_hip_scorer_set_weights(blitz_hip_scorer*, const float* weight
 blitz_status blitz_hip_graph_create(blitz_hip_scorer*, const blitz_mesh*, const blitz_hip_bounds*,
                                     blitz_hip_graph**, char* error, size_t capacity);
 void blitz_hip_graph_destroy(blitz_hip_graph*);
+/* Disjoint union of graphs of one scorer, in the order given: vertex,
+ * action and face ids of each part follow those of the parts before it, and
+ * each part keeps the vertex features of the bounds it was built with, so
+ * one forward pass or training batch spans several meshes. The reference
+ * trains on many meshes through its resident dataset
+ * (training/resident.hpp:344-488); this is the device side of that for the
+ * edge-graph scorer. The parts stay valid and independent. */
+blitz_status blitz_hip_graph_concat(blitz_hip_scorer*, blitz_hip_graph* const* parts, size_t count,
+                                    blitz_hip_graph** out, char* error, size_t capacity);
 blitz_status blitz_hip_graph_info(const blitz_hip_graph*, uint32_t* vertices, uint32_t* actions);
 blitz_status blitz_hip_graph_features(blitz_hip_scorer*, const blitz_hip_graph*, float* out, size_t capacity);
 blitz_status blitz_hip_graph_actions(const blitz_hip_graph*, uint32_t* src, uint32_t* dst, size_t capacity);
