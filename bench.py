@@ -642,7 +642,7 @@ def main() -> int:
         for _ in range(args.warmup):
             m = audit.evaluate(dref, dcand, center, radius, settings)
         sync()
-        raster_ms = compare_ms = distance_ms = band_ms = project_ms = 0.0
+        raster_ms = compare_ms = distance_ms = band_ms = project_ms = split_ms = 0.0
         launches = band_launches = 0
         distance_views = 0
         t0 = time.perf_counter()
@@ -652,6 +652,7 @@ def main() -> int:
             raster_ms += t["raster_ms"]
             band_ms += t["band_ms"]
             project_ms += t["project_ms"]
+            split_ms += t["split_ms"]
             band_launches += t["band_launches"]
             compare_ms += t["compare_ms"]
             distance_ms += t["distance_ms"]
@@ -711,6 +712,7 @@ def main() -> int:
                      "algorithmic_bytes_per_launch": per_launch_bytes,
                      "kernel_ms_per_launch": {k: round(v, 4) for k, v in kernels.items()},
                      "raster_pass_ms": round(raster_ms / launches, 4),
+                     "face_split_ms": round(split_ms / band_launches, 4),
                      "fp64_vector": {"kernel": "raster_band_kernel", "ops_per_launch": fp64_ops,
                                      "achieved": round(fp64_tflops, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                                      "frac": round(fp64_tflops / FP64_PEAK_TFLOPS, 4)}},

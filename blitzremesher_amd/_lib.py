@@ -81,7 +81,8 @@ class MeasurementC(C.Structure):
 class AuditOptions(C.Structure):
     _fields_ = [
         ("struct_size", C.c_uint32), ("version", C.c_uint32), ("device", C.c_int32),
-        ("memory_mib", C.c_uint32), ("cache_rasters", C.c_uint8), ("reserved", C.c_uint8 * 7),
+        ("memory_mib", C.c_uint32), ("cache_rasters", C.c_uint8), ("face_split", C.c_uint8),
+        ("reserved", C.c_uint8 * 6),
     ]
 
 
@@ -104,7 +105,7 @@ class AuditTiming(C.Structure):
         ("total_ms", C.c_double), ("raster_triangles", C.c_uint64), ("mask_bytes", C.c_uint64),
         ("project_ms", C.c_double), ("band_ms", C.c_double), ("band_launches", C.c_uint32),
         ("patch_fallbacks", C.c_uint32), ("patch_ms", C.c_double), ("appearance_ms", C.c_double),
-        ("attributed_views", C.c_uint32), ("reserved", C.c_uint32),
+        ("attributed_views", C.c_uint32), ("split_launches", C.c_uint32), ("split_ms", C.c_double),
     ]
 
 

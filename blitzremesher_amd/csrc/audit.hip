@@ -28,6 +28,9 @@ constexpr double kPi = 3.14159265358979323846;
 constexpr double kInf = std::numeric_limits<double>::infinity();
 constexpr uint64_t kMaxSamples = 64000000;  // src/evaluate.cpp:129, src/neural/internal.hpp:11
 constexpr uint32_t kNone = 0xFFFFFFFFu;
+// Meshes of at least this many faces get a triple table at upload and are
+// split against their candidates when the raster is large (raster_launch).
+constexpr uint32_t kSplitMinFaces = 4096;
 
 double dot(F3 a, F3 b) { return double(a.x) * b.x + double(a.y) * b.y + double(a.z) * b.z; }
 F3 cross(F3 a, F3 b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
@@ -138,10 +141,11 @@ DeviceMesh::~DeviceMesh() {
     (void)hipFree(indices);
     (void)hipFree(two_sided);
     (void)hipFree(materials);
+    (void)hipFree(face_table);
 }
 
-Audit::Audit(int device, uint64_t memory_bytes, bool cache_rasters)
-    : device_(device), memory_bytes_(memory_bytes), cache_rasters_(cache_rasters) {
+Audit::Audit(int device, uint64_t memory_bytes, bool cache_rasters, uint8_t face_split)
+    : device_(device), memory_bytes_(memory_bytes), cache_rasters_(cache_rasters), face_split_(face_split) {
     DeviceGuard guard(device_);
     hip_check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
     for (auto& e : events_) hip_check(hipEventCreate(&e), "hipEventCreate");
@@ -264,6 +268,13 @@ std::unique_ptr<DeviceMesh> Audit::upload(const HostMesh& m) {
         hip_check(hipMemcpyAsync(d->materials, m.materials.data(), size_t(d->faces) * 2, hipMemcpyHostToDevice,
                                  stream_),
                   "upload materials");
+    }
+    // The triple table of a mesh that may be the reference of a face split
+    // (raster_launch): built once per upload.
+    if (face_split_ == 2 || (face_split_ == 0 && d->faces >= kSplitMinFaces)) {
+        hip_check(hipMalloc(&d->face_table, size_t(face_table_slots(d->faces)) * 4), "hipMalloc");
+        launch_face_table(d->indices, d->faces, d->face_table, stream_);
+        hip_check(hipGetLastError(), "face table launch");
     }
     hip_check(hipStreamSynchronize(stream_), "upload");
     return d;
@@ -568,8 +579,28 @@ void Audit::launch_tasks(const std::vector<RasterTask>& list, size_t slot, uint3
     launch_raster(d, make_raster_grid(uint32_t(list.size()), tiles, faces), tiles, stream_);
 }
 
+FaceSplit Audit::split_faces(const DeviceMesh& a, const DeviceMesh& b) {
+    const size_t fa = std::max<uint32_t>(1, a.faces), fb = std::max<uint32_t>(1, b.faces);
+    uint32_t* blocks = scratch_as<uint32_t>(40, 3 * size_t(std::max(1u, face_split_blocks(a.faces, b.faces))));
+    uint8_t* matched = scratch_as<uint8_t>(41, (fa + 7) / 8 * 8);
+    uint32_t* match = scratch_as<uint32_t>(42, fb);
+    FaceSplit sp{};
+    sp.shared_tri = scratch_as<uint32_t>(43, 3 * fb);
+    sp.shared_ids_a = scratch_as<uint32_t>(44, 2 * fb);
+    sp.shared_ids_b = sp.shared_ids_a + fb;
+    sp.a_tri = scratch_as<uint32_t>(45, 4 * fa);
+    sp.a_ids = sp.a_tri + 3 * fa;
+    sp.b_tri = scratch_as<uint32_t>(46, 4 * fb);
+    sp.b_ids = sp.b_tri + 3 * fb;
+    sp.counts = scratch_as<uint32_t>(47, 4);
+    launch_face_split(a.indices, a.two_sided, a.faces, a.face_table, b.indices, b.two_sided, b.faces, match, matched,
+                      blocks, sp, stream_);
+    return sp;
+}
+
 void Audit::raster_launch(const std::vector<RasterJob>& jobs, const std::vector<Camera>& cams, const Bounds& b,
-                          uint8_t ss, uint32_t size, bool force_two, bool want_depth, RasterLaunch& out) {
+                          uint8_t ss, uint32_t size, bool force_two, bool want_depth, RasterLaunch& out,
+                          const std::vector<std::pair<uint32_t, uint32_t>>* pairs) {
     const uint32_t tiles = (size + kTile - 1) / kTile;
     const size_t words = size_t(tiles) * tiles;
     // One projection per (vertex stream contents, view).
@@ -654,10 +685,77 @@ void Audit::raster_launch(const std::vector<RasterJob>& jobs, const std::vector<
     hip_check(hipMemcpyAsync(dp, projections.data(), projections.size() * sizeof(ProjectTask), hipMemcpyHostToDevice,
                              stream_),
               "upload projections");
+    // Reference/candidate pairs over one vertex set: their faces split into
+    // shared, reference-only and candidate-only lists (one split for every
+    // pair: all pairs hold the same two meshes), each list one raster task.
+    // The split costs a hash build and probe over both meshes' faces, so
+    // only rasters with at least 4M face-views are split.
+    std::vector<RasterTask> list;
+    bool split = false;
+    if (pairs && !pairs->empty()) {
+        const DeviceMesh& ra = *jobs[pairs->front().first].mesh;
+        const DeviceMesh& cb = *jobs[pairs->front().second].mesh;
+        split = ra.face_table != nullptr &&
+                (face_split_ == 2 || (face_split_ == 0 && cb.faces >= kSplitMinFaces &&
+                                      uint64_t(ra.faces + cb.faces) * pairs->size() >= (uint64_t(1) << 22)));
+        for (const auto& p : *pairs)
+            split = split && jobs[p.first].mesh == &ra && jobs[p.second].mesh == &cb &&
+                    job_slot[p.first] == job_slot[p.second];
+    }
+    if (split) {
+        hip_check(hipEventRecord(events_[4], stream_), "event");
+        const DeviceMesh& ra = *jobs[pairs->front().first].mesh;
+        const DeviceMesh& cb = *jobs[pairs->front().second].mesh;
+        const FaceSplit sp = split_faces(ra, cb);
+        hip_check(hipEventRecord(events_[10], stream_), "event");
+        std::vector<int32_t> pair_of(jobs.size(), -1);
+        for (size_t k = 0; k < pairs->size(); ++k) {
+            pair_of[(*pairs)[k].first] = int32_t(k);
+            pair_of[(*pairs)[k].second] = int32_t(k);
+        }
+        for (size_t j = 0; j < jobs.size(); ++j) {
+            if (pair_of[j] < 0) {
+                list.push_back(tasks[j]);
+                continue;
+            }
+            if ((*pairs)[size_t(pair_of[j])].first != j) continue;  // emitted with its reference job
+            const RasterTask& r = tasks[j];
+            const RasterTask& c = tasks[(*pairs)[size_t(pair_of[j])].second];
+            RasterTask t = r;
+            t.two_sided = nullptr;
+            t.or_flush = 1;
+            // Shared faces: both masks, both meshes' status words.
+            t.indices = sp.shared_tri;
+            t.ids = sp.shared_ids_a;
+            t.mask2 = c.mask;
+            t.status2 = c.status;
+            t.ids2 = sp.shared_ids_b;
+            t.faces = cb.faces;
+            t.count = sp.counts;
+            list.push_back(t);
+            t.mask2 = nullptr;
+            t.status2 = nullptr;
+            t.ids2 = nullptr;
+            // Reference only.
+            t.indices = sp.a_tri;
+            t.ids = sp.a_ids;
+            t.faces = ra.faces;
+            t.count = sp.counts + 1;
+            list.push_back(t);
+            // Candidate only.
+            t.indices = sp.b_tri;
+            t.ids = sp.b_ids;
+            t.mask = c.mask;
+            t.status = c.status;
+            t.faces = cb.faces;
+            t.count = sp.counts + 2;
+            list.push_back(t);
+        }
+    }
     hip_check(hipEventRecord(events_[5], stream_), "event");
     launch_project(dp, uint32_t(projections.size()), max_vertices, stream_);
     hip_check(hipEventRecord(events_[6], stream_), "event");
-    launch_tasks(tasks, 4, tiles);
+    launch_tasks(split ? list : tasks, 4, tiles);
     hip_check(hipEventRecord(events_[7], stream_), "event");
     hip_check(hipGetLastError(), "raster launch");
     // Status words go to pinned memory without a wait: the caller may queue
@@ -675,6 +773,7 @@ void Audit::raster_launch(const std::vector<RasterJob>& jobs, const std::vector<
     hip_check(hipMemcpyAsync(pinned_status_, status, init.size() * 4, hipMemcpyDeviceToHost, stream_),
               "download status");
     out.tasks = std::move(tasks);
+    out.split = split;
     out.words = words;
     out.tiles = tiles;
     out.depth.assign(jobs.size(), nullptr);
@@ -693,6 +792,12 @@ bool Audit::raster_finish(RasterLaunch& l, std::vector<uint32_t>& status_out, st
         timing_.project_ms += t_p;
         timing_.band_ms += t_b;
         ++timing_.band_launches;
+        if (l.split) {
+            float t_s = 0;
+            hip_check(hipEventElapsedTime(&t_s, events_[4], events_[10]), "elapsed");
+            timing_.split_ms += t_s;
+            ++timing_.split_launches;
+        }
     }
     timing_.raster_tasks += uint32_t(n);
     for (const auto& t : l.tasks) timing_.raster_triangles += t.faces;
@@ -814,9 +919,29 @@ void Audit::run_items(const DeviceMesh& reference, const std::vector<const Devic
         // out-of-frame face are then redrawn, copied and measured again.
         // Appearance rasters need the final masks first.
         const bool deferred = !attributed;
+        // Each drawn reference view with its first candidate, when every
+        // such pair holds the same candidate mesh.
+        std::vector<std::pair<uint32_t, uint32_t>> split_pairs;
+        if (deferred) {
+            const DeviceMesh* only = nullptr;
+            bool one = true;
+            for (uint32_t i = 0; i < ncand; ++i) {
+                const DeviceMesh* c = candidates[all[begin + i].candidate];
+                one = one && (!only || only == c);
+                only = c;
+            }
+            std::vector<uint8_t> paired(nref, 0);
+            for (uint32_t i = 0; one && i < ncand; ++i) {
+                const uint32_t r = ref_slot[all[begin + i].view];
+                if (ref_job[r] == kNone || paired[r]) continue;
+                paired[r] = 1;
+                split_pairs.push_back({ref_job[r], cand_job[i]});
+            }
+            if (!one) split_pairs.clear();
+        }
         RasterLaunch rl;
         if (deferred)
-            raster_launch(jobs, cams, b, ss, size, s.force_two_sided, false, rl);
+            raster_launch(jobs, cams, b, ss, size, s.force_two_sided, false, rl, &split_pairs);
         else
             rasterize(jobs, cams, b, ss, size, s.force_two_sided, st, attributed ? &drawn : nullptr,
                       attributed ? &zdepth : nullptr);

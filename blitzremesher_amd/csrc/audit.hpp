@@ -67,6 +67,9 @@ struct DeviceMesh {
     float4* positions = nullptr;
     uint32_t* indices = nullptr;
     uint8_t* two_sided = nullptr;  // per face; null when no face is two-sided
+    // Open-addressing table of face ids keyed by the ordered vertex triple
+    // (face_split.hip), face_table_slots(faces) words; null when not built.
+    uint32_t* face_table = nullptr;
     // Appearance streams (include/blitz/mesh.hpp:20): unit vertex normals,
     // vertex colours / 255 and per-face materials; null when absent.
     float4* normals = nullptr;
@@ -124,6 +127,8 @@ struct Timing {
     uint32_t patch_fallbacks = 0;
     double appearance_ms = 0;  // attribute rasters, normal angle and attributed distance (device time)
     uint32_t attributed_views = 0;
+    uint32_t split_launches = 0;  // raster launches drawn from a reference/candidate face split
+    double split_ms = 0;          // face split kernels (device time)
 };
 
 // A reference bound for patch audits (patch_audit.hpp).
@@ -143,7 +148,9 @@ class Audit {
     // cache_rasters: reuse the reference's coverage masks across audits
     // against the same uploaded reference (the reference's RasterMemo,
     // src/neural/evaluate.cu:386-511); results are unchanged.
-    Audit(int device, uint64_t memory_bytes, bool cache_rasters = false);
+    // face_split: 0 automatic, 1 never, 2 whenever a reference/candidate
+    // pair shares a vertex set (raster_launch).
+    Audit(int device, uint64_t memory_bytes, bool cache_rasters = false, uint8_t face_split = 0);
     ~Audit();
     Audit(const Audit&) = delete;
     Audit& operator=(const Audit&) = delete;
@@ -240,11 +247,19 @@ class Audit {
     struct RasterLaunch {
         std::vector<RasterTask> tasks;
         std::vector<const double*> depth;  // per job: its vertex set's view depths, when requested
+        bool split = false;                // drawn from a reference/candidate face split
         size_t words = 0;
         uint32_t tiles = 0;
     };
+    // `pairs` (reference job, candidate job) of one view each, all with the
+    // same two meshes: drawn from their face split when they share a vertex
+    // set and the raster is large enough to repay it.
     void raster_launch(const std::vector<RasterJob>& jobs, const std::vector<Camera>& cams, const Bounds& b,
-                       uint8_t ss, uint32_t size, bool force_two, bool want_depth, RasterLaunch& out);
+                       uint8_t ss, uint32_t size, bool force_two, bool want_depth, RasterLaunch& out,
+                       const std::vector<std::pair<uint32_t, uint32_t>>* pairs = nullptr);
+    // Queues the face split of reference a and candidate b (face_split.hip)
+    // into scratch slots 40-47.
+    FaceSplit split_faces(const DeviceMesh& a, const DeviceMesh& b);
     bool raster_finish(RasterLaunch& l, std::vector<uint32_t>& status, std::vector<RasterTask>* drawn = nullptr,
                        std::vector<const double*>* depth = nullptr);
     void launch_tasks(const std::vector<RasterTask>& list, size_t slot, uint32_t tiles);
@@ -287,13 +302,14 @@ class Audit {
     int device_;
     uint64_t memory_bytes_;
     bool cache_rasters_ = false;
+    uint8_t face_split_ = 0;
     uint32_t* pinned_status_ = nullptr;  // status words of the last raster launch (pinned host memory)
     size_t pinned_status_words_ = 0;
     RasterCache cache_;
     std::map<std::pair<uint64_t, uint64_t>, bool> position_pairs_;  // owner serials -> equal
     bool device_equal(const void* a, const void* b, size_t bytes);
     hipStream_t stream_{};
-    hipEvent_t events_[10]{};
+    hipEvent_t events_[11]{};
     std::vector<std::pair<void*, size_t>> slots_;
     Timing timing_;
 };

@@ -44,7 +44,7 @@ __global__ __launch_bounds__(256) void project_kernel(const ProjectTask* __restr
 }
 
 // Faces whose bounding box is at most 4 samples wide and 8 high are
-// rasterized by their own lane, a row per step (lane_bitmap4); other faces
+// rasterized by their own lane, a row per step (lane_bitmap_wave); other faces
 // within 8x8 samples take one whole-wave pass (lane = sample, ballot =
 // bitmap); for larger ones the wavefront first sorts the face's tiles into
 // empty, fully covered and undecided (classify_rect), then produces each
@@ -87,8 +87,9 @@ __global__ __launch_bounds__(kRasterThreads) void raster_band_kernel(const Raste
     for (uint32_t i = threadIdx.x; i < words; i += blockDim.x) band[i] = 0;
     __syncthreads();
     const int ymin = int(ty0 * kTile), ymax = min(int(ty1 * kTile), int(t.size)) - 1;
-    const uint32_t per = (t.faces + grid.chunks - 1) / grid.chunks;
-    const uint32_t f0 = chunk * per, f1 = min(t.faces, f0 + per);
+    const uint32_t faces = task_faces(t);
+    const uint32_t per = (faces + grid.chunks - 1) / grid.chunks;
+    const uint32_t f0 = chunk * per, f1 = min(faces, f0 + per);
     const int lane = threadIdx.x & 63;
     const uint32_t step = blockDim.x;
     // Two-deep pipeline: corners of batch i+1 and indices of batch i+2 load
@@ -108,31 +109,35 @@ __global__ __launch_bounds__(kRasterThreads) void raster_band_kernel(const Raste
         next = load_corners(t, tri_next, clamp_face(f + step));
         tri_next = load_face(t, clamp_face(f + 2 * step));
         Setup s;
-        bool wide = false, small_box = false;
+        int kind = 0;
         if (f < f1) {
-            const int kind = setup_triangle(t, corners, s);
-            if (corners.behind) t.status[1] = 1u;
-            if (kind == 2) {
-                atomicMin(&t.status[0], f);
-            } else if (kind == 1 && s.y1 >= ymin && s.y0 <= ymax) {
-                const int w = s.x1 - s.x0 + 1, h = s.y1 - s.y0 + 1;
-                if (w <= 4 && h <= 8) {
-                    // At most 2x2 tiles: an 8x8 local bitmap, then four words.
-                    const int ya = max(s.y0, ymin), yb = min(s.y1, ymax);
-                    uint64_t word[4];
-                    scatter_bitmap(lane_bitmap4(s, ya, yb), s.x0, s.y0, word);
-                    // Word k goes to tile row (y0 >> 3) + (k >> 1), column
-                    // (x0 >> 3) + (k & 1); a nonzero word holds samples of
-                    // rows [ya, yb], which lie in the band.
-                    const uint32_t at = (uint32_t(s.y0 >> 3) - ty0) * t.tiles + uint32_t(s.x0 >> 3);
-                    if (word[0]) atomicOr(&band[at], static_cast<unsigned long long>(word[0]));
-                    if (word[1]) atomicOr(&band[at + 1], static_cast<unsigned long long>(word[1]));
-                    if (word[2]) atomicOr(&band[at + t.tiles], static_cast<unsigned long long>(word[2]));
-                    if (word[3]) atomicOr(&band[at + t.tiles + 1], static_cast<unsigned long long>(word[3]));
-                } else {
-                    wide = true;
-                    small_box = w <= 8 && h <= 8;
-                }
+            kind = setup_triangle(t, corners, s);
+            if (corners.behind) report_behind(t);
+            if (kind == 2) report_out_of_frame(t, f);
+        }
+        const bool drawn = kind == 1 && s.y1 >= ymin && s.y0 <= ymax;
+        const int w = drawn ? s.x1 - s.x0 + 1 : 0, h = drawn ? s.y1 - s.y0 + 1 : 0;
+        // Boxes at most 4 samples wide and 8 high: the lane path, an 8x8
+        // local bitmap over at most 2x2 tiles, then four words.
+        const bool lane_face = drawn && w <= 4 && h <= 8;
+        const bool wide = drawn && !lane_face, small_box = wide && w <= 8 && h <= 8;
+        if (__ballot(lane_face)) {
+            const int ya = max(s.y0, ymin), yb = min(s.y1, ymax);
+            const int rows = lane_face ? yb - ya + 1 : 0;
+            const int max_rows = wave_max_small(rows);
+            const uint64_t local = __ballot(lane_face && w == 4) ? lane_bitmap_wave<4>(s, ya, rows, max_rows, w)
+                                                                 : lane_bitmap_wave<3>(s, ya, rows, max_rows, w);
+            if (lane_face) {
+                uint64_t word[4];
+                scatter_bitmap(local, s.x0, s.y0, word);
+                // Word k goes to tile row (y0 >> 3) + (k >> 1), column
+                // (x0 >> 3) + (k & 1); a nonzero word holds samples of rows
+                // [ya, yb], which lie in the band.
+                const uint32_t at = (uint32_t(s.y0 >> 3) - ty0) * t.tiles + uint32_t(s.x0 >> 3);
+                if (word[0]) atomicOr(&band[at], static_cast<unsigned long long>(word[0]));
+                if (word[1]) atomicOr(&band[at + 1], static_cast<unsigned long long>(word[1]));
+                if (word[2]) atomicOr(&band[at + t.tiles], static_cast<unsigned long long>(word[2]));
+                if (word[3]) atomicOr(&band[at + t.tiles + 1], static_cast<unsigned long long>(word[3]));
             }
         }
         // A few faces within 8x8 boxes: one whole-wave pass each, lane l
@@ -217,12 +222,17 @@ __global__ __launch_bounds__(kRasterThreads) void raster_band_kernel(const Raste
     }
     __syncthreads();
     uint64_t* out = t.mask + size_t(ty0) * t.tiles;
-    if (grid.chunks == 1) {
+    if (grid.chunks == 1 && !t.or_flush) {
         for (uint32_t i = threadIdx.x; i < words; i += blockDim.x) out[i] = band[i];
     } else {
+        // A shared face list's band goes into both masks.
+        uint64_t* out2 = t.mask2 ? t.mask2 + size_t(ty0) * t.tiles : nullptr;
         for (uint32_t i = threadIdx.x; i < words; i += blockDim.x) {
             const unsigned long long v = band[i];
-            if (v) atomicOr(reinterpret_cast<unsigned long long*>(out + i), v);
+            if (v) {
+                atomicOr(reinterpret_cast<unsigned long long*>(out + i), v);
+                if (out2) atomicOr(reinterpret_cast<unsigned long long*>(out2 + i), v);
+            }
         }
     }
 }

@@ -49,6 +49,33 @@ struct RasterTask {
     uint32_t size;             // raster side in samples
     uint32_t tiles;            // ceil(size / 8)
     uint8_t force_two;
+    // Face lists of a split reference/candidate pair (face_split.hip): entry
+    // f's id in its own mesh (bit 31: two-sided), which replaces f in the
+    // status words and `two_sided`; null for a whole mesh.
+    const uint32_t* ids = nullptr;
+    // Shared faces: the second mesh's mask, status words and ids.
+    uint64_t* mask2 = nullptr;
+    uint32_t* status2 = nullptr;
+    const uint32_t* ids2 = nullptr;
+    // Device-side face count (list length known only on the device), at
+    // most `faces`; null: `faces`.
+    const uint32_t* count = nullptr;
+    // Flush with atomicOr even when the task has one chunk (another task
+    // writes the same mask).
+    uint8_t or_flush = 0;
+};
+
+// Shared / reference-only / candidate-only face lists of a split pair:
+// triples (3 per face) and ids (bit 31: two-sided); counts[0..2] = lengths.
+struct FaceSplit {
+    uint32_t* shared_tri;
+    uint32_t* shared_ids_a;
+    uint32_t* shared_ids_b;
+    uint32_t* a_tri;
+    uint32_t* a_ids;
+    uint32_t* b_tri;
+    uint32_t* b_ids;
+    uint32_t* counts;
 };
 
 // Launch shape of the band raster: every task is split into `bands` bands of

@@ -379,7 +379,9 @@ blitz_status blitz_hip_audit_create(const blitz_hip_audit_options* o, blitz_hip_
             throw std::invalid_argument("null pointer, options size or version mismatch");
         if (!blitz_hip_available(o->device)) throw Unavailable("no gfx950 HIP device at the requested index");
         auto a = std::make_unique<blitz_hip_audit>();
-        a->impl = std::make_unique<Audit>(o->device, uint64_t(o->memory_mib) << 20, o->cache_rasters != 0);
+        if (o->face_split > 2) throw std::invalid_argument("face_split must be 0, 1 or 2");
+        a->impl = std::make_unique<Audit>(o->device, uint64_t(o->memory_mib) << 20, o->cache_rasters != 0,
+                                          o->face_split);
         *out = a.release();
     });
 }
@@ -986,6 +988,8 @@ blitz_status blitz_hip_audit_last_timing(const blitz_hip_audit* a, blitz_hip_aud
     out->patch_ms = t.patch_ms;
     out->appearance_ms = t.appearance_ms;
     out->attributed_views = t.attributed_views;
+    out->split_launches = t.split_launches;
+    out->split_ms = t.split_ms;
     return BLITZ_OK;
 }
 

@@ -69,8 +69,29 @@ __device__ __forceinline__ Corners load_corners(const RasterTask& t, uint3 tri, 
     k.a = make_double2(xy[2 * size_t(tri.x)], xy[2 * size_t(tri.x) + 1]);
     k.b = make_double2(xy[2 * size_t(tri.y)], xy[2 * size_t(tri.y) + 1]);
     k.c = make_double2(xy[2 * size_t(tri.z)], xy[2 * size_t(tri.z) + 1]);
-    k.two = t.force_two || (t.two_sided && global(t.two_sided)[f]);
+    k.two = t.force_two || (t.ids ? (global(t.ids)[f] >> 31) != 0u : (t.two_sided && global(t.two_sided)[f]));
     return k;
+}
+
+// Face f's index in its own mesh (split face lists carry it in `ids`).
+__device__ __forceinline__ uint32_t mesh_face(const uint32_t* ids, uint32_t f) {
+    return ids ? global(ids)[f] & 0x7FFFFFFFu : f;
+}
+
+// Faces a task draws: its device-side list length when it has one.
+__device__ __forceinline__ uint32_t task_faces(const RasterTask& t) {
+    return t.count ? min(*global(t.count), t.faces) : t.faces;
+}
+
+// Status words of a face's task (and of the second mesh of a shared face
+// list): the first out-of-frame face in mesh order, the behind-camera flag.
+__device__ __forceinline__ void report_out_of_frame(const RasterTask& t, uint32_t f) {
+    atomicMin(&t.status[0], mesh_face(t.ids, f));
+    if (t.status2) atomicMin(&t.status2[0], mesh_face(t.ids2, f));
+}
+__device__ __forceinline__ void report_behind(const RasterTask& t) {
+    t.status[1] = 1u;
+    if (t.status2) t.status2[1] = 1u;
 }
 
 // Bounding box and frame test of src/evaluate.cpp:155-160 for corners a, d, e
@@ -277,6 +298,60 @@ __device__ __forceinline__ uint64_t lane_bitmap4(const Setup& s, int ya, int yb)
         for (int c = 0; c < 4; ++c) {
             // Bitwise | so that all three tests issue without branches.
             const bool in = (c < w) & !(((ru - cu[c]) < nu) | ((rv - cv[c]) < nv) | ((rw - cw[c]) < nw));
+            row |= in ? 1u << c : 0u;
+        }
+        if (k < 4)
+            lo |= row << (8 * k);
+        else
+            hi |= row << (8 * (k - 4));
+        py += 1.0;
+    }
+    return ((uint64_t(hi) << 32) | lo) << (8 * (ya - s.y0));
+}
+
+// Largest v over the wave's lanes, for 0 <= v < 16, as a wave-uniform value
+// (four ballots). Call with every lane of the wave active.
+__device__ __forceinline__ int wave_max_small(int v) {
+    int m = 0;
+#pragma unroll
+    for (int b = 8; b > 0; b >>= 1)
+        if (__ballot(v >= m + b)) m += b;
+    return m;
+}
+
+// lane_bitmap4 for a whole wave at once: C (3 or 4) columns and `max_rows`
+// steps for every lane (both wave-uniform: the widest box and the most rows
+// of the wave's lane-path faces), the lane's own width `w` and row count
+// `rows` (0 for a lane with no such face) masking its bits. A uniform loop
+// keeps the row counter and the bit positions in scalar registers, and a
+// wave whose boxes are at most 3 wide (nearly all on fine meshes) skips the
+// fourth column. Same samples, same rounding, same bitmap as lane_bitmap4.
+template <int C>
+__device__ __forceinline__ uint64_t lane_bitmap_wave(const Setup& s, int ya, int rows, int max_rows, int w) {
+    const double au = s.cx - s.bx, bu = s.cy - s.by;
+    const double av = s.ax - s.cx, bv = s.ay - s.cy;
+    const double aw = s.bx - s.ax, bw = s.by - s.ay;
+    const double px0 = double(s.x0) + .5;
+    double cu[C], cv[C], cw[C];
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+        const double px = px0 + double(c);
+        cu[c] = bu * (px - s.bx);
+        cv[c] = bv * (px - s.cx);
+        cw[c] = bw * (px - s.ax);
+    }
+    // fl(u + r) < 0 tested as u < -r (see covers()).
+    const double nu = -s.r0, nv = -s.r1, nw = -s.r2;
+    double py = double(ya) + .5;
+    uint32_t lo = 0, hi = 0;
+    for (int k = 0; k < max_rows; ++k) {
+        const double ru = au * (py - s.by), rv = av * (py - s.cy), rw = aw * (py - s.ay);
+        const bool live = k < rows;
+        uint32_t row = 0;
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+            // Bitwise | so that all three tests issue without branches.
+            const bool in = live & (c < w) & !(((ru - cu[c]) < nu) | ((rv - cv[c]) < nv) | ((rw - cw[c]) < nw));
             row |= in ? 1u << c : 0u;
         }
         if (k < 4)

@@ -128,7 +128,10 @@ typedef struct blitz_hip_audit_options {
     uint32_t memory_mib;    /* workspace cap; 0 = no cap */
     uint8_t cache_rasters;  /* reuse the reference's masks across audits of one uploaded reference
                              * (RasterMemo, src/neural/evaluate.cu:386-511); 0 by default */
-    uint8_t reserved[7];
+    uint8_t face_split;     /* reference/candidate face split of coverage rasters over a shared vertex
+                             * set: 0 automatic (large rasters), 1 never, 2 whenever possible; the
+                             * masks are the same either way */
+    uint8_t reserved[6];
 } blitz_hip_audit_options;
 
 typedef struct blitz_hip_audit blitz_hip_audit;             /* AuditCuda (src/neural/internal.hpp:131) */
@@ -148,7 +151,8 @@ typedef struct blitz_hip_audit_timing {
     double patch_ms;           /* patch kernel launches (device time) */
     double appearance_ms;      /* attribute rasters, normal angle, attributed distance (device time) */
     uint32_t attributed_views; /* views whose attributed distance was measured */
-    uint32_t reserved;
+    uint32_t split_launches;   /* raster launches drawn from a reference/candidate face split */
+    double split_ms;           /* face split kernels (device time) */
 } blitz_hip_audit_timing;
 
 /* ---- audit ---------------------------------------------------------------- */
