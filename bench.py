@@ -661,6 +661,7 @@ def main() -> int:
         sync()
         elapsed = time.perf_counter() - t0
         ms_per_step = 1000.0 * elapsed / args.steps
+        audit_chunks = audit.last_timing()["raster_chunks"]
         samples = (int(-(-(settings.screen_size + 8) // 1)) * settings.supersample) ** 2
         views = settings.orthographic + settings.perspective
         b = algorithmic_bytes(ref, cand, views, samples, views)
@@ -713,6 +714,7 @@ def main() -> int:
                      "kernel_ms_per_launch": {k: round(v, 4) for k, v in kernels.items()},
                      "raster_pass_ms": round(raster_ms / launches, 4),
                      "face_split_ms": round(split_ms / band_launches, 4),
+                     "raster_chunks_per_view": audit_chunks,
                      "fp64_vector": {"kernel": "raster_band_kernel", "ops_per_launch": fp64_ops,
                                      "achieved": round(fp64_tflops, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                                      "frac": round(fp64_tflops / FP64_PEAK_TFLOPS, 4)}},

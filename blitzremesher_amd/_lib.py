@@ -82,7 +82,7 @@ class AuditOptions(C.Structure):
     _fields_ = [
         ("struct_size", C.c_uint32), ("version", C.c_uint32), ("device", C.c_int32),
         ("memory_mib", C.c_uint32), ("cache_rasters", C.c_uint8), ("face_split", C.c_uint8),
-        ("reserved", C.c_uint8 * 6),
+        ("autotune", C.c_uint8), ("reserved", C.c_uint8 * 5),
     ]
 
 
@@ -106,6 +106,7 @@ class AuditTiming(C.Structure):
         ("project_ms", C.c_double), ("band_ms", C.c_double), ("band_launches", C.c_uint32),
         ("patch_fallbacks", C.c_uint32), ("patch_ms", C.c_double), ("appearance_ms", C.c_double),
         ("attributed_views", C.c_uint32), ("split_launches", C.c_uint32), ("split_ms", C.c_double),
+        ("raster_chunks", C.c_uint32), ("tuned_launches", C.c_uint32),
     ]
 
 

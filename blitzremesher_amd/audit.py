@@ -278,7 +278,8 @@ class Binding:
 class Audit:
     """One HIP audit object (AuditCuda counterpart) on one device."""
 
-    def __init__(self, device: int = 0, memory_mib: int = 0, cache_rasters: bool = False, face_split: int = 0):
+    def __init__(self, device: int = 0, memory_mib: int = 0, cache_rasters: bool = False, face_split: int = 0,
+                 autotune: bool = True):
         lib = _lib.load()
         self.lib = lib
         opts = _lib.AuditOptions()
@@ -287,6 +288,7 @@ class Audit:
         opts.memory_mib = memory_mib
         opts.cache_rasters = int(cache_rasters)
         opts.face_split = int(face_split)
+        opts.autotune = int(autotune)
         handle = C.c_void_p()
         err = _lib.error_buffer()
         _lib.check(lib.blitz_hip_audit_create(C.byref(opts), C.byref(handle), err, len(err)), err)

@@ -144,8 +144,9 @@ DeviceMesh::~DeviceMesh() {
     (void)hipFree(face_table);
 }
 
-Audit::Audit(int device, uint64_t memory_bytes, bool cache_rasters, uint8_t face_split)
-    : device_(device), memory_bytes_(memory_bytes), cache_rasters_(cache_rasters), face_split_(face_split) {
+Audit::Audit(int device, uint64_t memory_bytes, bool cache_rasters, uint8_t face_split, bool autotune)
+    : device_(device), memory_bytes_(memory_bytes), cache_rasters_(cache_rasters), face_split_(face_split),
+      tuner_(autotune) {
     DeviceGuard guard(device_);
     hip_check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
     for (auto& e : events_) hip_check(hipEventCreate(&e), "hipEventCreate");
@@ -272,8 +273,8 @@ std::unique_ptr<DeviceMesh> Audit::upload(const HostMesh& m) {
     // The triple table of a mesh that may be the reference of a face split
     // (raster_launch): built once per upload.
     if (face_split_ == 2 || (face_split_ == 0 && d->faces >= kSplitMinFaces)) {
-        hip_check(hipMalloc(&d->face_table, size_t(face_table_slots(d->faces)) * 4), "hipMalloc");
-        launch_face_table(d->indices, d->faces, d->face_table, stream_);
+        hip_check(hipMalloc(&d->face_table, size_t(face_table_slots(d->faces)) * sizeof(uint4)), "hipMalloc");
+        launch_face_table(d->indices, d->two_sided, d->faces, d->face_table, stream_);
         hip_check(hipGetLastError(), "face table launch");
     }
     hip_check(hipStreamSynchronize(stream_), "upload");
@@ -576,7 +577,11 @@ void Audit::launch_tasks(const std::vector<RasterTask>& list, size_t slot, uint3
               "upload raster tasks");
     uint32_t faces = 0;
     for (const auto& t : list) faces = std::max(faces, t.faces);
-    launch_raster(d, make_raster_grid(uint32_t(list.size()), tiles, faces), tiles, stream_);
+    const auto launch = [&](const RasterGrid& g) { launch_raster(d, g, tiles, stream_); };
+    const RasterGrid g = tuner_.grid(uint32_t(list.size()), tiles, faces, stream_, launch);
+    launch(g);
+    timing_.raster_chunks = g.chunks;
+    timing_.tuned_launches += tuner_.last_tuned();
 }
 
 FaceSplit Audit::split_faces(const DeviceMesh& a, const DeviceMesh& b) {

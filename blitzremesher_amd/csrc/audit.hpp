@@ -13,6 +13,7 @@
 #include <vector>
 
 #include "audit_types.hpp"
+#include "autotune.hpp"
 
 namespace blitz_hip {
 
@@ -67,9 +68,9 @@ struct DeviceMesh {
     float4* positions = nullptr;
     uint32_t* indices = nullptr;
     uint8_t* two_sided = nullptr;  // per face; null when no face is two-sided
-    // Open-addressing table of face ids keyed by the ordered vertex triple
-    // (face_split.hip), face_table_slots(faces) words; null when not built.
-    uint32_t* face_table = nullptr;
+    // Open-addressing table of faces keyed by the ordered vertex triple
+    // (face_split.hip), face_table_slots(faces) slots; null when not built.
+    uint4* face_table = nullptr;
     // Appearance streams (include/blitz/mesh.hpp:20): unit vertex normals,
     // vertex colours / 255 and per-face materials; null when absent.
     float4* normals = nullptr;
@@ -129,6 +130,8 @@ struct Timing {
     uint32_t attributed_views = 0;
     uint32_t split_launches = 0;  // raster launches drawn from a reference/candidate face split
     double split_ms = 0;          // face split kernels (device time)
+    uint32_t raster_chunks = 0;   // face chunks per task of the last raster launch
+    uint32_t tuned_launches = 0;  // raster launches whose shape was measured in this call
 };
 
 // A reference bound for patch audits (patch_audit.hpp).
@@ -150,7 +153,9 @@ class Audit {
     // src/neural/evaluate.cu:386-511); results are unchanged.
     // face_split: 0 automatic, 1 never, 2 whenever a reference/candidate
     // pair shares a vertex set (raster_launch).
-    Audit(int device, uint64_t memory_bytes, bool cache_rasters = false, uint8_t face_split = 0);
+    // autotune: pick the raster launch shape by timing (autotune.hip).
+    Audit(int device, uint64_t memory_bytes, bool cache_rasters = false, uint8_t face_split = 0,
+          bool autotune = true);
     ~Audit();
     Audit(const Audit&) = delete;
     Audit& operator=(const Audit&) = delete;
@@ -303,6 +308,7 @@ class Audit {
     uint64_t memory_bytes_;
     bool cache_rasters_ = false;
     uint8_t face_split_ = 0;
+    RasterTuner tuner_;
     uint32_t* pinned_status_ = nullptr;  // status words of the last raster launch (pinned host memory)
     size_t pinned_status_words_ = 0;
     RasterCache cache_;

@@ -27,15 +27,15 @@ void launch_distance(const ViewPair* pairs, ViewSummary* summaries, uint32_t cou
 void launch_hausdorff(const ViewPair* pairs, ViewSummary* summaries, uint32_t count, uint32_t tiles,
                       hipStream_t stream);
 // Face split of mesh a (reference) and mesh b (candidate) over one vertex set
-// (face_split.hip). A mesh's triple table (face_table_slots(faces) words) is
-// built once by launch_face_table. Scratch of a split: `match` of b_faces
+// (face_split.hip). A mesh's triple table (face_table_slots(faces) slots of
+// 16 bytes) is built once by launch_face_table. Scratch of a split: `match` of b_faces
 // words, `matched` of a_faces bytes rounded up to 8, `block_counts` of
 // 3 * face_split_blocks() words; `out` lists sized for b_faces shared,
 // a_faces a-only and b_faces b-only entries.
 uint32_t face_table_slots(uint32_t faces);
 uint32_t face_split_blocks(uint32_t a_faces, uint32_t b_faces);
-void launch_face_table(const uint32_t* idx, uint32_t faces, uint32_t* table, hipStream_t stream);
-void launch_face_split(const uint32_t* a_idx, const uint8_t* a_two, uint32_t a_faces, const uint32_t* a_table,
+void launch_face_table(const uint32_t* idx, const uint8_t* two, uint32_t faces, uint4* table, hipStream_t stream);
+void launch_face_split(const uint32_t* a_idx, const uint8_t* a_two, uint32_t a_faces, const uint4* a_table,
                        const uint32_t* b_idx, const uint8_t* b_two, uint32_t b_faces, uint32_t* match,
                        uint8_t* matched, uint32_t* block_counts, const FaceSplit& out, hipStream_t stream);
 }  // namespace blitz_hip

@@ -315,6 +315,7 @@ blitz_status blitz_hip_audit_options_init(blitz_hip_audit_options* o, size_t n) 
     o->device = 0;
     o->memory_mib = 0;
     o->cache_rasters = 0;
+    o->autotune = 1;
     return BLITZ_OK;
 }
 
@@ -381,7 +382,7 @@ blitz_status blitz_hip_audit_create(const blitz_hip_audit_options* o, blitz_hip_
         auto a = std::make_unique<blitz_hip_audit>();
         if (o->face_split > 2) throw std::invalid_argument("face_split must be 0, 1 or 2");
         a->impl = std::make_unique<Audit>(o->device, uint64_t(o->memory_mib) << 20, o->cache_rasters != 0,
-                                          o->face_split);
+                                          o->face_split, o->autotune != 0);
         *out = a.release();
     });
 }
@@ -990,6 +991,8 @@ blitz_status blitz_hip_audit_last_timing(const blitz_hip_audit* a, blitz_hip_aud
     out->attributed_views = t.attributed_views;
     out->split_launches = t.split_launches;
     out->split_ms = t.split_ms;
+    out->raster_chunks = t.raster_chunks;
+    out->tuned_launches = t.tuned_launches;
     return BLITZ_OK;
 }
 

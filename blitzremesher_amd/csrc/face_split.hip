@@ -17,9 +17,11 @@
 // and the raster sets up 28 % fewer faces.
 //
 // Pairing: every uploaded mesh that may be a reference holds an
-// open-addressing table of its face ids keyed by the ordered triple (built
-// once, at upload; linear probing, at most half full, so every probe sequence
-// ends at an empty slot). Per raster, each candidate face probes the
+// open-addressing table of its faces keyed by the ordered triple (built once,
+// at upload; linear probing, at most half full, so every probe sequence ends
+// at an empty slot). A slot holds the triple and the face id with its
+// sidedness bit, so a probe step is one 16-byte load. Per raster, each
+// candidate face probes the
 // reference's table; list positions come from per-block counts and one scan
 // of them, so each list keeps its mesh's face order. List entries carry the
 // face's id in its own mesh (bit 31: drawn two-sided), so out-of-frame faces
@@ -33,9 +35,11 @@ namespace {
 
 constexpr uint32_t kEmpty = 0xFFFFFFFFu;
 constexpr uint32_t kTwoSided = 0x80000000u;
-// Faces per block of the count and scatter kernels: 256 threads x 16 rounds.
+// Faces per block of the count and scatter kernels: 256 threads x 4 rounds
+// (about 900 blocks for the 914,000 faces of a 522,000-face reference and
+// its 392,000-face candidate).
 constexpr uint32_t kSplitThreads = 256;
-constexpr uint32_t kSplitRounds = 16;
+constexpr uint32_t kSplitRounds = 4;
 constexpr uint32_t kSplitBlock = kSplitThreads * kSplitRounds;
 
 __device__ __forceinline__ uint32_t triple_hash(uint32_t a, uint32_t b, uint32_t c) {
@@ -56,21 +60,30 @@ __device__ __forceinline__ uint32_t sided(const uint8_t* two, uint32_t f) {
     return two && two[f] ? kTwoSided : 0u;
 }
 
-__global__ __launch_bounds__(256) void table_clear_kernel(uint32_t* table, uint32_t slots) {
+__global__ __launch_bounds__(256) void table_clear_kernel(uint4* table, uint32_t slots) {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < slots; i += gridDim.x * blockDim.x)
-        table[i] = kEmpty;
+        table[i] = make_uint4(0, 0, 0, kEmpty);
 }
 
-__global__ __launch_bounds__(256) void table_insert_kernel(const uint32_t* __restrict__ idx, uint32_t faces,
-                                                           uint32_t* table, uint32_t mask) {
+// Slot: (triple, face id | sidedness bit); the id word is claimed first, the
+// triple written after (probes run in a later launch).
+__global__ __launch_bounds__(256) void table_insert_kernel(const uint32_t* __restrict__ idx,
+                                                           const uint8_t* __restrict__ two, uint32_t faces,
+                                                           uint4* table, uint32_t mask) {
     const uint32_t f = blockIdx.x * blockDim.x + threadIdx.x;
     if (f >= faces) return;
     const uint3 t = triple(idx, f);
+    const uint32_t id = f | sided(two, f);
     uint32_t h = triple_hash(t.x, t.y, t.z) & mask;
     // At most `faces` of the > 2 * faces slots are ever taken, so a free slot
     // is reached within `faces` steps.
     for (uint32_t step = 0; step <= mask; ++step) {
-        if (atomicCAS(&table[h], kEmpty, f) == kEmpty) return;
+        if (atomicCAS(&table[h].w, kEmpty, id) == kEmpty) {
+            table[h].x = t.x;
+            table[h].y = t.y;
+            table[h].z = t.z;
+            return;
+        }
         h = (h + 1) & mask;
     }
 }
@@ -78,9 +91,7 @@ __global__ __launch_bounds__(256) void table_insert_kernel(const uint32_t* __res
 // match[g]: the first reference face (in probe order) with candidate face g's
 // triple and sidedness, or kEmpty; matched[f] = 1 for every reference face
 // some candidate face matched (plain stores of one value).
-__global__ __launch_bounds__(256) void split_probe_kernel(const uint32_t* __restrict__ a_idx,
-                                                          const uint8_t* __restrict__ a_two,
-                                                          const uint32_t* __restrict__ table, uint32_t mask,
+__global__ __launch_bounds__(256) void split_probe_kernel(const uint4* __restrict__ table, uint32_t mask,
                                                           const uint32_t* __restrict__ b_idx,
                                                           const uint8_t* __restrict__ b_two, uint32_t b_faces,
                                                           uint32_t* match, uint8_t* matched) {
@@ -90,11 +101,10 @@ __global__ __launch_bounds__(256) void split_probe_kernel(const uint32_t* __rest
     const uint32_t side = sided(b_two, g);
     uint32_t h = triple_hash(t.x, t.y, t.z) & mask, found = kEmpty;
     for (uint32_t step = 0; step <= mask; ++step) {
-        const uint32_t f = table[h];
-        if (f == kEmpty) break;
-        const uint3 u = triple(a_idx, f);
-        if (u.x == t.x && u.y == t.y && u.z == t.z && sided(a_two, f) == side) {
-            found = f;
+        const uint4 e = table[h];
+        if (e.w == kEmpty) break;
+        if (e.x == t.x && e.y == t.y && e.z == t.z && (e.w & kTwoSided) == side) {
+            found = e.w & ~kTwoSided;
             break;
         }
         h = (h + 1) & mask;
@@ -245,20 +255,20 @@ uint32_t face_split_blocks(uint32_t a_faces, uint32_t b_faces) {
     return (a_faces + kSplitBlock - 1) / kSplitBlock + (b_faces + kSplitBlock - 1) / kSplitBlock;
 }
 
-void launch_face_table(const uint32_t* idx, uint32_t faces, uint32_t* table, hipStream_t stream) {
+void launch_face_table(const uint32_t* idx, const uint8_t* two, uint32_t faces, uint4* table, hipStream_t stream) {
     const uint32_t slots = face_table_slots(faces);
     table_clear_kernel<<<std::min<uint32_t>(2048, (slots + 255) / 256), 256, 0, stream>>>(table, slots);
-    if (faces) table_insert_kernel<<<(faces + 255) / 256, 256, 0, stream>>>(idx, faces, table, slots - 1);
+    if (faces) table_insert_kernel<<<(faces + 255) / 256, 256, 0, stream>>>(idx, two, faces, table, slots - 1);
 }
 
-void launch_face_split(const uint32_t* a_idx, const uint8_t* a_two, uint32_t a_faces, const uint32_t* a_table,
+void launch_face_split(const uint32_t* a_idx, const uint8_t* a_two, uint32_t a_faces, const uint4* a_table,
                        const uint32_t* b_idx, const uint8_t* b_two, uint32_t b_faces, uint32_t* match,
                        uint8_t* matched, uint32_t* block_counts, const FaceSplit& out, hipStream_t stream) {
     const uint32_t mask = face_table_slots(a_faces) - 1;
     if (a_faces) launch_fill(reinterpret_cast<uint64_t*>(matched), (a_faces + 7) / 8, 0, stream);
     if (b_faces)
-        split_probe_kernel<<<(b_faces + 255) / 256, 256, 0, stream>>>(a_idx, a_two, a_table, mask, b_idx, b_two,
-                                                                       b_faces, match, matched);
+        split_probe_kernel<<<(b_faces + 255) / 256, 256, 0, stream>>>(a_table, mask, b_idx, b_two, b_faces, match,
+                                                                       matched);
     SplitSides s{a_faces, b_faces, (b_faces + kSplitBlock - 1) / kSplitBlock, match, matched};
     const uint32_t blocks = face_split_blocks(a_faces, b_faces);
     if (blocks) split_count_kernel<<<blocks, kSplitThreads, 0, stream>>>(s, block_counts);

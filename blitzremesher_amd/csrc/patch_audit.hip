@@ -110,7 +110,8 @@ std::unique_ptr<Binding> Audit::bind(const HostMesh& reference, const Bounds& b,
     bd->tasks.resize(views * sizeof(RasterTask));
     hip_check(hipMemcpyAsync(bd->tasks.p, tasks.data(), views * sizeof(RasterTask), hipMemcpyHostToDevice, stream_),
               "upload raster tasks");
-    launch_raster(bd->tasks.as<RasterTask>(), make_raster_grid(views, tiles, faces), tiles, stream_);
+    const auto launch = [&](const RasterGrid& g) { launch_raster(bd->tasks.as<RasterTask>(), g, tiles, stream_); };
+    launch(tuner_.grid(views, tiles, faces, stream_, launch));
     // Covered samples per view (compare of each mask with itself).
     std::vector<ViewPair> pairs(views);
     for (uint32_t v = 0; v < views; ++v) {

@@ -131,7 +131,9 @@ typedef struct blitz_hip_audit_options {
     uint8_t face_split;     /* reference/candidate face split of coverage rasters over a shared vertex
                              * set: 0 automatic (large rasters), 1 never, 2 whenever possible; the
                              * masks are the same either way */
-    uint8_t reserved[6];
+    uint8_t autotune;       /* 1 (default): time the raster launch shapes on the first launch of each
+                             * shape class and keep the fastest; 0: fixed heuristic shapes */
+    uint8_t reserved[5];
 } blitz_hip_audit_options;
 
 typedef struct blitz_hip_audit blitz_hip_audit;             /* AuditCuda (src/neural/internal.hpp:131) */
@@ -153,6 +155,8 @@ typedef struct blitz_hip_audit_timing {
     uint32_t attributed_views; /* views whose attributed distance was measured */
     uint32_t split_launches;   /* raster launches drawn from a reference/candidate face split */
     double split_ms;           /* face split kernels (device time) */
+    uint32_t raster_chunks;    /* face chunks per task of the last raster launch */
+    uint32_t tuned_launches;   /* raster launches whose shape was measured in this call */
 } blitz_hip_audit_timing;
 
 /* ---- audit ---------------------------------------------------------------- */

@@ -105,3 +105,29 @@ def test_gpu_split_and_whole_meshes_give_the_same_measurements():
             got[mode] = [audit.evaluate(dref, dcand, center, radius, s).as_tuple() for _ in range(2)]
             assert (audit.last_timing()["split_launches"] >= 1) == (mode == 2)
     assert got[1] == got[2]
+
+
+@pytest.mark.gpu
+def test_gpu_autotuned_raster_shapes_give_the_same_measurements():
+    """The raster launch shape measured on the first launch of a shape class
+    (every candidate drawn into the same masks) changes no Measurement field;
+    the second audit of the class reuses the choice."""
+    from blitzremesher_amd import Audit
+    ref, cand = heightfield_pair()
+    center, radius = bounds(ref)
+    # A raster side no other test uses, so this shape class is new here.
+    s = EvalSettings(profile=0, screen_size=57, supersample=4, max_supersample=4, limit=1e3, orthographic=24,
+                     perspective=0, rotation_seed=0xB1172026, force_two_sided=True)
+    with Audit(0, autotune=False) as audit:
+        dref, dcand = audit.upload(ref), audit.upload(cand)
+        want = audit.evaluate(dref, dcand, center, radius, s).as_tuple()
+        assert audit.last_timing()["tuned_launches"] == 0
+    with Audit(0, autotune=True) as audit:
+        dref, dcand = audit.upload(ref), audit.upload(cand)
+        first = audit.evaluate(dref, dcand, center, radius, s).as_tuple()
+        tuned = audit.last_timing()["tuned_launches"]
+        second = audit.evaluate(dref, dcand, center, radius, s).as_tuple()
+        assert audit.last_timing()["tuned_launches"] == 0
+    assert tuned >= 1
+    assert first == want and second == want
+    assert want == Oracle("oracle").evaluate(ref, cand, center, radius, s).as_tuple()
