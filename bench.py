@@ -368,7 +368,7 @@ def neural_leg(train_steps: int, budget: int) -> dict:
     return out
 
 
-def training_leg(steps: int, count: int, pools_per_mesh: int, holdout: int = 8) -> dict:
+def training_leg(steps: int, count: int, pools_per_mesh: int, holdout: int = 8, chain_repeats: int = 2) -> dict:
     """Scorer training from method-generated labels: `count` seeded meshes of
     uneven size (2k-100k triangles, log-uniform; spheres, tori, extruded
     boxes, heightfields, seed 5); up to `pools_per_mesh` teacher pools of 16
@@ -440,6 +440,43 @@ def training_leg(steps: int, count: int, pools_per_mesh: int, holdout: int = 8) 
                                "held_out_before": round(top1_preferred(data, before, held_pools), 4),
                                "held_out_after": round(top1_preferred(data, after, held_pools), 4)}}
         data.graph.release()
+        out["chain_learned_vs_plain"] = learned_chain(scorer, chain_repeats)
+    return out
+
+
+def learned_chain(scorer, repeats: int) -> dict:
+    """The chain leg's 4-level chain of the 10,242-vertex icosphere (not one
+    of the training meshes) with the executor's collapses ranked by the
+    trained scorer's BF16 logits (ranking 0) and by the plain current-plane
+    order (ranking 4), same gates and budgets: ms per chain, four-gate
+    audited collapses (and per second), trials, rejected collapses and the
+    triangles of every level."""
+    import numpy as np
+    from blitzremesher_amd import Audit, meshes
+    from blitzremesher_amd.chain import ChainOptions, generate
+    mesh = meshes.icosphere(5, 0.02, 1)
+    settings = c1_chain_settings()
+    out = {}
+    for name, ranking in (("learned", 0), ("plain", 4)):
+        options = ChainOptions(action_trials=64, action_batch=32, ranking=ranking, overdraw_tiebreak=True)
+        with Audit(0, cache_rasters=True) as audit:
+            generate(audit, mesh, settings, options, scorer=scorer)  # warm-up
+            runs = [generate(audit, mesh, settings, options, scorer=scorer) for _ in range(repeats)]
+        r = runs[-1]
+        ms = float(np.median([x.info["ms"] for x in runs]))
+        i = r.info
+        out[name] = {"ms_per_chain": round(ms, 1), "triangles": [l.triangles for l in r.lods],
+                     "four_gate_audited_collapses": i["legal_collapses"],
+                     "four_gate_audited_collapses_per_s": round(i["legal_collapses"] / (ms * 1e-3), 1),
+                     "rejected_collapses": i["rejected_collapses"], "trials": i["action_trials"],
+                     "levels_passed": all(l.source.passed and l.adjacent.passed for l in r.lods[1:]),
+                     "identical_runs": all([l.triangles for l in x.lods] == [l.triangles for l in r.lods]
+                                           for x in runs)}
+    lp, pp = out["learned"], out["plain"]
+    out["ratio_collapses_per_s"] = round(lp["four_gate_audited_collapses_per_s"] /
+                                         max(pp["four_gate_audited_collapses_per_s"], 1e-9), 3)
+    out["learned_triangles_at_most_plain"] = all(a <= b for a, b in zip(lp["triangles"], pp["triangles"]))
+    out["target_ratio"] = 10.0
     return out
 
 
@@ -692,7 +729,7 @@ def main() -> int:
     neural = None if args.no_neural else neural_leg(args.neural_train_steps, args.executor_budget)
     chain = None if args.no_chain else chain_leg(args.chain_repeats, chain_cpu)
     training = None if args.no_training else training_leg(args.training_steps, args.training_meshes,
-                                                          args.training_pools)
+                                                          args.training_pools, chain_repeats=args.chain_repeats)
     executor = None if args.no_executor else executor_leg(args.executor_budget,
                                                           0 if args.no_cpu else args.executor_cpu_budget)
 

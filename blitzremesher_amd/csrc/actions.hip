@@ -1218,13 +1218,16 @@ std::vector<uint32_t> execute_actions(int device, const HostMesh& input, const f
     HostMesh next_mesh;
     next_mesh.positions = state.positions;
     bool next_ready = false;
+    // One scorer graph for every state (rank() waits for its sort, so the
+    // next state's build may overwrite the buffers).
+    std::unique_ptr<Graph> graph;
     const auto rank_state = [&](ActionSet* s, const HostMesh& mesh) {
-        std::unique_ptr<Graph> graph;
         if (ranking == 0) {
-            graph = scorer->graph(mesh, *bounds);
+            if (!graph) graph = std::make_unique<Graph>();
+            scorer->rebuild(*graph, mesh, *bounds);
             scorer->forward(*graph, Precision::Bf16);
         }
-        s->rank(ranking, diameter, graph.get(), nullptr);
+        s->rank(ranking, diameter, ranking == 0 ? graph.get() : nullptr, nullptr);
     };
     while (state.indices.size() / 3 > target && stats.trials < budget) {
         auto t = std::chrono::steady_clock::now();

@@ -146,9 +146,16 @@ static void size_forward(Graph& g) {
 }
 
 std::unique_ptr<Graph> Scorer::graph(const HostMesh& m, const Bounds& b) {
+    auto g = std::make_unique<Graph>();
+    rebuild(*g, m, b);
+    return g;
+}
+
+void Scorer::rebuild(Graph& graph, const HostMesh& m, const Bounds& b) {
     if (!(b.radius > 0) || !std::isfinite(b.radius)) throw std::invalid_argument("invalid bounds");
     DeviceGuard guard(device_);
-    auto g = std::make_unique<Graph>();
+    Graph* g = &graph;
+    g->has_outputs = false;
     const size_t nv = m.positions.size() / 3, nf = m.indices.size() / 3;
     const Topology t = build_topology(m.indices.data(), nf, nv);
     g->vertices = uint32_t(nv);
@@ -188,7 +195,6 @@ std::unique_ptr<Graph> Scorer::graph(const HostMesh& m, const Bounds& b) {
     size_forward(*g);
     hip_check(hipGetLastError(), "features launch");
     hip_check(hipStreamSynchronize(stream_), "graph");
-    return g;
 }
 
 std::unique_ptr<Graph> Scorer::concat(const Graph* const* parts, size_t count) {

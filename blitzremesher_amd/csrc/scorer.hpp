@@ -64,6 +64,9 @@ class Scorer {
     Scorer& operator=(const Scorer&) = delete;
     void set_weights(const float* weights, size_t count);
     std::unique_ptr<Graph> graph(const HostMesh& m, const Bounds& b);
+    // graph() into an existing graph, reusing its device buffers where they
+    // are large enough (one graph per mesh state of an executor run).
+    void rebuild(Graph& g, const HostMesh& m, const Bounds& b);
     // Disjoint union of graphs built by this scorer, in order: the vertex,
     // action, face and face-list ids of each part are offset by the counts
     // of the parts before it, and each part keeps the vertex features of its
