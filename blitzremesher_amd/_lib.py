@@ -295,6 +295,8 @@ PROTOTYPES = {
     "blitz_hip_coverage_distance_noalloc": (C.c_int, [C.c_int32, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint8,
                                                       C.c_void_p, C.c_void_p, *_ERR]),
     "blitz_hip_audit_last_timing": (C.c_int, [_P, C.POINTER(AuditTiming)]),
+    "blitz_hip_device_primitive": (C.c_int, [C.c_int32, C.c_uint32, C.c_void_p, C.c_void_p, C.c_uint32, C.c_int32,
+                                             C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, *_ERR]),
     "blitz_hip_reduce_reuse": (C.c_int, [C.POINTER(BlitzMesh), C.c_size_t, C.POINTER(C.c_uint32), C.c_size_t,
                                          C.POINTER(C.c_size_t), *_ERR]),
     "blitz_hip_scorer_weight_count": (C.c_size_t, []),

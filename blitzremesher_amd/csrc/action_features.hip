@@ -16,20 +16,23 @@
 // edge per thread); the only sorts are of 32-bit corner keys and 64-bit edge
 // keys (radix, stable), as in the action topology build.
 #include <hip/hip_runtime.h>
-#include <hipcub/device/device_radix_sort.hpp>
-#include <hipcub/device/device_run_length_encode.hpp>
-#include <hipcub/device/device_scan.hpp>
 
 #include <algorithm>
 #include <cmath>
 #include <stdexcept>
 
 #include "actions.hpp"
+#include "device_sort.hpp"
 #include "policy.hpp"
 #include "vec_device.hpp"
 
 namespace blitz_hip {
 namespace {
+
+// The 64-bit edge keys as the sort primitives take them (same width).
+static_assert(sizeof(unsigned long long) == sizeof(uint64_t));
+inline uint64_t* as_u64(unsigned long long* p) { return reinterpret_cast<uint64_t*>(p); }
+inline const uint64_t* as_u64(const unsigned long long* p) { return reinterpret_cast<const uint64_t*>(p); }
 
 constexpr uint32_t kChannels = 24;
 // Graph flags (src/neural/internal.hpp:14).
@@ -502,12 +505,12 @@ void ActionSet::graph_features_device(const VertexStreams& streams, float* devic
             temp_bytes = std::max(temp_bytes, need);
         };
         uint32_t* k32 = nullptr;
-        unsigned long long* k64 = nullptr;
-        take(hipcub::DeviceScan::ExclusiveSum(nullptr, need, k32, k32, int(n + 1), st));
-        take(hipcub::DeviceRadixSort::SortPairs(nullptr, need, k32, k32, k32, k32, int(corners), 0, 32, st));
-        take(hipcub::DeviceRadixSort::SortKeys(nullptr, need, k64, k64, int(corners), 0, 64, st));
-        take(hipcub::DeviceRunLengthEncode::Encode(nullptr, need, k64, k64, k32, k32, int(corners), st));
-        take(hipcub::DeviceRadixSort::SortKeys(nullptr, need, k64, k64, int(2 * corners), 0, 64, st));
+        uint64_t* k64 = nullptr;
+        take(exclusive_sum(nullptr, need, k32, k32, n + 1, st));
+        take(radix_sort_pairs(nullptr, need, k32, k32, k32, k32, corners, 0, 32, st));
+        take(radix_sort_pairs(nullptr, need, k64, k64, nullptr, nullptr, corners, 0, 64, st));
+        take(run_length_encode(nullptr, need, k64, k64, k32, k32, corners, st));
+        take(radix_sort_pairs(nullptr, need, k64, k64, nullptr, nullptr, 2 * corners, 0, 64, st));
     }
     const size_t n1 = size_t(n) + 1, nc = corners;
     uint32_t *group_count, *group_offsets, *cursor, *members, *canonical, *flags, *corner_key, *corner_id,
@@ -558,7 +561,7 @@ void ActionSet::graph_features_device(const VertexStreams& streams, float* devic
     hip_check(hipMemsetAsync(cursor, 0, 4 * size_t(n), st), "clear");
     group_count_kernel<<<blocks(n), 256, 0, st>>>(geometry_.as<uint32_t>(), n, group_count);
     run("group offsets", [&](size_t& bt) {
-        return hipcub::DeviceScan::ExclusiveSum(temp, bt, group_count, group_offsets, int(n + 1), st);
+        return exclusive_sum(temp, bt, group_count, group_offsets, n + 1, st);
     });
     group_fill_kernel<<<blocks(n), 256, 0, st>>>(geometry_.as<uint32_t>(), n, group_offsets, cursor, members);
     canonical_kernel<<<blocks(n), 256, 0, st>>>(a, geometry_.as<uint32_t>(), group_offsets, members, n, canonical,
@@ -567,23 +570,22 @@ void ActionSet::graph_features_device(const VertexStreams& streams, float* devic
     graph_faces_kernel<<<blocks(faces), 256, 0, st>>>(positions_.as<float4>(), indices_.as<uint32_t>(), canonical,
                                                       faces, frame, face_normal, corner_key, corner_id, edge_key);
     run("corner sort", [&](size_t& bt) {
-        return hipcub::DeviceRadixSort::SortPairs(temp, bt, corner_key, key_sorted, corner_id, corner_sorted,
-                                                  int(corners), 0, 32, st);
+        return radix_sort_pairs(temp, bt, corner_key, key_sorted, corner_id, corner_sorted, corners, 0, 32, st);
     });
     hip_check(hipMemsetAsync(vertex_count, 0, 4 * n1, st), "clear");
     count_kernel<<<blocks(corners), 256, 0, st>>>(corner_key, corners, vertex_count);
     run("corner offsets", [&](size_t& bt) {
-        return hipcub::DeviceScan::ExclusiveSum(temp, bt, vertex_count, vertex_offsets, int(n + 1), st);
+        return exclusive_sum(temp, bt, vertex_count, vertex_offsets, n + 1, st);
     });
     graph_vertices_kernel<<<blocks(n), 256, 0, st>>>(vertex_offsets, corner_sorted, face_normal,
                                                      has_materials_ ? materials_.as<uint16_t>() : nullptr, n, flags,
                                                      normal_sum);
     // ---- unique edges: flags, degrees, sorted neighbour lists.
     run("edge sort", [&](size_t& bt) {
-        return hipcub::DeviceRadixSort::SortKeys(temp, bt, edge_key, edge_sorted, int(corners), 0, 64, st);
+        return radix_sort_pairs(temp, bt, as_u64(edge_key), as_u64(edge_sorted), nullptr, nullptr, corners, 0, 64, st);
     });
     run("edge runs", [&](size_t& bt) {
-        return hipcub::DeviceRunLengthEncode::Encode(temp, bt, edge_sorted, unique, counts, runs, int(corners), st);
+        return run_length_encode(temp, bt, as_u64(edge_sorted), as_u64(unique), counts, runs, corners, st);
     });
     uint32_t nruns = 0;
     hip_check(hipMemcpyAsync(&nruns, runs, 4, hipMemcpyDeviceToHost, st), "download runs");
@@ -597,12 +599,13 @@ void ActionSet::graph_features_device(const VertexStreams& streams, float* devic
     if (nruns) {
         graph_edges_kernel<<<blocks(nruns), 256, 0, st>>>(unique, counts, nruns, flags, degree, directed);
         run("neighbour sort", [&](size_t& bt) {
-            return hipcub::DeviceRadixSort::SortKeys(temp, bt, directed, directed_sorted, int(2 * nruns), 0, 64, st);
+            return radix_sort_pairs(temp, bt, as_u64(directed), as_u64(directed_sorted), nullptr, nullptr, 2 * nruns, 0,
+                                    64, st);
         });
         low_words_kernel<<<blocks(2 * nruns), 256, 0, st>>>(directed_sorted, 2 * nruns, neighbors);
     }
     run("neighbour offsets", [&](size_t& bt) {
-        return hipcub::DeviceScan::ExclusiveSum(temp, bt, degree, neighbor_offsets, int(n + 1), st);
+        return exclusive_sum(temp, bt, degree, neighbor_offsets, n + 1, st);
     });
     // ---- the 24 channels.
     graph_features_kernel<<<blocks(n), 256, 0, st>>>(positions_.as<float4>(), a, normal_sum, flags, neighbor_offsets,

@@ -23,15 +23,13 @@
 // order, testing and claiming a footprint (both endpoints and their
 // neighbours) 64 neighbour slots at a time.
 #include "actions.hpp"
+#include "device_sort.hpp"
 #include "vec_device.hpp"
 
 #include <algorithm>
 #include <chrono>
 #include <cstring>
 #include <future>
-#include <hipcub/device/device_radix_sort.hpp>
-#include <hipcub/device/device_run_length_encode.hpp>
-#include <hipcub/device/device_scan.hpp>
 #include <stdexcept>
 
 namespace blitz_hip {
@@ -893,14 +891,14 @@ void ActionSet::build(const HostMesh& m, const float* uv) {
     uint4* face_geo = grow<uint4>(face_geo_, faces_);
     size_t bytes = 16;
     {
-        // Library sorts remain only in the weld (run when positions change).
+        // Workspace of the weld's sorts and scan (run when positions change).
         size_t need = 0;
         auto take = [&](hipError_t e) {
             hip_check(e, "workspace query");
             bytes = std::max(bytes, need);
         };
-        take(hipcub::DeviceRadixSort::SortPairs(nullptr, need, key_tmp, key_sorted, ids, ord, int(n), 0, 32, st));
-        take(hipcub::DeviceScan::InclusiveScan(nullptr, need, ord2, key_sorted, hipcub::Max(), int(n), st));
+        take(radix_sort_pairs(nullptr, need, key_tmp, key_sorted, ids, ord, n, 0, 32, st));
+        take(inclusive_max(nullptr, need, ord2, key_sorted, n, st));
     }
     void* work = grow<char>(temp, bytes);
     auto run = [&](const char* what, auto call) {
@@ -932,8 +930,7 @@ void ActionSet::build(const HostMesh& m, const float* uv) {
         auto sort_pass = [&](const uint32_t* key, const uint32_t* order_in, uint32_t* order_out) {
             gather_kernel<<<blocks_for(n), 256, 0, st>>>(key, order_in, n, key_tmp);
             run("weld sort", [&](size_t& b) {
-                return hipcub::DeviceRadixSort::SortPairs(work, b, key_tmp, key_sorted, order_in, order_out, int(n), 0,
-                                                          32, st);
+                return radix_sort_pairs(work, b, key_tmp, key_sorted, order_in, order_out, n, 0, 32, st);
             });
         };
         sort_pass(kz, ids, ord);
@@ -941,7 +938,7 @@ void ActionSet::build(const HostMesh& m, const float* uv) {
         sort_pass(kx, ord2, ord);
         weld_heads_kernel<<<blocks_for(n), 256, 0, st>>>(ord, kx, ky, kz, n, ord2);
         run("weld scan", [&](size_t& b) {
-            return hipcub::DeviceScan::InclusiveScan(work, b, ord2, key_sorted, hipcub::Max(), int(n), st);
+            return inclusive_max(work, b, ord2, key_sorted, n, st);
         });
         weld_assign_kernel<<<blocks_for(n), 256, 0, st>>>(ord, key_sorted, n, geometry);
         weld_positions_ = m.positions;
@@ -1100,11 +1097,11 @@ void ActionSet::run_rank(const void* input, uint32_t* order_out) {
     in.bad = bad;
     rank_kernel<<<blocks_for(n), 256, 0, st>>>(topo(), in);
     size_t need = 0;
-    hip_check(hipcub::DeviceRadixSort::SortPairsDescending(nullptr, need, keys, keys + n, idx, order, int(n), 0, 32, st),
+    hip_check(radix_sort_pairs_descending(nullptr, need, keys, keys + n, idx, order, n, st),
               "sort workspace");
     void* temp = grow<char>(temp_, std::max(need, temp_.bytes));
     size_t b = temp_.bytes;
-    hip_check(hipcub::DeviceRadixSort::SortPairsDescending(temp, b, keys, keys + n, idx, order, int(n), 0, 32, st),
+    hip_check(radix_sort_pairs_descending(temp, b, keys, keys + n, idx, order, n, st),
               "rank sort");
     uint32_t flag = 0;
     hip_check(hipMemcpyAsync(&flag, bad, 4, hipMemcpyDeviceToHost, st), "download rank flag");
@@ -1153,10 +1150,10 @@ std::vector<uint32_t> ActionSet::trial(const uint32_t* rows, size_t n, std::vect
     const Topo t = topo();
     trial_keep_kernel<<<blocks_for(faces_), 256, 0, st>>>(t, remap, next, keep);
     size_t need = 0;
-    hip_check(hipcub::DeviceScan::ExclusiveSum(nullptr, need, keep, slot, int(faces_), st), "scan workspace");
+    hip_check(exclusive_sum(nullptr, need, keep, slot, faces_, st), "scan workspace");
     void* temp = grow<char>(temp_, std::max(need, temp_.bytes));
     size_t b = temp_.bytes;
-    hip_check(hipcub::DeviceScan::ExclusiveSum(temp, b, keep, slot, int(faces_), st), "trial slots");
+    hip_check(exclusive_sum(temp, b, keep, slot, faces_, st), "trial slots");
     uint32_t tail[2] = {0, 0};
     hip_check(hipMemcpyAsync(&tail[0], slot + faces_ - 1, 4, hipMemcpyDeviceToHost, st), "download count");
     hip_check(hipMemcpyAsync(&tail[1], keep + faces_ - 1, 4, hipMemcpyDeviceToHost, st), "download count");

@@ -20,6 +20,7 @@
 #include "digest.hpp"
 #include "noalloc.hpp"
 #include "audit.hpp"
+#include "device_sort.hpp"
 #include "patch_audit.hpp"
 #include "patches.hpp"
 #include "scorer.hpp"
@@ -1389,4 +1390,95 @@ blitz_status blitz_hip_chain_proposal_of(const blitz_hip_chain* c, size_t i, bli
         out->gate = p.gate;
     });
 }
+}
+
+blitz_status blitz_hip_device_primitive(int32_t device, uint32_t kind, const void* keys, const uint32_t* values,
+                                        uint32_t n, int32_t begin_bit, int32_t end_bit, void* keys_out,
+                                        uint32_t* values_out, uint32_t* runs_out, char* error, size_t capacity) {
+    return guarded(error, capacity, [&] {
+        if (kind > 6) throw std::invalid_argument("unknown primitive");
+        if (n && (!keys || !keys_out)) throw std::invalid_argument("null primitive arrays");
+        if (kind == 5 && (!runs_out || (n && !values_out))) throw std::invalid_argument("run-length outputs required");
+        if (kind <= 1 && (begin_bit < 0 || end_bit < begin_bit || end_bit > (kind ? 64 : 32)))
+            throw std::invalid_argument("invalid key bits");
+        if (!blitz_hip_available(device)) throw Unavailable("no gfx950 HIP device at the requested index");
+        DeviceGuard guard(device);
+        const size_t width = (kind == 1 || kind == 5 || kind == 6) ? 8 : 4;
+        hipStream_t st{};
+        hip_check(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "hipStreamCreate");
+        std::vector<void*> owned;
+        auto alloc = [&](size_t bytes) {
+            void* p = nullptr;
+            hip_check(hipMalloc(&p, std::max<size_t>(bytes, 16)), "hipMalloc");
+            owned.push_back(p);
+            return p;
+        };
+        try {
+            void* dk = alloc(n * width);
+            void* dko = alloc(n * width);
+            uint32_t* dv = values ? static_cast<uint32_t*>(alloc(n * 4)) : nullptr;
+            uint32_t* dvo = values_out ? static_cast<uint32_t*>(alloc(n * 4)) : nullptr;
+            uint32_t* druns = static_cast<uint32_t*>(alloc(4));
+            if (n) hip_check(hipMemcpyAsync(dk, keys, n * width, hipMemcpyHostToDevice, st), "upload keys");
+            if (n && dv) hip_check(hipMemcpyAsync(dv, values, n * 4, hipMemcpyHostToDevice, st), "upload values");
+            const auto run = [&](auto call) {
+                size_t bytes = 0;
+                hip_check(call(nullptr, bytes), "workspace query");
+                void* temp = alloc(bytes);
+                hip_check(call(temp, bytes), "primitive");
+            };
+            switch (kind) {
+            case 0:
+                run([&](void* t, size_t& b) {
+                    return radix_sort_pairs(t, b, static_cast<const uint32_t*>(dk), static_cast<uint32_t*>(dko), dv, dvo,
+                                            n, begin_bit, end_bit, st);
+                });
+                break;
+            case 1:
+                run([&](void* t, size_t& b) {
+                    return radix_sort_pairs(t, b, static_cast<const uint64_t*>(dk), static_cast<uint64_t*>(dko), dv, dvo,
+                                            n, begin_bit, end_bit, st);
+                });
+                break;
+            case 2:
+                run([&](void* t, size_t& b) {
+                    return radix_sort_pairs_descending(t, b, static_cast<const float*>(dk), static_cast<float*>(dko), dv,
+                                                       dvo, n, st);
+                });
+                break;
+            case 3:
+                run([&](void* t, size_t& b) {
+                    return exclusive_sum(t, b, static_cast<const uint32_t*>(dk), static_cast<uint32_t*>(dko), n, st);
+                });
+                break;
+            case 4:
+                run([&](void* t, size_t& b) {
+                    return inclusive_max(t, b, static_cast<const uint32_t*>(dk), static_cast<uint32_t*>(dko), n, st);
+                });
+                break;
+            case 5:
+                run([&](void* t, size_t& b) {
+                    return run_length_encode(t, b, static_cast<const uint64_t*>(dk), static_cast<uint64_t*>(dko), dvo,
+                                             druns, n, st);
+                });
+                break;
+            default:
+                run([&](void* t, size_t& b) {
+                    return exclusive_sum(t, b, static_cast<const uint64_t*>(dk), static_cast<uint64_t*>(dko), n, st);
+                });
+                break;
+            }
+            if (n) hip_check(hipMemcpyAsync(keys_out, dko, n * width, hipMemcpyDeviceToHost, st), "download keys");
+            if (n && values_out) hip_check(hipMemcpyAsync(values_out, dvo, n * 4, hipMemcpyDeviceToHost, st), "download values");
+            if (runs_out) hip_check(hipMemcpyAsync(runs_out, druns, 4, hipMemcpyDeviceToHost, st), "download runs");
+            hip_check(hipStreamSynchronize(st), "primitive");
+        } catch (...) {
+            (void)hipStreamSynchronize(st);
+            for (void* p : owned) (void)hipFree(p);
+            (void)hipStreamDestroy(st);
+            throw;
+        }
+        for (void* p : owned) (void)hipFree(p);
+        (void)hipStreamDestroy(st);
+    });
 }

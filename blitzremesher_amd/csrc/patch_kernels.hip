@@ -16,9 +16,10 @@
 #include <hip/hip_runtime.h>
 
 #include <climits>
-#include <hipcub/device/device_scan.hpp>
+#include <stdexcept>
 
 #include "patch_kernels.hpp"
+#include "device_sort.hpp"
 #include "raster_device.hpp"
 
 namespace blitz_hip {
@@ -348,16 +349,19 @@ void launch_bin_fill(const RasterTask* views, uint32_t count, uint32_t max_faces
 }
 
 size_t scan_workspace(size_t n) {
+    if (n > UINT32_MAX) throw std::invalid_argument("scan exceeds 32-bit counts");
     size_t bytes = 0;
-    hip_check(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, static_cast<const unsigned long long*>(nullptr),
-                                               static_cast<unsigned long long*>(nullptr), int(n)),
+    hip_check(exclusive_sum(nullptr, bytes, static_cast<const uint64_t*>(nullptr), nullptr, uint32_t(n), nullptr),
               "scan workspace");
     return bytes;
 }
 
 void launch_exclusive_scan(void* temp, size_t temp_bytes, const unsigned long long* in, unsigned long long* out,
                            size_t n, hipStream_t stream) {
-    hip_check(hipcub::DeviceScan::ExclusiveSum(temp, temp_bytes, in, out, int(n), stream), "exclusive scan");
+    static_assert(sizeof(unsigned long long) == sizeof(uint64_t));
+    hip_check(exclusive_sum(temp, temp_bytes, reinterpret_cast<const uint64_t*>(in), reinterpret_cast<uint64_t*>(out),
+                            uint32_t(n), stream),
+              "exclusive scan");
 }
 
 void launch_patches(const PatchInput& in, hipStream_t stream) {

@@ -35,10 +35,10 @@
 //   31 constant 1.
 #include <hip/hip_runtime.h>
 
-#include <hipcub/device/device_radix_sort.hpp>
 #include <stdexcept>
 
 #include "scorer_kernels.hpp"
+#include "device_sort.hpp"
 #include "vec_device.hpp"
 
 namespace blitz_hip {
@@ -686,17 +686,13 @@ void launch_scores(const ScoreInput& in, hipStream_t stream) {
 
 size_t sort_workspace(uint32_t n) {
     size_t bytes = 0;
-    (void)hipcub::DeviceRadixSort::SortPairsDescending(nullptr, bytes, static_cast<const float*>(nullptr),
-                                                       static_cast<float*>(nullptr),
-                                                       static_cast<const uint32_t*>(nullptr),
-                                                       static_cast<uint32_t*>(nullptr), int(n));
+    (void)radix_sort_pairs_descending(nullptr, bytes, nullptr, nullptr, nullptr, nullptr, n, nullptr);
     return bytes;
 }
 
 hipError_t launch_sort(void* temp, size_t temp_bytes, const float* keys_in, float* keys_out, const uint32_t* order_in,
                        uint32_t* order_out, uint32_t n, hipStream_t stream) {
-    return hipcub::DeviceRadixSort::SortPairsDescending(temp, temp_bytes, keys_in, keys_out, order_in, order_out,
-                                                        int(n), 0, 32, stream);
+    return radix_sort_pairs_descending(temp, temp_bytes, keys_in, keys_out, order_in, order_out, n, stream);
 }
 
 void launch_teacher(const TeacherInput& in, hipStream_t stream) {

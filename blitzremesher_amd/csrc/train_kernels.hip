@@ -11,13 +11,13 @@
 // Gradient clipping by global norm and AdamW restate
 // training/update_cuda.cu:338-393 operation for operation.
 #include <hip/hip_runtime.h>
-#include <hipcub/hipcub.hpp>
 
 #include <string>
 
 #include <stdexcept>
 
 #include "train_kernels.hpp"
+#include "device_sort.hpp"
 
 namespace blitz_hip {
 namespace {
@@ -539,8 +539,7 @@ void launch_gather_edge_backward(const float* dz, const uint32_t* src, const uin
     const uint32_t entries = 2 * count;
     if (entries) {
         edge_count<<<dim3((entries + 255) / 256), dim3(256), 0, stream>>>(src, dst, count, counts);
-        hip_check_scan(hipcub::DeviceScan::ExclusiveSum(scan_temp, scan_bytes, counts, offsets, int(vertices + 1),
-                                                        stream));
+        hip_check_scan(exclusive_sum(scan_temp, scan_bytes, counts, offsets, vertices + 1, stream));
         edge_fill<<<dim3((entries + 255) / 256), dim3(256), 0, stream>>>(src, dst, count, offsets, cursor, list);
     } else {
         fill_u32<<<dim3(std::min<uint32_t>(1024, (vertices + 1 + 255) / 256)), dim3(256), 0, stream>>>(
@@ -551,8 +550,7 @@ void launch_gather_edge_backward(const float* dz, const uint32_t* src, const uin
 
 size_t gather_edge_scan_bytes(uint32_t vertices) {
     size_t bytes = 0;
-    (void)hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, static_cast<const uint32_t*>(nullptr),
-                                           static_cast<uint32_t*>(nullptr), int(vertices + 1));
+    (void)exclusive_sum(nullptr, bytes, static_cast<const uint32_t*>(nullptr), nullptr, vertices + 1, nullptr);
     return std::max<size_t>(bytes, 16);
 }
 

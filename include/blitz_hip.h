@@ -574,6 +574,19 @@ blitz_status blitz_hip_coverage_distance_noalloc(int32_t device, const uint64_t*
 
 blitz_status blitz_hip_audit_last_timing(const blitz_hip_audit*, blitz_hip_audit_timing*);
 
+/* No counterpart in the reference (which calls CUB): the library's own device
+ * sort, scan and run-length primitives (device_sort.hip) on host arrays, for
+ * tests of the primitives themselves. kind 0: stable ascending sort of
+ * uint32 keys by bits [begin_bit, end_bit), values carried (input positions
+ * when `values` is null); 1: the same for uint64 keys; 2: stable descending
+ * sort of float keys (-0 equal to +0); 3: exclusive sum of uint32; 4:
+ * inclusive max of uint32; 5: run-length encoding of uint64 keys (unique keys
+ * to keys_out, counts to values_out, run count to *runs_out); 6: exclusive
+ * sum of uint64. Outputs hold n elements. */
+blitz_status blitz_hip_device_primitive(int32_t device, uint32_t kind, const void* keys, const uint32_t* values,
+                                        uint32_t n, int32_t begin_bit, int32_t end_bit, void* keys_out,
+                                        uint32_t* values_out, uint32_t* runs_out, char* error, size_t capacity);
+
 /* ---- classical proposer --------------------------------------------------- */
 
 /* Quadric edge collapse with reused vertices: the classical proposal
