@@ -33,10 +33,13 @@ METRIC = "audited collapses/s & ms per LOD chain at ≤1px error; raster audit %
 
 
 def workload():
+    """The raster-audit configuration; the candidate is the reference's
+    classical reduction (endpoint output) of the heightfield to 75 %, made by
+    the device reducer (same triangles as the reference's reduce())."""
     from blitzremesher_amd import EvalSettings, meshes
-    from blitzremesher_amd.audit import bounds, reduce_reuse
+    from blitzremesher_amd.audit import bounds, reduce_reuse_device
     ref = meshes.heightfield(512, 0.1, 2)
-    cand = reduce_reuse(ref, int(round(ref.triangles * 0.75)))
+    cand = reduce_reuse_device(ref, int(round(ref.triangles * 0.75)))
     center, radius = bounds(ref)
     # The heightfield is an open surface seen from all around the sphere, so
     # both sides are drawn (force_two_sided); with one-sided faces the views
@@ -585,6 +588,37 @@ def chain_leg(repeats: int, cpu_reference: dict | None) -> dict:
             "cpu_reference_classical": cpu_reference}
 
 
+def reducer_leg(repeats: int) -> dict:
+    """The classical reducer (the reference's reduce(), endpoint output) of
+    the 512 x 512 heightfield to 75 % on the device, beside the reference
+    build's own reduce() on one core (same triangles, checked)."""
+    import numpy as np
+    from blitzremesher_amd import meshes
+    from blitzremesher_amd.audit import reduce_reuse_device
+    mesh = meshes.heightfield(512, 0.1, 2)
+    target = int(round(mesh.triangles * 0.75))
+    reduce_reuse_device(mesh, target)  # warm-up
+    times, report = [], {}
+    for _ in range(repeats):
+        t0 = time.perf_counter()
+        got = reduce_reuse_device(mesh, target, report=report)
+        times.append(time.perf_counter() - t0)
+    ms = 1000 * float(np.median(times))
+    out = {"workload": "reduce() with endpoint output of the 512x512 heightfield (522,242 F) to 75 %",
+           "ms": round(ms, 2), "triangles": got.triangles, "collapses": report.get("collapses"),
+           "passes": report.get("passes"), "greedy_rounds": report.get("rounds"),
+           "collapses_per_s": round(report.get("collapses", 0) / (ms * 1e-3), 1)}
+    sys.path.insert(0, str(ROOT / "tests"))
+    from oracle_lib import Oracle, available
+    if available("ref"):
+        t0 = time.perf_counter()
+        want = Oracle("ref").reduce_reuse(mesh, target)
+        cpu = time.perf_counter() - t0
+        out["cpu_reference"] = {"ms": round(1000 * cpu, 1), "cores": 1, "kind": "reference",
+                                "identical_triangles": bool(np.array_equal(want.indices, got.indices))}
+    return out
+
+
 def copy_probe(mib: int) -> int:
     """Runs the counter-calibration copy kernel (see profiles/) and prints its
     bandwidth."""
@@ -642,6 +676,7 @@ def main() -> int:
     ap.add_argument("--no-neural", action="store_true")
     ap.add_argument("--no-chain", action="store_true")
     ap.add_argument("--no-training", action="store_true")
+    ap.add_argument("--no-reducer", action="store_true")
     ap.add_argument("--training-steps", type=int, default=200)
     ap.add_argument("--training-meshes", type=int, default=64)
     ap.add_argument("--training-pools", type=int, default=96)
@@ -730,6 +765,7 @@ def main() -> int:
     chain = None if args.no_chain else chain_leg(args.chain_repeats, chain_cpu)
     training = None if args.no_training else training_leg(args.training_steps, args.training_meshes,
                                                           args.training_pools, chain_repeats=args.chain_repeats)
+    reducer = None if args.no_reducer else reducer_leg(3)
     executor = None if args.no_executor else executor_leg(args.executor_budget,
                                                           0 if args.no_cpu else args.executor_cpu_budget)
 
@@ -762,6 +798,7 @@ def main() -> int:
         "cpu_baseline": cpu,
         "parity_on_cpu_sample": parity,
         "chain": chain,
+        "reducer": reducer,
         "patches": patches,
         "executor": executor,
         "appearance": appearance,

@@ -86,6 +86,11 @@ class AuditOptions(C.Structure):
     ]
 
 
+class ReduceReportC(C.Structure):
+    _fields_ = [("struct_size", C.c_uint32), ("passes", C.c_uint32), ("rounds", C.c_uint32),
+                ("tail_collapses", C.c_uint32), ("collapses", C.c_uint64)]
+
+
 class TrainSettingsC(C.Structure):
     _fields_ = [
         ("margin", C.c_float), ("penalty", C.c_float), ("lr", C.c_float), ("decay", C.c_float),
@@ -299,6 +304,8 @@ PROTOTYPES = {
                                              C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, *_ERR]),
     "blitz_hip_reduce_reuse": (C.c_int, [C.POINTER(BlitzMesh), C.c_size_t, C.POINTER(C.c_uint32), C.c_size_t,
                                          C.POINTER(C.c_size_t), *_ERR]),
+    "blitz_hip_reduce_reuse_device": (C.c_int, [C.c_int32, C.POINTER(BlitzMesh), C.c_size_t, C.POINTER(C.c_uint32),
+                                                C.c_size_t, C.POINTER(C.c_size_t), C.POINTER(ReduceReportC), *_ERR]),
     "blitz_hip_scorer_weight_count": (C.c_size_t, []),
     "blitz_hip_scorer_create": (C.c_int, [C.c_int32, C.POINTER(C.c_float), C.c_size_t, C.POINTER(_P), *_ERR]),
     "blitz_hip_scorer_destroy": (None, [_P]),

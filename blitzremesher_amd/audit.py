@@ -167,6 +167,25 @@ def reduce_reuse(mesh: Mesh, target_triangles: int) -> Mesh:
     return Mesh(mesh.positions, out[: count.value].reshape(-1, 3).copy(), None, mesh.double_sided)
 
 
+def reduce_reuse_device(mesh: Mesh, target_triangles: int, device: int = 0, report: dict | None = None) -> Mesh:
+    """blitz::reduce with endpoint (Reuse) output on the device
+    (src/simplify.cpp:43-329): the reference's triangles, in its order.
+    `report`, when given, receives passes, greedy rounds and collapses."""
+    lib = _lib.load()
+    out = np.zeros(max(1, mesh.indices.size), np.uint32)
+    count = C.c_size_t()
+    err = _lib.error_buffer()
+    desc = mesh.descriptor()
+    rep = _lib.ReduceReportC()
+    rep.struct_size = C.sizeof(rep)
+    _lib.check(lib.blitz_hip_reduce_reuse_device(device, C.byref(desc), int(target_triangles),
+                                                 out.ctypes.data_as(C.POINTER(C.c_uint32)), out.size,
+                                                 C.byref(count), C.byref(rep), err, len(err)), err)
+    if report is not None:
+        report.update({name: getattr(rep, name) for name, _ in rep._fields_ if name != "struct_size"})
+    return Mesh(mesh.positions, out[: count.value].reshape(-1, 3).copy(), None, mesh.double_sided)
+
+
 def _bounds_c(center, radius) -> _lib.Bounds:
     b = _lib.Bounds()
     c = np.asarray(center, dtype=np.float32)

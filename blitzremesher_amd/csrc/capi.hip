@@ -21,6 +21,7 @@
 #include "noalloc.hpp"
 #include "audit.hpp"
 #include "device_sort.hpp"
+#include "reduce.hpp"
 #include "patch_audit.hpp"
 #include "patches.hpp"
 #include "scorer.hpp"
@@ -1480,5 +1481,27 @@ blitz_status blitz_hip_device_primitive(int32_t device, uint32_t kind, const voi
         }
         for (void* p : owned) (void)hipFree(p);
         (void)hipStreamDestroy(st);
+    });
+}
+
+blitz_status blitz_hip_reduce_reuse_device(int32_t device, const blitz_mesh* m, size_t target, uint32_t* out,
+                                           size_t out_capacity, size_t* count, blitz_hip_reduce_report* report,
+                                           char* error, size_t capacity) {
+    return guarded(error, capacity, [&] {
+        if (!out || !count) throw std::invalid_argument("null reduction output");
+        if (report && report->struct_size != sizeof(*report)) throw std::invalid_argument("report size mismatch");
+        const HostMesh h = host_mesh(m);
+        if (!blitz_hip_available(device)) throw Unavailable("no gfx950 HIP device at the requested index");
+        ReduceReport r{};
+        const auto idx = reduce_reuse_device(device, h, target, &r);
+        if (idx.size() > out_capacity) throw std::invalid_argument("index capacity too small");
+        std::memcpy(out, idx.data(), idx.size() * 4);
+        *count = idx.size();
+        if (report) {
+            report->passes = r.passes;
+            report->rounds = r.rounds;
+            report->tail_collapses = r.tail_collapses;
+            report->collapses = r.collapses;
+        }
     });
 }

@@ -597,6 +597,22 @@ blitz_status blitz_hip_reduce_reuse(const blitz_mesh*, size_t target_triangles, 
                                     size_t index_capacity, size_t* index_count, char* error,
                                     size_t capacity);
 
+/* blitz::reduce with OutputMode::Reuse and the default quadric objective
+ * (src/simplify.cpp:43-329, reduce :337-367) on the device: the pass
+ * structure, candidate costs, (cost, u, v) order, link and fold tests and
+ * budget of the reference, every pass's walk as a greedy independent set in
+ * rank order. Same triangles, in the same order, as the reference. Reads
+ * positions, indices, materials and normals. `report` (may be null) receives
+ * passes, greedy rounds and collapses. */
+typedef struct blitz_hip_reduce_report {
+    uint32_t struct_size;
+    uint32_t passes, rounds, tail_collapses;
+    uint64_t collapses;
+} blitz_hip_reduce_report;
+blitz_status blitz_hip_reduce_reuse_device(int32_t device, const blitz_mesh*, size_t target_triangles,
+                                           uint32_t* indices_out, size_t index_capacity, size_t* index_count,
+                                           blitz_hip_reduce_report* report, char* error, size_t capacity);
+
 /* ---- edge-graph proposal scorer ------------------------------------------ */
 
 /* The HIP scorer ranks every directed contraction (u -> v) of a mesh: 32

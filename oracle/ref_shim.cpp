@@ -223,6 +223,26 @@ int64_t ref_reduce_reuse(const float* pos, uint64_t nv, const uint32_t* idx, uin
     }
 }
 
+// The same reduction with per-face materials and per-vertex normals (null
+// when absent): material junctions lock, normals add the bend cost.
+int64_t ref_reduce_reuse_streams(const float* pos, uint64_t nv, const uint32_t* idx, uint64_t ni,
+                                 const uint16_t* mat, uint64_t nm, const float* normals, uint64_t target,
+                                 uint32_t* out, uint64_t capacity) {
+    try {
+        ReduceSettings rs;
+        rs.output = OutputMode::Reuse;
+        rs.target_triangles = target;
+        MeshView v = view_of(pos, nv, idx, ni, mat, nm, nullptr, 0);
+        if (normals) v.normals = Stream<Vec3>(reinterpret_cast<const Vec3*>(normals), nv);
+        auto lod = reduce(v, rs);
+        if (!lod.shared_vertices || lod.data.indices.size() > capacity) return -1;
+        std::memcpy(out, lod.data.indices.data(), lod.data.indices.size() * 4);
+        return int64_t(lod.data.indices.size());
+    } catch (...) {
+        return -1;
+    }
+}
+
 // Classical CPU generator (src/chain.cpp generate) on the coverage profile.
 // Writes per-level triangle counts and the adjacent/source audit maxima.
 struct ChainSettings {

@@ -255,6 +255,23 @@ class Oracle:
         assert n >= 0
         return Mesh(mesh.positions, out[:n].reshape(-1, 3).copy(), None, mesh.double_sided)
 
+    def reduce_reuse_streams(self, mesh: Mesh, target: int) -> np.ndarray:
+        """reduce_reuse with the mesh's materials and normals; the indices."""
+        assert self.kind == "ref"
+        out = np.zeros(max(1, mesh.indices.size), np.uint32)
+        f = self.lib.ref_reduce_reuse_streams
+        f.restype = C.c_int64
+        f.argtypes = [C.POINTER(C.c_float), C.c_uint64, C.POINTER(C.c_uint32), C.c_uint64, C.POINTER(C.c_uint16),
+                      C.c_uint64, C.POINTER(C.c_float), C.c_uint64, C.POINTER(C.c_uint32), C.c_uint64]
+        mat = mesh.materials
+        nrm = None if mesh.normals is None else np.ascontiguousarray(mesh.normals, np.float32)
+        n = f(_ptr(mesh.positions, C.c_float), mesh.positions.shape[0], _ptr(mesh.indices, C.c_uint32),
+              mesh.indices.size, _ptr(mat, C.c_uint16) if mat is not None else None,
+              0 if mat is None else mat.size, _ptr(nrm, C.c_float) if nrm is not None else None, target,
+              out.ctypes.data_as(C.POINTER(C.c_uint32)), out.size)
+        assert n >= 0
+        return out[:n].reshape(-1, 3).copy()
+
     def generate(self, mesh: Mesh, cs: ChainSettings):
         assert self.kind == "ref"
         tri = np.zeros(64, np.uint32)
