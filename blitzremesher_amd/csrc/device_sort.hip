@@ -171,6 +171,42 @@ __global__ __launch_bounds__(kThreads) void scan_apply_kernel(const T* in, T* ou
     }
 }
 
+// Short scans (every radix pass of a few tens of thousands of keys scans
+// 256 counts per tile): one workgroup walks the array in chunks of 4,096 and
+// carries the running total, one launch instead of three.
+constexpr uint32_t kSmallThreads = 1024;
+constexpr uint32_t kSmallItems = 4;
+constexpr uint32_t kSmallScan = 16 * kSmallThreads * kSmallItems;
+
+template <class T, class Op, bool Inclusive>
+__global__ __launch_bounds__(kSmallThreads) void scan_small_kernel(const T* in, T* out, uint32_t n, Op op) {
+    __shared__ T wave_total[kSmallThreads / 64];
+    __shared__ T carry;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    for (uint32_t base = 0; base < n; base += kSmallThreads * kSmallItems) {
+        const uint32_t first = base + threadIdx.x * kSmallItems;
+        T v[kSmallItems];
+        T sum = 0;
+        for (uint32_t k = 0; k < kSmallItems; ++k) {
+            v[k] = first + k < n ? in[first + k] : T(0);
+            sum = op(sum, v[k]);
+        }
+        const Scanned<T> s = block_scan(sum, op, wave_total);
+        const T c = carry;
+        T run = op(c, s.exclusive);
+        for (uint32_t k = 0; k < kSmallItems; ++k) {
+            if (first + k >= n) break;
+            const T next = op(run, v[k]);
+            out[first + k] = Inclusive ? next : run;
+            run = next;
+        }
+        __syncthreads();
+        if (threadIdx.x == kSmallThreads - 1) carry = op(c, s.inclusive);
+        __syncthreads();
+    }
+}
+
 template <class T, class Op, bool Inclusive>
 hipError_t scan(void* temp, size_t& temp_bytes, const T* in, T* out, uint32_t n, hipStream_t stream, Op op) {
     const uint32_t blocks = std::max(1u, (n + kTile - 1) / kTile);
@@ -181,6 +217,10 @@ hipError_t scan(void* temp, size_t& temp_bytes, const T* in, T* out, uint32_t n,
     }
     if (temp_bytes < need) return hipErrorInvalidValue;
     if (!n) return hipSuccess;
+    if (n <= kSmallScan) {
+        scan_small_kernel<T, Op, Inclusive><<<1, kSmallThreads, 0, stream>>>(in, out, n, op);
+        return hipGetLastError();
+    }
     T* totals = static_cast<T*>(temp);
     scan_totals_kernel<T, Op><<<blocks, kThreads, 0, stream>>>(in, n, totals, op);
     scan_carry_kernel<T, Op><<<1, kThreads, 0, stream>>>(totals, blocks, op);
