@@ -97,6 +97,45 @@ def raster_fp64_ops(meshes_, center, radius, settings) -> int:
     return total
 
 
+_cpu_worker = {}
+
+
+def _cpu_worker_init(ref, cand, center, radius, sample):
+    _cpu_worker.update(ref=ref, cand=cand, center=center, radius=radius, sample=sample)
+
+
+def _cpu_worker_audit(_=None) -> float:
+    sys.path.insert(0, str(ROOT / "tests"))
+    from oracle_lib import Oracle
+    w = _cpu_worker
+    t0 = time.perf_counter()
+    Oracle("ref").evaluate(w["ref"], w["cand"], w["center"], w["radius"], w["sample"])
+    return time.perf_counter() - t0
+
+
+def cpu_baseline_all_cores(ref, cand, center, radius, settings, sample_views: int) -> dict | None:
+    """The reference CPU audit on every core at once: one worker process per
+    core, each auditing the same `sample_views`-view sample; audits/s of the
+    64-view workload = cores x sample / 64 / wall."""
+    sys.path.insert(0, str(ROOT / "tests"))
+    from oracle_lib import available
+    from blitzremesher_amd import EvalSettings
+    if not available("ref"):
+        return None
+    import multiprocessing as mp
+    cores = max(1, min(16, len(os.sched_getaffinity(0))))
+    sample = EvalSettings(**{**settings.__dict__, "orthographic": sample_views})
+    with mp.get_context("spawn").Pool(cores, initializer=_cpu_worker_init,
+                                      initargs=(ref, cand, center, radius, sample)) as pool:
+        pool.map(_cpu_worker_audit, range(cores))  # start-up and first touch
+        t0 = time.perf_counter()
+        pool.map(_cpu_worker_audit, range(cores))
+        wall = time.perf_counter() - t0
+    return {"value": cores * sample_views / settings.orthographic / wall, "unit": "audits/s", "cores": cores,
+            "kind": "reference", "nproc": os.cpu_count(), "cpu_model": cpu_model(),
+            "sample": f"{cores} processes, each a {sample_views}-view sample of the same audit, in {wall:.2f} s"}
+
+
 def cpu_baseline(ref, cand, center, radius, settings, sample_views: int):
     """Times the reference CPU audit (oracle/_ref build) or, without it, the
     oracle restatement, on `sample_views` views of the same raster size, one
@@ -757,6 +796,9 @@ def main() -> int:
             cpu, sample, cpu_m = cpu_baseline(ref, cand, center, radius, settings, args.cpu_sample_views)
             gpu_sample = audit.evaluate(dref, dcand, center, radius, sample)
             parity = gpu_sample.as_tuple() == cpu_m.as_tuple()
+            cpu_all = cpu_baseline_all_cores(ref, cand, center, radius, settings, 8)
+            if cpu_all:
+                cpu["all_cores"] = cpu_all
     scorer = None if args.no_scorer else scorer_leg(max(3, min(args.steps, 10)))
     patches = None if args.no_patches else patch_leg(ref, center, radius, settings, args.patches,
                                                        max(3, min(args.steps, 10)))
