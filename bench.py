@@ -511,6 +511,7 @@ def learned_chain(scorer, repeats: int) -> dict:
                      "four_gate_audited_collapses": i["legal_collapses"],
                      "four_gate_audited_collapses_per_s": round(i["legal_collapses"] / (ms * 1e-3), 1),
                      "rejected_collapses": i["rejected_collapses"], "trials": i["action_trials"],
+                     "stage_ms": {k: round(i[f"{k}_ms"], 1) for k in ("executor", "gate", "rank", "speculative")},
                      "levels_passed": all(l.source.passed and l.adjacent.passed for l in r.lods[1:]),
                      "identical_runs": all([l.triangles for l in x.lods] == [l.triangles for l in r.lods]
                                            for x in runs)}
@@ -736,9 +737,16 @@ def main() -> int:
 
     from blitzremesher_amd import Audit, _lib
     import numpy as np
+    started = time.perf_counter()
+
+    def progress(step: str) -> None:
+        print(f"bench: {step} done at {time.perf_counter() - started:.1f} s", file=sys.stderr, flush=True)
+
     # CPU-only baselines run first, before this process touches the GPU.
     chain_cpu = None if (args.no_cpu or args.no_chain) else chain_cpu_baseline()
+    progress("CPU chain baseline")
     ref, cand, center, radius, settings = workload()
+    progress("workload")
     # torch is not imported: its wheel bundles a second HIP runtime, and two
     # HIP runtimes in one process cannot share the device. The library's own
     # hipDeviceSynchronize brackets the timed region instead.
@@ -799,17 +807,28 @@ def main() -> int:
             cpu_all = cpu_baseline_all_cores(ref, cand, center, radius, settings, 8)
             if cpu_all:
                 cpu["all_cores"] = cpu_all
-    scorer = None if args.no_scorer else scorer_leg(max(3, min(args.steps, 10)))
-    patches = None if args.no_patches else patch_leg(ref, center, radius, settings, args.patches,
-                                                       max(3, min(args.steps, 10)))
-    appearance = None if args.no_appearance else appearance_leg(max(3, min(args.steps, 10)))
-    neural = None if args.no_neural else neural_leg(args.neural_train_steps, args.executor_budget)
-    chain = None if args.no_chain else chain_leg(args.chain_repeats, chain_cpu)
-    training = None if args.no_training else training_leg(args.training_steps, args.training_meshes,
-                                                          args.training_pools, chain_repeats=args.chain_repeats)
-    reducer = None if args.no_reducer else reducer_leg(3)
-    executor = None if args.no_executor else executor_leg(args.executor_budget,
-                                                          0 if args.no_cpu else args.executor_cpu_budget)
+    progress("raster audit")
+
+    def leg(skip: bool, name: str, run):
+        if skip:
+            return None
+        result = run()
+        progress(name)
+        return result
+
+    reps = max(3, min(args.steps, 10))
+    scorer = leg(args.no_scorer, "scorer", lambda: scorer_leg(reps))
+    patches = leg(args.no_patches, "patches",
+                  lambda: patch_leg(ref, center, radius, settings, args.patches, reps))
+    appearance = leg(args.no_appearance, "appearance", lambda: appearance_leg(reps))
+    neural = leg(args.no_neural, "neural", lambda: neural_leg(args.neural_train_steps, args.executor_budget))
+    chain = leg(args.no_chain, "chain", lambda: chain_leg(args.chain_repeats, chain_cpu))
+    training = leg(args.no_training, "training",
+                   lambda: training_leg(args.training_steps, args.training_meshes, args.training_pools,
+                                        chain_repeats=args.chain_repeats))
+    reducer = leg(args.no_reducer, "reducer", lambda: reducer_leg(3))
+    executor = leg(args.no_executor, "executor",
+                   lambda: executor_leg(args.executor_budget, 0 if args.no_cpu else args.executor_cpu_budget))
 
     line = {
         "metric": METRIC,

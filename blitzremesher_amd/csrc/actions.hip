@@ -1216,12 +1216,14 @@ std::vector<uint32_t> execute_actions(int device, const HostMesh& input, const f
     next_mesh.positions = state.positions;
     bool next_ready = false;
     // One scorer graph for every state (rank() waits for its sort, so the
-    // next state's build may overwrite the buffers).
+    // next state's build may overwrite the buffers); the states share the
+    // input's vertex stream, uploaded once.
     std::unique_ptr<Graph> graph;
     const auto rank_state = [&](ActionSet* s, const HostMesh& mesh) {
         if (ranking == 0) {
+            const bool same_positions = graph != nullptr;
             if (!graph) graph = std::make_unique<Graph>();
-            scorer->rebuild(*graph, mesh, *bounds);
+            scorer->rebuild(*graph, mesh, *bounds, same_positions);
             scorer->forward(*graph, Precision::Bf16);
         }
         s->rank(ranking, diameter, ranking == 0 ? graph.get() : nullptr, nullptr);

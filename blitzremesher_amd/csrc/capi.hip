@@ -1056,9 +1056,15 @@ blitz_status blitz_hip_graph_create(blitz_hip_scorer* s, const blitz_mesh* m, co
         auto g = std::make_unique<blitz_hip_graph>();
         g->owner = s;
         g->graph = scorer.graph(h, bounds_from(b));
-        const Topology t = build_topology(h.indices.data(), h.indices.size() / 3, h.positions.size() / 3);
-        g->src = t.src;
-        g->dst = t.neighbors;
+        // The directed actions as the device topology holds them.
+        const Graph& d = *g->graph;
+        g->src.resize(d.actions);
+        g->dst.resize(d.actions);
+        if (d.actions) {
+            hip_check(hipMemcpy(g->src.data(), d.src.p, size_t(d.actions) * 4, hipMemcpyDeviceToHost), "download actions");
+            hip_check(hipMemcpy(g->dst.data(), d.neighbors.p, size_t(d.actions) * 4, hipMemcpyDeviceToHost),
+                      "download actions");
+        }
         *out = g.release();
     });
 }

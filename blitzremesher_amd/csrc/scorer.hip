@@ -5,6 +5,7 @@
 #include <cstring>
 #include <stdexcept>
 
+#include "graph_topology.hpp"
 #include "scorer_kernels.hpp"
 
 namespace blitz_hip {
@@ -151,29 +152,24 @@ std::unique_ptr<Graph> Scorer::graph(const HostMesh& m, const Bounds& b) {
     return g;
 }
 
-void Scorer::rebuild(Graph& graph, const HostMesh& m, const Bounds& b) {
+void Scorer::rebuild(Graph& graph, const HostMesh& m, const Bounds& b, bool same_positions) {
     if (!(b.radius > 0) || !std::isfinite(b.radius)) throw std::invalid_argument("invalid bounds");
     DeviceGuard guard(device_);
     Graph* g = &graph;
     g->has_outputs = false;
     const size_t nv = m.positions.size() / 3, nf = m.indices.size() / 3;
-    const Topology t = build_topology(m.indices.data(), nf, nv);
+    if (nv > UINT32_MAX - 1 || nf * 3 > UINT32_MAX) throw std::invalid_argument("graph too large");
+    same_positions = same_positions && g->vertices == nv && g->positions.p;
     g->vertices = uint32_t(nv);
     g->faces = uint32_t(nf);
-    g->actions = uint32_t(t.neighbors.size());
-    g->corners = uint32_t(t.faces.size());
     g->diameter = b.radius * 2;
-    std::vector<float> pos(nv * 4, 0.f);
-    for (size_t i = 0; i < nv; ++i) std::memcpy(&pos[4 * i], &m.positions[3 * i], 12);
-    upload(g->positions, pos, stream_);
+    if (!same_positions) {
+        std::vector<float> pos(nv * 4, 0.f);
+        for (size_t i = 0; i < nv; ++i) std::memcpy(&pos[4 * i], &m.positions[3 * i], 12);
+        upload(g->positions, pos, stream_);
+    }
     upload(g->indices, m.indices, stream_);
-    upload(g->offsets, t.offsets, stream_);
-    upload(g->neighbors, t.neighbors, stream_);
-    upload(g->src, t.src, stream_);
-    upload(g->reverse, t.reverse, stream_);
-    upload(g->face_offsets, t.face_offsets, stream_);
-    upload(g->faces_of, t.faces, stream_);
-    upload(g->boundary, t.boundary, stream_);
+    build_graph_topology(*g, stream_);
     g->features.resize(nv * kFeatures * 4);
     FeatureInput in{};
     in.positions = g->positions.as<float4>();

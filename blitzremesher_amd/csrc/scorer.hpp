@@ -1,4 +1,4 @@
-// Host controller of the edge-graph proposal scorer: mesh topology (host),
+// Host controller of the edge-graph proposal scorer: mesh topology,
 // vertex features, FP32/BF16 forward passes, ranking and placement decode
 // (HIP). Counterpart of the reference's ActionCuda / GpuActionState ranking
 // (src/neural/action.hpp:75, src/neural/action_gpu.cu:1751-1798).
@@ -31,6 +31,8 @@ struct Graph {
     DeviceBuffer positions, indices, offsets, neighbors, src, reverse, face_offsets, faces_of, boundary;
     DeviceBuffer features, features_bf16, outputs, h_a, h_b, z, hq_a, hq_b, zq, pq;
     DeviceBuffer keys, order, keys_out, order_out, sort_temp;
+    // Scratch of the device topology build.
+    DeviceBuffer build_keys, build_sorted, build_unique, build_counts, build_values, build_temp;
     // Training: activations of every layer (FP32 and BF16), layer inputs,
     // batch buffers and gradients.
     DeviceBuffer t_h[4], t_hq[4], t_z[3];
@@ -65,8 +67,10 @@ class Scorer {
     void set_weights(const float* weights, size_t count);
     std::unique_ptr<Graph> graph(const HostMesh& m, const Bounds& b);
     // graph() into an existing graph, reusing its device buffers where they
-    // are large enough (one graph per mesh state of an executor run).
-    void rebuild(Graph& g, const HostMesh& m, const Bounds& b);
+    // are large enough (one graph per mesh state of an executor run). With
+    // `same_positions` the graph keeps the positions of its previous build
+    // (the caller guarantees the vertex stream is unchanged).
+    void rebuild(Graph& g, const HostMesh& m, const Bounds& b, bool same_positions = false);
     // Disjoint union of graphs built by this scorer, in order: the vertex,
     // action, face and face-list ids of each part are offset by the counts
     // of the parts before it, and each part keeps the vertex features of its

@@ -233,6 +233,58 @@ def test_gpu_features_and_actions_match_oracle(scorer, case):
     assert np.array_equal(g.features(), want)
 
 
+def fan(k: int) -> Mesh:
+    """An open disc: centre vertex 0 of valence k joined to a rim of k
+    vertices (every rim edge a boundary edge)."""
+    t = 2 * np.pi * np.arange(k) / k
+    p = np.zeros((k + 1, 3), np.float32)
+    p[1:, 0], p[1:, 1], p[1:, 2] = np.cos(t), np.sin(t), 0.05 * np.sin(3 * t)
+    f = [(0, 1 + i, 2 + i) for i in range(k - 1)]
+    return Mesh(p, np.array(f, np.uint32))
+
+
+def ragged() -> Mesh:
+    """A heightfield with unused vertices in its vertex stream (before,
+    between and after the used ones), faces with a repeated corner and a
+    face of one vertex repeated."""
+    base = meshes.heightfield(10, 0.1, 4)
+    nv = len(base.positions)
+    rng = np.random.default_rng(9)
+    extra = rng.normal(0, 1, (7, 3)).astype(np.float32)
+    positions = np.concatenate([extra[:2], base.positions[: nv // 2], extra[2:4], base.positions[nv // 2:], extra[4:]])
+    remap = np.concatenate([np.arange(nv // 2) + 2, np.arange(nv // 2, nv) + 4]).astype(np.uint32)
+    faces = remap[base.indices]
+    degenerate = np.array([[faces[3, 0], faces[3, 0], faces[3, 1]], [faces[7, 2], faces[9, 1], faces[9, 1]],
+                           [faces[11, 0], faces[11, 0], faces[11, 0]]], np.uint32)
+    return Mesh(positions, np.concatenate([faces[:20], degenerate, faces[20:]]).astype(np.uint32))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", ["fan", "ragged", "one_face", "empty"])
+def test_gpu_device_topology_matches_oracle(scorer, case):
+    """The device-built topology (edge sort, run-length counts, directed
+    sort, row starts) gives the oracle's actions on meshes with a vertex of
+    valence 300, an open boundary, vertices no face uses (before, between
+    and after used ones), degenerate faces, one face and no faces."""
+    if case == "fan":
+        mesh = fan(300)
+    elif case == "ragged":
+        mesh = ragged()
+    elif case == "one_face":
+        mesh = Mesh(np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [5, 5, 5]], np.float32), np.array([[0, 1, 2]], np.uint32))
+    else:
+        mesh = Mesh(np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], np.float32), np.zeros((0, 3), np.uint32))
+    g, c, r, offsets, neighbors, acts = graph_case(scorer, mesh)
+    assert np.array_equal(g.actions_array(), acts.astype(np.uint32).reshape(-1, 2))
+    if case == "fan":
+        assert np.diff(offsets)[0] == 300
+        want = so.features(mesh.positions, mesh.indices, c, 2 * r).astype(np.float32)
+        assert np.array_equal(g.features(), want)
+    # A second build into the same graph (buffers reused) gives the same.
+    g2 = scorer.graph(mesh, c, r)
+    assert np.array_equal(g2.actions_array(), g.actions_array())
+
+
 @pytest.mark.gpu
 @pytest.mark.parametrize("case", sorted(CASES))
 def test_gpu_fp32_forward_is_bit_exact(scorer, case):
