@@ -150,7 +150,7 @@ __device__ FaceRec face_record(const AttrTask& T, uint32_t f, global_ptr<double>
     Setup s;
     FaceRec r;
     r.face = f;
-    if (setup_triangle(t, c, s) == 1) {
+    if (setup_triangle(t, c, s, double(t.size)) == 1) {
         const double area = edge(c.a.x, c.a.y, c.b.x, c.b.y, c.c.x, c.c.y);
         const bool back = area < 0;
         r.ax = s.ax;

@@ -50,6 +50,12 @@ __global__ __launch_bounds__(256) void project_kernel(const ProjectTask* __restr
 // empty, fully covered and undecided (classify_rect), then produces each
 // undecided tile's word with a ballot.
 
+// Faces at most this many samples wide are cut into 4-column quads, wider
+// ones (queued, up to kWideQueue per workgroup) into 64-sample spans (see
+// raster_band_kernel).
+constexpr int kQuadFaceWidth = 12;
+constexpr uint32_t kWideQueue = 4096;
+
 // Workgroups a raster launch should reach (four per CU of the 256-CU part).
 constexpr uint32_t kFillBlocks = 1024;
 
@@ -62,6 +68,8 @@ constexpr uint32_t kFillBlocks = 1024;
 __global__ __launch_bounds__(kRasterThreads) void raster_band_kernel(const RasterTask* __restrict__ tasks,
                                                                      RasterGrid grid) {
     extern __shared__ unsigned long long band[];
+    __shared__ uint32_t wide_queue[kWideQueue];
+    __shared__ uint32_t wide_count;
     uint32_t task_id, band_id, chunk;
     if (grid.chunks % 8 == 0) {
         const uint32_t xcd = blockIdx.x % 8, slot = blockIdx.x / 8, per_band = grid.chunks / 8;
@@ -85,6 +93,7 @@ __global__ __launch_bounds__(kRasterThreads) void raster_band_kernel(const Raste
     const uint32_t ty1 = min(t.tiles, ty0 + grid.band_rows);
     const uint32_t words = (ty1 - ty0) * t.tiles;
     for (uint32_t i = threadIdx.x; i < words; i += blockDim.x) band[i] = 0;
+    if (threadIdx.x == 0) wide_count = 0;
     __syncthreads();
     const int ymin = int(ty0 * kTile), ymax = min(int(ty1 * kTile), int(t.size)) - 1;
     const uint32_t faces = task_faces(t);
@@ -111,7 +120,13 @@ __global__ __launch_bounds__(kRasterThreads) void raster_band_kernel(const Raste
         Setup s;
         int kind = 0;
         if (f < f1) {
-            kind = setup_triangle(t, corners, s);
+            // The frame side is converted in the loop: hoisted out of it,
+            // the converted pair stays live across the whole loop and is
+            // spilled to scratch. The empty asm (t is the block's task, so
+            // its size is uniform) keeps the conversion in place.
+            uint32_t side = t.size;
+            asm volatile("" : "+s"(side));
+            kind = setup_triangle(t, corners, s, double(side));
             if (corners.behind) report_behind(t);
             if (kind == 2) report_out_of_frame(t, f);
         }
@@ -120,7 +135,21 @@ __global__ __launch_bounds__(kRasterThreads) void raster_band_kernel(const Raste
         // Boxes at most 4 samples wide and 8 high: the lane path, an 8x8
         // local bitmap over at most 2x2 tiles, then four words.
         const bool lane_face = drawn && w <= 4 && h <= 8;
-        const bool wide = drawn && !lane_face, small_box = wide && w <= 8 && h <= 8;
+        // Faces wider than kQuadFaceWidth samples wait in the block's queue
+        // and are drawn after the face loop; with the queue full they take
+        // the quad items below.
+        const bool big = drawn && w > kQuadFaceWidth;
+        const uint64_t bigs = __ballot(big);
+        bool queued = false;
+        if (bigs) {
+            uint32_t first = 0;
+            if (lane == 0) first = atomicAdd(&wide_count, uint32_t(__popcll(bigs)));
+            first = __shfl(first, 0);
+            const uint32_t slot = first + uint32_t(__popcll(bigs & ((1ull << lane) - 1)));
+            queued = big && slot < kWideQueue;
+            if (queued) wide_queue[slot] = f;
+        }
+        const bool wide = drawn && !lane_face && !queued, small_box = wide && w <= 8 && h <= 8;
         if (__ballot(lane_face)) {
             const int ya = max(s.y0, ymin), yb = min(s.y1, ymax);
             const int rows = lane_face ? yb - ya + 1 : 0;
@@ -217,6 +246,77 @@ __global__ __launch_bounds__(kRasterThreads) void raster_band_kernel(const Raste
                 if (word)
                     atomicOr(&band[uint32_t(ty - int(ty0)) * t.tiles + uint32_t(qx >> 1)],
                              static_cast<unsigned long long>(word));
+            }
+        }
+    }
+    __syncthreads();
+    // Queued faces, wider than kQuadFaceWidth samples: set up again and cut
+    // into items of one tile row by one 64-sample span (eight tiles), dealt
+    // to the lanes of each wave as above; an undecided span's rows are
+    // found as intervals (span_half_words), so the work of a row does not
+    // grow with the face's width. Drawn here, after the face loop, their
+    // registers do not add to the loop's. Queue entries go to the waves in
+    // turn (entry i0 + lane * waves + wave), so a short queue still spreads
+    // over every wave of the workgroup.
+    const uint32_t queued_faces = min(wide_count, kWideQueue);
+    const uint32_t waves = blockDim.x / 64, wave = threadIdx.x / 64;
+    for (uint32_t i0 = 0; i0 < queued_faces; i0 += blockDim.x) {
+        const uint32_t i = i0 + uint32_t(lane) * waves + wave;
+        Setup s;
+        bool mine = false;
+        if (i < queued_faces) {
+            const uint32_t f = wide_queue[i];
+            mine = setup_triangle(t, load_corners(t, load_face(t, f), f), s, double(t.size)) == 1;
+        }
+        uint32_t items = 0, across = 0;
+        int span0 = 0, row0 = 0, ya_face = 0, yb_face = 0;
+        if (mine) {
+            ya_face = max(s.y0, ymin);
+            yb_face = min(s.y1, ymax);
+            span0 = s.x0 >> 6;
+            row0 = ya_face >> 3;
+            across = uint32_t((s.x1 >> 6) - span0 + 1);
+            items = across * uint32_t((yb_face >> 3) - row0 + 1);
+        }
+        uint32_t inclusive = items;
+        for (int d = 1; d < 64; d <<= 1) {
+            const uint32_t v = __shfl_up(inclusive, d);
+            if (lane >= d) inclusive += v;
+        }
+        const uint32_t total = __shfl(inclusive, 63);
+        for (uint32_t base = 0; base < total; base += 64) {
+            const uint32_t k = base + uint32_t(lane);
+            const bool live = k < total;
+            const uint32_t key = live ? k : total - 1;
+            int lo = 0;
+            for (int half = 32; half > 0; half >>= 1)
+                if (__shfl(inclusive, lo + half - 1) <= key) lo += half;
+            const int owner = lo;
+            const uint32_t o_end = __shfl(inclusive, owner), o_items = __shfl(items, owner);
+            const uint32_t o_across = __shfl(across, owner);
+            const int o_span = __shfl(span0, owner), o_row = __shfl(row0, owner);
+            const int o_ya = __shfl(ya_face, owner), o_yb = __shfl(yb_face, owner);
+            const Setup q = shuffle_setup(s, owner);
+            const uint32_t j = key - (o_end - o_items);
+            const int sx = o_span + int(j % o_across), ty = o_row + int(j / o_across);
+            const int xa = max(64 * sx, q.x0), xb = min(64 * sx + 63, q.x1);
+            const int ya = max(8 * ty, o_ya), yb = min(8 * ty + 7, o_yb);
+            // Drawn faces have finite corners inside the frame
+            // (box_in_frame), so classify_rect decides or leaves undecided
+            // on the samples alone.
+            const int kind = live ? classify_rect(q, xa, xb, ya, yb) : 0;
+            if (kind == 0) continue;
+            unsigned long long* row = &band[uint32_t(ty - int(ty0)) * t.tiles + uint32_t(8 * sx)];
+            // Rows 0-3 and 4-7 of the tile row in turn; tiles past the
+            // span's last sample get zero words.
+            for (int half = 0; half < 2; ++half) {
+                const int ra = max(ya, 8 * ty + 4 * half), rb = min(yb, 8 * ty + 4 * half + 3);
+                if (ra > rb) continue;
+                uint32_t word[8];
+                span_half_words(q, 64 * sx, xa, xb, ra, rb, kind == 1, word);
+#pragma unroll
+                for (int c = 0; c < 8; ++c)
+                    if (word[c]) atomicOr(&row[c], static_cast<unsigned long long>(word[c]) << (32 * half));
             }
         }
     }

@@ -38,7 +38,7 @@ __device__ __forceinline__ unsigned long long wave_total(unsigned long long v) {
 __device__ __forceinline__ bool face_tiles(const RasterTask& t, uint32_t f, int& tx0, int& ty0, int& tx1, int& ty1) {
     const Corners k = load_corners(t, load_face(t, f), f);
     Setup s;
-    if (setup_triangle(t, k, s) != 1) return false;
+    if (setup_triangle(t, k, s, double(t.size)) != 1) return false;
     tx0 = s.x0 >> 3;
     ty0 = s.y0 >> 3;
     tx1 = s.x1 >> 3;
@@ -125,7 +125,7 @@ __global__ __launch_bounds__(256) void patch_kernel(PatchInput in) {
         const uint32_t f = add ? 0u : removed[i];
         const Corners k = load_corners(t, add ? added_face(i - nr) : load_face(vt, f), f);
         Setup s;
-        const int kind = setup_triangle(t, k, s);
+        const int kind = setup_triangle(t, k, s, double(t.size));
         if (kind == 2 || k.behind) atomicOr(&flags, kFallback);
         if (kind == 1) {
             atomicMin(&box[0], s.x0 >> 3);
@@ -228,7 +228,7 @@ __global__ __launch_bounds__(256) void patch_kernel(PatchInput in) {
         if (e < na) {
             const Corners k = load_corners(at, added_face(e), 0);
             Setup s;
-            if (setup_triangle(at, k, s) == 1) draw(s);
+            if (setup_triangle(at, k, s, double(at.size)) == 1) draw(s);
             continue;
         }
         const uint32_t j = e - na;
@@ -245,7 +245,7 @@ __global__ __launch_bounds__(256) void patch_kernel(PatchInput in) {
         if (contains(removed, nr, f)) continue;
         const Corners k = load_corners(vt, load_face(vt, f), f);
         Setup s;
-        if (setup_triangle(vt, k, s) != 1) continue;
+        if (setup_triangle(vt, k, s, double(vt.size)) != 1) continue;
         // A face binned to several dirty tiles is drawn from the first one.
         const int fx = max(s.x0 >> 3, bx0), fy = max(s.y0 >> 3, by0);
         if (uint32_t(fy) * tiles + uint32_t(fx) != tile) continue;

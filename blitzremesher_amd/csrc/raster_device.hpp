@@ -125,7 +125,8 @@ __device__ __forceinline__ bool box_in_frame(double2 a, double2 d, double2 e, do
 // Triangle setup of src/evaluate.cpp:144-165 on projected vertices.
 // Returns 0 = nothing to draw, 1 = draw, 2 = out of frame (the reference stops
 // at the first such face).
-__device__ __forceinline__ int setup_triangle(const RasterTask& t, const Corners& k, Setup& s) {
+// `side` is double(t.size), the frame's side in samples.
+__device__ __forceinline__ int setup_triangle(const RasterTask& t, const Corners& k, Setup& s, double side) {
     if (k.behind) return 0;
     const double2 a = k.a;
     double2 d = k.b, e = k.c;
@@ -139,7 +140,7 @@ __device__ __forceinline__ int setup_triangle(const RasterTask& t, const Corners
         e = tmp;
     }
     double minx, maxx, miny, maxy;
-    if (!box_in_frame(a, d, e, area, double(t.size), minx, maxx, miny, maxy)) return 2;
+    if (!box_in_frame(a, d, e, area, side, minx, maxx, miny, maxy)) return 2;
     s.x0 = max(0, int(floor(minx)));
     s.x1 = min(int(t.size) - 1, int(floor(maxx)));
     s.y0 = max(0, int(floor(miny)));
@@ -400,6 +401,123 @@ __device__ __forceinline__ uint64_t quad_word(const Setup& s, int q, int xa, int
         py += 1.0;
     }
     return word;
+}
+
+// One edge of a face along one sample row: the value at sample x is
+// fl(t - fl(b * fl(x + .5 - p))) with t = fl(a * fl(py - q)) fixed for the
+// row, and the sample passes the edge when !(value < nr), nr = -r -- the
+// arithmetic of covers() and quad_word().
+struct RowEdge {
+    double t, b, p, nr;
+};
+
+__device__ __forceinline__ bool row_pass(const RowEdge& e, int x) {
+    const double px = double(x) + .5;
+    return !((e.t - e.b * (px - e.p)) < e.nr);
+}
+
+// Narrows [l, r] to its samples that pass edge e. fl(px - p), fl(b * d) and
+// fl(t - m) are each monotone in their varying operand, so along a row the
+// value never rises (b > 0) or never falls (b < 0) as x grows: the passing
+// samples are a prefix (b > 0) or a suffix (b < 0) of [l, r], and all or
+// none of it when b == 0. The end is guessed from the real crossing
+// px = p + (t - nr) / b and then settled by testing samples, so the result
+// is exact whatever the guess; a guess off by k samples costs k tests.
+__device__ __forceinline__ void clip_row(const RowEdge& e, int& l, int& r) {
+    if (l > r) return;
+    if (!(e.b != 0)) {
+        if (!row_pass(e, l)) r = l - 1;
+        return;
+    }
+    // An approximate reciprocal: the guess only seeds the search.
+    const double guess = e.p + (e.t - e.nr) * double(__builtin_amdgcn_rcpf(float(e.b))) - .5;
+    if (e.b > 0) {
+        // Last passing sample, l - 1 when none passes.
+        int x = int(floor(fmin(fmax(guess, double(l - 1)), double(r))));
+        if (x >= l && !row_pass(e, x)) {
+            do --x;
+            while (x >= l && !row_pass(e, x));
+        } else {
+            while (x < r && row_pass(e, x + 1)) ++x;
+        }
+        r = x;
+    } else {
+        // First passing sample, r + 1 when none passes.
+        int x = int(ceil(fmin(fmax(guess, double(l)), double(r + 1))));
+        if (x <= r && !row_pass(e, x)) {
+            do ++x;
+            while (x <= r && !row_pass(e, x));
+        } else {
+            while (x > l && row_pass(e, x - 1)) --x;
+        }
+        l = x;
+    }
+}
+
+// In-place transpose of a 4x4 byte matrix held as four words: afterwards
+// byte k of word j is what byte j of word k was (2-byte blocks swapped
+// between words k and k+2, then bytes between words k and k+1).
+__device__ __forceinline__ void transpose4(uint32_t& a0, uint32_t& a1, uint32_t& a2, uint32_t& a3) {
+    uint32_t t = ((a0 >> 16) ^ a2) & 0x0000FFFFu;
+    a2 ^= t;
+    a0 ^= t << 16;
+    t = ((a1 >> 16) ^ a3) & 0x0000FFFFu;
+    a3 ^= t;
+    a1 ^= t << 16;
+    t = ((a0 >> 8) ^ a1) & 0x00FF00FFu;
+    a1 ^= t;
+    a0 ^= t << 8;
+    t = ((a2 >> 8) ^ a3) & 0x00FF00FFu;
+    a3 ^= t;
+    a2 ^= t << 8;
+}
+
+// One half (four sample rows) of the eight tile words of one tile row
+// under the samples [base, base + 63] (base a multiple of 64): the samples
+// [xa, xb] x [ya, yb] of that span the face covers, the rows within one
+// half of the tile row. w[c] gets bits 8 * (y & 3) + (x & 7) of tile
+// base / 8 + c: the low half of its word for rows 0-3 of the tile, the high
+// half for rows 4-7. With `full` (the rectangle classified as wholly
+// covered) every row is [xa, xb]; otherwise each row's covered samples are
+// the interval left by clip_row() over the three edges. Row r becomes a
+// 64-bit column mask; its low four bytes are tiles 0-3 and its high four
+// tiles 4-7, and two 4x4 byte transposes turn the rows into tile words.
+__device__ __forceinline__ void span_half_words(const Setup& s, int base, int xa, int xb, int ya, int yb, bool full,
+                                                uint32_t w[8]) {
+    RowEdge eu{0, s.cy - s.by, s.bx, -s.r0};
+    RowEdge ev{0, s.ay - s.cy, s.cx, -s.r1};
+    RowEdge ew{0, s.by - s.ay, s.ax, -s.r2};
+    const double au = s.cx - s.bx, av = s.ax - s.cx, aw = s.bx - s.ax;
+    uint64_t m0 = 0, m1 = 0, m2 = 0, m3 = 0;
+    double py = double(ya) + .5;
+    for (int y = ya; y <= yb; ++y) {
+        int l = xa, r = xb;
+        if (!full) {
+            eu.t = au * (py - s.by);
+            ev.t = av * (py - s.cy);
+            ew.t = aw * (py - s.ay);
+            clip_row(eu, l, r);
+            clip_row(ev, l, r);
+            clip_row(ew, l, r);
+        }
+        const uint64_t mask = l <= r ? (~0ull >> (63 - (r - base))) & (~0ull << (l - base)) : 0;
+        const int row = y & 3;
+        m0 = row == 0 ? mask : m0;
+        m1 = row == 1 ? mask : m1;
+        m2 = row == 2 ? mask : m2;
+        m3 = row == 3 ? mask : m3;
+        py += 1.0;
+    }
+    w[0] = uint32_t(m0);
+    w[1] = uint32_t(m1);
+    w[2] = uint32_t(m2);
+    w[3] = uint32_t(m3);
+    w[4] = uint32_t(m0 >> 32);
+    w[5] = uint32_t(m1 >> 32);
+    w[6] = uint32_t(m2 >> 32);
+    w[7] = uint32_t(m3 >> 32);
+    transpose4(w[0], w[1], w[2], w[3]);
+    transpose4(w[4], w[5], w[6], w[7]);
 }
 
 // Moves an 8x8 bitmap anchored at sample (x0, y0) into the up-to-four 8x8

@@ -23,8 +23,11 @@ PRELUDE = """#include <algorithm>
 #include <random>
 #include <cstdlib>
 using std::fabs; using std::isfinite; using std::isnan; using std::fmax; using std::fmin; using std::max; using std::min;
+using std::floor; using std::ceil;
 #define __device__
 #define __forceinline__ inline
+// The device's approximate reciprocal only seeds a search; any value works.
+#define __builtin_amdgcn_rcpf(x) (1.0f / (x))
 struct double2 {
     double x, y;
 };
@@ -48,7 +51,7 @@ static bool covers_ref(const Setup& s, int x, int y) {
 int main(int argc, char** argv) {
     const int iterations = atoi(argv[1]);
     std::mt19937_64 rng(11);
-    long rects = 0, full = 0, empty = 0, bad = 0;
+    long rects = 0, full = 0, empty = 0, bad = 0, spans_mixed = 0;
     for (int it = 0; it < iterations; ++it) {
         const int mode = it % 4;
         const double span = it % 3 == 0 ? 24 : 80;
@@ -220,7 +223,73 @@ int main(int argc, char** argv) {
                        std::floor(miny) != std::floor(rminy) || std::floor(maxy) != std::floor(rmaxy);
         }
     }
-    printf("%ld %ld %ld %ld\n", rects, full, empty, bad);
+    // span_words over 64-sample spans of faces up to 600 samples across
+    // (and thin slivers), full and interval forms, against the per-sample
+    // predicate; rows clipped as a band clips them.
+    {
+        for (int it = 0; it < iterations / 4; ++it) {
+            const double span = it % 4 == 0 ? 600 : it % 4 == 1 ? 140 : it % 4 == 2 ? 40 : 12;
+            const double o = 8 + (it % 7) * 13.25;
+            auto coord = [&]() {
+                double v = o + std::uniform_real_distribution<double>(0, span)(rng);
+                if (it % 5 == 1) v = std::round(v * 2) / 2;
+                if (it % 5 == 2) v = std::round(v) + .5;
+                return v;
+            };
+            Setup s;
+            s.ax = coord(); s.ay = coord(); s.bx = coord(); s.by = coord(); s.cx = coord(); s.cy = coord();
+            if (it % 6 == 3) s.by = s.ay;  // a horizontal edge (b == 0)
+            if (it % 9 == 4) s.cx = s.bx + 1e-9, s.cy = s.by + 300;  // a nearly vertical sliver
+            s.r0 = .5 * (fabs(s.cx - s.bx) + fabs(s.cy - s.by));
+            s.r1 = .5 * (fabs(s.ax - s.cx) + fabs(s.ay - s.cy));
+            s.r2 = .5 * (fabs(s.bx - s.ax) + fabs(s.by - s.ay));
+            if (it % 7 == 0) s.r0 = s.r1 = s.r2 = 0;
+            s.x0 = int(std::floor(std::min({s.ax, s.bx, s.cx})));
+            s.x1 = int(std::floor(std::max({s.ax, s.bx, s.cx})));
+            s.y0 = int(std::floor(std::min({s.ay, s.by, s.cy})));
+            s.y1 = int(std::floor(std::max({s.ay, s.by, s.cy})));
+            for (int sx = s.x0 >> 6; sx <= s.x1 >> 6; ++sx)
+                for (int ty = s.y0 >> 3; ty <= s.y1 >> 3; ++ty) {
+                    const int xa = std::max(64 * sx, s.x0), xb = std::min(64 * sx + 63, s.x1);
+                    int ya = std::max(ty * 8, s.y0), yb = std::min(ty * 8 + 7, s.y1);
+                    if (it % 3 == 0 && ya < yb) ++ya;  // band-clipped rows
+                    if (it % 4 == 0 && ya < yb) --yb;
+                    uint64_t want[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+                    int in = 0, n = 0;
+                    for (int y = ya; y <= yb; ++y)
+                        for (int x = xa; x <= xb; ++x) {
+                            ++n;
+                            if (covers_ref(s, x, y)) {
+                                ++in;
+                                want[(x - 64 * sx) >> 3] |= 1ull << (8 * (y & 7) + (x & 7));
+                            }
+                        }
+                    const int kind = classify_rect(s, xa, xb, ya, yb);
+                    // The two halves of the tile row, as the raster takes them.
+                    auto words = [&](bool whole, uint64_t got[8]) {
+                        for (int k = 0; k < 8; ++k) got[k] = 0;
+                        for (int half = 0; half < 2; ++half) {
+                            const int ra = std::max(ya, 8 * ty + 4 * half), rb = std::min(yb, 8 * ty + 4 * half + 3);
+                            if (ra > rb) continue;
+                            uint32_t w[8];
+                            span_half_words(s, 64 * sx, xa, xb, ra, rb, whole, w);
+                            for (int k = 0; k < 8; ++k) got[k] |= uint64_t(w[k]) << (32 * half);
+                        }
+                    };
+                    uint64_t got[8];
+                    words(false, got);
+                    for (int k = 0; k < 8; ++k) bad += got[k] != want[k];
+                    if (kind == 1) {
+                        words(true, got);
+                        for (int k = 0; k < 8; ++k) bad += got[k] != want[k];
+                        ++full;
+                    }
+                    ++rects;
+                    if (in && in < n) ++spans_mixed;
+                }
+        }
+    }
+    printf("%ld %ld %ld %ld %ld\n", rects, full, empty, bad, spans_mixed);
     return 0;
 }
 """
@@ -237,7 +306,12 @@ def _function(text: str, signature: str) -> str:
     raise AssertionError(signature)
 
 
-def _run(tmp_path: Path, source: str, iterations: int) -> tuple[int, int, int, int]:
+def _struct(text: str, opening: str) -> str:
+    i = text.index(opening)
+    return text[i:text.index("};", i) + 2]
+
+
+def _run(tmp_path: Path, source: str, iterations: int) -> tuple[int, ...]:
     src = tmp_path / "check.cpp"
     exe = tmp_path / "check"
     src.write_text(source)
@@ -257,7 +331,9 @@ def _source(plant: tuple[str, str] | None = None) -> str:
              _function(text, "bool box_in_frame("), _function(text, "bool covers("),
              _function(text, "int classify_rect("), _function(text, "uint64_t rect_word("),
              _function(text, "uint64_t lane_bitmap("), _function(text, "uint64_t lane_bitmap4("),
-             _function(text, "uint64_t quad_word("), _function(text, "void scatter_bitmap(")]
+             _function(text, "uint64_t quad_word("), _function(text, "void scatter_bitmap("),
+             _struct(text, "struct RowEdge {"), _function(text, "bool row_pass("), _function(text, "void clip_row("),
+             _function(text, "void transpose4("), _function(text, "void span_half_words(")]
     return PRELUDE + "\n\n".join(parts) + CHECK
 
 
@@ -266,10 +342,11 @@ needs_gxx = pytest.mark.skipif(shutil.which("g++") is None, reason="g++ not avai
 
 @needs_gxx
 def test_device_raster_math_agrees_with_every_sample(tmp_path):
-    rects, full, empty, bad = _run(tmp_path, _source(), 40000)
+    rects, full, empty, bad, spans_mixed = _run(tmp_path, _source(), 40000)
     assert bad == 0
-    # Every class occurs, so each branch of the classifier is exercised.
-    assert full > 1000 and empty > 1000 and rects - full - empty > 1000
+    # Every class occurs, so each branch of the classifier is exercised,
+    # and many spans are partly covered (the row intervals decide them).
+    assert full > 1000 and empty > 1000 and rects - full - empty > 1000 and spans_mixed > 1000
 
 
 @needs_gxx
@@ -299,7 +376,25 @@ def test_device_raster_math_agrees_with_every_sample(tmp_path):
     # The frame test's upper bound taken as inclusive.
     ("return !(minx < 0 || miny < 0 || maxx > size || maxy > size);",
      "return !(minx < 0 || miny < 0 || maxx >= size || maxy >= size);"),
+    # A row's last passing sample overshot by one (the walk tests x, not x + 1).
+    ("while (x < r && row_pass(e, x + 1)) ++x;", "while (x < r && row_pass(e, x)) ++x;"),
+    # A suffix edge's first sample left one short.
+    ("while (x > l && row_pass(e, x - 1)) --x;", "while (x > l && row_pass(e, x)) --x;"),
+    # Prefix and suffix edges exchanged.
+    ("    if (e.b > 0) {\n        // Last passing", "    if (e.b < 0) {\n        // Last passing"),
+    # A constant edge (b == 0) taken as passing everywhere.
+    ("        if (!row_pass(e, l)) r = l - 1;\n", ""),
+    # A row's mask one column too wide.
+    ("(~0ull >> (63 - (r - base)))", "(~0ull >> (62 - (r - base)))"),
+    # The byte round of the transpose moving 2-byte blocks.
+    ("    t = ((a0 >> 8) ^ a1) & 0x00FF00FFu;", "    t = ((a0 >> 8) ^ a1) & 0x0000FFFFu;"),
+    # Tiles 4-7 taken from the low half of the row masks.
+    ("    w[4] = uint32_t(m0 >> 32);", "    w[4] = uint32_t(m0);"),
+    # A half's rows placed by their tile row instead of their half.
+    ("        const int row = y & 3;", "        const int row = y & 7;"),
+    # Row terms not stepped between rows of a span.
+    ("        py += 1.0;\n    }\n    w[0] = uint32_t(m0);", "    }\n    w[0] = uint32_t(m0);"),
 ])
 def test_check_catches_planted_mistakes(tmp_path, plant):
-    _, _, _, bad = _run(tmp_path, _source(plant), 4000)
+    _, _, _, bad, _ = _run(tmp_path, _source(plant), 4000)
     assert bad > 0
