@@ -410,7 +410,8 @@ def neural_leg(train_steps: int, budget: int) -> dict:
     return out
 
 
-def training_leg(steps: int, count: int, pools_per_mesh: int, holdout: int = 8, chain_repeats: int = 2) -> dict:
+def training_leg(steps: int, count: int, pools_per_mesh: int, holdout: int = 8, chain_repeats: int = 2,
+                 batched_meshes: int = 8, batched_workers: int = 4) -> dict:
     """Scorer training from method-generated labels: `count` seeded meshes of
     uneven size (2k-100k triangles, log-uniform; spheres, tori, extruded
     boxes, heightfields, seed 5); up to `pools_per_mesh` teacher pools of 16
@@ -483,6 +484,89 @@ def training_leg(steps: int, count: int, pools_per_mesh: int, holdout: int = 8, 
                                "held_out_after": round(top1_preferred(data, after, held_pools), 4)}}
         data.graph.release()
         out["chain_learned_vs_plain"] = learned_chain(scorer, chain_repeats)
+        if batched_meshes:
+            out["batched_chains"] = batched_chains(scorer.weights(), batched_meshes, batched_workers)
+    return out
+
+
+def c5_chain_settings():
+    """The batched-generation chain: 6 levels from 512 px to 16 px,
+    transition and LOD0 bounds 1 px, coverage profile, 32 orthographic search
+    and audit views, ss 4 search / 8 audit / 16 max, candidate budget 8, beam
+    2, max changed area 1."""
+    from blitzremesher_amd.chain import ChainSettings
+    return ChainSettings(levels=6, profile=0, base_pixels=512, last_pixels=16, max_lod0_delta_px=1.0,
+                         transition=((0, 1), (1, 1)), search_orthographic=32, search_perspective=0,
+                         search_seed=0xB1172024, audit_orthographic=32, audit_perspective=0, audit_seed=0xB1172026,
+                         search_supersample=4, audit_supersample=8, max_supersample=16, candidate_budget=8,
+                         beam_width=2, max_changed_area=1.0)
+
+
+def batched_chains(weights, count: int, workers: int) -> dict:
+    """Batched end-to-end generation on a bounded sample of the 128-mesh set
+    (1k-250k triangles, log-uniform, seed 11; spheres, tori, extruded boxes,
+    heightfields): `count` meshes, every 8th by size from the smallest, each
+    a 6-level chain (c5_chain_settings) with `workers` chains in flight (one
+    audit object, and for learned ranking one scorer with the trained
+    weights, per worker thread). Learned (BF16 logits) and the plain
+    current-plane order run the same meshes; reported per ranking: wall time,
+    four-gate audited collapses (and per second), trials and the emitted
+    triangles summed over the meshes per level."""
+    import threading
+    from concurrent.futures import ThreadPoolExecutor
+    import numpy as np
+    from blitzremesher_amd import Audit, meshes
+    from blitzremesher_amd.chain import ChainOptions, generate
+    from blitzremesher_amd.scorer import Scorer
+    everything = meshes.training_set(128, 1000, 250000, seed=11)
+    order = np.argsort([m.triangles for _, m in everything], kind="stable")
+    pick = [int(i) for i in order[0:8 * count:8]]
+    settings = c5_chain_settings()
+    sizes = [everything[i][1].triangles for i in pick]
+    out = {"workload": f"{len(pick)} of the 128 procedural meshes (every 8th by size from the smallest: "
+                       f"{min(sizes)}-{max(sizes)} triangles; set of 1k-250k, seed 11), 6-level chains "
+                       "(512 -> 16 px, 1 px bounds, 32 views, ss 4/8/16, budget 8, beam 2; 64 trials x batch 32 "
+                       f"per proposal), {workers} chains in flight",
+           "meshes": [{"family": everything[i][0], "triangles": everything[i][1].triangles} for i in pick]}
+    # The raster tuner's first launches of each shape, outside the timing.
+    with Audit(0, cache_rasters=True) as audit:
+        generate(audit, everything[pick[0]][1], settings,
+                 ChainOptions(action_trials=64, action_batch=32, ranking=4, overdraw_tiebreak=True))
+    for name, ranking in (("learned", 0), ("plain", 4)):
+        local = threading.local()
+        owned = []
+        lock = threading.Lock()
+
+        def run(i):
+            if not hasattr(local, "audit"):
+                local.audit = Audit(0, cache_rasters=True)
+                local.scorer = Scorer(weights) if ranking == 0 else None
+                with lock:
+                    owned.append((local.audit, local.scorer))
+            options = ChainOptions(action_trials=64, action_batch=32, ranking=ranking, overdraw_tiebreak=True)
+            return generate(local.audit, everything[i][1], settings, options, scorer=local.scorer)
+
+        t0 = time.perf_counter()
+        with ThreadPoolExecutor(workers) as pool:
+            results = list(pool.map(run, pick))
+        wall = time.perf_counter() - t0
+        for audit, scorer in owned:
+            audit.close()
+            if scorer is not None:
+                scorer.close()
+        collapses = sum(r.info["legal_collapses"] for r in results)
+        out[name] = {"wall_s": round(wall, 2), "four_gate_audited_collapses": collapses,
+                     "four_gate_audited_collapses_per_s": round(collapses / wall, 1),
+                     "trials": sum(r.info["action_trials"] for r in results),
+                     "rejected_collapses": sum(r.info["rejected_collapses"] for r in results),
+                     "triangles_per_level": [sum(r.lods[k].triangles for r in results) for k in range(6)],
+                     "levels_passed": all(l.source.passed and l.adjacent.passed for r in results for l in r.lods[1:]),
+                     "status": sorted({int(r.status) for r in results})}
+    lp, pp = out["learned"], out["plain"]
+    out["ratio_collapses_per_s"] = round(lp["four_gate_audited_collapses_per_s"] /
+                                         max(pp["four_gate_audited_collapses_per_s"], 1e-9), 3)
+    out["learned_triangles_at_most_plain"] = all(a <= b for a, b in zip(lp["triangles_per_level"],
+                                                                        pp["triangles_per_level"]))
     return out
 
 
@@ -721,6 +805,8 @@ def main() -> int:
     ap.add_argument("--training-meshes", type=int, default=64)
     ap.add_argument("--training-pools", type=int, default=96)
     ap.add_argument("--chain-repeats", type=int, default=2)
+    ap.add_argument("--batched-meshes", type=int, default=8)
+    ap.add_argument("--batched-workers", type=int, default=4)
     ap.add_argument("--neural-train-steps", type=int, default=300)
     ap.add_argument("--executor-budget", type=int, default=400)
     ap.add_argument("--executor-cpu-budget", type=int, default=3)
@@ -825,7 +911,8 @@ def main() -> int:
     chain = leg(args.no_chain, "chain", lambda: chain_leg(args.chain_repeats, chain_cpu))
     training = leg(args.no_training, "training",
                    lambda: training_leg(args.training_steps, args.training_meshes, args.training_pools,
-                                        chain_repeats=args.chain_repeats))
+                                        chain_repeats=args.chain_repeats, batched_meshes=args.batched_meshes,
+                                        batched_workers=args.batched_workers))
     reducer = leg(args.no_reducer, "reducer", lambda: reducer_leg(3))
     executor = leg(args.no_executor, "executor",
                    lambda: executor_leg(args.executor_budget, 0 if args.no_cpu else args.executor_cpu_budget))

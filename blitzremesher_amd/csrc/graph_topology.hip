@@ -3,8 +3,7 @@
 // face edges dropped), the source vertex and the reverse entry of every
 // directed action, per-vertex counts of boundary edges (edges with exactly
 // one face corner pair) and the vertex -> face CSR in ascending face order
-// (a face listed once per corner of the vertex). The arrays equal those of
-// build_topology() (scorer.hip) element for element; the mesh states of an
+// (a face listed once per corner of the vertex). The mesh states of an
 // executor run are ranked from them without a host pass over the faces.
 //
 // Packed keys: with b = bits of (vertices - 1), an edge (min, max) is

@@ -6,9 +6,11 @@
 namespace blitz_hip {
 
 struct Graph;
-// From g.indices (g.faces x 3, resident) and g.vertices: offsets,
-// neighbors, src, reverse, face_offsets, faces_of and boundary, equal to
-// build_topology()'s arrays, plus g.actions and g.corners. Waits for
+// From g.indices (g.faces x 3, resident) and g.vertices: the neighbour CSR
+// (offsets, neighbors; rows ascending, degenerate face edges dropped), the
+// source (src) and reverse entry of every directed action, the vertex ->
+// face CSR in ascending face order (face_offsets, faces_of) and per-vertex
+// boundary-edge counts (boundary), plus g.actions and g.corners. Waits for
 // `stream` once (the edge count sizes the action arrays).
 void build_graph_topology(Graph& g, hipStream_t stream);
 

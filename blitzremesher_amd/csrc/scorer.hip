@@ -26,68 +26,6 @@ GraphView Graph::view() const {
             actions};
 }
 
-// Undirected neighbour CSR with ascending rows (degenerate face edges
-// dropped), the directed action of every entry and its reverse, the
-// vertex -> face-corner CSR in ascending face order and per-vertex counts of
-// boundary edges (edges used by exactly one face corner pair).
-Topology build_topology(const uint32_t* idx, size_t nf, size_t nv) {
-    Topology t;
-    std::vector<uint64_t> edges;
-    edges.reserve(nf * 3);
-    for (size_t f = 0; f < nf; ++f)
-        for (int j = 0; j < 3; ++j) {
-            const uint32_t a = idx[3 * f + j], b = idx[3 * f + (j + 1) % 3];
-            if (a == b) continue;
-            edges.push_back(a < b ? (uint64_t(a) << 32) | b : (uint64_t(b) << 32) | a);
-        }
-    std::sort(edges.begin(), edges.end());
-    t.boundary.assign(nv, 0);
-    std::vector<uint64_t> unique;
-    unique.reserve(edges.size());
-    for (size_t i = 0; i < edges.size();) {
-        size_t j = i + 1;
-        while (j < edges.size() && edges[j] == edges[i]) ++j;
-        unique.push_back(edges[i]);
-        if (j - i == 1) {
-            ++t.boundary[edges[i] >> 32];
-            ++t.boundary[uint32_t(edges[i])];
-        }
-        i = j;
-    }
-    t.offsets.assign(nv + 1, 0);
-    for (uint64_t e : unique) {
-        ++t.offsets[(e >> 32) + 1];
-        ++t.offsets[uint32_t(e) + 1];
-    }
-    for (size_t v = 0; v < nv; ++v) t.offsets[v + 1] += t.offsets[v];
-    t.neighbors.resize(t.offsets[nv]);
-    std::vector<uint32_t> cursor(t.offsets.begin(), t.offsets.end() - 1);
-    for (uint64_t e : unique) {
-        const uint32_t a = uint32_t(e >> 32), b = uint32_t(e);
-        t.neighbors[cursor[a]++] = b;
-        t.neighbors[cursor[b]++] = a;
-    }
-    t.src.resize(t.neighbors.size());
-    t.reverse.resize(t.neighbors.size());
-    for (size_t v = 0; v < nv; ++v) {
-        std::sort(t.neighbors.begin() + t.offsets[v], t.neighbors.begin() + t.offsets[v + 1]);
-        for (uint32_t k = t.offsets[v]; k < t.offsets[v + 1]; ++k) t.src[k] = uint32_t(v);
-    }
-    for (size_t k = 0; k < t.neighbors.size(); ++k) {
-        const uint32_t w = t.neighbors[k], v = t.src[k];
-        const auto begin = t.neighbors.begin() + t.offsets[w], end = t.neighbors.begin() + t.offsets[w + 1];
-        t.reverse[k] = uint32_t(std::lower_bound(begin, end, v) - t.neighbors.begin());
-    }
-    t.face_offsets.assign(nv + 1, 0);
-    for (size_t i = 0; i < nf * 3; ++i) ++t.face_offsets[idx[i] + 1];
-    for (size_t v = 0; v < nv; ++v) t.face_offsets[v + 1] += t.face_offsets[v];
-    t.faces.resize(nf * 3);
-    std::vector<uint32_t> fc(t.face_offsets.begin(), t.face_offsets.end() - 1);
-    for (size_t f = 0; f < nf; ++f)
-        for (int j = 0; j < 3; ++j) t.faces[fc[idx[3 * f + j]]++] = uint32_t(f);
-    return t;
-}
-
 namespace {
 template <class T> void upload(DeviceBuffer& d, const std::vector<T>& v, hipStream_t s) {
     d.resize(v.size() * sizeof(T));

@@ -15,13 +15,6 @@
 
 namespace blitz_hip {
 
-// Host topology of one mesh (see scorer_types.hpp GraphView).
-struct Topology {
-    std::vector<uint32_t> offsets, neighbors, src, reverse;
-    std::vector<uint32_t> face_offsets, faces;  // vertex -> incident face corners, ascending face
-    std::vector<uint32_t> boundary;             // edges with one incident face, per vertex
-};
-Topology build_topology(const uint32_t* indices, size_t faces, size_t vertices);
 
 
 struct Graph {

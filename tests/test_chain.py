@@ -146,6 +146,37 @@ def test_gpu_chain_matches_reference_golden(name, cache):
 
 
 @pytest.mark.gpu
+def test_gpu_chains_in_flight_match_reference_golden():
+    """Several chains at once, one audit object per thread (as the batched
+    generation runs them): each still emits the reference's chain."""
+    import threading
+    from concurrent.futures import ThreadPoolExecutor
+    from blitzremesher_amd import Audit
+    from blitzremesher_amd.chain import generate
+    names = sorted(chain_cases()) * 2
+    local = threading.local()
+    owned = []
+    lock = threading.Lock()
+
+    def run(name):
+        if not hasattr(local, "audit"):
+            local.audit = Audit(0, cache_rasters=True)
+            with lock:
+                owned.append(local.audit)
+        _, settings, options = chain_cases()[name]
+        return generate(local.audit, golden_mesh(name), settings, options)
+
+    try:
+        with ThreadPoolExecutor(4) as pool:
+            results = list(pool.map(run, names))
+    finally:
+        for audit in owned:
+            audit.close()
+    for name, r in zip(names, results):
+        check_chain(r, golden()[name], name)
+
+
+@pytest.mark.gpu
 def test_gpu_chain_overdraw_tiebreak_keeps_the_counts():
     # Ties are broken only among finalists of equal per-level triangle counts
     # and storage, each confirmed by full audits first.
