@@ -149,15 +149,66 @@ __device__ void sort_list(uint32_t* p, uint32_t n) {
 // whose length is the edge's face count (action.cpp:77-92: one face marks
 // both ends boundary, more than two marks them invalid; each end sets its
 // own flag here). degree = distinct neighbours, up = those above u.
+// Per-vertex lists longer than kShortList entries are sorted by
+// long_sort_kernel, one wave per vertex in LDS (up to kLongList entries):
+// a thread walking a high-valence vertex's list (the pole of a
+// latitude-longitude sphere has one neighbour per longitude) through
+// global memory would hold up the whole build.
+constexpr uint32_t kShortList = 32;
+constexpr uint32_t kLongList = 1024;
+
+// Stable rank sort of the per-vertex lists with more than kShortList
+// entries, one wave per vertex (four per workgroup): the wave copies the
+// list into LDS and lane i writes entry i to the number of entries smaller
+// than it or equal and earlier. Adjacency lists first, then face lists.
+// Longer lists than kLongList are heap-sorted in place by the wave's first
+// lane.
+__global__ __launch_bounds__(256) void long_sort_kernel(uint32_t n, const uint32_t* __restrict__ face_offsets,
+                                                        uint32_t* faces, const uint32_t* __restrict__ adj_offsets,
+                                                        uint32_t* adj) {
+    __shared__ uint32_t lds[4][kLongList];
+    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const uint32_t u = blockIdx.x * 4 + wave;
+    for (int which = 0; which < 2; ++which) {
+        uint32_t* list = nullptr;
+        uint32_t m = 0;
+        if (u < n) {
+            const uint32_t* offsets = which ? face_offsets : adj_offsets;
+            list = (which ? faces : adj) + offsets[u];
+            m = offsets[u + 1] - offsets[u];
+        }
+        const bool mine = m > kShortList && m <= kLongList;
+        if (mine)
+            for (uint32_t i = lane; i < m; i += 64) lds[wave][i] = list[i];
+        __syncthreads();
+        if (mine) {
+            for (uint32_t i = lane; i < m; i += 64) {
+                const uint32_t v = lds[wave][i];
+                uint32_t rank = 0;
+                for (uint32_t j = 0; j < m; ++j) {
+                    const uint32_t w = lds[wave][j];
+                    rank += uint32_t(w < v) | uint32_t(w == v && j < i);
+                }
+                list[rank] = v;
+            }
+        } else if (m > kLongList && lane == 0) {
+            sort_list(list, m);
+        }
+        __syncthreads();
+    }
+}
+
 __global__ void topo_vertex_kernel(uint32_t n, const uint32_t* __restrict__ face_offsets, uint32_t* faces,
                                    const uint32_t* __restrict__ adj_offsets, uint32_t* adj, uint32_t* degree,
                                    uint32_t* up, uint8_t* boundary, uint8_t* invalid) {
     const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
     if (u >= n) return;
-    sort_list(faces + face_offsets[u], face_offsets[u + 1] - face_offsets[u]);
+    // Longer lists are sorted already (long_sort_kernel).
+    const uint32_t d = face_offsets[u + 1] - face_offsets[u];
+    if (d <= kShortList) sort_list(faces + face_offsets[u], d);
     uint32_t* a = adj + adj_offsets[u];
     const uint32_t m = adj_offsets[u + 1] - adj_offsets[u];
-    sort_list(a, m);
+    if (m <= kShortList) sort_list(a, m);
     uint32_t distinct = 0, above = 0;
     bool open = false, bad = false;
     for (uint32_t i = 0; i < m;) {
@@ -340,6 +391,8 @@ __global__ __launch_bounds__(256) void fans_kernel(Topo t, uint8_t* invalid, uin
         if (reached != (slots == 32 ? ~0u : (1u << slots) - 1)) invalid[u] = 1;
         return;
     }
+    // Longer fans within kLongList faces and neighbours: long_fans_kernel.
+    if (f1 - f0 <= kLongList && t.offsets[u + 1] - t.offsets[u] <= kLongList) return;
     uint32_t other[2];
     for (uint32_t k = f0; k < f1; ++k)
         if (!link_of(t, t.faces[k], u, other)) {
@@ -372,6 +425,69 @@ __global__ __launch_bounds__(256) void fans_kernel(Topo t, uint8_t* invalid, uin
             invalid[u] = 1;
             return;
         }
+}
+
+// The same fan test for vertices with more than kFan faces or neighbours
+// (and at most kLongList of each), one wave per vertex: the lanes find each
+// face's link as two neighbour slots (link_of, find_neighbor) into LDS, and
+// the wave's first lane joins the slots of every link with union-find in LDS
+// and checks that every neighbour lies in the first face's component -- the
+// component the reference's flood from that face's first link vertex
+// reaches.
+__global__ __launch_bounds__(256) void long_fans_kernel(Topo t, uint8_t* invalid) {
+    __shared__ uint32_t link_a[4][kLongList], link_b[4][kLongList], parent[4][kLongList];
+    __shared__ uint32_t broken[4];
+    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const uint32_t u = blockIdx.x * 4 + wave;
+    uint32_t f0 = 0, d = 0, n0 = 0, slots = 0;
+    bool mine = false;
+    if (u < t.vertices) {
+        f0 = t.face_offsets[u];
+        d = t.face_offsets[u + 1] - f0;
+        n0 = t.offsets[u];
+        slots = t.offsets[u + 1] - n0;
+        mine = d && !invalid[u] && (d > kFan || slots > kFan) && d <= kLongList && slots <= kLongList;
+    }
+    if (lane == 0) broken[wave] = 0;
+    __syncthreads();
+    if (mine) {
+        for (uint32_t k = lane; k < d; k += 64) {
+            uint32_t other[2];
+            uint32_t a = kNone, b = kNone;
+            if (!link_of(t, t.faces[f0 + k], u, other)) {
+                broken[wave] = 1;
+            } else {
+                a = find_neighbor(t, u, other[0]);
+                b = find_neighbor(t, u, other[1]);
+            }
+            link_a[wave][k] = a == kNone ? kNone : a - n0;
+            link_b[wave][k] = b == kNone ? kNone : b - n0;
+        }
+        for (uint32_t i = lane; i < slots; i += 64) parent[wave][i] = i;
+    }
+    __syncthreads();
+    if (mine && lane == 0) {
+        uint32_t* p = parent[wave];
+        const auto find = [&](uint32_t x) {
+            while (p[x] != x) {
+                p[x] = p[p[x]];
+                x = p[x];
+            }
+            return x;
+        };
+        bool bad = broken[wave] != 0 || link_a[wave][0] == kNone;
+        if (!bad) {
+            for (uint32_t k = 0; k < d; ++k) {
+                const uint32_t a = link_a[wave][k], b = link_b[wave][k];
+                if (a == kNone || b == kNone) continue;
+                const uint32_t ra = find(a), rb = find(b);
+                if (ra != rb) p[max(ra, rb)] = min(ra, rb);
+            }
+            const uint32_t root = find(link_a[wave][0]);
+            for (uint32_t i = 0; i < slots && !bad; ++i) bad = find(i) != root;
+        }
+        if (bad) invalid[u] = 1;
+    }
 }
 
 // include/blitz/mesh.hpp:22-26
@@ -541,16 +657,26 @@ __device__ bool legal_action(const Topo& t, uint32_t from, uint32_t to, LegalAct
         uint32_t ids[3];
         moved_triple(geo, from, to, ids);
         if (ids[0] == ids[1] || ids[1] == ids[2]) return false;
-        for (uint32_t l = g0; l < g1; ++l) {
-            const uint4 other = t.face_geo[t.faces[l]];
-            if (other.x == ids[0] && other.y == ids[1] && other.z == ids[2]) return false;
-        }
-        for (uint32_t l = f0; l < k; ++l) {
-            const uint4 g = t.face_geo[t.faces[l]];
-            if (g.x == to || g.y == to || g.z == to) continue;
-            uint32_t other[3];
-            moved_triple(g, from, to, other);
-            if (other[0] == ids[0] && other[1] == ids[1] && other[2] == ids[2]) return false;
+        // Duplicates, searched among the faces of one of the moved face's
+        // two other vertices (the one with fewer faces), which every face
+        // that could match contains: a face around `to` with the moved
+        // triple, or another moved face -- replacing `from` by `to` is
+        // one-to-one on triples holding `from` once and not `to`, so two
+        // moved faces coincide after the move exactly when their triples
+        // coincide before it. (Scanning the faces around `to` and around
+        // `from` instead is quadratic in the valence of a pole.)
+        uint32_t o0 = kNone, o1 = kNone;
+        for (uint32_t j = 0; j < 3; ++j)
+            if (ids[j] != to) (o0 == kNone ? o0 : o1) = ids[j];
+        const uint32_t c0 = t.face_offsets[o0 + 1] - t.face_offsets[o0];
+        const uint32_t c1 = t.face_offsets[o1 + 1] - t.face_offsets[o1];
+        const uint32_t pick = c0 <= c1 ? o0 : o1;
+        for (uint32_t l = t.face_offsets[pick], l1 = t.face_offsets[pick + 1]; l < l1; ++l) {
+            const uint32_t h = t.faces[l];
+            if (h == f) continue;
+            const uint4 g = t.face_geo[h];
+            if ((g.x == ids[0] && g.y == ids[1] && g.z == ids[2]) || (g.x == geo.x && g.y == geo.y && g.z == geo.z))
+                return false;
         }
         uint32_t old[3], next[3];
         for (uint32_t j = 0; j < 3; ++j) {
@@ -954,6 +1080,7 @@ void ActionSet::build(const HostMesh& m, const float* uv) {
     scan_kernel<<<1, 1024, 0, st>>>(face_count, face_offsets, adj_count, adj_offsets, uint32_t(n1));
     topo_fill_kernel<<<blocks_for(corners), 256, 0, st>>>(idx, geometry, corners, face_offsets, adj_offsets,
                                                           face_cursor, adj_cursor, faces, adj);
+    long_sort_kernel<<<(n + 3) / 4, 256, 0, st>>>(n, face_offsets, faces, adj_offsets, adj);
     topo_vertex_kernel<<<blocks_for(n), 256, 0, st>>>(n, face_offsets, faces, adj_offsets, adj, degree, up, boundary,
                                                       invalid);
     set_u32_kernel<<<1, 1, 0, st>>>(degree + n, 0);
@@ -969,6 +1096,7 @@ void ActionSet::build(const HostMesh& m, const float* uv) {
            m.materials.empty() ? nullptr : materials_.as<uint16_t>(), geometry, invalid, boundary, offsets,
            neighbors, face_offsets, faces, n, faces_, face_geo};
     fans_kernel<<<blocks_for(n), 256, 0, st>>>(t, invalid, visited);
+    long_fans_kernel<<<(n + 3) / 4, 256, 0, st>>>(t, invalid);
     // ---- legality of every directed edge, compacted in enumeration order.
     legal_count_ = 0;
     if (slots) {

@@ -184,15 +184,46 @@ def test_gpu_actions_match_golden():
     check_against_golden(lambda m, uv: ActionSet(m, uv))
 
 
+def bipyramid(k: int) -> Mesh:
+    """A closed k-gon bipyramid: two apexes of valence k."""
+    t = 2 * np.pi * np.arange(k) / k
+    p = np.zeros((k + 2, 3), np.float32)
+    p[:k, 0], p[:k, 1] = np.cos(t), np.sin(t)
+    p[k, 2], p[k + 1, 2] = 1.0, -1.0
+    f = [(k, i, (i + 1) % k) for i in range(k)] + [(k + 1, (i + 1) % k, i) for i in range(k)]
+    return Mesh(p, np.array(f, np.uint32))
+
+
+def bowtie(k: int) -> Mesh:
+    """Two closed cones joined at their apex (vertex 0): the apex's link is
+    two separate cycles of k vertices, so it is not one fan."""
+    t = 2 * np.pi * np.arange(k) / k
+    ring = np.stack([np.cos(t), np.sin(t), np.zeros(k)], 1)
+    p = np.concatenate([[[0, 0, 0]], ring + [0, 0, 1], [[0, 0, 1.2]], ring + [0, 0, -1], [[0, 0, -1.2]]])
+    up, cap_up, down, cap_down = 1, 1 + k, 2 + k, 2 + 2 * k
+    f = [(0, up + (i + 1) % k, up + i) for i in range(k)] + [(cap_up, up + i, up + (i + 1) % k) for i in range(k)]
+    f += [(0, down + i, down + (i + 1) % k) for i in range(k)]
+    f += [(cap_down, down + (i + 1) % k, down + i) for i in range(k)]
+    return Mesh(p.astype(np.float32), np.array(f, np.uint32))
+
+
 @pytest.mark.gpu
 def test_gpu_actions_match_oracle():
+    """Includes vertices whose lists take the wave-per-vertex sort and fan
+    test (latitude-longitude sphere poles, a 300-valence bipyramid, a
+    non-manifold apex of two 80-face cones) and ones past the LDS lists
+    (a 1,100-valence bipyramid)."""
     from blitzremesher_amd.actions import ActionSet
     cases = {"icosphere4": (meshes.icosphere(4, 0.03, 7), None),
              "heightfield": (meshes.jitter(meshes.heightfield(64, 0.5, 8), 0.01, 8, 0.5), None),
-             "torus": (meshes.torus(1.0, 0.3, 60, 24, 0.02, 9), None)}
+             "torus": (meshes.torus(1.0, 0.3, 60, 24, 0.02, 9), None),
+             "latlong_sphere": (meshes.training_mesh("sphere", 25000, 17), None),
+             "bipyramid300": (bipyramid(300), None), "bowtie80": (bowtie(80), None),
+             "bipyramid1100": (bipyramid(1100), None)}
     cases.update(action_cases())
     for name, (mesh, uv) in cases.items():
         gpu, cpu = ActionSet(mesh, uv), Actions("oracle", mesh, uv)
+        print(name, len(cpu.legal), flush=True)
         assert np.array_equal(gpu.legal, cpu.legal), name
         rng = np.random.default_rng(31)
         for maximum, budget in ((64, 10 ** 9), (7, 10 ** 9), (64, 9), (64, 1)):
