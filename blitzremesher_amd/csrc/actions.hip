@@ -1313,7 +1313,7 @@ std::vector<uint32_t> ActionSet::trial(const uint32_t* rows, size_t n, std::vect
 std::vector<uint32_t> execute_actions(int device, const HostMesh& input, const float* uv, uint64_t target,
                                       uint32_t budget, uint32_t ranking, uint32_t batch, const ActionGate& gate,
                                       ExecuteStats& stats, std::vector<uint16_t>* materials_out, Scorer* scorer,
-                                      const Bounds* bounds, ExecutorCache* cache) {
+                                      const Bounds* bounds, ExecutorCache* cache, const ActionLadderGate* ladder) {
     if (!batch || batch > 64) throw std::invalid_argument("action batch outside 1..64");
     if (ranking == 2) throw Unavailable("shuffled ranking is not implemented by the HIP executor");
     if (ranking > 4) throw std::invalid_argument("unknown ranking");
@@ -1452,6 +1452,41 @@ std::vector<uint32_t> execute_actions(int device, const HostMesh& input, const f
                 stats.rejected += picked.size();
                 if (picked.size() == 1) break;
                 picked.resize(picked.size() / 2);
+                if (ladder) {
+                    // The rest of the ladder in one call: every shorter
+                    // prefix the trial budget leaves, trialled, then audited
+                    // together. The first accepted one is the one the
+                    // one-by-one loop keeps; trials and rejections are
+                    // counted up to it, as that loop counts them.
+                    std::vector<std::vector<uint32_t>> candidates;
+                    std::vector<std::vector<uint16_t>> kept;
+                    std::vector<size_t> sizes;
+                    t = std::chrono::steady_clock::now();
+                    for (size_t s = picked.size(); s >= 1 && stats.trials + candidates.size() < budget; s /= 2) {
+                        std::vector<uint16_t> m;
+                        candidates.push_back(set->trial(picked.data(), s, &m));
+                        kept.push_back(std::move(m));
+                        sizes.push_back(s);
+                    }
+                    stats.trial_ms += since(t);
+                    if (candidates.empty()) break;
+                    t = std::chrono::steady_clock::now();
+                    const int p = (*ladder)(candidates, kept);
+                    stats.gate_ms += since(t);
+                    if (p >= int(candidates.size())) throw std::logic_error("ladder gate index out of range");
+                    const size_t counted = p >= 0 ? size_t(p) + 1 : candidates.size();
+                    stats.trials += counted;
+                    for (size_t i = 0; i < counted; ++i)
+                        if (int(i) != p) stats.rejected += sizes[i];
+                    if (p >= 0) {
+                        state.indices = std::move(candidates[size_t(p)]);
+                        state.materials = std::move(kept[size_t(p)]);
+                        stats.accepted += sizes[size_t(p)];
+                        ++stats.accepted_batches;
+                        accepted = true;
+                    }
+                    break;
+                }
             }
             // A rejected singleton is not retried: continue after the first pick.
             position = first_pick + 1;

@@ -179,6 +179,12 @@ class ActionSet {
 // The gate of the executor: true when the candidate (index buffer, kept
 // materials) is accepted.
 using ActionGate = std::function<bool(const std::vector<uint32_t>&, const std::vector<uint16_t>&)>;
+// The gate over the rest of a halving ladder at once: candidates in ladder
+// order (each a shorter prefix of the rejected batch); returns the index of
+// the first accepted one, -1 when none is. It must decide each candidate as
+// the one-candidate gate would.
+using ActionLadderGate =
+    std::function<int(const std::vector<std::vector<uint32_t>>&, const std::vector<std::vector<uint16_t>>&)>;
 struct ExecuteStats {
     uint64_t ranked = 0, trials = 0, accepted = 0, rejected = 0;
     uint32_t states = 0;  // action states built (one per accepted batch, plus the last)
@@ -212,11 +218,14 @@ struct ExecutorCache {
 // graph, built with `bounds`; 1 constant, 3 shortest edge, 4 current plane),
 // take the greedy batch of up to `batch` actions from the rank position,
 // trial it, ask the gate, commit or halve; a rejected singleton moves past
-// the batch's first action. Returns the final index buffer.
+// the batch's first action. With `ladder`, once a batch is rejected the
+// shorter prefixes (as many as the trial budget leaves) are trialled and
+// audited in one ladder call; output and counters are the one-by-one
+// loop's. Returns the final index buffer.
 std::vector<uint32_t> execute_actions(int device, const HostMesh& input, const float* uv, uint64_t target,
                                       uint32_t budget, uint32_t ranking, uint32_t batch, const ActionGate& gate,
                                       ExecuteStats& stats, std::vector<uint16_t>* materials_out = nullptr,
                                       Scorer* scorer = nullptr, const Bounds* bounds = nullptr,
-                                      ExecutorCache* cache = nullptr);
+                                      ExecutorCache* cache = nullptr, const ActionLadderGate* ladder = nullptr);
 
 }  // namespace blitz_hip

@@ -809,22 +809,44 @@ blitz_status blitz_hip_execute_actions(blitz_hip_audit* a, const blitz_mesh* inp
         // The gate: an exact audit of the candidate (the input's vertex
         // streams, already on the device, with the trial's triangles)
         // against the input.
-        const ActionGate accept = [&](const std::vector<uint32_t>& idx, const std::vector<uint16_t>& mats) {
+        const auto derive = [&](const std::vector<uint32_t>& idx, const std::vector<uint16_t>& mats) {
             Digest d;
             d.a = reference->vertex_digest[0];
             d.b = reference->vertex_digest[1];
             digest_triangles(d, idx.data(), idx.size(), mats.data(), mats.size(), source.double_sided.data(),
                              source.double_sided.size());
             const uint64_t digest[2] = {d.a, d.b};
-            auto candidate = audit.derive(*reference, idx, mats, source.double_sided, digest);
+            return audit.derive(*reference, idx, mats, source.double_sided, digest);
+        };
+        const ActionGate accept = [&](const std::vector<uint32_t>& idx, const std::vector<uint16_t>& mats) {
+            auto candidate = derive(idx, mats);
             ++audits;
             return audit.evaluate(*reference, *candidate, bounds, verdict).passed;
+        };
+        // The rest of a rejected batch's halving ladder, audited in one
+        // batch (each candidate's measurement equals its single audit).
+        const ActionLadderGate ladder = [&](const std::vector<std::vector<uint32_t>>& idx,
+                                            const std::vector<std::vector<uint16_t>>& mats) {
+            std::vector<std::shared_ptr<const DeviceMesh>> owned;
+            std::vector<const DeviceMesh*> candidates;
+            for (size_t i = 0; i < idx.size(); ++i) {
+                owned.push_back(derive(idx[i], mats[i]));
+                candidates.push_back(owned.back().get());
+            }
+            const auto m = audit.evaluate_batch(*reference, candidates, bounds, verdict);
+            for (size_t i = 0; i < m.size(); ++i)
+                if (m[i].passed) {
+                    audits += i + 1;
+                    return int(i);
+                }
+            audits += m.size();
+            return -1;
         };
         ExecuteStats st;
         if (ranking == 0 && (!scorer || !scorer->impl)) throw std::invalid_argument("learned ranking needs a scorer");
         const auto out = execute_actions(audit.device(), source, uv.empty() ? nullptr : uv.data(), target, budget,
                                          ranking, batch, accept, st, nullptr, scorer ? scorer->impl.get() : nullptr,
-                                         &bounds);
+                                         &bounds, nullptr, &ladder);
         if (out.size() > index_capacity) throw std::invalid_argument("index capacity too small");
         std::memcpy(indices, out.data(), out.size() * 4);
         *index_count = out.size();

@@ -347,11 +347,45 @@ ChainLod ChainSearch::propose(const std::shared_ptr<Node>& parent, size_t target
         r_.gate_ms += since(t);
         return pass;
     };
+    // The rest of a rejected trial's halving ladder: each gate audits every
+    // shorter prefix in one batch, and the gate audits are counted trial by
+    // trial up to the first prefix all four accept, as the one-by-one gate
+    // counts them.
+    const ActionLadderGate ladder = [&](const std::vector<std::vector<uint32_t>>& idx,
+                                        const std::vector<std::vector<uint16_t>>& mats) {
+        if (s_.cancelled && s_.cancelled()) return -1;
+        const auto t = Clock::now();
+        std::vector<std::shared_ptr<const DeviceMesh>> owned;
+        std::vector<const DeviceMesh*> candidates;
+        for (size_t i = 0; i < idx.size(); ++i) {
+            owned.push_back(derive(ChainLod{idx[i], mats[i]}));
+            candidates.push_back(owned.back().get());
+        }
+        std::vector<Measurement> measured[4];
+        std::vector<std::function<void()>> tasks;
+        for (int g = 0; g < 4; ++g)
+            tasks.push_back([&, g] {
+                measured[g] = gate_audit_[g]->evaluate_batch(*refs[g], candidates, bounds_, bounded[g]);
+            });
+        pool_.run(tasks);
+        int first = -1;
+        for (size_t i = 0; i < candidates.size() && first < 0; ++i) {
+            bool pass = true;
+            for (int g = 0; g < 4 && pass; ++g) {
+                ++r_.gate_audits;
+                r_.bounded_audits += clamped[g];
+                pass = measured[g][i].passed;
+            }
+            if (pass) first = int(i);
+        }
+        r_.gate_ms += since(t);
+        return first;
+    };
     ExecuteStats st;
     ChainLod out;
     const auto t = Clock::now();
     out.indices = execute_actions(audit_.device(), source_, uv_, target, o_.action_trials, o_.ranking, o_.action_batch,
-                                  gate, st, &out.materials, scorer_, &bounds_, &executor_);
+                                  gate, st, &out.materials, scorer_, &bounds_, &executor_, &ladder);
     r_.executor_ms += since(t);
     r_.build_ms += st.build_ms;
     r_.rank_ms += st.rank_ms;
