@@ -560,6 +560,9 @@ def batched_chains(weights, count: int, workers: int) -> dict:
                      "trials": sum(r.info["action_trials"] for r in results),
                      "rejected_collapses": sum(r.info["rejected_collapses"] for r in results),
                      "triangles_per_level": [sum(r.lods[k].triangles for r in results) for k in range(6)],
+                     "stage_ms_summed": {k: round(sum(r.info[f"{k}_ms"] for r in results), 1)
+                                         for k in ("executor", "gate", "select", "trial", "speculative",
+                                                   "speculative_build", "offer", "confirm")},
                      "levels_passed": all(l.source.passed and l.adjacent.passed for r in results for l in r.lods[1:]),
                      "status": sorted({int(r.status) for r in results})}
     lp, pp = out["learned"], out["plain"]
