@@ -800,6 +800,141 @@ __global__ __launch_bounds__(64) void claim_kernel(Topo t, const LegalAction* __
     }
 }
 
+// claim_kernel with its per-vertex state in LDS hash tables instead of
+// global arrays: the chunk's lane masks (`touch`, cleared per chunk) and the
+// vertices claimed so far in this call. Insert attempts are counted ahead
+// of each insert and capped at three quarters of a table, so probes always
+// end; past the cap the kernel sets out[3 + 64] and stops, and the caller
+// runs claim_kernel instead. The picks are claim_kernel's.
+constexpr uint32_t kTouchSlots = 4096, kClaimSlots = 2048;
+
+__device__ __forceinline__ uint32_t slot_of(uint32_t v, uint32_t slots) { return (v * 2654435761u) & (slots - 1); }
+
+__global__ __launch_bounds__(64) void claim_lds_kernel(Topo t, const LegalAction* __restrict__ legal,
+                                                       const uint32_t* __restrict__ rows, uint32_t n, uint32_t maximum,
+                                                       unsigned long long removable, uint32_t strict, uint32_t* out) {
+    __shared__ uint32_t touch_key[kTouchSlots];
+    __shared__ unsigned long long touch_mask[kTouchSlots];
+    __shared__ uint32_t claim_key[kClaimSlots];
+    __shared__ uint32_t attempts[2];  // touch inserts this chunk, claim inserts this call
+    __shared__ uint32_t overflow;
+    const uint32_t lane = threadIdx.x;
+    const unsigned long long bit = 1ull << lane;
+    for (uint32_t i = lane; i < kTouchSlots; i += 64) {
+        touch_key[i] = kNone;
+        touch_mask[i] = 0;
+    }
+    for (uint32_t i = lane; i < kClaimSlots; i += 64) claim_key[i] = kNone;
+    if (lane == 0) {
+        attempts[0] = attempts[1] = 0;
+        overflow = 0;
+    }
+    __syncthreads();
+    const auto touch = [&](uint32_t v) {
+        if (atomicAdd(&attempts[0], 1u) >= kTouchSlots / 4 * 3) {
+            overflow = 1;
+            return;
+        }
+        for (uint32_t h = slot_of(v, kTouchSlots);; h = (h + 1) & (kTouchSlots - 1)) {
+            const uint32_t k = atomicCAS(&touch_key[h], kNone, v);
+            if (k == kNone || k == v) {
+                atomicOr(&touch_mask[h], bit);
+                return;
+            }
+        }
+    };
+    const auto touched = [&](uint32_t v) -> unsigned long long {
+        for (uint32_t h = slot_of(v, kTouchSlots);; h = (h + 1) & (kTouchSlots - 1)) {
+            const uint32_t k = touch_key[h];
+            if (k == v) return touch_mask[h];
+            if (k == kNone) return 0;
+        }
+    };
+    const auto claim = [&](uint32_t v) {
+        if (atomicAdd(&attempts[1], 1u) >= kClaimSlots / 4 * 3) {
+            overflow = 1;
+            return;
+        }
+        for (uint32_t h = slot_of(v, kClaimSlots);; h = (h + 1) & (kClaimSlots - 1)) {
+            const uint32_t k = atomicCAS(&claim_key[h], kNone, v);
+            if (k == kNone || k == v) return;
+        }
+    };
+    const auto claimed = [&](uint32_t v) {
+        for (uint32_t h = slot_of(v, kClaimSlots);; h = (h + 1) & (kClaimSlots - 1)) {
+            const uint32_t k = claim_key[h];
+            if (k == v) return true;
+            if (k == kNone) return false;
+        }
+    };
+    uint32_t picked = 0, error = 0, first = kNone;
+    bool done = false;
+    for (uint32_t k0 = 0; k0 < n && !done; k0 += 64) {
+        const uint32_t k = k0 + lane;
+        const bool valid = k < n;
+        LegalAction a{};
+        if (valid) a = legal[rows[k]];
+        const auto footprint = [&](auto&& visit) {
+            for (uint32_t e = 0; e < 2; ++e) {
+                const uint32_t u = e ? a.to : a.from;
+                visit(u);
+                const uint32_t j1 = t.offsets[u + 1];
+                for (uint32_t j = t.offsets[u]; j < j1; ++j) visit(t.neighbors[j]);
+            }
+        };
+        bool hit = false;
+        if (valid)
+            footprint([&](uint32_t v) {
+                hit |= claimed(v);
+                touch(v);
+            });
+        __syncthreads();
+        if (overflow) break;
+        unsigned long long conflict = 0;
+        if (valid) footprint([&](uint32_t v) { conflict |= touched(v); });
+        conflict &= ~bit;
+        // Sequential replay, as in claim_kernel.
+        const unsigned long long hits = __ballot(hit), valids = __ballot(valid);
+        const uint32_t before = picked;
+        unsigned long long taken = 0;
+        for (uint32_t j = 0; j < 64; ++j) {
+            if (!((valids >> j) & 1)) break;
+            const uint32_t removed = uint32_t(__shfl(int(a.removed), int(j)));
+            const unsigned long long meets = __shfl(conflict, int(j));
+            if (removed > removable || ((hits >> j) & 1) || (meets & taken)) {
+                error |= strict;
+                continue;
+            }
+            taken |= 1ull << j;
+            if (!picked) first = k0 + j;
+            ++picked;
+            removable -= removed;
+            if (picked == maximum || !removable) {
+                done = true;
+                break;
+            }
+        }
+        if (taken & bit) {
+            out[3 + before + uint32_t(__popcll(taken & (bit - 1)))] = rows[k];
+            footprint(claim);
+        }
+        __syncthreads();
+        for (uint32_t i = lane; i < kTouchSlots; i += 64) {
+            touch_key[i] = kNone;
+            touch_mask[i] = 0;
+        }
+        if (lane == 0) attempts[0] = 0;
+        __syncthreads();
+        if (overflow) break;
+    }
+    if (lane == 0) {
+        out[0] = picked;
+        out[1] = error;
+        out[2] = first;
+        out[3 + 64] = overflow;
+    }
+}
+
 // Ranking keys of the legal actions (src/neural/action_gpu.cu:809-844 with
 // the scores of src/neural/action.cpp:355-370, tests/neural_action_gpu.cpp:
 // 155-160): 1 constant, 3 shortest edge, 4 current plane, 0 learned (the
@@ -1140,8 +1275,20 @@ ActionTopo ActionSet::topo() const {
 uint32_t ActionSet::claim_device(const uint32_t* drows, size_t n, uint32_t maximum, uint64_t removable, bool strict,
                                  std::vector<uint32_t>& picked) {
     const hipStream_t st = stream_;
+    uint32_t* out = grow<uint32_t>(claim_out_, 3 + 64 + 1);
+    uint32_t host[3 + 64 + 1];
+    // The LDS form first; the global form when a table would fill.
+    claim_lds_kernel<<<1, 64, 0, st>>>(topo(), legal_.as<LegalAction>(), drows, uint32_t(n), maximum, removable,
+                                       strict ? 1u : 0u, out);
+    hip_check(hipGetLastError(), "claim launch");
+    hip_check(hipMemcpyAsync(host, out, sizeof host, hipMemcpyDeviceToHost, st), "download selection");
+    hip_check(hipStreamSynchronize(st), "claim");
+    if (!host[3 + 64]) {
+        if (strict && host[1]) throw std::invalid_argument("illegal, stale or overlapping endpoint actions");
+        picked.assign(host + 3, host + 3 + host[0]);
+        return host[2];
+    }
     uint32_t* used = grow<uint32_t>(used_, vertices_);
-    uint32_t* out = grow<uint32_t>(claim_out_, 3 + 64);
     // Lane masks per vertex, zero between chunks; the claim kernel clears
     // what it sets, so they are zeroed once per state.
     if (!touch_ready_) {
@@ -1154,8 +1301,7 @@ uint32_t ActionSet::claim_device(const uint32_t* drows, size_t n, uint32_t maxim
     claim_kernel<<<1, 64, 0, st>>>(topo(), legal_.as<LegalAction>(), drows, uint32_t(n), maximum, removable,
                                    strict ? 1u : 0u, used, touch_.as<unsigned long long>(), out);
     hip_check(hipGetLastError(), "claim launch");
-    uint32_t host[3 + 64];
-    hip_check(hipMemcpyAsync(host, out, sizeof host, hipMemcpyDeviceToHost, st), "download selection");
+    hip_check(hipMemcpyAsync(host, out, (3 + 64) * 4, hipMemcpyDeviceToHost, st), "download selection");
     hip_check(hipStreamSynchronize(st), "claim");
     if (strict && host[1]) throw std::invalid_argument("illegal, stale or overlapping endpoint actions");
     picked.assign(host + 3, host + 3 + host[0]);
