@@ -202,6 +202,11 @@ __global__ void topo_vertex_kernel(uint32_t n, const uint32_t* __restrict__ face
                                    const uint32_t* __restrict__ adj_offsets, uint32_t* adj, uint32_t* degree,
                                    uint32_t* up, uint8_t* boundary, uint8_t* invalid) {
     const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u == 0) {
+        // The scans' last entries (offsets[n], edge_offsets[n] = totals).
+        degree[n] = 0;
+        up[n] = 0;
+    }
     if (u >= n) return;
     // Longer lists are sorted already (long_sort_kernel).
     const uint32_t d = face_offsets[u + 1] - face_offsets[u];
@@ -243,59 +248,6 @@ __global__ void topo_emit_kernel(uint32_t n, const uint32_t* __restrict__ adj_of
     }
 }
 
-// Exclusive prefix sums of one or two uint32 arrays of `count` entries in
-// one workgroup: tiles of 4,096 entries, four per thread scanned in
-// registers, the waves' totals combined through LDS, the running total
-// carried from tile to tile. Sized for per-state topology arrays.
-__global__ __launch_bounds__(1024) void scan_kernel(const uint32_t* __restrict__ a_in, uint32_t* a_out,
-                                                    const uint32_t* __restrict__ b_in, uint32_t* b_out,
-                                                    uint32_t count) {
-    __shared__ uint32_t wave_total[2][16];
-    __shared__ uint32_t carry[2];
-    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    if (threadIdx.x < 2) carry[threadIdx.x] = 0;
-    __syncthreads();
-    const int arrays = b_in ? 2 : 1;
-    for (uint32_t base = 0; base < count; base += 4096) {
-        uint32_t v[2][4], lanes[2];
-        for (int r = 0; r < arrays; ++r) {
-            const uint32_t* in = r ? b_in : a_in;
-            uint32_t sum = 0;
-            for (int k = 0; k < 4; ++k) {
-                const uint32_t i = base + 4 * threadIdx.x + k;
-                v[r][k] = sum;
-                sum += i < count ? in[i] : 0u;
-            }
-            // Inclusive scan of the threads' sums within the wave.
-            uint32_t x = sum;
-            for (uint32_t d = 1; d < 64; d <<= 1) {
-                const uint32_t y = __shfl_up(x, d);
-                if (lane >= d) x += y;
-            }
-            lanes[r] = x - sum;
-            if (lane == 63) wave_total[r][wave] = x;
-        }
-        __syncthreads();
-        for (int r = 0; r < arrays; ++r) {
-            uint32_t before = carry[r];
-            for (uint32_t w = 0; w < wave; ++w) before += wave_total[r][w];
-            uint32_t* out = r ? b_out : a_out;
-            for (int k = 0; k < 4; ++k) {
-                const uint32_t i = base + 4 * threadIdx.x + k;
-                if (i < count) out[i] = before + lanes[r] + v[r][k];
-            }
-        }
-        __syncthreads();
-        if (threadIdx.x < uint32_t(arrays)) {
-            uint32_t total = 0;
-            for (uint32_t w = 0; w < 16; ++w) total += wave_total[threadIdx.x][w];
-            carry[threadIdx.x] += total;
-        }
-        __syncthreads();
-    }
-}
-
-__global__ void set_u32_kernel(uint32_t* p, uint32_t v) { *p = v; }
 
 __device__ uint32_t find_neighbor(const Topo& t, uint32_t u, uint32_t v) {
     uint32_t lo = t.offsets[u], hi = t.offsets[u + 1];
@@ -699,6 +651,7 @@ __device__ bool legal_action(const Topo& t, uint32_t from, uint32_t to, LegalAct
 __global__ void legal_kernel(Topo t, const unsigned long long* __restrict__ edges, uint32_t count, LegalAction* out,
                              uint32_t* flags) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) flags[2 * count] = 0;  // the scan's last entry (the legal count)
     if (i >= 2 * count) return;
     const unsigned long long key = edges[i / 2];
     const uint32_t a = uint32_t(key >> 32), b = uint32_t(key);
@@ -1160,6 +1113,8 @@ void ActionSet::build(const HostMesh& m, const float* uv) {
         };
         take(radix_sort_pairs(nullptr, need, key_tmp, key_sorted, ids, ord, n, 0, 32, st));
         take(inclusive_max(nullptr, need, ord2, key_sorted, n, st));
+        take(exclusive_sum(nullptr, need, face_count, face_offsets, uint32_t(n1), st));
+        take(exclusive_sum(nullptr, need, flags, where, uint32_t(max_slots + 1), st));
     }
     void* work = grow<char>(temp, bytes);
     auto run = [&](const char* what, auto call) {
@@ -1212,15 +1167,15 @@ void ActionSet::build(const HostMesh& m, const float* uv) {
     const uint32_t* idx = indices_.as<uint32_t>();
     topo_count_kernel<<<blocks_for(corners), 256, 0, st>>>(idx, geometry, corners, face_count, adj_count, invalid);
     face_geo_kernel<<<blocks_for(faces_), 256, 0, st>>>(idx, geometry, faces_, face_geo);
-    scan_kernel<<<1, 1024, 0, st>>>(face_count, face_offsets, adj_count, adj_offsets, uint32_t(n1));
+    run("face scan", [&](size_t& b) { return exclusive_sum(work, b, face_count, face_offsets, uint32_t(n1), st); });
+    run("adjacency scan", [&](size_t& b) { return exclusive_sum(work, b, adj_count, adj_offsets, uint32_t(n1), st); });
     topo_fill_kernel<<<blocks_for(corners), 256, 0, st>>>(idx, geometry, corners, face_offsets, adj_offsets,
                                                           face_cursor, adj_cursor, faces, adj);
     long_sort_kernel<<<(n + 3) / 4, 256, 0, st>>>(n, face_offsets, faces, adj_offsets, adj);
     topo_vertex_kernel<<<blocks_for(n), 256, 0, st>>>(n, face_offsets, faces, adj_offsets, adj, degree, up, boundary,
                                                       invalid);
-    set_u32_kernel<<<1, 1, 0, st>>>(degree + n, 0);
-    set_u32_kernel<<<1, 1, 0, st>>>(up + n, 0);
-    scan_kernel<<<1, 1024, 0, st>>>(degree, offsets, up, edge_offsets, uint32_t(n1));
+    run("degree scan", [&](size_t& b) { return exclusive_sum(work, b, degree, offsets, uint32_t(n1), st); });
+    run("edge scan", [&](size_t& b) { return exclusive_sum(work, b, up, edge_offsets, uint32_t(n1), st); });
     topo_emit_kernel<<<blocks_for(n), 256, 0, st>>>(n, adj_offsets, adj, offsets, edge_offsets, neighbors, unique);
     uint32_t edge_total = 0;
     hip_check(hipMemcpyAsync(&edge_total, edge_offsets + n, 4, hipMemcpyDeviceToHost, st), "download edges");
@@ -1236,8 +1191,7 @@ void ActionSet::build(const HostMesh& m, const float* uv) {
     legal_count_ = 0;
     if (slots) {
         legal_kernel<<<blocks_for(slots), 256, 0, st>>>(t, unique, edges_, all, flags);
-        set_u32_kernel<<<1, 1, 0, st>>>(flags + slots, 0);
-        scan_kernel<<<1, 1024, 0, st>>>(flags, where, nullptr, nullptr, slots + 1);
+        run("legal scan", [&](size_t& b) { return exclusive_sum(work, b, flags, where, slots + 1, st); });
         compact_kernel<<<blocks_for(slots), 256, 0, st>>>(all, flags, where, slots, legal);
         hip_check(hipMemcpyAsync(&legal_count_, where + slots, 4, hipMemcpyDeviceToHost, st), "download count");
     }
