@@ -675,6 +675,39 @@ def chain_cpu_baseline() -> dict | None:
             "note": "reference generate(): QEM reduce proposals + CPU coverage audits, same mesh and settings"}
 
 
+def chains_in_flight(mesh, settings, options, workers: int, chains: int, single) -> dict:
+    """`chains` copies of one chain run `workers` at a time, one audit object
+    per worker thread: chains per second of the whole set, and whether every
+    chain equals the single-threaded one."""
+    import threading
+    from concurrent.futures import ThreadPoolExecutor
+    from blitzremesher_amd import Audit
+    from blitzremesher_amd.chain import generate
+    local = threading.local()
+    owned = []
+    lock = threading.Lock()
+
+    def run(_):
+        if not hasattr(local, "audit"):
+            local.audit = Audit(0, cache_rasters=True)
+            with lock:
+                owned.append(local.audit)
+        return generate(local.audit, mesh, settings, options)
+
+    with ThreadPoolExecutor(workers) as pool:  # one untimed round
+        list(pool.map(run, range(workers)))
+        t0 = time.perf_counter()
+        results = list(pool.map(run, range(chains)))
+        wall = time.perf_counter() - t0
+    for audit in owned:
+        audit.close()
+    same = all([l.indices.tobytes() for l in x.lods] == [l.indices.tobytes() for l in single.lods] for x in results)
+    return {"chains": chains, "wall_s": round(wall, 2), "chains_per_s": round(chains / wall, 3),
+            "ms_per_chain_amortized": round(1000 * wall / chains, 1),
+            "four_gate_audited_collapses_per_s": round(sum(x.info["legal_collapses"] for x in results) / wall, 1),
+            "identical_to_single": same}
+
+
 def chain_leg(repeats: int, cpu_reference: dict | None) -> dict:
     """LOD chains of the CPU-runnable icosphere (10,242 V / 20,480 F, radial
     noise 0.02, seed 1) under c1_chain_settings: explicit reuse output,
@@ -697,6 +730,7 @@ def chain_leg(repeats: int, cpu_reference: dict | None) -> dict:
     r = runs[-1]
     ms = float(np.median([x.info["ms"] for x in runs]))
     i = r.info
+    in_flight = {str(w): chains_in_flight(mesh, settings, options, w, 2 * w, r) for w in (4, 8)}
     return {"workload": "4-level LOD chain (256 -> 32 px, transition and LOD0 bounds 1 px, 16 views, ss 4/8/16, "
                         "budget 8, beam 2) of the 10,242-vertex icosphere; device collapse executor in "
                         "current-plane order (64 trials x batch 32 per proposal), four-gate certified trials, "
@@ -712,6 +746,7 @@ def chain_leg(repeats: int, cpu_reference: dict | None) -> dict:
             "executor_stage_ms": {k: round(i[f"{k}_ms"], 1) for k in ("build", "rank", "select", "trial",
                                                                       "speculative")},
             "levels_passed": all(l.source.passed and l.adjacent.passed for l in r.lods[1:]),
+            "chains_in_flight": in_flight,
             "cpu_reference_classical": cpu_reference}
 
 
