@@ -250,8 +250,6 @@ class ChainSearch {
     bool cancelled() const { return r_.status == 2 || (s_.cancelled && s_.cancelled()); }
     std::shared_ptr<const DeviceMesh> derive(const ChainLod& lod);
     std::shared_ptr<const DeviceMesh> device_of(const std::shared_ptr<Node>& n);
-    Measurement measure(const DeviceMesh& reference, const DeviceMesh& candidate, const Settings& e,
-                        bool verdict_only);
     ChainLod propose(const std::shared_ptr<Node>& parent, size_t target, unsigned level);
     bool confirm();
     double tiebreak();
@@ -294,17 +292,6 @@ std::shared_ptr<const DeviceMesh> ChainSearch::derive(const ChainLod& lod) {
 std::shared_ptr<const DeviceMesh> ChainSearch::device_of(const std::shared_ptr<Node>& n) {
     if (!n->device) n->device = derive(n->lod);
     return n->device;
-}
-
-// hooks.evaluate (src/neural/generate.cpp:215-224) and certify_device
-// (:347-358): the refinement ceiling bounded by the sample cap.
-Measurement ChainSearch::measure(const DeviceMesh& reference, const DeviceMesh& candidate, const Settings& e,
-                                 bool verdict_only) {
-    Settings bounded = e;
-    bounded.max_supersample = bounded_refinement(e);
-    if (bounded.max_supersample < e.max_supersample) ++r_.bounded_audits;
-    bounded.verdict_only = verdict_only;
-    return audit_.evaluate(reference, candidate, bounds_, bounded);
 }
 
 // propose_guarded (src/neural/generate.cpp:304-450) for a direct reuse
@@ -550,20 +537,30 @@ ChainResult ChainSearch::run() {
             const bool changed = !same_data(candidate, previous);
             const auto prev_dev = device_of(parent);
             const auto cand_dev = changed ? derive(candidate) : prev_dev;
+            // The four gates (hooks.evaluate, src/neural/generate.cpp:215-224,
+            // with the refinement ceiling bounded by the sample cap as in
+            // certify_device, :347-358) run at once, one per audit object;
+            // their results are then taken in the reference's order (the
+            // first failing gate ends the offer), so the counters are the
+            // one-by-one ones.
+            const DeviceMesh* refs[4] = {source_dev_.get(), prev_dev.get(), source_dev_.get(), prev_dev.get()};
+            Measurement m[4];
+            std::vector<std::function<void()>> tasks;
+            for (int g = 0; g < 4; ++g)
+                tasks.push_back([&, g] {
+                    Settings bounded = gates_[g];
+                    bounded.max_supersample = bounded_refinement(gates_[g]);
+                    bounded.verdict_only = false;
+                    m[g] = gate_audit_[g]->evaluate(*refs[g], *cand_dev, bounds_, bounded);
+                });
+            pool_.run(tasks);
             uint8_t gate = 0;
-            if (!accepted(measure(*source_dev_, *cand_dev, gates_[0], false), 0))
-                gate = 1;
-            else if (!accepted(measure(*prev_dev, *cand_dev, gates_[1], false), 1))
-                gate = 2;
-            else {
-                candidate.source_error = measure(*source_dev_, *cand_dev, gates_[2], false);
-                if (!accepted(candidate.source_error, 2))
-                    gate = 3;
-                else {
-                    candidate.adjacent = measure(*prev_dev, *cand_dev, gates_[3], false);
-                    if (!accepted(candidate.adjacent, 3)) gate = 4;
-                }
+            for (int g = 0; g < 4 && !gate; ++g) {
+                if (bounded_refinement(gates_[g]) < gates_[g].max_supersample) ++r_.bounded_audits;
+                if (!accepted(m[g], size_t(g))) gate = uint8_t(g + 1);
             }
+            candidate.source_error = m[2];
+            candidate.adjacent = m[3];
             r_.offer_ms += since(t);
             if (gate) return gate;
             candidate.schedule = step;
