@@ -1447,13 +1447,17 @@ std::vector<uint32_t> execute_actions(int device, const HostMesh& input, const f
     // next state's build may overwrite the buffers); the states share the
     // input's vertex stream, uploaded once.
     std::unique_ptr<Graph> graph;
+    // The scorer's graph of a mesh state and its BF16 logits (learned
+    // ranking only); independent of the action state, so it can run beside
+    // that state's build.
+    const auto score_state = [&](const HostMesh& mesh) {
+        const bool same_positions = graph != nullptr;
+        if (!graph) graph = std::make_unique<Graph>();
+        scorer->rebuild(*graph, mesh, *bounds, same_positions);
+        scorer->forward(*graph, Precision::Bf16);
+    };
     const auto rank_state = [&](ActionSet* s, const HostMesh& mesh) {
-        if (ranking == 0) {
-            const bool same_positions = graph != nullptr;
-            if (!graph) graph = std::make_unique<Graph>();
-            scorer->rebuild(*graph, mesh, *bounds, same_positions);
-            scorer->forward(*graph, Precision::Bf16);
-        }
+        if (ranking == 0) score_state(mesh);
         s->rank(ranking, diameter, ranking == 0 ? graph.get() : nullptr, nullptr);
     };
     while (state.indices.size() / 3 > target && stats.trials < budget) {
@@ -1518,19 +1522,27 @@ std::vector<uint32_t> execute_actions(int device, const HostMesh& input, const f
                 // Speculate only when an acceptance would continue the loop.
                 bool built = false;
                 if (candidate.size() / 3 > target && stats.trials < budget) {
+                    // With learned ranking the scorer's graph and logits of
+                    // the next state are computed on another thread (the
+                    // scorer's own stream) while its action state builds.
+                    std::future<void> scored;
                     try {
                         const auto tb = std::chrono::steady_clock::now();
                         next_mesh.indices = candidate;
                         next_mesh.materials = materials;
+                        if (ranking == 0) scored = std::async(std::launch::async, [&] { score_state(next_mesh); });
                         if (c.spec)
                             c.spec->rebuild(next_mesh, uv);
                         else
                             c.spec = std::make_unique<ActionSet>(device, next_mesh, uv);
                         stats.speculative_build_ms += since(tb);
-                        if (c.spec->legal_count()) rank_state(c.spec.get(), next_mesh);
+                        if (scored.valid()) scored.get();
+                        if (c.spec->legal_count())
+                            c.spec->rank(ranking, diameter, ranking == 0 ? graph.get() : nullptr, nullptr);
                         stats.speculative_ms += since(tb);
                         built = true;
                     } catch (...) {
+                        if (scored.valid()) scored.wait();
                         verdict.wait();
                         throw;
                     }
